@@ -1,0 +1,16 @@
+"""MI355X-native shadow pipeline (light-view depth, hard/PCF/RBSM shadows,
+area-light soft shadows) behind the C ABI of include/gi.h.
+
+The compute path is libgi.so (hand-written gfx950 HIP kernels); this package
+is the host-side harness: deterministic scenes, camera math through the C
+ABI, and pass functions that mirror the reference's per-technique passes.
+"""
+from . import _lib, camera, scenes
+from ._lib import GiError, lib
+from .passes import (DeviceMesh, GBuffer, ShadowParams, SoftParams, biased, compute_hard_shadows,
+                     phong_uniforms, render_gbuffer, render_monte_carlo, render_shadow_map, soft_uniforms,
+                     view_matrices)
+
+__all__ = ["_lib", "camera", "scenes", "GiError", "lib", "DeviceMesh", "GBuffer", "ShadowParams",
+           "SoftParams", "biased", "compute_hard_shadows", "phong_uniforms", "render_gbuffer",
+           "render_monte_carlo", "render_shadow_map", "soft_uniforms", "view_matrices"]

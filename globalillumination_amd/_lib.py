@@ -1,0 +1,132 @@
+"""ctypes binding of libgi.so (the C ABI declared in include/gi.h).
+
+torch is imported first so that the HIP runtime torch ships is the one libgi.so
+binds to: the kernels then run on torch's streams and in torch's memory.  There
+is no CPU fallback: if the library is missing or fails to load, every pass
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import torch  # noqa: F401  (loads the HIP runtime before libgi.so)
+
+LIB_PATH = Path(__file__).resolve().parent / "libgi.so"
+
+GI_OK = 0
+GI_EINVAL = -1
+GI_ENONFINITE = -2
+GI_EDEVICE = -3
+GI_ENOSPACE = -4
+GI_ENOTSUP = -5
+
+GI_NAIVE, GI_PCF, GI_RBSM, GI_RPCF, GI_RBSM_FILTERED = range(5)
+GI_MAX_SAMPLES = 4096
+GI_MAX_EDGE = 16384
+
+F16 = ctypes.c_float * 16
+F9 = ctypes.c_float * 9
+F3 = ctypes.c_float * 3
+
+
+class GiRasterOpts(ctypes.Structure):
+    _fields_ = [("polygon_offset_factor", ctypes.c_float),
+                ("polygon_offset_units", ctypes.c_float)]
+
+
+class GiGBuffer(ctypes.Structure):
+    _fields_ = [("position_dev", ctypes.c_void_p),
+                ("normal_dev", ctypes.c_void_p),
+                ("width", ctypes.c_int),
+                ("height", ctypes.c_int)]
+
+
+class GiShadowParams(ctypes.Structure):
+    _fields_ = [("light_mvp", F16),
+                ("shadow_map_width", ctypes.c_int),
+                ("shadow_map_height", ctypes.c_int),
+                ("technique", ctypes.c_int),
+                ("kernel_order", ctypes.c_int),
+                ("penumbra_size", ctypes.c_int),
+                ("max_search", ctypes.c_int),
+                ("depth_threshold", ctypes.c_float),
+                ("shadow_intensity", ctypes.c_float),
+                ("pre_evaluation", ctypes.c_int),
+                ("mv", F16),
+                ("normal_matrix", F9),
+                ("light_position", F3)]
+
+
+class GiSoftParams(ctypes.Structure):
+    _fields_ = [("shadow_intensity", ctypes.c_float),
+                ("pcf", ctypes.c_int),
+                ("kernel_order", ctypes.c_int),
+                ("penumbra", ctypes.c_float),
+                ("inclusive", ctypes.c_int)]
+
+
+class GiError(RuntimeError):
+    def __init__(self, fn: str, status: int, msg: str):
+        super().__init__(f"{fn} failed: {msg} ({status})")
+        self.status = status
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "gi_api_version": (ctypes.c_int, []),
+    "gi_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "gi_perspective": (ctypes.c_int, [ctypes.c_float] * 4 + [ctypes.c_void_p]),
+    "gi_ortho": (ctypes.c_int, [ctypes.c_float] * 6 + [ctypes.c_void_p]),
+    "gi_lookat": (ctypes.c_int, [ctypes.c_void_p] * 4),
+    "gi_mat4_mul": (ctypes.c_int, [ctypes.c_void_p] * 3),
+    "gi_bias_matrix": (ctypes.c_int, [ctypes.c_float, ctypes.c_void_p]),
+    "gi_uniform_sampling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p]),
+    "gi_soft_light_trans": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "gi_depth_pass_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 4),
+    "gi_depth_pass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(GiRasterOpts), ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_size_t, ctypes.c_void_p]),
+    "gi_gbuffer_pass_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 3),
+    "gi_gbuffer_pass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_void_p]),
+    "gi_hard_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p,
+                                           ctypes.POINTER(GiShadowParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libgi.so once; raises when it is absent (no fallback exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                               "`python -c 'import __graft_entry__ as g; g.build()'`")
+        handle = ctypes.CDLL(str(LIB_PATH), mode=ctypes.RTLD_GLOBAL)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(fn_name: str, status: int) -> None:
+    if status != GI_OK:
+        msg = lib().gi_status_string(status).decode()
+        raise GiError(fn_name, status, msg)
+
+
+def call(fn_name: str, *args) -> None:
+    check(fn_name, getattr(lib(), fn_name)(*args))

@@ -1,0 +1,698 @@
+// raster.hip — light-view depth pass and camera visibility/G-buffer pass for gfx950.
+//
+// Replaces the reference's fixed-function OpenGL rasterization of the light
+// view (ShadowMapping/src/main.cpp:221-274 with Scene.vert:11-20, depth into a
+// GL_DEPTH_COMPONENT32F texture cleared to 1.0, depth test LESS, no culling)
+// and of the camera G-buffer (GBuffer.vert:15-21, GBuffer.frag:35-37).
+//
+// Design (binned raster, one launch sequence for a whole batch of maps):
+//   1. k_count    one thread per (triangle, map): transform, clip, snap, and
+//                 count the 32x32 screen tiles each piece overlaps;
+//   2. k_scan     one workgroup per map: exclusive scan of the tile counts;
+//   3. k_scatter  same threads as k_count: append the piece id to every
+//                 overlapped tile's list;
+//   4. k_raster   one workgroup per (tile, map): resolve the tile's list into an
+//                 LDS tile with ds_min on the FP32 depth bits (non-negative
+//                 floats order like uint32) or on 64-bit (depth, id) keys,
+//                 then write the tile to HBM once with 16-byte stores.
+// atomicMin makes the result independent of list order, so the output is
+// deterministic.  When a map's lists would exceed the workspace, that map falls
+// back to every workgroup scanning the whole mesh (slower, same result).
+//
+// Raster rules (the choices the GL spec leaves to the implementation):
+//   clip near (z >= -w), far (z <= w) and a guard band |x|,|y| <= 4w with
+//   Sutherland-Hodgman (intersection taken from the inside vertex, so a shared
+//   edge clips identically in both triangles), fan triangulation; window
+//   coordinates snapped to 1/256 pixel with round-to-nearest-even; pixel
+//   centres at (i+.5, j+.5); top-left tie rule in GL's y-up window space;
+//   window depth interpolated linearly in screen space in FP64 from the exact
+//   integer edge functions, rounded to FP32, clamped to [0, 1]; optional
+//   polygon offset factor*max|dz/dx|,|dz/dy| + units*2^(e-23).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/gi.h"
+#include "gi_device.hpp"
+
+namespace gi {
+
+constexpr int kTile = 32;
+constexpr int kTilePx = kTile * kTile;
+constexpr int kRasterThreads = 256;
+constexpr float kGuard = 4.0f;
+constexpr int kMaxPoly = 9;
+constexpr int kSelfRasterMaxPx = 32;  // bbox∩tile area a lone thread rasterizes itself
+constexpr int kScanThreads = 1024;
+
+struct MeshView {
+    const float* __restrict__ pos;  // nverts x 3
+    const float* __restrict__ nrm;  // nverts x 3 or null
+    const int* __restrict__ idx;    // ntris x 3
+    int nt;
+};
+
+// Per-map transform: a device array of 16-float matrices, or one by value.
+struct MvpSource {
+    const float* __restrict__ dev;
+    Mat4 single;
+    __device__ __forceinline__ void load(int map, float m[16]) const {
+        if (dev) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) m[i] = dev[16 * map + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) m[i] = single.m[i];
+        }
+    }
+};
+
+struct RasterCfg {
+    int w, h;
+    int tiles_x, tiles_y;
+    float off_factor, off_units;
+};
+
+template <int NA>
+struct CV {
+    float c[4];
+    float a[NA > 0 ? NA : 1];
+};
+
+template <int NA>
+struct Tri {
+    long long X[3], Y[3];
+    float w[3];
+    float a[3][NA > 0 ? NA : 1];
+    int front;
+    int px0, px1, py0, py1;
+    int tl[3];
+    double z0, dz1, dz2, inv_area, offset;
+};
+
+__device__ __forceinline__ long long llmin(long long a, long long b) { return a < b ? a : b; }
+__device__ __forceinline__ long long llmax(long long a, long long b) { return a < b ? b : a; }
+
+__device__ __forceinline__ float plane_dist(const float c[4], int p) {
+    switch (p) {
+    case 0: return c[2] + c[3];
+    case 1: return c[3] - c[2];
+    case 2: return c[0] + kGuard * c[3];
+    case 3: return kGuard * c[3] - c[0];
+    case 4: return c[1] + kGuard * c[3];
+    default: return kGuard * c[3] - c[1];
+    }
+}
+
+template <int NA>
+__device__ void load_tri(const MeshView& m, const float mvp[16], int tri, CV<NA> v[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int vi = m.idx[3 * tri + k];
+        float x = m.pos[3 * vi + 0], y = m.pos[3 * vi + 1], z = m.pos[3 * vi + 2];
+        mat4_vec4(mvp, x, y, z, 1.0f, v[k].c);
+        if constexpr (NA == 6) {
+            v[k].a[0] = x;
+            v[k].a[1] = y;
+            v[k].a[2] = z;
+            v[k].a[3] = m.nrm[3 * vi + 0];
+            v[k].a[4] = m.nrm[3 * vi + 1];
+            v[k].a[5] = m.nrm[3 * vi + 2];
+        }
+    }
+}
+
+template <int NA>
+__device__ __forceinline__ void lerp_cv(const CV<NA>& in, const CV<NA>& out, float din, float dout,
+                                        CV<NA>& r) {
+    float t = din / (din - dout);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.c[k] = in.c[k] + t * (out.c[k] - in.c[k]);
+#pragma unroll
+    for (int k = 0; k < NA; ++k) r.a[k] = in.a[k] + t * (out.a[k] - in.a[k]);
+}
+
+// Fills `t` from three clip-space vertices.  Returns false for an empty piece.
+template <int NA>
+__device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2, const RasterCfg& cfg,
+                            Tri<NA>& t) {
+    const CV<NA>* vs[3] = {&v0, &v1, &v2};
+    float hw = (float)cfg.w * 0.5f, hh = (float)cfg.h * 0.5f;
+    float z[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float* c = vs[i]->c;
+        if (!(c[3] > 0.0f)) return false;
+        float xn = c[0] / c[3], yn = c[1] / c[3], zn = c[2] / c[3];
+        float xw = xn * hw + hw;
+        float yw = yn * hh + hh;
+        z[i] = zn * 0.5f + 0.5f;
+        t.X[i] = (long long)rintf(xw * 256.0f);
+        t.Y[i] = (long long)rintf(yw * 256.0f);
+        t.w[i] = c[3];
+#pragma unroll
+        for (int q = 0; q < NA; ++q) t.a[i][q] = vs[i]->a[q];
+    }
+    long long area = (t.X[1] - t.X[0]) * (t.Y[2] - t.Y[0]) - (t.Y[1] - t.Y[0]) * (t.X[2] - t.X[0]);
+    if (area == 0) return false;
+    t.front = area > 0;
+    if (area < 0) {
+        long long sx = t.X[1]; t.X[1] = t.X[2]; t.X[2] = sx;
+        long long sy = t.Y[1]; t.Y[1] = t.Y[2]; t.Y[2] = sy;
+        float sz = z[1]; z[1] = z[2]; z[2] = sz;
+        float sw = t.w[1]; t.w[1] = t.w[2]; t.w[2] = sw;
+#pragma unroll
+        for (int q = 0; q < NA; ++q) { float sa = t.a[1][q]; t.a[1][q] = t.a[2][q]; t.a[2][q] = sa; }
+        area = -area;
+    }
+    long long minX = llmin(t.X[0], llmin(t.X[1], t.X[2])), maxX = llmax(t.X[0], llmax(t.X[1], t.X[2]));
+    long long minY = llmin(t.Y[0], llmin(t.Y[1], t.Y[2])), maxY = llmax(t.Y[0], llmax(t.Y[1], t.Y[2]));
+    long long px0 = (minX - 128 + 255) >> 8, px1 = (maxX - 128) >> 8;
+    long long py0 = (minY - 128 + 255) >> 8, py1 = (maxY - 128) >> 8;
+    if (px0 < 0) px0 = 0;
+    if (py0 < 0) py0 = 0;
+    if (px1 > cfg.w - 1) px1 = cfg.w - 1;
+    if (py1 > cfg.h - 1) py1 = cfg.h - 1;
+    if (px0 > px1 || py0 > py1) return false;
+    t.px0 = (int)px0; t.px1 = (int)px1; t.py0 = (int)py0; t.py1 = (int)py1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int a = (i + 1) % 3, b = (i + 2) % 3;
+        long long dy = t.Y[b] - t.Y[a], dx = t.X[b] - t.X[a];
+        t.tl[i] = (dy < 0 || (dy == 0 && dx < 0)) ? 1 : 0;
+    }
+    t.z0 = (double)z[0];
+    t.dz1 = (double)z[1] - (double)z[0];
+    t.dz2 = (double)z[2] - (double)z[0];
+    t.inv_area = 1.0 / (double)area;
+    t.offset = 0.0;
+    if (cfg.off_factor != 0.0f || cfg.off_units != 0.0f) {
+        double dzdx = ((double)(t.Y[2] - t.Y[0]) * t.dz1 + (double)(t.Y[0] - t.Y[1]) * t.dz2) * t.inv_area * 256.0;
+        double dzdy = ((double)(t.X[0] - t.X[2]) * t.dz1 + (double)(t.X[1] - t.X[0]) * t.dz2) * t.inv_area * 256.0;
+        double m = fmax(fabs(dzdx), fabs(dzdy));
+        float zmax = fmaxf(fmaxf(fabsf(z[0]), fabsf(z[1])), fabsf(z[2]));
+        double r = 0.0;
+        if (zmax > 0.0f) {
+            int e;
+            frexpf(zmax, &e);
+            r = ldexp(1.0, (e - 1) - 23);
+        }
+        t.offset = (double)cfg.off_factor * m + (double)cfg.off_units * r;
+    }
+    return true;
+}
+
+// Clips a triangle that crosses a clip plane and sets up fan piece `piece`.
+// Returns the number of fan pieces; `ok` tells whether `piece` is non-empty.
+template <int NA>
+__device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
+                                           Tri<NA>& t, bool& ok) {
+    CV<NA> poly[kMaxPoly], tmp[kMaxPoly];
+    poly[0] = v[0];
+    poly[1] = v[1];
+    poly[2] = v[2];
+    int n = 3;
+    ok = false;
+    for (int p = 0; p < 6; ++p) {
+        int m = 0;
+        for (int k = 0; k < n; ++k) {
+            const CV<NA>& prev = poly[(k + n - 1) % n];
+            const CV<NA>& cur = poly[k];
+            float dp = plane_dist(prev.c, p), dc = plane_dist(cur.c, p);
+            if (dc >= 0.0f) {
+                if (dp < 0.0f) lerp_cv<NA>(cur, prev, dc, dp, tmp[m++]);
+                tmp[m++] = cur;
+            } else if (dp >= 0.0f) {
+                lerp_cv<NA>(prev, cur, dp, dc, tmp[m++]);
+            }
+        }
+        n = m;
+        if (n < 3) return 0;
+        for (int k = 0; k < n; ++k) poly[k] = tmp[k];
+    }
+    if (piece + 2 < n) ok = setup_piece<NA>(poly[0], poly[piece + 1], poly[piece + 2], cfg, t);
+    return n - 2;
+}
+
+// Number of fan pieces of `tri` (0 when trivially rejected) and the setup of
+// piece `piece`; `ok` is false when that piece covers no pixel centre.
+template <int NA>
+__device__ int get_piece(const MeshView& m, const float mvp[16], int tri, int piece, const RasterCfg& cfg,
+                         Tri<NA>& t, bool& ok) {
+    CV<NA> v[3];
+    load_tri<NA>(m, mvp, tri, v);
+    bool inside_all = true;
+    for (int p = 0; p < 6; ++p) {
+        float d0 = plane_dist(v[0].c, p), d1 = plane_dist(v[1].c, p), d2 = plane_dist(v[2].c, p);
+        int nin = (d0 >= 0.0f) + (d1 >= 0.0f) + (d2 >= 0.0f);
+        if (nin == 0) { ok = false; return 0; }
+        if (nin != 3) inside_all = false;
+    }
+    if (inside_all) {
+        ok = (piece == 0) && setup_piece<NA>(v[0], v[1], v[2], cfg, t);
+        return 1;
+    }
+    return clip_and_setup<NA>(v, piece, cfg, t, ok);
+}
+
+// Edge functions at the centre of pixel (i, j); false when not covered.
+template <int NA>
+__device__ __forceinline__ bool eval_pixel(const Tri<NA>& t, int i, int j, long long e[3], float& zout) {
+    long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int a = (k + 1) % 3, b = (k + 2) % 3;
+        e[k] = (t.X[b] - t.X[a]) * (Py - t.Y[a]) - (t.Y[b] - t.Y[a]) * (Px - t.X[a]);
+    }
+    if (e[0] + t.tl[0] <= 0 || e[1] + t.tl[1] <= 0 || e[2] + t.tl[2] <= 0) return false;
+    double zd = t.z0 + ((double)e[1] * t.dz1 + (double)e[2] * t.dz2) * t.inv_area;
+    zd = zd + t.offset;
+    float z = (float)zd;
+    if (!(z > 0.0f)) z = 0.0f;
+    if (z > 1.0f) z = 1.0f;
+    zout = z;
+    return true;
+}
+
+// Conservative test: can any pixel centre of [x0,x1]x[y0,y1] be inside?
+template <int NA>
+__device__ __forceinline__ bool rect_overlap(const Tri<NA>& t, int x0, int x1, int y0, int y1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int a = (k + 1) % 3, b = (k + 2) % 3;
+        long long ex = t.X[b] - t.X[a], ey = t.Y[b] - t.Y[a];
+        long long Px = (long long)((ey < 0) ? x1 : x0) * 256 + 128;
+        long long Py = (long long)((ex > 0) ? y1 : y0) * 256 + 128;
+        long long e = ex * (Py - t.Y[a]) - ey * (Px - t.X[a]);
+        if (e + t.tl[k] <= 0) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+
+// ---------------------------------------------------------------------------
+// Binning
+// ---------------------------------------------------------------------------
+template <bool kScatter>
+__global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, uint32_t* counts,
+                                             const uint32_t* offsets, const int* overflow, uint32_t* entries,
+                                             uint32_t cap) {
+    int tri = blockIdx.x * blockDim.x + threadIdx.x;
+    int map = blockIdx.y;
+    if (tri >= mesh.nt) return;
+    int ntiles = cfg.tiles_x * cfg.tiles_y;
+    if (kScatter && overflow[map]) return;
+    float mvp[16];
+    mvps.load(map, mvp);
+    uint32_t* cnt = counts + (size_t)map * ntiles;
+    int npieces = 1;
+    for (int piece = 0; piece < npieces; ++piece) {
+        Tri<0> t;
+        bool ok;
+        npieces = get_piece<0>(mesh, mvp, tri, piece, cfg, t, ok);
+        if (!ok) continue;
+        uint32_t entry = (uint32_t)tri * 8u + (uint32_t)piece;
+        int tx0 = t.px0 / kTile, tx1 = t.px1 / kTile, ty0 = t.py0 / kTile, ty1 = t.py1 / kTile;
+        bool single = (tx0 == tx1 && ty0 == ty1);
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                if (!single) {
+                    int x0 = max(tx * kTile, t.px0), x1 = min(tx * kTile + kTile - 1, t.px1);
+                    int y0 = max(ty * kTile, t.py0), y1 = min(ty * kTile + kTile - 1, t.py1);
+                    if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
+                }
+                int tile = ty * cfg.tiles_x + tx;
+                if (!kScatter) {
+                    atomicAdd(&cnt[tile], 1u);
+                } else {
+                    uint32_t pos = offsets[(size_t)map * (ntiles + 1) + tile] + atomicAdd(&cnt[tile], 1u);
+                    if (pos < cap) entries[(size_t)map * cap + pos] = entry;
+                }
+            }
+    }
+}
+
+// Exclusive scan of one map's tile counts; resets counts to 0 (they become
+// the scatter cursors) and flags maps whose lists exceed `cap` entries.
+__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* counts, uint32_t* offsets, int* overflow,
+                                                       int ntiles, uint32_t cap) {
+    __shared__ unsigned long long part[kScanThreads];
+    int map = blockIdx.x;
+    uint32_t* c = counts + (size_t)map * ntiles;
+    uint32_t* o = offsets + (size_t)map * (ntiles + 1);
+    int per = (ntiles + kScanThreads - 1) / kScanThreads;
+    int b = threadIdx.x * per, e = min(b + per, ntiles);
+    unsigned long long s = 0;
+    for (int i = b; i < e; ++i) s += c[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        unsigned long long v = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    unsigned long long run = part[threadIdx.x] - s;  // exclusive prefix
+    for (int i = b; i < e; ++i) {
+        o[i] = (uint32_t)run;
+        run += c[i];
+        c[i] = 0;
+    }
+    if (threadIdx.x == kScanThreads - 1) {
+        unsigned long long total = part[kScanThreads - 1];
+        o[ntiles] = (uint32_t)(total > cap ? cap : total);
+        overflow[map] = total > cap ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile raster
+// ---------------------------------------------------------------------------
+template <bool kVis>
+struct KeyOf;
+template <>
+struct KeyOf<false> {
+    using T = uint32_t;
+    static constexpr uint32_t clear = 0x3F800000u;  // 1.0f
+};
+template <>
+struct KeyOf<true> {
+    using T = unsigned long long;
+    static constexpr unsigned long long clear = ~0ull;
+};
+
+template <bool kVis>
+__device__ __forceinline__ void raster_rect(const Tri<0>& t, uint32_t id, int x0, int y0, int rx0, int rx1,
+                                            int ry0, int ry1, int first, int stride,
+                                            typename KeyOf<kVis>::T* tile) {
+    int rw = rx1 - rx0 + 1;
+    int n = rw * (ry1 - ry0 + 1);
+    for (int p = first; p < n; p += stride) {
+        int i = rx0 + p % rw, j = ry0 + p / rw;
+        long long e[3];
+        float z;
+        if (!eval_pixel<0>(t, i, j, e, z)) continue;
+        int slot = (j - y0) * kTile + (i - x0);
+        if constexpr (kVis) {
+            unsigned long long key = ((unsigned long long)fbits(z) << 32) | id;
+            atomicMin(&tile[slot], key);
+        } else {
+            atomicMin(&tile[slot], fbits(z));
+        }
+    }
+}
+
+template <bool kVis>
+__global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg,
+                                                           const uint32_t* offsets, const int* overflow,
+                                                           const uint32_t* entries, uint32_t cap, void* out) {
+    using K = typename KeyOf<kVis>::T;
+    __shared__ K tile[kTilePx];
+    __shared__ uint32_t bigq[kRasterThreads * 7];
+    __shared__ int nbig;
+    const int map = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int tx = blockIdx.x % cfg.tiles_x, ty = blockIdx.x / cfg.tiles_x;
+    const int x0 = tx * kTile, y0 = ty * kTile;
+    const int x1 = min(x0 + kTile - 1, cfg.w - 1), y1 = min(y0 + kTile - 1, cfg.h - 1);
+    const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    for (int p = tid; p < kTilePx; p += kRasterThreads) tile[p] = KeyOf<kVis>::clear;
+    if (tid == 0) nbig = 0;
+    float mvp[16];
+    mvps.load(map, mvp);
+    const bool fallback = overflow[map] != 0;
+    uint32_t begin = 0, end;
+    if (!fallback) {
+        begin = offsets[(size_t)map * (ntiles + 1) + blockIdx.x];
+        end = offsets[(size_t)map * (ntiles + 1) + blockIdx.x + 1];
+    } else {
+        end = (uint32_t)mesh.nt;
+    }
+    __syncthreads();
+    for (uint32_t base = begin; base < end; base += kRasterThreads) {
+        uint32_t k = base + tid;
+        if (k < end) {
+            int tri, piece0, npieces = 1;
+            if (!fallback) {
+                uint32_t entry = entries[(size_t)map * cap + k];
+                tri = (int)(entry >> 3);
+                piece0 = (int)(entry & 7u);
+            } else {
+                tri = (int)k;
+                piece0 = 0;
+            }
+            for (int piece = piece0; piece < piece0 + npieces; ++piece) {
+                Tri<0> t;
+                bool ok;
+                int np = get_piece<0>(mesh, mvp, tri, piece, cfg, t, ok);
+                if (fallback) npieces = np;
+                if (!ok) continue;
+                int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
+                if (rx0 > rx1 || ry0 > ry1) continue;
+                int area = (rx1 - rx0 + 1) * (ry1 - ry0 + 1);
+                uint32_t id = (uint32_t)tri * 8u + (uint32_t)piece;
+                if (area <= kSelfRasterMaxPx) {
+                    raster_rect<kVis>(t, id, x0, y0, rx0, rx1, ry0, ry1, 0, 1, tile);
+                } else {
+                    int slot = atomicAdd(&nbig, 1);
+                    bigq[slot] = id;
+                }
+            }
+        }
+        __syncthreads();
+        int nb = nbig;
+        for (int q = 0; q < nb; ++q) {
+            uint32_t id = bigq[q];
+            Tri<0> t;
+            bool ok;
+            get_piece<0>(mesh, mvp, (int)(id >> 3), (int)(id & 7u), cfg, t, ok);
+            int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
+            raster_rect<kVis>(t, id, x0, y0, rx0, rx1, ry0, ry1, tid, kRasterThreads, tile);
+        }
+        __syncthreads();
+        if (tid == 0) nbig = 0;
+        __syncthreads();
+    }
+    // Write the tile once.
+    const size_t plane = (size_t)cfg.w * cfg.h;
+    if constexpr (!kVis) {
+        float* dst = reinterpret_cast<float*>(out) + (size_t)map * plane;
+        const bool full = (x1 - x0 + 1 == kTile) && ((cfg.w & 3) == 0);
+        if (full) {
+            for (int q = tid; q < kTilePx / 4; q += kRasterThreads) {
+                int r = q / (kTile / 4), cidx = (q % (kTile / 4)) * 4;
+                int j = y0 + r;
+                if (j > y1) continue;
+                float4 v;
+                v.x = __uint_as_float(tile[r * kTile + cidx + 0]);
+                v.y = __uint_as_float(tile[r * kTile + cidx + 1]);
+                v.z = __uint_as_float(tile[r * kTile + cidx + 2]);
+                v.w = __uint_as_float(tile[r * kTile + cidx + 3]);
+                *reinterpret_cast<float4*>(dst + (size_t)j * cfg.w + x0 + cidx) = v;
+            }
+        } else {
+            for (int p = tid; p < kTilePx; p += kRasterThreads) {
+                int i = x0 + p % kTile, j = y0 + p / kTile;
+                if (i > x1 || j > y1) continue;
+                dst[(size_t)j * cfg.w + i] = __uint_as_float(tile[p]);
+            }
+        }
+    } else {
+        unsigned long long* dst = reinterpret_cast<unsigned long long*>(out) + (size_t)map * plane;
+        for (int p = tid; p < kTilePx; p += kRasterThreads) {
+            int i = x0 + p % kTile, j = y0 + p / kTile;
+            if (i > x1 || j > y1) continue;
+            dst[(size_t)j * cfg.w + i] = tile[p];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// G-buffer resolve: GBuffer.frag:35-37 (position, normal, gl_FrontFacing)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gbuffer_resolve(MeshView mesh, Mat4 mvp_arg, RasterCfg cfg,
+                                                         const unsigned long long* vis, float* pos_out,
+                                                         float* nrm_out) {
+    size_t npx = (size_t)cfg.w * cfg.h;
+    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npx) return;
+    unsigned long long key = vis[k];
+    float out[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (key != ~0ull) {
+        uint32_t id = (uint32_t)(key & 0xffffffffull);
+        Tri<6> t;
+        bool ok;
+        get_piece<6>(mesh, mvp_arg.m, (int)(id >> 3), (int)(id & 7u), cfg, t, ok);
+        int i = (int)(k % cfg.w), j = (int)(k / cfg.w);
+        long long e[3];
+        float z;
+        if (ok && eval_pixel<6>(t, i, j, e, z)) {
+            double b0 = (double)e[0] / (double)t.w[0];
+            double b1 = (double)e[1] / (double)t.w[1];
+            double b2 = (double)e[2] / (double)t.w[2];
+            double den = (b0 + b1) + b2;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                double num = (b0 * (double)t.a[0][q] + b1 * (double)t.a[1][q]) + b2 * (double)t.a[2][q];
+                out[q] = (float)(num / den);
+            }
+            if (!t.front) {
+                out[3] = -out[3];
+                out[4] = -out[4];
+                out[5] = -out[5];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        pos_out[q * npx + k] = out[q];
+        nrm_out[q * npx + k] = out[3 + q];
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Workspace {
+    uint32_t* counts;
+    uint32_t* offsets;
+    int* overflow;
+    uint32_t* entries;
+    void* vis;  // gbuffer only
+    uint32_t cap;
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+uint32_t entry_cap(int ntris, int ntiles) {
+    unsigned long long cap = 8ull * (unsigned long long)ntris + 4ull * (unsigned long long)ntiles + 1024ull;
+    if (cap > 0xF0000000ull) cap = 0xF0000000ull;
+    return (uint32_t)cap;
+}
+
+size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char* base) {
+    int tx = (w + kTile - 1) / kTile, ty = (h + kTile - 1) / kTile;
+    size_t ntiles = (size_t)tx * ty;
+    uint32_t cap = entry_cap(ntris, (int)ntiles);
+    size_t off = 0;
+    size_t o_counts = off; off = align256(off + sizeof(uint32_t) * ntiles * nmaps);
+    size_t o_offsets = off; off = align256(off + sizeof(uint32_t) * (ntiles + 1) * nmaps);
+    size_t o_over = off; off = align256(off + sizeof(int) * nmaps);
+    size_t o_entries = off; off = align256(off + sizeof(uint32_t) * (size_t)cap * nmaps);
+    size_t o_vis = off;
+    if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
+    if (ws) {
+        ws->counts = reinterpret_cast<uint32_t*>(base + o_counts);
+        ws->offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
+        ws->overflow = reinterpret_cast<int*>(base + o_over);
+        ws->entries = reinterpret_cast<uint32_t*>(base + o_entries);
+        ws->vis = vis ? reinterpret_cast<void*>(base + o_vis) : nullptr;
+        ws->cap = cap;
+    }
+    return off;
+}
+
+bool size_ok(int w, int h) { return w > 0 && h > 0 && w <= GI_MAX_EDGE && h <= GI_MAX_EDGE; }
+
+template <bool kVis>
+int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const RasterCfg& cfg, const Workspace& ws,
+               void* out, hipStream_t s) {
+    int ntiles = cfg.tiles_x * cfg.tiles_y;
+    size_t ncounts = (size_t)ntiles * nmaps;
+    size_t fb = (ncounts + 255) / 256;
+    unsigned fill_blocks = (unsigned)(fb < 4096 ? fb : 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, ws.counts, ncounts, 0u);
+    dim3 gb((mesh.nt + 255) / 256, nmaps);
+    if (mesh.nt > 0) {
+        hipLaunchKernelGGL(k_bin<false>, gb, dim3(256), 0, s, mesh, mvps, cfg, ws.counts, ws.offsets,
+                           ws.overflow, ws.entries, ws.cap);
+    }
+    hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, ws.counts, ws.offsets, ws.overflow, ntiles,
+                       ws.cap);
+    if (mesh.nt > 0) {
+        hipLaunchKernelGGL(k_bin<true>, gb, dim3(256), 0, s, mesh, mvps, cfg, ws.counts, ws.offsets, ws.overflow,
+                           ws.entries, ws.cap);
+    }
+    hipLaunchKernelGGL(k_raster<kVis>, dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg,
+                       ws.offsets, ws.overflow, ws.entries, ws.cap, out);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+}  // namespace
+}  // namespace gi
+
+using namespace gi;
+
+extern "C" {
+
+size_t gi_depth_pass_workspace_size(int ntris, int nmaps, int width, int height) {
+    if (ntris < 0 || nmaps <= 0 || !size_ok(width, height)) return 0;
+    return layout(ntris, nmaps, width, height, false, nullptr, nullptr);
+}
+
+int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                  const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                  float* depth_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+    if (!positions_dev || (!indices_dev && ntris > 0) || !mvps_dev || !depth_out_dev || !workspace_dev)
+        return GI_EINVAL;
+    if (nverts <= 0 || ntris < 0 || nmaps <= 0) return GI_EINVAL;
+    if (ntris >= (1 << 29)) return GI_ENOTSUP;
+    if (!size_ok(width, height)) return GI_ENOTSUP;
+    if ((reinterpret_cast<uintptr_t>(depth_out_dev) & 15u) != 0) return GI_EINVAL;
+    float off_f = opts ? opts->polygon_offset_factor : 0.0f;
+    float off_u = opts ? opts->polygon_offset_units : 0.0f;
+    if (!std::isfinite(off_f) || !std::isfinite(off_u)) return GI_ENONFINITE;
+    Workspace ws;
+    size_t need = layout(ntris, nmaps, width, height, false, &ws, reinterpret_cast<char*>(workspace_dev));
+    if (workspace_bytes < need) return GI_ENOSPACE;
+    MeshView mesh{positions_dev, nullptr, indices_dev, ntris};
+    MvpSource mvps{};
+    mvps.dev = mvps_dev;
+    RasterCfg cfg{width, height, (width + kTile - 1) / kTile, (height + kTile - 1) / kTile, off_f, off_u};
+    return run_raster<false>(mesh, mvps, nmaps, cfg, ws, depth_out_dev, reinterpret_cast<hipStream_t>(stream));
+}
+
+size_t gi_gbuffer_pass_workspace_size(int ntris, int width, int height) {
+    if (ntris < 0 || !size_ok(width, height)) return 0;
+    return layout(ntris, 1, width, height, true, nullptr, nullptr);
+}
+
+int gi_gbuffer_pass(const float* positions_dev, const float* normals_dev, int nverts, const int32_t* indices_dev,
+                    int ntris, const float mvp[16], int width, int height, float* position_out_dev,
+                    float* normal_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+    if (!positions_dev || !normals_dev || (!indices_dev && ntris > 0) || !mvp || !position_out_dev ||
+        !normal_out_dev || !workspace_dev)
+        return GI_EINVAL;
+    if (nverts <= 0 || ntris < 0) return GI_EINVAL;
+    if (ntris >= (1 << 29)) return GI_ENOTSUP;
+    if (!size_ok(width, height)) return GI_ENOTSUP;
+    for (int i = 0; i < 16; ++i)
+        if (!std::isfinite(mvp[i])) return GI_ENONFINITE;
+    Workspace ws;
+    size_t need = layout(ntris, 1, width, height, true, &ws, reinterpret_cast<char*>(workspace_dev));
+    if (workspace_bytes < need) return GI_ENOSPACE;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    MeshView mesh{positions_dev, normals_dev, indices_dev, ntris};
+    MvpSource mvps{};
+    mvps.dev = nullptr;
+    for (int i = 0; i < 16; ++i) mvps.single.m[i] = mvp[i];
+    RasterCfg cfg{width, height, (width + kTile - 1) / kTile, (height + kTile - 1) / kTile, 0.0f, 0.0f};
+    int rc = run_raster<true>(mesh, mvps, 1, cfg, ws, ws.vis, s);
+    if (rc != GI_OK) return rc;
+    size_t npx = (size_t)width * height;
+    hipLaunchKernelGGL(k_gbuffer_resolve, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, mesh, mvps.single,
+                       cfg, reinterpret_cast<const unsigned long long*>(ws.vis), position_out_dev, normal_out_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+"""Host-side pass interface over libgi.so, mirroring the reference's passes.
+
+The reference drives each technique with free pass functions in its apps'
+main.cpp and a ``ShadowParams`` struct handed to MyGLGeometryViewer::configure*:
+
+    renderShadowMap      ShadowMapping/src/main.cpp:350-361
+    renderGBuffer        ShadowMapping/src/main.cpp:363-372
+    computeHardShadows   ShadowMapping/src/main.cpp:400-414
+    renderMonteCarlo     SoftShadowMapping/src/main.cpp:756-811
+
+The functions below keep those names (in snake case) and that order of work.
+Device memory and streams come from torch; every pass runs a HIP kernel of
+libgi.so on the current torch stream, and there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, camera
+from .scenes import Mesh, View
+
+TECHNIQUES = {"naive": _lib.GI_NAIVE, "pcf": _lib.GI_PCF, "rbsm": _lib.GI_RBSM,
+              "rpcf": _lib.GI_RPCF, "rbsm_filtered": _lib.GI_RBSM_FILTERED}
+
+
+def _stream_handle(stream=None) -> ctypes.c_void_p:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dptr(t: torch.Tensor) -> ctypes.c_void_p:
+    """Device pointer of a contiguous tensor (NULL for an empty one)."""
+    if not t.is_cuda:
+        raise ValueError("libgi passes take device tensors")
+    if not t.is_contiguous():
+        raise ValueError("libgi passes take contiguous tensors")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _f(arr, n) -> ctypes.Array:
+    a = np.ascontiguousarray(arr, dtype=np.float32).reshape(-1)
+    if a.size != n:
+        raise ValueError(f"expected {n} floats, got {a.size}")
+    return (ctypes.c_float * n)(*a.tolist())
+
+
+class Workspace:
+    """Grow-only device scratch owned by the caller side (the library never
+    allocates)."""
+
+    def __init__(self):
+        self._buf: dict = {}
+
+    def get(self, nbytes: int, device) -> torch.Tensor:
+        key = str(device)
+        buf = self._buf.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+            self._buf[key] = buf
+        return buf
+
+
+_WS = Workspace()
+
+
+@dataclass
+class DeviceMesh:
+    positions: torch.Tensor  # (nv, 3) float32
+    normals: torch.Tensor    # (nv, 3) float32
+    indices: torch.Tensor    # (nt, 3) int32
+
+    @staticmethod
+    def from_mesh(m: Mesh, device="cuda") -> "DeviceMesh":
+        return DeviceMesh(torch.from_numpy(np.ascontiguousarray(m.positions)).to(device),
+                          torch.from_numpy(np.ascontiguousarray(m.normals)).to(device),
+                          torch.from_numpy(np.ascontiguousarray(m.indices)).to(device))
+
+    @property
+    def nverts(self) -> int:
+        return int(self.positions.shape[0])
+
+    @property
+    def ntris(self) -> int:
+        return int(self.indices.shape[0])
+
+
+# ---------------------------------------------------------------------------
+# Matrices (MyGLGeometryViewer::configureAmbient / configureShadow)
+# ---------------------------------------------------------------------------
+def view_matrices(v: View):
+    """(P, V, P*V) of a view; model = identity (MyGLGeometryViewer.cpp:14-26)."""
+    if v.ortho is not None:
+        P = camera.ortho(*v.ortho)
+    else:
+        P = camera.perspective(v.fov, v.width / v.height, v.znear, v.zfar)
+    V = camera.lookat(v.eye, v.at, v.up)
+    return P, V, camera.mat4_mul(P, V)
+
+
+def biased(mvp: np.ndarray, zbias: float = 0.0) -> np.ndarray:
+    """B * lightMVP (MyGLGeometryViewer.cpp:139-146), constant bias in B[3][2]."""
+    return camera.mat4_mul(camera.bias_matrix(zbias), mvp)
+
+
+@dataclass
+class ShadowParams:
+    """Mirror of ShadowParams (ShadowMapping/include/Viewers/ShadowParams.h:6-37)
+    restricted to the fields the hard-shadow passes read."""
+    lightMVP: np.ndarray                 # unbiased P*V*M of the light
+    shadowMapWidth: int
+    shadowMapHeight: int
+    technique: str = "naive"
+    kernelOrder: int = 3
+    penumbraSize: int = 1
+    maxSearch: int = 16
+    depthThreshold: float = 2.5e-6
+    shadowIntensity: float = 0.25
+    zbias: float = 0.0
+    preEvaluation: bool = False
+    MV: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=np.float32).reshape(-1))
+    normalMatrix: np.ndarray = field(default_factory=lambda: np.eye(3, dtype=np.float32).reshape(-1))
+    lightPosition: tuple = (0.0, 0.0, 0.0)
+
+    def to_c(self) -> _lib.GiShadowParams:
+        p = _lib.GiShadowParams()
+        p.light_mvp = _f(biased(self.lightMVP, self.zbias), 16)
+        p.shadow_map_width = int(self.shadowMapWidth)
+        p.shadow_map_height = int(self.shadowMapHeight)
+        p.technique = TECHNIQUES[self.technique]
+        p.kernel_order = int(self.kernelOrder)
+        p.penumbra_size = int(self.penumbraSize)
+        p.max_search = int(self.maxSearch)
+        p.depth_threshold = float(self.depthThreshold)
+        p.shadow_intensity = float(self.shadowIntensity)
+        p.pre_evaluation = 1 if self.preEvaluation else 0
+        p.mv = _f(self.MV, 16)
+        p.normal_matrix = _f(self.normalMatrix, 9)
+        p.light_position = _f(self.lightPosition, 3)
+        return p
+
+
+def phong_uniforms(cam: View, light_eye) -> tuple:
+    """MV, normalMatrix and lightPosition as displaySceneFromGBuffer uploads
+    them: lightPosition is the world light rotated 180 deg about y
+    (ShadowMapping/src/main.cpp:309, configurePhong MyGLGeometryViewer.cpp:108-134)."""
+    MV = camera.lookat(cam.eye, cam.at, cam.up)
+    nm = camera.normal_matrix(MV)
+    lx, ly, lz = (float(c) for c in light_eye)
+    return MV, nm, (-lx, ly, -lz)
+
+
+# ---------------------------------------------------------------------------
+# Passes
+# ---------------------------------------------------------------------------
+def render_shadow_map(mesh: DeviceMesh, mvps: np.ndarray, width: int, height: int,
+                      polygon_offset=(0.0, 0.0), out: torch.Tensor | None = None,
+                      stream=None) -> torch.Tensor:
+    """Depth of `mesh` from each light in `mvps` ((S, 16) unbiased P*V*M) into
+    an (S, height, width) FP32 tensor (renderShadowMap, and renderMonteCarlo's
+    per-sample loop SoftShadowMapping/src/main.cpp:774-793)."""
+    m = np.ascontiguousarray(mvps, dtype=np.float32).reshape(-1, 16)
+    S = m.shape[0]
+    dev = mesh.positions.device
+    mvps_dev = torch.from_numpy(m).to(dev)
+    if out is None:
+        out = torch.empty((S, height, width), dtype=torch.float32, device=dev)
+    nbytes = _lib.lib().gi_depth_pass_workspace_size(mesh.ntris, S, width, height)
+    if nbytes == 0:
+        raise _lib.GiError("gi_depth_pass_workspace_size", _lib.GI_EINVAL, "bad size")
+    ws = _WS.get(nbytes, dev)
+    opts = _lib.GiRasterOpts(float(polygon_offset[0]), float(polygon_offset[1]))
+    _lib.call("gi_depth_pass", _dptr(mesh.positions), mesh.nverts, _dptr(mesh.indices), mesh.ntris,
+              _dptr(mvps_dev), S, width, height, ctypes.byref(opts), _dptr(out), _dptr(ws),
+              ctypes.c_size_t(ws.numel()), _stream_handle(stream))
+    return out
+
+
+@dataclass
+class GBuffer:
+    position: torch.Tensor  # (3, H, W)
+    normal: torch.Tensor    # (3, H, W), facing-flipped
+
+    @property
+    def width(self) -> int:
+        return int(self.position.shape[2])
+
+    @property
+    def height(self) -> int:
+        return int(self.position.shape[1])
+
+    def to_c(self, with_normals: bool = True) -> _lib.GiGBuffer:
+        return _lib.GiGBuffer(_dptr(self.position).value,
+                              _dptr(self.normal).value if with_normals else None,
+                              self.width, self.height)
+
+
+def render_gbuffer(mesh: DeviceMesh, mvp: np.ndarray, width: int, height: int, stream=None) -> GBuffer:
+    """Camera G-buffer (renderGBuffer, GBuffer.vert/.frag)."""
+    dev = mesh.positions.device
+    pos = torch.empty((3, height, width), dtype=torch.float32, device=dev)
+    nrm = torch.empty((3, height, width), dtype=torch.float32, device=dev)
+    nbytes = _lib.lib().gi_gbuffer_pass_workspace_size(mesh.ntris, width, height)
+    ws = _WS.get(nbytes, dev)
+    _lib.call("gi_gbuffer_pass", _dptr(mesh.positions), _dptr(mesh.normals), mesh.nverts, _dptr(mesh.indices),
+              mesh.ntris, _f(mvp, 16), width, height, _dptr(pos), _dptr(nrm), _dptr(ws),
+              ctypes.c_size_t(ws.numel()), _stream_handle(stream))
+    return GBuffer(pos, nrm)
+
+
+def compute_hard_shadows(gbuf: GBuffer, depth: torch.Tensor, params: ShadowParams,
+                         out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """Hard-shadow screen pass (computeHardShadows); returns the R channel."""
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=gbuf.position.device)
+    g = gbuf.to_c(with_normals=params.preEvaluation)
+    p = params.to_c()
+    _lib.call("gi_hard_shadow_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out),
+              _stream_handle(stream))
+    return out
+
+
+@dataclass
+class SoftParams:
+    shadowIntensity: float = 0.25
+    pcf: bool = False
+    kernelOrder: int = 3
+    penumbra: float = 1.0
+    inclusive: bool = True  # the reference loop (AccurateSoftShadow.frag:108-109)
+
+    def to_c(self) -> _lib.GiSoftParams:
+        return _lib.GiSoftParams(float(self.shadowIntensity), 1 if self.pcf else 0, int(self.kernelOrder),
+                                 float(self.penumbra), 1 if self.inclusive else 0)
+
+
+def soft_uniforms(light_mvps: np.ndarray, zbias: float = 0.0):
+    """(common B*lightMVP, lightMVPTrans) of configureShadow
+    (SoftShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:168-195): columns 0-2
+    of the LAST sample's biased matrix, column 3 of every sample's."""
+    m = np.ascontiguousarray(light_mvps, dtype=np.float32).reshape(-1, 16)
+    b = np.stack([biased(x, zbias) for x in m])
+    return b[-1].copy(), camera.soft_light_trans(b)
+
+
+def render_monte_carlo(gbuf: GBuffer, maps: torch.Tensor, light_mvps: np.ndarray, params: SoftParams,
+                       zbias: float = 0.0, out: torch.Tensor | None = None, trans_dev: torch.Tensor | None = None,
+                       stream=None) -> torch.Tensor:
+    """Area-light screen pass of renderMonteCarlo (AccurateSoftShadow.frag)."""
+    S, smh, smw = (int(x) for x in maps.shape)
+    common, trans = soft_uniforms(light_mvps, zbias)
+    if trans.shape[0] != S:
+        raise ValueError("one light matrix per depth map is required")
+    if trans_dev is None:
+        trans_dev = torch.from_numpy(trans).to(maps.device)
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=maps.device)
+    g = gbuf.to_c(with_normals=False)
+    p = params.to_c()
+    _lib.call("gi_soft_shadow_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, _f(common, 16),
+              _dptr(trans_dev), ctypes.byref(p), _dptr(out), _stream_handle(stream))
+    return out

@@ -1,0 +1,199 @@
+/*
+ * gi.h — C ABI of the MI355X shadow pipeline (libgi.so).
+ *
+ * Drop-in boundary for the reference's per-technique shadow passes
+ * (MarcioCerqueira/GlobalIllumination).  The reference has no FFI: its pass
+ * interface is the set of `void(void)` pass functions in each app's main.cpp,
+ * the `ShadowParams` struct handed by value to MyGLGeometryViewer::configure*,
+ * and the GL texture contract (D32F depth texture, RGBA32F G-buffer, R = shadow
+ * output).  Each entry point below names the reference function it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: GI_OK (0) or a negative gi_status.
+ *     Nothing calls exit(); errors are never printed by the library.
+ *   - Matrices are column-major float[16] (glm / OpenGL layout).
+ *   - Images are row-major with row 0 at the bottom (GL window y), so texel
+ *     (i, j) of a W x H map is map[j*W + i].
+ *   - Pointers whose name ends in _dev are device (HBM) pointers; all other
+ *     pointers are host pointers.  All buffers are owned by the caller; the
+ *     passes allocate nothing and only use the caller's workspace.
+ *   - `stream` is a hipStream_t (NULL = the legacy default stream).  Passes
+ *     are stream-ordered and asynchronous; the library keeps no hidden state
+ *     between calls, so calls on different streams are independent.
+ */
+#ifndef GI_H
+#define GI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GI_API_VERSION 1
+
+typedef enum gi_status {
+    GI_OK = 0,
+    GI_EINVAL = -1,      /* null pointer, non-positive size, bad enum, too many samples */
+    GI_ENONFINITE = -2,  /* a matrix or scalar parameter is NaN / Inf */
+    GI_EDEVICE = -3,     /* a HIP call failed */
+    GI_ENOSPACE = -4,    /* workspace smaller than the matching *_workspace_size() */
+    GI_ENOTSUP = -5      /* size outside what the kernels support */
+} gi_status;
+
+typedef void* gi_stream; /* hipStream_t */
+
+/* Maximum area-light samples of one soft pass: AccurateSoftShadow.frag:9 caps
+ * the reference at 289; HBM holds far more. */
+#define GI_MAX_SAMPLES 4096
+/* Largest shadow-map / view edge the raster supports (guard band arithmetic). */
+#define GI_MAX_EDGE 16384
+
+int gi_api_version(void);
+const char* gi_status_string(int status);
+
+/* ------------------------------------------------------------------------ */
+/* Light camera and texture-space projection (host-side, FP32 glm 0.9.3.1)   */
+/* ------------------------------------------------------------------------ */
+
+/* glm::perspective with fovy in DEGREES.  Replaces the projection built by
+ * MyGLGeometryViewer::configureAmbient (ShadowMapping/src/Viewers/
+ * MyGLGeometryViewer.cpp:14-26) via glm/gtc/matrix_transform.inl:223-244. */
+int gi_perspective(float fovy_deg, float aspect, float znear, float zfar, float out[16]);
+/* glm::ortho (glm/gtc/matrix_transform.inl:163-182); the reference has no
+ * orthographic light, used for directional lights. */
+int gi_ortho(float l, float r, float b, float t, float znear, float zfar, float out[16]);
+/* glm::lookAt (glm/gtc/matrix_transform.inl:383-411), the view matrix of
+ * MyGLGeometryViewer::configureAmbient. */
+int gi_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);
+/* glm mat4 * mat4 (glm/core/type_mat4x4.inl:757-777). */
+int gi_mat4_mul(const float a[16], const float b[16], float out[16]);
+/* Texture-space bias matrix B of MyGLGeometryViewer::configureShadow
+ * (ShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:139-145).  A constant
+ * window-depth bias is folded into B[3][2] = 0.5 - zbias (zbias = 0 gives the
+ * reference matrix exactly). */
+int gi_bias_matrix(float zbias, float out[16]);
+/* UniformSampledLightSource::computeUniformSampling
+ * (SoftShadowMapping/src/Scene/LightSource/UniformSampledLightSource.cpp:27-37):
+ * offset `base` on a sqrt(N) x sqrt(N) grid of side `size` in world XY. */
+int gi_uniform_sampling(const float base[3], int size, int nsamples, int index, float out[3]);
+/* Column 3 of B*lightMVP_i for each sample (the lightMVPTrans[] uniform,
+ * SoftShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:184-195).
+ * light_mvps: S x 16 (already biased); trans_out: S x 4. */
+int gi_soft_light_trans(const float* light_mvps, int nsamples, float* trans_out);
+
+/* ------------------------------------------------------------------------ */
+/* Light-view depth pass (a2)                                                */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces renderShadowMap / displaySceneFromLightPOV
+ * (ShadowMapping/src/main.cpp:221-274, :350-361) and the per-sample loop of
+ * renderMonteCarlo (SoftShadowMapping/src/main.cpp:774-793): depth of every
+ * triangle into an FP32 map cleared to 1.0, depth test LESS, no culling. */
+typedef struct gi_raster_opts {
+    float polygon_offset_factor; /* glPolygonOffset(factor, units); both 0 = off */
+    float polygon_offset_units;  /* the reference apps use (4, 20), main.cpp:246 */
+} gi_raster_opts;
+
+/* Workspace bytes for gi_depth_pass over `nmaps` maps of one mesh. */
+size_t gi_depth_pass_workspace_size(int ntris, int nmaps, int width, int height);
+
+/* positions_dev: nverts x 3 floats; indices_dev: ntris x 3 int32;
+ * mvps_dev: nmaps x 16 floats, the unbiased P*V*M of each map (device);
+ * depth_out_dev: nmaps x height x width floats. */
+int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                  const float* mvps_dev, int nmaps, int width, int height,
+                  const gi_raster_opts* opts, float* depth_out_dev,
+                  void* workspace_dev, size_t workspace_bytes, gi_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Camera G-buffer (input format of the screen passes, a6)                   */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces renderGBuffer (ShadowMapping/src/main.cpp:363-372) with
+ * GBuffer.vert:15-21 / GBuffer.frag:35-37: perspective-correct object-space
+ * position and normal of the nearest surface.  Output layout (SoA planes):
+ *   position_out_dev: 3 planes (x, y, z) of width*height floats,
+ *   normal_out_dev:   3 planes, the normal NEGATED where the surface is
+ *                     back-facing (gl_FrontFacing == false),
+ * and zeros for background pixels (the reference clears to 0 and detects the
+ * background by vertex.x == 0, Shadow.frag:244). */
+size_t gi_gbuffer_pass_workspace_size(int ntris, int width, int height);
+int gi_gbuffer_pass(const float* positions_dev, const float* normals_dev, int nverts,
+                    const int32_t* indices_dev, int ntris, const float mvp[16],
+                    int width, int height, float* position_out_dev, float* normal_out_dev,
+                    void* workspace_dev, size_t workspace_bytes, gi_stream stream);
+
+typedef struct gi_gbuffer {
+    const float* position_dev; /* 3 planes of width*height */
+    const float* normal_dev;   /* 3 planes, facing-flipped; may be NULL when unused */
+    int width, height;
+} gi_gbuffer;
+
+/* ------------------------------------------------------------------------ */
+/* Hard-shadow screen pass (a8-a12)                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Technique, i.e. the ShadowParams flags of ShadowMapping/include/Viewers/
+ * ShadowParams.h:18-31 and the shader computeHardShadows picks
+ * (ShadowMapping/src/main.cpp:400-414). */
+typedef enum gi_technique {
+    GI_NAIVE = 0,         /* Shadow.frag naive == 1 */
+    GI_PCF = 1,           /* Shadow.frag bilinearPCF == 1, PCF() :86-116 */
+    GI_RBSM = 2,          /* NonConservativeSMSR.frag SMSR == 1 :288-304 */
+    GI_RPCF = 3,          /* NonConservativeSMSR.frag RPCFPlusSMSR == 1 :306-349 */
+    GI_RBSM_FILTERED = 4  /* FilteredRBSM.frag (RSMSS) :306-349 */
+} gi_technique;
+
+/* Mirrors ShadowParams (ShadowMapping/include/Viewers/ShadowParams.h:6-37)
+ * plus the Phong uniforms configurePhong uploads for the pre-evaluation. */
+typedef struct gi_shadow_params {
+    float light_mvp[16];     /* B * lightMVP (configureShadow :146) */
+    int   shadow_map_width;  /* shadowMapWidth */
+    int   shadow_map_height; /* shadowMapHeight */
+    int   technique;         /* gi_technique */
+    int   kernel_order;      /* kernelOrder (PCF / RPCF loop) */
+    int   penumbra_size;     /* penumbraSize (PCF half-width in texels) */
+    int   max_search;        /* maxSearch (RBSM edge walk) */
+    float depth_threshold;   /* depthThreshold (RBSM) */
+    float shadow_intensity;  /* shadowIntensity */
+    int   pre_evaluation;    /* 1: computePreEvaluationBasedOnNormalOrientation */
+    float mv[16];            /* MV (camera view * model) */
+    float normal_matrix[9];  /* normalMatrix, column-major mat3 */
+    float light_position[3]; /* lightPosition as configurePhong uploads it */
+} gi_shadow_params;
+
+/* Replaces computeHardShadows -> displaySceneFromGBuffer(shader)
+ * (ShadowMapping/src/main.cpp:400-414, :302-348).  depth_dev: one map of
+ * shadow_map_height x shadow_map_width; out_dev: width*height floats holding
+ * the R channel (1 = lit, shadow_intensity = shadowed, 0 = background). */
+int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev,
+                        const gi_shadow_params* params, float* out_dev, gi_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Area-light soft-shadow screen pass (a13)                                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gi_soft_params {
+    float shadow_intensity;
+    int   pcf;          /* 0: one tap per sample (AccurateSoftShadow.frag:94-97);
+                           1: PCF loop per sample (:99-121) */
+    int   kernel_order; /* PCF loop: step = 2*penumbra/kernel_order (reference: 3) */
+    float penumbra;     /* PCF loop: half-width in texels (reference: 1) */
+    int   inclusive;    /* 1: `<=` loop bound as in the reference (4x4 taps for
+                           k=3); 0: `<` as Shadow.frag's PCF (3x3 taps) */
+} gi_soft_params;
+
+/* Replaces renderMonteCarlo's screen pass (SoftShadowMapping/src/main.cpp:806,
+ * AccurateSoftShadow.frag:52-133).  maps_dev: nsamples x smh x smw floats;
+ * light_mvp: host B*lightMVP whose columns 0-2 all samples share; trans_dev:
+ * device nsamples x 4 floats (gi_soft_light_trans).  out_dev: width*height. */
+int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples,
+                        int smw, int smh, const float light_mvp[16], const float* trans_dev,
+                        const gi_soft_params* params, float* out_dev, gi_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GI_H */

@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (oracle/libgi_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker of the HIP path.  The product
+package (globalillumination_amd) never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libgi_oracle.so"
+
+NAIVE, PCF, RBSM, RPCF, RBSM_FILTERED = range(5)
+TECHNIQUES = {"naive": NAIVE, "pcf": PCF, "rbsm": RBSM, "rpcf": RPCF, "rbsm_filtered": RBSM_FILTERED}
+
+F16 = ctypes.c_float * 16
+
+
+class OrParams(ctypes.Structure):
+    _fields_ = [("light_mvp", F16), ("sm_width", ctypes.c_int), ("sm_height", ctypes.c_int),
+                ("technique", ctypes.c_int), ("kernel_order", ctypes.c_int), ("penumbra_size", ctypes.c_int),
+                ("max_search", ctypes.c_int), ("depth_threshold", ctypes.c_float),
+                ("shadow_intensity", ctypes.c_float), ("pre_evaluation", ctypes.c_int), ("mv", F16),
+                ("normal_matrix", ctypes.c_float * 9), ("light_position", ctypes.c_float * 3)]
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        sig = {
+            "or_perspective": [f, f, f, f, vp], "or_ortho": [f] * 6 + [vp], "or_lookat": [vp] * 4,
+            "or_mat4_mul": [vp] * 3, "or_bias_matrix": [f, vp], "or_uniform_sampling": [vp, i, i, i, vp],
+            "or_fetch": [vp, i, i, f, f],
+            "or_hard_shadow": [vp, vp, i, i, vp, ctypes.POINTER(OrParams), vp],
+            "or_soft_shadow": [vp, i, i, vp, i, i, i, vp, vp, f, i, i, f, i, vp],
+            "or_depth_raster": [vp, i, vp, i, vp, i, i, f, f, vp],
+            "or_depth_raster_brute": [vp, i, vp, i, vp, i, i, f, f, vp],
+            "or_visibility_raster": [vp, i, vp, i, vp, i, i, vp],
+            "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp],
+            "or_set_threads": [i], "or_get_threads": [],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_float if name == "or_fetch" else (None if name in (
+                "or_perspective", "or_ortho", "or_lookat", "or_mat4_mul", "or_bias_matrix",
+                "or_uniform_sampling", "or_set_threads") else ctypes.c_int)
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _c32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_threads(int(n))
+
+
+def threads() -> int:
+    return int(lib().or_get_threads())
+
+
+def perspective(fovy, aspect, zn, zf):
+    o = np.zeros(16, np.float32)
+    lib().or_perspective(fovy, aspect, zn, zf, _p(o))
+    return o
+
+
+def ortho(l, r, b, t, zn, zf):
+    o = np.zeros(16, np.float32)
+    lib().or_ortho(l, r, b, t, zn, zf, _p(o))
+    return o
+
+
+def lookat(eye, at, up):
+    e, a, u = _c32(eye), _c32(at), _c32(up)
+    o = np.zeros(16, np.float32)
+    lib().or_lookat(_p(e), _p(a), _p(u), _p(o))
+    return o
+
+
+def mat4_mul(a, b):
+    a, b = _c32(a), _c32(b)
+    o = np.zeros(16, np.float32)
+    lib().or_mat4_mul(_p(a), _p(b), _p(o))
+    return o
+
+
+def bias_matrix(zbias=0.0):
+    o = np.zeros(16, np.float32)
+    lib().or_bias_matrix(zbias, _p(o))
+    return o
+
+
+def uniform_sampling(base, size, n, index):
+    b = _c32(base)
+    o = np.zeros(3, np.float32)
+    lib().or_uniform_sampling(_p(b), size, n, index, _p(o))
+    return o
+
+
+def fetch(depth: np.ndarray, s: float, t: float) -> float:
+    d = _c32(depth)
+    h, w = d.shape
+    return float(lib().or_fetch(_p(d), w, h, s, t))
+
+
+def make_params(light_mvp_biased, smw, smh, technique="naive", kernel_order=3, penumbra_size=1,
+                max_search=16, depth_threshold=2.5e-6, shadow_intensity=0.25, pre_evaluation=False,
+                mv=None, normal_matrix=None, light_position=(0, 0, 0)) -> OrParams:
+    p = OrParams()
+    p.light_mvp = F16(*_c32(light_mvp_biased).reshape(-1).tolist())
+    p.sm_width, p.sm_height = int(smw), int(smh)
+    p.technique = TECHNIQUES[technique] if isinstance(technique, str) else int(technique)
+    p.kernel_order, p.penumbra_size, p.max_search = int(kernel_order), int(penumbra_size), int(max_search)
+    p.depth_threshold, p.shadow_intensity = float(depth_threshold), float(shadow_intensity)
+    p.pre_evaluation = 1 if pre_evaluation else 0
+    mv = np.eye(4, dtype=np.float32).reshape(-1) if mv is None else _c32(mv).reshape(-1)
+    nm = np.eye(3, dtype=np.float32).reshape(-1) if normal_matrix is None else _c32(normal_matrix).reshape(-1)
+    p.mv = F16(*mv.tolist())
+    p.normal_matrix = (ctypes.c_float * 9)(*nm.tolist())
+    p.light_position = (ctypes.c_float * 3)(*[float(x) for x in light_position])
+    return p
+
+
+def hard_shadow(pos: np.ndarray, nrm: np.ndarray | None, depth: np.ndarray, params: OrParams) -> np.ndarray:
+    pos = _c32(pos)
+    _, h, w = pos.shape
+    nrm_c = _c32(nrm) if nrm is not None else None
+    d = _c32(depth)
+    out = np.empty((h, w), np.float32)
+    rc = lib().or_hard_shadow(_p(pos), _p(nrm_c) if nrm_c is not None else None, w, h, _p(d),
+                              ctypes.byref(params), _p(out))
+    if rc:
+        raise RuntimeError(f"or_hard_shadow failed ({rc})")
+    return out
+
+
+def soft_shadow(pos: np.ndarray, maps: np.ndarray, light_mvp_biased, trans, si=0.25, pcf=False,
+                kernel_order=3, penumbra=1.0, inclusive=True) -> np.ndarray:
+    pos, maps = _c32(pos), _c32(maps)
+    _, h, w = pos.shape
+    S, smh, smw = maps.shape
+    m, t = _c32(light_mvp_biased).reshape(-1), _c32(trans).reshape(-1)
+    out = np.empty((h, w), np.float32)
+    rc = lib().or_soft_shadow(_p(pos), w, h, _p(maps), S, smw, smh, _p(m), _p(t), si, 1 if pcf else 0,
+                              kernel_order, penumbra, 1 if inclusive else 0, _p(out))
+    if rc:
+        raise RuntimeError(f"or_soft_shadow failed ({rc})")
+    return out
+
+
+def _mesh_args(positions, indices):
+    pos = _c32(positions).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+    return pos, idx
+
+
+def depth_raster(positions, indices, mvp, w, h, offset=(0.0, 0.0), brute=False) -> np.ndarray:
+    pos, idx = _mesh_args(positions, indices)
+    m = _c32(mvp).reshape(-1)
+    out = np.empty((h, w), np.float32)
+    fn = lib().or_depth_raster_brute if brute else lib().or_depth_raster
+    rc = fn(_p(pos), pos.shape[0], _p(idx), idx.shape[0], _p(m), w, h, offset[0], offset[1], _p(out))
+    if rc:
+        raise RuntimeError(f"or_depth_raster failed ({rc})")
+    return out
+
+
+def visibility_raster(positions, indices, mvp, w, h) -> np.ndarray:
+    pos, idx = _mesh_args(positions, indices)
+    m = _c32(mvp).reshape(-1)
+    out = np.empty((h, w), np.uint64)
+    rc = lib().or_visibility_raster(_p(pos), pos.shape[0], _p(idx), idx.shape[0], _p(m), w, h, _p(out))
+    if rc:
+        raise RuntimeError(f"or_visibility_raster failed ({rc})")
+    return out
+
+
+def gbuffer(positions, normals, indices, mvp, w, h):
+    pos, idx = _mesh_args(positions, indices)
+    nrm = _c32(normals).reshape(-1, 3)
+    m = _c32(mvp).reshape(-1)
+    po = np.empty((3, h, w), np.float32)
+    no = np.empty((3, h, w), np.float32)
+    rc = lib().or_gbuffer(_p(pos), _p(nrm), pos.shape[0], _p(idx), idx.shape[0], _p(m), w, h, _p(po), _p(no))
+    if rc:
+        raise RuntimeError(f"or_gbuffer failed ({rc})")
+    return po, no

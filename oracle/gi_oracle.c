@@ -1,0 +1,951 @@
+/*
+ * gi_oracle.c — TEST INFRASTRUCTURE ONLY (parity checker + "port" CPU baseline).
+ *
+ * A plain-C, FP32 restatement of the reference's shadow passes, written op by
+ * op from the GLSL so that the HIP kernels can be checked bit for bit.  Built
+ * with -ffp-contract=off: every a*b+c below is a rounded multiply followed by
+ * a rounded add, exactly as written.  Division and sqrt are IEEE correctly
+ * rounded.  GLSL helpers are restated with the GLSL spec definitions:
+ *   mix(x,y,a) = x*(1-a) + y*a,  max(x,y) = x < y ? y : x,
+ *   min(x,y) = y < x ? y : x,    clamp(x,a,b) = min(max(x,a),b),
+ *   fract(x) = x - floor(x).
+ *
+ * Rasterization follows the OpenGL rules the reference's fixed-function depth
+ * pass relies on, with the choices that the GL spec leaves to the
+ * implementation stated here:
+ *   - clip against near (z >= -w) and far (z <= w), and against a guard band
+ *     |x|,|y| <= 4w; Sutherland-Hodgman, intersections computed from the
+ *     inside vertex so shared edges clip identically; fan triangulation;
+ *   - window coordinates snapped to 1/256 pixel (rintf);
+ *   - pixel centres at (i+0.5, j+0.5); top-left tie rule in GL's y-up window
+ *     space; no culling (ShadowMapping/src/main.cpp:271);
+ *   - window depth interpolated linearly in screen space in FP64 from the
+ *     integer edge functions, then rounded to FP32 and clamped to [0,1];
+ *   - polygon offset o = factor*max|dz/dx|,|dz/dy| + units*2^(e-23).
+ *
+ * This file is test infrastructure: nothing under globalillumination_amd/
+ * links or calls it.
+ */
+#include "gi_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static int g_threads = 0;
+void or_set_threads(int n) { g_threads = n; }
+int or_get_threads(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+#ifdef _OPENMP
+#define OMP_THREADS num_threads(or_get_threads())
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* GLSL helper semantics                                                      */
+/* ------------------------------------------------------------------------ */
+static inline float gmax(float x, float y) { return x < y ? y : x; }
+static inline float gmin(float x, float y) { return y < x ? y : x; }
+static inline float gclamp(float x, float a, float b) { return gmin(gmax(x, a), b); }
+static inline float gmix(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+static inline float gfract(float x) { return x - floorf(x); }
+static inline float cmp_le(float z, float d) { return (z <= d) ? 1.0f : 0.0f; }
+
+/* ------------------------------------------------------------------------ */
+/* glm 0.9.3.1 (a1, a4)                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* include/glm/gtc/matrix_transform.inl:223-244; radians() per
+ * include/glm/core/func_trigonometric.inl:35-44 */
+void or_perspective(float fovy, float aspect, float zn, float zf, float out[16])
+{
+    const float pi = 3.1415926535897932384626433832795f;
+    float rad = (fovy / 2.0f) * (pi / 180.0f);
+    float range = tanf(rad) * zn;
+    float left = -range * aspect;
+    float right = range * aspect;
+    float bottom = -range;
+    float top = range;
+    memset(out, 0, 16 * sizeof(float));
+    out[0 * 4 + 0] = (2.0f * zn) / (right - left);
+    out[1 * 4 + 1] = (2.0f * zn) / (top - bottom);
+    out[2 * 4 + 2] = -(zf + zn) / (zf - zn);
+    out[2 * 4 + 3] = -1.0f;
+    out[3 * 4 + 2] = -(2.0f * zf * zn) / (zf - zn);
+}
+
+/* include/glm/gtc/matrix_transform.inl:163-182 */
+void or_ortho(float l, float r, float b, float t, float zn, float zf, float out[16])
+{
+    memset(out, 0, 16 * sizeof(float));
+    out[0] = out[5] = out[10] = out[15] = 1.0f;
+    out[0 * 4 + 0] = 2.0f / (r - l);
+    out[1 * 4 + 1] = 2.0f / (t - b);
+    out[2 * 4 + 2] = -2.0f / (zf - zn);
+    out[3 * 4 + 0] = -(r + l) / (r - l);
+    out[3 * 4 + 1] = -(t + b) / (t - b);
+    out[3 * 4 + 2] = -(zf + zn) / (zf - zn);
+}
+
+static void v3_normalize(const float v[3], float o[3])
+{
+    /* include/glm/core/func_geometric.inl:239-248, inversesqrt = 1/sqrt */
+    float sqr = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    float inv = 1.0f / sqrtf(sqr);
+    o[0] = v[0] * inv; o[1] = v[1] * inv; o[2] = v[2] * inv;
+}
+
+static void v3_cross(const float x[3], const float y[3], float o[3])
+{
+    /* include/glm/core/func_geometric.inl:199-211 */
+    float a = x[1] * y[2] - y[1] * x[2];
+    float b = x[2] * y[0] - y[2] * x[0];
+    float c = x[0] * y[1] - y[0] * x[1];
+    o[0] = a; o[1] = b; o[2] = c;
+}
+
+/* include/glm/gtc/matrix_transform.inl:383-411 and translate() :32-41 */
+void or_lookat(const float eye[3], const float center[3], const float up[3], float out[16])
+{
+    float fd[3] = {center[0] - eye[0], center[1] - eye[1], center[2] - eye[2]};
+    float f[3], u[3], s[3], uu[3];
+    v3_normalize(fd, f);
+    v3_normalize(up, u);
+    float fu[3];
+    v3_cross(f, u, fu);
+    v3_normalize(fu, s);
+    v3_cross(s, f, uu);
+    float m[16];
+    memset(m, 0, sizeof(m));
+    m[0] = m[5] = m[10] = m[15] = 1.0f;
+    m[0 * 4 + 0] = s[0];  m[1 * 4 + 0] = s[1];  m[2 * 4 + 0] = s[2];
+    m[0 * 4 + 1] = uu[0]; m[1 * 4 + 1] = uu[1]; m[2 * 4 + 1] = uu[2];
+    m[0 * 4 + 2] = -f[0]; m[1 * 4 + 2] = -f[1]; m[2 * 4 + 2] = -f[2];
+    /* translate(m, -eye): Result[3] = m[0]*v0 + m[1]*v1 + m[2]*v2 + m[3] */
+    float v0 = -eye[0], v1 = -eye[1], v2 = -eye[2];
+    for (int r = 0; r < 4; ++r) {
+        float acc = m[0 * 4 + r] * v0;
+        acc = acc + m[1 * 4 + r] * v1;
+        acc = acc + m[2 * 4 + r] * v2;
+        acc = acc + m[3 * 4 + r];
+        out[3 * 4 + r] = acc;
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 4; ++r) out[c * 4 + r] = m[c * 4 + r];
+}
+
+/* include/glm/core/type_mat4x4.inl:757-777: Result[j] = A0*B[j][0] + A1*B[j][1]
+ * + A2*B[j][2] + A3*B[j][3], summed left to right */
+void or_mat4_mul(const float a[16], const float b[16], float out[16])
+{
+    float r[16];
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i) {
+            float acc = a[0 * 4 + i] * b[j * 4 + 0];
+            acc = acc + a[1 * 4 + i] * b[j * 4 + 1];
+            acc = acc + a[2 * 4 + i] * b[j * 4 + 2];
+            acc = acc + a[3 * 4 + i] * b[j * 4 + 3];
+            r[j * 4 + i] = acc;
+        }
+    memcpy(out, r, sizeof(r));
+}
+
+/* ShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:139-145 */
+void or_bias_matrix(float zbias, float out[16])
+{
+    memset(out, 0, 16 * sizeof(float));
+    out[0 * 4 + 0] = 0.5f;
+    out[1 * 4 + 1] = 0.5f;
+    out[2 * 4 + 2] = 0.5f;
+    out[3 * 4 + 0] = 0.5f;
+    out[3 * 4 + 1] = 0.5f;
+    out[3 * 4 + 2] = 0.5f - zbias;
+    out[3 * 4 + 3] = 1.0f;
+}
+
+/* SoftShadowMapping/src/Scene/LightSource/UniformSampledLightSource.cpp:27-37.
+ * `size` is an int, as LightSource::size is (LightSource.h:26). */
+void or_uniform_sampling(const float base[3], int size, int nsamples, int index, float out[3])
+{
+    float halfSize = (float)((double)size / 2.0);
+    float factor = sqrtf((float)nsamples);
+    float sampleSize = (float)((double)(factor - 1.0f) / 2.0);
+    out[0] = base[0]; out[1] = base[1]; out[2] = base[2];
+    out[0] += (((float)(index % (int)factor) - sampleSize) / sampleSize) * halfSize;
+    out[1] += (((float)(int)((float)index / factor) - sampleSize) / sampleSize) * halfSize;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Texture sampling (a14)                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* GL_NEAREST with CLAMP_TO_BORDER, border 0: texel floor(s*W), floor(t*H);
+ * outside [0,W)x[0,H) returns the border depth 0.
+ * ShadowMapping/src/Viewers/MyGLTextureViewer.cpp:3-28 */
+float or_fetch(const float* map, int w, int h, float s, float t)
+{
+    float u = s * (float)w;
+    float v = t * (float)h;
+    if (!(u >= 0.0f && u < (float)w && v >= 0.0f && v < (float)h)) return 0.0f;
+    int i = (int)floorf(u);
+    int j = (int)floorf(v);
+    return map[(size_t)j * (size_t)w + (size_t)i];
+}
+
+/* ------------------------------------------------------------------------ */
+/* mat * vec in GLSL column order, summed left to right                       */
+/* ------------------------------------------------------------------------ */
+static void m4v(const float m[16], float x, float y, float z, float w, float o[4])
+{
+    for (int i = 0; i < 4; ++i) {
+        float acc = m[0 * 4 + i] * x;
+        acc = acc + m[1 * 4 + i] * y;
+        acc = acc + m[2 * 4 + i] * z;
+        acc = acc + m[3 * 4 + i] * w;
+        o[i] = acc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Hard shadows: Shadow.frag (a8-a10)                                         */
+/* ------------------------------------------------------------------------ */
+
+/* ShadowMapping/Shaders/Shadow.frag:222-238.  The normal arrives pre-flipped
+ * (negating before normalMatrix*n is bit-identical to negating after). */
+static float pre_evaluation(const or_params* p, float vx, float vy, float vz,
+                            float nx, float ny, float nz)
+{
+    float ve[4];
+    m4v(p->mv, vx, vy, vz, 1.0f, ve);
+    const float* nm = p->normal_matrix;
+    float n3[3];
+    for (int i = 0; i < 3; ++i) {
+        float acc = nm[0 * 3 + i] * nx;
+        acc = acc + nm[1 * 3 + i] * ny;
+        acc = acc + nm[2 * 3 + i] * nz;
+        n3[i] = acc;
+    }
+    float N[3], L[3];
+    v3_normalize(n3, N);
+    float ld[3] = {p->light_position[0] - ve[0], p->light_position[1] - ve[1],
+                   p->light_position[2] - ve[2]};
+    v3_normalize(ld, L);
+    float d = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
+    return (gmax(d, 0.0f) == 0.0f) ? p->shadow_intensity : 1.0f;
+}
+
+/* ShadowMapping/Shaders/Shadow.frag:86-116 (bilinearPCF branch) */
+static float pcf(const float* D, int W, int H, const or_params* p, float s, float t, float z)
+{
+    float incrWidth = 1.0f / (float)W;
+    float incrHeight = 1.0f / (float)H;
+    float illuminationCount = 0.0f;
+    float offset = (float)p->penumbra_size;
+    float stepSize = 2.0f * offset / (float)p->kernel_order;
+    int count = 0;
+    for (float w = -offset; w < offset; w += stepSize) {
+        for (float h = -offset; h < offset; h += stepSize) {
+            float d = or_fetch(D, W, H, s + w * incrWidth, t + h * incrHeight);
+            if (z <= d) illuminationCount = illuminationCount + 1.0f;
+            else illuminationCount = illuminationCount + p->shadow_intensity;
+            count++;
+        }
+    }
+    return illuminationCount / (float)count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* RBSM: ShadowMapping/Shaders/RBSM/NonConservativeSMSR.frag and             */
+/* FilteredRBSM.frag                                                          */
+/* ------------------------------------------------------------------------ */
+typedef struct rb_ctx {
+    const float* D;
+    int W, H;
+    float stepx, stepy;   /* shadowMapStep = (1/W, 1/H), MyGLGeometryViewer.cpp:238 */
+    float thr;            /* depthThreshold */
+    int maxSearch;
+    float si;
+    int filtered;         /* 1: FilteredRBSM.frag variants */
+} rb_ctx;
+
+static inline float rb_tex(const rb_ctx* c, float s, float t) { return or_fetch(c->D, c->W, c->H, s, t); }
+
+/* NonConservativeSMSR.frag:23-54 with breakForUmbra = breakForNoUmbra = false */
+static void get_disc4(const rb_ctx* c, float x, float y, float z, float dir[4])
+{
+    x = x - c->stepx;
+    dir[0] = cmp_le(z, rb_tex(c, x, y));
+    x = x + 2.0f * c->stepx;
+    dir[1] = cmp_le(z, rb_tex(c, x, y));
+    x = x - c->stepx;
+    y = y + c->stepy;
+    dir[2] = cmp_le(z, rb_tex(c, x, y));
+    y = y - 2.0f * c->stepy;
+    dir[3] = cmp_le(z, rb_tex(c, x, y));
+}
+
+/* NonConservativeSMSR.frag:56-94 */
+static int get_disc_f(const rb_ctx* c, float x, float y, float z, float dirx, float diry, float discType)
+{
+    if (dirx == 0.0f) {
+        x = x - c->stepx;
+        float left = cmp_le(z, rb_tex(c, x, y));
+        if (fabsf(left - discType) == 0.0f) return 1;
+        x = x + 2.0f * c->stepx;
+        float right = cmp_le(z, rb_tex(c, x, y));
+        if (fabsf(right - discType) == 0.0f) return 1;
+        x = x - c->stepx;
+    }
+    if (diry == 0.0f) {
+        y = y + c->stepy;
+        float bottom = cmp_le(z, rb_tex(c, x, y));
+        if (fabsf(bottom - discType) == 0.0f) return 1;
+        y = y - 2.0f * c->stepy;
+        float top = cmp_le(z, rb_tex(c, x, y));
+        if (fabsf(top - discType) == 0.0f) return 1;
+    }
+    return 0;
+}
+
+/* NonConservativeSMSR.frag:96-178; `newDepth` (a shader global, :21) is an
+ * out-parameter here */
+static int get_disc_v(const rb_ctx* c, float x, float y, float z, float dirx, float diry,
+                      float dr, float dg, float db, float* newDepth)
+{
+    float rx = x, ry = y;
+    *newDepth = z;
+    if (dirx == 0.0f) {
+        if (dr == 0.5f || dr == 0.75f) {
+            rx = x - c->stepx;
+            float d = rb_tex(c, rx, ry);
+            if (db == 1.0f) {
+                if (fabsf(z - d) < c->thr) { z = z - c->thr; *newDepth = z; }
+            }
+            float left = cmp_le(z, d);
+            if (fabsf(left - db) == 0.0f) return 1;
+        }
+        if (dr == 0.75f || dr == 0.25f) {
+            rx = x + c->stepx;
+            float d = rb_tex(c, rx, ry);
+            if (db == 1.0f) {
+                if (fabsf(z - d) < c->thr) { z = z - c->thr; *newDepth = z; }
+            }
+            float right = cmp_le(z, d);
+            if (fabsf(right - db) == 0.0f) return 1;
+        }
+    }
+    if (diry == 0.0f) {
+        if (dg == 0.5f || dg == 0.75f) {
+            ry = y + c->stepy;
+            float d = rb_tex(c, rx, ry);
+            if (db == 1.0f) {
+                if (fabsf(z - d) < c->thr) { z = z - c->thr; *newDepth = z; }
+            }
+            float bottom = cmp_le(z, d);
+            if (fabsf(bottom - db) == 0.0f) return 1;
+        }
+        if (dg == 0.75f || dg == 0.25f) {
+            ry = y - c->stepy;
+            float d = rb_tex(c, rx, ry);
+            if (db == 1.0f) {
+                if (fabsf(z - d) < c->thr) { z = z - c->thr; *newDepth = z; }
+            }
+            float top = cmp_le(z, d);
+            if (fabsf(top - db) == 0.0f) return 1;
+        }
+    }
+    return 0;
+}
+
+/* NonConservativeSMSR.frag:180-232 */
+static float disc_length(const rb_ctx* c, float dr, float dg, float db, float x, float y, float z,
+                         float dirx, float diry, float subCoord)
+{
+    float cx = x, cy = y, cz = z;
+    float foundEdgeEnd = 0.0f;
+    float dist = 0.0f;
+    float stx = dirx * c->stepx, sty = diry * c->stepy;
+    float newDepth = 0.0f;
+    cx = cx + stx;
+    cy = cy + sty;
+    for (int it = 0; it < c->maxSearch; it++) {
+        float d = rb_tex(c, cx, cy);
+        if (db == 0.0f)
+            if (fabsf(cz - d) < c->thr) cz = cz - c->thr;
+        float center = cmp_le(cz, d);
+        if (fabsf(center - db) == 0.0f) {
+            int hasDisc = get_disc_f(c, cx, cy, cz, 0.0f, 0.0f, db);
+            foundEdgeEnd = hasDisc ? 1.0f : 0.0f;
+            break;
+        } else {
+            int hasDisc = get_disc_v(c, cx, cy, cz, dirx, diry, dr, dg, db, &newDepth);
+            if (!hasDisc) break;
+        }
+        dist = dist + 1.0f;
+        cx = cx + stx;
+        cy = cy + sty;
+        if (db == 1.0f) cz = newDepth;
+    }
+    float a = -(dist + (1.0f - subCoord));
+    float b = dist + (1.0f - subCoord);
+    return gmix(a, b, foundEdgeEnd);
+}
+
+/* NonConservativeSMSR.frag:234-243 (binary) / FilteredRBSM.frag:234-243 */
+static float rel_position(const rb_ctx* c, float dx, float dy, float shadow)
+{
+    float T = 1.0f;
+    if (dx < 0.0f && dy < 0.0f) T = 0.0f;
+    if (dx > 0.0f && dy > 0.0f) T = -2.0f;
+    float edgeLength = gmin(fabsf(dx) + fabsf(dy), (float)c->maxSearch);
+    if (c->filtered)
+        return (T * fabsf(gmax(T * dx, T * dy))) / edgeLength;
+    return (gmax(T, 2.0f * shadow - 1.0f) * fabsf(gmax(T * dx, T * dy))) / edgeLength;
+}
+
+/* NonConservativeSMSR.frag:266-274 / FilteredRBSM.frag:266-272 */
+static float revectorize(const rb_ctx* c, float rx, float ry, float shadow)
+{
+    if (c->filtered) {
+        if (rx * ry < 0.0f) return (1.0f - shadow) + (2.0f * shadow - 1.0f) * gmax(rx, ry);
+        else if (rx * ry == 0.0f) return shadow;
+        else return gclamp((1.0f - shadow) + (2.0f * shadow - 1.0f) * (rx + ry), 0.0f, 1.0f);
+    }
+    if ((rx * ry == 2.0f * shadow) ||
+        ((fabsf(rx) * fabsf(ry) > 0.0f) &&
+         ((1.0f - shadow) + (2.0f * shadow - 1.0f) * (fabsf(rx) + fabsf(ry)) < 0.5f)))
+        return 0.0f;
+    return 1.0f;
+}
+
+/* NonConservativeSMSR.frag:276-286 */
+static void compute_discontinuity(const rb_ctx* c, float x, float y, float z, float dC,
+                                  float* r, float* g, float* b)
+{
+    float center = cmp_le(z, dC);
+    float discType = 1.0f - center;
+    float dir[4];
+    get_disc4(c, x, y, z, dir);
+    float disc[4];
+    for (int k = 0; k < 4; ++k) disc[k] = fabsf(dir[k] - center);
+    *r = (2.0f * disc[0] + disc[1]) / 4.0f;
+    *g = (2.0f * disc[2] + disc[3]) / 4.0f;
+    *b = discType;
+}
+
+/* NonConservativeSMSR.frag:245-264 + revectorizeShadow */
+static float revectorized_value(const rb_ctx* c, float dr, float dg, float db,
+                                float x, float y, float z, float subx, float suby, float shadow)
+{
+    float left = disc_length(c, dr, dg, db, x, y, z, -1.0f, 0.0f, 1.0f - subx);
+    float right = disc_length(c, dr, dg, db, x, y, z, 1.0f, 0.0f, subx);
+    float down = disc_length(c, dr, dg, db, x, y, z, 0.0f, -1.0f, 1.0f - suby);
+    float up = disc_length(c, dr, dg, db, x, y, z, 0.0f, 1.0f, suby);
+    float rx = rel_position(c, left, right, shadow);
+    float ry = rel_position(c, down, up, shadow);
+    return revectorize(c, rx, ry, shadow);
+}
+
+/* NonConservativeSMSR.frag:288-304 */
+static float shadow_from_smsr(const rb_ctx* c, float x, float y, float z)
+{
+    float dC = rb_tex(c, x, y);
+    float dr, dg, db;
+    compute_discontinuity(c, x, y, z, dC, &dr, &dg, &db);
+    float subx = gfract(x * (float)c->W);
+    float suby = gfract(y * (float)c->H);
+    float shadow = cmp_le(z, dC);
+    if (dr > 0.0f || dg > 0.0f) {
+        if (dr == 0.75f && dg == 0.75f) return gmix(1.0f - shadow, 1.0f, c->si);
+        float v = revectorized_value(c, dr, dg, db, x, y, z, subx, suby, shadow);
+        return gmix(v, 1.0f, c->si);
+    }
+    return gmix(shadow, 1.0f, c->si);
+}
+
+/* NonConservativeSMSR.frag:306-349 / FilteredRBSM.frag:306-349 */
+static float shadow_from_rpcf(const rb_ctx* c, int kernelOrder, int penumbraSize,
+                              float x, float y, float z)
+{
+    float incrWidth = 1.0f / (float)c->W;
+    float incrHeight = 1.0f / (float)c->H;
+    float illuminationCount = 0.0f;
+    float offset = (float)penumbraSize;
+    float stepSize = 2.0f * offset / (float)kernelOrder;
+    int count = 0;
+    for (float w = -offset; w <= offset; w += stepSize) {
+        for (float h = -offset; h <= offset; h += stepSize) {
+            float sx = x + w * incrWidth;
+            float sy = y + h * incrHeight;
+            float d = rb_tex(c, sx, sy);
+            float shadow = cmp_le(z, d);
+            float dr, dg, db;
+            compute_discontinuity(c, sx, sy, z, d, &dr, &dg, &db);
+            if (dr > 0.0f || dg > 0.0f) {
+                float subx = gfract(sx * (float)c->W);
+                float suby = gfract(sy * (float)c->H);
+                float v = revectorized_value(c, dr, dg, db, sx, sy, z, subx, suby, shadow);
+                illuminationCount = illuminationCount + gmix(v, 1.0f, c->si);
+            } else {
+                illuminationCount = illuminationCount + gmix(shadow, 1.0f, c->si);
+            }
+            count++;
+        }
+    }
+    return illuminationCount / (float)count;
+}
+
+static float hard_pixel(const or_params* p, const float* D, float vx, float vy, float vz,
+                        float nx, float ny, float nz)
+{
+    int W = p->sm_width, H = p->sm_height;
+    float sc[4];
+    m4v(p->light_mvp, vx, vy, vz, 1.0f, sc);
+    float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+    float shadow = p->pre_evaluation ? pre_evaluation(p, vx, vy, vz, nx, ny, nz) : 1.0f;
+    if (p->technique == OR_NAIVE || p->technique == OR_PCF) {
+        /* Shadow.frag:240-273 */
+        if (sc[3] > 0.0f && shadow == 1.0f) {
+            if (p->technique == OR_NAIVE) {
+                float d = or_fetch(D, W, H, n0, n1);
+                shadow = (n2 <= d) ? 1.0f : p->shadow_intensity;
+            } else {
+                shadow = pcf(D, W, H, p, n0, n1, n2);
+            }
+        }
+        return shadow;
+    }
+    /* NonConservativeSMSR.frag:367-401 / FilteredRBSM.frag:367-402 */
+    rb_ctx c;
+    c.D = D; c.W = W; c.H = H;
+    c.stepx = (float)(1.0 / (double)W);
+    c.stepy = (float)(1.0 / (double)H);
+    c.thr = p->depth_threshold;
+    c.maxSearch = p->max_search;
+    c.si = p->shadow_intensity;
+    c.filtered = (p->technique == OR_RBSM_FILTERED);
+    if (shadow == 1.0f) {
+        if (p->technique == OR_RBSM) shadow = shadow_from_smsr(&c, n0, n1, n2);
+        else shadow = shadow_from_rpcf(&c, p->kernel_order, p->penumbra_size, n0, n1, n2);
+    }
+    return shadow;
+}
+
+int or_hard_shadow(const float* pos, const float* nrm, int w, int h, const float* depth,
+                   const or_params* p, float* out)
+{
+    if (!pos || !depth || !p || !out || w <= 0 || h <= 0) return -1;
+    if (p->pre_evaluation && !nrm) return -1;
+    size_t plane = (size_t)w * (size_t)h;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { out[k] = 0.0f; continue; }
+        float nx = 0, ny = 0, nz = 0;
+        if (nrm) { nx = nrm[k]; ny = nrm[plane + k]; nz = nrm[2 * plane + k]; }
+        out[k] = hard_pixel(p, depth, vx, vy, vz, nx, ny, nz);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Soft shadows: AccurateSoftShadow.frag:52-133 (a13)                         */
+/* ------------------------------------------------------------------------ */
+int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsamples,
+                   int smw, int smh, const float m[16], const float* trans,
+                   float si, int pcf_mode, int kernel_order, float penumbra,
+                   int inclusive, float* out)
+{
+    if (!pos || !maps || !trans || !out || w <= 0 || h <= 0 || nsamples <= 0) return -1;
+    size_t plane = (size_t)w * (size_t)h;
+    size_t mapsz = (size_t)smw * (size_t)smh;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { out[k] = 0.0f; continue; }
+        /* :70-73, v.w is ignored */
+        float cm[4];
+        for (int i = 0; i < 4; ++i) {
+            float acc = m[0 * 4 + i] * vx;
+            acc = acc + m[1 * 4 + i] * vy;
+            acc = acc + m[2 * 4 + i] * vz;
+            cm[i] = acc;
+        }
+        float accShadow = 0.0f, count = 0.0f;
+        for (int idx = 0; idx < nsamples; ++idx) {
+            const float* T = trans + 4 * idx;
+            float cx = cm[0] + T[0], cy = cm[1] + T[1], cz = cm[2] + T[2], cw = cm[3] + T[3];
+            float sx = cx / cw, sy = cy / cw, sz = cz / cw;
+            const float* D = maps + (size_t)idx * mapsz;
+            if (pcf_mode == 0) {
+                float d = or_fetch(D, smw, smh, sx, sy);
+                accShadow = accShadow + ((sz <= d) ? 1.0f : si);
+            } else {
+                float incrWidth = 1.0f / (float)smw;
+                float incrHeight = 1.0f / (float)smh;
+                float offset = penumbra;
+                float stepSize = 2.0f * offset / (float)kernel_order;
+                float illuminationCount = 0.0f;
+                int PCFCount = 0;
+                for (float ww = -offset; inclusive ? (ww <= offset) : (ww < offset); ww += stepSize) {
+                    for (float hh = -offset; inclusive ? (hh <= offset) : (hh < offset); hh += stepSize) {
+                        float d = or_fetch(D, smw, smh, sx + ww * incrWidth, sy + hh * incrHeight);
+                        if (sz <= d) illuminationCount = illuminationCount + 1.0f;
+                        else illuminationCount = illuminationCount + si;
+                        PCFCount++;
+                    }
+                }
+                accShadow = accShadow + illuminationCount / (float)PCFCount;
+            }
+            count = count + 1.0f;
+        }
+        out[k] = accShadow / count;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Rasterization (a2, a6)                                                     */
+/* ------------------------------------------------------------------------ */
+#define OR_GUARD 4.0f
+#define OR_MAXV 9
+
+typedef struct cvert { float c[4]; float a[6]; } cvert;
+
+static float plane_dist(const float c[4], int plane)
+{
+    switch (plane) {
+    case 0: return c[2] + c[3];            /* near   z >= -w */
+    case 1: return c[3] - c[2];            /* far    z <=  w */
+    case 2: return c[0] + OR_GUARD * c[3]; /* left   x >= -Gw */
+    case 3: return OR_GUARD * c[3] - c[0]; /* right  x <=  Gw */
+    case 4: return c[1] + OR_GUARD * c[3]; /* bottom */
+    default: return OR_GUARD * c[3] - c[1];/* top */
+    }
+}
+
+static void lerp_vert(const cvert* in, const cvert* out, float din, float dout, int nattr, cvert* r)
+{
+    float t = din / (din - dout);
+    for (int k = 0; k < 4; ++k) r->c[k] = in->c[k] + t * (out->c[k] - in->c[k]);
+    for (int k = 0; k < nattr; ++k) r->a[k] = in->a[k] + t * (out->a[k] - in->a[k]);
+}
+
+/* Returns the polygon vertex count (0 when rejected). */
+static int clip_triangle(const cvert v[3], int nattr, cvert poly[OR_MAXV])
+{
+    int inside_all = 1;
+    for (int p = 0; p < 6; ++p) {
+        int nin = 0;
+        for (int k = 0; k < 3; ++k) nin += plane_dist(v[k].c, p) >= 0.0f;
+        if (nin == 0) return 0;
+        if (nin != 3) inside_all = 0;
+    }
+    poly[0] = v[0]; poly[1] = v[1]; poly[2] = v[2];
+    if (inside_all) return 3;
+    int n = 3;
+    cvert tmp[OR_MAXV];
+    for (int p = 0; p < 6; ++p) {
+        int m = 0;
+        for (int k = 0; k < n; ++k) {
+            const cvert* prev = &poly[(k + n - 1) % n];
+            const cvert* cur = &poly[k];
+            float dp = plane_dist(prev->c, p), dc = plane_dist(cur->c, p);
+            if (dc >= 0.0f) {
+                if (dp < 0.0f) lerp_vert(cur, prev, dc, dp, nattr, &tmp[m++]);
+                tmp[m++] = *cur;
+            } else if (dp >= 0.0f) {
+                lerp_vert(prev, cur, dp, dc, nattr, &tmp[m++]);
+            }
+        }
+        n = m;
+        if (n < 3) return 0;
+        memcpy(poly, tmp, (size_t)n * sizeof(cvert));
+    }
+    return n;
+}
+
+typedef struct tri2d {
+    long long X[3], Y[3];
+    float z[3], w[3];
+    float a[3][6];
+    long long area;
+    int front;
+    int px0, px1, py0, py1;
+    int tl[3];
+    double z0, dz1, dz2, inv_area, offset;
+} tri2d;
+
+/* Builds the screen-space triangle of fan piece (0, k+1, k+2). 0 = empty. */
+static int setup_piece(const cvert* poly, int k, int nattr, int W, int H,
+                       float off_factor, float off_units, tri2d* t)
+{
+    const cvert* vs[3] = {&poly[0], &poly[k + 1], &poly[k + 2]};
+    float hw = (float)W * 0.5f, hh = (float)H * 0.5f;
+    for (int i = 0; i < 3; ++i) {
+        const float* c = vs[i]->c;
+        if (!(c[3] > 0.0f)) return 0;
+        float xn = c[0] / c[3], yn = c[1] / c[3], zn = c[2] / c[3];
+        float xw = xn * hw + hw;
+        float yw = yn * hh + hh;
+        float zw = zn * 0.5f + 0.5f;
+        t->X[i] = (long long)rintf(xw * 256.0f);
+        t->Y[i] = (long long)rintf(yw * 256.0f);
+        t->z[i] = zw;
+        t->w[i] = c[3];
+        for (int q = 0; q < nattr; ++q) t->a[i][q] = vs[i]->a[q];
+    }
+    long long area = (t->X[1] - t->X[0]) * (t->Y[2] - t->Y[0]) - (t->Y[1] - t->Y[0]) * (t->X[2] - t->X[0]);
+    if (area == 0) return 0;
+    t->front = area > 0;
+    if (area < 0) {
+        long long tx = t->X[1]; t->X[1] = t->X[2]; t->X[2] = tx;
+        long long ty = t->Y[1]; t->Y[1] = t->Y[2]; t->Y[2] = ty;
+        float tz = t->z[1]; t->z[1] = t->z[2]; t->z[2] = tz;
+        float tw = t->w[1]; t->w[1] = t->w[2]; t->w[2] = tw;
+        for (int q = 0; q < nattr; ++q) { float ta = t->a[1][q]; t->a[1][q] = t->a[2][q]; t->a[2][q] = ta; }
+        area = -area;
+    }
+    t->area = area;
+    long long minX = t->X[0], maxX = t->X[0], minY = t->Y[0], maxY = t->Y[0];
+    for (int i = 1; i < 3; ++i) {
+        if (t->X[i] < minX) minX = t->X[i];
+        if (t->X[i] > maxX) maxX = t->X[i];
+        if (t->Y[i] < minY) minY = t->Y[i];
+        if (t->Y[i] > maxY) maxY = t->Y[i];
+    }
+    long long px0 = (minX - 128 + 255) >> 8, px1 = (maxX - 128) >> 8;
+    long long py0 = (minY - 128 + 255) >> 8, py1 = (maxY - 128) >> 8;
+    if (px0 < 0) px0 = 0;
+    if (py0 < 0) py0 = 0;
+    if (px1 > W - 1) px1 = W - 1;
+    if (py1 > H - 1) py1 = H - 1;
+    if (px0 > px1 || py0 > py1) return 0;
+    t->px0 = (int)px0; t->px1 = (int)px1; t->py0 = (int)py0; t->py1 = (int)py1;
+    /* edge i is opposite vertex i: (v1->v2), (v2->v0), (v0->v1) */
+    for (int i = 0; i < 3; ++i) {
+        int a = (i + 1) % 3, b = (i + 2) % 3;
+        long long dy = t->Y[b] - t->Y[a], dx = t->X[b] - t->X[a];
+        t->tl[i] = (dy < 0 || (dy == 0 && dx < 0)) ? 1 : 0;
+    }
+    t->z0 = (double)t->z[0];
+    t->dz1 = (double)t->z[1] - (double)t->z[0];
+    t->dz2 = (double)t->z[2] - (double)t->z[0];
+    t->inv_area = 1.0 / (double)area;
+    t->offset = 0.0;
+    if (off_factor != 0.0f || off_units != 0.0f) {
+        double dzdx = ((double)(t->Y[2] - t->Y[0]) * t->dz1 + (double)(t->Y[0] - t->Y[1]) * t->dz2) * t->inv_area * 256.0;
+        double dzdy = ((double)(t->X[0] - t->X[2]) * t->dz1 + (double)(t->X[1] - t->X[0]) * t->dz2) * t->inv_area * 256.0;
+        double m = fmax(fabs(dzdx), fabs(dzdy));
+        float zmax = fmaxf(fmaxf(fabsf(t->z[0]), fabsf(t->z[1])), fabsf(t->z[2]));
+        double r = 0.0;
+        if (zmax > 0.0f) { int e; frexpf(zmax, &e); r = ldexp(1.0, (e - 1) - 23); }
+        t->offset = (double)off_factor * m + (double)off_units * r;
+    }
+    return 1;
+}
+
+/* Coverage + window depth at pixel (i, j). Returns 0 when not covered. */
+static int eval_pixel(const tri2d* t, int i, int j, long long e[3], float* zout)
+{
+    long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+    for (int k = 0; k < 3; ++k) {
+        int a = (k + 1) % 3, b = (k + 2) % 3;
+        e[k] = (t->X[b] - t->X[a]) * (Py - t->Y[a]) - (t->Y[b] - t->Y[a]) * (Px - t->X[a]);
+        if (e[k] + t->tl[k] <= 0) return 0;
+    }
+    double zd = t->z0 + ((double)e[1] * t->dz1 + (double)e[2] * t->dz2) * t->inv_area;
+    zd = zd + t->offset;
+    float z = (float)zd;
+    if (!(z > 0.0f)) z = 0.0f;
+    if (z > 1.0f) z = 1.0f;
+    *zout = z;
+    return 1;
+}
+
+static void load_tri(const float* pos, const float* nrm, const int* idx, int tri,
+                     const float mvp[16], cvert v[3])
+{
+    for (int k = 0; k < 3; ++k) {
+        int vi = idx[3 * tri + k];
+        const float* p = pos + 3 * (size_t)vi;
+        m4v(mvp, p[0], p[1], p[2], 1.0f, v[k].c);
+        v[k].a[0] = p[0]; v[k].a[1] = p[1]; v[k].a[2] = p[2];
+        if (nrm) {
+            const float* q = nrm + 3 * (size_t)vi;
+            v[k].a[3] = q[0]; v[k].a[4] = q[1]; v[k].a[5] = q[2];
+        } else {
+            v[k].a[3] = v[k].a[4] = v[k].a[5] = 0.0f;
+        }
+    }
+}
+
+static int check_mesh(const float* pos, int nv, const int* idx, int nt)
+{
+    if (!pos || !idx || nv <= 0 || nt < 0) return -1;
+    for (long long k = 0; k < 3LL * nt; ++k)
+        if (idx[k] < 0 || idx[k] >= nv) return -1;
+    return 0;
+}
+
+static inline unsigned int fbits(float f) { unsigned int u; memcpy(&u, &f, 4); return u; }
+
+int or_depth_raster(const float* pos, int nv, const int* idx, int nt, const float mvp[16],
+                    int w, int h, float off_factor, float off_units, float* depth_out)
+{
+    if (check_mesh(pos, nv, idx, nt) || !depth_out || w <= 0 || h <= 0) return -1;
+    size_t npx = (size_t)w * (size_t)h;
+    for (size_t k = 0; k < npx; ++k) depth_out[k] = 1.0f;
+    int nbands = or_get_threads() * 4;
+    if (nbands > h) nbands = h;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) OMP_THREADS
+#endif
+    for (int band = 0; band < nbands; ++band) {
+        int y0 = (int)((long long)h * band / nbands), y1 = (int)((long long)h * (band + 1) / nbands) - 1;
+        for (int tri = 0; tri < nt; ++tri) {
+            cvert v[3], poly[OR_MAXV];
+            load_tri(pos, NULL, idx, tri, mvp, v);
+            int n = clip_triangle(v, 0, poly);
+            for (int k = 0; k + 2 < n; ++k) {
+                tri2d t;
+                if (!setup_piece(poly, k, 0, w, h, off_factor, off_units, &t)) continue;
+                int ya = t.py0 > y0 ? t.py0 : y0, yb = t.py1 < y1 ? t.py1 : y1;
+                for (int j = ya; j <= yb; ++j)
+                    for (int i = t.px0; i <= t.px1; ++i) {
+                        long long e[3];
+                        float z;
+                        if (!eval_pixel(&t, i, j, e, &z)) continue;
+                        float* dst = &depth_out[(size_t)j * w + i];
+                        if (z < *dst) *dst = z;
+                    }
+            }
+        }
+    }
+    return 0;
+}
+
+int or_depth_raster_brute(const float* pos, int nv, const int* idx, int nt, const float mvp[16],
+                          int w, int h, float off_factor, float off_units, float* depth_out)
+{
+    if (check_mesh(pos, nv, idx, nt) || !depth_out || w <= 0 || h <= 0) return -1;
+    for (int j = 0; j < h; ++j)
+        for (int i = 0; i < w; ++i) {
+            float best = 1.0f;
+            for (int tri = 0; tri < nt; ++tri) {
+                cvert v[3], poly[OR_MAXV];
+                load_tri(pos, NULL, idx, tri, mvp, v);
+                int n = clip_triangle(v, 0, poly);
+                for (int k = 0; k + 2 < n; ++k) {
+                    tri2d t;
+                    if (!setup_piece(poly, k, 0, w, h, off_factor, off_units, &t)) continue;
+                    if (i < t.px0 || i > t.px1 || j < t.py0 || j > t.py1) continue;
+                    long long e[3];
+                    float z;
+                    if (eval_pixel(&t, i, j, e, &z) && z < best) best = z;
+                }
+            }
+            depth_out[(size_t)j * w + i] = best;
+        }
+    return 0;
+}
+
+int or_visibility_raster(const float* pos, int nv, const int* idx, int nt, const float mvp[16],
+                         int w, int h, uint64_t* vis)
+{
+    if (check_mesh(pos, nv, idx, nt) || !vis || w <= 0 || h <= 0) return -1;
+    if (nt >= (1 << 29)) return -1;
+    size_t npx = (size_t)w * (size_t)h;
+    for (size_t k = 0; k < npx; ++k) vis[k] = ~(uint64_t)0;
+    int nbands = or_get_threads() * 4;
+    if (nbands > h) nbands = h;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) OMP_THREADS
+#endif
+    for (int band = 0; band < nbands; ++band) {
+        int y0 = (int)((long long)h * band / nbands), y1 = (int)((long long)h * (band + 1) / nbands) - 1;
+        for (int tri = 0; tri < nt; ++tri) {
+            cvert v[3], poly[OR_MAXV];
+            load_tri(pos, NULL, idx, tri, mvp, v);
+            int n = clip_triangle(v, 0, poly);
+            for (int k = 0; k + 2 < n; ++k) {
+                tri2d t;
+                if (!setup_piece(poly, k, 0, w, h, 0.0f, 0.0f, &t)) continue;
+                uint64_t id = (uint64_t)tri * 8u + (uint64_t)k;
+                int ya = t.py0 > y0 ? t.py0 : y0, yb = t.py1 < y1 ? t.py1 : y1;
+                for (int j = ya; j <= yb; ++j)
+                    for (int i = t.px0; i <= t.px1; ++i) {
+                        long long e[3];
+                        float z;
+                        if (!eval_pixel(&t, i, j, e, &z)) continue;
+                        uint64_t key = ((uint64_t)fbits(z) << 32) | id;
+                        uint64_t* dst = &vis[(size_t)j * w + i];
+                        if (key < *dst) *dst = key;
+                    }
+            }
+        }
+    }
+    return 0;
+}
+
+/* GBuffer.vert:15-21 / GBuffer.frag:35-37: perspective-correct interpolation
+ * of the object-space position and normal (FP64), gl_FrontFacing from the
+ * window-space winding (CCW = front). */
+int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
+               const float mvp[16], int w, int h, float* pos_out, float* nrm_out)
+{
+    if (!nrm || !pos_out || !nrm_out) return -1;
+    size_t npx = (size_t)w * (size_t)h;
+    uint64_t* vis = (uint64_t*)malloc(npx * sizeof(uint64_t));
+    if (!vis) return -1;
+    int rc = or_visibility_raster(pos, nv, idx, nt, mvp, w, h, vis);
+    if (rc) { free(vis); return rc; }
+    long long n = (long long)npx;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1024) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        uint64_t key = vis[k];
+        float out[6] = {0, 0, 0, 0, 0, 0};
+        if (key != ~(uint64_t)0) {
+            unsigned id = (unsigned)(key & 0xffffffffu);
+            int tri = (int)(id >> 3), piece = (int)(id & 7u);
+            cvert v[3], poly[OR_MAXV];
+            load_tri(pos, nrm, idx, tri, mvp, v);
+            int nvp = clip_triangle(v, 6, poly);
+            tri2d t;
+            if (piece + 2 < nvp && setup_piece(poly, piece, 6, w, h, 0.0f, 0.0f, &t)) {
+                int i = (int)(k % w), j = (int)(k / w);
+                long long e[3];
+                float z;
+                if (eval_pixel(&t, i, j, e, &z)) {
+                    double b0 = (double)e[0] / (double)t.w[0];
+                    double b1 = (double)e[1] / (double)t.w[1];
+                    double b2 = (double)e[2] / (double)t.w[2];
+                    double den = (b0 + b1) + b2;
+                    for (int q = 0; q < 6; ++q) {
+                        double num = (b0 * (double)t.a[0][q] + b1 * (double)t.a[1][q]) + b2 * (double)t.a[2][q];
+                        out[q] = (float)(num / den);
+                    }
+                    if (!t.front) { out[3] = -out[3]; out[4] = -out[4]; out[5] = -out[5]; }
+                }
+            }
+        }
+        for (int q = 0; q < 3; ++q) { pos_out[q * npx + k] = out[q]; nrm_out[q * npx + k] = out[3 + q]; }
+    }
+    free(vis);
+    return 0;
+}

@@ -1,0 +1,98 @@
+/*
+ * gi_oracle.h — TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C, FP32, no FMA contraction) of the reference's shadow
+ * pipeline, used as the parity checker for the HIP kernels and as the "port"
+ * CPU baseline.  Nothing in the product (globalillumination_amd/) links,
+ * imports or calls this library; only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py do.
+ *
+ * Every function cites the reference file:line it restates.  Paths are relative
+ * to the reference repository root (MarcioCerqueira/GlobalIllumination).
+ */
+#ifndef GI_ORACLE_H
+#define GI_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- glm 0.9.3.1 math, FP32 (a1, a4) -------------------------------------- */
+/* include/glm/gtc/matrix_transform.inl:223-244 (fovy in degrees) */
+void or_perspective(float fovy_deg, float aspect, float znear, float zfar, float out[16]);
+/* include/glm/gtc/matrix_transform.inl:200-220 (ortho with near/far) */
+void or_ortho(float l, float r, float b, float t, float znear, float zfar, float out[16]);
+/* include/glm/gtc/matrix_transform.inl:383-411 */
+void or_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);
+/* include/glm/core/type_mat4x4.inl:757-777 */
+void or_mat4_mul(const float a[16], const float b[16], float out[16]);
+/* ShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:139-145, with the constant
+ * depth bias folded into B[3][2] = 0.5 - zbias */
+void or_bias_matrix(float zbias, float out[16]);
+/* SoftShadowMapping/src/Scene/LightSource/UniformSampledLightSource.cpp:27-37 */
+void or_uniform_sampling(const float base[3], int size, int nsamples, int index, float out[3]);
+
+/* ---- texture sampling (a14) ------------------------------------------------ */
+/* NEAREST + CLAMP_TO_BORDER(0): ShadowMapping/src/Viewers/MyGLTextureViewer.cpp:3-28 */
+float or_fetch(const float* map, int w, int h, float s, float t);
+
+/* ---- per-technique screen passes (a8-a13) ---------------------------------- */
+enum { OR_NAIVE = 0, OR_PCF = 1, OR_RBSM = 2, OR_RPCF = 3, OR_RBSM_FILTERED = 4 };
+
+typedef struct or_params {
+    float light_mvp[16];       /* B * lightMVP, column-major */
+    int   sm_width, sm_height;
+    int   technique;
+    int   kernel_order;
+    int   penumbra_size;
+    int   max_search;
+    float depth_threshold;
+    float shadow_intensity;
+    int   pre_evaluation;
+    float mv[16];
+    float normal_matrix[9];
+    float light_position[3];
+} or_params;
+
+/* pos/nrm: 3 planes of w*h floats each (normal pre-flipped by facing);
+ * out: w*h floats.  Background (x == 0) writes 0. */
+int or_hard_shadow(const float* pos, const float* nrm, int w, int h,
+                   const float* depth, const or_params* p, float* out);
+
+/* AccurateSoftShadow.frag:52-133.  pcf_mode 0: single tap; 1: PCF loop with
+ * (kernel_order, penumbra, inclusive '<=' when inclusive != 0). */
+int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsamples,
+                   int smw, int smh, const float light_mvp[16], const float* trans,
+                   float si, int pcf_mode, int kernel_order, float penumbra,
+                   int inclusive, float* out);
+
+/* ---- rasterization (a2, a6) ------------------------------------------------ */
+/* Depth of all triangles into a w*h FP32 map cleared to 1.0, depth test LESS,
+ * optional polygon offset (ShadowMapping/src/main.cpp:246-247).  Iterates each
+ * triangle's bounding box. */
+int or_depth_raster(const float* pos, int nv, const int* idx, int nt,
+                    const float mvp[16], int w, int h, float off_factor,
+                    float off_units, float* depth_out);
+/* Same result computed per texel over every triangle (tiny inputs only). */
+int or_depth_raster_brute(const float* pos, int nv, const int* idx, int nt,
+                          const float mvp[16], int w, int h, float off_factor,
+                          float off_units, float* depth_out);
+/* Visibility keys ((depth bits << 32) | tri*8+sub), cleared to ~0. */
+int or_visibility_raster(const float* pos, int nv, const int* idx, int nt,
+                         const float mvp[16], int w, int h, uint64_t* vis_out);
+/* Camera G-buffer: ShadowMapping/Shaders/GBuffer/GBuffer.vert:15-21,
+ * GBuffer.frag:35-37.  pos_out / nrm_out: 3 planes each; normal pre-flipped
+ * when back-facing; background = 0. */
+int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
+               const float mvp[16], int w, int h, float* pos_out, float* nrm_out);
+
+/* threads used by the OpenMP loops (0 = runtime default) */
+void or_set_threads(int n);
+int  or_get_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,206 @@
+"""Parity of the HIP passes (through the C ABI) against the CPU oracle.
+
+Binary visibility, depth maps and G-buffers must be bit-identical; PCF, RBSM
+and soft-shadow values must agree within 1e-5 absolute (they are bit-identical
+as well, which is asserted first and reported on failure).
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import bits_equal, camera_mvp, light_mvps, mismatch_report, shrink_soup
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # absolute tolerance for PCF / RBSM / soft-shadow values
+
+
+def _np(t: torch.Tensor) -> np.ndarray:
+    return t.detach().cpu().numpy()
+
+
+def _depth_both(gi, oracle_lib, mesh, mvps, w, h, offset=(0.0, 0.0)):
+    dm = gi.DeviceMesh.from_mesh(mesh)
+    got = _np(gi.render_shadow_map(dm, mvps, w, h, polygon_offset=offset))
+    want = np.stack([oracle_lib.depth_raster(mesh.positions, mesh.indices, m, w, h, offset) for m in mvps])
+    return got, want
+
+
+def test_depth_pass_config1(gi, oracle_lib, cuda):
+    wl = gi.scenes.config1()
+    mvps = light_mvps(wl)
+    got, want = _depth_both(gi, oracle_lib, wl.mesh, mvps, 1024, 1024)
+    assert bits_equal(got, want), mismatch_report(got, want, "cfg1 depth")
+    assert (want < 1.0).mean() > 0.5
+
+
+def test_depth_pass_polygon_offset(gi, oracle_lib, cuda):
+    wl = gi.scenes.config1()
+    mvps = light_mvps(wl)
+    got, want = _depth_both(gi, oracle_lib, wl.mesh, mvps, 1024, 1024, offset=(4.0, 20.0))
+    assert bits_equal(got, want), mismatch_report(got, want, "cfg1 depth, offset (4,20)")
+
+
+def test_depth_pass_soup_ortho(gi, oracle_lib, cuda):
+    mesh = shrink_soup(20_000)
+    wl = gi.scenes.config2()
+    light = gi.scenes.ortho_fit((0.3, -1.0, 0.2), mesh, 512, 512)
+    mvps = np.stack([gi.passes.view_matrices(light)[2]])
+    got, want = _depth_both(gi, oracle_lib, mesh, mvps, 512, 512)
+    assert bits_equal(got, want), mismatch_report(got, want, "soup depth")
+    del wl
+
+
+def test_depth_pass_batch_of_area_light_samples(gi, oracle_lib, cuda):
+    wl = gi.scenes.config3(nsamples=4, sm=1024, freq=40)
+    mvps = light_mvps(wl)
+    got, want = _depth_both(gi, oracle_lib, wl.mesh, mvps, 1024, 1024)
+    assert bits_equal(got, want), mismatch_report(got, want, "area-light depth batch")
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (33, 17), (100, 257)])
+def test_depth_pass_ragged_sizes(gi, oracle_lib, cuda, w, h):
+    wl = gi.scenes.config1()
+    light = gi.scenes.View((4.0, 8.0, 4.0), (0.0, 0.0, 0.0), w, h)
+    mvps = np.stack([gi.passes.view_matrices(light)[2]])
+    got, want = _depth_both(gi, oracle_lib, wl.mesh, mvps, w, h)
+    assert bits_equal(got, want), mismatch_report(got, want, f"{w}x{h} depth")
+
+
+def test_depth_pass_empty_mesh(gi, cuda):
+    mesh = gi.scenes.ground_quad(1.0, 1.0, 0.0)
+    dm = gi.DeviceMesh.from_mesh(mesh)
+    dm.indices = dm.indices[:0].contiguous()
+    mvps = np.stack([gi.passes.view_matrices(gi.scenes.View((0, 5, 0.1), (0, 0, 0), 64, 64))[2]])
+    got = _np(gi.render_shadow_map(dm, mvps, 64, 64))
+    assert (got == 1.0).all()
+
+
+def test_depth_pass_overflow_fallback(gi, oracle_lib, cuda):
+    # 64 large overlapping triangles: every one touches every tile, so the tile
+    # lists exceed the workspace bound and the raster scans the whole mesh.
+    rng = np.random.default_rng(7)
+    n = 64
+    pos = rng.uniform(-30, 30, size=(n, 3, 3)).astype(np.float32)
+    pos[:, :, 1] = rng.uniform(0.0, 2.0, size=(n, 3))
+    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
+    mesh = gi.scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    light = gi.scenes.View((0.0, 50.0, 0.1), (0.0, 0.0, 0.0), 1024, 1024, fov=90.0)
+    mvps = np.stack([gi.passes.view_matrices(light)[2]])
+    got, want = _depth_both(gi, oracle_lib, mesh, mvps, 1024, 1024)
+    assert bits_equal(got, want), mismatch_report(got, want, "overflow fallback")
+
+
+def test_depth_pass_deterministic(gi, cuda):
+    wl = gi.scenes.config3(nsamples=4, sm=2048)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    mvps = light_mvps(wl)
+    a = _np(gi.render_shadow_map(dm, mvps, 2048, 2048))
+    b = _np(gi.render_shadow_map(dm, mvps, 2048, 2048))
+    assert bits_equal(a, b)
+    assert a.min() >= 0.0 and a.max() <= 1.0
+
+
+def test_gbuffer_config1(gi, oracle_lib, cuda):
+    wl = gi.scenes.config1()
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    mvp = camera_mvp(wl)
+    g = gi.render_gbuffer(dm, mvp, 800, 600)
+    pos, nrm = oracle_lib.gbuffer(wl.mesh.positions, wl.mesh.normals, wl.mesh.indices, mvp, 800, 600)
+    assert bits_equal(_np(g.position), pos), mismatch_report(_np(g.position), pos, "gbuffer position")
+    assert bits_equal(_np(g.normal), nrm), mismatch_report(_np(g.normal), nrm, "gbuffer normal")
+
+
+def _hard_case(gi, oracle_lib, wl, technique, pre_eval, smw, smh, extra=None):
+    extra = dict(extra or {})
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    cam_mvp = camera_mvp(wl)
+    g = gi.render_gbuffer(dm, cam_mvp, wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)[:1]
+    depth = gi.render_shadow_map(dm, mvps, smw, smh)
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    params = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=smw, shadowMapHeight=smh, technique=technique,
+                             zbias=wl.zbias, preEvaluation=pre_eval, MV=MV, normalMatrix=nm, lightPosition=lpos,
+                             **extra)
+    got = _np(gi.compute_hard_shadows(g, depth[0], params))
+    op = oracle_lib.make_params(gi.biased(mvps[0], wl.zbias), smw, smh, technique,
+                                kernel_order=params.kernelOrder, penumbra_size=params.penumbraSize,
+                                max_search=params.maxSearch, depth_threshold=params.depthThreshold,
+                                shadow_intensity=params.shadowIntensity, pre_evaluation=pre_eval, mv=MV,
+                                normal_matrix=nm, light_position=lpos)
+    want = oracle_lib.hard_shadow(_np(g.position), _np(g.normal), _np(depth[0]), op)
+    return got, want
+
+
+@pytest.mark.parametrize("technique", ["naive", "pcf"])
+@pytest.mark.parametrize("pre_eval", [False, True])
+def test_hard_shadow_config1(gi, oracle_lib, cuda, technique, pre_eval):
+    wl = gi.scenes.config1()
+    got, want = _hard_case(gi, oracle_lib, wl, technique, pre_eval, 1024, 1024,
+                           {"kernelOrder": 3, "penumbraSize": 1})
+    if technique == "naive":
+        assert bits_equal(got, want), mismatch_report(got, want, "naive")
+        assert set(np.unique(want).tolist()) <= {0.0, 0.25, 1.0}
+    else:
+        assert np.abs(got - want).max() <= TOL, mismatch_report(got, want, "pcf")
+        assert bits_equal(got, want), mismatch_report(got, want, "pcf")
+
+
+@pytest.mark.parametrize("technique", ["rbsm", "rpcf", "rbsm_filtered"])
+def test_rbsm_soup(gi, oracle_lib, cuda, technique):
+    mesh = shrink_soup(30_000)
+    wl = gi.scenes.config2(ntris=10, view_w=640, view_h=360, sm=1024)
+    wl.mesh = mesh
+    wl.lights = [gi.scenes.ortho_fit((0.3, -1.0, 0.2), mesh, 1024, 1024)]
+    got, want = _hard_case(gi, oracle_lib, wl, technique, False, 1024, 1024,
+                           {"maxSearch": 16, "depthThreshold": 2.5e-6, "kernelOrder": 3, "penumbraSize": 1})
+    assert np.abs(got - want).max() <= TOL, mismatch_report(got, want, technique)
+    assert bits_equal(got, want), mismatch_report(got, want, technique)
+
+
+def _soft_case(gi, oracle_lib, wl, sm, params):
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    cam_mvp = camera_mvp(wl)
+    g = gi.render_gbuffer(dm, cam_mvp, wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, sm, sm)
+    got = _np(gi.render_monte_carlo(g, maps, mvps, params, zbias=wl.zbias))
+    common, trans = gi.soft_uniforms(mvps, wl.zbias)
+    want = oracle_lib.soft_shadow(_np(g.position), _np(maps), common, trans, si=params.shadowIntensity,
+                                  pcf=params.pcf, kernel_order=params.kernelOrder, penumbra=params.penumbra,
+                                  inclusive=params.inclusive)
+    return got, want
+
+
+@pytest.mark.parametrize("pcf,inclusive", [(False, True), (True, True), (True, False)])
+def test_soft_shadow_area_light(gi, oracle_lib, cuda, pcf, inclusive):
+    wl = gi.scenes.config3(nsamples=9, view_w=480, view_h=270, sm=1024, freq=40)
+    params = gi.SoftParams(pcf=pcf, inclusive=inclusive)
+    got, want = _soft_case(gi, oracle_lib, wl, 1024, params)
+    assert np.abs(got - want).max() <= TOL, mismatch_report(got, want, "soft")
+    assert bits_equal(got, want), mismatch_report(got, want, "soft")
+    vis = want[want != 0]
+    assert vis.min() >= 0.25 - 1e-7 and vis.max() <= 1.0 + 1e-7
+
+
+def test_soft_shadow_full_size_properties(gi, cuda):
+    """1080p, 64 samples, 2048^2 maps: the config-3 frame.  Properties that hold
+    at any size: background pixels 0, lit range [si, 1], and the frame equals
+    the mean of per-sample single-light frames computed separately."""
+    wl = gi.scenes.config3()
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 1920, 1080)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, 2048, 2048)
+    params = gi.SoftParams(pcf=True, inclusive=False)
+    full = gi.render_monte_carlo(g, maps, mvps, params)
+    bg = g.position[0] == 0
+    assert bool((full[bg] == 0).all())
+    fg = full[~bg]
+    assert float(fg.min()) >= 0.25 - 1e-6 and float(fg.max()) <= 1.0 + 1e-6
+    acc = torch.zeros_like(full)
+    for i in range(0, 64, 8):
+        part = gi.render_monte_carlo(g, maps[i:i + 8].contiguous(), mvps[i:i + 8], params)
+        acc += part * 8
+    mean = acc / 64
+    assert float((mean[~bg] - fg).abs().max()) <= 1e-5
