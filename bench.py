@@ -30,7 +30,8 @@ def _parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cfg3", choices=["cfg1", "cfg2", "cfg3"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-samples", type=int, default=2, help="light samples in the CPU baseline sample")
+    ap.add_argument("--cpu-samples", type=int, default=64, help="light samples in the CPU baseline sample")
+    ap.add_argument("--no-footprint", action="store_true", help="skip counting the unique texels gathered")
     return ap.parse_args()
 
 
@@ -161,7 +162,7 @@ def main():
     # Roofline of the screen pass (single kernel per call): algorithmic bytes =
     # G-buffer position reads + output writes + unique texels gathered.
     taps = tap_offsets(3, 1.0, False)
-    texels = touched_texels(gi, torch, g, mvps, smw, smw, taps)
+    texels = 0 if args.no_footprint else touched_texels(gi, torch, g, mvps, smw, smw, taps)
     npx = W * H
     screen_bytes = npx * (12 + 4) + texels * 4
     depth_bytes = S * smw * smw * 4 + S * (wl.mesh.ntris * 12 + wl.mesh.nverts * 12)

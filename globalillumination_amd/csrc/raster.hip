@@ -293,20 +293,101 @@ __device__ __forceinline__ bool rect_overlap(const Tri<NA>& t, int x0, int x1, i
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
 
 // ---------------------------------------------------------------------------
+// Compact setup record of one screen-space piece.  Wide pieces get one record
+// per map in HBM (written once by the binning pass); pieces a whole workgroup
+// rasterizes are shared through LDS.
+// ---------------------------------------------------------------------------
+struct Setup {
+    int X[3], Y[3];            // 24.8 fixed-point window coordinates
+    int px0, px1, py0, py1;    // pixel-centre bounding box (clamped)
+    int tl;                    // bit k: top-left tie bit of the edge opposite vertex k
+    uint32_t id;               // tri * 8 + fan piece
+    double z0, dz1, dz2, inv_area, offset;
+};
+
+__device__ __forceinline__ Setup to_setup(const Tri<0>& t, uint32_t id) {
+    Setup s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s.X[i] = (int)t.X[i];
+        s.Y[i] = (int)t.Y[i];
+    }
+    s.px0 = t.px0; s.px1 = t.px1; s.py0 = t.py0; s.py1 = t.py1;
+    s.tl = t.tl[0] | (t.tl[1] << 1) | (t.tl[2] << 2);
+    s.id = id;
+    s.z0 = t.z0; s.dz1 = t.dz1; s.dz2 = t.dz2; s.inv_area = t.inv_area; s.offset = t.offset;
+    return s;
+}
+
+// Same arithmetic as eval_pixel on the compact record.
+__device__ __forceinline__ bool eval_setup(const Setup& s, int i, int j, float& zout) {
+    const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+    long long e0 = (long long)(s.X[2] - s.X[1]) * (Py - s.Y[1]) - (long long)(s.Y[2] - s.Y[1]) * (Px - s.X[1]);
+    long long e1 = (long long)(s.X[0] - s.X[2]) * (Py - s.Y[2]) - (long long)(s.Y[0] - s.Y[2]) * (Px - s.X[2]);
+    long long e2 = (long long)(s.X[1] - s.X[0]) * (Py - s.Y[0]) - (long long)(s.Y[1] - s.Y[0]) * (Px - s.X[0]);
+    if (e0 + (s.tl & 1) <= 0 || e1 + ((s.tl >> 1) & 1) <= 0 || e2 + ((s.tl >> 2) & 1) <= 0) return false;
+    double zd = s.z0 + ((double)e1 * s.dz1 + (double)e2 * s.dz2) * s.inv_area;
+    zd = zd + s.offset;
+    float z = (float)zd;
+    if (!(z > 0.0f)) z = 0.0f;
+    if (z > 1.0f) z = 1.0f;
+    zout = z;
+    return true;
+}
+
+__device__ __forceinline__ bool rect_overlap_s(const Setup& s, int x0, int x1, int y0, int y1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        int a = (k + 1) % 3, b = (k + 2) % 3;
+        long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
+        long long Px = (long long)((ey < 0) ? x1 : x0) * 256 + 128;
+        long long Py = (long long)((ex > 0) ? y1 : y0) * 256 + 128;
+        long long e = ex * (Py - s.Y[a]) - ey * (Px - s.X[a]);
+        if (e + ((s.tl >> k) & 1) <= 0) return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
 // Binning
 // ---------------------------------------------------------------------------
+constexpr int kWideSpan = 4;            // pieces overlapping more tiles are "wide"
+constexpr uint32_t kWideBit = 0x80000000u;
+constexpr int kWideBlocks = 64;         // workgroups per map binning wide pieces
+
+struct BinBufs {
+    uint32_t* counts;    // nmaps x ntiles (scatter cursors after k_scan)
+    uint32_t* offsets;   // nmaps x (ntiles + 1)
+    int* overflow;       // nmaps: lists or wide records exceed the workspace
+    int* wide_over;      // nmaps
+    uint32_t* nwide;     // nmaps
+    Setup* wide;         // nmaps x wcap
+    uint32_t* entries;   // nmaps x cap
+    uint32_t cap, wcap;
+};
+
+__device__ __forceinline__ void bin_tile(const BinBufs& b, int map, int ntiles, int tile, uint32_t entry,
+                                         bool scatter) {
+    uint32_t* cnt = b.counts + (size_t)map * ntiles;
+    if (!scatter) {
+        atomicAdd(&cnt[tile], 1u);
+    } else {
+        uint32_t pos = b.offsets[(size_t)map * (ntiles + 1) + tile] + atomicAdd(&cnt[tile], 1u);
+        if (pos < b.cap) b.entries[(size_t)map * b.cap + pos] = entry;
+    }
+}
+
+// One thread per (triangle, map).  Pieces spanning at most kWideSpan tiles are
+// binned here; wider pieces get a setup record for k_bin_wide (count pass).
 template <bool kScatter>
-__global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, uint32_t* counts,
-                                             const uint32_t* offsets, const int* overflow, uint32_t* entries,
-                                             uint32_t cap) {
+__global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
     int tri = blockIdx.x * blockDim.x + threadIdx.x;
     int map = blockIdx.y;
     if (tri >= mesh.nt) return;
     int ntiles = cfg.tiles_x * cfg.tiles_y;
-    if (kScatter && overflow[map]) return;
+    if (kScatter && b.overflow[map]) return;
     float mvp[16];
     mvps.load(map, mvp);
-    uint32_t* cnt = counts + (size_t)map * ntiles;
     int npieces = 1;
     for (int piece = 0; piece < npieces; ++piece) {
         Tri<0> t;
@@ -315,37 +396,60 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
         if (!ok) continue;
         uint32_t entry = (uint32_t)tri * 8u + (uint32_t)piece;
         int tx0 = t.px0 / kTile, tx1 = t.px1 / kTile, ty0 = t.py0 / kTile, ty1 = t.py1 / kTile;
-        bool single = (tx0 == tx1 && ty0 == ty1);
+        int span = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+        if (span > kWideSpan) {
+            if (!kScatter) {
+                uint32_t w = atomicAdd(&b.nwide[map], 1u);
+                if (w < b.wcap) b.wide[(size_t)map * b.wcap + w] = to_setup(t, entry);
+                else b.wide_over[map] = 1;
+            }
+            continue;
+        }
         for (int ty = ty0; ty <= ty1; ++ty)
             for (int tx = tx0; tx <= tx1; ++tx) {
-                if (!single) {
+                if (span > 1) {
                     int x0 = max(tx * kTile, t.px0), x1 = min(tx * kTile + kTile - 1, t.px1);
                     int y0 = max(ty * kTile, t.py0), y1 = min(ty * kTile + kTile - 1, t.py1);
                     if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
                 }
-                int tile = ty * cfg.tiles_x + tx;
-                if (!kScatter) {
-                    atomicAdd(&cnt[tile], 1u);
-                } else {
-                    uint32_t pos = offsets[(size_t)map * (ntiles + 1) + tile] + atomicAdd(&cnt[tile], 1u);
-                    if (pos < cap) entries[(size_t)map * cap + pos] = entry;
-                }
+                bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, entry, kScatter);
             }
     }
 }
 
+// Wide pieces: a workgroup per record, its threads walk the record's tiles.
+template <bool kScatter>
+__global__ __launch_bounds__(256) void k_bin_wide(RasterCfg cfg, BinBufs b) {
+    const int map = blockIdx.y;
+    if (kScatter && b.overflow[map]) return;
+    const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    const uint32_t n = min(b.nwide[map], b.wcap);
+    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+        const Setup s = b.wide[(size_t)map * b.wcap + w];
+        int tx0 = s.px0 / kTile, tx1 = s.px1 / kTile, ty0 = s.py0 / kTile, ty1 = s.py1 / kTile;
+        int ntx = tx1 - tx0 + 1;
+        int span = ntx * (ty1 - ty0 + 1);
+        for (int q = threadIdx.x; q < span; q += blockDim.x) {
+            int tx = tx0 + q % ntx, ty = ty0 + q / ntx;
+            int x0 = max(tx * kTile, s.px0), x1 = min(tx * kTile + kTile - 1, s.px1);
+            int y0 = max(ty * kTile, s.py0), y1 = min(ty * kTile + kTile - 1, s.py1);
+            if (!rect_overlap_s(s, x0, x1, y0, y1)) continue;
+            bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, kWideBit | w, kScatter);
+        }
+    }
+}
+
 // Exclusive scan of one map's tile counts; resets counts to 0 (they become
-// the scatter cursors) and flags maps whose lists exceed `cap` entries.
-__global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* counts, uint32_t* offsets, int* overflow,
-                                                       int ntiles, uint32_t cap) {
+// the scatter cursors) and flags maps whose lists exceed the workspace.
+__global__ __launch_bounds__(kScanThreads) void k_scan(BinBufs b, int ntiles) {
     __shared__ unsigned long long part[kScanThreads];
     int map = blockIdx.x;
-    uint32_t* c = counts + (size_t)map * ntiles;
-    uint32_t* o = offsets + (size_t)map * (ntiles + 1);
+    uint32_t* c = b.counts + (size_t)map * ntiles;
+    uint32_t* o = b.offsets + (size_t)map * (ntiles + 1);
     int per = (ntiles + kScanThreads - 1) / kScanThreads;
-    int b = threadIdx.x * per, e = min(b + per, ntiles);
+    int lo = threadIdx.x * per, hi = min(lo + per, ntiles);
     unsigned long long s = 0;
-    for (int i = b; i < e; ++i) s += c[i];
+    for (int i = lo; i < hi; ++i) s += c[i];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int off = 1; off < kScanThreads; off <<= 1) {
@@ -355,15 +459,16 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(uint32_t* counts, uint32_
         __syncthreads();
     }
     unsigned long long run = part[threadIdx.x] - s;  // exclusive prefix
-    for (int i = b; i < e; ++i) {
+    for (int i = lo; i < hi; ++i) {
         o[i] = (uint32_t)run;
         run += c[i];
         c[i] = 0;
     }
     if (threadIdx.x == kScanThreads - 1) {
         unsigned long long total = part[kScanThreads - 1];
-        o[ntiles] = (uint32_t)(total > cap ? cap : total);
-        overflow[map] = total > cap ? 1 : 0;
+        bool over = total > b.cap || b.wide_over[map] != 0;
+        o[ntiles] = (uint32_t)(total > b.cap ? b.cap : total);
+        b.overflow[map] = over ? 1 : 0;
     }
 }
 
@@ -384,20 +489,17 @@ struct KeyOf<true> {
 };
 
 template <bool kVis>
-__device__ __forceinline__ void raster_rect(const Tri<0>& t, uint32_t id, int x0, int y0, int rx0, int rx1,
-                                            int ry0, int ry1, int first, int stride,
-                                            typename KeyOf<kVis>::T* tile) {
+__device__ __forceinline__ void raster_rect(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
+                                            int first, int stride, typename KeyOf<kVis>::T* tile) {
     int rw = rx1 - rx0 + 1;
     int n = rw * (ry1 - ry0 + 1);
     for (int p = first; p < n; p += stride) {
         int i = rx0 + p % rw, j = ry0 + p / rw;
-        long long e[3];
         float z;
-        if (!eval_pixel<0>(t, i, j, e, z)) continue;
+        if (!eval_setup(s, i, j, z)) continue;
         int slot = (j - y0) * kTile + (i - x0);
         if constexpr (kVis) {
-            unsigned long long key = ((unsigned long long)fbits(z) << 32) | id;
-            atomicMin(&tile[slot], key);
+            atomicMin(&tile[slot], ((unsigned long long)fbits(z) << 32) | s.id);
         } else {
             atomicMin(&tile[slot], fbits(z));
         }
@@ -405,13 +507,12 @@ __device__ __forceinline__ void raster_rect(const Tri<0>& t, uint32_t id, int x0
 }
 
 template <bool kVis>
-__global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg,
-                                                           const uint32_t* offsets, const int* overflow,
-                                                           const uint32_t* entries, uint32_t cap, void* out) {
+__global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
+                                                           void* out) {
     using K = typename KeyOf<kVis>::T;
     __shared__ K tile[kTilePx];
-    __shared__ uint32_t bigq[kRasterThreads * 7];
-    __shared__ int nbig;
+    __shared__ Setup rec[kRasterThreads];
+    __shared__ int nrec;
     const int map = blockIdx.y;
     const int tid = threadIdx.x;
     const int tx = blockIdx.x % cfg.tiles_x, ty = blockIdx.x / cfg.tiles_x;
@@ -419,60 +520,54 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
     const int x1 = min(x0 + kTile - 1, cfg.w - 1), y1 = min(y0 + kTile - 1, cfg.h - 1);
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     for (int p = tid; p < kTilePx; p += kRasterThreads) tile[p] = KeyOf<kVis>::clear;
-    if (tid == 0) nbig = 0;
+    if (tid == 0) nrec = 0;
     float mvp[16];
     mvps.load(map, mvp);
-    const bool fallback = overflow[map] != 0;
+    // Overflowed maps scan every (triangle, piece) pair instead of a list.
+    const bool fallback = b.overflow[map] != 0;
     uint32_t begin = 0, end;
     if (!fallback) {
-        begin = offsets[(size_t)map * (ntiles + 1) + blockIdx.x];
-        end = offsets[(size_t)map * (ntiles + 1) + blockIdx.x + 1];
+        begin = b.offsets[(size_t)map * (ntiles + 1) + blockIdx.x];
+        end = b.offsets[(size_t)map * (ntiles + 1) + blockIdx.x + 1];
     } else {
-        end = (uint32_t)mesh.nt;
+        end = (uint32_t)mesh.nt * 8u;
     }
     __syncthreads();
     for (uint32_t base = begin; base < end; base += kRasterThreads) {
         uint32_t k = base + tid;
         if (k < end) {
-            int tri, piece0, npieces = 1;
-            if (!fallback) {
-                uint32_t entry = entries[(size_t)map * cap + k];
-                tri = (int)(entry >> 3);
-                piece0 = (int)(entry & 7u);
+            uint32_t entry = fallback ? k : b.entries[(size_t)map * b.cap + k];
+            if (entry & kWideBit) {
+                int slot = atomicAdd(&nrec, 1);
+                rec[slot] = b.wide[(size_t)map * b.wcap + (entry & ~kWideBit)];
             } else {
-                tri = (int)k;
-                piece0 = 0;
-            }
-            for (int piece = piece0; piece < piece0 + npieces; ++piece) {
                 Tri<0> t;
                 bool ok;
-                int np = get_piece<0>(mesh, mvp, tri, piece, cfg, t, ok);
-                if (fallback) npieces = np;
-                if (!ok) continue;
-                int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
-                if (rx0 > rx1 || ry0 > ry1) continue;
-                int area = (rx1 - rx0 + 1) * (ry1 - ry0 + 1);
-                uint32_t id = (uint32_t)tri * 8u + (uint32_t)piece;
-                if (area <= kSelfRasterMaxPx) {
-                    raster_rect<kVis>(t, id, x0, y0, rx0, rx1, ry0, ry1, 0, 1, tile);
-                } else {
-                    int slot = atomicAdd(&nbig, 1);
-                    bigq[slot] = id;
+                get_piece<0>(mesh, mvp, (int)(entry >> 3), (int)(entry & 7u), cfg, t, ok);
+                if (ok) {
+                    int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
+                    if (rx0 <= rx1 && ry0 <= ry1) {
+                        Setup s = to_setup(t, entry);
+                        if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) <= kSelfRasterMaxPx) {
+                            raster_rect<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, 0, 1, tile);
+                        } else {
+                            int slot = atomicAdd(&nrec, 1);
+                            rec[slot] = s;
+                        }
+                    }
                 }
             }
         }
         __syncthreads();
-        int nb = nbig;
-        for (int q = 0; q < nb; ++q) {
-            uint32_t id = bigq[q];
-            Tri<0> t;
-            bool ok;
-            get_piece<0>(mesh, mvp, (int)(id >> 3), (int)(id & 7u), cfg, t, ok);
-            int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
-            raster_rect<kVis>(t, id, x0, y0, rx0, rx1, ry0, ry1, tid, kRasterThreads, tile);
+        const int nq = nrec;
+        for (int q = 0; q < nq; ++q) {
+            const Setup& s = rec[q];
+            int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
+            if (rx0 > rx1 || ry0 > ry1) continue;
+            raster_rect<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, kRasterThreads, tile);
         }
         __syncthreads();
-        if (tid == 0) nbig = 0;
+        if (tid == 0) nrec = 0;
         __syncthreads();
     }
     // Write the tile once.
@@ -564,42 +659,61 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
 namespace {
 
 struct Workspace {
-    uint32_t* counts;
-    uint32_t* offsets;
-    int* overflow;
-    uint32_t* entries;
+    BinBufs bins;
     void* vis;  // gbuffer only
-    uint32_t cap;
 };
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 uint32_t entry_cap(int ntris, int ntiles) {
-    unsigned long long cap = 8ull * (unsigned long long)ntris + 4ull * (unsigned long long)ntiles + 1024ull;
-    if (cap > 0xF0000000ull) cap = 0xF0000000ull;
+    unsigned long long cap = 4ull * (unsigned long long)ntris + 4ull * (unsigned long long)ntiles + 1024ull;
+    if (cap > 0x7FFFFFFFull) cap = 0x7FFFFFFFull;
     return (uint32_t)cap;
 }
+
+uint32_t wide_cap(int ntris) { return 1024u + (uint32_t)ntris / 16u; }
 
 size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char* base) {
     int tx = (w + kTile - 1) / kTile, ty = (h + kTile - 1) / kTile;
     size_t ntiles = (size_t)tx * ty;
     uint32_t cap = entry_cap(ntris, (int)ntiles);
+    uint32_t wcap = wide_cap(ntris);
     size_t off = 0;
     size_t o_counts = off; off = align256(off + sizeof(uint32_t) * ntiles * nmaps);
+    size_t o_nwide = off; off = align256(off + sizeof(uint32_t) * nmaps);
+    size_t o_wover = off; off = align256(off + sizeof(int) * nmaps);
+    size_t zero_end = off;  // [0, zero_end) is cleared before binning
     size_t o_offsets = off; off = align256(off + sizeof(uint32_t) * (ntiles + 1) * nmaps);
     size_t o_over = off; off = align256(off + sizeof(int) * nmaps);
+    size_t o_wide = off; off = align256(off + sizeof(Setup) * (size_t)wcap * nmaps);
     size_t o_entries = off; off = align256(off + sizeof(uint32_t) * (size_t)cap * nmaps);
     size_t o_vis = off;
     if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
     if (ws) {
-        ws->counts = reinterpret_cast<uint32_t*>(base + o_counts);
-        ws->offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
-        ws->overflow = reinterpret_cast<int*>(base + o_over);
-        ws->entries = reinterpret_cast<uint32_t*>(base + o_entries);
+        BinBufs& b = ws->bins;
+        b.counts = reinterpret_cast<uint32_t*>(base + o_counts);
+        b.nwide = reinterpret_cast<uint32_t*>(base + o_nwide);
+        b.wide_over = reinterpret_cast<int*>(base + o_wover);
+        b.offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
+        b.overflow = reinterpret_cast<int*>(base + o_over);
+        b.wide = reinterpret_cast<Setup*>(base + o_wide);
+        b.entries = reinterpret_cast<uint32_t*>(base + o_entries);
+        b.cap = cap;
+        b.wcap = wcap;
         ws->vis = vis ? reinterpret_cast<void*>(base + o_vis) : nullptr;
-        ws->cap = cap;
     }
+    (void)zero_end;
     return off;
+}
+
+// Words to clear before binning: counts, nwide and wide_over are laid out
+// contiguously at the start of the workspace.
+size_t zero_words(int nmaps, const RasterCfg& cfg) {
+    size_t ntiles = (size_t)cfg.tiles_x * cfg.tiles_y;
+    size_t off = align256(sizeof(uint32_t) * ntiles * nmaps);
+    off = align256(off + sizeof(uint32_t) * nmaps);
+    off = align256(off + sizeof(int) * nmaps);
+    return off / sizeof(uint32_t);
 }
 
 bool size_ok(int w, int h) { return w > 0 && h > 0 && w <= GI_MAX_EDGE && h <= GI_MAX_EDGE; }
@@ -607,24 +721,24 @@ bool size_ok(int w, int h) { return w > 0 && h > 0 && w <= GI_MAX_EDGE && h <= G
 template <bool kVis>
 int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const RasterCfg& cfg, const Workspace& ws,
                void* out, hipStream_t s) {
-    int ntiles = cfg.tiles_x * cfg.tiles_y;
-    size_t ncounts = (size_t)ntiles * nmaps;
-    size_t fb = (ncounts + 255) / 256;
+    const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    const BinBufs& b = ws.bins;
+    size_t nz = zero_words(nmaps, cfg);
+    size_t fb = (nz + 255) / 256;
     unsigned fill_blocks = (unsigned)(fb < 4096 ? fb : 4096);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, ws.counts, ncounts, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fill_blocks), dim3(256), 0, s, b.counts, nz, 0u);
     dim3 gb((mesh.nt + 255) / 256, nmaps);
+    dim3 gw(kWideBlocks, nmaps);
     if (mesh.nt > 0) {
-        hipLaunchKernelGGL(k_bin<false>, gb, dim3(256), 0, s, mesh, mvps, cfg, ws.counts, ws.offsets,
-                           ws.overflow, ws.entries, ws.cap);
+        hipLaunchKernelGGL(k_bin<false>, gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        hipLaunchKernelGGL(k_bin_wide<false>, gw, dim3(256), 0, s, cfg, b);
     }
-    hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, ws.counts, ws.offsets, ws.overflow, ntiles,
-                       ws.cap);
+    hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, b, ntiles);
     if (mesh.nt > 0) {
-        hipLaunchKernelGGL(k_bin<true>, gb, dim3(256), 0, s, mesh, mvps, cfg, ws.counts, ws.offsets, ws.overflow,
-                           ws.entries, ws.cap);
+        hipLaunchKernelGGL(k_bin<true>, gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        hipLaunchKernelGGL(k_bin_wide<true>, gw, dim3(256), 0, s, cfg, b);
     }
-    hipLaunchKernelGGL(k_raster<kVis>, dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg,
-                       ws.offsets, ws.overflow, ws.entries, ws.cap, out);
+    hipLaunchKernelGGL(k_raster<kVis>, dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
@@ -646,7 +760,7 @@ int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices
     if (!positions_dev || (!indices_dev && ntris > 0) || !mvps_dev || !depth_out_dev || !workspace_dev)
         return GI_EINVAL;
     if (nverts <= 0 || ntris < 0 || nmaps <= 0) return GI_EINVAL;
-    if (ntris >= (1 << 29)) return GI_ENOTSUP;
+    if (ntris >= (1 << 28)) return GI_ENOTSUP;
     if (!size_ok(width, height)) return GI_ENOTSUP;
     if ((reinterpret_cast<uintptr_t>(depth_out_dev) & 15u) != 0) return GI_EINVAL;
     float off_f = opts ? opts->polygon_offset_factor : 0.0f;
@@ -674,7 +788,7 @@ int gi_gbuffer_pass(const float* positions_dev, const float* normals_dev, int nv
         !normal_out_dev || !workspace_dev)
         return GI_EINVAL;
     if (nverts <= 0 || ntris < 0) return GI_EINVAL;
-    if (ntris >= (1 << 29)) return GI_ENOTSUP;
+    if (ntris >= (1 << 28)) return GI_ENOTSUP;
     if (!size_ok(width, height)) return GI_ENOTSUP;
     for (int i = 0; i < 16; ++i)
         if (!std::isfinite(mvp[i])) return GI_ENONFINITE;
