@@ -206,7 +206,7 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
 // Clips a triangle that crosses a clip plane and sets up fan piece `piece`.
 // Returns the number of fan pieces; `ok` tells whether `piece` is non-empty.
 template <int NA>
-__device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
+__device__ __forceinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
                                            Tri<NA>& t, bool& ok) {
     CV<NA> poly[kMaxPoly], tmp[kMaxPoly];
     poly[0] = v[0];
@@ -488,21 +488,77 @@ struct KeyOf<true> {
     static constexpr unsigned long long clear = ~0ull;
 };
 
+// Edge functions of `s` at pixel centre (i, j) and their per-pixel steps.
+// Stepping adds exact integers, so every value equals eval_setup's.
+struct Edges {
+    long long e[3];
+    long long dx[3], dy[3];  // change per +1 pixel in x / y
+};
+
+__device__ __forceinline__ void edges_at(const Setup& s, int i, int j, Edges& E) {
+    const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int a = (k + 1) % 3, b = (k + 2) % 3;
+        const long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
+        E.e[k] = ex * (Py - s.Y[a]) - ey * (Px - s.X[a]);
+        E.dx[k] = -ey * 256;
+        E.dy[k] = ex * 256;
+    }
+}
+
 template <bool kVis>
-__device__ __forceinline__ void raster_rect(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
-                                            int first, int stride, typename KeyOf<kVis>::T* tile) {
-    int rw = rx1 - rx0 + 1;
-    int n = rw * (ry1 - ry0 + 1);
-    for (int p = first; p < n; p += stride) {
-        int i = rx0 + p % rw, j = ry0 + p / rw;
-        float z;
-        if (!eval_setup(s, i, j, z)) continue;
-        int slot = (j - y0) * kTile + (i - x0);
-        if constexpr (kVis) {
-            atomicMin(&tile[slot], ((unsigned long long)fbits(z) << 32) | s.id);
-        } else {
-            atomicMin(&tile[slot], fbits(z));
+__device__ __forceinline__ void plot(const Setup& s, const long long e[3], int slot, typename KeyOf<kVis>::T* tile) {
+    if (e[0] + (s.tl & 1) <= 0 || e[1] + ((s.tl >> 1) & 1) <= 0 || e[2] + ((s.tl >> 2) & 1) <= 0) return;
+    double zd = s.z0 + ((double)e[1] * s.dz1 + (double)e[2] * s.dz2) * s.inv_area;
+    zd = zd + s.offset;
+    float z = (float)zd;
+    if (!(z > 0.0f)) z = 0.0f;
+    if (z > 1.0f) z = 1.0f;
+    if constexpr (kVis) {
+        atomicMin(&tile[slot], ((unsigned long long)fbits(z) << 32) | s.id);
+    } else {
+        atomicMin(&tile[slot], fbits(z));
+    }
+}
+
+// One thread covers the whole (small) rectangle, row by row.
+template <bool kVis>
+__device__ __forceinline__ void raster_rect_self(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
+                                                 typename KeyOf<kVis>::T* tile) {
+    Edges E;
+    edges_at(s, rx0, ry0, E);
+    long long row[3] = {E.e[0], E.e[1], E.e[2]};
+    for (int j = ry0; j <= ry1; ++j) {
+        long long e[3] = {row[0], row[1], row[2]};
+        for (int i = rx0; i <= rx1; ++i) {
+            plot<kVis>(s, e, (j - y0) * kTile + (i - x0), tile);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) e[k] += E.dx[k];
         }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) row[k] += E.dy[k];
+    }
+}
+
+// The workgroup covers the rectangle: thread t owns tile column t % 32 and
+// every (kRasterThreads / 32)-th row from row t / 32.
+template <bool kVis>
+__device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
+                                                  int tid, typename KeyOf<kVis>::T* tile) {
+    constexpr int kRowStep = kRasterThreads / kTile;
+    const int i = x0 + (tid % kTile);
+    if (i < rx0 || i > rx1) return;
+    int j = y0 + tid / kTile;
+    if (j < ry0) j += ((ry0 - j + kRowStep - 1) / kRowStep) * kRowStep;
+    if (j > ry1) return;
+    Edges E;
+    edges_at(s, i, j, E);
+    long long e[3] = {E.e[0], E.e[1], E.e[2]};
+    for (; j <= ry1; j += kRowStep) {
+        plot<kVis>(s, e, (j - y0) * kTile + (i - x0), tile);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e[k] += E.dy[k] * kRowStep;
     }
 }
 
@@ -549,7 +605,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
                     if (rx0 <= rx1 && ry0 <= ry1) {
                         Setup s = to_setup(t, entry);
                         if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) <= kSelfRasterMaxPx) {
-                            raster_rect<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, 0, 1, tile);
+                            raster_rect_self<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
                         } else {
                             int slot = atomicAdd(&nrec, 1);
                             rec[slot] = s;
@@ -564,7 +620,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
             const Setup& s = rec[q];
             int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
             if (rx0 > rx1 || ry0 > ry1) continue;
-            raster_rect<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, kRasterThreads, tile);
+            raster_rect_group<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, tile);
         }
         __syncthreads();
         if (tid == 0) nrec = 0;

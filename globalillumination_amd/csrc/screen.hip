@@ -328,17 +328,35 @@ __global__ __launch_bounds__(kBlockX* kBlockY) void k_hard(GBufferView gb, Depth
     out[k] = shadow;
 }
 
+constexpr int kMaxTaps = 16;
+
 struct SoftUniforms {
     float m[12];  // rows 0-3 of columns 0-2 of B*lightMVP (AccurateSoftShadow.frag:70-73)
     float si;
     int nsamples;
     int pcf;
-    int kernel_order;
-    float penumbra;
-    int inclusive;
+    int ntaps;             // taps per axis of the PCF loop
+    float taps[kMaxTaps];  // loop values w of `for (w = -offset; w </<= offset; w += step)`
 };
 
-// AccurateSoftShadow.frag:52-133 (monteCarlo: accFactor = 1).
+// Rows of NT consecutive texels; selecting element k of a row.
+template <int NT>
+struct Row { float v[NT]; };
+
+template <int NT>
+__device__ __forceinline__ float pick(const Row<NT>& r, int k) {
+    float x = r.v[0];
+#pragma unroll
+    for (int q = 1; q < NT; ++q) x = (k == q) ? r.v[q] : x;
+    return x;
+}
+
+// AccurateSoftShadow.frag:52-133 (monteCarlo: accFactor = 1).  NT > 0: the
+// PCF loop has NT taps per axis, and when every tap of a sample lies inside
+// the map the NT x NT taps are served by NT row loads of NT consecutive
+// texels (taps of one sample span at most NT texels per axis because the
+// tap offsets span less than NT texels).  NT == 0: generic per-tap loop.
+template <int NT>
 __global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
                                                             int smh, const float4* __restrict__ trans,
                                                             SoftUniforms u, float* __restrict__ out) {
@@ -364,8 +382,6 @@ __global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const
     const size_t mapsz = (size_t)smw * smh;
     DepthMap D{maps, smw, smh, (float)smw, (float)smh};
     const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
-    const float offset = u.penumbra;
-    const float stepSize = 2.0f * offset / (float)u.kernel_order;
     float acc = 0.0f, count = 0.0f;
     for (int i = 0; i < u.nsamples; ++i) {
         float4 T = trans[i];
@@ -375,22 +391,73 @@ __global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const
         if (!u.pcf) {
             float d = D.fetch(sx, sy);
             acc = acc + ((sz <= d) ? 1.0f : u.si);
-        } else {
+        } else if constexpr (NT > 0) {
+            float tx[NT], ty[NT];
+            int iu[NT], iv[NT];
+            bool inside = true;
+#pragma unroll
+            for (int a = 0; a < NT; ++a) {
+                tx[a] = sx + u.taps[a] * incrWidth;
+                ty[a] = sy + u.taps[a] * incrHeight;
+                float uu = tx[a] * D.fw, vv = ty[a] * D.fh;
+                inside = inside && (uu >= 0.0f && uu < D.fw && vv >= 0.0f && vv < D.fh);
+                iu[a] = (int)floorf(uu);
+                iv[a] = (int)floorf(vv);
+            }
+            // one row of NT texels from iu[0] must cover every tap column
+            inside = inside && (iu[NT - 1] - iu[0] <= NT - 1) && (iu[0] + NT - 1 < smw);
             float illum = 0.0f;
-            int n = 0;
-            for (float w = -offset; u.inclusive ? (w <= offset) : (w < offset); w += stepSize) {
-                float tx = sx + w * incrWidth;
-                for (float h = -offset; u.inclusive ? (h <= offset) : (h < offset); h += stepSize) {
-                    float d = D.fetch(tx, sy + h * incrHeight);
-                    illum = illum + ((sz <= d) ? 1.0f : u.si);
-                    n++;
+            if (inside) {
+                Row<NT> rows[NT];
+#pragma unroll
+                for (int b = 0; b < NT; ++b)
+                    __builtin_memcpy(&rows[b], D.texels + (size_t)iv[b] * smw + iu[0], sizeof(Row<NT>));
+#pragma unroll
+                for (int a = 0; a < NT; ++a) {
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) {
+                        float d = pick<NT>(rows[b], iu[a] - iu[0]);
+                        illum = illum + ((sz <= d) ? 1.0f : u.si);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int a = 0; a < NT; ++a) {
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) {
+                        float d = D.fetch(tx[a], ty[b]);
+                        illum = illum + ((sz <= d) ? 1.0f : u.si);
+                    }
                 }
             }
-            acc = acc + illum / (float)n;
+            acc = acc + illum / (float)(NT * NT);
+        } else {
+            float illum = 0.0f;
+            for (int a = 0; a < u.ntaps; ++a) {
+                float tx = sx + u.taps[a] * incrWidth;
+                for (int b = 0; b < u.ntaps; ++b) {
+                    float d = D.fetch(tx, sy + u.taps[b] * incrHeight);
+                    illum = illum + ((sz <= d) ? 1.0f : u.si);
+                }
+            }
+            acc = acc + illum / (float)(u.ntaps * u.ntaps);
         }
         count = count + 1.0f;
     }
     out[k] = acc / count;
+}
+
+// Tap values of the PCF loop `for (w = -offset; w < offset (or <=); w += step)`
+// with step = 2*offset/kernelOrder, accumulated in FP32 exactly as the shader
+// loop does (AccurateSoftShadow.frag:101-109, Shadow.frag:93-99).
+int pcf_taps(float offset, int kernel_order, bool inclusive, float* taps, int max_taps) {
+    float stepSize = 2.0f * offset / (float)kernel_order;
+    int n = 0;
+    for (float w = -offset; inclusive ? (w <= offset) : (w < offset); w += stepSize) {
+        if (n == max_taps) return -1;
+        taps[n++] = w;
+    }
+    return n;
 }
 
 namespace {
@@ -470,7 +537,7 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE) return GI_EINVAL;
     if (!finite_all(light_mvp, 16) || !std::isfinite(p->shadow_intensity) || !std::isfinite(p->penumbra))
         return GI_ENONFINITE;
-    if (p->pcf && (p->kernel_order <= 0 || p->kernel_order > 64 || !(p->penumbra > 0.0f) || p->penumbra > 64.0f))
+    if (p->pcf && (p->kernel_order <= 0 || p->kernel_order > 14 || !(p->penumbra > 0.0f) || p->penumbra > 64.0f))
         return GI_EINVAL;
     if ((reinterpret_cast<uintptr_t>(trans_dev) & 15u) != 0) return GI_EINVAL;
     SoftUniforms u;
@@ -478,13 +545,22 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     u.si = p->shadow_intensity;
     u.nsamples = nsamples;
     u.pcf = p->pcf ? 1 : 0;
-    u.kernel_order = p->pcf ? p->kernel_order : 1;
-    u.penumbra = p->pcf ? p->penumbra : 1.0f;
-    u.inclusive = p->inclusive ? 1 : 0;
+    u.ntaps = 1;
+    for (int i = 0; i < kMaxTaps; ++i) u.taps[i] = 0.0f;
+    if (p->pcf) {
+        u.ntaps = pcf_taps(p->penumbra, p->kernel_order, p->inclusive != 0, u.taps, kMaxTaps);
+        if (u.ntaps <= 0) return GI_ENOTSUP;
+    }
     GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_soft, grid_for(gb.w, gb.h), dim3(kBlockX, kBlockY), 0, s, gb, maps_dev, smw, smh,
-                       reinterpret_cast<const float4*>(trans_dev), u, out_dev);
+    const float4* tr = reinterpret_cast<const float4*>(trans_dev);
+    dim3 grid = grid_for(gb.w, gb.h), block(kBlockX, kBlockY);
+    if (u.pcf && u.ntaps == 3)
+        hipLaunchKernelGGL(k_soft<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
+    else if (u.pcf && u.ntaps == 4)
+        hipLaunchKernelGGL(k_soft<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
+    else
+        hipLaunchKernelGGL(k_soft<0>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
