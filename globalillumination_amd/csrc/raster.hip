@@ -38,12 +38,12 @@
 
 namespace gi {
 
-constexpr int kTile = 32;
+constexpr int kTile = 64;
 constexpr int kTilePx = kTile * kTile;
 constexpr int kRasterThreads = 256;
 constexpr float kGuard = 4.0f;
 constexpr int kMaxPoly = 9;
-constexpr int kSelfRasterMaxPx = 32;  // bbox∩tile area a lone thread rasterizes itself
+constexpr int kSelfRasterMaxPx = 256;  // bbox∩tile area a lone thread rasterizes itself
 constexpr int kScanThreads = 1024;
 
 struct MeshView {
@@ -206,7 +206,7 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
 // Clips a triangle that crosses a clip plane and sets up fan piece `piece`.
 // Returns the number of fan pieces; `ok` tells whether `piece` is non-empty.
 template <int NA>
-__device__ __forceinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
+__device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
                                            Tri<NA>& t, bool& ok) {
     CV<NA> poly[kMaxPoly], tmp[kMaxPoly];
     poly[0] = v[0];
@@ -254,6 +254,31 @@ __device__ int get_piece(const MeshView& m, const float mvp[16], int tri, int pi
         return 1;
     }
     return clip_and_setup<NA>(v, piece, cfg, t, ok);
+}
+
+// Clip-free classification used by the hot kernels: kReject (outside a clip
+// plane), kClip (crosses a plane: handled by k_clip through setup records) or
+// kInside, in which case `t` is set up and `ok` tells whether it covers a
+// pixel centre.
+enum { kReject = 0, kClip = 1, kInside = 2 };
+
+template <int NA>
+__device__ __forceinline__ int classify(const MeshView& m, const float mvp[16], int tri, const RasterCfg& cfg,
+                                        Tri<NA>& t, bool& ok) {
+    CV<NA> v[3];
+    load_tri<NA>(m, mvp, tri, v);
+    bool inside_all = true;
+    ok = false;
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        float d0 = plane_dist(v[0].c, p), d1 = plane_dist(v[1].c, p), d2 = plane_dist(v[2].c, p);
+        int nin = (d0 >= 0.0f) + (d1 >= 0.0f) + (d2 >= 0.0f);
+        if (nin == 0) return kReject;
+        if (nin != 3) inside_all = false;
+    }
+    if (!inside_all) return kClip;
+    ok = setup_piece<NA>(v[0], v[1], v[2], cfg, t);
+    return kInside;
 }
 
 // Edge functions at the centre of pixel (i, j); false when not covered.
@@ -354,6 +379,7 @@ __device__ __forceinline__ bool rect_overlap_s(const Setup& s, int x0, int x1, i
 constexpr int kWideSpan = 4;            // pieces overlapping more tiles are "wide"
 constexpr uint32_t kWideBit = 0x80000000u;
 constexpr int kWideBlocks = 64;         // workgroups per map binning wide pieces
+constexpr int kClipBlocks = 4;          // workgroups per map clipping listed triangles
 
 struct BinBufs {
     uint32_t* counts;    // nmaps x ntiles (scatter cursors after k_scan)
@@ -362,6 +388,8 @@ struct BinBufs {
     int* wide_over;      // nmaps
     uint32_t* nwide;     // nmaps
     Setup* wide;         // nmaps x wcap
+    uint32_t* nclip;     // nmaps
+    uint32_t* clip;      // nmaps x wcap: triangles crossing a clip plane
     uint32_t* entries;   // nmaps x cap
     uint32_t cap, wcap;
 };
@@ -377,43 +405,122 @@ __device__ __forceinline__ void bin_tile(const BinBufs& b, int map, int ntiles, 
     }
 }
 
-// One thread per (triangle, map).  Pieces spanning at most kWideSpan tiles are
-// binned here; wider pieces get a setup record for k_bin_wide (count pass).
+// Appends a setup record (count pass only); records past the capacity flag
+// the map for the fallback raster.
+__device__ __forceinline__ void push_record(const BinBufs& b, int map, const Tri<0>& t, uint32_t entry) {
+    uint32_t w = atomicAdd(&b.nwide[map], 1u);
+    if (w < b.wcap) b.wide[(size_t)map * b.wcap + w] = to_setup(t, entry);
+    else b.wide_over[map] = 1;
+}
+
+// One thread per (triangle, map).  Triangles inside every clip plane whose
+// piece spans at most kWideSpan tiles are binned here; wider ones get a setup
+// record (binned by k_bin_wide), and triangles crossing a clip plane are
+// listed for k_clip.
+//
+// Tile counts are first aggregated in an LDS histogram, so the workgroup
+// issues one device-scope atomic per touched tile instead of one per
+// triangle: consecutive triangles of a mesh are neighbours and share tiles,
+// and device-scope atomics on the same line serialize outside the XCD's L2.
+constexpr int kBinHist = 4096;  // largest tile grid aggregated in LDS (64 x 64 tiles)
+
 template <bool kScatter>
 __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
-    int tri = blockIdx.x * blockDim.x + threadIdx.x;
-    int map = blockIdx.y;
-    if (tri >= mesh.nt) return;
-    int ntiles = cfg.tiles_x * cfg.tiles_y;
-    if (kScatter && b.overflow[map]) return;
+    __shared__ uint32_t hist[kBinHist];
+    const int tid = threadIdx.x;
+    const int tri = blockIdx.x * blockDim.x + tid;
+    const int map = blockIdx.y;
+    const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    const bool lds = ntiles <= kBinHist;
+    if (kScatter && b.overflow[map]) return;  // uniform per workgroup
+    if (lds)
+        for (int p = tid; p < ntiles; p += blockDim.x) hist[p] = 0;
     float mvp[16];
     mvps.load(map, mvp);
-    int npieces = 1;
-    for (int piece = 0; piece < npieces; ++piece) {
+    // Up to kWideSpan tiles per thread, kept in registers (static indices).
+    int tile_of[kWideSpan];
+    uint32_t rank_of[kWideSpan];
+#pragma unroll
+    for (int k = 0; k < kWideSpan; ++k) tile_of[k] = -1;
+    const uint32_t entry = (uint32_t)tri * 8u;
+    if (tri < mesh.nt) {
         Tri<0> t;
         bool ok;
-        npieces = get_piece<0>(mesh, mvp, tri, piece, cfg, t, ok);
-        if (!ok) continue;
-        uint32_t entry = (uint32_t)tri * 8u + (uint32_t)piece;
-        int tx0 = t.px0 / kTile, tx1 = t.px1 / kTile, ty0 = t.py0 / kTile, ty1 = t.py1 / kTile;
-        int span = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-        if (span > kWideSpan) {
+        int cls = classify<0>(mesh, mvp, tri, cfg, t, ok);
+        if (cls == kClip) {
             if (!kScatter) {
-                uint32_t w = atomicAdd(&b.nwide[map], 1u);
-                if (w < b.wcap) b.wide[(size_t)map * b.wcap + w] = to_setup(t, entry);
+                uint32_t c = atomicAdd(&b.nclip[map], 1u);
+                if (c < b.wcap) b.clip[(size_t)map * b.wcap + c] = (uint32_t)tri;
                 else b.wide_over[map] = 1;
             }
-            continue;
-        }
-        for (int ty = ty0; ty <= ty1; ++ty)
-            for (int tx = tx0; tx <= tx1; ++tx) {
-                if (span > 1) {
-                    int x0 = max(tx * kTile, t.px0), x1 = min(tx * kTile + kTile - 1, t.px1);
-                    int y0 = max(ty * kTile, t.py0), y1 = min(ty * kTile + kTile - 1, t.py1);
-                    if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
+        } else if (ok) {
+            int tx0 = t.px0 / kTile, tx1 = t.px1 / kTile, ty0 = t.py0 / kTile, ty1 = t.py1 / kTile;
+            int ntx = tx1 - tx0 + 1;
+            int span = ntx * (ty1 - ty0 + 1);
+            if (span > kWideSpan) {
+                if (!kScatter) push_record(b, map, t, entry);
+            } else {
+#pragma unroll
+                for (int k = 0; k < kWideSpan; ++k) {
+                    if (k >= span) continue;
+                    int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
+                    if (span > 1) {
+                        int x0 = max(tx * kTile, t.px0), x1 = min(tx * kTile + kTile - 1, t.px1);
+                        int y0 = max(ty * kTile, t.py0), y1 = min(ty * kTile + kTile - 1, t.py1);
+                        if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
+                    }
+                    tile_of[k] = ty * cfg.tiles_x + tx;
                 }
-                bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, entry, kScatter);
             }
+        }
+    }
+    if (!lds) {
+#pragma unroll
+        for (int k = 0; k < kWideSpan; ++k)
+            if (tile_of[k] >= 0) bin_tile(b, map, ntiles, tile_of[k], entry, kScatter);
+        return;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kWideSpan; ++k)
+        if (tile_of[k] >= 0) rank_of[k] = atomicAdd(&hist[tile_of[k]], 1u);
+    __syncthreads();
+    uint32_t* cnt = b.counts + (size_t)map * ntiles;
+    for (int p = tid; p < ntiles; p += blockDim.x) {
+        uint32_t c = hist[p];
+        if (c == 0) continue;
+        if (!kScatter) atomicAdd(&cnt[p], c);
+        else hist[p] = atomicAdd(&cnt[p], c);  // this workgroup's base within the tile's list
+    }
+    if (!kScatter) return;
+    __syncthreads();
+    const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
+#pragma unroll
+    for (int k = 0; k < kWideSpan; ++k) {
+        if (tile_of[k] < 0) continue;
+        uint32_t pos = off[tile_of[k]] + hist[tile_of[k]] + rank_of[k];
+        if (pos < b.cap) b.entries[(size_t)map * b.cap + pos] = entry;
+    }
+}
+
+// Clips the listed triangles; every non-empty fan piece becomes a setup
+// record.  Runs between the count passes, before k_bin_wide.
+__global__ __launch_bounds__(256) void k_clip(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
+    const int map = blockIdx.y;
+    const uint32_t n = min(b.nclip[map], b.wcap);
+    float mvp[16];
+    mvps.load(map, mvp);
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        int tri = (int)b.clip[(size_t)map * b.wcap + c];
+        CV<0> v[3];
+        load_tri<0>(mesh, mvp, tri, v);
+        int npieces = 1;
+        for (int piece = 0; piece < npieces; ++piece) {
+            Tri<0> t;
+            bool ok;
+            npieces = clip_and_setup<0>(v, piece, cfg, t, ok);
+            if (ok) push_record(b, map, t, (uint32_t)tri * 8u + (uint32_t)piece);
+        }
     }
 }
 
@@ -562,16 +669,14 @@ __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0
     }
 }
 
-template <bool kVis>
-__global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
-                                                           void* out) {
-    using K = typename KeyOf<kVis>::T;
-    __shared__ K tile[kTilePx];
-    __shared__ Setup rec[kRasterThreads];
-    __shared__ int nrec;
-    const int map = blockIdx.y;
+// One (tile, map) of the raster.  kFallback: the map's lists overflowed the
+// workspace, so every (triangle, piece) pair is visited with full clipping.
+template <bool kVis, bool kFallback>
+__device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
+                                            const BinBufs& b, void* out, int map, int tile_id,
+                                            typename KeyOf<kVis>::T* tile, Setup* rec, int& nrec) {
     const int tid = threadIdx.x;
-    const int tx = blockIdx.x % cfg.tiles_x, ty = blockIdx.x / cfg.tiles_x;
+    const int tx = tile_id % cfg.tiles_x, ty = tile_id / cfg.tiles_x;
     const int x0 = tx * kTile, y0 = ty * kTile;
     const int x1 = min(x0 + kTile - 1, cfg.w - 1), y1 = min(y0 + kTile - 1, cfg.h - 1);
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
@@ -579,12 +684,10 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
     if (tid == 0) nrec = 0;
     float mvp[16];
     mvps.load(map, mvp);
-    // Overflowed maps scan every (triangle, piece) pair instead of a list.
-    const bool fallback = b.overflow[map] != 0;
     uint32_t begin = 0, end;
-    if (!fallback) {
-        begin = b.offsets[(size_t)map * (ntiles + 1) + blockIdx.x];
-        end = b.offsets[(size_t)map * (ntiles + 1) + blockIdx.x + 1];
+    if constexpr (!kFallback) {
+        begin = b.offsets[(size_t)map * (ntiles + 1) + tile_id];
+        end = b.offsets[(size_t)map * (ntiles + 1) + tile_id + 1];
     } else {
         end = (uint32_t)mesh.nt * 8u;
     }
@@ -592,14 +695,17 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
     for (uint32_t base = begin; base < end; base += kRasterThreads) {
         uint32_t k = base + tid;
         if (k < end) {
-            uint32_t entry = fallback ? k : b.entries[(size_t)map * b.cap + k];
-            if (entry & kWideBit) {
+            uint32_t entry = kFallback ? k : b.entries[(size_t)map * b.cap + k];
+            if (!kFallback && (entry & kWideBit)) {
                 int slot = atomicAdd(&nrec, 1);
                 rec[slot] = b.wide[(size_t)map * b.wcap + (entry & ~kWideBit)];
             } else {
                 Tri<0> t;
                 bool ok;
-                get_piece<0>(mesh, mvp, (int)(entry >> 3), (int)(entry & 7u), cfg, t, ok);
+                if constexpr (kFallback)
+                    get_piece<0>(mesh, mvp, (int)(entry >> 3), (int)(entry & 7u), cfg, t, ok);
+                else
+                    classify<0>(mesh, mvp, (int)(entry >> 3), cfg, t, ok);
                 if (ok) {
                     int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
                     if (rx0 <= rx1 && ry0 <= ry1) {
@@ -657,6 +763,33 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
             if (i > x1 || j > y1) continue;
             dst[(size_t)j * cfg.w + i] = tile[p];
         }
+    }
+}
+
+// Binned raster: one workgroup per (tile, map); overflowed maps are left to
+// k_raster_fallback.
+template <bool kVis>
+__global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
+                                                           void* out) {
+    __shared__ typename KeyOf<kVis>::T tile[kTilePx];
+    __shared__ Setup rec[kRasterThreads];
+    __shared__ int nrec;
+    const int map = blockIdx.y;
+    if (b.overflow[map]) return;
+    raster_tile<kVis, false>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
+}
+
+// One workgroup per tile, looping over the maps whose lists overflowed.
+template <bool kVis>
+__global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mesh, MvpSource mvps, RasterCfg cfg,
+                                                                    BinBufs b, void* out, int nmaps) {
+    __shared__ typename KeyOf<kVis>::T tile[kTilePx];
+    __shared__ Setup rec[kRasterThreads];
+    __shared__ int nrec;
+    for (int map = 0; map < nmaps; ++map) {
+        if (!b.overflow[map]) continue;
+        raster_tile<kVis, true>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
+        __syncthreads();
     }
 }
 
@@ -738,10 +871,12 @@ size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char*
     size_t o_counts = off; off = align256(off + sizeof(uint32_t) * ntiles * nmaps);
     size_t o_nwide = off; off = align256(off + sizeof(uint32_t) * nmaps);
     size_t o_wover = off; off = align256(off + sizeof(int) * nmaps);
+    size_t o_nclip = off; off = align256(off + sizeof(uint32_t) * nmaps);
     size_t zero_end = off;  // [0, zero_end) is cleared before binning
     size_t o_offsets = off; off = align256(off + sizeof(uint32_t) * (ntiles + 1) * nmaps);
     size_t o_over = off; off = align256(off + sizeof(int) * nmaps);
     size_t o_wide = off; off = align256(off + sizeof(Setup) * (size_t)wcap * nmaps);
+    size_t o_clip = off; off = align256(off + sizeof(uint32_t) * (size_t)wcap * nmaps);
     size_t o_entries = off; off = align256(off + sizeof(uint32_t) * (size_t)cap * nmaps);
     size_t o_vis = off;
     if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
@@ -753,6 +888,8 @@ size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char*
         b.offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
         b.overflow = reinterpret_cast<int*>(base + o_over);
         b.wide = reinterpret_cast<Setup*>(base + o_wide);
+        b.nclip = reinterpret_cast<uint32_t*>(base + o_nclip);
+        b.clip = reinterpret_cast<uint32_t*>(base + o_clip);
         b.entries = reinterpret_cast<uint32_t*>(base + o_entries);
         b.cap = cap;
         b.wcap = wcap;
@@ -769,6 +906,7 @@ size_t zero_words(int nmaps, const RasterCfg& cfg) {
     size_t off = align256(sizeof(uint32_t) * ntiles * nmaps);
     off = align256(off + sizeof(uint32_t) * nmaps);
     off = align256(off + sizeof(int) * nmaps);
+    off = align256(off + sizeof(uint32_t) * nmaps);
     return off / sizeof(uint32_t);
 }
 
@@ -787,6 +925,7 @@ int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Ras
     dim3 gw(kWideBlocks, nmaps);
     if (mesh.nt > 0) {
         hipLaunchKernelGGL(k_bin<false>, gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        hipLaunchKernelGGL(k_clip, dim3(kClipBlocks, nmaps), dim3(256), 0, s, mesh, mvps, cfg, b);
         hipLaunchKernelGGL(k_bin_wide<false>, gw, dim3(256), 0, s, cfg, b);
     }
     hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, b, ntiles);
@@ -795,6 +934,8 @@ int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Ras
         hipLaunchKernelGGL(k_bin_wide<true>, gw, dim3(256), 0, s, cfg, b);
     }
     hipLaunchKernelGGL(k_raster<kVis>, dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out);
+    hipLaunchKernelGGL(k_raster_fallback<kVis>, dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out,
+                       nmaps);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

@@ -91,6 +91,22 @@ def test_depth_pass_overflow_fallback(gi, oracle_lib, cuda):
     assert bits_equal(got, want), mismatch_report(got, want, "overflow fallback")
 
 
+def test_depth_pass_wide_record_overflow(gi, oracle_lib, cuda):
+    # 3000 mid-sized triangles, each wider than the binning's per-thread span:
+    # more setup records than the workspace holds, so the map falls back.
+    rng = np.random.default_rng(11)
+    n = 3000
+    cen = rng.uniform(-6, 6, size=(n, 1, 3))
+    pos = (cen + rng.uniform(-3, 3, size=(n, 3, 3))).astype(np.float32)
+    pos[:, :, 1] = rng.uniform(0.0, 3.0, size=(n, 3))
+    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
+    mesh = gi.scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    light = gi.scenes.View((0.0, 40.0, 0.1), (0.0, 0.0, 0.0), 512, 512, fov=45.0)
+    mvps = np.stack([gi.passes.view_matrices(light)[2]])
+    got, want = _depth_both(gi, oracle_lib, mesh, mvps, 512, 512)
+    assert bits_equal(got, want), mismatch_report(got, want, "wide record overflow")
+
+
 def test_depth_pass_deterministic(gi, cuda):
     wl = gi.scenes.config3(nsamples=4, sm=2048)
     dm = gi.DeviceMesh.from_mesh(wl.mesh)
@@ -172,15 +188,32 @@ def _soft_case(gi, oracle_lib, wl, sm, params):
     return got, want
 
 
-@pytest.mark.parametrize("pcf,inclusive", [(False, True), (True, True), (True, False)])
-def test_soft_shadow_area_light(gi, oracle_lib, cuda, pcf, inclusive):
+@pytest.mark.parametrize("pcf,inclusive,order,penumbra", [
+    (False, True, 3, 1.0),   # one tap per sample
+    (True, True, 3, 1.0),    # the reference loop: 4x4 taps
+    (True, False, 3, 1.0),   # 3x3 taps
+    (True, False, 3, 2.0),   # 3 taps spanning more than 2 texels: per-tap path
+    (True, True, 7, 1.5),    # 8 taps per axis: generic loop
+])
+def test_soft_shadow_area_light(gi, oracle_lib, cuda, pcf, inclusive, order, penumbra):
     wl = gi.scenes.config3(nsamples=9, view_w=480, view_h=270, sm=1024, freq=40)
-    params = gi.SoftParams(pcf=pcf, inclusive=inclusive)
+    params = gi.SoftParams(pcf=pcf, inclusive=inclusive, kernelOrder=order, penumbra=penumbra)
     got, want = _soft_case(gi, oracle_lib, wl, 1024, params)
     assert np.abs(got - want).max() <= TOL, mismatch_report(got, want, "soft")
     assert bits_equal(got, want), mismatch_report(got, want, "soft")
     vis = want[want != 0]
     assert vis.min() >= 0.25 - 1e-7 and vis.max() <= 1.0 + 1e-7
+
+
+def test_soft_shadow_map_border(gi, oracle_lib, cuda):
+    """A light frustum narrower than the scene: taps fall off the map edge and
+    read the border (depth 0, shadowed)."""
+    wl = gi.scenes.config3(nsamples=4, view_w=320, view_h=200, sm=256, freq=20)
+    for v in wl.lights:
+        v.fov = 20.0
+    params = gi.SoftParams(pcf=True, inclusive=False)
+    got, want = _soft_case(gi, oracle_lib, wl, 256, params)
+    assert bits_equal(got, want), mismatch_report(got, want, "soft border")
 
 
 def test_soft_shadow_full_size_properties(gi, cuda):
