@@ -25,9 +25,10 @@
 //   edge clips identically in both triangles), fan triangulation; window
 //   coordinates snapped to 1/256 pixel with round-to-nearest-even; pixel
 //   centres at (i+.5, j+.5); top-left tie rule in GL's y-up window space;
-//   window depth interpolated linearly in screen space in FP64 from the exact
-//   integer edge functions, rounded to FP32, clamped to [0, 1]; optional
-//   polygon offset factor*max|dz/dx|,|dz/dy| + units*2^(e-23).
+//   window depth linear in screen space, evaluated as an FP32 plane about
+//   vertex 0, z = z0 + (A*(Px-X0) + B*(Py-Y0)), whose slopes are derived in
+//   FP64 from the exact integer geometry and rounded once; clamped to [0, 1];
+//   optional polygon offset factor*max|dz/dx|,|dz/dy| + units*2^(e-23).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -45,6 +46,7 @@ constexpr float kGuard = 4.0f;
 constexpr int kMaxPoly = 9;
 constexpr int kSelfRasterMaxPx = 256;  // bbox∩tile area a lone thread rasterizes itself
 constexpr int kScanThreads = 1024;
+constexpr int kRecQueue = 64;  // LDS setup records a workgroup rasterizes together per chunk
 
 struct MeshView {
     const float* __restrict__ pos;  // nverts x 3
@@ -88,7 +90,7 @@ struct Tri {
     int front;
     int px0, px1, py0, py1;
     int tl[3];
-    double z0, dz1, dz2, inv_area, offset;
+    float z0, A, B, zoff;  // depth plane about vertex 0, per 1/256 pixel
 };
 
 __device__ __forceinline__ long long llmin(long long a, long long b) { return a < b ? a : b; }
@@ -182,15 +184,17 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
         long long dy = t.Y[b] - t.Y[a], dx = t.X[b] - t.X[a];
         t.tl[i] = (dy < 0 || (dy == 0 && dx < 0)) ? 1 : 0;
     }
-    t.z0 = (double)z[0];
-    t.dz1 = (double)z[1] - (double)z[0];
-    t.dz2 = (double)z[2] - (double)z[0];
-    t.inv_area = 1.0 / (double)area;
-    t.offset = 0.0;
+    const double dz1 = (double)z[1] - (double)z[0];
+    const double dz2 = (double)z[2] - (double)z[0];
+    const double inv_area = 1.0 / (double)area;
+    const double a = ((double)(t.Y[2] - t.Y[0]) * dz1 + (double)(t.Y[0] - t.Y[1]) * dz2) * inv_area;
+    const double b = ((double)(t.X[0] - t.X[2]) * dz1 + (double)(t.X[1] - t.X[0]) * dz2) * inv_area;
+    t.z0 = z[0];
+    t.A = (float)a;
+    t.B = (float)b;
+    t.zoff = 0.0f;
     if (cfg.off_factor != 0.0f || cfg.off_units != 0.0f) {
-        double dzdx = ((double)(t.Y[2] - t.Y[0]) * t.dz1 + (double)(t.Y[0] - t.Y[1]) * t.dz2) * t.inv_area * 256.0;
-        double dzdy = ((double)(t.X[0] - t.X[2]) * t.dz1 + (double)(t.X[1] - t.X[0]) * t.dz2) * t.inv_area * 256.0;
-        double m = fmax(fabs(dzdx), fabs(dzdy));
+        double m = fmax(fabs(a), fabs(b)) * 256.0;
         float zmax = fmaxf(fmaxf(fabsf(z[0]), fabsf(z[1])), fabsf(z[2]));
         double r = 0.0;
         if (zmax > 0.0f) {
@@ -198,7 +202,7 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
             frexpf(zmax, &e);
             r = ldexp(1.0, (e - 1) - 23);
         }
-        t.offset = (double)cfg.off_factor * m + (double)cfg.off_units * r;
+        t.zoff = (float)((double)cfg.off_factor * m + (double)cfg.off_units * r);
     }
     return true;
 }
@@ -291,9 +295,9 @@ __device__ __forceinline__ bool eval_pixel(const Tri<NA>& t, int i, int j, long 
         e[k] = (t.X[b] - t.X[a]) * (Py - t.Y[a]) - (t.Y[b] - t.Y[a]) * (Px - t.X[a]);
     }
     if (e[0] + t.tl[0] <= 0 || e[1] + t.tl[1] <= 0 || e[2] + t.tl[2] <= 0) return false;
-    double zd = t.z0 + ((double)e[1] * t.dz1 + (double)e[2] * t.dz2) * t.inv_area;
-    zd = zd + t.offset;
-    float z = (float)zd;
+    float dx = (float)(Px - t.X[0]), dy = (float)(Py - t.Y[0]);
+    float z = t.z0 + (t.A * dx + t.B * dy);
+    z = z + t.zoff;
     if (!(z > 0.0f)) z = 0.0f;
     if (z > 1.0f) z = 1.0f;
     zout = z;
@@ -327,7 +331,7 @@ struct Setup {
     int px0, px1, py0, py1;    // pixel-centre bounding box (clamped)
     int tl;                    // bit k: top-left tie bit of the edge opposite vertex k
     uint32_t id;               // tri * 8 + fan piece
-    double z0, dz1, dz2, inv_area, offset;
+    float z0, A, B, zoff;      // depth plane about vertex 0
 };
 
 __device__ __forceinline__ Setup to_setup(const Tri<0>& t, uint32_t id) {
@@ -340,7 +344,7 @@ __device__ __forceinline__ Setup to_setup(const Tri<0>& t, uint32_t id) {
     s.px0 = t.px0; s.px1 = t.px1; s.py0 = t.py0; s.py1 = t.py1;
     s.tl = t.tl[0] | (t.tl[1] << 1) | (t.tl[2] << 2);
     s.id = id;
-    s.z0 = t.z0; s.dz1 = t.dz1; s.dz2 = t.dz2; s.inv_area = t.inv_area; s.offset = t.offset;
+    s.z0 = t.z0; s.A = t.A; s.B = t.B; s.zoff = t.zoff;
     return s;
 }
 
@@ -351,9 +355,9 @@ __device__ __forceinline__ bool eval_setup(const Setup& s, int i, int j, float& 
     long long e1 = (long long)(s.X[0] - s.X[2]) * (Py - s.Y[2]) - (long long)(s.Y[0] - s.Y[2]) * (Px - s.X[2]);
     long long e2 = (long long)(s.X[1] - s.X[0]) * (Py - s.Y[0]) - (long long)(s.Y[1] - s.Y[0]) * (Px - s.X[0]);
     if (e0 + (s.tl & 1) <= 0 || e1 + ((s.tl >> 1) & 1) <= 0 || e2 + ((s.tl >> 2) & 1) <= 0) return false;
-    double zd = s.z0 + ((double)e1 * s.dz1 + (double)e2 * s.dz2) * s.inv_area;
-    zd = zd + s.offset;
-    float z = (float)zd;
+    float dx = (float)(Px - s.X[0]), dy = (float)(Py - s.Y[0]);
+    float z = s.z0 + (s.A * dx + s.B * dy);
+    z = z + s.zoff;
     if (!(z > 0.0f)) z = 0.0f;
     if (z > 1.0f) z = 1.0f;
     zout = z;
@@ -614,38 +618,69 @@ __device__ __forceinline__ void edges_at(const Setup& s, int i, int j, Edges& E)
     }
 }
 
-template <bool kVis>
-__device__ __forceinline__ void plot(const Setup& s, const long long e[3], int slot, typename KeyOf<kVis>::T* tile) {
-    if (e[0] + (s.tl & 1) <= 0 || e[1] + ((s.tl >> 1) & 1) <= 0 || e[2] + ((s.tl >> 2) & 1) <= 0) return;
-    double zd = s.z0 + ((double)e[1] * s.dz1 + (double)e[2] * s.dz2) * s.inv_area;
-    zd = zd + s.offset;
-    float z = (float)zd;
+// Depth of a covered pixel centre at fixed-point offset (dx, dy) from vertex 0.
+__device__ __forceinline__ uint32_t plane_bits(const Setup& s, int dx, int dy) {
+    float z = s.z0 + (s.A * (float)dx + s.B * (float)dy);
+    z = z + s.zoff;
     if (!(z > 0.0f)) z = 0.0f;
     if (z > 1.0f) z = 1.0f;
+    return fbits(z);
+}
+
+template <bool kVis>
+__device__ __forceinline__ void store_min(const Setup& s, uint32_t zb, int slot, typename KeyOf<kVis>::T* tile) {
     if constexpr (kVis) {
-        atomicMin(&tile[slot], ((unsigned long long)fbits(z) << 32) | s.id);
+        atomicMin(&tile[slot], ((unsigned long long)zb << 32) | s.id);
     } else {
-        atomicMin(&tile[slot], fbits(z));
+        atomicMin(&tile[slot], zb);
     }
 }
 
-// One thread covers the whole (small) rectangle, row by row.
+template <bool kVis, typename E>
+__device__ __forceinline__ void plot(const Setup& s, const E e[3], int i, int j, int slot,
+                                     typename KeyOf<kVis>::T* tile) {
+    if (e[0] + (E)(s.tl & 1) <= 0 || e[1] + (E)((s.tl >> 1) & 1) <= 0 || e[2] + (E)((s.tl >> 2) & 1) <= 0) return;
+    store_min<kVis>(s, plane_bits(s, i * 256 + 128 - s.X[0], j * 256 + 128 - s.Y[0]), slot, tile);
+}
+
+// One thread covers the whole (small) rectangle, row by row.  When every
+// edge value inside the triangle's bounding box fits in 32 bits, the exact
+// stepping runs in int32.
+template <bool kVis, typename E>
+__device__ __forceinline__ void raster_rect_self_t(const Setup& s, const Edges& E64, int x0, int y0, int rx0, int rx1,
+                                                   int ry0, int ry1, typename KeyOf<kVis>::T* tile) {
+    E row[3], dx[3], dy[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        row[k] = (E)E64.e[k];
+        dx[k] = (E)E64.dx[k];
+        dy[k] = (E)E64.dy[k];
+    }
+    for (int j = ry0; j <= ry1; ++j) {
+        E e[3] = {row[0], row[1], row[2]};
+        for (int i = rx0; i <= rx1; ++i) {
+            plot<kVis, E>(s, e, i, j, (j - y0) * kTile + (i - x0), tile);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) e[k] += dx[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) row[k] += dy[k];
+    }
+}
+
 template <bool kVis>
 __device__ __forceinline__ void raster_rect_self(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
                                                  typename KeyOf<kVis>::T* tile) {
     Edges E;
     edges_at(s, rx0, ry0, E);
-    long long row[3] = {E.e[0], E.e[1], E.e[2]};
-    for (int j = ry0; j <= ry1; ++j) {
-        long long e[3] = {row[0], row[1], row[2]};
-        for (int i = rx0; i <= rx1; ++i) {
-            plot<kVis>(s, e, (j - y0) * kTile + (i - x0), tile);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) e[k] += E.dx[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) row[k] += E.dy[k];
-    }
+    const long long bw = (long long)max(s.X[0], max(s.X[1], s.X[2])) - min(s.X[0], min(s.X[1], s.X[2]));
+    const long long bh = (long long)max(s.Y[0], max(s.Y[1], s.Y[2])) - min(s.Y[0], min(s.Y[1], s.Y[2]));
+    // |e| <= 2*bw*bh inside the box and one step past it adds at most
+    // 256*max(bw, bh), so with these bounds every value stays below 2^30.
+    if (bw * bh < (1ll << 28) && bw < (1 << 20) && bh < (1 << 20))
+        raster_rect_self_t<kVis, int>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
+    else
+        raster_rect_self_t<kVis, long long>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
 }
 
 // The workgroup covers the rectangle: thread t owns tile column t % 32 and
@@ -663,7 +698,7 @@ __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0
     edges_at(s, i, j, E);
     long long e[3] = {E.e[0], E.e[1], E.e[2]};
     for (; j <= ry1; j += kRowStep) {
-        plot<kVis>(s, e, (j - y0) * kTile + (i - x0), tile);
+        plot<kVis, long long>(s, e, i, j, (j - y0) * kTile + (i - x0), tile);
 #pragma unroll
         for (int k = 0; k < 3; ++k) e[k] += E.dy[k] * kRowStep;
     }
@@ -697,8 +732,14 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
         if (k < end) {
             uint32_t entry = kFallback ? k : b.entries[(size_t)map * b.cap + k];
             if (!kFallback && (entry & kWideBit)) {
+                const Setup& w = b.wide[(size_t)map * b.wcap + (entry & ~kWideBit)];
                 int slot = atomicAdd(&nrec, 1);
-                rec[slot] = b.wide[(size_t)map * b.wcap + (entry & ~kWideBit)];
+                if (slot < kRecQueue) {
+                    rec[slot] = w;
+                } else {
+                    int rx0 = max(x0, w.px0), rx1 = min(x1, w.px1), ry0 = max(y0, w.py0), ry1 = min(y1, w.py1);
+                    if (rx0 <= rx1 && ry0 <= ry1) raster_rect_self<kVis>(w, x0, y0, rx0, rx1, ry0, ry1, tile);
+                }
             } else {
                 Tri<0> t;
                 bool ok;
@@ -710,20 +751,18 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
                     int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
                     if (rx0 <= rx1 && ry0 <= ry1) {
                         Setup s = to_setup(t, entry);
-                        if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) <= kSelfRasterMaxPx) {
-                            raster_rect_self<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
-                        } else {
-                            int slot = atomicAdd(&nrec, 1);
-                            rec[slot] = s;
-                        }
+                        int slot = kRecQueue;
+                        if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kSelfRasterMaxPx) slot = atomicAdd(&nrec, 1);
+                        if (slot < kRecQueue) rec[slot] = s;
+                        else raster_rect_self<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
                     }
                 }
             }
         }
         __syncthreads();
-        const int nq = nrec;
+        const int nq = min(nrec, kRecQueue);
         for (int q = 0; q < nq; ++q) {
-            const Setup& s = rec[q];
+            const Setup s = rec[q];
             int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
             if (rx0 > rx1 || ry0 > ry1) continue;
             raster_rect_group<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, tile);
@@ -772,7 +811,7 @@ template <bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
                                                            void* out) {
     __shared__ typename KeyOf<kVis>::T tile[kTilePx];
-    __shared__ Setup rec[kRasterThreads];
+    __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     const int map = blockIdx.y;
     if (b.overflow[map]) return;
@@ -784,7 +823,7 @@ template <bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mesh, MvpSource mvps, RasterCfg cfg,
                                                                     BinBufs b, void* out, int nmaps) {
     __shared__ typename KeyOf<kVis>::T tile[kTilePx];
-    __shared__ Setup rec[kRasterThreads];
+    __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     for (int map = 0; map < nmaps; ++map) {
         if (!b.overflow[map]) continue;

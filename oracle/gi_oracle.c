@@ -19,9 +19,12 @@
  *   - window coordinates snapped to 1/256 pixel (rintf);
  *   - pixel centres at (i+0.5, j+0.5); top-left tie rule in GL's y-up window
  *     space; no culling (ShadowMapping/src/main.cpp:271);
- *   - window depth interpolated linearly in screen space in FP64 from the
- *     integer edge functions, then rounded to FP32 and clamped to [0,1];
- *   - polygon offset o = factor*max|dz/dx|,|dz/dy| + units*2^(e-23).
+ *   - window depth interpolated linearly in screen space as an FP32 plane
+ *     about vertex 0: z = z0 + (A*(Px-X0) + B*(Py-Y0)), with the slopes A, B
+ *     derived in FP64 from the exact integer geometry and rounded once to
+ *     FP32; then clamped to [0,1];
+ *   - polygon offset o = factor*max|dz/dx|,|dz/dy| + units*2^(e-23), rounded
+ *     to FP32 and added to z before the clamp.
  *
  * This file is test infrastructure: nothing under globalillumination_amd/
  * links or calls it.
@@ -686,7 +689,7 @@ typedef struct tri2d {
     int front;
     int px0, px1, py0, py1;
     int tl[3];
-    double z0, dz1, dz2, inv_area, offset;
+    float z0, A, B, zoff;  /* depth plane about vertex 0, per 1/256 pixel */
 } tri2d;
 
 /* Builds the screen-space triangle of fan piece (0, k+1, k+2). 0 = empty. */
@@ -741,19 +744,21 @@ static int setup_piece(const cvert* poly, int k, int nattr, int W, int H,
         long long dy = t->Y[b] - t->Y[a], dx = t->X[b] - t->X[a];
         t->tl[i] = (dy < 0 || (dy == 0 && dx < 0)) ? 1 : 0;
     }
-    t->z0 = (double)t->z[0];
-    t->dz1 = (double)t->z[1] - (double)t->z[0];
-    t->dz2 = (double)t->z[2] - (double)t->z[0];
-    t->inv_area = 1.0 / (double)area;
-    t->offset = 0.0;
+    double dz1 = (double)t->z[1] - (double)t->z[0];
+    double dz2 = (double)t->z[2] - (double)t->z[0];
+    double inv_area = 1.0 / (double)area;
+    double a = ((double)(t->Y[2] - t->Y[0]) * dz1 + (double)(t->Y[0] - t->Y[1]) * dz2) * inv_area;
+    double b = ((double)(t->X[0] - t->X[2]) * dz1 + (double)(t->X[1] - t->X[0]) * dz2) * inv_area;
+    t->z0 = t->z[0];
+    t->A = (float)a;
+    t->B = (float)b;
+    t->zoff = 0.0f;
     if (off_factor != 0.0f || off_units != 0.0f) {
-        double dzdx = ((double)(t->Y[2] - t->Y[0]) * t->dz1 + (double)(t->Y[0] - t->Y[1]) * t->dz2) * t->inv_area * 256.0;
-        double dzdy = ((double)(t->X[0] - t->X[2]) * t->dz1 + (double)(t->X[1] - t->X[0]) * t->dz2) * t->inv_area * 256.0;
-        double m = fmax(fabs(dzdx), fabs(dzdy));
+        double m = fmax(fabs(a), fabs(b)) * 256.0;
         float zmax = fmaxf(fmaxf(fabsf(t->z[0]), fabsf(t->z[1])), fabsf(t->z[2]));
         double r = 0.0;
         if (zmax > 0.0f) { int e; frexpf(zmax, &e); r = ldexp(1.0, (e - 1) - 23); }
-        t->offset = (double)off_factor * m + (double)off_units * r;
+        t->zoff = (float)((double)off_factor * m + (double)off_units * r);
     }
     return 1;
 }
@@ -767,9 +772,9 @@ static int eval_pixel(const tri2d* t, int i, int j, long long e[3], float* zout)
         e[k] = (t->X[b] - t->X[a]) * (Py - t->Y[a]) - (t->Y[b] - t->Y[a]) * (Px - t->X[a]);
         if (e[k] + t->tl[k] <= 0) return 0;
     }
-    double zd = t->z0 + ((double)e[1] * t->dz1 + (double)e[2] * t->dz2) * t->inv_area;
-    zd = zd + t->offset;
-    float z = (float)zd;
+    float dx = (float)(Px - t->X[0]), dy = (float)(Py - t->Y[0]);
+    float z = t->z0 + (t->A * dx + t->B * dy);
+    z = z + t->zoff;
     if (!(z > 0.0f)) z = 0.0f;
     if (z > 1.0f) z = 1.0f;
     *zout = z;
