@@ -492,7 +492,9 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
     if (!finite_all(p->light_mvp, 16) || !std::isfinite(p->shadow_intensity) || !std::isfinite(p->depth_threshold))
         return GI_ENONFINITE;
     bool uses_loop = p->technique == GI_PCF || p->technique == GI_RPCF || p->technique == GI_RBSM_FILTERED;
-    if (uses_loop && (p->kernel_order <= 0 || p->penumbra_size < 0 || p->kernel_order > 64 ||
+    // The PCF loops step by 2*penumbra/kernelOrder: a zero penumbra would never
+    // advance, so it is rejected along with orders past the tap budget.
+    if (uses_loop && (p->kernel_order <= 0 || p->penumbra_size <= 0 || p->kernel_order > 64 ||
                       p->penumbra_size > 64))
         return GI_EINVAL;
     if (p->technique >= GI_RBSM && (p->max_search < 0 || p->max_search > 4096)) return GI_EINVAL;

@@ -546,6 +546,8 @@ int or_hard_shadow(const float* pos, const float* nrm, int w, int h, const float
 {
     if (!pos || !depth || !p || !out || w <= 0 || h <= 0) return -1;
     if (p->pre_evaluation && !nrm) return -1;
+    if (p->technique != OR_NAIVE && p->technique != OR_RBSM && (p->kernel_order <= 0 || p->penumbra_size <= 0))
+        return -1; /* the PCF loop would not advance */
     size_t plane = (size_t)w * (size_t)h;
     long long n = (long long)plane;
 #ifdef _OPENMP
@@ -570,6 +572,7 @@ int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsampl
                    int inclusive, float* out)
 {
     if (!pos || !maps || !trans || !out || w <= 0 || h <= 0 || nsamples <= 0) return -1;
+    if (pcf_mode && (kernel_order <= 0 || !(penumbra > 0.0f))) return -1; /* loop would not advance */
     size_t plane = (size_t)w * (size_t)h;
     size_t mapsz = (size_t)smw * (size_t)smh;
     long long n = (long long)plane;

@@ -1,0 +1,273 @@
+"""Pins of the oracle's screen passes (a8-a14) against closed forms,
+invariants and tiny brute-force cases.
+
+Shadow coordinates are driven directly: with the identity as B*lightMVP, a
+G-buffer position (x, y, z) projects to texture coordinate (x, y) and depth z,
+so every expected value below is computed by hand from the map contents.
+"""
+import math
+
+import numpy as np
+import pytest
+
+SI = 0.25
+W = H = 16
+
+
+def run_hard(oracle_lib, depth, pts, technique="naive", mvp=None, **kw):
+    pts = np.asarray(pts, dtype=np.float32)
+    n = len(pts)
+    pos = np.zeros((3, 1, n), np.float32)
+    pos[:, 0, :] = pts.T
+    nrm = kw.pop("normals", None)
+    if nrm is not None:
+        nrm = np.asarray(nrm, np.float32).T.reshape(3, 1, n).copy()
+    m = np.eye(4, dtype=np.float32).reshape(-1) if mvp is None else mvp
+    h, w = depth.shape
+    params = oracle_lib.make_params(m, w, h, technique, shadow_intensity=kw.pop("si", SI), **kw)
+    return oracle_lib.hard_shadow(pos, nrm, depth, params)[0]
+
+
+def test_fetch_nearest_and_border(oracle_lib):
+    d = np.arange(W * H, dtype=np.float32).reshape(H, W) + 1.0
+    for i, j in [(0, 0), (3, 7), (15, 15)]:
+        for fx, fy in [(0.0, 0.0), (0.5, 0.5), (0.999, 0.001)]:
+            assert oracle_lib.fetch(d, (i + fx) / W, (j + fy) / H) == d[j, i]
+    # CLAMP_TO_BORDER with border depth 0: outside [0, 1) reads 0
+    for s, t in [(-1e-6, 0.5), (1.0, 0.5), (0.5, 1.0), (0.5, -0.25), (float("nan"), 0.5)]:
+        assert oracle_lib.fetch(d, s, t) == 0.0
+
+
+def test_naive_closed_form(oracle_lib):
+    rng = np.random.default_rng(0)
+    d = rng.uniform(0.2, 0.8, size=(H, W)).astype(np.float32)
+    pts = rng.uniform(0.01, 0.99, size=(200, 3)).astype(np.float32)
+    got = run_hard(oracle_lib, d, pts)
+    i = np.floor(pts[:, 0] * W).astype(int)
+    j = np.floor(pts[:, 1] * H).astype(int)
+    want = np.where(pts[:, 2] <= d[j, i], 1.0, SI)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_naive_special_cases(oracle_lib):
+    d = np.full((H, W), 0.5, np.float32)
+    # x == 0 is background (Shadow.frag:244): the pass writes 0
+    assert run_hard(oracle_lib, d, [(0.0, 0.5, 0.1)])[0] == 0.0
+    # outside the map: border depth 0 -> shadowed
+    assert run_hard(oracle_lib, d, [(1.25, 0.5, 0.1)])[0] == SI
+    # w <= 0 (behind the light): lit without a test (Shadow.frag:251)
+    m = np.eye(4, dtype=np.float32)
+    m[3, 3] = -1.0  # column-major index 15: w = -1
+    assert run_hard(oracle_lib, d, [(0.5, 0.5, 0.9)], mvp=m.reshape(-1))[0] == 1.0
+    # depth equal to the stored value is lit (<=)
+    assert run_hard(oracle_lib, d, [(0.5, 0.5, 0.5)])[0] == 1.0
+
+
+def pcf_taps(k, offset, inclusive):
+    """Tap offsets of the PCF loop in float32 (Shadow.frag:93-99)."""
+    off = np.float32(offset)
+    step = np.float32(np.float32(2.0) * off / np.float32(k))
+    w, taps = -off, []
+    while (w <= off) if inclusive else (w < off):
+        taps.append(float(w))
+        w = np.float32(w + step)
+    return taps
+
+
+def test_pcf_tap_tables():
+    assert len(pcf_taps(3, 1, False)) == 3 and len(pcf_taps(3, 1, True)) == 4
+    np.testing.assert_allclose(pcf_taps(3, 1, False), [-1, -1 / 3, 1 / 3], atol=1e-6)
+    np.testing.assert_allclose(pcf_taps(3, 1, True), [-1, -1 / 3, 1 / 3, 1], atol=1e-6)
+    assert len(pcf_taps(7, 1, False)) == 7
+
+
+@pytest.mark.parametrize("fx", [0.1, 0.4, 0.6, 0.9])
+@pytest.mark.parametrize("fy", [0.2, 0.7])
+def test_pcf_counts_taps(oracle_lib, fx, fy):
+    """3x3 PCF (k=3, offset 1, '<' loop) over a map whose only lit column is
+    i0 and only lit row is j0: the result is the fraction of the 9 taps at
+    offsets {-1, -1/3, +1/3} texels that land on a lit texel."""
+    i0, j0 = 7, 9
+    d = np.zeros((H, W), np.float32)
+    d[:, i0] = 1.0
+    d[j0, :] = 1.0
+    x, y = (i0 + fx) / W, (j0 + fy) / H
+    got = run_hard(oracle_lib, d, [(x, y, 0.5)], technique="pcf", kernel_order=3, penumbra_size=1)
+    offs = [-1.0, -1.0 / 3.0, 1.0 / 3.0]
+    lit = 0
+    for a in offs:
+        for b in offs:
+            ti = math.floor((i0 + fx) + a)
+            tj = math.floor((j0 + fy) + b)
+            lit += (ti == i0) or (tj == j0)
+    want = (lit * 1.0 + (9 - lit) * SI) / 9.0
+    assert abs(got - want) < 1e-6
+
+
+def test_pcf_uniform_maps(oracle_lib):
+    pts = [(0.3, 0.6, 0.5), (0.71, 0.12, 0.5)]
+    lit = run_hard(oracle_lib, np.ones((H, W), np.float32), pts, technique="pcf")
+    dark = run_hard(oracle_lib, np.zeros((H, W), np.float32), pts, technique="pcf")
+    np.testing.assert_allclose(lit, 1.0)
+    np.testing.assert_allclose(dark, SI)
+
+
+def test_pre_evaluation(oracle_lib):
+    """Shadow.frag:222-238: N = normalize(normalMatrix * n) (pre-flipped for
+    back faces), L = normalize(lightPosition - MV*v); dot(N, L) <= 0 gives si
+    without a shadow-map test."""
+    d = np.ones((H, W), np.float32)  # everything lit by the map
+    pts = [(0.5, 0.5, 0.0)] * 3
+    normals = [(0, 0, 1), (0, 0, -1), (1, 0, 0)]  # facing the light, away, perpendicular
+    got = run_hard(oracle_lib, d, pts, technique="naive", pre_evaluation=True, normals=normals,
+                   light_position=(0.5, 0.5, 10.0))
+    np.testing.assert_array_equal(got, [1.0, SI, SI])
+
+
+def test_soft_shadow_limits(oracle_lib):
+    rng = np.random.default_rng(2)
+    S = 4
+    pos = np.zeros((3, 1, 8), np.float32)
+    pos[0, 0] = rng.uniform(0.2, 0.8, 8)
+    pos[1, 0] = rng.uniform(0.2, 0.8, 8)
+    pos[2, 0] = 0.5
+    m = np.eye(4, dtype=np.float32).reshape(-1)
+    trans = np.tile(np.array([0, 0, 0, 1], np.float32), (S, 1))
+    m_common = m.copy()
+    m_common[12:16] = 0  # columns 0-2 only; column 3 comes from lightMVPTrans
+    lit = oracle_lib.soft_shadow(pos, np.ones((S, H, W), np.float32), m_common, trans, si=SI)
+    dark = oracle_lib.soft_shadow(pos, np.zeros((S, H, W), np.float32), m_common, trans, si=SI, pcf=True)
+    np.testing.assert_allclose(lit, 1.0)
+    np.testing.assert_allclose(dark, SI)
+
+
+def test_soft_shadow_single_sample_equals_naive_and_mean(oracle_lib):
+    """N = 1 reduces AccurateSoftShadow to the naive test; N > 1 is the mean
+    of the per-sample tests (accFactor = 1)."""
+    rng = np.random.default_rng(4)
+    S = 5
+    maps = rng.uniform(0.3, 0.7, size=(S, H, W)).astype(np.float32)
+    pts = rng.uniform(0.05, 0.95, size=(64, 3)).astype(np.float32)
+    pos = np.ascontiguousarray(pts.T.reshape(3, 1, -1))
+    m = np.eye(4, dtype=np.float32).reshape(-1).copy()
+    m[12:16] = 0
+    offsets = rng.uniform(-0.05, 0.05, size=(S, 2)).astype(np.float32)
+    trans = np.zeros((S, 4), np.float32)
+    trans[:, :2] = offsets
+    trans[:, 3] = 1.0
+    per = []
+    for s in range(S):
+        one = oracle_lib.soft_shadow(pos, maps[s:s + 1], m, trans[s:s + 1], si=SI)[0]
+        i = np.floor((pts[:, 0] + offsets[s, 0]) * W).astype(int)
+        j = np.floor((pts[:, 1] + offsets[s, 1]) * H).astype(int)
+        inside = (i >= 0) & (i < W) & (j >= 0) & (j < H)
+        dd = np.where(inside, maps[s][np.clip(j, 0, H - 1), np.clip(i, 0, W - 1)], 0.0)
+        np.testing.assert_array_equal(one, np.where(pts[:, 2] <= dd, 1.0, SI))
+        per.append(one)
+    allS = oracle_lib.soft_shadow(pos, maps, m, trans, si=SI)[0]
+    np.testing.assert_allclose(allS, np.mean(per, axis=0), atol=2e-7)
+
+
+# --------------------------------------------------------------------------
+# RBSM (NonConservativeSMSR.frag / FilteredRBSM.frag)
+# --------------------------------------------------------------------------
+RW = 32
+
+
+def staircase():
+    ii, jj = np.meshgrid(np.arange(RW), np.arange(RW))
+    d = np.ones((RW, RW), np.float32)
+    d[(ii + jj) < 32] = 0.0  # texel (i, j) occluded when i + j <= 31
+    return d
+
+
+def rbsm_points(rng, n, near_line=True):
+    pts = []
+    while len(pts) < n:
+        i = int(rng.integers(4, 28))
+        j = 32 - i + int(rng.integers(-2, 2)) if near_line else int(rng.integers(2, 30))
+        fx, fy = rng.random(2)
+        pts.append((np.float32((i + fx) / RW), np.float32((j + fy) / RW), 0.5))
+    return np.asarray(pts, np.float32)
+
+
+def test_rbsm_revectorizes_45_degree_staircase(oracle_lib):
+    """The 45-degree staircase is revectorized into the line through the step
+    midpoints, x + y = 32.5 (texel units): binary RBSM classifies every point
+    by that line, while the naive test follows the stairs."""
+    rng = np.random.default_rng(0)
+    pts = rbsm_points(rng, 3000)
+    d = staircase()
+    got = run_hard(oracle_lib, d, pts, technique="rbsm", si=0.0, max_search=16, depth_threshold=0.0)
+    naive = run_hard(oracle_lib, d, pts, technique="naive", si=0.0)
+    s = pts[:, 0] * RW + pts[:, 1] * RW
+    far = np.abs(s - 32.5) > 0.05
+    want = (s >= 32.5).astype(np.float32)
+    np.testing.assert_array_equal(got[far], want[far])
+    assert (naive[far] != want[far]).mean() > 0.02  # the plain test differs
+
+
+def test_rbsm_axis_aligned_edge_is_unchanged(oracle_lib):
+    # 64 x 64 map, samples at least maxSearch texels from the map border so
+    # the edge walks never read the border (which would look like a corner)
+    n = 64
+    d = np.ones((n, n), np.float32)
+    d[:, :32] = 0.0
+    pts = np.array([((i + f) / n, (j + 0.5) / n, 0.5) for i in range(26, 38) for f in (0.05, 0.5, 0.95)
+                    for j in (20, 33, 43)], np.float32)
+    rb = run_hard(oracle_lib, d, pts, technique="rbsm", si=0.0, depth_threshold=0.0)
+    nv = run_hard(oracle_lib, d, pts, technique="naive", si=0.0)
+    np.testing.assert_array_equal(rb, nv)
+    # RPCF of binary revectorized taps equals the plain average of the 4x4 taps
+    rp = run_hard(oracle_lib, d, pts, technique="rpcf", si=0.0, kernel_order=3, penumbra_size=1)
+    taps = [-1.0, -1.0 / 3.0, 1.0 / 3.0, 1.0]
+    want = [np.mean([d[int(math.floor(y * n + b)), int(math.floor(x * n + a))] for a in taps for b in taps])
+            for x, y, _ in pts]
+    np.testing.assert_allclose(rp, want, atol=1e-6)
+    # the fractional variant ramps monotonically across the edge
+    fl = run_hard(oracle_lib, d, pts[::3], technique="rbsm_filtered", si=0.0, kernel_order=3, penumbra_size=1)
+    assert (np.diff(fl) >= -1e-7).all() and fl[0] == 0.0 and fl[-1] == 1.0
+
+
+def test_rbsm_no_discontinuity_equals_naive(oracle_lib):
+    rng = np.random.default_rng(1)
+    pts = rng.uniform(0.1, 0.9, size=(100, 3)).astype(np.float32)
+    for val in (0.0, 1.0):
+        d = np.full((RW, RW), val, np.float32)
+        for tech in ("rbsm", "rpcf", "rbsm_filtered"):
+            got = run_hard(oracle_lib, d, pts, technique=tech, kernel_order=3, penumbra_size=1)
+            want = np.where(pts[:, 2] <= val, 1.0, SI)
+            np.testing.assert_allclose(got, want, atol=1e-7)
+
+
+def test_rbsm_binary_transpose_symmetry(oracle_lib):
+    """With depthThreshold = 0 the four edge walks treat x and y alike, so the
+    transposed map sampled at swapped coordinates gives the same result."""
+    rng = np.random.default_rng(2)
+    d = staircase()
+    d[5:9, 20:25] = 0.0
+    d[20:27, 3:6] = 0.0
+    pts = np.array([((rng.integers(0, RW) + (rng.integers(0, 16) + 0.5) / 16) / RW,
+                     (rng.integers(0, RW) + (rng.integers(0, 16) + 0.5) / 16) / RW, 0.5) for _ in range(2000)],
+                   np.float32)
+    a = run_hard(oracle_lib, d, pts, technique="rbsm", depth_threshold=0.0)
+    b = run_hard(oracle_lib, np.ascontiguousarray(d.T), pts[:, [1, 0, 2]], technique="rbsm", depth_threshold=0.0)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_rbsm_values_in_range(oracle_lib):
+    rng = np.random.default_rng(3)
+    d = (rng.random((RW, RW)) > 0.5).astype(np.float32)
+    pts = rng.uniform(0.05, 0.95, size=(500, 3)).astype(np.float32)
+    pts[:, 2] = 0.5
+    for tech in ("rbsm", "rpcf", "rbsm_filtered"):
+        got = run_hard(oracle_lib, d, pts, technique=tech, kernel_order=3, penumbra_size=1)
+        assert got.min() >= SI - 1e-7 and got.max() <= 1.0 + 1e-7
+        if tech == "rbsm":
+            assert set(np.unique(got).tolist()) <= {SI, 1.0}
+
+
+def test_loops_that_cannot_advance_are_rejected(oracle_lib):
+    d = np.ones((4, 4), np.float32)
+    with pytest.raises(RuntimeError):
+        run_hard(oracle_lib, d, [(0.5, 0.5, 0.5)], technique="pcf", penumbra_size=0)
