@@ -32,6 +32,8 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=64, help="light samples in the CPU baseline sample")
     ap.add_argument("--no-footprint", action="store_true", help="skip counting the unique texels gathered")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
 
 
@@ -112,6 +114,16 @@ def main():
     import numpy as np
     import torch
 
+    if args.calibrate_copy:
+        n = 256 << 20
+        x = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 3}))
+        return
+
     import globalillumination_amd as gi
     from globalillumination_amd import passes
 
@@ -137,7 +149,7 @@ def main():
     def frame(ev=None):
         if ev:
             ev[0].record(stream)
-        passes.render_shadow_map(dm, mvps, smw, smw, out=maps)
+        passes.render_shadow_map(dm, mvps, smw, smw, out=maps, raster_events=(ev[3], ev[4]) if ev else None)
         if ev:
             ev[1].record(stream)
         passes.render_monte_carlo(g, maps, mvps, params, out=out, trans_dev=trans_dev)
@@ -147,7 +159,11 @@ def main():
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    # events: frame start, depth done, screen done, raster kernel begin / end
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    for e in evs:
+        for x in e:
+            x.record(stream)  # creates the underlying hipEvent
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -157,28 +173,34 @@ def main():
     ms_step = wall * 1e3 / args.steps
     depth_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     screen_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    raster_ms = float(np.mean([e[3].elapsed_time(e[4]) for e in evs]))
     value = W * H / (ms_step * 1e-3) / 1e6
 
-    # Roofline of the screen pass (single kernel per call): algorithmic bytes =
-    # G-buffer position reads + output writes + unique texels gathered.
+    # Algorithmic bytes per launch.  Screen pass (k_soft): G-buffer position
+    # reads (12 B/px) + the R32F output (4 B/px) + every unique depth texel the
+    # taps gather (4 B each).  Tile raster (k_raster): the maps written once
+    # (4 B/texel) + the triangle vertices read per map (3 x 12 B per triangle).
     taps = tap_offsets(3, 1.0, False)
     texels = 0 if args.no_footprint else touched_texels(gi, torch, g, mvps, smw, smw, taps)
     npx = W * H
     screen_bytes = npx * (12 + 4) + texels * 4
-    depth_bytes = S * smw * smw * 4 + S * (wl.mesh.ntris * 12 + wl.mesh.nverts * 12)
+    raster_bytes = S * smw * smw * 4 + S * wl.mesh.ntris * 36
     screen_gbs = screen_bytes / (screen_ms * 1e-3) / 1e9
-    depth_gbs = depth_bytes / (depth_ms * 1e-3) / 1e9
-    if depth_ms >= screen_ms:
-        dom = {"kernel": "depth pass (gi_depth_pass, 5 launches)", "bound": "hbm", "achieved": depth_gbs,
-               "bytes": depth_bytes, "ms": depth_ms}
+    raster_gbs = raster_bytes / (raster_ms * 1e-3) / 1e9
+    if raster_ms >= screen_ms:
+        dom = {"kernel": "k_raster (tile raster of the depth pass)", "achieved": raster_gbs, "bytes": raster_bytes,
+               "ms": raster_ms}
     else:
-        dom = {"kernel": "k_soft", "bound": "hbm", "achieved": screen_gbs, "bytes": screen_bytes, "ms": screen_ms}
+        dom = {"kernel": "k_soft (soft-shadow screen pass)", "achieved": screen_gbs, "bytes": screen_bytes,
+               "ms": screen_ms}
     roof = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["kernel"],
             "algorithmic_bytes": int(dom["bytes"]), "ms": round(dom["ms"], 4),
-            "screen_pass": {"ms": round(screen_ms, 4), "GB/s": round(screen_gbs, 1), "bytes": int(screen_bytes),
+            "screen_pass": {"kernel": "k_soft", "ms": round(screen_ms, 4), "GB/s": round(screen_gbs, 1),
+                            "frac": round(screen_gbs / HBM_PEAK_GBS, 4), "bytes": int(screen_bytes),
                             "unique_texels": int(texels)},
-            "depth_pass": {"ms": round(depth_ms, 4), "GB/s": round(depth_gbs, 1), "bytes": int(depth_bytes)}}
+            "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
+                           "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)}}
     cpu = None
     if not args.no_cpu_baseline:
         maps_np = None

@@ -32,7 +32,9 @@ F3 = ctypes.c_float * 3
 
 class GiRasterOpts(ctypes.Structure):
     _fields_ = [("polygon_offset_factor", ctypes.c_float),
-                ("polygon_offset_units", ctypes.c_float)]
+                ("polygon_offset_units", ctypes.c_float),
+                ("raster_begin_event", ctypes.c_void_p),
+                ("raster_end_event", ctypes.c_void_p)]
 
 
 class GiGBuffer(ctypes.Structure):

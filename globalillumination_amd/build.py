@@ -79,7 +79,23 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([hipcc, "-shared", f"--offload-arch={ARCH}", "-o", str(tmp), *map(str, objs)], verbose)
     os.replace(tmp, LIB)
+    build_host_test(hipcc, verbose)
     return LIB
+
+
+HOST_TEST = ROOT / "tests" / "cpp" / "host_pipeline_test"
+
+
+def build_host_test(hipcc: str, verbose: bool = False) -> Path:
+    """The C++ host pass interface (host/shadow_passes.cpp) linked into the
+    driver the GPU tests run; it calls libgi.so through the C ABI only."""
+    tmp = HOST_TEST.with_suffix(".tmp")
+    _run([hipcc, "-O2", "-std=c++17", "-ffp-contract=off", f"-I{ROOT / 'include'}",
+          str(ROOT / "tests" / "cpp" / "host_pipeline_test.cpp"), str(PKG / "host" / "shadow_passes.cpp"),
+          f"-L{PKG}", "-lgi", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../globalillumination_amd",
+          "-o", str(tmp)], verbose)
+    os.replace(tmp, HOST_TEST)
+    return HOST_TEST
 
 
 if __name__ == "__main__":

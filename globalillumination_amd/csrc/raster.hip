@@ -953,7 +953,7 @@ bool size_ok(int w, int h) { return w > 0 && h > 0 && w <= GI_MAX_EDGE && h <= G
 
 template <bool kVis>
 int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const RasterCfg& cfg, const Workspace& ws,
-               void* out, hipStream_t s) {
+               void* out, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     const BinBufs& b = ws.bins;
     size_t nz = zero_words(nmaps, cfg);
@@ -972,7 +972,9 @@ int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Ras
         hipLaunchKernelGGL(k_bin<true>, gb, dim3(256), 0, s, mesh, mvps, cfg, b);
         hipLaunchKernelGGL(k_bin_wide<true>, gw, dim3(256), 0, s, cfg, b);
     }
+    if (ev_begin && hipEventRecord(ev_begin, s) != hipSuccess) return GI_EDEVICE;
     hipLaunchKernelGGL(k_raster<kVis>, dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out);
+    if (ev_end && hipEventRecord(ev_end, s) != hipSuccess) return GI_EDEVICE;
     hipLaunchKernelGGL(k_raster_fallback<kVis>, dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out,
                        nmaps);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
@@ -1009,7 +1011,9 @@ int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices
     MvpSource mvps{};
     mvps.dev = mvps_dev;
     RasterCfg cfg{width, height, (width + kTile - 1) / kTile, (height + kTile - 1) / kTile, off_f, off_u};
-    return run_raster<false>(mesh, mvps, nmaps, cfg, ws, depth_out_dev, reinterpret_cast<hipStream_t>(stream));
+    return run_raster<false>(mesh, mvps, nmaps, cfg, ws, depth_out_dev, reinterpret_cast<hipStream_t>(stream),
+                             opts ? reinterpret_cast<hipEvent_t>(opts->raster_begin_event) : nullptr,
+                             opts ? reinterpret_cast<hipEvent_t>(opts->raster_end_event) : nullptr);
 }
 
 size_t gi_gbuffer_pass_workspace_size(int ntris, int width, int height) {

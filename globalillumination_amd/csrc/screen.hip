@@ -351,11 +351,77 @@ __device__ __forceinline__ float pick(const Row<NT>& r, int k) {
     return x;
 }
 
-// AccurateSoftShadow.frag:52-133 (monteCarlo: accFactor = 1).  NT > 0: the
-// PCF loop has NT taps per axis, and when every tap of a sample lies inside
+// One sample's term of AccurateSoftShadow.frag:75-121 for a pixel whose
+// columns-0-2 product with B*lightMVP is `cm` (monteCarlo: accFactor = 1).
+// NT > 0: the PCF loop has NT taps per axis, and when every tap lies inside
 // the map the NT x NT taps are served by NT row loads of NT consecutive
-// texels (taps of one sample span at most NT texels per axis because the
-// tap offsets span less than NT texels).  NT == 0: generic per-tap loop.
+// texels (the taps of one sample span fewer than NT texels per axis, which
+// the guard below checks).  NT == 0: generic per-tap loop.
+template <int NT>
+__device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms& u, const float cm[4], float4 T,
+                                           float incrWidth, float incrHeight) {
+    float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
+    float sx = cx / cw, sy = cy / cw, sz = cz / cw;
+    if (!u.pcf) {
+        float d = D.fetch(sx, sy);
+        return (sz <= d) ? 1.0f : u.si;
+    }
+    if constexpr (NT > 0) {
+        float tx[NT], ty[NT];
+        int iu[NT], iv[NT];
+        bool inside = true;
+#pragma unroll
+        for (int a = 0; a < NT; ++a) {
+            tx[a] = sx + u.taps[a] * incrWidth;
+            ty[a] = sy + u.taps[a] * incrHeight;
+            float uu = tx[a] * D.fw, vv = ty[a] * D.fh;
+            inside = inside && (uu >= 0.0f && uu < D.fw && vv >= 0.0f && vv < D.fh);
+            iu[a] = (int)floorf(uu);
+            iv[a] = (int)floorf(vv);
+        }
+        // one row of NT texels from iu[0] must cover every tap column
+        inside = inside && (iu[NT - 1] - iu[0] <= NT - 1) && (iu[0] + NT - 1 < D.w);
+        float illum = 0.0f;
+        if (inside) {
+            Row<NT> rows[NT];
+#pragma unroll
+            for (int b = 0; b < NT; ++b)
+                __builtin_memcpy(&rows[b], D.texels + (size_t)iv[b] * D.w + iu[0], sizeof(Row<NT>));
+#pragma unroll
+            for (int a = 0; a < NT; ++a) {
+#pragma unroll
+                for (int b = 0; b < NT; ++b) {
+                    float d = pick<NT>(rows[b], iu[a] - iu[0]);
+                    illum = illum + ((sz <= d) ? 1.0f : u.si);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < NT; ++a) {
+#pragma unroll
+                for (int b = 0; b < NT; ++b) {
+                    float d = D.fetch(tx[a], ty[b]);
+                    illum = illum + ((sz <= d) ? 1.0f : u.si);
+                }
+            }
+        }
+        return illum / (float)(NT * NT);
+    } else {
+        float illum = 0.0f;
+        for (int a = 0; a < u.ntaps; ++a) {
+            float txa = sx + u.taps[a] * incrWidth;
+            for (int b = 0; b < u.ntaps; ++b) {
+                float d = D.fetch(txa, sy + u.taps[b] * incrHeight);
+                illum = illum + ((sz <= d) ? 1.0f : u.si);
+            }
+        }
+        return illum / (float)(u.ntaps * u.ntaps);
+    }
+}
+
+// AccurateSoftShadow.frag:52-133: one thread per pixel, samples in order
+// 0..N-1.  Blocks are ordered XCD-aware (pixel_of_block) so each XCD's L2
+// serves one contiguous band of the view.
 template <int NT>
 __global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
                                                             int smh, const float4* __restrict__ trans,
@@ -365,83 +431,29 @@ __global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const
     if (x >= gb.w || y >= gb.h) return;
     const size_t plane = (size_t)gb.w * gb.h;
     const size_t k = (size_t)y * gb.w + x;
-    float vx = gb.pos[k];
-    if (vx == 0.0f) {
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {  // Shadow.frag:244 background test; the cleared target holds 0
         out[k] = 0.0f;
         return;
     }
-    float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
     float cm[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        float acc = u.m[0 * 4 + i] * vx;
-        acc = acc + u.m[1 * 4 + i] * vy;
-        acc = acc + u.m[2 * 4 + i] * vz;
-        cm[i] = acc;
+    for (int i = 0; i < 4; ++i) {  // :70-73, v.w ignored
+        float a = u.m[0 * 4 + i] * vx;
+        a = a + u.m[1 * 4 + i] * vy;
+        a = a + u.m[2 * 4 + i] * vz;
+        cm[i] = a;
     }
     const size_t mapsz = (size_t)smw * smh;
     DepthMap D{maps, smw, smh, (float)smw, (float)smh};
     const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
     float acc = 0.0f, count = 0.0f;
+    // unrolled so several samples' gathers are in flight; the sum stays in order
+#pragma unroll 4
     for (int i = 0; i < u.nsamples; ++i) {
-        float4 T = trans[i];
-        float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
-        float sx = cx / cw, sy = cy / cw, sz = cz / cw;
         D.texels = maps + (size_t)i * mapsz;
-        if (!u.pcf) {
-            float d = D.fetch(sx, sy);
-            acc = acc + ((sz <= d) ? 1.0f : u.si);
-        } else if constexpr (NT > 0) {
-            float tx[NT], ty[NT];
-            int iu[NT], iv[NT];
-            bool inside = true;
-#pragma unroll
-            for (int a = 0; a < NT; ++a) {
-                tx[a] = sx + u.taps[a] * incrWidth;
-                ty[a] = sy + u.taps[a] * incrHeight;
-                float uu = tx[a] * D.fw, vv = ty[a] * D.fh;
-                inside = inside && (uu >= 0.0f && uu < D.fw && vv >= 0.0f && vv < D.fh);
-                iu[a] = (int)floorf(uu);
-                iv[a] = (int)floorf(vv);
-            }
-            // one row of NT texels from iu[0] must cover every tap column
-            inside = inside && (iu[NT - 1] - iu[0] <= NT - 1) && (iu[0] + NT - 1 < smw);
-            float illum = 0.0f;
-            if (inside) {
-                Row<NT> rows[NT];
-#pragma unroll
-                for (int b = 0; b < NT; ++b)
-                    __builtin_memcpy(&rows[b], D.texels + (size_t)iv[b] * smw + iu[0], sizeof(Row<NT>));
-#pragma unroll
-                for (int a = 0; a < NT; ++a) {
-#pragma unroll
-                    for (int b = 0; b < NT; ++b) {
-                        float d = pick<NT>(rows[b], iu[a] - iu[0]);
-                        illum = illum + ((sz <= d) ? 1.0f : u.si);
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int a = 0; a < NT; ++a) {
-#pragma unroll
-                    for (int b = 0; b < NT; ++b) {
-                        float d = D.fetch(tx[a], ty[b]);
-                        illum = illum + ((sz <= d) ? 1.0f : u.si);
-                    }
-                }
-            }
-            acc = acc + illum / (float)(NT * NT);
-        } else {
-            float illum = 0.0f;
-            for (int a = 0; a < u.ntaps; ++a) {
-                float tx = sx + u.taps[a] * incrWidth;
-                for (int b = 0; b < u.ntaps; ++b) {
-                    float d = D.fetch(tx, sy + u.taps[b] * incrHeight);
-                    illum = illum + ((sz <= d) ? 1.0f : u.si);
-                }
-            }
-            acc = acc + illum / (float)(u.ntaps * u.ntaps);
-        }
+        acc = acc + soft_term<NT>(D, u, cm, trans[i], incrWidth, incrHeight);
         count = count + 1.0f;
     }
     out[k] = acc / count;

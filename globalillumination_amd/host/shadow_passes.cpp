@@ -1,0 +1,328 @@
+// shadow_passes.cpp — the reference apps' pass sequence over the C ABI.
+#include "shadow_passes.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace gi::host {
+
+namespace {
+
+#define GI_TRY(expr)                         \
+    do {                                     \
+        int gi_rc_ = (expr);                 \
+        if (gi_rc_ != GI_OK) return gi_rc_;  \
+    } while (0)
+
+int hip_ok(hipError_t e) { return e == hipSuccess ? GI_OK : GI_EDEVICE; }
+
+Mat4 identity() {
+    Mat4 m{};
+    m[0] = m[5] = m[10] = m[15] = 1.0f;
+    return m;
+}
+
+Mat4 mul(const Mat4& a, const Mat4& b) {
+    Mat4 r{};
+    gi_mat4_mul(a.data(), b.data(), r.data());
+    return r;
+}
+
+int upload(DeviceBuffer& buf, const void* src, size_t bytes, hipStream_t s) {
+    GI_TRY(buf.allocate(bytes));
+    if (bytes == 0) return GI_OK;
+    GI_TRY(hip_ok(hipMemcpyAsync(buf.get(), src, bytes, hipMemcpyHostToDevice, s)));
+    return hip_ok(hipStreamSynchronize(s));
+}
+
+// glm::inverseTranspose(mat3(mv)) as configurePhong uploads it
+// (ShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:113-116); it only feeds
+// the sign test of the pre-evaluation, so it is formed in double and rounded.
+void normal_matrix(const Mat4& mv, float out[9]) {
+    double m[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) m[c][r] = mv[c * 4 + r];
+    double cof[3][3];
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) {
+            int c1 = (c + 1) % 3, c2 = (c + 2) % 3, r1 = (r + 1) % 3, r2 = (r + 2) % 3;
+            cof[c][r] = m[c1][r1] * m[c2][r2] - m[c1][r2] * m[c2][r1];
+        }
+    double det = m[0][0] * cof[0][0] + m[0][1] * cof[0][1] + m[0][2] * cof[0][2];
+    // inverse(M)^T = cofactor(M) / det
+    for (int c = 0; c < 3; ++c)
+        for (int r = 0; r < 3; ++r) out[c * 3 + r] = static_cast<float>(cof[c][r] / det);
+}
+
+size_t bytes_of(int n, size_t each) { return static_cast<size_t>(n) * each; }
+
+}  // namespace
+
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr_) (void)hipFree(ptr_);
+}
+
+int DeviceBuffer::allocate(size_t bytes) {
+    if (bytes <= bytes_ && ptr_) return GI_OK;
+    if (ptr_) {
+        (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    if (bytes == 0) return GI_OK;
+    if (hipMalloc(&ptr_, bytes) != hipSuccess) {
+        ptr_ = nullptr;
+        return GI_EDEVICE;
+    }
+    bytes_ = bytes;
+    return GI_OK;
+}
+
+int GeometryViewer::configureAmbient(int width, int height) {
+    GI_TRY(gi_perspective(fov, static_cast<float>(width) / static_cast<float>(height), zNear, zFar,
+                          projection.data()));
+    GI_TRY(gi_lookat(eye.data(), look.data(), up.data(), view.data()));
+    model = identity();
+    return GI_OK;
+}
+
+Mat4 GeometryViewer::mvp() const { return mul(mul(projection, view), model); }
+
+// ---------------------------------------------------------------------------
+// HardShadowPipeline
+// ---------------------------------------------------------------------------
+HardShadowPipeline::~HardShadowPipeline() {
+    if (ownStream_ && stream) (void)hipStreamDestroy(stream);
+}
+
+int HardShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight, int shadowMapWidth,
+                             int shadowMapHeight) {
+    if (windowWidth <= 0 || windowHeight <= 0 || shadowMapWidth <= 0 || shadowMapHeight <= 0) return GI_EINVAL;
+    if (mesh.nverts() <= 0 || mesh.normals.size() != mesh.positions.size()) return GI_EINVAL;
+    if (!stream) {
+        GI_TRY(hip_ok(hipStreamCreate(&stream)));
+        ownStream_ = true;
+    }
+    windowWidth_ = windowWidth;
+    windowHeight_ = windowHeight;
+    nverts_ = mesh.nverts();
+    ntris_ = mesh.ntris();
+    shadowParams.shadowMapWidth = shadowMapWidth;
+    shadowParams.shadowMapHeight = shadowMapHeight;
+    GI_TRY(upload(pos_, mesh.positions.data(), mesh.positions.size() * sizeof(float), stream));
+    GI_TRY(upload(nrm_, mesh.normals.data(), mesh.normals.size() * sizeof(float), stream));
+    GI_TRY(upload(idx_, mesh.indices.data(), mesh.indices.size() * sizeof(int32_t), stream));
+    GI_TRY(mvp_.allocate(16 * sizeof(float)));
+    GI_TRY(depth_.allocate(bytes_of(shadowMapWidth * shadowMapHeight, sizeof(float))));
+    const int npx = windowWidth * windowHeight;
+    GI_TRY(gpos_.allocate(bytes_of(npx, 3 * sizeof(float))));
+    GI_TRY(gnrm_.allocate(bytes_of(npx, 3 * sizeof(float))));
+    GI_TRY(hard_.allocate(bytes_of(npx, sizeof(float))));
+    size_t ws = gi_depth_pass_workspace_size(ntris_, 1, shadowMapWidth, shadowMapHeight);
+    size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
+    if (ws == 0 || wg == 0) return GI_ENOTSUP;
+    return ws_.allocate(ws > wg ? ws : wg);
+}
+
+// displaySceneFromLightPOV (main.cpp:221-274) inside renderShadowMap
+// (:350-361): lightMVP = P*V*M of the light, depth of every triangle.
+int HardShadowPipeline::renderShadowMap() {
+    GeometryViewer light;
+    light.eye = lightEye;
+    light.look = lightAt;
+    light.up = lightUp;
+    GI_TRY(light.configureAmbient(shadowParams.shadowMapWidth, shadowParams.shadowMapHeight));
+    shadowParams.lightMVP = light.mvp();
+    shadowParams.lightMV = mul(light.view, light.model);
+    shadowParams.lightP = light.projection;
+    GI_TRY(hip_ok(hipMemcpyAsync(mvp_.get(), shadowParams.lightMVP.data(), 16 * sizeof(float),
+                                 hipMemcpyHostToDevice, stream)));
+    return gi_depth_pass(pos_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_, mvp_.as<float>(), 1,
+                         shadowParams.shadowMapWidth, shadowParams.shadowMapHeight, &rasterOpts,
+                         depth_.as<float>(), ws_.get(), ws_.size(), stream);
+}
+
+// displaySceneFromCameraPOV(GBUFFER_SHADER) (main.cpp:276-300, :363-372)
+int HardShadowPipeline::renderGBuffer() {
+    GeometryViewer cam;
+    cam.fov = cameraFov;
+    cam.eye = cameraEye;
+    cam.look = cameraAt;
+    cam.up = cameraUp;
+    GI_TRY(cam.configureAmbient(windowWidth_, windowHeight_));
+    cameraMVP_ = cam.mvp();
+    cameraMV_ = mul(cam.view, cam.model);
+    const int npx = windowWidth_ * windowHeight_;
+    (void)npx;
+    return gi_gbuffer_pass(pos_.as<float>(), nrm_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_,
+                           cameraMVP_.data(), windowWidth_, windowHeight_, gpos_.as<float>(), gnrm_.as<float>(),
+                           ws_.get(), ws_.size(), stream);
+}
+
+// The shader computeHardShadows picks from the flags (main.cpp:400-414).
+int HardShadowPipeline::technique() const {
+    const ShadowParams& p = shadowParams;
+    if (p.SMSR) return GI_RBSM;
+    if (p.RPCFPlusSMSR) return GI_RPCF;
+    if (p.RSMSS || p.RPCFPlusRSMSS) return GI_RBSM_FILTERED;
+    if (p.naive) return GI_NAIVE;
+    return GI_PCF;  // Shadow.frag falls through to PCF (:265-266)
+}
+
+int HardShadowPipeline::computeHardShadows() {
+    const ShadowParams& sp = shadowParams;
+    if (sp.conservative) return GI_ENOTSUP;  // ConservativeSMSR.frag is not on this path
+    if (!sp.naive && !sp.SMSR && !sp.RPCFPlusSMSR && !sp.RSMSS && !sp.RPCFPlusRSMSS && !sp.bilinearPCF)
+        return GI_ENOTSUP;  // Shadow.frag's PCF reads an unset depth without bilinearPCF (:101-104)
+    gi_shadow_params p{};
+    float B[16];
+    GI_TRY(gi_bias_matrix(sp.zBias, B));
+    GI_TRY(gi_mat4_mul(B, sp.lightMVP.data(), p.light_mvp));  // configureShadow :139-146
+    p.shadow_map_width = sp.shadowMapWidth;
+    p.shadow_map_height = sp.shadowMapHeight;
+    p.technique = technique();
+    p.kernel_order = sp.kernelOrder;
+    p.penumbra_size = sp.penumbraSize;
+    p.max_search = sp.maxSearch;
+    p.depth_threshold = sp.depthThreshold;
+    p.shadow_intensity = sp.shadowIntensity;
+    p.pre_evaluation = sp.preEvaluation ? 1 : 0;
+    // configurePhong(lightEye rotated 180 degrees about y, cameraEye) (main.cpp:309)
+    std::memcpy(p.mv, cameraMV_.data(), sizeof(p.mv));
+    normal_matrix(cameraMV_, p.normal_matrix);
+    p.light_position[0] = -lightEye[0];
+    p.light_position[1] = lightEye[1];
+    p.light_position[2] = -lightEye[2];
+    gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_};
+    return gi_hard_shadow_pass(&g, depth_.as<float>(), &p, hard_.as<float>(), stream);
+}
+
+int HardShadowPipeline::display() {
+    GI_TRY(renderShadowMap());
+    GI_TRY(renderGBuffer());
+    return computeHardShadows();
+}
+
+int HardShadowPipeline::download(std::vector<float>& hardShadow, std::vector<float>* depth,
+                                 std::vector<float>* gbufferPosition, std::vector<float>* gbufferNormal) {
+    const size_t npx = static_cast<size_t>(windowWidth_) * windowHeight_;
+    hardShadow.resize(npx);
+    GI_TRY(hip_ok(hipMemcpyAsync(hardShadow.data(), hard_.get(), npx * sizeof(float), hipMemcpyDeviceToHost,
+                                 stream)));
+    if (depth) {
+        depth->resize(static_cast<size_t>(shadowParams.shadowMapWidth) * shadowParams.shadowMapHeight);
+        GI_TRY(hip_ok(hipMemcpyAsync(depth->data(), depth_.get(), depth->size() * sizeof(float),
+                                     hipMemcpyDeviceToHost, stream)));
+    }
+    if (gbufferPosition) {
+        gbufferPosition->resize(3 * npx);
+        GI_TRY(hip_ok(hipMemcpyAsync(gbufferPosition->data(), gpos_.get(), 3 * npx * sizeof(float),
+                                     hipMemcpyDeviceToHost, stream)));
+    }
+    if (gbufferNormal) {
+        gbufferNormal->resize(3 * npx);
+        GI_TRY(hip_ok(hipMemcpyAsync(gbufferNormal->data(), gnrm_.get(), 3 * npx * sizeof(float),
+                                     hipMemcpyDeviceToHost, stream)));
+    }
+    return hip_ok(hipStreamSynchronize(stream));
+}
+
+// ---------------------------------------------------------------------------
+// SoftShadowPipeline
+// ---------------------------------------------------------------------------
+SoftShadowPipeline::~SoftShadowPipeline() {
+    if (ownStream_ && stream) (void)hipStreamDestroy(stream);
+}
+
+int SoftShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight, int shadowMapWidth,
+                             int shadowMapHeight, int maxSamples) {
+    if (windowWidth <= 0 || windowHeight <= 0 || shadowMapWidth <= 0 || shadowMapHeight <= 0) return GI_EINVAL;
+    if (maxSamples <= 0 || maxSamples > GI_MAX_SAMPLES) return GI_EINVAL;
+    if (mesh.nverts() <= 0 || mesh.normals.size() != mesh.positions.size()) return GI_EINVAL;
+    if (!stream) {
+        GI_TRY(hip_ok(hipStreamCreate(&stream)));
+        ownStream_ = true;
+    }
+    windowWidth_ = windowWidth;
+    windowHeight_ = windowHeight;
+    smw_ = shadowMapWidth;
+    smh_ = shadowMapHeight;
+    maxSamples_ = maxSamples;
+    nverts_ = mesh.nverts();
+    ntris_ = mesh.ntris();
+    GI_TRY(upload(pos_, mesh.positions.data(), mesh.positions.size() * sizeof(float), stream));
+    GI_TRY(upload(nrm_, mesh.normals.data(), mesh.normals.size() * sizeof(float), stream));
+    GI_TRY(upload(idx_, mesh.indices.data(), mesh.indices.size() * sizeof(int32_t), stream));
+    GI_TRY(mvps_.allocate(bytes_of(maxSamples, 16 * sizeof(float))));
+    GI_TRY(trans_.allocate(bytes_of(maxSamples, 4 * sizeof(float))));
+    GI_TRY(maps_.allocate(static_cast<size_t>(maxSamples) * smw_ * smh_ * sizeof(float)));
+    const int npx = windowWidth * windowHeight;
+    GI_TRY(gpos_.allocate(bytes_of(npx, 3 * sizeof(float))));
+    GI_TRY(gnrm_.allocate(bytes_of(npx, 3 * sizeof(float))));
+    GI_TRY(soft_.allocate(bytes_of(npx, sizeof(float))));
+    size_t ws = gi_depth_pass_workspace_size(ntris_, maxSamples, smw_, smh_);
+    size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
+    if (ws == 0 || wg == 0) return GI_ENOTSUP;
+    return ws_.allocate(ws > wg ? ws : wg);
+}
+
+int SoftShadowPipeline::renderGBuffer() {
+    GeometryViewer cam;
+    cam.fov = cameraFov;
+    cam.eye = cameraEye;
+    cam.look = cameraAt;
+    cam.up = cameraUp;
+    GI_TRY(cam.configureAmbient(windowWidth_, windowHeight_));
+    cameraMVP_ = cam.mvp();
+    return gi_gbuffer_pass(pos_.as<float>(), nrm_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_,
+                           cameraMVP_.data(), windowWidth_, windowHeight_, gpos_.as<float>(), gnrm_.as<float>(),
+                           ws_.get(), ws_.size(), stream);
+}
+
+int SoftShadowPipeline::renderMonteCarlo() {
+    const int n = numberOfSamples;
+    if (n <= 1 || n > maxSamples_) return GI_EINVAL;
+    lightMVPs_.assign(n, Mat4{});
+    std::vector<float> biased(static_cast<size_t>(n) * 16), trans(static_cast<size_t>(n) * 4);
+    float B[16];
+    GI_TRY(gi_bias_matrix(0.0f, B));
+    for (int i = 0; i < n; ++i) {
+        // UniformSampledLightSource::getEye / getAt (:13-25): both shifted alike
+        GeometryViewer light;
+        GI_TRY(gi_uniform_sampling(lightEye.data(), lightSize, n, i, light.eye.data()));
+        GI_TRY(gi_uniform_sampling(lightAt.data(), lightSize, n, i, light.look.data()));
+        light.up = lightUp;
+        GI_TRY(light.configureAmbient(smw_, smh_));
+        lightMVPs_[i] = light.mvp();  // shadowParams.lightMVPs[i] = lightMVP (:790)
+        GI_TRY(gi_mat4_mul(B, lightMVPs_[i].data(), &biased[16 * i]));
+    }
+    GI_TRY(gi_soft_light_trans(biased.data(), n, trans.data()));  // configureShadow :184-195
+    GI_TRY(hip_ok(hipMemcpyAsync(mvps_.get(), lightMVPs_.data(), bytes_of(n, 16 * sizeof(float)),
+                                 hipMemcpyHostToDevice, stream)));
+    GI_TRY(hip_ok(hipMemcpyAsync(trans_.get(), trans.data(), bytes_of(n, 4 * sizeof(float)),
+                                 hipMemcpyHostToDevice, stream)));
+    gi_raster_opts opts{4.0f, 20.0f, nullptr, nullptr};  // glPolygonOffset(4, 20), Soft main.cpp:295
+    GI_TRY(gi_depth_pass(pos_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_, mvps_.as<float>(), n, smw_, smh_,
+                         &opts, maps_.as<float>(), ws_.get(), ws_.size(), stream));
+    gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_};
+    gi_soft_params sp{shadowIntensity, adaptiveSamplingLowerAccuracy ? 1 : 0, 3, 1.0f, 1};
+    // columns 0-2 come from the last sample's B*lightMVP (MyGLGeometryViewer.cpp:168-170)
+    GI_TRY(gi_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, &biased[16 * (n - 1)], trans_.as<float>(),
+                               &sp, soft_.as<float>(), stream));
+    return hip_ok(hipStreamSynchronize(stream));  // the trans upload's host buffer goes out of scope
+}
+
+int SoftShadowPipeline::download(std::vector<float>& softShadow, std::vector<float>* maps) {
+    const size_t npx = static_cast<size_t>(windowWidth_) * windowHeight_;
+    softShadow.resize(npx);
+    GI_TRY(hip_ok(hipMemcpyAsync(softShadow.data(), soft_.get(), npx * sizeof(float), hipMemcpyDeviceToHost,
+                                 stream)));
+    if (maps) {
+        maps->resize(static_cast<size_t>(numberOfSamples) * smw_ * smh_);
+        GI_TRY(hip_ok(hipMemcpyAsync(maps->data(), maps_.get(), maps->size() * sizeof(float),
+                                     hipMemcpyDeviceToHost, stream)));
+    }
+    return hip_ok(hipStreamSynchronize(stream));
+}
+
+}  // namespace gi::host

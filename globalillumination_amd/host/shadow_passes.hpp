@@ -1,0 +1,151 @@
+// shadow_passes.hpp — C++ host pass interface over the C ABI of include/gi.h.
+//
+// Mirrors the structure of the reference apps: the ShadowParams struct
+// (ShadowMapping/include/Viewers/ShadowParams.h:6-37), the light/camera setup
+// of MyGLGeometryViewer (ShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:6-26)
+// and the pass functions of ShadowMapping/src/main.cpp (renderShadowMap :350,
+// renderGBuffer :363, computeHardShadows :400, display :459) and
+// SoftShadowMapping/src/main.cpp (renderMonteCarlo :756).  Where the reference
+// binds GL textures, these objects own device buffers allocated once in init()
+// (as initGL allocates its textures, main.cpp:886-899); every pass is then a
+// libgi kernel launch on the pipeline's stream.  Passes return a gi_status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/gi.h"
+
+namespace gi::host {
+
+struct Mesh {
+    std::vector<float> positions;  // nverts x 3
+    std::vector<float> normals;    // nverts x 3
+    std::vector<int32_t> indices;  // ntris x 3
+    int nverts() const { return static_cast<int>(positions.size() / 3); }
+    int ntris() const { return static_cast<int>(indices.size() / 3); }
+};
+
+using Mat4 = std::array<float, 16>;  // column-major, glm layout
+using Vec3 = std::array<float, 3>;
+
+// ShadowMapping/include/Viewers/ShadowParams.h:6-37 (GL texture handles are
+// the pipeline's device buffers).
+struct ShadowParams {
+    Mat4 lightMVP{};
+    Mat4 lightMV{};
+    Mat4 lightP{};
+    int shadowMapWidth = 0;
+    int shadowMapHeight = 0;
+    int maxSearch = 16;       // main.cpp:872
+    int kernelOrder = 7;      // gaussianFilter->getOrder(), main.cpp:859
+    int penumbraSize = 1;     // main.cpp:871
+    float depthThreshold = 0.0f;
+    float shadowIntensity = 0.25f;  // main.cpp:877
+    bool tricubicPCF = false;
+    bool bilinearPCF = false;
+    bool naive = true;        // main.cpp:876 default
+    bool SMSR = false;
+    bool RPCFPlusSMSR = false;
+    bool RSMSS = false;
+    bool RPCFPlusRSMSS = false;
+    bool conservative = false;
+    bool preEvaluation = true;  // computePreEvaluationBasedOnNormalOrientation
+    float zBias = 0.0f;         // constant bias folded into B[3][2]
+};
+
+// MyGLGeometryViewer: fov 45, near 1, far 1000 (:3-12); configureAmbient
+// builds P = perspective(fov, W/H, near, far), V = lookAt(eye, look, up), M = I.
+struct GeometryViewer {
+    float fov = 45.0f, zNear = 1.0f, zFar = 1000.0f;
+    Vec3 eye{}, look{}, up{0.0f, 0.0f, 1.0f};
+    Mat4 projection{}, view{}, model{};
+    int configureAmbient(int width, int height);
+    Mat4 mvp() const;  // projection * view * model
+};
+
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    int allocate(size_t bytes);
+    void* get() const { return ptr_; }
+    size_t size() const { return bytes_; }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr_); }
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// The hard-shadow frame of ShadowMapping's display() (main.cpp:459-472):
+// renderShadowMap -> renderGBuffer -> computeHardShadows.
+class HardShadowPipeline {
+public:
+    ShadowParams shadowParams;
+    Vec3 lightEye{}, lightAt{}, lightUp{0.0f, 0.0f, 1.0f};     // main.cpp:866-867
+    Vec3 cameraEye{}, cameraAt{}, cameraUp{0.0f, 0.0f, 1.0f};
+    float cameraFov = 45.0f;
+    gi_raster_opts rasterOpts{4.0f, 20.0f, nullptr, nullptr};  // glPolygonOffset(4, 20), main.cpp:246
+    hipStream_t stream = nullptr;
+
+    ~HardShadowPipeline();
+    // initGL (main.cpp:839-953): device buffers for the map, G-buffer, output.
+    int init(const Mesh& mesh, int windowWidth, int windowHeight, int shadowMapWidth, int shadowMapHeight);
+    int renderShadowMap();     // main.cpp:350-361
+    int renderGBuffer();       // main.cpp:363-372
+    int computeHardShadows();  // main.cpp:400-414
+    int display();             // main.cpp:459-472 without the GL swap
+    int download(std::vector<float>& hardShadow, std::vector<float>* depth = nullptr,
+                 std::vector<float>* gbufferPosition = nullptr, std::vector<float>* gbufferNormal = nullptr);
+    const Mat4& cameraMVP() const { return cameraMVP_; }
+    const Mat4& cameraMV() const { return cameraMV_; }
+
+private:
+    int technique() const;
+    int windowWidth_ = 0, windowHeight_ = 0, nverts_ = 0, ntris_ = 0;
+    bool ownStream_ = false;
+    Mat4 cameraMVP_{}, cameraMV_{};
+    DeviceBuffer pos_, nrm_, idx_, mvp_, depth_, gpos_, gnrm_, hard_, ws_;
+};
+
+// The Monte-Carlo area light of SoftShadowMapping (renderMonteCarlo,
+// SoftShadowMapping/src/main.cpp:756-811): one depth map per point-light
+// sample, then AccurateSoftShadow over the G-buffer.
+class SoftShadowPipeline {
+public:
+    float shadowIntensity = 0.25f;                 // Soft main.cpp:1611
+    bool adaptiveSamplingLowerAccuracy = false;    // 4x4 PCF per sample when set
+    Vec3 lightEye{}, lightAt{}, lightUp{0.0f, 0.0f, 1.0f};
+    int lightSize = 16;                            // LightSource::size (int), main.cpp:1598
+    int numberOfSamples = 289;                     // main.cpp:1600
+    Vec3 cameraEye{}, cameraAt{}, cameraUp{0.0f, 0.0f, 1.0f};
+    float cameraFov = 45.0f;
+    hipStream_t stream = nullptr;
+
+    ~SoftShadowPipeline();
+    int init(const Mesh& mesh, int windowWidth, int windowHeight, int shadowMapWidth, int shadowMapHeight,
+             int maxSamples);
+    int renderGBuffer();
+    // UniformSampledLightSource placement (eye and look-at shifted together,
+    // UniformSampledLightSource.cpp:13-37), the N depth maps, then the screen pass.
+    int renderMonteCarlo();
+    int download(std::vector<float>& softShadow, std::vector<float>* maps = nullptr);
+    const std::vector<Mat4>& lightMVPs() const { return lightMVPs_; }
+    const Mat4& cameraMVP() const { return cameraMVP_; }
+
+private:
+    int windowWidth_ = 0, windowHeight_ = 0, smw_ = 0, smh_ = 0, maxSamples_ = 0, nverts_ = 0, ntris_ = 0;
+    bool ownStream_ = false;
+    std::vector<Mat4> lightMVPs_;
+    Mat4 cameraMVP_{};
+    DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, soft_, ws_;
+};
+
+}  // namespace gi::host

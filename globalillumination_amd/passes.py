@@ -158,10 +158,12 @@ def phong_uniforms(cam: View, light_eye) -> tuple:
 # ---------------------------------------------------------------------------
 def render_shadow_map(mesh: DeviceMesh, mvps: np.ndarray, width: int, height: int,
                       polygon_offset=(0.0, 0.0), out: torch.Tensor | None = None,
-                      stream=None) -> torch.Tensor:
+                      stream=None, raster_events=None) -> torch.Tensor:
     """Depth of `mesh` from each light in `mvps` ((S, 16) unbiased P*V*M) into
     an (S, height, width) FP32 tensor (renderShadowMap, and renderMonteCarlo's
-    per-sample loop SoftShadowMapping/src/main.cpp:774-793)."""
+    per-sample loop SoftShadowMapping/src/main.cpp:774-793).  `raster_events`:
+    optional pair of torch.cuda.Event (enable_timing) recorded around the
+    tile-raster kernel."""
     m = np.ascontiguousarray(mvps, dtype=np.float32).reshape(-1, 16)
     S = m.shape[0]
     dev = mesh.positions.device
@@ -172,7 +174,8 @@ def render_shadow_map(mesh: DeviceMesh, mvps: np.ndarray, width: int, height: in
     if nbytes == 0:
         raise _lib.GiError("gi_depth_pass_workspace_size", _lib.GI_EINVAL, "bad size")
     ws = _WS.get(nbytes, dev)
-    opts = _lib.GiRasterOpts(float(polygon_offset[0]), float(polygon_offset[1]))
+    ev = (None, None) if raster_events is None else (raster_events[0].cuda_event, raster_events[1].cuda_event)
+    opts = _lib.GiRasterOpts(float(polygon_offset[0]), float(polygon_offset[1]), ev[0], ev[1])
     _lib.call("gi_depth_pass", _dptr(mesh.positions), mesh.nverts, _dptr(mesh.indices), mesh.ntris,
               _dptr(mvps_dev), S, width, height, ctypes.byref(opts), _dptr(out), _dptr(ws),
               ctypes.c_size_t(ws.numel()), _stream_handle(stream))

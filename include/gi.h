@@ -94,6 +94,9 @@ int gi_soft_light_trans(const float* light_mvps, int nsamples, float* trans_out)
 typedef struct gi_raster_opts {
     float polygon_offset_factor; /* glPolygonOffset(factor, units); both 0 = off */
     float polygon_offset_units;  /* the reference apps use (4, 20), main.cpp:246 */
+    void* raster_begin_event;    /* optional hipEvent_t recorded on `stream` right
+                                    before the tile-raster kernel, or NULL */
+    void* raster_end_event;      /* optional hipEvent_t recorded right after it */
 } gi_raster_opts;
 
 /* Workspace bytes for gi_depth_pass over `nmaps` maps of one mesh. */
