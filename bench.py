@@ -32,6 +32,7 @@ def _parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=64, help="light samples in the CPU baseline sample")
     ap.add_argument("--no-footprint", action="store_true", help="skip counting the unique texels gathered")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-pass timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -103,6 +104,49 @@ def cpu_baseline(wl, mvps, g_pos, maps_np, smw, samples, gi):
             "sample": f"{samples} of {len(mvps)} light samples (depth maps + screen pass) on the full "
                       f"{g_pos.shape[2]}x{g_pos.shape[1]} view, scaled x{len(mvps) // samples}",
             "seconds": round(t_depth + t_screen, 3)}
+
+
+def diff_timing(gi, torch, steps, warmup):
+    """Forward and forward+backward time of the differentiable soft-shadow
+    pass on its own workload (16 samples, 1024^2 maps, 512^2 view, 50k
+    triangles): HIP events around the library calls on the current stream;
+    the backward recomputes the depth pass.  Matrices are resident."""
+    import numpy as np
+
+    from globalillumination_amd import passes
+    from globalillumination_amd.diff import AreaLight, DiffScene
+
+    wl = gi.scenes.config5()
+    light, L, r = AreaLight.from_workload(wl)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, passes.view_matrices(wl.camera)[2], wl.camera.width, wl.camera.height)
+    scene = DiffScene(dm.indices, g, light)
+    out = torch.empty((g.height, g.width), dtype=torch.float32, device=dm.positions.device)
+    go = torch.randn((g.height, g.width), dtype=torch.float32, device=dm.positions.device,
+                     generator=torch.Generator(device=dm.positions.device).manual_seed(5))
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        scene.forward(L, r, dm.positions, out=out)
+        scene.backward_device(L, r, dm.positions, go)
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for e in evs:
+        for x in e:
+            x.record(stream)
+    torch.cuda.synchronize()
+    for e in evs:
+        e[0].record(stream)
+        scene.forward(L, r, dm.positions, out=out)
+        e[1].record(stream)
+        scene.forward(L, r, dm.positions, out=out)
+        scene.backward_device(L, r, dm.positions, go)
+        e[2].record(stream)
+    torch.cuda.synchronize()
+    fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    fb = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    return {"workload": "differentiable soft shadows: 16 samples, 1024^2 maps, 512x512 view, "
+                        f"{wl.mesh.ntris:,}-triangle displaced icosphere + ground",
+            "fwd_ms": round(fwd, 4), "fwd_bwd_ms": round(fb, 4), "fwd_bwd_over_fwd": round(fb / fwd, 3),
+            "steps": steps}
 
 
 def main():
@@ -201,6 +245,7 @@ def main():
                             "unique_texels": int(texels)},
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
                            "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)}}
+    diff = None if args.no_diff else diff_timing(gi, torch, 10, 2)
     cpu = None
     if not args.no_cpu_baseline:
         maps_np = None
@@ -213,7 +258,7 @@ def main():
                                    "view, 3x3 PCF per sample, 100,822-triangle displaced icosphere + ground",
                        "global_batch": 1, "samples": S, "shadow_map": smw, "view": [W, H],
                        "triangles": wl.mesh.ntris},
-            "roofline": roof, "cpu_baseline": cpu}
+            "roofline": roof, "cpu_baseline": cpu, "differentiable": diff}
     print(json.dumps(line), flush=True)
 
 

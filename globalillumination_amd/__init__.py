@@ -5,12 +5,12 @@ The compute path is libgi.so (hand-written gfx950 HIP kernels); this package
 is the host-side harness: deterministic scenes, camera math through the C
 ABI, and pass functions that mirror the reference's per-technique passes.
 """
-from . import _lib, camera, scenes
+from . import _lib, camera, diff, scenes
 from ._lib import GiError, lib
 from .passes import (DeviceMesh, GBuffer, ShadowParams, SoftParams, biased, compute_hard_shadows,
                      phong_uniforms, render_gbuffer, render_monte_carlo, render_shadow_map, soft_uniforms,
                      view_matrices)
 
-__all__ = ["_lib", "camera", "scenes", "GiError", "lib", "DeviceMesh", "GBuffer", "ShadowParams",
+__all__ = ["_lib", "camera", "diff", "scenes", "GiError", "lib", "DeviceMesh", "GBuffer", "ShadowParams",
            "SoftParams", "biased", "compute_hard_shadows", "phong_uniforms", "render_gbuffer",
            "render_monte_carlo", "render_shadow_map", "soft_uniforms", "view_matrices"]

@@ -68,6 +68,12 @@ class GiSoftParams(ctypes.Structure):
                 ("inclusive", ctypes.c_int)]
 
 
+class GiDiffParams(ctypes.Structure):
+    _fields_ = [("tau", ctypes.c_float),
+                ("bias", ctypes.c_float),
+                ("shadow_intensity", ctypes.c_float)]
+
+
 class GiError(RuntimeError):
     def __init__(self, fn: str, status: int, msg: str):
         super().__init__(f"{fn} failed: {msg} ({status})")
@@ -103,6 +109,16 @@ _SIGS = {
     "gi_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),
+    "gi_diff_soft_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(GiGBuffer), ctypes.POINTER(GiDiffParams),
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "gi_diff_soft_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.POINTER(GiGBuffer), ctypes.POINTER(GiDiffParams),
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -1,0 +1,483 @@
+// diff.hip — differentiable area-light soft shadows for gfx950.
+//
+// No reference counterpart: the reference's shadow tests are step functions.
+// The continuous model keeps the structure of AccurateSoftShadow.frag (one
+// depth map per light sample, the per-pixel mean over samples) and makes each
+// sample's visibility smooth:
+//   texel depth d  = window depth of the plane of the triangle that covers
+//                    the texel centre (visibility raster, IDs held fixed),
+//                    through the triangle's unsnapped projected vertices;
+//   q              = sum over the 4 texels around (s*W - .5, t*H - .5) of
+//                    smoothstep-bilinear weights * sigmoid((d - z + bias)/tau);
+//   V_i            = si + (1 - si) * q;   S = mean_i V_i.
+// The backward pass recomputes the visibility raster and texel depths (they
+// are not stored by the forward pass), then
+//   k_diff_shade_bwd  pixel adjoint: dL/d(B*M'_i) per block and dL/dd per
+//                     texel (scatter-add),
+//   k_diff_texel_bwd  texel adjoint: dL/dM'_i per block, dL/dvertex
+//                     (scatter-add),
+//   k_diff_reduce     fixed-order reduction of the per-block partials.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/gi.h"
+#include "gi_device.hpp"
+#include "gi_internal.hpp"
+
+namespace gi {
+namespace {
+
+constexpr int kThreads = 256;
+
+struct DiffCfg {
+    int smw, smh, S;
+    int gw, gh;
+    float tau, bias, si;
+};
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+__device__ __forceinline__ float smoothw(float f) { return f * f * (3.0f - 2.0f * f); }
+__device__ __forceinline__ float dsmoothw(float f) { return 6.0f * f * (1.0f - f); }
+
+// Texel depth of the continuous model (oracle: or_diff_depth).
+__device__ __forceinline__ bool tri_screen(const float* __restrict__ pos, const int* __restrict__ idx, int tri,
+                                           const float* m, int w, int h, float X[3], float Y[3], float Z[3],
+                                           float C[3][4]) {
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        int v = idx[3 * tri + j];
+        mat4_vec4(m, pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0f, C[j]);
+        ok = ok && (C[j][3] > 0.0f);
+        X[j] = (C[j][0] / C[j][3]) * (0.5f * w) + 0.5f * w;
+        Y[j] = (C[j][1] / C[j][3]) * (0.5f * h) + 0.5f * h;
+        Z[j] = (C[j][2] / C[j][3]) * 0.5f + 0.5f;
+    }
+    return ok;
+}
+
+__global__ __launch_bounds__(kThreads) void k_diff_depth(const float* __restrict__ pos, const int* __restrict__ idx,
+                                                         int ntris, const float* __restrict__ mvps,
+                                                         const unsigned long long* __restrict__ vis, DiffCfg c,
+                                                         float* __restrict__ depth) {
+    const size_t plane = (size_t)c.smw * c.smh;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int map = blockIdx.y;
+    if (k >= plane) return;
+    const unsigned long long key = vis[(size_t)map * plane + k];
+    float d = 1.0f;
+    if (key != ~0ull) {
+        const int tri = (int)((key & 0xffffffffull) >> 3);
+        float m[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) m[i] = mvps[16 * map + i];
+        float X[3], Y[3], Z[3], C[3][4];
+        if (tri < ntris && tri_screen(pos, idx, tri, m, c.smw, c.smh, X, Y, Z, C)) {
+            float D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+            if (D != 0.0f) {
+                float xc = (float)(k % c.smw) + 0.5f, yc = (float)(k / c.smw) + 0.5f;
+                float l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
+                float l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
+                d = Z[0] + l1 * (Z[1] - Z[0]) + l2 * (Z[2] - Z[0]);
+            }
+        }
+    }
+    depth[(size_t)map * plane + k] = d;
+}
+
+struct Tap {
+    float s, t, z, cw, cx, cy, cz;
+    int a0, b0;
+    float fu, fv;
+};
+
+__device__ __forceinline__ Tap project(const float* m, float x, float y, float z, const DiffCfg& c) {
+    float cc[4];
+    mat4_vec4(m, x, y, z, 1.0f, cc);
+    Tap t;
+    t.cx = cc[0]; t.cy = cc[1]; t.cz = cc[2]; t.cw = cc[3];
+    t.s = cc[0] / cc[3];
+    t.t = cc[1] / cc[3];
+    t.z = cc[2] / cc[3];
+    float u = t.s * (float)c.smw - 0.5f, v = t.t * (float)c.smh - 0.5f;
+    float fa = floorf(u), fb = floorf(v);
+    t.a0 = (int)fa;
+    t.b0 = (int)fb;
+    t.fu = u - fa;
+    t.fv = v - fb;
+    return t;
+}
+
+__device__ __forceinline__ float texel(const float* __restrict__ d, int a, int b, const DiffCfg& c) {
+    return (a >= 0 && a < c.smw && b >= 0 && b < c.smh) ? d[(size_t)b * c.smw + a] : 0.0f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_diff_shade(const float* __restrict__ gpos,
+                                                         const float* __restrict__ biased,
+                                                         const float* __restrict__ depth, DiffCfg c,
+                                                         float* __restrict__ out) {
+    const size_t plane = (size_t)c.gw * c.gh;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= plane) return;
+    const float x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
+    if (x == 0.0f) {
+        out[k] = 0.0f;
+        return;
+    }
+    const size_t msz = (size_t)c.smw * c.smh;
+    float acc = 0.0f;
+    for (int i = 0; i < c.S; ++i) {
+        const Tap t = project(biased + 16 * i, x, y, z, c);
+        const float* d = depth + (size_t)i * msz;
+        const float wu[2] = {1.0f - smoothw(t.fu), smoothw(t.fu)};
+        const float wv[2] = {1.0f - smoothw(t.fv), smoothw(t.fv)};
+        float q = 0.0f;
+#pragma unroll
+        for (int db = 0; db < 2; ++db)
+#pragma unroll
+            for (int da = 0; da < 2; ++da)
+                q += wu[da] * wv[db] * sigmoidf((texel(d, t.a0 + da, t.b0 + db, c) - t.z + c.bias) / c.tau);
+        acc += c.si + (1.0f - c.si) * q;
+    }
+    out[k] = acc / (float)c.S;
+}
+
+// Block sum of 16 per-thread values into partial[block][16] (fixed order).
+__device__ __forceinline__ void block_sum16(const float v[16], float* __restrict__ partial) {
+    __shared__ float red[kThreads / 64][16];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        float x = v[e];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) red[wave][e] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        float s = 0.0f;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
+        partial[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// Pixel adjoint.  grid (pixel blocks, S).
+__global__ __launch_bounds__(kThreads) void k_diff_shade_bwd(const float* __restrict__ gpos,
+                                                             const float* __restrict__ biased,
+                                                             const float* __restrict__ depth,
+                                                             const float* __restrict__ grad_out, DiffCfg c,
+                                                             float* __restrict__ grad_depth,
+                                                             float* __restrict__ partial) {
+    const size_t plane = (size_t)c.gw * c.gh;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    float gm[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) gm[e] = 0.0f;
+    if (k < plane && gpos[k] != 0.0f) {
+        const float x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
+        const float gq = grad_out[k] / (float)c.S * (1.0f - c.si);
+        const Tap t = project(biased + 16 * i, x, y, z, c);
+        const size_t msz = (size_t)c.smw * c.smh;
+        const float* d = depth + (size_t)i * msz;
+        const float su = smoothw(t.fu), sv = smoothw(t.fv);
+        const float wu[2] = {1.0f - su, su}, wv[2] = {1.0f - sv, sv};
+        const float dwu[2] = {-dsmoothw(t.fu), dsmoothw(t.fu)}, dwv[2] = {-dsmoothw(t.fv), dsmoothw(t.fv)};
+        float gz = 0.0f, gfu = 0.0f, gfv = 0.0f;
+#pragma unroll
+        for (int db = 0; db < 2; ++db)
+#pragma unroll
+            for (int da = 0; da < 2; ++da) {
+                const int a = t.a0 + da, b = t.b0 + db;
+                const bool in = a >= 0 && a < c.smw && b >= 0 && b < c.smh;
+                const float dd = in ? d[(size_t)b * c.smw + a] : 0.0f;
+                const float sg = sigmoidf((dd - t.z + c.bias) / c.tau);
+                const float ds = sg * (1.0f - sg) / c.tau;
+                const float w = wu[da] * wv[db];
+                if (in && gq != 0.0f) unsafeAtomicAdd(&grad_depth[(size_t)i * msz + (size_t)b * c.smw + a], gq * w * ds);
+                gz -= w * ds;
+                gfu += dwu[da] * wv[db] * sg;
+                gfv += wu[da] * dwv[db] * sg;
+            }
+        const float gs = gq * gfu * (float)c.smw, gt = gq * gfv * (float)c.smh;
+        gz *= gq;
+        const float iw = 1.0f / t.cw;
+        const float gc[4] = {gs * iw, gt * iw, gz * iw, -(gs * t.cx + gt * t.cy + gz * t.cz) * iw * iw};
+        const float xp[4] = {x, y, z, 1.0f};
+#pragma unroll
+        for (int col = 0; col < 4; ++col)
+#pragma unroll
+            for (int row = 0; row < 4; ++row) gm[col * 4 + row] = gc[row] * xp[col];
+    }
+    block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
+}
+
+// Texel adjoint.  grid (texel blocks, S).  A wave whose active texels all
+// belong to one triangle (large triangles such as a ground plane cover whole
+// rows of texels) sums its vertex gradients across lanes and issues 9 atomics
+// instead of 9 per lane; other waves add lane by lane.
+__global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __restrict__ pos, const int* __restrict__ idx,
+                                                             int ntris, const float* __restrict__ mvps,
+                                                             const unsigned long long* __restrict__ vis,
+                                                             const float* __restrict__ grad_depth, DiffCfg c,
+                                                             float* __restrict__ grad_pos,
+                                                             float* __restrict__ partial) {
+    const size_t msz = (size_t)c.smw * c.smh;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    float gm[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) gm[e] = 0.0f;
+    float gv[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gv[j][0] = gv[j][1] = gv[j][2] = 0.0f;
+    int tri = -1;
+    const float G = (k < msz) ? grad_depth[(size_t)i * msz + k] : 0.0f;
+    const unsigned long long key = (k < msz) ? vis[(size_t)i * msz + k] : ~0ull;
+    if (G != 0.0f && key != ~0ull) {
+        const int t = (int)((key & 0xffffffffull) >> 3);
+        float m[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) m[e] = mvps[16 * i + e];
+        float X[3], Y[3], Z[3], C[3][4];
+        if (t < ntris && tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C)) {
+            const float D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+            if (D != 0.0f) {
+                tri = t;
+                const float xc = (float)(k % c.smw) + 0.5f, yc = (float)(k / c.smw) + 0.5f;
+                const float l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
+                const float l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
+                const float lam[3] = {1.0f - l1 - l2, l1, l2};
+                // plane gradient in window x, y; d moves by -lambda_j * grad when vertex j moves
+                const float gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
+                const float gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const float gX = -G * lam[j] * gx * (0.5f * c.smw);
+                    const float gY = -G * lam[j] * gy * (0.5f * c.smh);
+                    const float gZ = G * lam[j] * 0.5f;
+                    const float iw = 1.0f / C[j][3];
+                    const float gc[4] = {gX * iw, gY * iw, gZ * iw,
+                                         -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
+                    const int v = idx[3 * t + j];
+                    const float vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0f};
+#pragma unroll
+                    for (int col = 0; col < 4; ++col)
+#pragma unroll
+                        for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        float g = 0.0f;
+#pragma unroll
+                        for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
+                        gv[j][a] = g;
+                    }
+                }
+            }
+        }
+    }
+    // every lane of the wave reaches this point
+    const unsigned long long act = __ballot(tri >= 0);
+    if (act != 0ull) {
+        const int first = __ffsll((long long)act) - 1;
+        const int t0 = __shfl(tri, first, 64);
+        const bool uniform = __ballot(tri >= 0 && tri != t0) == 0ull;
+        if (uniform) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float x = gv[j][a];
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+                    gv[j][a] = x;
+                }
+            if ((threadIdx.x % 64) == 0) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    const int v = idx[3 * t0 + j];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], gv[j][a]);
+                }
+            }
+        } else if (tri >= 0) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int v = idx[3 * tri + j];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], gv[j][a]);
+            }
+        }
+    }
+    block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
+}
+
+// out[i][e] = sum over blocks of partial[i][block][e]: one workgroup per
+// sample, each thread sums a fixed stride of blocks, then a fixed-order block
+// reduction (the result does not depend on scheduling).
+__global__ __launch_bounds__(kThreads) void k_diff_reduce(const float* __restrict__ partial, int nblocks,
+                                                          float* __restrict__ out) {
+    const int i = blockIdx.x;
+    float acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    for (int b = threadIdx.x; b < nblocks; b += kThreads) {
+        const float4* p = reinterpret_cast<const float4*>(partial + ((size_t)i * nblocks + b) * 16);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 v = p[q];
+            acc[4 * q] += v.x;
+            acc[4 * q + 1] += v.y;
+            acc[4 * q + 2] += v.z;
+            acc[4 * q + 3] += v.w;
+        }
+    }
+    block_sum16(acc, out + 16 * (size_t)i);
+}
+
+__global__ void k_fill_f32(float* p, size_t n, float v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+struct DiffWs {
+    unsigned long long* vis;
+    float* depth;
+    float* grad_depth;
+    float* part_px;
+    float* part_tx;
+    void* raster;
+    size_t raster_bytes;
+    int px_blocks, tx_blocks;
+};
+
+size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* ws, char* base) {
+    const size_t msz = (size_t)smw * smh;
+    const int px_blocks = (int)(((size_t)gw * gh + kThreads - 1) / kThreads);
+    const int tx_blocks = (int)((msz + kThreads - 1) / kThreads);
+    size_t off = 0;
+    size_t o_vis = off; off = align256(off + sizeof(unsigned long long) * msz * S);
+    size_t o_depth = off; off = align256(off + sizeof(float) * msz * S);
+    size_t o_gd = off; off = align256(off + sizeof(float) * msz * S);
+    size_t o_ppx = off; off = align256(off + sizeof(float) * 16 * (size_t)px_blocks * S);
+    size_t o_ptx = off; off = align256(off + sizeof(float) * 16 * (size_t)tx_blocks * S);
+    size_t rbytes = internal::vis_raster_workspace_size(ntris, S, smw, smh);
+    size_t o_r = off; off = align256(off + rbytes);
+    if (ws) {
+        ws->vis = reinterpret_cast<unsigned long long*>(base + o_vis);
+        ws->depth = reinterpret_cast<float*>(base + o_depth);
+        ws->grad_depth = reinterpret_cast<float*>(base + o_gd);
+        ws->part_px = reinterpret_cast<float*>(base + o_ppx);
+        ws->part_tx = reinterpret_cast<float*>(base + o_ptx);
+        ws->raster = base + o_r;
+        ws->raster_bytes = rbytes;
+        ws->px_blocks = px_blocks;
+        ws->tx_blocks = tx_blocks;
+    }
+    return rbytes ? off : 0;
+}
+
+bool finite_all(const float* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+int check_common(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris, const float* mvps_dev,
+                 const float* biased_dev, int S, int smw, int smh, const gi_gbuffer* gb, const gi_diff_params* p,
+                 void* ws, DiffCfg& c) {
+    if (!positions_dev || (!indices_dev && ntris > 0) || !mvps_dev || !biased_dev || !gb || !gb->position_dev || !p ||
+        !ws)
+        return GI_EINVAL;
+    if (nverts <= 0 || ntris < 0 || S <= 0 || S > GI_MAX_SAMPLES) return GI_EINVAL;
+    if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE || gb->width <= 0 || gb->height <= 0 ||
+        gb->width > GI_MAX_EDGE || gb->height > GI_MAX_EDGE)
+        return GI_EINVAL;
+    float sc[3] = {p->tau, p->bias, p->shadow_intensity};
+    if (!finite_all(sc, 3)) return GI_ENONFINITE;
+    if (!(p->tau > 0.0f)) return GI_EINVAL;
+    c = DiffCfg{smw, smh, S, gb->width, gb->height, p->tau, p->bias, p->shadow_intensity};
+    return GI_OK;
+}
+
+int recompute(const float* pos, int nverts, const int32_t* idx, int ntris, const float* mvps, const DiffCfg& c,
+              const DiffWs& w, hipStream_t s) {
+    int rc = internal::vis_raster(pos, nverts, idx, ntris, mvps, c.S, c.smw, c.smh, w.vis, w.raster,
+                                  w.raster_bytes, s);
+    if (rc != GI_OK) return rc;
+    const size_t msz = (size_t)c.smw * c.smh;
+    hipLaunchKernelGGL(k_diff_depth, dim3((unsigned)((msz + kThreads - 1) / kThreads), c.S), dim3(kThreads), 0, s,
+                       pos, idx, ntris, mvps, w.vis, c, w.depth);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+}  // namespace
+}  // namespace gi
+
+using namespace gi;
+
+extern "C" {
+
+size_t gi_diff_workspace_size(int ntris, int nsamples, int smw, int smh, int view_w, int view_h) {
+    if (ntris < 0 || nsamples <= 0 || smw <= 0 || smh <= 0 || view_w <= 0 || view_h <= 0) return 0;
+    return diff_layout(ntris, nsamples, smw, smh, view_w, view_h, nullptr, nullptr);
+}
+
+int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                         const float* mvps_dev, const float* biased_mvps_dev, int nsamples, int smw, int smh,
+                         const gi_gbuffer* gbuffer, const gi_diff_params* params, float* out_dev,
+                         void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+    DiffCfg c;
+    int rc = check_common(positions_dev, nverts, indices_dev, ntris, mvps_dev, biased_mvps_dev, nsamples, smw, smh,
+                          gbuffer, params, workspace_dev, c);
+    if (rc != GI_OK) return rc;
+    if (!out_dev) return GI_EINVAL;
+    DiffWs w;
+    size_t need = diff_layout(ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
+    if (need == 0) return GI_ENOTSUP;
+    if (workspace_bytes < need) return GI_ENOSPACE;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    rc = recompute(positions_dev, nverts, indices_dev, ntris, mvps_dev, c, w, s);
+    if (rc != GI_OK) return rc;
+    hipLaunchKernelGGL(k_diff_shade, dim3(w.px_blocks), dim3(kThreads), 0, s, gbuffer->position_dev, biased_mvps_dev,
+                       w.depth, c, out_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                          const float* mvps_dev, const float* biased_mvps_dev, int nsamples, int smw, int smh,
+                          const gi_gbuffer* gbuffer, const gi_diff_params* params, const float* grad_out_dev,
+                          float* grad_biased_mvps_dev, float* grad_mvps_dev, float* grad_positions_dev,
+                          void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+    DiffCfg c;
+    int rc = check_common(positions_dev, nverts, indices_dev, ntris, mvps_dev, biased_mvps_dev, nsamples, smw, smh,
+                          gbuffer, params, workspace_dev, c);
+    if (rc != GI_OK) return rc;
+    if (!grad_out_dev || !grad_biased_mvps_dev || !grad_mvps_dev || !grad_positions_dev) return GI_EINVAL;
+    DiffWs w;
+    size_t need = diff_layout(ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
+    if (need == 0) return GI_ENOTSUP;
+    if (workspace_bytes < need) return GI_ENOSPACE;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // the forward intermediates are recomputed, not stored
+    rc = recompute(positions_dev, nverts, indices_dev, ntris, mvps_dev, c, w, s);
+    if (rc != GI_OK) return rc;
+    const size_t msz = (size_t)smw * smh * nsamples;
+    hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(kThreads), 0, s, w.grad_depth, msz, 0.0f);
+    hipLaunchKernelGGL(k_diff_shade_bwd, dim3(w.px_blocks, nsamples), dim3(kThreads), 0, s, gbuffer->position_dev,
+                       biased_mvps_dev, w.depth, grad_out_dev, c, w.grad_depth, w.part_px);
+    hipLaunchKernelGGL(k_diff_texel_bwd, dim3(w.tx_blocks, nsamples), dim3(kThreads), 0, s, positions_dev,
+                       indices_dev, ntris, mvps_dev, w.vis, w.grad_depth, c, grad_positions_dev, w.part_tx);
+    hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_px, w.px_blocks,
+                       grad_biased_mvps_dev);
+    hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_tx, w.tx_blocks, grad_mvps_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+}  // extern "C"

@@ -36,6 +36,7 @@
 
 #include "../../include/gi.h"
 #include "gi_device.hpp"
+#include "gi_internal.hpp"
 
 namespace gi {
 
@@ -981,6 +982,31 @@ int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Ras
 }
 
 }  // namespace
+
+namespace internal {
+
+size_t vis_raster_workspace_size(int ntris, int nmaps, int w, int h) {
+    if (ntris < 0 || nmaps <= 0 || !size_ok(w, h)) return 0;
+    return layout(ntris, nmaps, w, h, false, nullptr, nullptr);
+}
+
+int vis_raster(const float* pos, int nverts, const int32_t* idx, int ntris, const float* mvps_dev, int nmaps, int w,
+               int h, unsigned long long* vis_out, void* ws, size_t ws_bytes, hipStream_t s) {
+    if (!pos || (!idx && ntris > 0) || !mvps_dev || !vis_out || !ws || nverts <= 0 || ntris < 0 || nmaps <= 0)
+        return GI_EINVAL;
+    if (ntris >= (1 << 28)) return GI_ENOTSUP;
+    if (!size_ok(w, h)) return GI_ENOTSUP;
+    Workspace wsp;
+    size_t need = layout(ntris, nmaps, w, h, false, &wsp, reinterpret_cast<char*>(ws));
+    if (ws_bytes < need) return GI_ENOSPACE;
+    MeshView mesh{pos, nullptr, idx, ntris};
+    MvpSource mvps{};
+    mvps.dev = mvps_dev;
+    RasterCfg cfg{w, h, (w + kTile - 1) / kTile, (h + kTile - 1) / kTile, 0.0f, 0.0f};
+    return run_raster<true>(mesh, mvps, nmaps, cfg, wsp, vis_out, s);
+}
+
+}  // namespace internal
 }  // namespace gi
 
 using namespace gi;

@@ -338,4 +338,5 @@ def config5(nsamples: int = 16, view: int = 512, sm: int = 1024, freq: int = 50)
     offs = stratified_jitter(nsamples, seed=5)
     lights = area_light_samples((3.0, 7.0, 2.0), (0.0, 0.0, 0.0), 0.5, offs, sm, sm)
     cam = View((0.0, 3.0, 6.0), (0.0, 1.0, 0.0), view, view, fov=60.0)
-    return Workload("cfg5_diff", mesh, cam, lights, "soft", extra={"offsets": offs, "side": 0.5})
+    return Workload("cfg5_diff", mesh, cam, lights, "soft",
+                    extra={"offsets": offs, "side": 0.5, "center": (3.0, 7.0, 2.0), "at": (0.0, 0.0, 0.0)})

@@ -54,6 +54,8 @@ def lib() -> ctypes.CDLL:
             "or_depth_raster_brute": [vp, i, vp, i, vp, i, i, f, f, vp],
             "or_visibility_raster": [vp, i, vp, i, vp, i, i, vp],
             "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp],
+            "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
+            "or_diff_soft": [vp, i, i, vp, i, i, i, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp],
             "or_set_threads": [i], "or_get_threads": [],
         }
         for name, args in sig.items():
@@ -209,3 +211,36 @@ def gbuffer(positions, normals, indices, mvp, w, h):
     if rc:
         raise RuntimeError(f"or_gbuffer failed ({rc})")
     return po, no
+
+
+def _c64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def diff_depth(positions, indices, mvp64, vis: np.ndarray) -> np.ndarray:
+    """Continuous-model texel depths (float64) of one map; vis from
+    visibility_raster fixes the triangle of every texel."""
+    pos = _c64(positions).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+    m = _c64(mvp64).reshape(-1)
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    h, w = v.shape
+    out = np.empty((h, w), np.float64)
+    rc = lib().or_diff_depth(_p(pos), pos.shape[0], _p(idx), idx.shape[0], _p(m), w, h, _p(v), _p(out))
+    if rc:
+        raise RuntimeError(f"or_diff_depth failed ({rc})")
+    return out
+
+
+def diff_soft(gpos: np.ndarray, maps64: np.ndarray, biased_mvps64, tau, bias, si) -> np.ndarray:
+    """Continuous-model soft visibility (float64) of every G-buffer pixel."""
+    g = _c32(gpos)
+    _, gh, gw = g.shape
+    maps = _c64(maps64)
+    S, h, w = maps.shape
+    m = _c64(biased_mvps64).reshape(S, 16)
+    out = np.empty((gh, gw), np.float64)
+    rc = lib().or_diff_soft(_p(g), gw, gh, _p(maps), S, w, h, _p(m), float(tau), float(bias), float(si), _p(out))
+    if rc:
+        raise RuntimeError(f"or_diff_soft failed ({rc})")
+    return out

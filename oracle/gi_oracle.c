@@ -957,3 +957,85 @@ int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int n
     free(vis);
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Differentiable soft shadows: the continuous model in FP64                  */
+/* (no reference counterpart; checker for the adjoint kernels)                */
+/* ------------------------------------------------------------------------ */
+static void m4v_d(const double m[16], double x, double y, double z, double w, double o[4])
+{
+    for (int i = 0; i < 4; ++i)
+        o[i] = m[0 * 4 + i] * x + m[1 * 4 + i] * y + m[2 * 4 + i] * z + m[3 * 4 + i] * w;
+}
+
+int or_diff_depth(const double* pos, int nv, const int* idx, int nt, const double mvp[16], int w, int h,
+                  const uint64_t* vis, double* depth_out)
+{
+    if (!pos || !idx || !vis || !depth_out || w <= 0 || h <= 0 || nv <= 0 || nt < 0) return -1;
+    long long n = (long long)w * h;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        uint64_t key = vis[k];
+        if (key == ~(uint64_t)0) { depth_out[k] = 1.0; continue; }
+        int tri = (int)((key & 0xffffffffu) >> 3);
+        if (tri >= nt) { depth_out[k] = 1.0; continue; }
+        double X[3], Y[3], Z[3];
+        int ok = 1;
+        for (int j = 0; j < 3; ++j) {
+            const double* v = pos + 3 * (size_t)idx[3 * tri + j];
+            double c[4];
+            m4v_d(mvp, v[0], v[1], v[2], 1.0, c);
+            if (!(c[3] > 0.0)) ok = 0;
+            X[j] = (c[0] / c[3]) * (0.5 * w) + 0.5 * w;
+            Y[j] = (c[1] / c[3]) * (0.5 * h) + 0.5 * h;
+            Z[j] = (c[2] / c[3]) * 0.5 + 0.5;
+        }
+        double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+        if (!ok || D == 0.0) { depth_out[k] = 1.0; continue; }
+        double xc = (double)(k % w) + 0.5, yc = (double)(k / w) + 0.5;
+        double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
+        double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
+        depth_out[k] = Z[0] + l1 * (Z[1] - Z[0]) + l2 * (Z[2] - Z[0]);
+    }
+    return 0;
+}
+
+static double smooth_w(double f) { return f * f * (3.0 - 2.0 * f); }
+static double sigmoid_d(double x) { return 1.0 / (1.0 + exp(-x)); }
+
+int or_diff_soft(const float* gpos, int gw, int gh, const double* maps, int nsamples, int w, int h,
+                 const double* biased_mvps, double tau, double bias, double si, double* out)
+{
+    if (!gpos || !maps || !biased_mvps || !out || nsamples <= 0 || !(tau > 0.0)) return -1;
+    size_t plane = (size_t)gw * gh, msz = (size_t)w * h;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        double x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
+        if (gpos[k] == 0.0f) { out[k] = 0.0; continue; }
+        double acc = 0.0;
+        for (int i = 0; i < nsamples; ++i) {
+            double c[4];
+            m4v_d(biased_mvps + 16 * i, x, y, z, 1.0, c);
+            double s = c[0] / c[3], t = c[1] / c[3], zz = c[2] / c[3];
+            double u = s * w - 0.5, v = t * h - 0.5;
+            double a0 = floor(u), b0 = floor(v);
+            double fu = u - a0, fv = v - b0;
+            double wu[2] = {1.0 - smooth_w(fu), smooth_w(fu)}, wv[2] = {1.0 - smooth_w(fv), smooth_w(fv)};
+            double q = 0.0;
+            for (int db = 0; db < 2; ++db)
+                for (int da = 0; da < 2; ++da) {
+                    long long a = (long long)a0 + da, b = (long long)b0 + db;
+                    double d = (a >= 0 && a < w && b >= 0 && b < h) ? maps[(size_t)i * msz + (size_t)b * w + a] : 0.0;
+                    q += wu[da] * wv[db] * sigmoid_d((d - zz + bias) / tau);
+                }
+            acc += si + (1.0 - si) * q;
+        }
+        out[k] = acc / nsamples;
+    }
+    return 0;
+}

@@ -22,7 +22,7 @@ extern "C" {
 /* ---- glm 0.9.3.1 math, FP32 (a1, a4) -------------------------------------- */
 /* include/glm/gtc/matrix_transform.inl:223-244 (fovy in degrees) */
 void or_perspective(float fovy_deg, float aspect, float znear, float zfar, float out[16]);
-/* include/glm/gtc/matrix_transform.inl:200-220 (ortho with near/far) */
+/* include/glm/gtc/matrix_transform.inl:163-182 (ortho with near/far) */
 void or_ortho(float l, float r, float b, float t, float znear, float zfar, float out[16]);
 /* include/glm/gtc/matrix_transform.inl:383-411 */
 void or_lookat(const float eye[3], const float center[3], const float up[3], float out[16]);
@@ -87,6 +87,22 @@ int or_visibility_raster(const float* pos, int nv, const int* idx, int nt,
  * when back-facing; background = 0. */
 int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
                const float mvp[16], int w, int h, float* pos_out, float* nrm_out);
+
+/* ---- differentiable soft shadows (continuous model, FP64) ------------------ */
+/* Texel depth of the continuous model: for every texel whose visibility key
+ * (or_visibility_raster) names triangle T, the window depth of T's plane
+ * through its UNSNAPPED projected vertices at the texel centre; 1.0 where no
+ * triangle covers the texel.  mvp: the unbiased P*V of the map (FP64). */
+int or_diff_depth(const double* pos, int nv, const int* idx, int nt, const double mvp[16], int w, int h,
+                  const uint64_t* vis, double* depth_out);
+/* Soft visibility of the continuous model for every G-buffer pixel:
+ *   S = (1/N) sum_i [si + (1 - si) * q_i],
+ *   q_i = sum over the 4 texels around (s*W - .5, t*H - .5) of smoothstep-
+ *         bilinear weights times sigmoid((d - z + bias) / tau),
+ * (s, t, z) = perspective divide of B*M'_i * (x, 1); texels off the map have
+ * depth 0.  Background pixels (x == 0) give 0.  maps: N x h x w (FP64). */
+int or_diff_soft(const float* gpos, int gw, int gh, const double* maps, int nsamples, int w, int h,
+                 const double* biased_mvps, double tau, double bias, double si, double* out);
 
 /* threads used by the OpenMP loops (0 = runtime default) */
 void or_set_threads(int n);

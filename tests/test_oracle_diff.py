@@ -1,0 +1,148 @@
+"""CPU pins of the float64 continuous model (or_diff_depth, or_diff_soft)
+that the differentiable kernels are checked against: closed forms of the
+model, not a retyped formula.
+
+- Texel depth of a planar receiver equals the window depth of the ray through
+  the texel centre intersected with the plane (a projective map takes the
+  plane to a plane, so the barycentric plane interpolation is exact).
+- Texels of the depth raster's triangles agree with the FP32 depth raster up
+  to its 1/256-px vertex snap.
+- With every texel at one depth d, the smoothstep-bilinear weights sum to one
+  and q = sigmoid((d - z + bias)/tau) exactly, wherever the sample lands.
+- A sample on a texel centre reads that one texel; a sample half-way between
+  two texel centres averages the two sigmoids with weight 1/2.
+- Samples off the map read depth 0 (full shadow for tau << z).
+"""
+import numpy as np
+import pytest
+
+from globalillumination_amd import scenes
+from globalillumination_amd.diff import BIAS, AreaLight, colmajor
+
+TAU, BIAS_Z, SI = 2e-4, 2e-3, 0.25
+
+
+def _sig(x):
+    with np.errstate(over="ignore"):
+        return 1.0 / (1.0 + np.exp(-x))
+
+
+@pytest.fixture(scope="module")
+def ground_setup(oracle_lib):
+    mesh = scenes.ground_quad(20.0, 20.0, 0.0)
+    light = AreaLight((0.0, 0.0, 0.0), np.array([[0.3, -0.2]]), 128)
+    M = light.matrices(np.array([3.0, 7.0, 2.0]), 0.25)[0]
+    vis = oracle_lib.visibility_raster(mesh.positions, mesh.indices, colmajor(M).astype(np.float32), 128, 128)
+    d = oracle_lib.diff_depth(mesh.positions.astype(np.float64), mesh.indices, colmajor(M), vis)
+    return mesh, M, vis, d
+
+
+def test_diff_depth_plane_closed_form(ground_setup):
+    mesh, M, vis, d = ground_setup
+    covered = vis != np.uint64(0xFFFFFFFFFFFFFFFF)
+    assert covered.mean() > 0.99  # the 20x20 ground fills the light's view
+    Minv = np.linalg.inv(M)
+    ys, xs = np.nonzero(covered)
+    sel = np.random.default_rng(0).choice(len(xs), size=500, replace=False)
+    for y, x in zip(ys[sel], xs[sel]):
+        xn, yn = 2.0 * (x + 0.5) / 128 - 1.0, 2.0 * (y + 0.5) / 128 - 1.0
+        p0 = Minv @ np.array([xn, yn, -1.0, 1.0])
+        p1 = Minv @ np.array([xn, yn, 1.0, 1.0])
+        p0, p1 = p0[:3] / p0[3], p1[:3] / p1[3]
+        t = p0[1] / (p0[1] - p1[1])  # y = 0
+        hit = p0 + t * (p1 - p0)
+        c = M @ np.array([*hit, 1.0])
+        assert abs(d[y, x] - (0.5 * c[2] / c[3] + 0.5)) <= 1e-12
+
+
+def test_diff_depth_matches_raster_up_to_snap(oracle_lib):
+    ico = scenes.displaced_icosphere(8, seed=1)
+    light = AreaLight((0.0, 0.0, 0.0), np.array([[0.0, 0.0]]), 128)
+    M = light.matrices(np.array([3.0, 7.0, 2.0]), 0.25)[0]
+    m32 = colmajor(M).astype(np.float32)
+    vis = oracle_lib.visibility_raster(ico.positions, ico.indices, m32, 128, 128)
+    d = oracle_lib.diff_depth(ico.positions.astype(np.float64), ico.indices, colmajor(M), vis)
+    r = oracle_lib.depth_raster(ico.positions, ico.indices, m32, 128, 128)
+    cov = vis != np.uint64(0xFFFFFFFFFFFFFFFF)
+    assert cov.mean() > 0.1
+    assert (d[~cov] == 1.0).all() and (r[~cov] == 1.0).all()
+    # vertex snap of 1/256 px times the steepest depth slope seen here
+    assert np.abs(d[cov] - r[cov]).max() < 2e-4
+    assert np.median(np.abs(d[cov] - r[cov])) < 2e-6
+
+
+def _ortho_setup(n=8):
+    """Receiver points whose light-space coordinates are chosen directly: an
+    identity 'matrix' B*M = I maps (x, y, z) to (s, t, z)."""
+    return np.eye(4).reshape(1, 16)
+
+
+def _gpos(points):
+    p = np.asarray(points, np.float32)  # (k, 3) as (s, t, z); x must be nonzero
+    return np.ascontiguousarray(p.T.reshape(3, 1, -1))
+
+
+def test_diff_soft_constant_map_is_one_sigmoid(oracle_lib):
+    rng = np.random.default_rng(1)
+    w = h = 16
+    dval = 0.7
+    maps = np.full((1, h, w), dval)
+    # sample positions strictly inside the texel-centre grid
+    st = rng.uniform(0.6 / w, 1.0 - 0.6 / w, size=(200, 2))
+    z = rng.uniform(dval - 3e-3, dval + 3e-3, size=200)
+    pts = np.c_[st, z]
+    pos = _gpos(pts)
+    got = oracle_lib.diff_soft(pos, maps, _ortho_setup(), TAU, BIAS_Z, SI)[0]
+    z32 = pos[2, 0].astype(np.float64)
+    want = SI + (1 - SI) * _sig((dval - z32 + BIAS_Z) / TAU)
+    assert np.abs(got - want).max() <= 1e-12
+
+
+def test_diff_soft_texel_centre_and_midpoint(oracle_lib):
+    w = h = 8
+    maps = np.random.default_rng(2).uniform(0.5, 0.6, size=(1, h, w))
+    z = 0.55
+    # on the centre of texel (3, 5): reads only that texel
+    s, t = (3 + 0.5) / w, (5 + 0.5) / h
+    # half-way between texel centres (2, 4) and (3, 4) in s
+    s2, t2 = (3.0) / w, (4 + 0.5) / h
+    pos = _gpos([[s, t, z], [s2, t2, z]])
+    got = oracle_lib.diff_soft(pos, maps, _ortho_setup(), TAU, BIAS_Z, SI)[0]
+    zf = np.float64(np.float32(z))
+    q0 = _sig((maps[0, 5, 3] - zf + BIAS_Z) / TAU)
+    q1 = 0.5 * _sig((maps[0, 4, 2] - zf + BIAS_Z) / TAU) + 0.5 * _sig((maps[0, 4, 3] - zf + BIAS_Z) / TAU)
+    sf, s2f = np.float64(np.float32(s)), np.float64(np.float32(s2))
+    assert sf * w - 0.5 == 3.0 and s2f * w - 0.5 == 2.5  # exact sample positions
+    assert abs(got[0] - (SI + (1 - SI) * q0)) <= 1e-12
+    assert abs(got[1] - (SI + (1 - SI) * q1)) <= 1e-12
+
+
+def test_diff_soft_off_map_reads_zero_and_background(oracle_lib):
+    maps = np.full((2, 8, 8), 0.9)
+    pos = _gpos([[1.5, 0.5, 0.5], [-0.7, 0.5, 0.5], [0.0, 0.5, 0.5]])
+    mv = np.repeat(_ortho_setup(), 2, axis=0)
+    got = oracle_lib.diff_soft(pos, maps, mv, TAU, BIAS_Z, SI)[0]
+    want_off = SI + (1 - SI) * _sig((0.0 - 0.5 + BIAS_Z) / TAU)
+    assert abs(got[0] - want_off) <= 1e-12 and abs(got[1] - want_off) <= 1e-12
+    assert got[2] == 0.0  # x == 0 marks background in the G-buffer
+
+
+def test_diff_soft_mean_over_samples(oracle_lib):
+    """Two samples whose matrices are I and a shift by one map width: the
+    second reads off the map, so S is the mean of the lit and off-map values."""
+    maps = np.full((2, 8, 8), 0.9)
+    shift = np.eye(4)
+    shift[0, 3] = 1.0
+    mv = np.stack([colmajor(np.eye(4)), colmajor(shift)])
+    pos = _gpos([[0.5, 0.5, 0.5]])
+    got = oracle_lib.diff_soft(pos, maps, mv, TAU, BIAS_Z, SI)[0][0]
+    lit = SI + (1 - SI) * _sig((0.9 - 0.5 + BIAS_Z) / TAU)
+    off = SI + (1 - SI) * _sig((0.0 - 0.5 + BIAS_Z) / TAU)
+    assert abs(got - 0.5 * (lit + off)) <= 1e-12
+
+
+def test_bias_matrix_is_glm_bias():
+    """B maps clip [-1, 1] to [0, 1] (MyGLGeometryViewer.cpp:139-145 with no
+    depth bias)."""
+    p = np.array([-1.0, 1.0, 0.0, 1.0])
+    assert np.allclose(BIAS @ p, [0.0, 1.0, 0.5, 1.0])
