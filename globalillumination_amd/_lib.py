@@ -69,9 +69,9 @@ class GiSoftParams(ctypes.Structure):
 
 
 class GiDiffParams(ctypes.Structure):
-    _fields_ = [("tau", ctypes.c_float),
-                ("bias", ctypes.c_float),
-                ("shadow_intensity", ctypes.c_float)]
+    _fields_ = [("tau", ctypes.c_double),
+                ("bias", ctypes.c_double),
+                ("shadow_intensity", ctypes.c_double)]
 
 
 class GiError(RuntimeError):

@@ -31,55 +31,78 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// Geometry (projections, barycentrics, window depths, the sigmoid argument)
+// is evaluated in FP64: the argument (d - z + bias) / tau divides the
+// rounding of two window depths near 0.9 by tau ~ 2e-4, so FP32 geometry
+// would leave ~1e-3 noise in every term.  Texel depths are stored in FP64;
+// the scatter-added adjoints (texel depth, vertex position) are FP32.
 struct DiffCfg {
     int smw, smh, S;
     int gw, gh;
-    float tau, bias, si;
+    double tau, bias, si;
 };
 
-__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
-__device__ __forceinline__ float smoothw(float f) { return f * f * (3.0f - 2.0f * f); }
-__device__ __forceinline__ float dsmoothw(float f) { return 6.0f * f * (1.0f - f); }
+__device__ __forceinline__ double sigmoid_d(double x) { return 1.0 / (1.0 + exp(-x)); }
+__device__ __forceinline__ double smoothw(double f) { return f * f * (3.0 - 2.0 * f); }
+__device__ __forceinline__ double dsmoothw(double f) { return 6.0 * f * (1.0 - f); }
 
-// Texel depth of the continuous model (oracle: or_diff_depth).
+__device__ __forceinline__ void load_m(const double* __restrict__ src, double m[16]) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) m[i] = src[i];
+}
+
+// FP32 copies of the FP64 matrices for the visibility raster (round to
+// nearest even, as a host-side float cast).
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = __double2float_rn(src[i]);
+}
+
+__device__ __forceinline__ void mat4_vec4_d(const double m[16], double x, double y, double z, double w,
+                                            double out[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[r] = ((m[r] * x + m[4 + r] * y) + m[8 + r] * z) + m[12 + r] * w;
+}
+
+// Window coordinates of a triangle's unsnapped projected vertices
+// (oracle: or_diff_depth).  False when a vertex has w <= 0.
 __device__ __forceinline__ bool tri_screen(const float* __restrict__ pos, const int* __restrict__ idx, int tri,
-                                           const float* m, int w, int h, float X[3], float Y[3], float Z[3],
-                                           float C[3][4]) {
+                                           const double m[16], int w, int h, double X[3], double Y[3], double Z[3],
+                                           double C[3][4]) {
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         int v = idx[3 * tri + j];
-        mat4_vec4(m, pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0f, C[j]);
-        ok = ok && (C[j][3] > 0.0f);
-        X[j] = (C[j][0] / C[j][3]) * (0.5f * w) + 0.5f * w;
-        Y[j] = (C[j][1] / C[j][3]) * (0.5f * h) + 0.5f * h;
-        Z[j] = (C[j][2] / C[j][3]) * 0.5f + 0.5f;
+        mat4_vec4_d(m, pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0, C[j]);
+        ok = ok && (C[j][3] > 0.0);
+        X[j] = (C[j][0] / C[j][3]) * (0.5 * w) + 0.5 * w;
+        Y[j] = (C[j][1] / C[j][3]) * (0.5 * h) + 0.5 * h;
+        Z[j] = (C[j][2] / C[j][3]) * 0.5 + 0.5;
     }
     return ok;
 }
 
 __global__ __launch_bounds__(kThreads) void k_diff_depth(const float* __restrict__ pos, const int* __restrict__ idx,
-                                                         int ntris, const float* __restrict__ mvps,
+                                                         int ntris, const double* __restrict__ mvps,
                                                          const unsigned long long* __restrict__ vis, DiffCfg c,
-                                                         float* __restrict__ depth) {
+                                                         double* __restrict__ depth) {
     const size_t plane = (size_t)c.smw * c.smh;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int map = blockIdx.y;
     if (k >= plane) return;
     const unsigned long long key = vis[(size_t)map * plane + k];
-    float d = 1.0f;
+    double d = 1.0;
     if (key != ~0ull) {
         const int tri = (int)((key & 0xffffffffull) >> 3);
-        float m[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) m[i] = mvps[16 * map + i];
-        float X[3], Y[3], Z[3], C[3][4];
+        double m[16];
+        load_m(mvps + 16 * map, m);
+        double X[3], Y[3], Z[3], C[3][4];
         if (tri < ntris && tri_screen(pos, idx, tri, m, c.smw, c.smh, X, Y, Z, C)) {
-            float D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-            if (D != 0.0f) {
-                float xc = (float)(k % c.smw) + 0.5f, yc = (float)(k / c.smw) + 0.5f;
-                float l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
-                float l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
+            double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+            if (D != 0.0) {
+                double xc = (double)(k % c.smw) + 0.5, yc = (double)(k / c.smw) + 0.5;
+                double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
+                double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
                 d = Z[0] + l1 * (Z[1] - Z[0]) + l2 * (Z[2] - Z[0]);
             }
         }
@@ -88,76 +111,79 @@ __global__ __launch_bounds__(kThreads) void k_diff_depth(const float* __restrict
 }
 
 struct Tap {
-    float s, t, z, cw, cx, cy, cz;
+    double z, cw, cx, cy, cz;
     int a0, b0;
-    float fu, fv;
+    double fu, fv;
 };
 
-__device__ __forceinline__ Tap project(const float* m, float x, float y, float z, const DiffCfg& c) {
-    float cc[4];
-    mat4_vec4(m, x, y, z, 1.0f, cc);
+__device__ __forceinline__ Tap project(const double m[16], double x, double y, double z, const DiffCfg& c) {
+    double cc[4];
+    mat4_vec4_d(m, x, y, z, 1.0, cc);
     Tap t;
     t.cx = cc[0]; t.cy = cc[1]; t.cz = cc[2]; t.cw = cc[3];
-    t.s = cc[0] / cc[3];
-    t.t = cc[1] / cc[3];
+    const double s = cc[0] / cc[3], tt = cc[1] / cc[3];
     t.z = cc[2] / cc[3];
-    float u = t.s * (float)c.smw - 0.5f, v = t.t * (float)c.smh - 0.5f;
-    float fa = floorf(u), fb = floorf(v);
-    t.a0 = (int)fa;
-    t.b0 = (int)fb;
+    const double u = s * (double)c.smw - 0.5, v = tt * (double)c.smh - 0.5;
+    const double fa = floor(u), fb = floor(v);
+    // clamp before the int conversion; the texel reads are bounds-checked
+    t.a0 = (int)fmin(fmax(fa, -4.0), (double)c.smw + 4.0);
+    t.b0 = (int)fmin(fmax(fb, -4.0), (double)c.smh + 4.0);
     t.fu = u - fa;
     t.fv = v - fb;
     return t;
 }
 
-__device__ __forceinline__ float texel(const float* __restrict__ d, int a, int b, const DiffCfg& c) {
-    return (a >= 0 && a < c.smw && b >= 0 && b < c.smh) ? d[(size_t)b * c.smw + a] : 0.0f;
+__device__ __forceinline__ double texel(const double* __restrict__ d, int a, int b, const DiffCfg& c) {
+    return (a >= 0 && a < c.smw && b >= 0 && b < c.smh) ? d[(size_t)b * c.smw + a] : 0.0;
 }
 
 __global__ __launch_bounds__(kThreads) void k_diff_shade(const float* __restrict__ gpos,
-                                                         const float* __restrict__ biased,
-                                                         const float* __restrict__ depth, DiffCfg c,
+                                                         const double* __restrict__ biased,
+                                                         const double* __restrict__ depth, DiffCfg c,
                                                          float* __restrict__ out) {
     const size_t plane = (size_t)c.gw * c.gh;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= plane) return;
-    const float x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
-    if (x == 0.0f) {
+    const float xf = gpos[k];
+    if (xf == 0.0f) {
         out[k] = 0.0f;
         return;
     }
+    const double x = xf, y = gpos[plane + k], z = gpos[2 * plane + k];
     const size_t msz = (size_t)c.smw * c.smh;
-    float acc = 0.0f;
+    double acc = 0.0;
     for (int i = 0; i < c.S; ++i) {
-        const Tap t = project(biased + 16 * i, x, y, z, c);
-        const float* d = depth + (size_t)i * msz;
-        const float wu[2] = {1.0f - smoothw(t.fu), smoothw(t.fu)};
-        const float wv[2] = {1.0f - smoothw(t.fv), smoothw(t.fv)};
-        float q = 0.0f;
+        double m[16];
+        load_m(biased + 16 * i, m);
+        const Tap t = project(m, x, y, z, c);
+        const double* d = depth + (size_t)i * msz;
+        const double wu[2] = {1.0 - smoothw(t.fu), smoothw(t.fu)};
+        const double wv[2] = {1.0 - smoothw(t.fv), smoothw(t.fv)};
+        double q = 0.0;
 #pragma unroll
         for (int db = 0; db < 2; ++db)
 #pragma unroll
             for (int da = 0; da < 2; ++da)
-                q += wu[da] * wv[db] * sigmoidf((texel(d, t.a0 + da, t.b0 + db, c) - t.z + c.bias) / c.tau);
-        acc += c.si + (1.0f - c.si) * q;
+                q += wu[da] * wv[db] * sigmoid_d((texel(d, t.a0 + da, t.b0 + db, c) - t.z + c.bias) / c.tau);
+        acc += c.si + (1.0 - c.si) * q;
     }
-    out[k] = acc / (float)c.S;
+    out[k] = (float)(acc / (double)c.S);
 }
 
-// Block sum of 16 per-thread values into partial[block][16] (fixed order).
-__device__ __forceinline__ void block_sum16(const float v[16], float* __restrict__ partial) {
-    __shared__ float red[kThreads / 64][16];
+// Block sum of 16 per-thread values into partial[16] (fixed order).
+__device__ __forceinline__ void block_sum16(const double v[16], double* __restrict__ partial) {
+    __shared__ double red[kThreads / 64][16];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
-        float x = v[e];
+        double x = v[e];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
         if (lane == 0) red[wave][e] = x;
     }
     __syncthreads();
     if (threadIdx.x < 16) {
-        float s = 0.0f;
+        double s = 0.0;
 #pragma unroll
         for (int w = 0; w < kThreads / 64; ++w) s += red[w][threadIdx.x];
         partial[threadIdx.x] = s;
@@ -167,47 +193,50 @@ __device__ __forceinline__ void block_sum16(const float v[16], float* __restrict
 
 // Pixel adjoint.  grid (pixel blocks, S).
 __global__ __launch_bounds__(kThreads) void k_diff_shade_bwd(const float* __restrict__ gpos,
-                                                             const float* __restrict__ biased,
-                                                             const float* __restrict__ depth,
+                                                             const double* __restrict__ biased,
+                                                             const double* __restrict__ depth,
                                                              const float* __restrict__ grad_out, DiffCfg c,
                                                              float* __restrict__ grad_depth,
-                                                             float* __restrict__ partial) {
+                                                             double* __restrict__ partial) {
     const size_t plane = (size_t)c.gw * c.gh;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
-    float gm[16];
+    double gm[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) gm[e] = 0.0f;
+    for (int e = 0; e < 16; ++e) gm[e] = 0.0;
     if (k < plane && gpos[k] != 0.0f) {
-        const float x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
-        const float gq = grad_out[k] / (float)c.S * (1.0f - c.si);
-        const Tap t = project(biased + 16 * i, x, y, z, c);
+        const double x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
+        const double gq = (double)grad_out[k] / (double)c.S * (1.0 - c.si);
+        double m[16];
+        load_m(biased + 16 * i, m);
+        const Tap t = project(m, x, y, z, c);
         const size_t msz = (size_t)c.smw * c.smh;
-        const float* d = depth + (size_t)i * msz;
-        const float su = smoothw(t.fu), sv = smoothw(t.fv);
-        const float wu[2] = {1.0f - su, su}, wv[2] = {1.0f - sv, sv};
-        const float dwu[2] = {-dsmoothw(t.fu), dsmoothw(t.fu)}, dwv[2] = {-dsmoothw(t.fv), dsmoothw(t.fv)};
-        float gz = 0.0f, gfu = 0.0f, gfv = 0.0f;
+        const double* d = depth + (size_t)i * msz;
+        const double su = smoothw(t.fu), sv = smoothw(t.fv);
+        const double wu[2] = {1.0 - su, su}, wv[2] = {1.0 - sv, sv};
+        const double dwu[2] = {-dsmoothw(t.fu), dsmoothw(t.fu)}, dwv[2] = {-dsmoothw(t.fv), dsmoothw(t.fv)};
+        double gz = 0.0, gfu = 0.0, gfv = 0.0;
 #pragma unroll
         for (int db = 0; db < 2; ++db)
 #pragma unroll
             for (int da = 0; da < 2; ++da) {
                 const int a = t.a0 + da, b = t.b0 + db;
                 const bool in = a >= 0 && a < c.smw && b >= 0 && b < c.smh;
-                const float dd = in ? d[(size_t)b * c.smw + a] : 0.0f;
-                const float sg = sigmoidf((dd - t.z + c.bias) / c.tau);
-                const float ds = sg * (1.0f - sg) / c.tau;
-                const float w = wu[da] * wv[db];
-                if (in && gq != 0.0f) unsafeAtomicAdd(&grad_depth[(size_t)i * msz + (size_t)b * c.smw + a], gq * w * ds);
+                const double dd = in ? d[(size_t)b * c.smw + a] : 0.0;
+                const double sg = sigmoid_d((dd - t.z + c.bias) / c.tau);
+                const double ds = sg * (1.0 - sg) / c.tau;
+                const double w = wu[da] * wv[db];
+                if (in && gq != 0.0)
+                    unsafeAtomicAdd(&grad_depth[(size_t)i * msz + (size_t)b * c.smw + a], (float)(gq * w * ds));
                 gz -= w * ds;
                 gfu += dwu[da] * wv[db] * sg;
                 gfv += wu[da] * dwv[db] * sg;
             }
-        const float gs = gq * gfu * (float)c.smw, gt = gq * gfv * (float)c.smh;
+        const double gs = gq * gfu * (double)c.smw, gt = gq * gfv * (double)c.smh;
         gz *= gq;
-        const float iw = 1.0f / t.cw;
-        const float gc[4] = {gs * iw, gt * iw, gz * iw, -(gs * t.cx + gt * t.cy + gz * t.cz) * iw * iw};
-        const float xp[4] = {x, y, z, 1.0f};
+        const double iw = 1.0 / t.cw;
+        const double gc[4] = {gs * iw, gt * iw, gz * iw, -(gs * t.cx + gt * t.cy + gz * t.cz) * iw * iw};
+        const double xp[4] = {x, y, z, 1.0};
 #pragma unroll
         for (int col = 0; col < 4; ++col)
 #pragma unroll
@@ -221,57 +250,56 @@ __global__ __launch_bounds__(kThreads) void k_diff_shade_bwd(const float* __rest
 // rows of texels) sums its vertex gradients across lanes and issues 9 atomics
 // instead of 9 per lane; other waves add lane by lane.
 __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __restrict__ pos, const int* __restrict__ idx,
-                                                             int ntris, const float* __restrict__ mvps,
+                                                             int ntris, const double* __restrict__ mvps,
                                                              const unsigned long long* __restrict__ vis,
                                                              const float* __restrict__ grad_depth, DiffCfg c,
                                                              float* __restrict__ grad_pos,
-                                                             float* __restrict__ partial) {
+                                                             double* __restrict__ partial) {
     const size_t msz = (size_t)c.smw * c.smh;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
-    float gm[16];
+    double gm[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) gm[e] = 0.0f;
-    float gv[3][3];
+    for (int e = 0; e < 16; ++e) gm[e] = 0.0;
+    double gv[3][3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gv[j][0] = gv[j][1] = gv[j][2] = 0.0f;
+    for (int j = 0; j < 3; ++j) gv[j][0] = gv[j][1] = gv[j][2] = 0.0;
     int tri = -1;
-    const float G = (k < msz) ? grad_depth[(size_t)i * msz + k] : 0.0f;
+    const double G = (k < msz) ? (double)grad_depth[(size_t)i * msz + k] : 0.0;
     const unsigned long long key = (k < msz) ? vis[(size_t)i * msz + k] : ~0ull;
-    if (G != 0.0f && key != ~0ull) {
+    if (G != 0.0 && key != ~0ull) {
         const int t = (int)((key & 0xffffffffull) >> 3);
-        float m[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) m[e] = mvps[16 * i + e];
-        float X[3], Y[3], Z[3], C[3][4];
+        double m[16];
+        load_m(mvps + 16 * i, m);
+        double X[3], Y[3], Z[3], C[3][4];
         if (t < ntris && tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C)) {
-            const float D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-            if (D != 0.0f) {
+            const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+            if (D != 0.0) {
                 tri = t;
-                const float xc = (float)(k % c.smw) + 0.5f, yc = (float)(k / c.smw) + 0.5f;
-                const float l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
-                const float l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
-                const float lam[3] = {1.0f - l1 - l2, l1, l2};
+                const double xc = (double)(k % c.smw) + 0.5, yc = (double)(k / c.smw) + 0.5;
+                const double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
+                const double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
+                const double lam[3] = {1.0 - l1 - l2, l1, l2};
                 // plane gradient in window x, y; d moves by -lambda_j * grad when vertex j moves
-                const float gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
-                const float gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
+                const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
+                const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    const float gX = -G * lam[j] * gx * (0.5f * c.smw);
-                    const float gY = -G * lam[j] * gy * (0.5f * c.smh);
-                    const float gZ = G * lam[j] * 0.5f;
-                    const float iw = 1.0f / C[j][3];
-                    const float gc[4] = {gX * iw, gY * iw, gZ * iw,
-                                         -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
+                    const double gX = -G * lam[j] * gx * (0.5 * c.smw);
+                    const double gY = -G * lam[j] * gy * (0.5 * c.smh);
+                    const double gZ = G * lam[j] * 0.5;
+                    const double iw = 1.0 / C[j][3];
+                    const double gc[4] = {gX * iw, gY * iw, gZ * iw,
+                                          -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
                     const int v = idx[3 * t + j];
-                    const float vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0f};
+                    const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
 #pragma unroll
                     for (int col = 0; col < 4; ++col)
 #pragma unroll
                         for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
 #pragma unroll
                     for (int a = 0; a < 3; ++a) {
-                        float g = 0.0f;
+                        double g = 0.0;
 #pragma unroll
                         for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
                         gv[j][a] = g;
@@ -291,7 +319,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __rest
             for (int j = 0; j < 3; ++j)
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    float x = gv[j][a];
+                    double x = gv[j][a];
 #pragma unroll
                     for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
                     gv[j][a] = x;
@@ -301,7 +329,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __rest
                 for (int j = 0; j < 3; ++j) {
                     const int v = idx[3 * t0 + j];
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], gv[j][a]);
+                    for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], (float)gv[j][a]);
                 }
             }
         } else if (tri >= 0) {
@@ -309,7 +337,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __rest
             for (int j = 0; j < 3; ++j) {
                 const int v = idx[3 * tri + j];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], gv[j][a]);
+                for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], (float)gv[j][a]);
             }
         }
     }
@@ -319,24 +347,24 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __rest
 // out[i][e] = sum over blocks of partial[i][block][e]: one workgroup per
 // sample, each thread sums a fixed stride of blocks, then a fixed-order block
 // reduction (the result does not depend on scheduling).
-__global__ __launch_bounds__(kThreads) void k_diff_reduce(const float* __restrict__ partial, int nblocks,
-                                                          float* __restrict__ out) {
+__global__ __launch_bounds__(kThreads) void k_diff_reduce(const double* __restrict__ partial, int nblocks,
+                                                          double* __restrict__ out) {
+    __shared__ double res[16];
     const int i = blockIdx.x;
-    float acc[16];
+    double acc[16];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc[e] = 0.0f;
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += kThreads) {
-        const float4* p = reinterpret_cast<const float4*>(partial + ((size_t)i * nblocks + b) * 16);
+        const double2* p = reinterpret_cast<const double2*>(partial + ((size_t)i * nblocks + b) * 16);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 v = p[q];
-            acc[4 * q] += v.x;
-            acc[4 * q + 1] += v.y;
-            acc[4 * q + 2] += v.z;
-            acc[4 * q + 3] += v.w;
+        for (int q = 0; q < 8; ++q) {
+            const double2 v = p[q];
+            acc[2 * q] += v.x;
+            acc[2 * q + 1] += v.y;
         }
     }
-    block_sum16(acc, out + 16 * (size_t)i);
+    block_sum16(acc, res);
+    if (threadIdx.x < 16) out[16 * (size_t)i + threadIdx.x] = res[threadIdx.x];
 }
 
 __global__ void k_fill_f32(float* p, size_t n, float v) {
@@ -347,11 +375,12 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 struct DiffWs {
+    float* mvps32;
     unsigned long long* vis;
-    float* depth;
+    double* depth;
     float* grad_depth;
-    float* part_px;
-    float* part_tx;
+    double* part_px;
+    double* part_tx;
     void* raster;
     size_t raster_bytes;
     int px_blocks, tx_blocks;
@@ -362,19 +391,21 @@ size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* w
     const int px_blocks = (int)(((size_t)gw * gh + kThreads - 1) / kThreads);
     const int tx_blocks = (int)((msz + kThreads - 1) / kThreads);
     size_t off = 0;
+    size_t o_m32 = off; off = align256(off + sizeof(float) * 16 * (size_t)S);
     size_t o_vis = off; off = align256(off + sizeof(unsigned long long) * msz * S);
-    size_t o_depth = off; off = align256(off + sizeof(float) * msz * S);
+    size_t o_depth = off; off = align256(off + sizeof(double) * msz * S);
     size_t o_gd = off; off = align256(off + sizeof(float) * msz * S);
-    size_t o_ppx = off; off = align256(off + sizeof(float) * 16 * (size_t)px_blocks * S);
-    size_t o_ptx = off; off = align256(off + sizeof(float) * 16 * (size_t)tx_blocks * S);
+    size_t o_ppx = off; off = align256(off + sizeof(double) * 16 * (size_t)px_blocks * S);
+    size_t o_ptx = off; off = align256(off + sizeof(double) * 16 * (size_t)tx_blocks * S);
     size_t rbytes = internal::vis_raster_workspace_size(ntris, S, smw, smh);
     size_t o_r = off; off = align256(off + rbytes);
     if (ws) {
+        ws->mvps32 = reinterpret_cast<float*>(base + o_m32);
         ws->vis = reinterpret_cast<unsigned long long*>(base + o_vis);
-        ws->depth = reinterpret_cast<float*>(base + o_depth);
+        ws->depth = reinterpret_cast<double*>(base + o_depth);
         ws->grad_depth = reinterpret_cast<float*>(base + o_gd);
-        ws->part_px = reinterpret_cast<float*>(base + o_ppx);
-        ws->part_tx = reinterpret_cast<float*>(base + o_ptx);
+        ws->part_px = reinterpret_cast<double*>(base + o_ppx);
+        ws->part_tx = reinterpret_cast<double*>(base + o_ptx);
         ws->raster = base + o_r;
         ws->raster_bytes = rbytes;
         ws->px_blocks = px_blocks;
@@ -383,14 +414,14 @@ size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* w
     return rbytes ? off : 0;
 }
 
-bool finite_all(const float* v, int n) {
+bool finite_all(const double* v, int n) {
     for (int i = 0; i < n; ++i)
         if (!std::isfinite(v[i])) return false;
     return true;
 }
 
-int check_common(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris, const float* mvps_dev,
-                 const float* biased_dev, int S, int smw, int smh, const gi_gbuffer* gb, const gi_diff_params* p,
+int check_common(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                 const double* mvps_dev, const double* biased_dev, int S, int smw, int smh, const gi_gbuffer* gb, const gi_diff_params* p,
                  void* ws, DiffCfg& c) {
     if (!positions_dev || (!indices_dev && ntris > 0) || !mvps_dev || !biased_dev || !gb || !gb->position_dev || !p ||
         !ws)
@@ -399,16 +430,17 @@ int check_common(const float* positions_dev, int nverts, const int32_t* indices_
     if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE || gb->width <= 0 || gb->height <= 0 ||
         gb->width > GI_MAX_EDGE || gb->height > GI_MAX_EDGE)
         return GI_EINVAL;
-    float sc[3] = {p->tau, p->bias, p->shadow_intensity};
+    double sc[3] = {p->tau, p->bias, p->shadow_intensity};
     if (!finite_all(sc, 3)) return GI_ENONFINITE;
     if (!(p->tau > 0.0f)) return GI_EINVAL;
     c = DiffCfg{smw, smh, S, gb->width, gb->height, p->tau, p->bias, p->shadow_intensity};
     return GI_OK;
 }
 
-int recompute(const float* pos, int nverts, const int32_t* idx, int ntris, const float* mvps, const DiffCfg& c,
+int recompute(const float* pos, int nverts, const int32_t* idx, int ntris, const double* mvps, const DiffCfg& c,
               const DiffWs& w, hipStream_t s) {
-    int rc = internal::vis_raster(pos, nverts, idx, ntris, mvps, c.S, c.smw, c.smh, w.vis, w.raster,
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((16 * c.S + 255) / 256)), dim3(256), 0, s, mvps, w.mvps32, 16 * c.S);
+    int rc = internal::vis_raster(pos, nverts, idx, ntris, w.mvps32, c.S, c.smw, c.smh, w.vis, w.raster,
                                   w.raster_bytes, s);
     if (rc != GI_OK) return rc;
     const size_t msz = (size_t)c.smw * c.smh;
@@ -430,7 +462,7 @@ size_t gi_diff_workspace_size(int ntris, int nsamples, int smw, int smh, int vie
 }
 
 int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
-                         const float* mvps_dev, const float* biased_mvps_dev, int nsamples, int smw, int smh,
+                         const double* mvps_dev, const double* biased_mvps_dev, int nsamples, int smw, int smh,
                          const gi_gbuffer* gbuffer, const gi_diff_params* params, float* out_dev,
                          void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
     DiffCfg c;
@@ -451,9 +483,9 @@ int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* 
 }
 
 int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
-                          const float* mvps_dev, const float* biased_mvps_dev, int nsamples, int smw, int smh,
+                          const double* mvps_dev, const double* biased_mvps_dev, int nsamples, int smw, int smh,
                           const gi_gbuffer* gbuffer, const gi_diff_params* params, const float* grad_out_dev,
-                          float* grad_biased_mvps_dev, float* grad_mvps_dev, float* grad_positions_dev,
+                          double* grad_biased_mvps_dev, double* grad_mvps_dev, float* grad_positions_dev,
                           void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
     DiffCfg c;
     int rc = check_common(positions_dev, nverts, indices_dev, ntris, mvps_dev, biased_mvps_dev, nsamples, smw, smh,

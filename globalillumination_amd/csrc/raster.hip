@@ -40,8 +40,10 @@
 
 namespace gi {
 
+// Tile edge of the binned raster: a template parameter of every tile-shaped
+// kernel.  The single-instance passes use kTile; the batched path picks 32 or
+// 64 per batch.
 constexpr int kTile = 64;
-constexpr int kTilePx = kTile * kTile;
 constexpr int kRasterThreads = 256;
 constexpr float kGuard = 4.0f;
 constexpr int kMaxPoly = 9;
@@ -429,12 +431,12 @@ __device__ __forceinline__ void push_record(const BinBufs& b, int map, const Tri
 // and device-scope atomics on the same line serialize outside the XCD's L2.
 constexpr int kBinHist = 4096;  // largest tile grid aggregated in LDS (64 x 64 tiles)
 
-template <bool kScatter>
-__global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
+template <int TS, bool kScatter>
+__device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
+                                         const BinBufs& b, int bx, int map) {
     __shared__ uint32_t hist[kBinHist];
     const int tid = threadIdx.x;
-    const int tri = blockIdx.x * blockDim.x + tid;
-    const int map = blockIdx.y;
+    const int tri = bx * kRasterThreads + tid;
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     const bool lds = ntiles <= kBinHist;
     if (kScatter && b.overflow[map]) return;  // uniform per workgroup
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
                 else b.wide_over[map] = 1;
             }
         } else if (ok) {
-            int tx0 = t.px0 / kTile, tx1 = t.px1 / kTile, ty0 = t.py0 / kTile, ty1 = t.py1 / kTile;
+            int tx0 = t.px0 / TS, tx1 = t.px1 / TS, ty0 = t.py0 / TS, ty1 = t.py1 / TS;
             int ntx = tx1 - tx0 + 1;
             int span = ntx * (ty1 - ty0 + 1);
             if (span > kWideSpan) {
@@ -470,8 +472,8 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
                     if (k >= span) continue;
                     int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
                     if (span > 1) {
-                        int x0 = max(tx * kTile, t.px0), x1 = min(tx * kTile + kTile - 1, t.px1);
-                        int y0 = max(ty * kTile, t.py0), y1 = min(ty * kTile + kTile - 1, t.py1);
+                        int x0 = max(tx * TS, t.px0), x1 = min(tx * TS + TS - 1, t.px1);
+                        int y0 = max(ty * TS, t.py0), y1 = min(ty * TS + TS - 1, t.py1);
                         if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
                     }
                     tile_of[k] = ty * cfg.tiles_x + tx;
@@ -508,14 +510,20 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
     }
 }
 
+template <int TS, bool kScatter>
+__global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
+    bin_body<TS, kScatter>(mesh, mvps, cfg, b, blockIdx.x, blockIdx.y);
+}
+
 // Clips the listed triangles; every non-empty fan piece becomes a setup
-// record.  Runs between the count passes, before k_bin_wide.
-__global__ __launch_bounds__(256) void k_clip(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
-    const int map = blockIdx.y;
+// record.  Runs between the count passes, before k_bin_wide.  Workgroup
+// `part` of `nparts` takes every nparts-th stride of the map's list.
+__device__ __forceinline__ void clip_body(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
+                                          const BinBufs& b, int part, int nparts, int map) {
     const uint32_t n = min(b.nclip[map], b.wcap);
     float mvp[16];
     mvps.load(map, mvp);
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+    for (uint32_t c = part * kRasterThreads + threadIdx.x; c < n; c += nparts * kRasterThreads) {
         int tri = (int)b.clip[(size_t)map * b.wcap + c];
         CV<0> v[3];
         load_tri<0>(mesh, mvp, tri, v);
@@ -529,33 +537,40 @@ __global__ __launch_bounds__(256) void k_clip(MeshView mesh, MvpSource mvps, Ras
     }
 }
 
+__global__ __launch_bounds__(256) void k_clip(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
+    clip_body(mesh, mvps, cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
 // Wide pieces: a workgroup per record, its threads walk the record's tiles.
-template <bool kScatter>
-__global__ __launch_bounds__(256) void k_bin_wide(RasterCfg cfg, BinBufs b) {
-    const int map = blockIdx.y;
+template <int TS, bool kScatter>
+__device__ __forceinline__ void bin_wide_body(const RasterCfg& cfg, const BinBufs& b, int part, int nparts, int map) {
     if (kScatter && b.overflow[map]) return;
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     const uint32_t n = min(b.nwide[map], b.wcap);
-    for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    for (uint32_t w = part; w < n; w += nparts) {
         const Setup s = b.wide[(size_t)map * b.wcap + w];
-        int tx0 = s.px0 / kTile, tx1 = s.px1 / kTile, ty0 = s.py0 / kTile, ty1 = s.py1 / kTile;
+        int tx0 = s.px0 / TS, tx1 = s.px1 / TS, ty0 = s.py0 / TS, ty1 = s.py1 / TS;
         int ntx = tx1 - tx0 + 1;
         int span = ntx * (ty1 - ty0 + 1);
         for (int q = threadIdx.x; q < span; q += blockDim.x) {
             int tx = tx0 + q % ntx, ty = ty0 + q / ntx;
-            int x0 = max(tx * kTile, s.px0), x1 = min(tx * kTile + kTile - 1, s.px1);
-            int y0 = max(ty * kTile, s.py0), y1 = min(ty * kTile + kTile - 1, s.py1);
+            int x0 = max(tx * TS, s.px0), x1 = min(tx * TS + TS - 1, s.px1);
+            int y0 = max(ty * TS, s.py0), y1 = min(ty * TS + TS - 1, s.py1);
             if (!rect_overlap_s(s, x0, x1, y0, y1)) continue;
             bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, kWideBit | w, kScatter);
         }
     }
 }
 
+template <int TS, bool kScatter>
+__global__ __launch_bounds__(256) void k_bin_wide(RasterCfg cfg, BinBufs b) {
+    bin_wide_body<TS, kScatter>(cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
+}
+
 // Exclusive scan of one map's tile counts; resets counts to 0 (they become
 // the scatter cursors) and flags maps whose lists exceed the workspace.
-__global__ __launch_bounds__(kScanThreads) void k_scan(BinBufs b, int ntiles) {
+__device__ __forceinline__ void scan_body(const BinBufs& b, int ntiles, int map) {
     __shared__ unsigned long long part[kScanThreads];
-    int map = blockIdx.x;
     uint32_t* c = b.counts + (size_t)map * ntiles;
     uint32_t* o = b.offsets + (size_t)map * (ntiles + 1);
     int per = (ntiles + kScanThreads - 1) / kScanThreads;
@@ -583,6 +598,8 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(BinBufs b, int ntiles) {
         b.overflow[map] = over ? 1 : 0;
     }
 }
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(BinBufs b, int ntiles) { scan_body(b, ntiles, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 // Tile raster
@@ -647,7 +664,7 @@ __device__ __forceinline__ void plot(const Setup& s, const E e[3], int i, int j,
 // One thread covers the whole (small) rectangle, row by row.  When every
 // edge value inside the triangle's bounding box fits in 32 bits, the exact
 // stepping runs in int32.
-template <bool kVis, typename E>
+template <int TS, bool kVis, typename E>
 __device__ __forceinline__ void raster_rect_self_t(const Setup& s, const Edges& E64, int x0, int y0, int rx0, int rx1,
                                                    int ry0, int ry1, typename KeyOf<kVis>::T* tile) {
     E row[3], dx[3], dy[3];
@@ -660,7 +677,7 @@ __device__ __forceinline__ void raster_rect_self_t(const Setup& s, const Edges& 
     for (int j = ry0; j <= ry1; ++j) {
         E e[3] = {row[0], row[1], row[2]};
         for (int i = rx0; i <= rx1; ++i) {
-            plot<kVis, E>(s, e, i, j, (j - y0) * kTile + (i - x0), tile);
+            plot<kVis, E>(s, e, i, j, (j - y0) * TS + (i - x0), tile);
 #pragma unroll
             for (int k = 0; k < 3; ++k) e[k] += dx[k];
         }
@@ -669,7 +686,7 @@ __device__ __forceinline__ void raster_rect_self_t(const Setup& s, const Edges& 
     }
 }
 
-template <bool kVis>
+template <int TS, bool kVis>
 __device__ __forceinline__ void raster_rect_self(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
                                                  typename KeyOf<kVis>::T* tile) {
     Edges E;
@@ -679,27 +696,27 @@ __device__ __forceinline__ void raster_rect_self(const Setup& s, int x0, int y0,
     // |e| <= 2*bw*bh inside the box and one step past it adds at most
     // 256*max(bw, bh), so with these bounds every value stays below 2^30.
     if (bw * bh < (1ll << 28) && bw < (1 << 20) && bh < (1 << 20))
-        raster_rect_self_t<kVis, int>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
+        raster_rect_self_t<TS, kVis, int>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
     else
-        raster_rect_self_t<kVis, long long>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
+        raster_rect_self_t<TS, kVis, long long>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
 }
 
-// The workgroup covers the rectangle: thread t owns tile column t % 32 and
-// every (kRasterThreads / 32)-th row from row t / 32.
-template <bool kVis>
+// The workgroup covers the rectangle: thread t owns tile column t % TS and
+// every (kRasterThreads / TS)-th row from row t / TS.
+template <int TS, bool kVis>
 __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
                                                   int tid, typename KeyOf<kVis>::T* tile) {
-    constexpr int kRowStep = kRasterThreads / kTile;
-    const int i = x0 + (tid % kTile);
+    constexpr int kRowStep = kRasterThreads / TS;
+    const int i = x0 + (tid % TS);
     if (i < rx0 || i > rx1) return;
-    int j = y0 + tid / kTile;
+    int j = y0 + tid / TS;
     if (j < ry0) j += ((ry0 - j + kRowStep - 1) / kRowStep) * kRowStep;
     if (j > ry1) return;
     Edges E;
     edges_at(s, i, j, E);
     long long e[3] = {E.e[0], E.e[1], E.e[2]};
     for (; j <= ry1; j += kRowStep) {
-        plot<kVis, long long>(s, e, i, j, (j - y0) * kTile + (i - x0), tile);
+        plot<kVis, long long>(s, e, i, j, (j - y0) * TS + (i - x0), tile);
 #pragma unroll
         for (int k = 0; k < 3; ++k) e[k] += E.dy[k] * kRowStep;
     }
@@ -707,14 +724,15 @@ __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0
 
 // One (tile, map) of the raster.  kFallback: the map's lists overflowed the
 // workspace, so every (triangle, piece) pair is visited with full clipping.
-template <bool kVis, bool kFallback>
+template <int TS, bool kVis, bool kFallback>
 __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
                                             const BinBufs& b, void* out, int map, int tile_id,
                                             typename KeyOf<kVis>::T* tile, Setup* rec, int& nrec) {
     const int tid = threadIdx.x;
     const int tx = tile_id % cfg.tiles_x, ty = tile_id / cfg.tiles_x;
-    const int x0 = tx * kTile, y0 = ty * kTile;
-    const int x1 = min(x0 + kTile - 1, cfg.w - 1), y1 = min(y0 + kTile - 1, cfg.h - 1);
+    constexpr int kTilePx = TS * TS;
+    const int x0 = tx * TS, y0 = ty * TS;
+    const int x1 = min(x0 + TS - 1, cfg.w - 1), y1 = min(y0 + TS - 1, cfg.h - 1);
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     for (int p = tid; p < kTilePx; p += kRasterThreads) tile[p] = KeyOf<kVis>::clear;
     if (tid == 0) nrec = 0;
@@ -739,7 +757,7 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
                     rec[slot] = w;
                 } else {
                     int rx0 = max(x0, w.px0), rx1 = min(x1, w.px1), ry0 = max(y0, w.py0), ry1 = min(y1, w.py1);
-                    if (rx0 <= rx1 && ry0 <= ry1) raster_rect_self<kVis>(w, x0, y0, rx0, rx1, ry0, ry1, tile);
+                    if (rx0 <= rx1 && ry0 <= ry1) raster_rect_self<TS, kVis>(w, x0, y0, rx0, rx1, ry0, ry1, tile);
                 }
             } else {
                 Tri<0> t;
@@ -755,7 +773,7 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
                         int slot = kRecQueue;
                         if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kSelfRasterMaxPx) slot = atomicAdd(&nrec, 1);
                         if (slot < kRecQueue) rec[slot] = s;
-                        else raster_rect_self<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
+                        else raster_rect_self<TS, kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
                     }
                 }
             }
@@ -766,7 +784,7 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
             const Setup s = rec[q];
             int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
             if (rx0 > rx1 || ry0 > ry1) continue;
-            raster_rect_group<kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, tile);
+            raster_rect_group<TS, kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, tile);
         }
         __syncthreads();
         if (tid == 0) nrec = 0;
@@ -776,22 +794,22 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
     const size_t plane = (size_t)cfg.w * cfg.h;
     if constexpr (!kVis) {
         float* dst = reinterpret_cast<float*>(out) + (size_t)map * plane;
-        const bool full = (x1 - x0 + 1 == kTile) && ((cfg.w & 3) == 0);
+        const bool full = (x1 - x0 + 1 == TS) && ((cfg.w & 3) == 0);
         if (full) {
             for (int q = tid; q < kTilePx / 4; q += kRasterThreads) {
-                int r = q / (kTile / 4), cidx = (q % (kTile / 4)) * 4;
+                int r = q / (TS / 4), cidx = (q % (TS / 4)) * 4;
                 int j = y0 + r;
                 if (j > y1) continue;
                 float4 v;
-                v.x = __uint_as_float(tile[r * kTile + cidx + 0]);
-                v.y = __uint_as_float(tile[r * kTile + cidx + 1]);
-                v.z = __uint_as_float(tile[r * kTile + cidx + 2]);
-                v.w = __uint_as_float(tile[r * kTile + cidx + 3]);
+                v.x = __uint_as_float(tile[r * TS + cidx + 0]);
+                v.y = __uint_as_float(tile[r * TS + cidx + 1]);
+                v.z = __uint_as_float(tile[r * TS + cidx + 2]);
+                v.w = __uint_as_float(tile[r * TS + cidx + 3]);
                 *reinterpret_cast<float4*>(dst + (size_t)j * cfg.w + x0 + cidx) = v;
             }
         } else {
             for (int p = tid; p < kTilePx; p += kRasterThreads) {
-                int i = x0 + p % kTile, j = y0 + p / kTile;
+                int i = x0 + p % TS, j = y0 + p / TS;
                 if (i > x1 || j > y1) continue;
                 dst[(size_t)j * cfg.w + i] = __uint_as_float(tile[p]);
             }
@@ -799,7 +817,7 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
     } else {
         unsigned long long* dst = reinterpret_cast<unsigned long long*>(out) + (size_t)map * plane;
         for (int p = tid; p < kTilePx; p += kRasterThreads) {
-            int i = x0 + p % kTile, j = y0 + p / kTile;
+            int i = x0 + p % TS, j = y0 + p / TS;
             if (i > x1 || j > y1) continue;
             dst[(size_t)j * cfg.w + i] = tile[p];
         }
@@ -808,27 +826,27 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
 
 // Binned raster: one workgroup per (tile, map); overflowed maps are left to
 // k_raster_fallback.
-template <bool kVis>
+template <int TS, bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
                                                            void* out) {
-    __shared__ typename KeyOf<kVis>::T tile[kTilePx];
+    __shared__ typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     const int map = blockIdx.y;
     if (b.overflow[map]) return;
-    raster_tile<kVis, false>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
+    raster_tile<TS, kVis, false>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
 }
 
 // One workgroup per tile, looping over the maps whose lists overflowed.
-template <bool kVis>
+template <int TS, bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mesh, MvpSource mvps, RasterCfg cfg,
                                                                     BinBufs b, void* out, int nmaps) {
-    __shared__ typename KeyOf<kVis>::T tile[kTilePx];
+    __shared__ typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     for (int map = 0; map < nmaps; ++map) {
         if (!b.overflow[map]) continue;
-        raster_tile<kVis, true>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
+        raster_tile<TS, kVis, true>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
         __syncthreads();
     }
 }
@@ -902,8 +920,8 @@ uint32_t entry_cap(int ntris, int ntiles) {
 
 uint32_t wide_cap(int ntris) { return 1024u + (uint32_t)ntris / 16u; }
 
-size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char* base) {
-    int tx = (w + kTile - 1) / kTile, ty = (h + kTile - 1) / kTile;
+size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char* base, int ts = kTile) {
+    int tx = (w + ts - 1) / ts, ty = (h + ts - 1) / ts;
     size_t ntiles = (size_t)tx * ty;
     uint32_t cap = entry_cap(ntris, (int)ntiles);
     uint32_t wcap = wide_cap(ntris);
@@ -964,20 +982,21 @@ int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Ras
     dim3 gb((mesh.nt + 255) / 256, nmaps);
     dim3 gw(kWideBlocks, nmaps);
     if (mesh.nt > 0) {
-        hipLaunchKernelGGL(k_bin<false>, gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        hipLaunchKernelGGL((k_bin<kTile, false>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
         hipLaunchKernelGGL(k_clip, dim3(kClipBlocks, nmaps), dim3(256), 0, s, mesh, mvps, cfg, b);
-        hipLaunchKernelGGL(k_bin_wide<false>, gw, dim3(256), 0, s, cfg, b);
+        hipLaunchKernelGGL((k_bin_wide<kTile, false>), gw, dim3(256), 0, s, cfg, b);
     }
     hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, b, ntiles);
     if (mesh.nt > 0) {
-        hipLaunchKernelGGL(k_bin<true>, gb, dim3(256), 0, s, mesh, mvps, cfg, b);
-        hipLaunchKernelGGL(k_bin_wide<true>, gw, dim3(256), 0, s, cfg, b);
+        hipLaunchKernelGGL((k_bin<kTile, true>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        hipLaunchKernelGGL((k_bin_wide<kTile, true>), gw, dim3(256), 0, s, cfg, b);
     }
     if (ev_begin && hipEventRecord(ev_begin, s) != hipSuccess) return GI_EDEVICE;
-    hipLaunchKernelGGL(k_raster<kVis>, dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out);
+    hipLaunchKernelGGL((k_raster<kTile, kVis>), dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b,
+                       out);
     if (ev_end && hipEventRecord(ev_end, s) != hipSuccess) return GI_EDEVICE;
-    hipLaunchKernelGGL(k_raster_fallback<kVis>, dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b, out,
-                       nmaps);
+    hipLaunchKernelGGL((k_raster_fallback<kTile, kVis>), dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b,
+                       out, nmaps);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

@@ -135,15 +135,15 @@ class DiffScene:
         return len(self.light.offsets)
 
     def device_matrices(self, L, r, device):
-        """FP32 column-major (unbiased, biased) matrices of every sample on
+        """FP64 column-major (unbiased, biased) matrices of every sample on
         `device`; the last pair is cached."""
         key = (tuple(np.asarray(L, np.float64).tolist()), float(r), str(device))
         if self._mcache is not None and self._mcache[0] == key:
             return self._mcache[1], self._mcache[2]
         M = self.light.matrices(np.asarray(L, np.float64), float(r))
         Mb = np.einsum("ij,njk->nik", BIAS, M)
-        mv = torch.from_numpy(np.stack([colmajor(m) for m in M]).astype(np.float32)).to(device)
-        bv = torch.from_numpy(np.stack([colmajor(m) for m in Mb]).astype(np.float32)).to(device)
+        mv = torch.from_numpy(np.stack([colmajor(m) for m in M])).to(device)
+        bv = torch.from_numpy(np.stack([colmajor(m) for m in Mb])).to(device)
         self._mcache = (key, mv, bv)
         return mv, bv
 
@@ -176,8 +176,8 @@ class DiffScene:
         mv, bv = self.device_matrices(L, r, dev)
         v = vertices.detach().contiguous()
         go = grad_out.detach().contiguous().float()
-        gb_m = torch.empty((self.S, 16), dtype=torch.float32, device=dev)
-        g_m = torch.empty((self.S, 16), dtype=torch.float32, device=dev)
+        gb_m = torch.empty((self.S, 16), dtype=torch.float64, device=dev)
+        g_m = torch.empty((self.S, 16), dtype=torch.float64, device=dev)
         gv = torch.zeros_like(v)
         ws = self._workspace(dev)
         g = self.gbuffer.to_c(with_normals=False)
@@ -191,8 +191,8 @@ class DiffScene:
     def light_gradient(self, L, r, gb_m: torch.Tensor, g_m: torch.Tensor) -> np.ndarray:
         """dL/d(Lx, Ly, Lz, r) in float64 from the matrix adjoints:
         dL/dM'_i = dL/dM'_i (depth maps) + B^T dL/d(B M'_i) (receivers)."""
-        Gb = gb_m.double().cpu().numpy().reshape(self.S, 4, 4).transpose(0, 2, 1)
-        Gm = g_m.double().cpu().numpy().reshape(self.S, 4, 4).transpose(0, 2, 1)
+        Gb = gb_m.cpu().numpy().reshape(self.S, 4, 4).transpose(0, 2, 1)
+        Gm = g_m.cpu().numpy().reshape(self.S, 4, 4).transpose(0, 2, 1)
         G = Gm + np.einsum("ji,njk->nik", BIAS, Gb)
         J = self.light.jacobian(np.asarray(L, np.float64), float(r))
         return np.einsum("nij,nijk->k", G, J)

@@ -208,32 +208,35 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
  * depth of the triangle covering each texel (visibility IDs held fixed).
  * Gradients are those of this model at fixed triangle visibility. */
 typedef struct gi_diff_params {
-    float tau;              /* sigmoid width, window-depth units */
-    float bias;             /* depth bias added before the sigmoid */
-    float shadow_intensity; /* si */
+    double tau;              /* sigmoid width, window-depth units */
+    double bias;             /* depth bias added before the sigmoid */
+    double shadow_intensity; /* si */
 } gi_diff_params;
 
 size_t gi_diff_workspace_size(int ntris, int nsamples, int smw, int smh, int view_w, int view_h);
 
 /* mvps_dev: nsamples x 16 unbiased P*V of the samples (depth maps);
- * biased_mvps_dev: nsamples x 16 B*P*V (receivers).  out_dev: view pixels. */
+ * biased_mvps_dev: nsamples x 16 B*P*V (receivers); both FP64 column-major
+ * (the geometry is evaluated in FP64; the visibility raster that fixes the
+ * triangle of every texel uses their FP32 rounding).  out_dev: view pixels. */
 int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
-                         const float* mvps_dev, const float* biased_mvps_dev, int nsamples, int smw, int smh,
+                         const double* mvps_dev, const double* biased_mvps_dev, int nsamples, int smw, int smh,
                          const gi_gbuffer* gbuffer, const gi_diff_params* params, float* out_dev,
                          void* workspace_dev, size_t workspace_bytes, gi_stream stream);
 
 /* Adjoint of gi_diff_soft_forward for dL/dout = grad_out_dev.  Recomputes the
  * visibility raster and texel depths (nothing is kept from the forward pass).
  * Writes dL/d(B*P*V_i) (receiver projection) and dL/d(P*V_i) (depth maps),
- * nsamples x 16 each, column-major; ADDS dL/dposition into grad_positions_dev
- * (nverts x 3, zeroed by the caller).  The workspace and grad_positions_dev
- * take hardware float atomics: ordinary device memory (hipMalloc), not
- * fine-grained host-coherent memory.  The vertex sums are order-dependent in
- * the last bits; the matrix gradients are reduced in a fixed order. */
+ * nsamples x 16 FP64 each, column-major; ADDS dL/dposition into
+ * grad_positions_dev (nverts x 3, zeroed by the caller).  The workspace and
+ * grad_positions_dev take hardware float atomics: ordinary device memory
+ * (hipMalloc), not fine-grained host-coherent memory.  The vertex sums are
+ * order-dependent in the last bits; the matrix gradients are reduced in a
+ * fixed order. */
 int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
-                          const float* mvps_dev, const float* biased_mvps_dev, int nsamples, int smw, int smh,
+                          const double* mvps_dev, const double* biased_mvps_dev, int nsamples, int smw, int smh,
                           const gi_gbuffer* gbuffer, const gi_diff_params* params, const float* grad_out_dev,
-                          float* grad_biased_mvps_dev, float* grad_mvps_dev, float* grad_positions_dev,
+                          double* grad_biased_mvps_dev, double* grad_mvps_dev, float* grad_positions_dev,
                           void* workspace_dev, size_t workspace_bytes, gi_stream stream);
 
 #ifdef __cplusplus
