@@ -245,103 +245,136 @@ __global__ __launch_bounds__(kThreads) void k_diff_shade_bwd(const float* __rest
     block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
 }
 
-// Texel adjoint.  grid (texel blocks, S).  A wave whose active texels all
-// belong to one triangle (large triangles such as a ground plane cover whole
-// rows of texels) sums its vertex gradients across lanes and issues 9 atomics
-// instead of 9 per lane; other waves add lane by lane.
+// Texel adjoint.  grid (texel blocks, S); each thread takes kTexPerThread
+// texels (strided by the block size, so loads stay coalesced).
+//
+// Vertex gradients are scatter-adds, and a large triangle (a ground plane
+// covers most texels of every map) would send every wave's adds to the same
+// three vertices, one cache line that device-scope atomics from all XCDs then
+// serialize on.  So: a wave whose active texels all belong to one triangle
+// sums its vertex gradients across lanes, and lane 0 keeps a running sum while
+// consecutive texel rows stay on that triangle, flushing once when the
+// triangle changes; the flushes of block b go to copy b % ncopies of the
+// vertex-gradient array, and k_diff_vertex_reduce adds the copies in a fixed
+// order.  Waves spanning several triangles add lane by lane.
+constexpr int kTexPerThread = 4;
+
+__device__ __forceinline__ void flush_vertex(const int* __restrict__ idx, int tri, const double acc[9],
+                                             float* __restrict__ dst) {
+    if (tri < 0) return;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int v = idx[3 * tri + j];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&dst[3 * v + a], (float)acc[3 * j + a]);
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __restrict__ pos, const int* __restrict__ idx,
-                                                             int ntris, const double* __restrict__ mvps,
+                                                             int ntris, int nverts, const double* __restrict__ mvps,
                                                              const unsigned long long* __restrict__ vis,
                                                              const float* __restrict__ grad_depth, DiffCfg c,
-                                                             float* __restrict__ grad_pos,
+                                                             float* __restrict__ copies, int ncopies,
                                                              double* __restrict__ partial) {
     const size_t msz = (size_t)c.smw * c.smh;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
+    float* __restrict__ dst = copies + (size_t)(blockIdx.x % ncopies) * nverts * 3;
+    double m[16];
+    load_m(mvps + 16 * i, m);
     double gm[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) gm[e] = 0.0;
-    double gv[3][3];
+    const int lane = threadIdx.x % 64;
+    int cur = -1;  // lane 0: triangle of the running sum
+    double run[9];
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gv[j][0] = gv[j][1] = gv[j][2] = 0.0;
-    int tri = -1;
-    const double G = (k < msz) ? (double)grad_depth[(size_t)i * msz + k] : 0.0;
-    const unsigned long long key = (k < msz) ? vis[(size_t)i * msz + k] : ~0ull;
-    if (G != 0.0 && key != ~0ull) {
-        const int t = (int)((key & 0xffffffffull) >> 3);
-        double m[16];
-        load_m(mvps + 16 * i, m);
-        double X[3], Y[3], Z[3], C[3][4];
-        if (t < ntris && tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C)) {
-            const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-            if (D != 0.0) {
-                tri = t;
-                const double xc = (double)(k % c.smw) + 0.5, yc = (double)(k / c.smw) + 0.5;
-                const double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
-                const double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
-                const double lam[3] = {1.0 - l1 - l2, l1, l2};
-                // plane gradient in window x, y; d moves by -lambda_j * grad when vertex j moves
-                const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
-                const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
+    for (int e = 0; e < 9; ++e) run[e] = 0.0;
+    for (int r = 0; r < kTexPerThread; ++r) {
+        const size_t k = ((size_t)blockIdx.x * kTexPerThread + r) * kThreads + threadIdx.x;
+        double gv[9];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double gX = -G * lam[j] * gx * (0.5 * c.smw);
-                    const double gY = -G * lam[j] * gy * (0.5 * c.smh);
-                    const double gZ = G * lam[j] * 0.5;
-                    const double iw = 1.0 / C[j][3];
-                    const double gc[4] = {gX * iw, gY * iw, gZ * iw,
-                                          -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
-                    const int v = idx[3 * t + j];
-                    const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
+        for (int e = 0; e < 9; ++e) gv[e] = 0.0;
+        int tri = -1;
+        const double G = (k < msz) ? (double)grad_depth[(size_t)i * msz + k] : 0.0;
+        const unsigned long long key = (k < msz) ? vis[(size_t)i * msz + k] : ~0ull;
+        if (G != 0.0 && key != ~0ull) {
+            const int t = (int)((key & 0xffffffffull) >> 3);
+            double X[3], Y[3], Z[3], C[3][4];
+            if (t < ntris && tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C)) {
+                const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+                if (D != 0.0) {
+                    tri = t;
+                    const double xc = (double)(k % c.smw) + 0.5, yc = (double)(k / c.smw) + 0.5;
+                    const double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
+                    const double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
+                    const double lam[3] = {1.0 - l1 - l2, l1, l2};
+                    // plane gradient in window x, y; d moves by -lambda_j * grad when vertex j moves
+                    const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
+                    const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
 #pragma unroll
-                    for (int col = 0; col < 4; ++col)
+                    for (int j = 0; j < 3; ++j) {
+                        const double gX = -G * lam[j] * gx * (0.5 * c.smw);
+                        const double gY = -G * lam[j] * gy * (0.5 * c.smh);
+                        const double gZ = G * lam[j] * 0.5;
+                        const double iw = 1.0 / C[j][3];
+                        const double gc[4] = {gX * iw, gY * iw, gZ * iw,
+                                              -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
+                        const int v = idx[3 * t + j];
+                        const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
 #pragma unroll
-                        for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
+                        for (int col = 0; col < 4; ++col)
 #pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        double g = 0.0;
+                            for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
 #pragma unroll
-                        for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
-                        gv[j][a] = g;
+                        for (int a = 0; a < 3; ++a) {
+                            double g = 0.0;
+#pragma unroll
+                            for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
+                            gv[3 * j + a] = g;
+                        }
                     }
                 }
             }
         }
-    }
-    // every lane of the wave reaches this point
-    const unsigned long long act = __ballot(tri >= 0);
-    if (act != 0ull) {
+        // every lane of the wave reaches this point
+        const unsigned long long act = __ballot(tri >= 0);
+        if (act == 0ull) continue;
         const int first = __ffsll((long long)act) - 1;
         const int t0 = __shfl(tri, first, 64);
-        const bool uniform = __ballot(tri >= 0 && tri != t0) == 0ull;
-        if (uniform) {
+        if (__ballot(tri >= 0 && tri != t0) == 0ull) {
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+            for (int e = 0; e < 9; ++e) {
+                double x = gv[e];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    double x = gv[j][a];
+                for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+                gv[e] = x;
+            }
+            if (lane == 0) {
+                if (cur != t0) {
+                    flush_vertex(idx, cur, run, dst);
+                    cur = t0;
 #pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-                    gv[j][a] = x;
+                    for (int e = 0; e < 9; ++e) run[e] = 0.0;
                 }
-            if ((threadIdx.x % 64) == 0) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const int v = idx[3 * t0 + j];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], (float)gv[j][a]);
-                }
+                for (int e = 0; e < 9; ++e) run[e] += gv[e];
             }
         } else if (tri >= 0) {
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int v = idx[3 * tri + j];
-#pragma unroll
-                for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], (float)gv[j][a]);
-            }
+            flush_vertex(idx, tri, gv, dst);
         }
     }
+    if (lane == 0) flush_vertex(idx, cur, run, dst);
     block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
+}
+
+// grad_pos[e] += sum over copies c of copies[c][e], in copy order.
+__global__ void k_diff_vertex_reduce(const float* __restrict__ copies, int ncopies, size_t n,
+                                     float* __restrict__ grad_pos) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.0f;
+    for (int c = 0; c < ncopies; ++c) s += copies[(size_t)c * n + e];
+    grad_pos[e] += s;
 }
 
 // out[i][e] = sum over blocks of partial[i][block][e]: one workgroup per
@@ -374,7 +407,20 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+// Copies of the vertex-gradient array the texel adjoint spreads its adds
+// over (capped so the copies stay within kCopyBudget bytes).
+constexpr int kVertCopies = 64;
+constexpr size_t kCopyBudget = size_t(256) << 20;
+
+int vert_copies(int nverts) {
+    size_t per = (size_t)nverts * 3 * sizeof(float);
+    size_t n = per ? kCopyBudget / per : kVertCopies;
+    return (int)(n < 1 ? 1 : (n > (size_t)kVertCopies ? kVertCopies : n));
+}
+
 struct DiffWs {
+    float* copies;
+    int ncopies;
     float* mvps32;
     unsigned long long* vis;
     double* depth;
@@ -386,11 +432,13 @@ struct DiffWs {
     int px_blocks, tx_blocks;
 };
 
-size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* ws, char* base) {
+size_t diff_layout(int nverts, int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* ws, char* base) {
     const size_t msz = (size_t)smw * smh;
     const int px_blocks = (int)(((size_t)gw * gh + kThreads - 1) / kThreads);
-    const int tx_blocks = (int)((msz + kThreads - 1) / kThreads);
+    const int tx_blocks = (int)((msz + (size_t)kThreads * kTexPerThread - 1) / ((size_t)kThreads * kTexPerThread));
+    const int ncopies = vert_copies(nverts);
     size_t off = 0;
+    size_t o_cp = off; off = align256(off + sizeof(float) * 3 * (size_t)nverts * ncopies);
     size_t o_m32 = off; off = align256(off + sizeof(float) * 16 * (size_t)S);
     size_t o_vis = off; off = align256(off + sizeof(unsigned long long) * msz * S);
     size_t o_depth = off; off = align256(off + sizeof(double) * msz * S);
@@ -400,6 +448,8 @@ size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* w
     size_t rbytes = internal::vis_raster_workspace_size(ntris, S, smw, smh);
     size_t o_r = off; off = align256(off + rbytes);
     if (ws) {
+        ws->copies = reinterpret_cast<float*>(base + o_cp);
+        ws->ncopies = ncopies;
         ws->mvps32 = reinterpret_cast<float*>(base + o_m32);
         ws->vis = reinterpret_cast<unsigned long long*>(base + o_vis);
         ws->depth = reinterpret_cast<double*>(base + o_depth);
@@ -456,9 +506,12 @@ using namespace gi;
 
 extern "C" {
 
-size_t gi_diff_workspace_size(int ntris, int nsamples, int smw, int smh, int view_w, int view_h) {
-    if (ntris < 0 || nsamples <= 0 || smw <= 0 || smh <= 0 || view_w <= 0 || view_h <= 0) return 0;
-    return diff_layout(ntris, nsamples, smw, smh, view_w, view_h, nullptr, nullptr);
+size_t gi_diff_workspace_size(int nverts, int ntris, int nsamples, int smw, int smh, int view_w, int view_h) {
+    if (nverts <= 0 || ntris < 0 || nsamples <= 0 || nsamples > GI_MAX_SAMPLES || smw <= 0 || smh <= 0 ||
+        smw > GI_MAX_EDGE || smh > GI_MAX_EDGE || view_w <= 0 || view_h <= 0 || view_w > GI_MAX_EDGE ||
+        view_h > GI_MAX_EDGE)
+        return 0;
+    return diff_layout(nverts, ntris, nsamples, smw, smh, view_w, view_h, nullptr, nullptr);
 }
 
 int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
@@ -471,7 +524,8 @@ int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* 
     if (rc != GI_OK) return rc;
     if (!out_dev) return GI_EINVAL;
     DiffWs w;
-    size_t need = diff_layout(ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
+    size_t need =
+        diff_layout(nverts, ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
     if (need == 0) return GI_ENOTSUP;
     if (workspace_bytes < need) return GI_ENOSPACE;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -493,7 +547,8 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
     if (rc != GI_OK) return rc;
     if (!grad_out_dev || !grad_biased_mvps_dev || !grad_mvps_dev || !grad_positions_dev) return GI_EINVAL;
     DiffWs w;
-    size_t need = diff_layout(ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
+    size_t need =
+        diff_layout(nverts, ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
     if (need == 0) return GI_ENOTSUP;
     if (workspace_bytes < need) return GI_ENOSPACE;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -504,8 +559,12 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
     hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(kThreads), 0, s, w.grad_depth, msz, 0.0f);
     hipLaunchKernelGGL(k_diff_shade_bwd, dim3(w.px_blocks, nsamples), dim3(kThreads), 0, s, gbuffer->position_dev,
                        biased_mvps_dev, w.depth, grad_out_dev, c, w.grad_depth, w.part_px);
+    const size_t nv3 = (size_t)nverts * 3;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(kThreads), 0, s, w.copies, nv3 * w.ncopies, 0.0f);
     hipLaunchKernelGGL(k_diff_texel_bwd, dim3(w.tx_blocks, nsamples), dim3(kThreads), 0, s, positions_dev,
-                       indices_dev, ntris, mvps_dev, w.vis, w.grad_depth, c, grad_positions_dev, w.part_tx);
+                       indices_dev, ntris, nverts, mvps_dev, w.vis, w.grad_depth, c, w.copies, w.ncopies, w.part_tx);
+    hipLaunchKernelGGL(k_diff_vertex_reduce, dim3((unsigned)((nv3 + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       w.copies, w.ncopies, nv3, grad_positions_dev);
     hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_px, w.px_blocks,
                        grad_biased_mvps_dev);
     hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_tx, w.tx_blocks, grad_mvps_dev);

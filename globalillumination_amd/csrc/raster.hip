@@ -5,12 +5,15 @@
 // GL_DEPTH_COMPONENT32F texture cleared to 1.0, depth test LESS, no culling)
 // and of the camera G-buffer (GBuffer.vert:15-21, GBuffer.frag:35-37).
 //
-// Design (binned raster, one launch sequence for a whole batch of maps):
-//   1. k_count    one thread per (triangle, map): transform, clip, snap, and
-//                 count the 32x32 screen tiles each piece overlaps;
+// Design (binned raster, one launch sequence for a whole batch of maps; the
+// kernels live in raster_kernels.hpp):
+//   1. k_bin      one thread per (triangle, map): transform, classify against
+//                 the clip planes, snap, and count the screen tiles each
+//                 triangle overlaps (LDS-aggregated); wide and clipped pieces
+//                 become setup records (k_clip, k_bin_wide);
 //   2. k_scan     one workgroup per map: exclusive scan of the tile counts;
-//   3. k_scatter  same threads as k_count: append the piece id to every
-//                 overlapped tile's list;
+//   3. k_bin      scatter pass: append the triangle id to every overlapped
+//                 tile's list;
 //   4. k_raster   one workgroup per (tile, map): resolve the tile's list into an
 //                 LDS tile with ds_min on the FP32 depth bits (non-negative
 //                 floats order like uint32) or on 64-bit (depth, id) keys,
@@ -37,868 +40,9 @@
 #include "../../include/gi.h"
 #include "gi_device.hpp"
 #include "gi_internal.hpp"
+#include "raster_kernels.hpp"
 
 namespace gi {
-
-// Tile edge of the binned raster: a template parameter of every tile-shaped
-// kernel.  The single-instance passes use kTile; the batched path picks 32 or
-// 64 per batch.
-constexpr int kTile = 64;
-constexpr int kRasterThreads = 256;
-constexpr float kGuard = 4.0f;
-constexpr int kMaxPoly = 9;
-constexpr int kSelfRasterMaxPx = 256;  // bbox∩tile area a lone thread rasterizes itself
-constexpr int kScanThreads = 1024;
-constexpr int kRecQueue = 64;  // LDS setup records a workgroup rasterizes together per chunk
-
-struct MeshView {
-    const float* __restrict__ pos;  // nverts x 3
-    const float* __restrict__ nrm;  // nverts x 3 or null
-    const int* __restrict__ idx;    // ntris x 3
-    int nt;
-};
-
-// Per-map transform: a device array of 16-float matrices, or one by value.
-struct MvpSource {
-    const float* __restrict__ dev;
-    Mat4 single;
-    __device__ __forceinline__ void load(int map, float m[16]) const {
-        if (dev) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) m[i] = dev[16 * map + i];
-        } else {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) m[i] = single.m[i];
-        }
-    }
-};
-
-struct RasterCfg {
-    int w, h;
-    int tiles_x, tiles_y;
-    float off_factor, off_units;
-};
-
-template <int NA>
-struct CV {
-    float c[4];
-    float a[NA > 0 ? NA : 1];
-};
-
-template <int NA>
-struct Tri {
-    long long X[3], Y[3];
-    float w[3];
-    float a[3][NA > 0 ? NA : 1];
-    int front;
-    int px0, px1, py0, py1;
-    int tl[3];
-    float z0, A, B, zoff;  // depth plane about vertex 0, per 1/256 pixel
-};
-
-__device__ __forceinline__ long long llmin(long long a, long long b) { return a < b ? a : b; }
-__device__ __forceinline__ long long llmax(long long a, long long b) { return a < b ? b : a; }
-
-__device__ __forceinline__ float plane_dist(const float c[4], int p) {
-    switch (p) {
-    case 0: return c[2] + c[3];
-    case 1: return c[3] - c[2];
-    case 2: return c[0] + kGuard * c[3];
-    case 3: return kGuard * c[3] - c[0];
-    case 4: return c[1] + kGuard * c[3];
-    default: return kGuard * c[3] - c[1];
-    }
-}
-
-template <int NA>
-__device__ void load_tri(const MeshView& m, const float mvp[16], int tri, CV<NA> v[3]) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        int vi = m.idx[3 * tri + k];
-        float x = m.pos[3 * vi + 0], y = m.pos[3 * vi + 1], z = m.pos[3 * vi + 2];
-        mat4_vec4(mvp, x, y, z, 1.0f, v[k].c);
-        if constexpr (NA == 6) {
-            v[k].a[0] = x;
-            v[k].a[1] = y;
-            v[k].a[2] = z;
-            v[k].a[3] = m.nrm[3 * vi + 0];
-            v[k].a[4] = m.nrm[3 * vi + 1];
-            v[k].a[5] = m.nrm[3 * vi + 2];
-        }
-    }
-}
-
-template <int NA>
-__device__ __forceinline__ void lerp_cv(const CV<NA>& in, const CV<NA>& out, float din, float dout,
-                                        CV<NA>& r) {
-    float t = din / (din - dout);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) r.c[k] = in.c[k] + t * (out.c[k] - in.c[k]);
-#pragma unroll
-    for (int k = 0; k < NA; ++k) r.a[k] = in.a[k] + t * (out.a[k] - in.a[k]);
-}
-
-// Fills `t` from three clip-space vertices.  Returns false for an empty piece.
-template <int NA>
-__device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2, const RasterCfg& cfg,
-                            Tri<NA>& t) {
-    const CV<NA>* vs[3] = {&v0, &v1, &v2};
-    float hw = (float)cfg.w * 0.5f, hh = (float)cfg.h * 0.5f;
-    float z[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        const float* c = vs[i]->c;
-        if (!(c[3] > 0.0f)) return false;
-        float xn = c[0] / c[3], yn = c[1] / c[3], zn = c[2] / c[3];
-        float xw = xn * hw + hw;
-        float yw = yn * hh + hh;
-        z[i] = zn * 0.5f + 0.5f;
-        t.X[i] = (long long)rintf(xw * 256.0f);
-        t.Y[i] = (long long)rintf(yw * 256.0f);
-        t.w[i] = c[3];
-#pragma unroll
-        for (int q = 0; q < NA; ++q) t.a[i][q] = vs[i]->a[q];
-    }
-    long long area = (t.X[1] - t.X[0]) * (t.Y[2] - t.Y[0]) - (t.Y[1] - t.Y[0]) * (t.X[2] - t.X[0]);
-    if (area == 0) return false;
-    t.front = area > 0;
-    if (area < 0) {
-        long long sx = t.X[1]; t.X[1] = t.X[2]; t.X[2] = sx;
-        long long sy = t.Y[1]; t.Y[1] = t.Y[2]; t.Y[2] = sy;
-        float sz = z[1]; z[1] = z[2]; z[2] = sz;
-        float sw = t.w[1]; t.w[1] = t.w[2]; t.w[2] = sw;
-#pragma unroll
-        for (int q = 0; q < NA; ++q) { float sa = t.a[1][q]; t.a[1][q] = t.a[2][q]; t.a[2][q] = sa; }
-        area = -area;
-    }
-    long long minX = llmin(t.X[0], llmin(t.X[1], t.X[2])), maxX = llmax(t.X[0], llmax(t.X[1], t.X[2]));
-    long long minY = llmin(t.Y[0], llmin(t.Y[1], t.Y[2])), maxY = llmax(t.Y[0], llmax(t.Y[1], t.Y[2]));
-    long long px0 = (minX - 128 + 255) >> 8, px1 = (maxX - 128) >> 8;
-    long long py0 = (minY - 128 + 255) >> 8, py1 = (maxY - 128) >> 8;
-    if (px0 < 0) px0 = 0;
-    if (py0 < 0) py0 = 0;
-    if (px1 > cfg.w - 1) px1 = cfg.w - 1;
-    if (py1 > cfg.h - 1) py1 = cfg.h - 1;
-    if (px0 > px1 || py0 > py1) return false;
-    t.px0 = (int)px0; t.px1 = (int)px1; t.py0 = (int)py0; t.py1 = (int)py1;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        int a = (i + 1) % 3, b = (i + 2) % 3;
-        long long dy = t.Y[b] - t.Y[a], dx = t.X[b] - t.X[a];
-        t.tl[i] = (dy < 0 || (dy == 0 && dx < 0)) ? 1 : 0;
-    }
-    const double dz1 = (double)z[1] - (double)z[0];
-    const double dz2 = (double)z[2] - (double)z[0];
-    const double inv_area = 1.0 / (double)area;
-    const double a = ((double)(t.Y[2] - t.Y[0]) * dz1 + (double)(t.Y[0] - t.Y[1]) * dz2) * inv_area;
-    const double b = ((double)(t.X[0] - t.X[2]) * dz1 + (double)(t.X[1] - t.X[0]) * dz2) * inv_area;
-    t.z0 = z[0];
-    t.A = (float)a;
-    t.B = (float)b;
-    t.zoff = 0.0f;
-    if (cfg.off_factor != 0.0f || cfg.off_units != 0.0f) {
-        double m = fmax(fabs(a), fabs(b)) * 256.0;
-        float zmax = fmaxf(fmaxf(fabsf(z[0]), fabsf(z[1])), fabsf(z[2]));
-        double r = 0.0;
-        if (zmax > 0.0f) {
-            int e;
-            frexpf(zmax, &e);
-            r = ldexp(1.0, (e - 1) - 23);
-        }
-        t.zoff = (float)((double)cfg.off_factor * m + (double)cfg.off_units * r);
-    }
-    return true;
-}
-
-// Clips a triangle that crosses a clip plane and sets up fan piece `piece`.
-// Returns the number of fan pieces; `ok` tells whether `piece` is non-empty.
-template <int NA>
-__device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
-                                           Tri<NA>& t, bool& ok) {
-    CV<NA> poly[kMaxPoly], tmp[kMaxPoly];
-    poly[0] = v[0];
-    poly[1] = v[1];
-    poly[2] = v[2];
-    int n = 3;
-    ok = false;
-    for (int p = 0; p < 6; ++p) {
-        int m = 0;
-        for (int k = 0; k < n; ++k) {
-            const CV<NA>& prev = poly[(k + n - 1) % n];
-            const CV<NA>& cur = poly[k];
-            float dp = plane_dist(prev.c, p), dc = plane_dist(cur.c, p);
-            if (dc >= 0.0f) {
-                if (dp < 0.0f) lerp_cv<NA>(cur, prev, dc, dp, tmp[m++]);
-                tmp[m++] = cur;
-            } else if (dp >= 0.0f) {
-                lerp_cv<NA>(prev, cur, dp, dc, tmp[m++]);
-            }
-        }
-        n = m;
-        if (n < 3) return 0;
-        for (int k = 0; k < n; ++k) poly[k] = tmp[k];
-    }
-    if (piece + 2 < n) ok = setup_piece<NA>(poly[0], poly[piece + 1], poly[piece + 2], cfg, t);
-    return n - 2;
-}
-
-// Number of fan pieces of `tri` (0 when trivially rejected) and the setup of
-// piece `piece`; `ok` is false when that piece covers no pixel centre.
-template <int NA>
-__device__ int get_piece(const MeshView& m, const float mvp[16], int tri, int piece, const RasterCfg& cfg,
-                         Tri<NA>& t, bool& ok) {
-    CV<NA> v[3];
-    load_tri<NA>(m, mvp, tri, v);
-    bool inside_all = true;
-    for (int p = 0; p < 6; ++p) {
-        float d0 = plane_dist(v[0].c, p), d1 = plane_dist(v[1].c, p), d2 = plane_dist(v[2].c, p);
-        int nin = (d0 >= 0.0f) + (d1 >= 0.0f) + (d2 >= 0.0f);
-        if (nin == 0) { ok = false; return 0; }
-        if (nin != 3) inside_all = false;
-    }
-    if (inside_all) {
-        ok = (piece == 0) && setup_piece<NA>(v[0], v[1], v[2], cfg, t);
-        return 1;
-    }
-    return clip_and_setup<NA>(v, piece, cfg, t, ok);
-}
-
-// Clip-free classification used by the hot kernels: kReject (outside a clip
-// plane), kClip (crosses a plane: handled by k_clip through setup records) or
-// kInside, in which case `t` is set up and `ok` tells whether it covers a
-// pixel centre.
-enum { kReject = 0, kClip = 1, kInside = 2 };
-
-template <int NA>
-__device__ __forceinline__ int classify(const MeshView& m, const float mvp[16], int tri, const RasterCfg& cfg,
-                                        Tri<NA>& t, bool& ok) {
-    CV<NA> v[3];
-    load_tri<NA>(m, mvp, tri, v);
-    bool inside_all = true;
-    ok = false;
-#pragma unroll
-    for (int p = 0; p < 6; ++p) {
-        float d0 = plane_dist(v[0].c, p), d1 = plane_dist(v[1].c, p), d2 = plane_dist(v[2].c, p);
-        int nin = (d0 >= 0.0f) + (d1 >= 0.0f) + (d2 >= 0.0f);
-        if (nin == 0) return kReject;
-        if (nin != 3) inside_all = false;
-    }
-    if (!inside_all) return kClip;
-    ok = setup_piece<NA>(v[0], v[1], v[2], cfg, t);
-    return kInside;
-}
-
-// Edge functions at the centre of pixel (i, j); false when not covered.
-template <int NA>
-__device__ __forceinline__ bool eval_pixel(const Tri<NA>& t, int i, int j, long long e[3], float& zout) {
-    long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        int a = (k + 1) % 3, b = (k + 2) % 3;
-        e[k] = (t.X[b] - t.X[a]) * (Py - t.Y[a]) - (t.Y[b] - t.Y[a]) * (Px - t.X[a]);
-    }
-    if (e[0] + t.tl[0] <= 0 || e[1] + t.tl[1] <= 0 || e[2] + t.tl[2] <= 0) return false;
-    float dx = (float)(Px - t.X[0]), dy = (float)(Py - t.Y[0]);
-    float z = t.z0 + (t.A * dx + t.B * dy);
-    z = z + t.zoff;
-    if (!(z > 0.0f)) z = 0.0f;
-    if (z > 1.0f) z = 1.0f;
-    zout = z;
-    return true;
-}
-
-// Conservative test: can any pixel centre of [x0,x1]x[y0,y1] be inside?
-template <int NA>
-__device__ __forceinline__ bool rect_overlap(const Tri<NA>& t, int x0, int x1, int y0, int y1) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        int a = (k + 1) % 3, b = (k + 2) % 3;
-        long long ex = t.X[b] - t.X[a], ey = t.Y[b] - t.Y[a];
-        long long Px = (long long)((ey < 0) ? x1 : x0) * 256 + 128;
-        long long Py = (long long)((ex > 0) ? y1 : y0) * 256 + 128;
-        long long e = ex * (Py - t.Y[a]) - ey * (Px - t.X[a]);
-        if (e + t.tl[k] <= 0) return false;
-    }
-    return true;
-}
-
-__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
-
-// ---------------------------------------------------------------------------
-// Compact setup record of one screen-space piece.  Wide pieces get one record
-// per map in HBM (written once by the binning pass); pieces a whole workgroup
-// rasterizes are shared through LDS.
-// ---------------------------------------------------------------------------
-struct Setup {
-    int X[3], Y[3];            // 24.8 fixed-point window coordinates
-    int px0, px1, py0, py1;    // pixel-centre bounding box (clamped)
-    int tl;                    // bit k: top-left tie bit of the edge opposite vertex k
-    uint32_t id;               // tri * 8 + fan piece
-    float z0, A, B, zoff;      // depth plane about vertex 0
-};
-
-__device__ __forceinline__ Setup to_setup(const Tri<0>& t, uint32_t id) {
-    Setup s;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        s.X[i] = (int)t.X[i];
-        s.Y[i] = (int)t.Y[i];
-    }
-    s.px0 = t.px0; s.px1 = t.px1; s.py0 = t.py0; s.py1 = t.py1;
-    s.tl = t.tl[0] | (t.tl[1] << 1) | (t.tl[2] << 2);
-    s.id = id;
-    s.z0 = t.z0; s.A = t.A; s.B = t.B; s.zoff = t.zoff;
-    return s;
-}
-
-// Same arithmetic as eval_pixel on the compact record.
-__device__ __forceinline__ bool eval_setup(const Setup& s, int i, int j, float& zout) {
-    const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
-    long long e0 = (long long)(s.X[2] - s.X[1]) * (Py - s.Y[1]) - (long long)(s.Y[2] - s.Y[1]) * (Px - s.X[1]);
-    long long e1 = (long long)(s.X[0] - s.X[2]) * (Py - s.Y[2]) - (long long)(s.Y[0] - s.Y[2]) * (Px - s.X[2]);
-    long long e2 = (long long)(s.X[1] - s.X[0]) * (Py - s.Y[0]) - (long long)(s.Y[1] - s.Y[0]) * (Px - s.X[0]);
-    if (e0 + (s.tl & 1) <= 0 || e1 + ((s.tl >> 1) & 1) <= 0 || e2 + ((s.tl >> 2) & 1) <= 0) return false;
-    float dx = (float)(Px - s.X[0]), dy = (float)(Py - s.Y[0]);
-    float z = s.z0 + (s.A * dx + s.B * dy);
-    z = z + s.zoff;
-    if (!(z > 0.0f)) z = 0.0f;
-    if (z > 1.0f) z = 1.0f;
-    zout = z;
-    return true;
-}
-
-__device__ __forceinline__ bool rect_overlap_s(const Setup& s, int x0, int x1, int y0, int y1) {
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        int a = (k + 1) % 3, b = (k + 2) % 3;
-        long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
-        long long Px = (long long)((ey < 0) ? x1 : x0) * 256 + 128;
-        long long Py = (long long)((ex > 0) ? y1 : y0) * 256 + 128;
-        long long e = ex * (Py - s.Y[a]) - ey * (Px - s.X[a]);
-        if (e + ((s.tl >> k) & 1) <= 0) return false;
-    }
-    return true;
-}
-
-// ---------------------------------------------------------------------------
-// Binning
-// ---------------------------------------------------------------------------
-constexpr int kWideSpan = 4;            // pieces overlapping more tiles are "wide"
-constexpr uint32_t kWideBit = 0x80000000u;
-constexpr int kWideBlocks = 64;         // workgroups per map binning wide pieces
-constexpr int kClipBlocks = 4;          // workgroups per map clipping listed triangles
-
-struct BinBufs {
-    uint32_t* counts;    // nmaps x ntiles (scatter cursors after k_scan)
-    uint32_t* offsets;   // nmaps x (ntiles + 1)
-    int* overflow;       // nmaps: lists or wide records exceed the workspace
-    int* wide_over;      // nmaps
-    uint32_t* nwide;     // nmaps
-    Setup* wide;         // nmaps x wcap
-    uint32_t* nclip;     // nmaps
-    uint32_t* clip;      // nmaps x wcap: triangles crossing a clip plane
-    uint32_t* entries;   // nmaps x cap
-    uint32_t cap, wcap;
-};
-
-__device__ __forceinline__ void bin_tile(const BinBufs& b, int map, int ntiles, int tile, uint32_t entry,
-                                         bool scatter) {
-    uint32_t* cnt = b.counts + (size_t)map * ntiles;
-    if (!scatter) {
-        atomicAdd(&cnt[tile], 1u);
-    } else {
-        uint32_t pos = b.offsets[(size_t)map * (ntiles + 1) + tile] + atomicAdd(&cnt[tile], 1u);
-        if (pos < b.cap) b.entries[(size_t)map * b.cap + pos] = entry;
-    }
-}
-
-// Appends a setup record (count pass only); records past the capacity flag
-// the map for the fallback raster.
-__device__ __forceinline__ void push_record(const BinBufs& b, int map, const Tri<0>& t, uint32_t entry) {
-    uint32_t w = atomicAdd(&b.nwide[map], 1u);
-    if (w < b.wcap) b.wide[(size_t)map * b.wcap + w] = to_setup(t, entry);
-    else b.wide_over[map] = 1;
-}
-
-// One thread per (triangle, map).  Triangles inside every clip plane whose
-// piece spans at most kWideSpan tiles are binned here; wider ones get a setup
-// record (binned by k_bin_wide), and triangles crossing a clip plane are
-// listed for k_clip.
-//
-// Tile counts are first aggregated in an LDS histogram, so the workgroup
-// issues one device-scope atomic per touched tile instead of one per
-// triangle: consecutive triangles of a mesh are neighbours and share tiles,
-// and device-scope atomics on the same line serialize outside the XCD's L2.
-constexpr int kBinHist = 4096;  // largest tile grid aggregated in LDS (64 x 64 tiles)
-
-template <int TS, bool kScatter>
-__device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
-                                         const BinBufs& b, int bx, int map) {
-    __shared__ uint32_t hist[kBinHist];
-    const int tid = threadIdx.x;
-    const int tri = bx * kRasterThreads + tid;
-    const int ntiles = cfg.tiles_x * cfg.tiles_y;
-    const bool lds = ntiles <= kBinHist;
-    if (kScatter && b.overflow[map]) return;  // uniform per workgroup
-    if (lds)
-        for (int p = tid; p < ntiles; p += blockDim.x) hist[p] = 0;
-    float mvp[16];
-    mvps.load(map, mvp);
-    // Up to kWideSpan tiles per thread, kept in registers (static indices).
-    int tile_of[kWideSpan];
-    uint32_t rank_of[kWideSpan];
-#pragma unroll
-    for (int k = 0; k < kWideSpan; ++k) tile_of[k] = -1;
-    const uint32_t entry = (uint32_t)tri * 8u;
-    if (tri < mesh.nt) {
-        Tri<0> t;
-        bool ok;
-        int cls = classify<0>(mesh, mvp, tri, cfg, t, ok);
-        if (cls == kClip) {
-            if (!kScatter) {
-                uint32_t c = atomicAdd(&b.nclip[map], 1u);
-                if (c < b.wcap) b.clip[(size_t)map * b.wcap + c] = (uint32_t)tri;
-                else b.wide_over[map] = 1;
-            }
-        } else if (ok) {
-            int tx0 = t.px0 / TS, tx1 = t.px1 / TS, ty0 = t.py0 / TS, ty1 = t.py1 / TS;
-            int ntx = tx1 - tx0 + 1;
-            int span = ntx * (ty1 - ty0 + 1);
-            if (span > kWideSpan) {
-                if (!kScatter) push_record(b, map, t, entry);
-            } else {
-#pragma unroll
-                for (int k = 0; k < kWideSpan; ++k) {
-                    if (k >= span) continue;
-                    int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
-                    if (span > 1) {
-                        int x0 = max(tx * TS, t.px0), x1 = min(tx * TS + TS - 1, t.px1);
-                        int y0 = max(ty * TS, t.py0), y1 = min(ty * TS + TS - 1, t.py1);
-                        if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
-                    }
-                    tile_of[k] = ty * cfg.tiles_x + tx;
-                }
-            }
-        }
-    }
-    if (!lds) {
-#pragma unroll
-        for (int k = 0; k < kWideSpan; ++k)
-            if (tile_of[k] >= 0) bin_tile(b, map, ntiles, tile_of[k], entry, kScatter);
-        return;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kWideSpan; ++k)
-        if (tile_of[k] >= 0) rank_of[k] = atomicAdd(&hist[tile_of[k]], 1u);
-    __syncthreads();
-    uint32_t* cnt = b.counts + (size_t)map * ntiles;
-    for (int p = tid; p < ntiles; p += blockDim.x) {
-        uint32_t c = hist[p];
-        if (c == 0) continue;
-        if (!kScatter) atomicAdd(&cnt[p], c);
-        else hist[p] = atomicAdd(&cnt[p], c);  // this workgroup's base within the tile's list
-    }
-    if (!kScatter) return;
-    __syncthreads();
-    const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
-#pragma unroll
-    for (int k = 0; k < kWideSpan; ++k) {
-        if (tile_of[k] < 0) continue;
-        uint32_t pos = off[tile_of[k]] + hist[tile_of[k]] + rank_of[k];
-        if (pos < b.cap) b.entries[(size_t)map * b.cap + pos] = entry;
-    }
-}
-
-template <int TS, bool kScatter>
-__global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
-    bin_body<TS, kScatter>(mesh, mvps, cfg, b, blockIdx.x, blockIdx.y);
-}
-
-// Clips the listed triangles; every non-empty fan piece becomes a setup
-// record.  Runs between the count passes, before k_bin_wide.  Workgroup
-// `part` of `nparts` takes every nparts-th stride of the map's list.
-__device__ __forceinline__ void clip_body(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
-                                          const BinBufs& b, int part, int nparts, int map) {
-    const uint32_t n = min(b.nclip[map], b.wcap);
-    float mvp[16];
-    mvps.load(map, mvp);
-    for (uint32_t c = part * kRasterThreads + threadIdx.x; c < n; c += nparts * kRasterThreads) {
-        int tri = (int)b.clip[(size_t)map * b.wcap + c];
-        CV<0> v[3];
-        load_tri<0>(mesh, mvp, tri, v);
-        int npieces = 1;
-        for (int piece = 0; piece < npieces; ++piece) {
-            Tri<0> t;
-            bool ok;
-            npieces = clip_and_setup<0>(v, piece, cfg, t, ok);
-            if (ok) push_record(b, map, t, (uint32_t)tri * 8u + (uint32_t)piece);
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_clip(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
-    clip_body(mesh, mvps, cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
-}
-
-// Wide pieces: a workgroup per record, its threads walk the record's tiles.
-template <int TS, bool kScatter>
-__device__ __forceinline__ void bin_wide_body(const RasterCfg& cfg, const BinBufs& b, int part, int nparts, int map) {
-    if (kScatter && b.overflow[map]) return;
-    const int ntiles = cfg.tiles_x * cfg.tiles_y;
-    const uint32_t n = min(b.nwide[map], b.wcap);
-    for (uint32_t w = part; w < n; w += nparts) {
-        const Setup s = b.wide[(size_t)map * b.wcap + w];
-        int tx0 = s.px0 / TS, tx1 = s.px1 / TS, ty0 = s.py0 / TS, ty1 = s.py1 / TS;
-        int ntx = tx1 - tx0 + 1;
-        int span = ntx * (ty1 - ty0 + 1);
-        for (int q = threadIdx.x; q < span; q += blockDim.x) {
-            int tx = tx0 + q % ntx, ty = ty0 + q / ntx;
-            int x0 = max(tx * TS, s.px0), x1 = min(tx * TS + TS - 1, s.px1);
-            int y0 = max(ty * TS, s.py0), y1 = min(ty * TS + TS - 1, s.py1);
-            if (!rect_overlap_s(s, x0, x1, y0, y1)) continue;
-            bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, kWideBit | w, kScatter);
-        }
-    }
-}
-
-template <int TS, bool kScatter>
-__global__ __launch_bounds__(256) void k_bin_wide(RasterCfg cfg, BinBufs b) {
-    bin_wide_body<TS, kScatter>(cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
-}
-
-// Exclusive scan of one map's tile counts; resets counts to 0 (they become
-// the scatter cursors) and flags maps whose lists exceed the workspace.
-__device__ __forceinline__ void scan_body(const BinBufs& b, int ntiles, int map) {
-    __shared__ unsigned long long part[kScanThreads];
-    uint32_t* c = b.counts + (size_t)map * ntiles;
-    uint32_t* o = b.offsets + (size_t)map * (ntiles + 1);
-    int per = (ntiles + kScanThreads - 1) / kScanThreads;
-    int lo = threadIdx.x * per, hi = min(lo + per, ntiles);
-    unsigned long long s = 0;
-    for (int i = lo; i < hi; ++i) s += c[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < kScanThreads; off <<= 1) {
-        unsigned long long v = (threadIdx.x >= (unsigned)off) ? part[threadIdx.x - off] : 0ull;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    unsigned long long run = part[threadIdx.x] - s;  // exclusive prefix
-    for (int i = lo; i < hi; ++i) {
-        o[i] = (uint32_t)run;
-        run += c[i];
-        c[i] = 0;
-    }
-    if (threadIdx.x == kScanThreads - 1) {
-        unsigned long long total = part[kScanThreads - 1];
-        bool over = total > b.cap || b.wide_over[map] != 0;
-        o[ntiles] = (uint32_t)(total > b.cap ? b.cap : total);
-        b.overflow[map] = over ? 1 : 0;
-    }
-}
-
-__global__ __launch_bounds__(kScanThreads) void k_scan(BinBufs b, int ntiles) { scan_body(b, ntiles, blockIdx.x); }
-
-// ---------------------------------------------------------------------------
-// Tile raster
-// ---------------------------------------------------------------------------
-template <bool kVis>
-struct KeyOf;
-template <>
-struct KeyOf<false> {
-    using T = uint32_t;
-    static constexpr uint32_t clear = 0x3F800000u;  // 1.0f
-};
-template <>
-struct KeyOf<true> {
-    using T = unsigned long long;
-    static constexpr unsigned long long clear = ~0ull;
-};
-
-// Edge functions of `s` at pixel centre (i, j) and their per-pixel steps.
-// Stepping adds exact integers, so every value equals eval_setup's.
-struct Edges {
-    long long e[3];
-    long long dx[3], dy[3];  // change per +1 pixel in x / y
-};
-
-__device__ __forceinline__ void edges_at(const Setup& s, int i, int j, Edges& E) {
-    const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int a = (k + 1) % 3, b = (k + 2) % 3;
-        const long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
-        E.e[k] = ex * (Py - s.Y[a]) - ey * (Px - s.X[a]);
-        E.dx[k] = -ey * 256;
-        E.dy[k] = ex * 256;
-    }
-}
-
-// Depth of a covered pixel centre at fixed-point offset (dx, dy) from vertex 0.
-__device__ __forceinline__ uint32_t plane_bits(const Setup& s, int dx, int dy) {
-    float z = s.z0 + (s.A * (float)dx + s.B * (float)dy);
-    z = z + s.zoff;
-    if (!(z > 0.0f)) z = 0.0f;
-    if (z > 1.0f) z = 1.0f;
-    return fbits(z);
-}
-
-template <bool kVis>
-__device__ __forceinline__ void store_min(const Setup& s, uint32_t zb, int slot, typename KeyOf<kVis>::T* tile) {
-    if constexpr (kVis) {
-        atomicMin(&tile[slot], ((unsigned long long)zb << 32) | s.id);
-    } else {
-        atomicMin(&tile[slot], zb);
-    }
-}
-
-template <bool kVis, typename E>
-__device__ __forceinline__ void plot(const Setup& s, const E e[3], int i, int j, int slot,
-                                     typename KeyOf<kVis>::T* tile) {
-    if (e[0] + (E)(s.tl & 1) <= 0 || e[1] + (E)((s.tl >> 1) & 1) <= 0 || e[2] + (E)((s.tl >> 2) & 1) <= 0) return;
-    store_min<kVis>(s, plane_bits(s, i * 256 + 128 - s.X[0], j * 256 + 128 - s.Y[0]), slot, tile);
-}
-
-// One thread covers the whole (small) rectangle, row by row.  When every
-// edge value inside the triangle's bounding box fits in 32 bits, the exact
-// stepping runs in int32.
-template <int TS, bool kVis, typename E>
-__device__ __forceinline__ void raster_rect_self_t(const Setup& s, const Edges& E64, int x0, int y0, int rx0, int rx1,
-                                                   int ry0, int ry1, typename KeyOf<kVis>::T* tile) {
-    E row[3], dx[3], dy[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        row[k] = (E)E64.e[k];
-        dx[k] = (E)E64.dx[k];
-        dy[k] = (E)E64.dy[k];
-    }
-    for (int j = ry0; j <= ry1; ++j) {
-        E e[3] = {row[0], row[1], row[2]};
-        for (int i = rx0; i <= rx1; ++i) {
-            plot<kVis, E>(s, e, i, j, (j - y0) * TS + (i - x0), tile);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) e[k] += dx[k];
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) row[k] += dy[k];
-    }
-}
-
-template <int TS, bool kVis>
-__device__ __forceinline__ void raster_rect_self(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
-                                                 typename KeyOf<kVis>::T* tile) {
-    Edges E;
-    edges_at(s, rx0, ry0, E);
-    const long long bw = (long long)max(s.X[0], max(s.X[1], s.X[2])) - min(s.X[0], min(s.X[1], s.X[2]));
-    const long long bh = (long long)max(s.Y[0], max(s.Y[1], s.Y[2])) - min(s.Y[0], min(s.Y[1], s.Y[2]));
-    // |e| <= 2*bw*bh inside the box and one step past it adds at most
-    // 256*max(bw, bh), so with these bounds every value stays below 2^30.
-    if (bw * bh < (1ll << 28) && bw < (1 << 20) && bh < (1 << 20))
-        raster_rect_self_t<TS, kVis, int>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
-    else
-        raster_rect_self_t<TS, kVis, long long>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
-}
-
-// The workgroup covers the rectangle: thread t owns tile column t % TS and
-// every (kRasterThreads / TS)-th row from row t / TS.
-template <int TS, bool kVis>
-__device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
-                                                  int tid, typename KeyOf<kVis>::T* tile) {
-    constexpr int kRowStep = kRasterThreads / TS;
-    const int i = x0 + (tid % TS);
-    if (i < rx0 || i > rx1) return;
-    int j = y0 + tid / TS;
-    if (j < ry0) j += ((ry0 - j + kRowStep - 1) / kRowStep) * kRowStep;
-    if (j > ry1) return;
-    Edges E;
-    edges_at(s, i, j, E);
-    long long e[3] = {E.e[0], E.e[1], E.e[2]};
-    for (; j <= ry1; j += kRowStep) {
-        plot<kVis, long long>(s, e, i, j, (j - y0) * TS + (i - x0), tile);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) e[k] += E.dy[k] * kRowStep;
-    }
-}
-
-// One (tile, map) of the raster.  kFallback: the map's lists overflowed the
-// workspace, so every (triangle, piece) pair is visited with full clipping.
-template <int TS, bool kVis, bool kFallback>
-__device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
-                                            const BinBufs& b, void* out, int map, int tile_id,
-                                            typename KeyOf<kVis>::T* tile, Setup* rec, int& nrec) {
-    const int tid = threadIdx.x;
-    const int tx = tile_id % cfg.tiles_x, ty = tile_id / cfg.tiles_x;
-    constexpr int kTilePx = TS * TS;
-    const int x0 = tx * TS, y0 = ty * TS;
-    const int x1 = min(x0 + TS - 1, cfg.w - 1), y1 = min(y0 + TS - 1, cfg.h - 1);
-    const int ntiles = cfg.tiles_x * cfg.tiles_y;
-    for (int p = tid; p < kTilePx; p += kRasterThreads) tile[p] = KeyOf<kVis>::clear;
-    if (tid == 0) nrec = 0;
-    float mvp[16];
-    mvps.load(map, mvp);
-    uint32_t begin = 0, end;
-    if constexpr (!kFallback) {
-        begin = b.offsets[(size_t)map * (ntiles + 1) + tile_id];
-        end = b.offsets[(size_t)map * (ntiles + 1) + tile_id + 1];
-    } else {
-        end = (uint32_t)mesh.nt * 8u;
-    }
-    __syncthreads();
-    for (uint32_t base = begin; base < end; base += kRasterThreads) {
-        uint32_t k = base + tid;
-        if (k < end) {
-            uint32_t entry = kFallback ? k : b.entries[(size_t)map * b.cap + k];
-            if (!kFallback && (entry & kWideBit)) {
-                const Setup& w = b.wide[(size_t)map * b.wcap + (entry & ~kWideBit)];
-                int slot = atomicAdd(&nrec, 1);
-                if (slot < kRecQueue) {
-                    rec[slot] = w;
-                } else {
-                    int rx0 = max(x0, w.px0), rx1 = min(x1, w.px1), ry0 = max(y0, w.py0), ry1 = min(y1, w.py1);
-                    if (rx0 <= rx1 && ry0 <= ry1) raster_rect_self<TS, kVis>(w, x0, y0, rx0, rx1, ry0, ry1, tile);
-                }
-            } else {
-                Tri<0> t;
-                bool ok;
-                if constexpr (kFallback)
-                    get_piece<0>(mesh, mvp, (int)(entry >> 3), (int)(entry & 7u), cfg, t, ok);
-                else
-                    classify<0>(mesh, mvp, (int)(entry >> 3), cfg, t, ok);
-                if (ok) {
-                    int rx0 = max(x0, t.px0), rx1 = min(x1, t.px1), ry0 = max(y0, t.py0), ry1 = min(y1, t.py1);
-                    if (rx0 <= rx1 && ry0 <= ry1) {
-                        Setup s = to_setup(t, entry);
-                        int slot = kRecQueue;
-                        if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kSelfRasterMaxPx) slot = atomicAdd(&nrec, 1);
-                        if (slot < kRecQueue) rec[slot] = s;
-                        else raster_rect_self<TS, kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        const int nq = min(nrec, kRecQueue);
-        for (int q = 0; q < nq; ++q) {
-            const Setup s = rec[q];
-            int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
-            if (rx0 > rx1 || ry0 > ry1) continue;
-            raster_rect_group<TS, kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tid, tile);
-        }
-        __syncthreads();
-        if (tid == 0) nrec = 0;
-        __syncthreads();
-    }
-    // Write the tile once.
-    const size_t plane = (size_t)cfg.w * cfg.h;
-    if constexpr (!kVis) {
-        float* dst = reinterpret_cast<float*>(out) + (size_t)map * plane;
-        const bool full = (x1 - x0 + 1 == TS) && ((cfg.w & 3) == 0);
-        if (full) {
-            for (int q = tid; q < kTilePx / 4; q += kRasterThreads) {
-                int r = q / (TS / 4), cidx = (q % (TS / 4)) * 4;
-                int j = y0 + r;
-                if (j > y1) continue;
-                float4 v;
-                v.x = __uint_as_float(tile[r * TS + cidx + 0]);
-                v.y = __uint_as_float(tile[r * TS + cidx + 1]);
-                v.z = __uint_as_float(tile[r * TS + cidx + 2]);
-                v.w = __uint_as_float(tile[r * TS + cidx + 3]);
-                *reinterpret_cast<float4*>(dst + (size_t)j * cfg.w + x0 + cidx) = v;
-            }
-        } else {
-            for (int p = tid; p < kTilePx; p += kRasterThreads) {
-                int i = x0 + p % TS, j = y0 + p / TS;
-                if (i > x1 || j > y1) continue;
-                dst[(size_t)j * cfg.w + i] = __uint_as_float(tile[p]);
-            }
-        }
-    } else {
-        unsigned long long* dst = reinterpret_cast<unsigned long long*>(out) + (size_t)map * plane;
-        for (int p = tid; p < kTilePx; p += kRasterThreads) {
-            int i = x0 + p % TS, j = y0 + p / TS;
-            if (i > x1 || j > y1) continue;
-            dst[(size_t)j * cfg.w + i] = tile[p];
-        }
-    }
-}
-
-// Binned raster: one workgroup per (tile, map); overflowed maps are left to
-// k_raster_fallback.
-template <int TS, bool kVis>
-__global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
-                                                           void* out) {
-    __shared__ typename KeyOf<kVis>::T tile[TS * TS];
-    __shared__ Setup rec[kRecQueue];
-    __shared__ int nrec;
-    const int map = blockIdx.y;
-    if (b.overflow[map]) return;
-    raster_tile<TS, kVis, false>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
-}
-
-// One workgroup per tile, looping over the maps whose lists overflowed.
-template <int TS, bool kVis>
-__global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mesh, MvpSource mvps, RasterCfg cfg,
-                                                                    BinBufs b, void* out, int nmaps) {
-    __shared__ typename KeyOf<kVis>::T tile[TS * TS];
-    __shared__ Setup rec[kRecQueue];
-    __shared__ int nrec;
-    for (int map = 0; map < nmaps; ++map) {
-        if (!b.overflow[map]) continue;
-        raster_tile<TS, kVis, true>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// G-buffer resolve: GBuffer.frag:35-37 (position, normal, gl_FrontFacing)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gbuffer_resolve(MeshView mesh, Mat4 mvp_arg, RasterCfg cfg,
-                                                         const unsigned long long* vis, float* pos_out,
-                                                         float* nrm_out) {
-    size_t npx = (size_t)cfg.w * cfg.h;
-    size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= npx) return;
-    unsigned long long key = vis[k];
-    float out[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (key != ~0ull) {
-        uint32_t id = (uint32_t)(key & 0xffffffffull);
-        Tri<6> t;
-        bool ok;
-        get_piece<6>(mesh, mvp_arg.m, (int)(id >> 3), (int)(id & 7u), cfg, t, ok);
-        int i = (int)(k % cfg.w), j = (int)(k / cfg.w);
-        long long e[3];
-        float z;
-        if (ok && eval_pixel<6>(t, i, j, e, z)) {
-            double b0 = (double)e[0] / (double)t.w[0];
-            double b1 = (double)e[1] / (double)t.w[1];
-            double b2 = (double)e[2] / (double)t.w[2];
-            double den = (b0 + b1) + b2;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                double num = (b0 * (double)t.a[0][q] + b1 * (double)t.a[1][q]) + b2 * (double)t.a[2][q];
-                out[q] = (float)(num / den);
-            }
-            if (!t.front) {
-                out[3] = -out[3];
-                out[4] = -out[4];
-                out[5] = -out[5];
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-        pos_out[q * npx + k] = out[q];
-        nrm_out[q * npx + k] = out[3 + q];
-    }
-}
-
-__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = v;
-}
 
 // ---------------------------------------------------------------------------
 // Host side

@@ -15,463 +15,9 @@
 
 #include "../../include/gi.h"
 #include "gi_device.hpp"
+#include "screen_kernels.hpp"
 
 namespace gi {
-
-constexpr int kBlockX = 64;
-constexpr int kBlockY = 4;
-
-struct GBufferView {
-    const float* __restrict__ pos;  // 3 planes
-    const float* __restrict__ nrm;  // 3 planes (facing-flipped) or null
-    int w, h;
-};
-
-struct HardUniforms {
-    Mat4 light_mvp;
-    Mat4 mv;
-    Mat3 nm;
-    float lpos[3];
-    float si;
-    int kernel_order;
-    int penumbra;
-    int max_search;
-    float thr;
-    float stepx, stepy;  // shadowMapStep (MyGLGeometryViewer.cpp:238)
-    int pre_eval;
-};
-
-// Shadow.frag:222-238 (identical copies in the RBSM shaders :352-365).
-__device__ __forceinline__ float pre_evaluation(const HardUniforms& u, float vx, float vy, float vz, float nx,
-                                                float ny, float nz) {
-    float ve[4];
-    mat4_vec4(u.mv.m, vx, vy, vz, 1.0f, ve);
-    float n3[3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        float acc = u.nm.m[0 * 3 + i] * nx;
-        acc = acc + u.nm.m[1 * 3 + i] * ny;
-        acc = acc + u.nm.m[2 * 3 + i] * nz;
-        n3[i] = acc;
-    }
-    float N[3], L[3];
-    normalize3(n3, N);
-    float ld[3] = {u.lpos[0] - ve[0], u.lpos[1] - ve[1], u.lpos[2] - ve[2]};
-    normalize3(ld, L);
-    float d = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
-    return (gmax(d, 0.0f) == 0.0f) ? u.si : 1.0f;
-}
-
-// Shadow.frag:86-116 with bilinearPCF == 1.
-__device__ float pcf(const DepthMap& D, const HardUniforms& u, float s, float t, float z) {
-    float incrWidth = 1.0f / D.fw;
-    float incrHeight = 1.0f / D.fh;
-    float illum = 0.0f;
-    float offset = (float)u.penumbra;
-    float stepSize = 2.0f * offset / (float)u.kernel_order;
-    int count = 0;
-    for (float w = -offset; w < offset; w += stepSize) {
-        float sw = s + w * incrWidth;
-        for (float h = -offset; h < offset; h += stepSize) {
-            float d = D.fetch(sw, t + h * incrHeight);
-            illum = illum + ((z <= d) ? 1.0f : u.si);
-            count++;
-        }
-    }
-    return illum / (float)count;
-}
-
-// ---------------------------------------------------------------------------
-// Revectorization (RBSM).  `kFiltered` selects FilteredRBSM.frag's
-// estimateRelativePosition (:234-243) and revectorizeShadow (:266-272).
-// ---------------------------------------------------------------------------
-template <bool kFiltered>
-struct Rbsm {
-    DepthMap D;
-    float sx, sy;  // shadowMapStep
-    float thr;
-    int max_search;
-
-    __device__ __forceinline__ float tex(float s, float t) const { return D.fetch(s, t); }
-
-    // NonConservativeSMSR.frag:56-94
-    __device__ bool disc_f(float x, float y, float z, float dirx, float diry, float type) const {
-        if (dirx == 0.0f) {
-            x = x - sx;
-            if (fabsf(le01(z, tex(x, y)) - type) == 0.0f) return true;
-            x = x + 2.0f * sx;
-            if (fabsf(le01(z, tex(x, y)) - type) == 0.0f) return true;
-            x = x - sx;
-        }
-        if (diry == 0.0f) {
-            y = y + sy;
-            if (fabsf(le01(z, tex(x, y)) - type) == 0.0f) return true;
-            y = y - 2.0f * sy;
-            if (fabsf(le01(z, tex(x, y)) - type) == 0.0f) return true;
-        }
-        return false;
-    }
-
-    // NonConservativeSMSR.frag:96-178; writes the shader global `newDepth`.
-    __device__ bool disc_v(float x, float y, float z, float dirx, float diry, float dr, float dg, float db,
-                           float& newDepth) const {
-        float rx = x, ry = y;
-        newDepth = z;
-        if (dirx == 0.0f) {
-            if (dr == 0.5f || dr == 0.75f) {
-                rx = x - sx;
-                float d = tex(rx, ry);
-                if (db == 1.0f && fabsf(z - d) < thr) { z = z - thr; newDepth = z; }
-                if (fabsf(le01(z, d) - db) == 0.0f) return true;
-            }
-            if (dr == 0.75f || dr == 0.25f) {
-                rx = x + sx;
-                float d = tex(rx, ry);
-                if (db == 1.0f && fabsf(z - d) < thr) { z = z - thr; newDepth = z; }
-                if (fabsf(le01(z, d) - db) == 0.0f) return true;
-            }
-        }
-        if (diry == 0.0f) {
-            if (dg == 0.5f || dg == 0.75f) {
-                ry = y + sy;
-                float d = tex(rx, ry);
-                if (db == 1.0f && fabsf(z - d) < thr) { z = z - thr; newDepth = z; }
-                if (fabsf(le01(z, d) - db) == 0.0f) return true;
-            }
-            if (dg == 0.75f || dg == 0.25f) {
-                ry = y - sy;
-                float d = tex(rx, ry);
-                if (db == 1.0f && fabsf(z - d) < thr) { z = z - thr; newDepth = z; }
-                if (fabsf(le01(z, d) - db) == 0.0f) return true;
-            }
-        }
-        return false;
-    }
-
-    // NonConservativeSMSR.frag:180-232
-    __device__ float length(float dr, float dg, float db, float x, float y, float z, float dirx, float diry,
-                            float sub) const {
-        float cx = x, cy = y, cz = z;
-        float found = 0.0f, dist = 0.0f, newDepth = 0.0f;
-        float stx = dirx * sx, sty = diry * sy;
-        cx = cx + stx;
-        cy = cy + sty;
-        for (int it = 0; it < max_search; ++it) {
-            float d = tex(cx, cy);
-            if (db == 0.0f && fabsf(cz - d) < thr) cz = cz - thr;
-            float center = le01(cz, d);
-            if (fabsf(center - db) == 0.0f) {
-                found = disc_f(cx, cy, cz, 0.0f, 0.0f, db) ? 1.0f : 0.0f;
-                break;
-            }
-            if (!disc_v(cx, cy, cz, dirx, diry, dr, dg, db, newDepth)) break;
-            dist = dist + 1.0f;
-            cx = cx + stx;
-            cy = cy + sty;
-            if (db == 1.0f) cz = newDepth;
-        }
-        float a = -(dist + (1.0f - sub));
-        float b = dist + (1.0f - sub);
-        return gmix(a, b, found);
-    }
-
-    // estimateRelativePosition :234-243
-    __device__ float rel(float dx, float dy, float shadow) const {
-        float T = 1.0f;
-        if (dx < 0.0f && dy < 0.0f) T = 0.0f;
-        if (dx > 0.0f && dy > 0.0f) T = -2.0f;
-        float edge = gmin(fabsf(dx) + fabsf(dy), (float)max_search);
-        if (kFiltered) return (T * fabsf(gmax(T * dx, T * dy))) / edge;
-        return (gmax(T, 2.0f * shadow - 1.0f) * fabsf(gmax(T * dx, T * dy))) / edge;
-    }
-
-    // revectorizeShadow :266-274
-    __device__ float revectorize(float rx, float ry, float s) const {
-        if (kFiltered) {
-            if (rx * ry < 0.0f) return (1.0f - s) + (2.0f * s - 1.0f) * gmax(rx, ry);
-            if (rx * ry == 0.0f) return s;
-            return gclamp((1.0f - s) + (2.0f * s - 1.0f) * (rx + ry), 0.0f, 1.0f);
-        }
-        if ((rx * ry == 2.0f * s) ||
-            ((fabsf(rx) * fabsf(ry) > 0.0f) && ((1.0f - s) + (2.0f * s - 1.0f) * (fabsf(rx) + fabsf(ry)) < 0.5f)))
-            return 0.0f;
-        return 1.0f;
-    }
-
-    // computeDiscontinuity :276-286 (getDisc :23-54 without early outs)
-    __device__ void discontinuity(float x, float y, float z, float dC, float& r, float& g, float& b) const {
-        float center = le01(z, dC);
-        float xx = x - sx;
-        float dl = le01(z, tex(xx, y));
-        xx = xx + 2.0f * sx;
-        float drt = le01(z, tex(xx, y));
-        xx = xx - sx;
-        float yy = y + sy;
-        float db = le01(z, tex(xx, yy));
-        yy = yy - 2.0f * sy;
-        float dt = le01(z, tex(xx, yy));
-        r = (2.0f * fabsf(dl - center) + fabsf(drt - center)) / 4.0f;
-        g = (2.0f * fabsf(db - center) + fabsf(dt - center)) / 4.0f;
-        b = 1.0f - center;
-    }
-
-    // orientateDS :245-254 -> estimateRelativePosition :256-264 -> revectorizeShadow
-    __device__ float revectorized(float dr, float dg, float db, float x, float y, float z, float subx, float suby,
-                                  float shadow) const {
-        float left = length(dr, dg, db, x, y, z, -1.0f, 0.0f, 1.0f - subx);
-        float right = length(dr, dg, db, x, y, z, 1.0f, 0.0f, subx);
-        float down = length(dr, dg, db, x, y, z, 0.0f, -1.0f, 1.0f - suby);
-        float up = length(dr, dg, db, x, y, z, 0.0f, 1.0f, suby);
-        return revectorize(rel(left, right, shadow), rel(down, up, shadow), shadow);
-    }
-
-    // computeShadowFromSMSR :288-304
-    __device__ float smsr(float x, float y, float z, float si) const {
-        float dC = tex(x, y);
-        float r, g, b;
-        discontinuity(x, y, z, dC, r, g, b);
-        float subx = gfract(x * D.fw);
-        float suby = gfract(y * D.fh);
-        float shadow = le01(z, dC);
-        if (r > 0.0f || g > 0.0f) {
-            if (r == 0.75f && g == 0.75f) return gmix(1.0f - shadow, 1.0f, si);
-            return gmix(revectorized(r, g, b, x, y, z, subx, suby, shadow), 1.0f, si);
-        }
-        return gmix(shadow, 1.0f, si);
-    }
-
-    // computeShadowFromAccurateRPCF :306-349
-    __device__ float rpcf(float x, float y, float z, float si, int kernel_order, int penumbra) const {
-        float incrWidth = 1.0f / D.fw;
-        float incrHeight = 1.0f / D.fh;
-        float illum = 0.0f;
-        float offset = (float)penumbra;
-        float stepSize = 2.0f * offset / (float)kernel_order;
-        int count = 0;
-        for (float w = -offset; w <= offset; w += stepSize) {
-            for (float h = -offset; h <= offset; h += stepSize) {
-                float px = x + w * incrWidth;
-                float py = y + h * incrHeight;
-                float d = tex(px, py);
-                float shadow = le01(z, d);
-                float r, g, b;
-                discontinuity(px, py, z, d, r, g, b);
-                if (r > 0.0f || g > 0.0f) {
-                    float subx = gfract(px * D.fw);
-                    float suby = gfract(py * D.fh);
-                    illum = illum + gmix(revectorized(r, g, b, px, py, z, subx, suby, shadow), 1.0f, si);
-                } else {
-                    illum = illum + gmix(shadow, 1.0f, si);
-                }
-                count++;
-            }
-        }
-        return illum / (float)count;
-    }
-};
-
-// Pixel index with an XCD-aware block order: hardware hands consecutive
-// workgroups to the 8 XCDs round-robin, so block b is remapped to make each
-// XCD's blocks one contiguous band of rows, keeping neighbouring pixels'
-// shadow-map texels in the same L2.
-__device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
-    const int bx_n = (w + kBlockX - 1) / kBlockX;
-    const int nblocks = bx_n * (int)gridDim.y;
-    int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
-    int per = nblocks / 8;
-    int rem = nblocks - per * 8;
-    int xcd = b % 8, slot = b / 8;
-    int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-    if (lin >= nblocks) lin = b;
-    x = (lin % bx_n) * kBlockX + (int)threadIdx.x;
-    y = (lin / bx_n) * kBlockY + (int)threadIdx.y;
-}
-
-template <int kTech>
-__global__ __launch_bounds__(kBlockX* kBlockY) void k_hard(GBufferView gb, DepthMap D, HardUniforms u,
-                                                            float* __restrict__ out) {
-    int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
-    if (x >= gb.w || y >= gb.h) return;
-    const size_t plane = (size_t)gb.w * gb.h;
-    const size_t k = (size_t)y * gb.w + x;
-    float vx = gb.pos[k];
-    if (vx == 0.0f) {  // Shadow.frag:244 discards; the cleared target holds 0
-        out[k] = 0.0f;
-        return;
-    }
-    float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
-    float sc[4];
-    mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
-    float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
-    float shadow = 1.0f;
-    if (u.pre_eval)
-        shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
-    if constexpr (kTech == GI_NAIVE || kTech == GI_PCF) {
-        if (sc[3] > 0.0f && shadow == 1.0f) {
-            if constexpr (kTech == GI_NAIVE) {
-                float d = D.fetch(n0, n1);
-                shadow = (n2 <= d) ? 1.0f : u.si;
-            } else {
-                shadow = pcf(D, u, n0, n1, n2);
-            }
-        }
-    } else {
-        Rbsm<kTech == GI_RBSM_FILTERED> rb{D, u.stepx, u.stepy, u.thr, u.max_search};
-        if (shadow == 1.0f) {
-            if constexpr (kTech == GI_RBSM)
-                shadow = rb.smsr(n0, n1, n2, u.si);
-            else
-                shadow = rb.rpcf(n0, n1, n2, u.si, u.kernel_order, u.penumbra);
-        }
-    }
-    out[k] = shadow;
-}
-
-constexpr int kMaxTaps = 16;
-
-struct SoftUniforms {
-    float m[12];  // rows 0-3 of columns 0-2 of B*lightMVP (AccurateSoftShadow.frag:70-73)
-    float si;
-    int nsamples;
-    int pcf;
-    int ntaps;             // taps per axis of the PCF loop
-    float taps[kMaxTaps];  // loop values w of `for (w = -offset; w </<= offset; w += step)`
-};
-
-// Rows of NT consecutive texels; selecting element k of a row.
-template <int NT>
-struct Row { float v[NT]; };
-
-template <int NT>
-__device__ __forceinline__ float pick(const Row<NT>& r, int k) {
-    float x = r.v[0];
-#pragma unroll
-    for (int q = 1; q < NT; ++q) x = (k == q) ? r.v[q] : x;
-    return x;
-}
-
-// One sample's term of AccurateSoftShadow.frag:75-121 for a pixel whose
-// columns-0-2 product with B*lightMVP is `cm` (monteCarlo: accFactor = 1).
-// NT > 0: the PCF loop has NT taps per axis, and when every tap lies inside
-// the map the NT x NT taps are served by NT row loads of NT consecutive
-// texels (the taps of one sample span fewer than NT texels per axis, which
-// the guard below checks).  NT == 0: generic per-tap loop.
-template <int NT>
-__device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms& u, const float cm[4], float4 T,
-                                           float incrWidth, float incrHeight) {
-    float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
-    float sx = cx / cw, sy = cy / cw, sz = cz / cw;
-    if (!u.pcf) {
-        float d = D.fetch(sx, sy);
-        return (sz <= d) ? 1.0f : u.si;
-    }
-    if constexpr (NT > 0) {
-        float tx[NT], ty[NT];
-        int iu[NT], iv[NT];
-        bool inside = true;
-#pragma unroll
-        for (int a = 0; a < NT; ++a) {
-            tx[a] = sx + u.taps[a] * incrWidth;
-            ty[a] = sy + u.taps[a] * incrHeight;
-            float uu = tx[a] * D.fw, vv = ty[a] * D.fh;
-            inside = inside && (uu >= 0.0f && uu < D.fw && vv >= 0.0f && vv < D.fh);
-            iu[a] = (int)floorf(uu);
-            iv[a] = (int)floorf(vv);
-        }
-        // one row of NT texels from iu[0] must cover every tap column
-        inside = inside && (iu[NT - 1] - iu[0] <= NT - 1) && (iu[0] + NT - 1 < D.w);
-        float illum = 0.0f;
-        if (inside) {
-            Row<NT> rows[NT];
-#pragma unroll
-            for (int b = 0; b < NT; ++b)
-                __builtin_memcpy(&rows[b], D.texels + (size_t)iv[b] * D.w + iu[0], sizeof(Row<NT>));
-#pragma unroll
-            for (int a = 0; a < NT; ++a) {
-#pragma unroll
-                for (int b = 0; b < NT; ++b) {
-                    float d = pick<NT>(rows[b], iu[a] - iu[0]);
-                    illum = illum + ((sz <= d) ? 1.0f : u.si);
-                }
-            }
-        } else {
-#pragma unroll
-            for (int a = 0; a < NT; ++a) {
-#pragma unroll
-                for (int b = 0; b < NT; ++b) {
-                    float d = D.fetch(tx[a], ty[b]);
-                    illum = illum + ((sz <= d) ? 1.0f : u.si);
-                }
-            }
-        }
-        return illum / (float)(NT * NT);
-    } else {
-        float illum = 0.0f;
-        for (int a = 0; a < u.ntaps; ++a) {
-            float txa = sx + u.taps[a] * incrWidth;
-            for (int b = 0; b < u.ntaps; ++b) {
-                float d = D.fetch(txa, sy + u.taps[b] * incrHeight);
-                illum = illum + ((sz <= d) ? 1.0f : u.si);
-            }
-        }
-        return illum / (float)(u.ntaps * u.ntaps);
-    }
-}
-
-// AccurateSoftShadow.frag:52-133: one thread per pixel, samples in order
-// 0..N-1.  Blocks are ordered XCD-aware (pixel_of_block) so each XCD's L2
-// serves one contiguous band of the view.
-template <int NT>
-__global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
-                                                            int smh, const float4* __restrict__ trans,
-                                                            SoftUniforms u, float* __restrict__ out) {
-    int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
-    if (x >= gb.w || y >= gb.h) return;
-    const size_t plane = (size_t)gb.w * gb.h;
-    const size_t k = (size_t)y * gb.w + x;
-    const float vx = gb.pos[k];
-    if (vx == 0.0f) {  // Shadow.frag:244 background test; the cleared target holds 0
-        out[k] = 0.0f;
-        return;
-    }
-    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
-    float cm[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {  // :70-73, v.w ignored
-        float a = u.m[0 * 4 + i] * vx;
-        a = a + u.m[1 * 4 + i] * vy;
-        a = a + u.m[2 * 4 + i] * vz;
-        cm[i] = a;
-    }
-    const size_t mapsz = (size_t)smw * smh;
-    DepthMap D{maps, smw, smh, (float)smw, (float)smh};
-    const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
-    float acc = 0.0f, count = 0.0f;
-    // unrolled so several samples' gathers are in flight; the sum stays in order
-#pragma unroll 4
-    for (int i = 0; i < u.nsamples; ++i) {
-        D.texels = maps + (size_t)i * mapsz;
-        acc = acc + soft_term<NT>(D, u, cm, trans[i], incrWidth, incrHeight);
-        count = count + 1.0f;
-    }
-    out[k] = acc / count;
-}
-
-// Tap values of the PCF loop `for (w = -offset; w < offset (or <=); w += step)`
-// with step = 2*offset/kernelOrder, accumulated in FP32 exactly as the shader
-// loop does (AccurateSoftShadow.frag:101-109, Shadow.frag:93-99).
-int pcf_taps(float offset, int kernel_order, bool inclusive, float* taps, int max_taps) {
-    float stepSize = 2.0f * offset / (float)kernel_order;
-    int n = 0;
-    for (float w = -offset; inclusive ? (w <= offset) : (w < offset); w += stepSize) {
-        if (n == max_taps) return -1;
-        taps[n++] = w;
-    }
-    return n;
-}
-
 namespace {
 
 bool finite_all(const float* v, int n) {
