@@ -109,7 +109,7 @@ _SIGS = {
     "gi_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
-    "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 7),
+    "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),
     "gi_diff_soft_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.POINTER(GiGBuffer), ctypes.POINTER(GiDiffParams),

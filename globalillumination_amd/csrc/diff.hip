@@ -10,13 +10,21 @@
 //   q              = sum over the 4 texels around (s*W - .5, t*H - .5) of
 //                    smoothstep-bilinear weights * sigmoid((d - z + bias)/tau);
 //   V_i            = si + (1 - si) * q;   S = mean_i V_i.
-// The backward pass recomputes the visibility raster and texel depths (they
-// are not stored by the forward pass), then
-//   k_diff_shade_bwd  pixel adjoint: dL/d(B*M'_i) per block and dL/dd per
-//                     texel (scatter-add),
-//   k_diff_texel_bwd  texel adjoint: dL/dM'_i per block, dL/dvertex
-//                     (scatter-add),
-//   k_diff_reduce     fixed-order reduction of the per-block partials.
+//
+// A texel's depth and barycentrics are affine in the texel centre, so the
+// per-triangle part is set up once per (map, triangle) (k_diff_setup) and a
+// texel evaluates two affine forms.  The adjoint of the texel depths with
+// respect to the triangle's clip-space vertices is linear in G * lambda_j
+// (G = dL/dd, lambda_j the texel's barycentrics), so the texel adjoint only
+// sums w_j = sum G * lambda_j per (map, triangle) and k_diff_tri_bwd turns the
+// three sums into vertex and matrix gradients.
+//
+// Forward:  visibility raster, k_diff_setup, k_diff_depth, k_diff_shade.
+// Backward: the same first three (recomputed, nothing is kept from the
+//           forward pass), then k_diff_shade_bwd (pixel adjoint: dL/d(B*M'_i)
+//           per block, dL/dd per texel by scatter-add), k_diff_texel_bwd,
+//           k_diff_tri_bwd, and k_diff_reduce (fixed-order reduction of the
+//           per-block matrix partials).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,17 +38,50 @@ namespace gi {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kTexPerThread = 16;  // texels per thread in the texel adjoint
+constexpr int kGatherMax = 256;    // bbox texels up to which a triangle gathers its own adjoint
+constexpr int kLargeSlots = 4;     // large triangles a texel block sums in LDS
+
+// Per-block sum of one large triangle's G * lambda_j (tri = -1: unused).
+struct LargePart {
+    double w[3];
+    int tri;
+    int pad;
+};
 
 // Geometry (projections, barycentrics, window depths, the sigmoid argument)
 // is evaluated in FP64: the argument (d - z + bias) / tau divides the
 // rounding of two window depths near 0.9 by tau ~ 2e-4, so FP32 geometry
-// would leave ~1e-3 noise in every term.  Texel depths are stored in FP64;
-// the scatter-added adjoints (texel depth, vertex position) are FP32.
+// would leave ~1e-3 noise in every term.  Texel depths and the per-triangle
+// sums are FP64; the texel-depth adjoint and the vertex gradients are FP32.
 struct DiffCfg {
     int smw, smh, S;
     int gw, gh;
     double tau, bias, si;
 };
+
+// Affine form of one (map, triangle): with dx = x - X0, dy = y - Y0,
+//   l1 = a1*dx + b1*dy,  l2 = a2*dx + b2*dy,  d = Z0 + l1*dZ1 + l2*dZ2.
+// X0 = NaN marks a triangle without a plane (a vertex with w <= 0, or a
+// degenerate projection): its texels read depth 1 and get no gradient.
+// [bx0, bx1] x [by0, by1]: texels that can hold the triangle: texel x is
+// covered only if x + .5 lies in the triangle's snapped window extent, and
+// snapping moves a vertex by at most 1/512 px (bound taken as 1/256), clamped
+// to the map; empty when bx0 > bx1.
+struct TriPlane {
+    double X0, Y0;
+    double a1, b1, a2, b2;
+    double Z0, dZ1, dZ2;
+    short bx0, bx1, by0, by1;
+};
+
+__device__ __forceinline__ bool plane_ok(const TriPlane& p) { return p.X0 == p.X0; }
+__device__ __forceinline__ int bbox_area(const TriPlane& p) {
+    return (p.bx1 < p.bx0 || p.by1 < p.by0) ? 0 : (p.bx1 - p.bx0 + 1) * (p.by1 - p.by0 + 1);
+}
+// Triangles whose adjoint the texel pass scatters (k_diff_texel_bwd); the
+// others gather theirs in k_diff_tri_bwd.
+__device__ __forceinline__ bool plane_large(const TriPlane& p) { return bbox_area(p) > kGatherMax; }
 
 __device__ __forceinline__ double sigmoid_d(double x) { return 1.0 / (1.0 + exp(-x)); }
 __device__ __forceinline__ double smoothw(double f) { return f * f * (3.0 - 2.0 * f); }
@@ -82,29 +123,70 @@ __device__ __forceinline__ bool tri_screen(const float* __restrict__ pos, const 
     return ok;
 }
 
-__global__ __launch_bounds__(kThreads) void k_diff_depth(const float* __restrict__ pos, const int* __restrict__ idx,
-                                                         int ntris, const double* __restrict__ mvps,
-                                                         const unsigned long long* __restrict__ vis, DiffCfg c,
+// One thread per (triangle, map).  grid (triangle blocks, S).
+__global__ __launch_bounds__(kThreads) void k_diff_setup(const float* __restrict__ pos, const int* __restrict__ idx,
+                                                         int ntris, const double* __restrict__ mvps, DiffCfg c,
+                                                         TriPlane* __restrict__ planes,
+                                                         unsigned char* __restrict__ large) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    const int i = blockIdx.y;
+    if (t >= ntris) return;
+    double m[16];
+    load_m(mvps + 16 * i, m);
+    double X[3], Y[3], Z[3], C[3][4];
+    TriPlane p;
+    const bool ok = tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C);
+    const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+    p.bx0 = p.by0 = 1;
+    p.bx1 = p.by1 = 0;
+    if (ok && D != 0.0) {
+        const double e = 0.5 + 1.0 / 256.0;
+        const double x0 = ceil(fmin(X[0], fmin(X[1], X[2])) - e), x1 = floor(fmax(X[0], fmax(X[1], X[2])) - 0.5 + 1.0 / 256.0);
+        const double y0 = ceil(fmin(Y[0], fmin(Y[1], Y[2])) - e), y1 = floor(fmax(Y[0], fmax(Y[1], Y[2])) - 0.5 + 1.0 / 256.0);
+        p.bx0 = (short)fmax(0.0, fmin(x0, (double)c.smw));
+        p.bx1 = (short)fmax(-1.0, fmin(x1, (double)(c.smw - 1)));
+        p.by0 = (short)fmax(0.0, fmin(y0, (double)c.smh));
+        p.by1 = (short)fmax(-1.0, fmin(y1, (double)(c.smh - 1)));
+        p.X0 = X[0];
+        p.Y0 = Y[0];
+        p.a1 = (Y[2] - Y[0]) / D;
+        p.b1 = -(X[2] - X[0]) / D;
+        p.a2 = -(Y[1] - Y[0]) / D;
+        p.b2 = (X[1] - X[0]) / D;
+        p.Z0 = Z[0];
+        p.dZ1 = Z[1] - Z[0];
+        p.dZ2 = Z[2] - Z[0];
+    } else {
+        p.X0 = __builtin_nan("");
+        p.Y0 = p.a1 = p.b1 = p.a2 = p.b2 = p.Z0 = p.dZ1 = p.dZ2 = 0.0;
+    }
+    planes[(size_t)i * ntris + t] = p;
+    large[(size_t)i * ntris + t] = (plane_ok(p) && plane_large(p)) ? 1 : 0;
+}
+
+// Triangle of a visibility key (a piece id is tri * 8 + piece), or -1.
+__device__ __forceinline__ int key_tri(unsigned long long key, int ntris) {
+    if (key == ~0ull) return -1;
+    const int t = (int)((key & 0xffffffffull) >> 3);
+    return t < ntris ? t : -1;
+}
+
+__global__ __launch_bounds__(kThreads) void k_diff_depth(const unsigned long long* __restrict__ vis,
+                                                         const TriPlane* __restrict__ planes, int ntris, DiffCfg c,
                                                          double* __restrict__ depth) {
-    const size_t plane = (size_t)c.smw * c.smh;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t plane = (uint32_t)c.smw * (uint32_t)c.smh;  // < 2^31 (edges <= 16384)
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const int map = blockIdx.y;
     if (k >= plane) return;
-    const unsigned long long key = vis[(size_t)map * plane + k];
+    const int t = key_tri(vis[(size_t)map * plane + k], ntris);
     double d = 1.0;
-    if (key != ~0ull) {
-        const int tri = (int)((key & 0xffffffffull) >> 3);
-        double m[16];
-        load_m(mvps + 16 * map, m);
-        double X[3], Y[3], Z[3], C[3][4];
-        if (tri < ntris && tri_screen(pos, idx, tri, m, c.smw, c.smh, X, Y, Z, C)) {
-            double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-            if (D != 0.0) {
-                double xc = (double)(k % c.smw) + 0.5, yc = (double)(k / c.smw) + 0.5;
-                double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
-                double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
-                d = Z[0] + l1 * (Z[1] - Z[0]) + l2 * (Z[2] - Z[0]);
-            }
+    if (t >= 0) {
+        const TriPlane& p = planes[(size_t)map * ntris + t];
+        if (p.X0 == p.X0) {
+            const uint32_t ky = k / (uint32_t)c.smw, kx = k - ky * (uint32_t)c.smw;
+            const double dx = ((double)kx + 0.5) - p.X0, dy = ((double)ky + 0.5) - p.Y0;
+            const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
+            d = p.Z0 + l1 * p.dZ1 + l2 * p.dZ2;
         }
     }
     depth[(size_t)map * plane + k] = d;
@@ -245,136 +327,250 @@ __global__ __launch_bounds__(kThreads) void k_diff_shade_bwd(const float* __rest
     block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
 }
 
-// Texel adjoint.  grid (texel blocks, S); each thread takes kTexPerThread
-// texels (strided by the block size, so loads stay coalesced).
-//
-// Vertex gradients are scatter-adds, and a large triangle (a ground plane
-// covers most texels of every map) would send every wave's adds to the same
-// three vertices, one cache line that device-scope atomics from all XCDs then
-// serialize on.  So: a wave whose active texels all belong to one triangle
-// sums its vertex gradients across lanes, and lane 0 keeps a running sum while
-// consecutive texel rows stay on that triangle, flushing once when the
-// triangle changes; the flushes of block b go to copy b % ncopies of the
-// vertex-gradient array, and k_diff_vertex_reduce adds the copies in a fixed
-// order.  Waves spanning several triangles add lane by lane.
-constexpr int kTexPerThread = 4;
-
-__device__ __forceinline__ void flush_vertex(const int* __restrict__ idx, int tri, const double acc[9],
-                                             float* __restrict__ dst) {
+// Texel adjoint of the large triangles (bounding box over kGatherMax
+// texels; a ground plane covers most texels of every map): the sums of
+// G * lambda_j over each large triangle's texels.  grid (texel blocks, S);
+// each thread takes kTexPerThread texels strided by the block size
+// (coalesced).  Device-scope atomics on one triangle's cache line from every
+// wave would serialize, so a wave first reduces: its texels are 64
+// consecutive texels of a row, in which each triangle covers a contiguous
+// run, and a segmented scan leaves every run's sum in the run's last lane;
+// a wave that is one run keeps a running sum in lane 0 while consecutive rows
+// stay on that triangle.  Runs are added into kLargeSlots LDS slots per
+// block, written out as the block's LargePart entries that
+// k_diff_large_reduce adds into wsum; a block meeting more large triangles
+// than slots adds the rest to wsum with device atomics.
+template <int kSlots>
+__device__ __forceinline__ void add_slots(int* slot_tri, double (*slot_w)[3], double* __restrict__ wsum, int tri,
+                                          const double w[3]) {
     if (tri < 0) return;
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        const int v = idx[3 * tri + j];
+    for (int q = 0; q < kSlots; ++q) {
+        const int old = atomicCAS(&slot_tri[q], -1, tri);
+        if (old == -1 || old == tri) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&dst[3 * v + a], (float)acc[3 * j + a]);
+            for (int j = 0; j < 3; ++j) atomicAdd(&slot_w[q][j], w[j]);
+            return;
+        }
     }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) unsafeAtomicAdd(&wsum[3 * (size_t)tri + j], w[j]);
 }
 
-__global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const float* __restrict__ pos, const int* __restrict__ idx,
-                                                             int ntris, int nverts, const double* __restrict__ mvps,
-                                                             const unsigned long long* __restrict__ vis,
+__global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const unsigned long long* __restrict__ vis,
+                                                             const TriPlane* __restrict__ planes,
+                                                             const unsigned char* __restrict__ large, int ntris,
                                                              const float* __restrict__ grad_depth, DiffCfg c,
-                                                             float* __restrict__ copies, int ncopies,
-                                                             double* __restrict__ partial) {
-    const size_t msz = (size_t)c.smw * c.smh;
+                                                             double* __restrict__ wsum_all,
+                                                             LargePart* __restrict__ parts) {
+    __shared__ int slot_tri[kLargeSlots];
+    __shared__ double slot_w[kLargeSlots][3];
+    if (threadIdx.x < kLargeSlots) {
+        slot_tri[threadIdx.x] = -1;
+        slot_w[threadIdx.x][0] = slot_w[threadIdx.x][1] = slot_w[threadIdx.x][2] = 0.0;
+    }
+    __syncthreads();
+    const uint32_t msz = (uint32_t)c.smw * (uint32_t)c.smh;  // < 2^31 (edges <= 16384)
     const int i = blockIdx.y;
-    float* __restrict__ dst = copies + (size_t)(blockIdx.x % ncopies) * nverts * 3;
-    double m[16];
-    load_m(mvps + 16 * i, m);
-    double gm[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) gm[e] = 0.0;
+    const float* __restrict__ gd = grad_depth + (size_t)i * msz;
+    const unsigned long long* __restrict__ vk = vis + (size_t)i * msz;
+    double* __restrict__ wsum = wsum_all + (size_t)i * ntris * 3;
+    const TriPlane* __restrict__ pl = planes + (size_t)i * ntris;
+    const unsigned char* __restrict__ lg = large + (size_t)i * ntris;
     const int lane = threadIdx.x % 64;
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+    const uint32_t base = (uint32_t)blockIdx.x * (kTexPerThread * kThreads) + threadIdx.x;
     int cur = -1;  // lane 0: triangle of the running sum
-    double run[9];
-#pragma unroll
-    for (int e = 0; e < 9; ++e) run[e] = 0.0;
+    double run[3] = {0.0, 0.0, 0.0};
     for (int r = 0; r < kTexPerThread; ++r) {
-        const size_t k = ((size_t)blockIdx.x * kTexPerThread + r) * kThreads + threadIdx.x;
-        double gv[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) gv[e] = 0.0;
+        const uint32_t k = base + (uint32_t)r * kThreads;
+        double w[3] = {0.0, 0.0, 0.0};
         int tri = -1;
-        const double G = (k < msz) ? (double)grad_depth[(size_t)i * msz + k] : 0.0;
-        const unsigned long long key = (k < msz) ? vis[(size_t)i * msz + k] : ~0ull;
-        if (G != 0.0 && key != ~0ull) {
-            const int t = (int)((key & 0xffffffffull) >> 3);
-            double X[3], Y[3], Z[3], C[3][4];
-            if (t < ntris && tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C)) {
-                const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-                if (D != 0.0) {
+        const float G = (k < msz) ? gd[k] : 0.0f;
+        if (G != 0.0f) {
+            const int t = key_tri(vk[k], ntris);
+            if (t >= 0 && lg[t]) {
+                const TriPlane& p = pl[t];
+                {
                     tri = t;
-                    const double xc = (double)(k % c.smw) + 0.5, yc = (double)(k / c.smw) + 0.5;
-                    const double l1 = ((xc - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (yc - Y[0])) / D;
-                    const double l2 = ((X[1] - X[0]) * (yc - Y[0]) - (xc - X[0]) * (Y[1] - Y[0])) / D;
-                    const double lam[3] = {1.0 - l1 - l2, l1, l2};
-                    // plane gradient in window x, y; d moves by -lambda_j * grad when vertex j moves
-                    const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
-                    const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
-#pragma unroll
-                    for (int j = 0; j < 3; ++j) {
-                        const double gX = -G * lam[j] * gx * (0.5 * c.smw);
-                        const double gY = -G * lam[j] * gy * (0.5 * c.smh);
-                        const double gZ = G * lam[j] * 0.5;
-                        const double iw = 1.0 / C[j][3];
-                        const double gc[4] = {gX * iw, gY * iw, gZ * iw,
-                                              -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
-                        const int v = idx[3 * t + j];
-                        const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
-#pragma unroll
-                        for (int col = 0; col < 4; ++col)
-#pragma unroll
-                            for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            double g = 0.0;
-#pragma unroll
-                            for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
-                            gv[3 * j + a] = g;
-                        }
-                    }
+                    const uint32_t ky = k / (uint32_t)c.smw, kx = k - ky * (uint32_t)c.smw;
+                    const double dx = ((double)kx + 0.5) - p.X0, dy = ((double)ky + 0.5) - p.Y0;
+                    const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
+                    w[0] = (double)G * (1.0 - l1 - l2);
+                    w[1] = (double)G * l1;
+                    w[2] = (double)G * l2;
                 }
             }
         }
         // every lane of the wave reaches this point
-        const unsigned long long act = __ballot(tri >= 0);
-        if (act == 0ull) continue;
-        const int first = __ffsll((long long)act) - 1;
-        const int t0 = __shfl(tri, first, 64);
-        if (__ballot(tri >= 0 && tri != t0) == 0ull) {
+        const unsigned long long valid = __ballot(tri >= 0);
+        if (valid == 0ull) continue;
+        // lanes without a contribution (w = 0) join the run of the nearest
+        // contributing lane at or below them
+        const unsigned long long below = valid & upto;
+        const int src = below ? 63 - __clzll((long long)below) : __ffsll((long long)valid) - 1;
+        tri = __shfl(tri, src, 64);
+        const int prev = __shfl_up(tri, 1, 64);
+        const unsigned long long heads = __ballot(lane == 0 || tri != prev);
+        if (heads == 1ull) {  // one run: the whole wave is one triangle
 #pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                double x = gv[e];
+            for (int j = 0; j < 3; ++j) {
+                double x = w[j];
 #pragma unroll
                 for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
-                gv[e] = x;
+                w[j] = x;
             }
             if (lane == 0) {
-                if (cur != t0) {
-                    flush_vertex(idx, cur, run, dst);
-                    cur = t0;
-#pragma unroll
-                    for (int e = 0; e < 9; ++e) run[e] = 0.0;
+                if (cur != tri) {
+                    add_slots<kLargeSlots>(slot_tri, slot_w, wsum, cur, run);
+                    cur = tri;
+                    run[0] = run[1] = run[2] = 0.0;
                 }
-#pragma unroll
-                for (int e = 0; e < 9; ++e) run[e] += gv[e];
+                run[0] += w[0];
+                run[1] += w[1];
+                run[2] += w[2];
             }
-        } else if (tri >= 0) {
-            flush_vertex(idx, tri, gv, dst);
+            continue;
         }
+        // segmented inclusive scan: lane sums its run from the run's head
+        const int head = 63 - __clzll((long long)(heads & upto));
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double y = __shfl_up(w[j], off, 64);
+                if (lane - off >= head) w[j] += y;
+            }
+        }
+        const bool tail = (lane == 63) || ((heads >> (lane + 1)) & 1ull);
+        if (tail) add_slots<kLargeSlots>(slot_tri, slot_w, wsum, tri, w);
     }
-    if (lane == 0) flush_vertex(idx, cur, run, dst);
-    block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
+    if (lane == 0) add_slots<kLargeSlots>(slot_tri, slot_w, wsum, cur, run);
+    __syncthreads();
+    if (threadIdx.x < kLargeSlots) {
+        LargePart lp;
+        lp.tri = slot_tri[threadIdx.x];
+        lp.pad = 0;
+        lp.w[0] = slot_w[threadIdx.x][0];
+        lp.w[1] = slot_w[threadIdx.x][1];
+        lp.w[2] = slot_w[threadIdx.x][2];
+        parts[((size_t)i * gridDim.x + blockIdx.x) * kLargeSlots + threadIdx.x] = lp;
+    }
 }
 
-// grad_pos[e] += sum over copies c of copies[c][e], in copy order.
-__global__ void k_diff_vertex_reduce(const float* __restrict__ copies, int ncopies, size_t n,
-                                     float* __restrict__ grad_pos) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    float s = 0.0f;
-    for (int c = 0; c < ncopies; ++c) s += copies[(size_t)c * n + e];
-    grad_pos[e] += s;
+// Adds every block's LargePart entries of map blockIdx.x into wsum: the
+// entries first meet in LDS slots (few large triangles per map), then one
+// device atomic per slot.  grid (S).
+constexpr int kMapSlots = 32;
+
+__global__ __launch_bounds__(kThreads) void k_diff_large_reduce(const LargePart* __restrict__ parts, int tx_blocks,
+                                                                int ntris, double* __restrict__ wsum_all) {
+    __shared__ int slot_tri[kMapSlots];
+    __shared__ double slot_w[kMapSlots][3];
+    if (threadIdx.x < kMapSlots) {
+        slot_tri[threadIdx.x] = -1;
+        slot_w[threadIdx.x][0] = slot_w[threadIdx.x][1] = slot_w[threadIdx.x][2] = 0.0;
+    }
+    __syncthreads();
+    const int i = blockIdx.x;
+    double* __restrict__ wsum = wsum_all + (size_t)i * ntris * 3;
+    const LargePart* __restrict__ bp = parts + (size_t)i * tx_blocks * kLargeSlots;
+    for (int b = threadIdx.x; b < tx_blocks * kLargeSlots; b += kThreads) {
+        const LargePart lp = bp[b];
+        if (lp.tri >= 0) add_slots<kMapSlots>(slot_tri, slot_w, wsum, lp.tri, lp.w);
+    }
+    __syncthreads();
+    if (threadIdx.x < kMapSlots && slot_tri[threadIdx.x] >= 0) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            unsafeAtomicAdd(&wsum[3 * (size_t)slot_tri[threadIdx.x] + j], slot_w[threadIdx.x][j]);
+    }
+}
+
+// Triangle adjoint.  grid (triangle blocks, S).  w_j = sum of G * lambda_j
+// over the triangle's texels in map i: gathered here over the triangle's
+// bounding box (texels whose visibility key names it) for small triangles,
+// read from the texel pass's sums for large ones.  The texel depths then move
+// by
+//   dL = sum_j [ w_j/2 dz_j - w_j (dd/dX) dX_j - w_j (dd/dY) dY_j ]
+// (a texel's depth through the plane moves by -lambda_j * grad when vertex j
+// moves in the window, and by lambda_j times vertex j's window depth change),
+// chained through the perspective divide to vertex j's clip coordinates, then
+// to dL/dM'_i (block partials) and dL/dvertex (scatter-add).
+__global__ __launch_bounds__(kThreads) void k_diff_tri_bwd(const float* __restrict__ pos, const int* __restrict__ idx,
+                                                           int ntris, const double* __restrict__ mvps,
+                                                           const TriPlane* __restrict__ planes,
+                                                           const unsigned long long* __restrict__ vis,
+                                                           const float* __restrict__ grad_depth,
+                                                           const double* __restrict__ wsum_all, DiffCfg c,
+                                                           float* __restrict__ grad_pos,
+                                                           double* __restrict__ partial) {
+    const int t = blockIdx.x * kThreads + threadIdx.x;
+    const int i = blockIdx.y;
+    double gm[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) gm[e] = 0.0;
+    double w[3] = {0.0, 0.0, 0.0};
+    if (t < ntris) {
+        const TriPlane p = planes[(size_t)i * ntris + t];
+        if (plane_ok(p)) {
+            if (plane_large(p)) {
+                const double* ws = wsum_all + ((size_t)i * ntris + t) * 3;
+                w[0] = ws[0];
+                w[1] = ws[1];
+                w[2] = ws[2];
+            } else {
+                const size_t moff = (size_t)i * c.smw * c.smh;
+                for (int y = p.by0; y <= p.by1; ++y) {
+                    for (int x = p.bx0; x <= p.bx1; ++x) {
+                        const size_t k = moff + (size_t)y * c.smw + x;
+                        if (key_tri(vis[k], ntris) != t) continue;
+                        const float G = grad_depth[k];
+                        if (G == 0.0f) continue;
+                        const double dx = ((double)x + 0.5) - p.X0, dy = ((double)y + 0.5) - p.Y0;
+                        const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
+                        w[0] += (double)G * (1.0 - l1 - l2);
+                        w[1] += (double)G * l1;
+                        w[2] += (double)G * l2;
+                    }
+                }
+            }
+        }
+    }
+    if (w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0) {
+        double m[16];
+        load_m(mvps + 16 * i, m);
+        double X[3], Y[3], Z[3], C[3][4];
+        const bool ok = tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C);
+        const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+        if (ok && D != 0.0) {
+            const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
+            const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double gX = -w[j] * gx * (0.5 * c.smw);
+                const double gY = -w[j] * gy * (0.5 * c.smh);
+                const double gZ = w[j] * 0.5;
+                const double iw = 1.0 / C[j][3];
+                const double gc[4] = {gX * iw, gY * iw, gZ * iw,
+                                      -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
+                const int v = idx[3 * t + j];
+                const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
+#pragma unroll
+                for (int col = 0; col < 4; ++col)
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    double g = 0.0;
+#pragma unroll
+                    for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
+                    unsafeAtomicAdd(&grad_pos[3 * v + a], (float)g);
+                }
+            }
+        }
+    }
+    block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
 }
 
 // out[i][e] = sum over blocks of partial[i][block][e]: one workgroup per
@@ -405,60 +601,62 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
         p[i] = v;
 }
 
-inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
-
-// Copies of the vertex-gradient array the texel adjoint spreads its adds
-// over (capped so the copies stay within kCopyBudget bytes).
-constexpr int kVertCopies = 64;
-constexpr size_t kCopyBudget = size_t(256) << 20;
-
-int vert_copies(int nverts) {
-    size_t per = (size_t)nverts * 3 * sizeof(float);
-    size_t n = per ? kCopyBudget / per : kVertCopies;
-    return (int)(n < 1 ? 1 : (n > (size_t)kVertCopies ? kVertCopies : n));
+__global__ void k_fill_f64(double* p, size_t n, double v) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
 struct DiffWs {
-    float* copies;
-    int ncopies;
     float* mvps32;
     unsigned long long* vis;
     double* depth;
     float* grad_depth;
+    TriPlane* planes;
+    double* wsum;
     double* part_px;
-    double* part_tx;
+    double* part_tri;
+    LargePart* parts;
+    unsigned char* large;
     void* raster;
     size_t raster_bytes;
-    int px_blocks, tx_blocks;
+    int px_blocks, tri_blocks, tx_blocks;
 };
 
-size_t diff_layout(int nverts, int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* ws, char* base) {
+size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* ws, char* base) {
     const size_t msz = (size_t)smw * smh;
     const int px_blocks = (int)(((size_t)gw * gh + kThreads - 1) / kThreads);
+    const int tri_blocks = (ntris + kThreads - 1) / kThreads;
     const int tx_blocks = (int)((msz + (size_t)kThreads * kTexPerThread - 1) / ((size_t)kThreads * kTexPerThread));
-    const int ncopies = vert_copies(nverts);
     size_t off = 0;
-    size_t o_cp = off; off = align256(off + sizeof(float) * 3 * (size_t)nverts * ncopies);
     size_t o_m32 = off; off = align256(off + sizeof(float) * 16 * (size_t)S);
     size_t o_vis = off; off = align256(off + sizeof(unsigned long long) * msz * S);
     size_t o_depth = off; off = align256(off + sizeof(double) * msz * S);
     size_t o_gd = off; off = align256(off + sizeof(float) * msz * S);
+    size_t o_pl = off; off = align256(off + sizeof(TriPlane) * (size_t)ntris * S);
+    size_t o_ws = off; off = align256(off + sizeof(double) * 3 * (size_t)ntris * S);
     size_t o_ppx = off; off = align256(off + sizeof(double) * 16 * (size_t)px_blocks * S);
-    size_t o_ptx = off; off = align256(off + sizeof(double) * 16 * (size_t)tx_blocks * S);
+    size_t o_ptr = off; off = align256(off + sizeof(double) * 16 * (size_t)(tri_blocks > 0 ? tri_blocks : 1) * S);
+    size_t o_lp = off; off = align256(off + sizeof(LargePart) * kLargeSlots * (size_t)tx_blocks * S);
+    size_t o_lg = off; off = align256(off + (size_t)ntris * S);
     size_t rbytes = internal::vis_raster_workspace_size(ntris, S, smw, smh);
     size_t o_r = off; off = align256(off + rbytes);
     if (ws) {
-        ws->copies = reinterpret_cast<float*>(base + o_cp);
-        ws->ncopies = ncopies;
         ws->mvps32 = reinterpret_cast<float*>(base + o_m32);
         ws->vis = reinterpret_cast<unsigned long long*>(base + o_vis);
         ws->depth = reinterpret_cast<double*>(base + o_depth);
         ws->grad_depth = reinterpret_cast<float*>(base + o_gd);
+        ws->planes = reinterpret_cast<TriPlane*>(base + o_pl);
+        ws->wsum = reinterpret_cast<double*>(base + o_ws);
         ws->part_px = reinterpret_cast<double*>(base + o_ppx);
-        ws->part_tx = reinterpret_cast<double*>(base + o_ptx);
+        ws->part_tri = reinterpret_cast<double*>(base + o_ptr);
+        ws->parts = reinterpret_cast<LargePart*>(base + o_lp);
+        ws->large = reinterpret_cast<unsigned char*>(base + o_lg);
         ws->raster = base + o_r;
         ws->raster_bytes = rbytes;
         ws->px_blocks = px_blocks;
+        ws->tri_blocks = tri_blocks;
         ws->tx_blocks = tx_blocks;
     }
     return rbytes ? off : 0;
@@ -471,8 +669,8 @@ bool finite_all(const double* v, int n) {
 }
 
 int check_common(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
-                 const double* mvps_dev, const double* biased_dev, int S, int smw, int smh, const gi_gbuffer* gb, const gi_diff_params* p,
-                 void* ws, DiffCfg& c) {
+                 const double* mvps_dev, const double* biased_dev, int S, int smw, int smh, const gi_gbuffer* gb,
+                 const gi_diff_params* p, void* ws, DiffCfg& c) {
     if (!positions_dev || (!indices_dev && ntris > 0) || !mvps_dev || !biased_dev || !gb || !gb->position_dev || !p ||
         !ws)
         return GI_EINVAL;
@@ -482,20 +680,24 @@ int check_common(const float* positions_dev, int nverts, const int32_t* indices_
         return GI_EINVAL;
     double sc[3] = {p->tau, p->bias, p->shadow_intensity};
     if (!finite_all(sc, 3)) return GI_ENONFINITE;
-    if (!(p->tau > 0.0f)) return GI_EINVAL;
+    if (!(p->tau > 0.0)) return GI_EINVAL;
     c = DiffCfg{smw, smh, S, gb->width, gb->height, p->tau, p->bias, p->shadow_intensity};
     return GI_OK;
 }
 
+// Visibility raster, per-triangle planes and texel depths.
 int recompute(const float* pos, int nverts, const int32_t* idx, int ntris, const double* mvps, const DiffCfg& c,
               const DiffWs& w, hipStream_t s) {
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((16 * c.S + 255) / 256)), dim3(256), 0, s, mvps, w.mvps32, 16 * c.S);
     int rc = internal::vis_raster(pos, nverts, idx, ntris, w.mvps32, c.S, c.smw, c.smh, w.vis, w.raster,
                                   w.raster_bytes, s);
     if (rc != GI_OK) return rc;
+    if (ntris > 0)
+        hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)w.tri_blocks, c.S), dim3(kThreads), 0, s, pos, idx, ntris,
+                           mvps, c, w.planes, w.large);
     const size_t msz = (size_t)c.smw * c.smh;
     hipLaunchKernelGGL(k_diff_depth, dim3((unsigned)((msz + kThreads - 1) / kThreads), c.S), dim3(kThreads), 0, s,
-                       pos, idx, ntris, mvps, w.vis, c, w.depth);
+                       w.vis, w.planes, ntris, c, w.depth);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
@@ -506,12 +708,11 @@ using namespace gi;
 
 extern "C" {
 
-size_t gi_diff_workspace_size(int nverts, int ntris, int nsamples, int smw, int smh, int view_w, int view_h) {
-    if (nverts <= 0 || ntris < 0 || nsamples <= 0 || nsamples > GI_MAX_SAMPLES || smw <= 0 || smh <= 0 ||
-        smw > GI_MAX_EDGE || smh > GI_MAX_EDGE || view_w <= 0 || view_h <= 0 || view_w > GI_MAX_EDGE ||
-        view_h > GI_MAX_EDGE)
+size_t gi_diff_workspace_size(int ntris, int nsamples, int smw, int smh, int view_w, int view_h) {
+    if (ntris < 0 || nsamples <= 0 || nsamples > GI_MAX_SAMPLES || smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE ||
+        smh > GI_MAX_EDGE || view_w <= 0 || view_h <= 0 || view_w > GI_MAX_EDGE || view_h > GI_MAX_EDGE)
         return 0;
-    return diff_layout(nverts, ntris, nsamples, smw, smh, view_w, view_h, nullptr, nullptr);
+    return diff_layout(ntris, nsamples, smw, smh, view_w, view_h, nullptr, nullptr);
 }
 
 int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
@@ -524,8 +725,7 @@ int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* 
     if (rc != GI_OK) return rc;
     if (!out_dev) return GI_EINVAL;
     DiffWs w;
-    size_t need =
-        diff_layout(nverts, ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
+    size_t need = diff_layout(ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
     if (need == 0) return GI_ENOTSUP;
     if (workspace_bytes < need) return GI_ENOSPACE;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -547,8 +747,7 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
     if (rc != GI_OK) return rc;
     if (!grad_out_dev || !grad_biased_mvps_dev || !grad_mvps_dev || !grad_positions_dev) return GI_EINVAL;
     DiffWs w;
-    size_t need =
-        diff_layout(nverts, ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
+    size_t need = diff_layout(ntris, nsamples, smw, smh, c.gw, c.gh, &w, reinterpret_cast<char*>(workspace_dev));
     if (need == 0) return GI_ENOTSUP;
     if (workspace_bytes < need) return GI_ENOSPACE;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -559,15 +758,22 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
     hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(kThreads), 0, s, w.grad_depth, msz, 0.0f);
     hipLaunchKernelGGL(k_diff_shade_bwd, dim3(w.px_blocks, nsamples), dim3(kThreads), 0, s, gbuffer->position_dev,
                        biased_mvps_dev, w.depth, grad_out_dev, c, w.grad_depth, w.part_px);
-    const size_t nv3 = (size_t)nverts * 3;
-    hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(kThreads), 0, s, w.copies, nv3 * w.ncopies, 0.0f);
-    hipLaunchKernelGGL(k_diff_texel_bwd, dim3(w.tx_blocks, nsamples), dim3(kThreads), 0, s, positions_dev,
-                       indices_dev, ntris, nverts, mvps_dev, w.vis, w.grad_depth, c, w.copies, w.ncopies, w.part_tx);
-    hipLaunchKernelGGL(k_diff_vertex_reduce, dim3((unsigned)((nv3 + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
-                       w.copies, w.ncopies, nv3, grad_positions_dev);
     hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_px, w.px_blocks,
                        grad_biased_mvps_dev);
-    hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_tx, w.tx_blocks, grad_mvps_dev);
+    if (ntris > 0) {
+        hipLaunchKernelGGL(k_fill_f64, dim3(256), dim3(kThreads), 0, s, w.wsum, (size_t)3 * ntris * nsamples, 0.0);
+        hipLaunchKernelGGL(k_diff_texel_bwd, dim3(w.tx_blocks, nsamples), dim3(kThreads), 0, s, w.vis, w.planes,
+                           w.large, ntris, w.grad_depth, c, w.wsum, w.parts);
+        hipLaunchKernelGGL(k_diff_large_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.parts, w.tx_blocks, ntris,
+                           w.wsum);
+        hipLaunchKernelGGL(k_diff_tri_bwd, dim3(w.tri_blocks, nsamples), dim3(kThreads), 0, s, positions_dev,
+                           indices_dev, ntris, mvps_dev, w.planes, w.vis, w.grad_depth, w.wsum, c,
+                           grad_positions_dev, w.part_tri);
+        hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_tri, w.tri_blocks,
+                           grad_mvps_dev);
+    } else {
+        hipLaunchKernelGGL(k_fill_f64, dim3(1), dim3(kThreads), 0, s, grad_mvps_dev, (size_t)16 * nsamples, 0.0);
+    }
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

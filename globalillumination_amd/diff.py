@@ -147,10 +147,10 @@ class DiffScene:
         self._mcache = (key, mv, bv)
         return mv, bv
 
-    def _workspace(self, nverts, device):
+    def _workspace(self, device):
         sm = self.light.map_size
-        n = _lib.lib().gi_diff_workspace_size(int(nverts), int(self.indices.shape[0]), self.S, sm, sm,
-                                              self.gbuffer.width, self.gbuffer.height)
+        n = _lib.lib().gi_diff_workspace_size(int(self.indices.shape[0]), self.S, sm, sm, self.gbuffer.width,
+                                              self.gbuffer.height)
         if n == 0:
             raise _lib.GiError("gi_diff_workspace_size", _lib.GI_EINVAL, "unsupported sizes")
         return _WS.get(n, device)
@@ -161,7 +161,7 @@ class DiffScene:
         v = vertices.detach().contiguous()
         if out is None:
             out = torch.empty((self.gbuffer.height, self.gbuffer.width), dtype=torch.float32, device=dev)
-        ws = self._workspace(v.shape[0], dev)
+        ws = self._workspace(dev)
         g = self.gbuffer.to_c(with_normals=False)
         sm = self.light.map_size
         _lib.call("gi_diff_soft_forward", _dptr(v), int(v.shape[0]), _dptr(self.indices), int(self.indices.shape[0]),
@@ -179,7 +179,7 @@ class DiffScene:
         gb_m = torch.empty((self.S, 16), dtype=torch.float64, device=dev)
         g_m = torch.empty((self.S, 16), dtype=torch.float64, device=dev)
         gv = torch.zeros_like(v)
-        ws = self._workspace(v.shape[0], dev)
+        ws = self._workspace(dev)
         g = self.gbuffer.to_c(with_normals=False)
         sm = self.light.map_size
         _lib.call("gi_diff_soft_backward", _dptr(v), int(v.shape[0]), _dptr(self.indices),

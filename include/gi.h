@@ -213,7 +213,7 @@ typedef struct gi_diff_params {
     double shadow_intensity; /* si */
 } gi_diff_params;
 
-size_t gi_diff_workspace_size(int nverts, int ntris, int nsamples, int smw, int smh, int view_w, int view_h);
+size_t gi_diff_workspace_size(int ntris, int nsamples, int smw, int smh, int view_w, int view_h);
 
 /* mvps_dev: nsamples x 16 unbiased P*V of the samples (depth maps);
  * biased_mvps_dev: nsamples x 16 B*P*V (receivers); both FP64 column-major

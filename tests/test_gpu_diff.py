@@ -244,5 +244,5 @@ def test_diff_rejects_bad_arguments(small, gi):
     bad = DiffScene(small.dm.indices, small.gbuf, small.light, DiffParams(tau=0.0))
     with pytest.raises(_lib.GiError):
         bad.forward(small.L, small.r, torch.from_numpy(small.mesh.positions).cuda())
-    assert _lib.lib().gi_diff_workspace_size(8, 10, 0, 64, 64, 8, 8) == 0
+    assert _lib.lib().gi_diff_workspace_size(10, 0, 64, 64, 8, 8) == 0
     assert ctypes.sizeof(_lib.GiDiffParams) == 24
