@@ -21,6 +21,19 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = ROOT / "profiles" / "hbm_traffic_area_light_64x2048.md"
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` as recorded by the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, scaled by the
+    copy-kernel calibration); None when the summary has no such line."""
+    import re
+
+    if not TRAFFIC_SUMMARY.exists():
+        return None
+    m = re.search(rf"^traffic_bytes {re.escape(kernel)} (\d+)\s*$", TRAFFIC_SUMMARY.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def _parse():
@@ -33,6 +46,7 @@ def _parse():
     ap.add_argument("--cpu-samples", type=int, default=64, help="light samples in the CPU baseline sample")
     ap.add_argument("--no-footprint", action="store_true", help="skip counting the unique texels gathered")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-pass timing")
+    ap.add_argument("--diff-steps", type=int, default=10, help="timed steps of the differentiable pass")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -237,15 +251,17 @@ def main():
     else:
         dom = {"kernel": "k_soft (soft-shadow screen pass)", "achieved": screen_gbs, "bytes": screen_bytes,
                "ms": screen_ms}
+    dom_name = "k_raster" if dom["kernel"].startswith("k_raster") else "k_soft"
     roof = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom["kernel"],
+            "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name),
+            "traffic_source": str(TRAFFIC_SUMMARY.relative_to(ROOT)), "kernel": dom["kernel"],
             "algorithmic_bytes": int(dom["bytes"]), "ms": round(dom["ms"], 4),
             "screen_pass": {"kernel": "k_soft", "ms": round(screen_ms, 4), "GB/s": round(screen_gbs, 1),
                             "frac": round(screen_gbs / HBM_PEAK_GBS, 4), "bytes": int(screen_bytes),
-                            "unique_texels": int(texels)},
+                            "unique_texels": int(texels), "traffic": measured_traffic("k_soft")},
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
                            "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)}}
-    diff = None if args.no_diff else diff_timing(gi, torch, 10, 2)
+    diff = None if args.no_diff else diff_timing(gi, torch, args.diff_steps, 2)
     cpu = None
     if not args.no_cpu_baseline:
         maps_np = None
