@@ -74,6 +74,32 @@ class GiDiffParams(ctypes.Structure):
                 ("shadow_intensity", ctypes.c_double)]
 
 
+class GiBatchInstance(ctypes.Structure):
+    _fields_ = [("positions_dev", ctypes.c_void_p),
+                ("nverts", ctypes.c_int),
+                ("indices_dev", ctypes.c_void_p),
+                ("ntris", ctypes.c_int),
+                ("light_mvps_dev", ctypes.c_void_p),
+                ("nsamples", ctypes.c_int),
+                ("sm_width", ctypes.c_int),
+                ("sm_height", ctypes.c_int),
+                ("polygon_offset_factor", ctypes.c_float),
+                ("polygon_offset_units", ctypes.c_float),
+                ("maps_dev", ctypes.c_void_p),
+                ("gbuffer", GiGBuffer),
+                ("light_mvp", F16),
+                ("trans_dev", ctypes.c_void_p),
+                ("soft", GiSoftParams),
+                ("out_dev", ctypes.c_void_p)]
+
+
+class GiBatchTuning(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int),
+                ("raster_groups", ctypes.c_int),
+                ("screen_shape", ctypes.c_int),
+                ("screen_groups", ctypes.c_int)]
+
+
 class GiError(RuntimeError):
     def __init__(self, fn: str, status: int, msg: str):
         super().__init__(f"{fn} failed: {msg} ({status})")
@@ -119,6 +145,18 @@ _SIGS = {
                                              ctypes.c_int, ctypes.POINTER(GiGBuffer), ctypes.POINTER(GiDiffParams),
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                              ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
+    "gi_batch_workspace_size": (ctypes.c_size_t, [ctypes.POINTER(GiBatchInstance), ctypes.c_int]),
+    "gi_batch_create": (ctypes.c_int, [ctypes.POINTER(GiBatchInstance), ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "gi_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GiBatchTuning)]),
+    "gi_batch_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GiBatchTuning)]),
+    "gi_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.POINTER(GiBatchTuning), ctypes.POINTER(ctypes.c_float)]),
+    "gi_batch_graph_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "gi_batch_graph_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_batch_graph_destroy": (None, [ctypes.c_void_p]),
+    "gi_batch_destroy": (None, [ctypes.c_void_p]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -21,7 +21,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libgi.so"
 
-HIP_SOURCES = ["raster.hip", "screen.hip", "diff.hip"]
+HIP_SOURCES = ["raster.hip", "screen.hip", "diff.hip", "batch.hip"]
 CXX_SOURCES = ["gi_math.cpp"]
 
 ARCH = os.environ.get("GI_OFFLOAD_ARCH", "gfx950")

@@ -492,24 +492,27 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
 }
 
 // Clips the listed triangles; every non-empty fan piece becomes a setup
-// record.  Runs between the count passes, before k_bin_wide.  Workgroup
-// `part` of `nparts` takes every nparts-th stride of the map's list.
+// record.  Runs between the count passes, before k_bin_wide.  One thread per
+// (listed triangle, fan piece): each clips the triangle and keeps its own
+// piece, so the pieces of a triangle are set up in parallel.  Workgroup
+// `part` of `nparts` takes every nparts-th stride of the map's items.
+constexpr int kMaxPieces = kMaxPoly - 2;
+
 __device__ __forceinline__ void clip_body(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
                                           const BinBufs& b, int part, int nparts, int map) {
-    const uint32_t n = min(b.nclip[map], b.wcap);
+    const uint32_t n = min(b.nclip[map], b.wcap) * (uint32_t)kMaxPieces;
     float mvp[16];
     mvps.load(map, mvp);
-    for (uint32_t c = part * kRasterThreads + threadIdx.x; c < n; c += nparts * kRasterThreads) {
-        int tri = (int)b.clip[(size_t)map * b.wcap + c];
+    for (uint32_t item = part * kRasterThreads + threadIdx.x; item < n; item += nparts * kRasterThreads) {
+        const uint32_t c = item / kMaxPieces;
+        const int piece = (int)(item % kMaxPieces);
+        const int tri = (int)b.clip[(size_t)map * b.wcap + c];
         CV<0> v[3];
         load_tri<0>(mesh, mvp, tri, v);
-        int npieces = 1;
-        for (int piece = 0; piece < npieces; ++piece) {
-            Tri<0> t;
-            bool ok;
-            npieces = clip_and_setup<0>(v, piece, cfg, t, ok);
-            if (ok) push_record(b, map, t, (uint32_t)tri * 8u + (uint32_t)piece);
-        }
+        Tri<0> t;
+        bool ok;
+        clip_and_setup<0>(v, piece, cfg, t, ok);
+        if (ok) push_record(b, map, t, (uint32_t)tri * 8u + (uint32_t)piece);
     }
 }
 
@@ -875,6 +878,75 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
+
+// ---------------------------------------------------------------------------
+// Host side: workspace layout of the binned raster (shared by the single
+// passes and the batched frames)
+// ---------------------------------------------------------------------------
+struct Workspace {
+    BinBufs bins;
+    void* vis;  // gbuffer only
+};
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+inline uint32_t entry_cap(int ntris, int ntiles) {
+    unsigned long long cap = 4ull * (unsigned long long)ntris + 4ull * (unsigned long long)ntiles + 1024ull;
+    if (cap > 0x7FFFFFFFull) cap = 0x7FFFFFFFull;
+    return (uint32_t)cap;
+}
+
+inline uint32_t wide_cap(int ntris) { return 1024u + (uint32_t)ntris / 16u; }
+
+inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char* base, int ts = kTile) {
+    int tx = (w + ts - 1) / ts, ty = (h + ts - 1) / ts;
+    size_t ntiles = (size_t)tx * ty;
+    uint32_t cap = entry_cap(ntris, (int)ntiles);
+    uint32_t wcap = wide_cap(ntris);
+    size_t off = 0;
+    size_t o_counts = off; off = align256(off + sizeof(uint32_t) * ntiles * nmaps);
+    size_t o_nwide = off; off = align256(off + sizeof(uint32_t) * nmaps);
+    size_t o_wover = off; off = align256(off + sizeof(int) * nmaps);
+    size_t o_nclip = off; off = align256(off + sizeof(uint32_t) * nmaps);
+    size_t zero_end = off;  // [0, zero_end) is cleared before binning
+    size_t o_offsets = off; off = align256(off + sizeof(uint32_t) * (ntiles + 1) * nmaps);
+    size_t o_over = off; off = align256(off + sizeof(int) * nmaps);
+    size_t o_wide = off; off = align256(off + sizeof(Setup) * (size_t)wcap * nmaps);
+    size_t o_clip = off; off = align256(off + sizeof(uint32_t) * (size_t)wcap * nmaps);
+    size_t o_entries = off; off = align256(off + sizeof(uint32_t) * (size_t)cap * nmaps);
+    size_t o_vis = off;
+    if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
+    if (ws) {
+        BinBufs& b = ws->bins;
+        b.counts = reinterpret_cast<uint32_t*>(base + o_counts);
+        b.nwide = reinterpret_cast<uint32_t*>(base + o_nwide);
+        b.wide_over = reinterpret_cast<int*>(base + o_wover);
+        b.offsets = reinterpret_cast<uint32_t*>(base + o_offsets);
+        b.overflow = reinterpret_cast<int*>(base + o_over);
+        b.wide = reinterpret_cast<Setup*>(base + o_wide);
+        b.nclip = reinterpret_cast<uint32_t*>(base + o_nclip);
+        b.clip = reinterpret_cast<uint32_t*>(base + o_clip);
+        b.entries = reinterpret_cast<uint32_t*>(base + o_entries);
+        b.cap = cap;
+        b.wcap = wcap;
+        ws->vis = vis ? reinterpret_cast<void*>(base + o_vis) : nullptr;
+    }
+    (void)zero_end;
+    return off;
+}
+
+// Words to clear before binning: counts, nwide and wide_over are laid out
+// contiguously at the start of the workspace.
+inline size_t zero_words(int nmaps, const RasterCfg& cfg) {
+    size_t ntiles = (size_t)cfg.tiles_x * cfg.tiles_y;
+    size_t off = align256(sizeof(uint32_t) * ntiles * nmaps);
+    off = align256(off + sizeof(uint32_t) * nmaps);
+    off = align256(off + sizeof(int) * nmaps);
+    off = align256(off + sizeof(uint32_t) * nmaps);
+    return off / sizeof(uint32_t);
+}
+
+inline bool size_ok(int w, int h) { return w > 0 && h > 0 && w <= GI_MAX_EDGE && h <= GI_MAX_EDGE; }
 
 }  // namespace
 }  // namespace gi

@@ -92,35 +92,18 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
 int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
                         const float light_mvp[16], const float* trans_dev, const gi_soft_params* p, float* out_dev,
                         gi_stream stream) {
-    if (!gbuffer_ok(gbuffer) || !maps_dev || !light_mvp || !trans_dev || !p || !out_dev) return GI_EINVAL;
-    if (nsamples <= 0 || nsamples > GI_MAX_SAMPLES) return GI_EINVAL;
-    if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE) return GI_EINVAL;
-    if (!finite_all(light_mvp, 16) || !std::isfinite(p->shadow_intensity) || !std::isfinite(p->penumbra))
-        return GI_ENONFINITE;
-    if (p->pcf && (p->kernel_order <= 0 || p->kernel_order > 14 || !(p->penumbra > 0.0f) || p->penumbra > 64.0f))
-        return GI_EINVAL;
-    if ((reinterpret_cast<uintptr_t>(trans_dev) & 15u) != 0) return GI_EINVAL;
     SoftUniforms u;
-    for (int i = 0; i < 12; ++i) u.m[i] = light_mvp[i];
-    u.si = p->shadow_intensity;
-    u.nsamples = nsamples;
-    u.pcf = p->pcf ? 1 : 0;
-    u.ntaps = 1;
-    for (int i = 0; i < kMaxTaps; ++i) u.taps[i] = 0.0f;
-    if (p->pcf) {
-        u.ntaps = pcf_taps(p->penumbra, p->kernel_order, p->inclusive != 0, u.taps, kMaxTaps);
-        if (u.ntaps <= 0) return GI_ENOTSUP;
-    }
+    int rc = soft_setup(gbuffer, maps_dev, nsamples, smw, smh, light_mvp, trans_dev, p, out_dev, u);
+    if (rc != GI_OK) return rc;
     GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
     dim3 grid = grid_for(gb.w, gb.h), block(kBlockX, kBlockY);
-    if (u.pcf && u.ntaps == 3)
-        hipLaunchKernelGGL(k_soft<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
-    else if (u.pcf && u.ntaps == 4)
-        hipLaunchKernelGGL(k_soft<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
-    else
-        hipLaunchKernelGGL(k_soft<0>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
+    switch (soft_variant(u)) {
+    case 3: hipLaunchKernelGGL(k_soft<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    case 4: hipLaunchKernelGGL(k_soft<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    default: hipLaunchKernelGGL(k_soft<0>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    }
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

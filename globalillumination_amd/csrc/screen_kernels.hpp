@@ -415,16 +415,11 @@ __device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms
     }
 }
 
-// AccurateSoftShadow.frag:52-133: one thread per pixel, samples in order
-// 0..N-1.  Blocks are ordered XCD-aware (pixel_of_block) so each XCD's L2
-// serves one contiguous band of the view.
+// AccurateSoftShadow.frag:52-133 for pixel (x, y): samples in order 0..N-1.
 template <int NT>
-__global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
-                                                            int smh, const float4* __restrict__ trans,
-                                                            SoftUniforms u, float* __restrict__ out) {
-    int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
-    if (x >= gb.w || y >= gb.h) return;
+__device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* __restrict__ maps, int smw, int smh,
+                                           const float4* __restrict__ trans, const SoftUniforms& u,
+                                           float* __restrict__ out, int x, int y) {
     const size_t plane = (size_t)gb.w * gb.h;
     const size_t k = (size_t)y * gb.w + x;
     const float vx = gb.pos[k];
@@ -455,6 +450,18 @@ __global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const
     out[k] = acc / count;
 }
 
+// One thread per pixel.  Blocks are ordered XCD-aware (pixel_of_block) so each
+// XCD's L2 serves one contiguous band of the view.
+template <int NT>
+__global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
+                                                            int smh, const float4* __restrict__ trans,
+                                                            SoftUniforms u, float* __restrict__ out) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    soft_pixel<NT>(gb, maps, smw, smh, trans, u, out, x, y);
+}
+
 // Tap values of the PCF loop `for (w = -offset; w < offset (or <=); w += step)`
 // with step = 2*offset/kernelOrder, accumulated in FP32 exactly as the shader
 // loop does (AccurateSoftShadow.frag:101-109, Shadow.frag:93-99).
@@ -467,6 +474,45 @@ int pcf_taps(float offset, int kernel_order, bool inclusive, float* taps, int ma
     }
     return n;
 }
+
+// Host side: argument checks of the soft-shadow pass and its uniforms
+// (shared by the single pass and the batched frames).  Returns a GI_* status.
+inline bool soft_finite(const float* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+inline int soft_setup(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
+                      const float light_mvp[16], const float* trans_dev, const gi_soft_params* p,
+                      const float* out_dev, SoftUniforms& u) {
+    if (!gbuffer || !gbuffer->position_dev || gbuffer->width <= 0 || gbuffer->height <= 0 ||
+        gbuffer->width > GI_MAX_EDGE || gbuffer->height > GI_MAX_EDGE)
+        return GI_EINVAL;
+    if (!maps_dev || !light_mvp || !trans_dev || !p || !out_dev) return GI_EINVAL;
+    if (nsamples <= 0 || nsamples > GI_MAX_SAMPLES) return GI_EINVAL;
+    if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE) return GI_EINVAL;
+    if (!soft_finite(light_mvp, 16) || !std::isfinite(p->shadow_intensity) || !std::isfinite(p->penumbra))
+        return GI_ENONFINITE;
+    if (p->pcf && (p->kernel_order <= 0 || p->kernel_order > 14 || !(p->penumbra > 0.0f) || p->penumbra > 64.0f))
+        return GI_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(trans_dev) & 15u) != 0) return GI_EINVAL;
+    for (int i = 0; i < 12; ++i) u.m[i] = light_mvp[i];
+    u.si = p->shadow_intensity;
+    u.nsamples = nsamples;
+    u.pcf = p->pcf ? 1 : 0;
+    u.ntaps = 1;
+    for (int i = 0; i < kMaxTaps; ++i) u.taps[i] = 0.0f;
+    if (p->pcf) {
+        u.ntaps = pcf_taps(p->penumbra, p->kernel_order, p->inclusive != 0, u.taps, kMaxTaps);
+        if (u.ntaps <= 0) return GI_ENOTSUP;
+    }
+    return GI_OK;
+}
+
+// Template variant of k_soft / soft_pixel for uniforms u: 3 or 4 taps per
+// axis take the row-load path, everything else the generic loop.
+inline int soft_variant(const SoftUniforms& u) { return (u.pcf && (u.ntaps == 3 || u.ntaps == 4)) ? u.ntaps : 0; }
 
 }  // namespace
 }  // namespace gi

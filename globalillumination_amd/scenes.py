@@ -330,6 +330,32 @@ def config3(nsamples: int = 64, view_w: int = 1920, view_h: int = 1080, sm: int 
                     extra={"offsets": offs, "side": 0.5, "kernel_order": 3, "penumbra": 1.0})
 
 
+def config4(n: int = 32, seed: int = 4, map_sizes=(512, 1024, 2048, 4096),
+            views=((640, 480), (1280, 720), (1920, 1080)), tri_range=(5000, 300000), max_samples: int = 16,
+            soup_seed_base: int = 1000) -> list:
+    """Batched frames: `n` independent instances (seed 4).  Each draws a square
+    shadow-map size from `map_sizes`, a view from `views`, a triangle soup of
+    log-uniform size in `tri_range` (over a 40x40 ground quad) and 1 to
+    `max_samples` uniform samples of a 0.5-wide square light centred at a
+    random point above the scene and looking at the origin."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n):
+        sm = int(map_sizes[int(rng.integers(len(map_sizes)))])
+        vw, vh = views[int(rng.integers(len(views)))]
+        lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+        ntri = int(round(float(np.exp(rng.uniform(lo, hi)))))
+        ns = int(rng.integers(1, max_samples + 1))
+        soup = triangle_soup(ntri, seed=soup_seed_base + i)
+        mesh = Mesh.concat(soup, ground_quad(40.0, 40.0, 0.0))
+        center = (float(rng.uniform(-4.0, 4.0)), float(rng.uniform(7.0, 10.0)), float(rng.uniform(-4.0, 4.0)))
+        offs = rng.uniform(-0.5, 0.5, size=(ns, 2))
+        lights = area_light_samples(center, (0.0, 0.0, 0.0), 0.5, offs, sm, sm)
+        cam = View((0.0, 9.0, 14.0), (0.0, 1.0, 0.0), int(vw), int(vh), fov=60.0)
+        out.append(Workload(f"cfg4_{i}", mesh, cam, lights, "soft", extra={"triangles": ntri, "samples": ns}))
+    return out
+
+
 def config5(nsamples: int = 16, view: int = 512, sm: int = 1024, freq: int = 50) -> Workload:
     """Differentiable soft shadows: 16-sample light, 1024^2 maps, 512^2 view,
     50k-triangle displaced icosphere (seed 5; 20*50^2 = 50,000 triangles)."""

@@ -239,6 +239,69 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
                           double* grad_biased_mvps_dev, double* grad_mvps_dev, float* grad_positions_dev,
                           void* workspace_dev, size_t workspace_bytes, gi_stream stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched frames                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart: the reference renders one light at a time.  A
+ * batch is many independent instances, each the work of one
+ * renderMonteCarlo frame (SoftShadowMapping/src/main.cpp:756-811): its light
+ * samples' depth maps (gi_depth_pass) and the soft-shadow screen pass over its
+ * camera G-buffer (gi_soft_shadow_pass).  Every output is bit-identical to
+ * running those two passes per instance.  Instances may differ in mesh, map
+ * size, sample count, view size and soft-shadow parameters. */
+typedef struct gi_batch_instance {
+    const float* positions_dev;  /* nverts x 3 */
+    int nverts;
+    const int32_t* indices_dev;  /* ntris x 3 */
+    int ntris;
+    const float* light_mvps_dev; /* nsamples x 16: unbiased P*V per sample (depth maps) */
+    int nsamples;
+    int sm_width, sm_height;
+    float polygon_offset_factor, polygon_offset_units;
+    float* maps_dev;             /* nsamples x sm_height x sm_width, written (16-byte aligned) */
+    gi_gbuffer gbuffer;          /* camera G-buffer (position planes) */
+    float light_mvp[16];         /* B*lightMVP of the last sample (columns 0-2 used) */
+    const float* trans_dev;      /* nsamples x 4 (16-byte aligned) */
+    gi_soft_params soft;
+    float* out_dev;              /* width x height, written */
+} gi_batch_instance;
+
+/* Launch shape of a batch; gi_batch_autotune measures the candidates. */
+typedef struct gi_batch_tuning {
+    int tile;          /* raster tile edge: 32 or 64 */
+    int raster_groups; /* persistent workgroups per CU in the raster stages: 1..8 */
+    int screen_shape;  /* screen-pass pixel block: 0 = 64x4, 1 = 32x8, 2 = 16x16 */
+    int screen_groups; /* persistent workgroups per CU in the screen stage: 1..8 */
+} gi_batch_tuning;
+
+typedef struct gi_batch gi_batch;             /* plan: host object, device tables in the workspace */
+typedef struct gi_batch_graph gi_batch_graph; /* a plan's launches captured in a hipGraph */
+
+enum { GI_MAX_BATCH = 4096 };
+
+/* Device workspace a plan needs (0 when an instance is invalid). */
+size_t gi_batch_workspace_size(const gi_batch_instance* inst, int n);
+/* Validates every instance, lays out and uploads the plan's tables into the
+ * workspace (synchronously; set-up, not frame work).  The instances' buffers
+ * must stay valid while the plan is used. */
+int gi_batch_create(const gi_batch_instance* inst, int n, void* workspace_dev, size_t workspace_bytes,
+                    gi_batch** out);
+int gi_batch_set_tuning(gi_batch* b, const gi_batch_tuning* t);
+int gi_batch_get_tuning(const gi_batch* b, gi_batch_tuning* t);
+/* Enqueues the whole batch: the raster stages as persistent kernels over all
+ * instances, then the screen pass as one persistent kernel per PCF variant.
+ * Launches only (no allocation, copy or memset), so it can be captured. */
+int gi_batch_run(gi_batch* b, gi_stream stream);
+/* Times every tuning candidate on `stream` (reps runs each, after one
+ * warm-up), keeps the fastest in the plan and returns it with its time. */
+int gi_batch_autotune(gi_batch* b, gi_stream stream, int reps, gi_batch_tuning* best, float* best_ms);
+/* Captures gi_batch_run (with the plan's current tuning) into a graph. */
+int gi_batch_graph_create(gi_batch* b, gi_batch_graph** out);
+int gi_batch_graph_launch(gi_batch_graph* g, gi_stream stream);
+void gi_batch_graph_destroy(gi_batch_graph* g);
+void gi_batch_destroy(gi_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,175 @@
+"""Batched frames (gi_batch_*): every depth map and soft-shadow value must be
+bit-identical to the single-instance passes (gi_depth_pass +
+gi_soft_shadow_pass, which the other GPU tests check against the oracle),
+for every tuning, through a captured graph replayed several times, and on
+the full 32-instance batch with mixed map sizes, views, meshes and sample
+counts."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(workloads, softs):
+    from globalillumination_amd import passes
+    from globalillumination_amd.batch import BatchInstance
+
+    out = []
+    for wl, soft in zip(workloads, softs):
+        dm = passes.DeviceMesh.from_mesh(wl.mesh)
+        mvps = np.stack([passes.view_matrices(v)[2] for v in wl.lights])
+        g = passes.render_gbuffer(dm, passes.view_matrices(wl.camera)[2], wl.camera.width, wl.camera.height)
+        sm = wl.lights[0].width
+        out.append(BatchInstance(dm, mvps, sm, sm, g, soft, polygon_offset=(1.0, 2.0) if len(out) % 2 else (0.0, 0.0)))
+    return out
+
+
+def _small():
+    from globalillumination_amd import scenes
+    from globalillumination_amd.passes import SoftParams
+
+    wls = scenes.config4(n=7, seed=40, map_sizes=(96, 128, 200, 256), views=((96, 64), (160, 90), (64, 64)),
+                         tri_range=(300, 4000), max_samples=5)
+    # an instance with no occluder triangles at all
+    wls[3].mesh = scenes.Mesh(wls[3].mesh.positions[:3].copy(), wls[3].mesh.normals[:3].copy(),
+                              np.zeros((0, 3), np.int32))
+    softs = [SoftParams(pcf=True, kernelOrder=3, penumbra=1.0, inclusive=(i % 3 != 1)) if i % 3 != 2
+             else SoftParams(pcf=False) for i in range(len(wls))]
+    return wls, softs
+
+
+def _reference(insts):
+    import torch
+
+    maps, outs = [], []
+    for inst in insts:
+        inst.maps.fill_(-1.0)
+        inst.out.fill_(-1.0)
+        inst.run_unbatched()
+        maps.append(inst.maps.clone())
+        outs.append(inst.out.clone())
+    torch.cuda.synchronize()
+    return maps, outs
+
+
+def _clear(insts):
+    for inst in insts:
+        inst.maps.fill_(-1.0)
+        inst.out.fill_(-1.0)
+
+
+def _same(insts, maps, outs, what):
+    import torch
+
+    torch.cuda.synchronize()
+    for k, inst in enumerate(insts):
+        assert torch.equal(inst.maps.view(torch.int32), maps[k].view(torch.int32)), f"{what}: maps of instance {k}"
+        assert torch.equal(inst.out.view(torch.int32), outs[k].view(torch.int32)), f"{what}: shadow of instance {k}"
+
+
+@pytest.fixture(scope="module")
+def small(cuda, gi):
+    wls, softs = _small()
+    insts = _instances(wls, softs)
+    maps, outs = _reference(insts)
+    return insts, maps, outs
+
+
+@pytest.mark.parametrize("tile,rg,shape,sg", [(64, 4, 0, 4), (32, 1, 1, 2), (32, 8, 2, 1), (64, 2, 2, 8)])
+def test_batch_matches_single_passes(small, tile, rg, shape, sg):
+    from globalillumination_amd.batch import Batch
+
+    insts, maps, outs = small
+    print(f"batch: 7 instances, tuning tile={tile} raster_groups={rg} shape={shape} screen_groups={sg}", flush=True)
+    b = Batch(insts)
+    b.set_tuning(tile, rg, shape, sg)
+    _clear(insts)
+    b.run()
+    _same(insts, maps, outs, f"tuning {tile}/{rg}/{shape}/{sg}")
+    # the frame is a pure function of the inputs: a second run gives the same bits
+    b.run()
+    _same(insts, maps, outs, "second run")
+    b.close()
+
+
+def test_batch_graph_replays(small):
+    import torch
+    from globalillumination_amd.batch import Batch
+
+    insts, maps, outs = small
+    print("batch graph: capture, 3 replays with eager work between", flush=True)
+    b = Batch(insts)
+    b.set_tuning(32, 2, 1, 4)
+    g = b.graph()
+    for r in range(3):
+        _clear(insts)
+        torch.cuda.synchronize()
+        insts[0].run_unbatched()  # eager work between replays, then overwritten by the replay
+        g.launch()
+        _same(insts, maps, outs, f"replay {r}")
+    g.close()
+    b.close()
+
+
+def test_batch_autotune(small):
+    from globalillumination_amd.batch import Batch
+
+    insts, maps, outs = small
+    print("batch autotune (1 rep per candidate)", flush=True)
+    b = Batch(insts)
+    tuning, ms = b.autotune(reps=1)
+    assert tuning["tile"] in (32, 64) and 1 <= tuning["raster_groups"] <= 8 and 1 <= tuning["screen_groups"] <= 8
+    assert ms > 0.0
+    _clear(insts)
+    b.run()
+    _same(insts, maps, outs, "after autotune")
+    b.close()
+
+
+def test_batch_rejects_bad_plans(small, gi):
+    import ctypes
+
+    import torch
+    from globalillumination_amd import _lib
+    from globalillumination_amd.batch import Batch
+
+    insts, _, _ = small
+    c = (_lib.GiBatchInstance * 1)(insts[0].to_c())
+    need = _lib.lib().gi_batch_workspace_size(c, 1)
+    assert need > 0
+    ws = torch.empty(int(need), dtype=torch.uint8, device="cuda")
+    h = ctypes.c_void_p()
+    assert _lib.lib().gi_batch_create(c, 1, ctypes.c_void_p(ws.data_ptr()), need - 256, ctypes.byref(h)) == \
+        _lib.GI_ENOSPACE
+    bad = (_lib.GiBatchInstance * 1)(insts[0].to_c())
+    bad[0].nsamples = 0
+    assert _lib.lib().gi_batch_workspace_size(bad, 1) == 0
+    bad = (_lib.GiBatchInstance * 1)(insts[0].to_c())
+    bad[0].trans_dev = insts[0].trans_dev.data_ptr() + 4  # misaligned
+    assert _lib.lib().gi_batch_create(bad, 1, ctypes.c_void_p(ws.data_ptr()), need, ctypes.byref(h)) == \
+        _lib.GI_EINVAL
+    b = Batch(insts[:1])
+    with pytest.raises(_lib.GiError):
+        b.set_tuning(48, 1, 0, 1)
+    b.close()
+
+
+def test_batch_full_workload(cuda, gi):
+    """32 instances: maps 512^2-4096^2, views 640x480-1920x1080, 5k-300k
+    triangles, 1-16 samples (seed 4), autotuned, through a graph."""
+    from globalillumination_amd import scenes
+    from globalillumination_amd.batch import Batch
+    from globalillumination_amd.passes import SoftParams
+
+    print("batch: full 32-instance workload", flush=True)
+    wls = scenes.config4()
+    insts = _instances(wls, [SoftParams(pcf=True, kernelOrder=3, penumbra=1.0, inclusive=False)] * len(wls))
+    maps, outs = _reference(insts)
+    b = Batch(insts)
+    b.autotune(reps=1)
+    g = b.graph()
+    _clear(insts)
+    g.launch()
+    _same(insts, maps, outs, "full batch, graph")
+    g.close()
+    b.close()
