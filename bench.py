@@ -47,6 +47,8 @@ def _parse():
     ap.add_argument("--no-footprint", action="store_true", help="skip counting the unique texels gathered")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-pass timing")
     ap.add_argument("--diff-steps", type=int, default=10, help="timed steps of the differentiable pass")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames timing")
+    ap.add_argument("--batch-steps", type=int, default=5, help="timed steps of the batched frames")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -163,6 +165,61 @@ def diff_timing(gi, torch, steps, warmup):
             "steps": steps}
 
 
+def batch_timing(gi, torch, steps):
+    """The 32-instance batch (maps 512^2-4096^2, views 640x480-1920x1080,
+    5k-300k-triangle soups, 1-16 light samples, seed 4): the per-instance
+    passes one after another, the batch's persistent launches, and the same
+    launches replayed from a hipGraph, after the autotuner has picked the
+    launch shape.  HIP events on the current stream; inputs resident."""
+    import numpy as np
+
+    from globalillumination_amd import passes
+    from globalillumination_amd.batch import Batch, BatchInstance
+
+    wls = gi.scenes.config4()
+    soft = gi.SoftParams(pcf=True, kernelOrder=3, penumbra=1.0, inclusive=False)
+    insts = []
+    for wl in wls:
+        dm = gi.DeviceMesh.from_mesh(wl.mesh)
+        mvps = np.stack([passes.view_matrices(v)[2] for v in wl.lights])
+        g = gi.render_gbuffer(dm, passes.view_matrices(wl.camera)[2], wl.camera.width, wl.camera.height)
+        insts.append(BatchInstance(dm, mvps, wl.lights[0].width, wl.lights[0].width, g, soft))
+    b = Batch(insts)
+    tuning, tuned_ms = b.autotune(reps=2)
+    graph = b.graph()
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def unbatched():
+        for inst in insts:
+            inst.run_unbatched()
+
+    t_unb = timed(unbatched)
+    t_run = timed(b.run)
+    t_graph = timed(graph.launch)
+    graph.close()
+    b.close()
+    npx = sum(wl.camera.width * wl.camera.height for wl in wls)
+    return {"workload": "batched frames: 32 instances, maps 512^2-4096^2, views 640x480-1920x1080, "
+                        "5k-300k-triangle soups, 1-16 light samples (seed 4)",
+            "instances": len(wls), "view_pixels": int(npx), "light_samples": int(sum(len(w.lights) for w in wls)),
+            "triangles": int(sum(w.mesh.ntris for w in wls)),
+            "unbatched_ms": round(t_unb, 3), "batched_ms": round(t_run, 3), "graph_ms": round(t_graph, 3),
+            "speedup_graph_vs_unbatched": round(t_unb / t_graph, 3),
+            "graph_Mpix_per_s": round(npx / (t_graph * 1e-3) / 1e6, 1),
+            "tuning": tuning, "autotune_ms": round(tuned_ms, 3), "steps": steps}
+
+
 def main():
     args = _parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -262,6 +319,7 @@ def main():
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
                            "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)}}
     diff = None if args.no_diff else diff_timing(gi, torch, args.diff_steps, 2)
+    batched = None if args.no_batch else batch_timing(gi, torch, args.batch_steps)
     cpu = None
     if not args.no_cpu_baseline:
         maps_np = None
@@ -274,7 +332,7 @@ def main():
                                    "view, 3x3 PCF per sample, 100,822-triangle displaced icosphere + ground",
                        "global_batch": 1, "samples": S, "shadow_map": smw, "view": [W, H],
                        "triangles": wl.mesh.ntris},
-            "roofline": roof, "cpu_baseline": cpu, "differentiable": diff}
+            "roofline": roof, "cpu_baseline": cpu, "differentiable": diff, "batched": batched}
     print(json.dumps(line), flush=True)
 
 

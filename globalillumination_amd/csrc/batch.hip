@@ -45,7 +45,8 @@ constexpr int kXcds = 8;
 constexpr int kScreenThreads = 256;
 constexpr int kNtGroups = 3;  // soft variants: generic loop, 3 taps per axis, 4 taps per axis
 constexpr int kShapes = 3;    // screen blocks 64x4, 32x8, 16x16
-constexpr int kCounters = 64;  // [0] raster queue, [1 + 8 g + x] screen queue of group g on XCD x
+constexpr int kCounters = 64;  // [0] raster queue, [1 + 8 g + x] screen queue of group g on XCD x,
+                               // [63] the autotuner's overflow flag
 
 struct RasterJob {
     MeshView mesh;
@@ -163,6 +164,17 @@ __global__ __launch_bounds__(kScanThreads) void kb_scan(RasterTable t) {
         const RasterJob& J = t.jobs[j];
         scan_body(J.b, J.ntiles, (int)(item - pre[j]));
         __syncthreads();
+    }
+}
+
+// flag = 1 when any map of the batch overflowed its tile lists (its raster
+// then went through kb_raster_fallback).  One thread per (instance, map).
+__global__ void kb_overflow_any(RasterTable t, uint32_t* __restrict__ flag) {
+    const uint32_t* pre = t.prefix + kStScan * (t.njobs + 1);
+    const uint32_t total = pre[t.njobs];
+    for (uint32_t item = blockIdx.x * blockDim.x + threadIdx.x; item < total; item += gridDim.x * blockDim.x) {
+        const int j = find_job(pre, t.njobs, item);
+        if (t.jobs[j].b.overflow[item - pre[j]]) atomicOr(flag, 1u);
     }
 }
 
@@ -390,6 +402,20 @@ int run_screen(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
+// Whether the last depth run of tuning T overflowed any map's tile lists.
+int overflowed(gi_batch* b, const gi_batch_tuning& T, hipStream_t s, bool* any) {
+    const RasterVariant& R = b->rv[T.tile == 32 ? 0 : 1];
+    RasterTable tab{R.jobs, R.prefix, b->n, b->counters};
+    uint32_t* flag = b->counters + kCounters - 1;  // not used by the screen queues of the groups
+    hipLaunchKernelGGL(kb_zero, dim3(1), dim3(kCounters), 0, s, flag, 1);
+    hipLaunchKernelGGL(kb_overflow_any, dim3(64), dim3(256), 0, s, tab, flag);
+    uint32_t h = 0;
+    if (hipMemcpyAsync(&h, flag, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess) return GI_EDEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess) return GI_EDEVICE;
+    *any = h != 0;
+    return GI_OK;
+}
+
 int timed(gi_batch* b, const gi_batch_tuning& T, hipStream_t s, int reps, bool depth, float* ms) {
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return GI_EDEVICE;
@@ -398,6 +424,18 @@ int timed(gi_batch* b, const gi_batch_tuning& T, hipStream_t s, int reps, bool d
         return GI_EDEVICE;
     }
     int rc = depth ? run_depth(b, T, s) : run_screen(b, T, s);  // warm-up
+    if (depth && rc == GI_OK) {
+        // a tiling whose tile lists overflow takes the slow fallback raster:
+        // it is not a candidate
+        bool any = false;
+        rc = overflowed(b, T, s, &any);
+        if (rc == GI_OK && any) {
+            hipEventDestroy(e0);
+            hipEventDestroy(e1);
+            *ms = 1e30f;
+            return GI_OK;
+        }
+    }
     if (rc == GI_OK && hipEventRecord(e0, s) != hipSuccess) rc = GI_EDEVICE;
     for (int r = 0; r < reps && rc == GI_OK; ++r) rc = depth ? run_depth(b, T, s) : run_screen(b, T, s);
     if (rc == GI_OK && hipEventRecord(e1, s) != hipSuccess) rc = GI_EDEVICE;

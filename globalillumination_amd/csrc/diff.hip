@@ -38,9 +38,11 @@ namespace gi {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTexPerThread = 16;  // texels per thread in the texel adjoint
+constexpr int kTexPerThread = 4;   // texels per thread in the texel adjoint
 constexpr int kGatherMax = 256;    // bbox texels up to which a triangle gathers its own adjoint
 constexpr int kLargeSlots = 4;     // large triangles a texel block sums in LDS
+constexpr int kMapsPerThread = 1;  // maps a triangle-adjoint thread walks (vertex gradients summed over them)
+constexpr int kGatherBatch = 8;    // bounding-box texels whose keys a gather loads together
 
 // Per-block sum of one large triangle's G * lambda_j (tri = -1: unused).
 struct LargePart {
@@ -362,7 +364,8 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const unsigned long
                                                              const unsigned char* __restrict__ large, int ntris,
                                                              const float* __restrict__ grad_depth, DiffCfg c,
                                                              double* __restrict__ wsum_all,
-                                                             LargePart* __restrict__ parts) {
+                                                             LargePart* __restrict__ parts,
+                                                             unsigned char* __restrict__ touched) {
     __shared__ int slot_tri[kLargeSlots];
     __shared__ double slot_w[kLargeSlots][3];
     if (threadIdx.x < kLargeSlots) {
@@ -377,6 +380,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const unsigned long
     double* __restrict__ wsum = wsum_all + (size_t)i * ntris * 3;
     const TriPlane* __restrict__ pl = planes + (size_t)i * ntris;
     const unsigned char* __restrict__ lg = large + (size_t)i * ntris;
+    unsigned char* __restrict__ tch = touched + (size_t)i * ntris;
     const int lane = threadIdx.x % 64;
     const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
     const uint32_t base = (uint32_t)blockIdx.x * (kTexPerThread * kThreads) + threadIdx.x;
@@ -389,6 +393,7 @@ __global__ __launch_bounds__(kThreads) void k_diff_texel_bwd(const unsigned long
         const float G = (k < msz) ? gd[k] : 0.0f;
         if (G != 0.0f) {
             const int t = key_tri(vk[k], ntris);
+            if (t >= 0 && !lg[t]) tch[t] = 1;  // a small triangle with gradient: k_diff_tri_bwd gathers it
             if (t >= 0 && lg[t]) {
                 const TriPlane& p = pl[t];
                 {
@@ -487,7 +492,8 @@ __global__ __launch_bounds__(kThreads) void k_diff_large_reduce(const LargePart*
     }
 }
 
-// Triangle adjoint.  grid (triangle blocks, S).  w_j = sum of G * lambda_j
+// Triangle adjoint.  grid (triangle blocks, S / kMapsPerThread): a thread
+// walks kMapsPerThread maps of one triangle.  w_j = sum of G * lambda_j
 // over the triangle's texels in map i: gathered here over the triangle's
 // bounding box (texels whose visibility key names it) for small triangles,
 // read from the texel pass's sums for large ones.  The texel depths then move
@@ -502,75 +508,104 @@ __global__ __launch_bounds__(kThreads) void k_diff_tri_bwd(const float* __restri
                                                            const TriPlane* __restrict__ planes,
                                                            const unsigned long long* __restrict__ vis,
                                                            const float* __restrict__ grad_depth,
-                                                           const double* __restrict__ wsum_all, DiffCfg c,
+                                                           const double* __restrict__ wsum_all,
+                                                           const unsigned char* __restrict__ touched, DiffCfg c,
                                                            float* __restrict__ grad_pos,
                                                            double* __restrict__ partial) {
     const int t = blockIdx.x * kThreads + threadIdx.x;
-    const int i = blockIdx.y;
-    double gm[16];
+    // vertex gradients summed over this thread's maps, added once at the end
+    double gvs[3][3];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) gm[e] = 0.0;
-    double w[3] = {0.0, 0.0, 0.0};
-    if (t < ntris) {
-        const TriPlane p = planes[(size_t)i * ntris + t];
-        if (plane_ok(p)) {
-            if (plane_large(p)) {
-                const double* ws = wsum_all + ((size_t)i * ntris + t) * 3;
-                w[0] = ws[0];
-                w[1] = ws[1];
-                w[2] = ws[2];
-            } else {
-                const size_t moff = (size_t)i * c.smw * c.smh;
-                for (int y = p.by0; y <= p.by1; ++y) {
-                    for (int x = p.bx0; x <= p.bx1; ++x) {
-                        const size_t k = moff + (size_t)y * c.smw + x;
-                        if (key_tri(vis[k], ntris) != t) continue;
-                        const float G = grad_depth[k];
-                        if (G == 0.0f) continue;
-                        const double dx = ((double)x + 0.5) - p.X0, dy = ((double)y + 0.5) - p.Y0;
-                        const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
-                        w[0] += (double)G * (1.0 - l1 - l2);
-                        w[1] += (double)G * l1;
-                        w[2] += (double)G * l2;
+    for (int j = 0; j < 3; ++j) gvs[j][0] = gvs[j][1] = gvs[j][2] = 0.0;
+    const int i0 = blockIdx.y * kMapsPerThread;
+    const int i1 = min(i0 + kMapsPerThread, c.S);
+    for (int i = i0; i < i1; ++i) {
+        double gm[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) gm[e] = 0.0;
+        double w[3] = {0.0, 0.0, 0.0};
+        if (t < ntris) {
+            const TriPlane p = planes[(size_t)i * ntris + t];
+            if (plane_ok(p)) {
+                if (plane_large(p)) {
+                    const double* ws = wsum_all + ((size_t)i * ntris + t) * 3;
+                    w[0] = ws[0];
+                    w[1] = ws[1];
+                    w[2] = ws[2];
+                } else if (touched[(size_t)i * ntris + t]) {
+                    // bounding-box texels in batches whose loads are issued together
+                    const size_t moff = (size_t)i * c.smw * c.smh;
+                    const int bw = p.bx1 - p.bx0 + 1, area = bbox_area(p);
+                    for (int q0 = 0; q0 < area; q0 += kGatherBatch) {
+                        unsigned long long key[kGatherBatch];
+                        float G[kGatherBatch];
+#pragma unroll
+                        for (int u = 0; u < kGatherBatch; ++u) {
+                            const int q = q0 + u;
+                            const size_t k = moff + (size_t)(p.by0 + q / bw) * c.smw + (p.bx0 + q % bw);
+                            key[u] = q < area ? vis[k] : ~0ull;
+                            G[u] = q < area ? grad_depth[k] : 0.0f;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kGatherBatch; ++u) {
+                            if (G[u] == 0.0f || key_tri(key[u], ntris) != t) continue;
+                            const int q = q0 + u;
+                            const double dx = ((double)(p.bx0 + q % bw) + 0.5) - p.X0;
+                            const double dy = ((double)(p.by0 + q / bw) + 0.5) - p.Y0;
+                            const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
+                            w[0] += (double)G[u] * (1.0 - l1 - l2);
+                            w[1] += (double)G[u] * l1;
+                            w[2] += (double)G[u] * l2;
+                        }
                     }
                 }
             }
         }
-    }
-    if (w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0) {
-        double m[16];
-        load_m(mvps + 16 * i, m);
-        double X[3], Y[3], Z[3], C[3][4];
-        const bool ok = tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C);
-        const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
-        if (ok && D != 0.0) {
-            const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
-            const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
+        if (w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0) {
+            double m[16];
+            load_m(mvps + 16 * i, m);
+            double X[3], Y[3], Z[3], C[3][4];
+            const bool ok = tri_screen(pos, idx, t, m, c.smw, c.smh, X, Y, Z, C);
+            const double D = (X[1] - X[0]) * (Y[2] - Y[0]) - (X[2] - X[0]) * (Y[1] - Y[0]);
+            if (ok && D != 0.0) {
+                const double gx = ((Z[1] - Z[0]) * (Y[2] - Y[0]) - (Z[2] - Z[0]) * (Y[1] - Y[0])) / D;
+                const double gy = ((X[1] - X[0]) * (Z[2] - Z[0]) - (X[2] - X[0]) * (Z[1] - Z[0])) / D;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const double gX = -w[j] * gx * (0.5 * c.smw);
-                const double gY = -w[j] * gy * (0.5 * c.smh);
-                const double gZ = w[j] * 0.5;
-                const double iw = 1.0 / C[j][3];
-                const double gc[4] = {gX * iw, gY * iw, gZ * iw,
-                                      -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
-                const int v = idx[3 * t + j];
-                const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
+                for (int j = 0; j < 3; ++j) {
+                    const double gX = -w[j] * gx * (0.5 * c.smw);
+                    const double gY = -w[j] * gy * (0.5 * c.smh);
+                    const double gZ = w[j] * 0.5;
+                    const double iw = 1.0 / C[j][3];
+                    const double gc[4] = {gX * iw, gY * iw, gZ * iw,
+                                          -(gX * C[j][0] + gY * C[j][1] + gZ * C[j][2]) * iw * iw};
+                    const int v = idx[3 * t + j];
+                    const double vp[4] = {pos[3 * v], pos[3 * v + 1], pos[3 * v + 2], 1.0};
 #pragma unroll
-                for (int col = 0; col < 4; ++col)
+                    for (int col = 0; col < 4; ++col)
 #pragma unroll
-                    for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
+                        for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * vp[col];
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    double g = 0.0;
+                    for (int a = 0; a < 3; ++a) {
+                        double g = 0.0;
 #pragma unroll
-                    for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
-                    unsafeAtomicAdd(&grad_pos[3 * v + a], (float)g);
+                        for (int row = 0; row < 4; ++row) g += m[a * 4 + row] * gc[row];
+                        gvs[j][a] += g;
+                    }
                 }
             }
         }
+        block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
     }
-    block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
+    if (t < ntris && (gvs[0][0] != 0.0 || gvs[0][1] != 0.0 || gvs[0][2] != 0.0 || gvs[1][0] != 0.0 ||
+                      gvs[1][1] != 0.0 || gvs[1][2] != 0.0 || gvs[2][0] != 0.0 || gvs[2][1] != 0.0 ||
+                      gvs[2][2] != 0.0)) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int v = idx[3 * t + j];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&grad_pos[3 * v + a], (float)gvs[j][a]);
+        }
+    }
 }
 
 // out[i][e] = sum over blocks of partial[i][block][e]: one workgroup per
@@ -619,6 +654,7 @@ struct DiffWs {
     double* part_tri;
     LargePart* parts;
     unsigned char* large;
+    unsigned char* touched;
     void* raster;
     size_t raster_bytes;
     int px_blocks, tri_blocks, tx_blocks;
@@ -640,6 +676,7 @@ size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* w
     size_t o_ptr = off; off = align256(off + sizeof(double) * 16 * (size_t)(tri_blocks > 0 ? tri_blocks : 1) * S);
     size_t o_lp = off; off = align256(off + sizeof(LargePart) * kLargeSlots * (size_t)tx_blocks * S);
     size_t o_lg = off; off = align256(off + (size_t)ntris * S);
+    size_t o_tc = off; off = align256(off + (size_t)ntris * S);
     size_t rbytes = internal::vis_raster_workspace_size(ntris, S, smw, smh);
     size_t o_r = off; off = align256(off + rbytes);
     if (ws) {
@@ -653,6 +690,7 @@ size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* w
         ws->part_tri = reinterpret_cast<double*>(base + o_ptr);
         ws->parts = reinterpret_cast<LargePart*>(base + o_lp);
         ws->large = reinterpret_cast<unsigned char*>(base + o_lg);
+        ws->touched = reinterpret_cast<unsigned char*>(base + o_tc);
         ws->raster = base + o_r;
         ws->raster_bytes = rbytes;
         ws->px_blocks = px_blocks;
@@ -762,12 +800,16 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
                        grad_biased_mvps_dev);
     if (ntris > 0) {
         hipLaunchKernelGGL(k_fill_f64, dim3(256), dim3(kThreads), 0, s, w.wsum, (size_t)3 * ntris * nsamples, 0.0);
+        // touched flags: the region is 256-byte aligned and padded, so whole words clear it
+        hipLaunchKernelGGL(k_fill_f32, dim3(64), dim3(kThreads), 0, s, reinterpret_cast<float*>(w.touched),
+                           ((size_t)ntris * nsamples + 3) / 4, 0.0f);
         hipLaunchKernelGGL(k_diff_texel_bwd, dim3(w.tx_blocks, nsamples), dim3(kThreads), 0, s, w.vis, w.planes,
-                           w.large, ntris, w.grad_depth, c, w.wsum, w.parts);
+                           w.large, ntris, w.grad_depth, c, w.wsum, w.parts, w.touched);
         hipLaunchKernelGGL(k_diff_large_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.parts, w.tx_blocks, ntris,
                            w.wsum);
-        hipLaunchKernelGGL(k_diff_tri_bwd, dim3(w.tri_blocks, nsamples), dim3(kThreads), 0, s, positions_dev,
-                           indices_dev, ntris, mvps_dev, w.planes, w.vis, w.grad_depth, w.wsum, c,
+        hipLaunchKernelGGL(k_diff_tri_bwd, dim3(w.tri_blocks, (nsamples + kMapsPerThread - 1) / kMapsPerThread),
+                           dim3(kThreads), 0, s, positions_dev,
+                           indices_dev, ntris, mvps_dev, w.planes, w.vis, w.grad_depth, w.wsum, w.touched, c,
                            grad_positions_dev, w.part_tri);
         hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_tri, w.tri_blocks,
                            grad_mvps_dev);

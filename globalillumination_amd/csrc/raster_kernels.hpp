@@ -359,7 +359,14 @@ __device__ __forceinline__ bool rect_overlap_s(const Setup& s, int x0, int x1, i
 // ---------------------------------------------------------------------------
 // Binning
 // ---------------------------------------------------------------------------
-constexpr int kWideSpan = 4;            // pieces overlapping more tiles are "wide"
+// Pieces overlapping more than wide_span<TS>() tiles are "wide" (binned from
+// setup records by k_bin_wide): 4 tiles of 64x64 texels, the same texel area
+// for other tile edges, so the set of wide pieces does not grow as tiles
+// shrink.
+template <int TS>
+constexpr int wide_span() {
+    return 4 * (64 / TS) * (64 / TS);
+}
 constexpr uint32_t kWideBit = 0x80000000u;
 constexpr int kWideBlocks = 64;         // workgroups per map binning wide pieces
 constexpr int kClipBlocks = 4;          // workgroups per map clipping listed triangles
@@ -420,7 +427,8 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
         for (int p = tid; p < ntiles; p += blockDim.x) hist[p] = 0;
     float mvp[16];
     mvps.load(map, mvp);
-    // Up to kWideSpan tiles per thread, kept in registers (static indices).
+    // Up to wide_span<TS>() tiles per thread, kept in registers (static indices).
+    constexpr int kWideSpan = wide_span<TS>();
     int tile_of[kWideSpan];
     uint32_t rank_of[kWideSpan];
 #pragma unroll
@@ -890,8 +898,12 @@ struct Workspace {
 
 inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-inline uint32_t entry_cap(int ntris, int ntiles) {
-    unsigned long long cap = 4ull * (unsigned long long)ntris + 4ull * (unsigned long long)ntiles + 1024ull;
+// Tile-list entries per map: up to wide_span tiles per directly binned piece
+// (pieces are rarely that wide, so 4 tiles per triangle at the 64-texel edge,
+// scaled with the tile count for smaller edges) plus the wide pieces' tiles.
+inline uint32_t entry_cap(int ntris, int ntiles, int ts = kTile) {
+    const unsigned long long per = 4ull * (unsigned long long)(64 / ts) * (unsigned long long)(64 / ts);
+    unsigned long long cap = per * (unsigned long long)ntris + 4ull * (unsigned long long)ntiles + 1024ull;
     if (cap > 0x7FFFFFFFull) cap = 0x7FFFFFFFull;
     return (uint32_t)cap;
 }
@@ -901,7 +913,7 @@ inline uint32_t wide_cap(int ntris) { return 1024u + (uint32_t)ntris / 16u; }
 inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws, char* base, int ts = kTile) {
     int tx = (w + ts - 1) / ts, ty = (h + ts - 1) / ts;
     size_t ntiles = (size_t)tx * ty;
-    uint32_t cap = entry_cap(ntris, (int)ntiles);
+    uint32_t cap = entry_cap(ntris, (int)ntiles, ts);
     uint32_t wcap = wide_cap(ntris);
     size_t off = 0;
     size_t o_counts = off; off = align256(off + sizeof(uint32_t) * ntiles * nmaps);
