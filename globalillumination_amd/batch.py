@@ -20,7 +20,7 @@ from . import _lib
 from .passes import DeviceMesh, GBuffer, SoftParams, _dptr, _stream_handle, render_monte_carlo, render_shadow_map, \
     soft_uniforms
 
-SHAPES = {0: (64, 4), 1: (32, 8), 2: (16, 16)}
+SHAPES = {0: (64, 4), 1: (32, 8), 2: (16, 16), 3: "16x16 of 8x8 wave tiles"}
 
 
 @dataclass
@@ -128,7 +128,7 @@ class Batch:
         return {"tile": t.tile, "raster_groups": t.raster_groups, "screen_shape": SHAPES[t.screen_shape],
                 "screen_groups": t.screen_groups}
 
-    def set_tuning(self, tile=64, raster_groups=4, screen_shape=0, screen_groups=4) -> None:
+    def set_tuning(self, tile=64, raster_groups=4, screen_shape=3, screen_groups=4) -> None:
         t = _lib.GiBatchTuning(int(tile), int(raster_groups), int(screen_shape), int(screen_groups))
         _lib.call("gi_batch_set_tuning", self.handle, ctypes.byref(t))
 

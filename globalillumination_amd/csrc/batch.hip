@@ -44,7 +44,7 @@ constexpr int kFillChunk = 4096;  // words cleared per fill item
 constexpr int kXcds = 8;
 constexpr int kScreenThreads = 256;
 constexpr int kNtGroups = 3;  // soft variants: generic loop, 3 taps per axis, 4 taps per axis
-constexpr int kShapes = 3;    // screen blocks 64x4, 32x8, 16x16
+constexpr int kShapes = 4;    // screen blocks 64x4, 32x8, 16x16 (row-major waves), 16x16 of 8x8 wave tiles
 constexpr int kCounters = 64;  // [0] raster queue, [1 + 8 g + x] screen queue of group g on XCD x,
                                // [63] the autotuner's overflow flag
 
@@ -224,13 +224,14 @@ __global__ __launch_bounds__(kRasterThreads) void kb_raster_fallback(RasterTable
 // (instance, pixel block) items.  XCD x (hardware hands consecutive
 // workgroups to the XCDs round-robin) drains items [xr[2x], xr[2x+1]) through
 // its own queue.
-template <int NT, int BX, int BY>
+template <int NT, int BX, int BY, bool kWaveTiles>
 __global__ __launch_bounds__(kScreenThreads) void kb_soft(SoftTable t) {
     static_assert(BX * BY == kScreenThreads, "block shape");
     __shared__ uint32_t s_item;
     const int xcd = (int)(blockIdx.x % kXcds);
     const uint32_t lo = t.xr[2 * xcd], hi = t.xr[2 * xcd + 1];
-    const int tx = (int)threadIdx.x % BX, ty = (int)threadIdx.x / BX;
+    int tx = (int)threadIdx.x % BX, ty = (int)threadIdx.x / BX;
+    if (kWaveTiles) wave_tile_xy((int)threadIdx.x, tx, ty);
     for (;;) {
         if (threadIdx.x == 0) s_item = lo + atomicAdd(&t.counters[xcd], 1u);
         __syncthreads();
@@ -250,8 +251,8 @@ __global__ __launch_bounds__(kScreenThreads) void kb_soft(SoftTable t) {
 // ---------------------------------------------------------------------------
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-const int kShapeX[kShapes] = {64, 32, 16};
-const int kShapeY[kShapes] = {4, 8, 16};
+const int kShapeX[kShapes] = {64, 32, 16, 16};
+const int kShapeY[kShapes] = {4, 8, 16, 16};
 const int kGroupNt[kNtGroups] = {0, 3, 4};
 const int kTiles[2] = {32, 64};
 
@@ -379,9 +380,10 @@ int run_depth(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
 template <int NT>
 void launch_soft(int shape, unsigned grid, const SoftTable& tab, hipStream_t s) {
     switch (shape) {
-    case 0: hipLaunchKernelGGL((kb_soft<NT, 64, 4>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
-    case 1: hipLaunchKernelGGL((kb_soft<NT, 32, 8>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
-    default: hipLaunchKernelGGL((kb_soft<NT, 16, 16>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
+    case 0: hipLaunchKernelGGL((kb_soft<NT, 64, 4, false>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
+    case 1: hipLaunchKernelGGL((kb_soft<NT, 32, 8, false>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
+    case 2: hipLaunchKernelGGL((kb_soft<NT, 16, 16, false>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
+    default: hipLaunchKernelGGL((kb_soft<NT, 16, 16, true>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
     }
 }
 
@@ -488,7 +490,7 @@ int gi_batch_create(const gi_batch_instance* inst, int n, void* workspace_dev, s
     if (!b) return GI_ENOSPACE;
     b->n = n;
     b->ncu = ncu;
-    b->tuning = gi_batch_tuning{64, 4, 0, 4};
+    b->tuning = gi_batch_tuning{64, 4, 3, 4};
     b->counters = reinterpret_cast<uint32_t*>(base + L.counters);
     for (int v = 0; v < 2; ++v) {
         const int ts = kTiles[v];

@@ -78,7 +78,7 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
     DepthMap D{depth_dev, p->shadow_map_width, p->shadow_map_height, (float)p->shadow_map_width,
                (float)p->shadow_map_height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    dim3 grid = grid_for(gb.w, gb.h), block(kBlockX, kBlockY);
+    dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
     switch (p->technique) {
     case GI_NAIVE: hipLaunchKernelGGL(k_hard<GI_NAIVE>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_PCF: hipLaunchKernelGGL(k_hard<GI_PCF>, grid, block, 0, s, gb, D, u, out_dev); break;
@@ -98,7 +98,7 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
-    dim3 grid = grid_for(gb.w, gb.h), block(kBlockX, kBlockY);
+    dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
     switch (soft_variant(u)) {
     case 3: hipLaunchKernelGGL(k_soft<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
     case 4: hipLaunchKernelGGL(k_soft<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;

@@ -14,8 +14,21 @@ namespace gi {
 namespace {
 
 
-constexpr int kBlockX = 64;
-constexpr int kBlockY = 4;
+// A workgroup covers a 16x16-pixel tile made of four 8x8-pixel wave tiles.
+// A wave's pixels then project to a compact patch of each shadow map (about
+// 16x16 texels at the usual 2x minification), so one gather instruction
+// touches a few cache lines; 64 pixels of one row would project along a
+// slanted line and touch a line per lane.
+constexpr int kBlockX = 16;
+constexpr int kBlockY = 16;
+constexpr int kBlockThreads = kBlockX * kBlockY;
+
+// Pixel offset of thread `tid` inside its workgroup tile.
+__device__ __forceinline__ void wave_tile_xy(int tid, int& px, int& py) {
+    const int lane = tid & 63, wave = tid >> 6;
+    px = (wave & 1) * 8 + (lane & 7);
+    py = (wave >> 1) * 8 + (lane >> 3);
+}
 
 struct GBufferView {
     const float* __restrict__ pos;  // 3 planes
@@ -279,12 +292,14 @@ __device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
     int xcd = b % 8, slot = b / 8;
     int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
     if (lin >= nblocks) lin = b;
-    x = (lin % bx_n) * kBlockX + (int)threadIdx.x;
-    y = (lin / bx_n) * kBlockY + (int)threadIdx.y;
+    int px, py;
+    wave_tile_xy((int)threadIdx.x, px, py);
+    x = (lin % bx_n) * kBlockX + px;
+    y = (lin / bx_n) * kBlockY + py;
 }
 
 template <int kTech>
-__global__ __launch_bounds__(kBlockX* kBlockY) void k_hard(GBufferView gb, DepthMap D, HardUniforms u,
+__global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap D, HardUniforms u,
                                                             float* __restrict__ out) {
     int x, y;
     pixel_of_block(gb.w, gb.h, x, y);
@@ -453,7 +468,7 @@ __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* _
 // One thread per pixel.  Blocks are ordered XCD-aware (pixel_of_block) so each
 // XCD's L2 serves one contiguous band of the view.
 template <int NT>
-__global__ __launch_bounds__(kBlockX* kBlockY) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
+__global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
                                                             int smh, const float4* __restrict__ trans,
                                                             SoftUniforms u, float* __restrict__ out) {
     int x, y;
