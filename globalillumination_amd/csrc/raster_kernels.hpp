@@ -23,9 +23,16 @@ constexpr int kTile = 64;
 constexpr int kRasterThreads = 256;
 constexpr float kGuard = 4.0f;
 constexpr int kMaxPoly = 9;
-constexpr int kSelfRasterMaxPx = 256;  // bbox∩tile area a lone thread rasterizes itself
+// Launch-shape constants; a build may override them (-DGI_SELF_RASTER_MAX_PX=...).
+#ifndef GI_SELF_RASTER_MAX_PX
+#define GI_SELF_RASTER_MAX_PX 256
+#endif
+#ifndef GI_REC_QUEUE
+#define GI_REC_QUEUE 64
+#endif
+constexpr int kSelfRasterMaxPx = GI_SELF_RASTER_MAX_PX;  // bbox∩tile area a lone thread rasterizes itself
 constexpr int kScanThreads = 1024;
-constexpr int kRecQueue = 64;  // LDS setup records a workgroup rasterizes together per chunk
+constexpr int kRecQueue = GI_REC_QUEUE;  // LDS setup records a workgroup rasterizes together per chunk
 
 struct MeshView {
     const float* __restrict__ pos;  // nverts x 3

@@ -348,6 +348,8 @@ struct SoftUniforms {
     int pcf;
     int ntaps;             // taps per axis of the PCF loop
     float taps[kMaxTaps];  // loop values w of `for (w = -offset; w </<= offset; w += step)`
+    float tapw[kMaxTaps];  // taps[a] * (1/W) and taps[a] * (1/H): the shader's products, in FP32
+    float taph[kMaxTaps];
 };
 
 // Rows of NT consecutive texels; selecting element k of a row.
@@ -380,16 +382,17 @@ __device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms
     if constexpr (NT > 0) {
         float tx[NT], ty[NT];
         int iu[NT], iv[NT];
-        bool inside = true;
 #pragma unroll
         for (int a = 0; a < NT; ++a) {
-            tx[a] = sx + u.taps[a] * incrWidth;
-            ty[a] = sy + u.taps[a] * incrHeight;
-            float uu = tx[a] * D.fw, vv = ty[a] * D.fh;
-            inside = inside && (uu >= 0.0f && uu < D.fw && vv >= 0.0f && vv < D.fh);
-            iu[a] = (int)floorf(uu);
-            iv[a] = (int)floorf(vv);
+            tx[a] = sx + u.tapw[a];
+            ty[a] = sy + u.taph[a];
+            iu[a] = (int)floorf(tx[a] * D.fw);
+            iv[a] = (int)floorf(ty[a] * D.fh);
         }
+        // every tap inside [0, W) x [0, H): the taps increase with a and rounding
+        // is monotone, so the first and last bound the others (NaN fails)
+        bool inside = tx[0] * D.fw >= 0.0f && tx[NT - 1] * D.fw < D.fw && ty[0] * D.fh >= 0.0f &&
+                      ty[NT - 1] * D.fh < D.fh;
         // one row of NT texels from iu[0] must cover every tap column
         inside = inside && (iu[NT - 1] - iu[0] <= NT - 1) && (iu[0] + NT - 1 < D.w);
         float illum = 0.0f;
@@ -517,10 +520,18 @@ inline int soft_setup(const gi_gbuffer* gbuffer, const float* maps_dev, int nsam
     u.nsamples = nsamples;
     u.pcf = p->pcf ? 1 : 0;
     u.ntaps = 1;
-    for (int i = 0; i < kMaxTaps; ++i) u.taps[i] = 0.0f;
+    for (int i = 0; i < kMaxTaps; ++i) u.taps[i] = u.tapw[i] = u.taph[i] = 0.0f;
     if (p->pcf) {
         u.ntaps = pcf_taps(p->penumbra, p->kernel_order, p->inclusive != 0, u.taps, kMaxTaps);
         if (u.ntaps <= 0) return GI_ENOTSUP;
+        // the kernel's incrWidth = 1.0f / (float)W; the taps must increase for
+        // the extremes-only bounds test
+        const float incr_w = 1.0f / (float)smw, incr_h = 1.0f / (float)smh;
+        for (int i = 0; i < u.ntaps; ++i) {
+            u.tapw[i] = u.taps[i] * incr_w;
+            u.taph[i] = u.taps[i] * incr_h;
+            if (i > 0 && !(u.taps[i] > u.taps[i - 1])) return GI_ENOTSUP;
+        }
     }
     return GI_OK;
 }
