@@ -41,7 +41,17 @@ class GiGBuffer(ctypes.Structure):
     _fields_ = [("position_dev", ctypes.c_void_p),
                 ("normal_dev", ctypes.c_void_p),
                 ("width", ctypes.c_int),
-                ("height", ctypes.c_int)]
+                ("height", ctypes.c_int),
+                ("facing_dev", ctypes.c_void_p)]
+
+
+class GiPhongParams(ctypes.Structure):
+    _fields_ = [("mv", ctypes.c_float * 16),
+                ("normal_matrix", ctypes.c_float * 9),
+                ("light_position", ctypes.c_float * 3),
+                ("shadow_intensity", ctypes.c_float),
+                ("color", ctypes.c_float * 4),
+                ("clear_color", ctypes.c_float * 4)]
 
 
 class GiShadowParams(ctypes.Structure):
@@ -128,13 +138,15 @@ _SIGS = {
     "gi_gbuffer_pass_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 3),
     "gi_gbuffer_pass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
-                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                       ctypes.c_void_p]),
+                                       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_size_t, ctypes.c_void_p]),
     "gi_hard_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p,
                                            ctypes.POINTER(GiShadowParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.POINTER(GiPhongParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),
     "gi_diff_soft_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,

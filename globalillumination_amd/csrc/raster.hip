@@ -147,7 +147,8 @@ size_t gi_gbuffer_pass_workspace_size(int ntris, int width, int height) {
 
 int gi_gbuffer_pass(const float* positions_dev, const float* normals_dev, int nverts, const int32_t* indices_dev,
                     int ntris, const float mvp[16], int width, int height, float* position_out_dev,
-                    float* normal_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+                    float* normal_out_dev, float* facing_out_dev, void* workspace_dev, size_t workspace_bytes,
+                    gi_stream stream) {
     if (!positions_dev || !normals_dev || (!indices_dev && ntris > 0) || !mvp || !position_out_dev ||
         !normal_out_dev || !workspace_dev)
         return GI_EINVAL;
@@ -169,7 +170,8 @@ int gi_gbuffer_pass(const float* positions_dev, const float* normals_dev, int nv
     if (rc != GI_OK) return rc;
     size_t npx = (size_t)width * height;
     hipLaunchKernelGGL(k_gbuffer_resolve, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, mesh, mvps.single,
-                       cfg, reinterpret_cast<const unsigned long long*>(ws.vis), position_out_dev, normal_out_dev);
+                       cfg, reinterpret_cast<const unsigned long long*>(ws.vis), position_out_dev, normal_out_dev,
+                       facing_out_dev);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

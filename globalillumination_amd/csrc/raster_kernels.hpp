@@ -848,37 +848,71 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mes
 // ---------------------------------------------------------------------------
 // G-buffer resolve: GBuffer.frag:35-37 (position, normal, gl_FrontFacing)
 // ---------------------------------------------------------------------------
+// Perspective-correct attributes of the visible piece at pixel k; false when
+// the piece does not cover the pixel centre.
+__device__ __forceinline__ bool resolve_attrs(const Tri<6>& t, int i, int j, float out[6]) {
+    long long e[3];
+    float z;
+    if (!eval_pixel<6>(t, i, j, e, z)) return false;
+    double b0 = (double)e[0] / (double)t.w[0];
+    double b1 = (double)e[1] / (double)t.w[1];
+    double b2 = (double)e[2] / (double)t.w[2];
+    double den = (b0 + b1) + b2;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        double num = (b0 * (double)t.a[0][q] + b1 * (double)t.a[1][q]) + b2 * (double)t.a[2][q];
+        out[q] = (float)(num / den);
+    }
+    return true;
+}
+
+// Clipped pieces (rare) set up through the out-of-line clipper; kept apart so
+// that the common path's triangle record stays in registers.
+__device__ __noinline__ bool resolve_clipped(const MeshView& mesh, const float* mvp, int tri, int piece,
+                                             const RasterCfg& cfg, int i, int j, float out[6], int& front) {
+    Tri<6> t;
+    bool ok;
+    get_piece<6>(mesh, mvp, tri, piece, cfg, t, ok);
+    front = t.front;
+    return ok && resolve_attrs(t, i, j, out);
+}
+
+// facing_out (optional): gl_FrontFacing as 1 / 0 (GBuffer.frag:36), 0 for
+// background.  The normal is stored negated on back faces.
 __global__ __launch_bounds__(256) void k_gbuffer_resolve(MeshView mesh, Mat4 mvp_arg, RasterCfg cfg,
                                                          const unsigned long long* vis, float* pos_out,
-                                                         float* nrm_out) {
+                                                         float* nrm_out, float* facing_out) {
     size_t npx = (size_t)cfg.w * cfg.h;
     size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npx) return;
     unsigned long long key = vis[k];
     float out[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float facing = 0.0f;
     if (key != ~0ull) {
         uint32_t id = (uint32_t)(key & 0xffffffffull);
+        const int tri = (int)(id >> 3), piece = (int)(id & 7u);
+        const int i = (int)(k % cfg.w), j = (int)(k / cfg.w);
         Tri<6> t;
         bool ok;
-        get_piece<6>(mesh, mvp_arg.m, (int)(id >> 3), (int)(id & 7u), cfg, t, ok);
-        int i = (int)(k % cfg.w), j = (int)(k / cfg.w);
-        long long e[3];
-        float z;
-        if (ok && eval_pixel<6>(t, i, j, e, z)) {
-            double b0 = (double)e[0] / (double)t.w[0];
-            double b1 = (double)e[1] / (double)t.w[1];
-            double b2 = (double)e[2] / (double)t.w[2];
-            double den = (b0 + b1) + b2;
-#pragma unroll
-            for (int q = 0; q < 6; ++q) {
-                double num = (b0 * (double)t.a[0][q] + b1 * (double)t.a[1][q]) + b2 * (double)t.a[2][q];
-                out[q] = (float)(num / den);
-            }
-            if (!t.front) {
+        const int cls = classify<6>(mesh, mvp_arg.m, tri, cfg, t, ok);
+        bool hit = false;
+        int front = 0;
+        if (cls == kInside) {
+            hit = ok && piece == 0 && resolve_attrs(t, i, j, out);
+            front = t.front;
+        } else if (cls == kClip) {
+            hit = resolve_clipped(mesh, mvp_arg.m, tri, piece, cfg, i, j, out, front);
+        }
+        if (hit) {
+            facing = front ? 1.0f : 0.0f;
+            if (!front) {
                 out[3] = -out[3];
                 out[4] = -out[4];
                 out[5] = -out[5];
             }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) out[q] = 0.0f;
         }
     }
 #pragma unroll
@@ -886,6 +920,7 @@ __global__ __launch_bounds__(256) void k_gbuffer_resolve(MeshView mesh, Mat4 mvp
         pos_out[q * npx + k] = out[q];
         nrm_out[q * npx + k] = out[3 + q];
     }
+    if (facing_out) facing_out[k] = facing;
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {

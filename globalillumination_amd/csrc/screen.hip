@@ -107,4 +107,30 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
+int gi_phong_pass(const gi_gbuffer* gbuffer, const float* shadow_dev, const float* color_dev,
+                  const gi_phong_params* p, float* rgba_out_dev, gi_stream stream) {
+    if (!gbuffer_ok(gbuffer) || !gbuffer->normal_dev || !gbuffer->facing_dev || !shadow_dev || !p || !rgba_out_dev)
+        return GI_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(rgba_out_dev) & 15u) != 0 ||
+        (color_dev && (reinterpret_cast<uintptr_t>(color_dev) & 15u) != 0))
+        return GI_EINVAL;
+    if (!finite_all(p->mv, 16) || !finite_all(p->normal_matrix, 9) || !finite_all(p->light_position, 3) ||
+        !std::isfinite(p->shadow_intensity) || !finite_all(p->color, 4) || !finite_all(p->clear_color, 4))
+        return GI_ENONFINITE;
+    PhongUniforms u;
+    for (int i = 0; i < 16; ++i) u.mv.m[i] = p->mv[i];
+    for (int i = 0; i < 9; ++i) u.nm.m[i] = p->normal_matrix[i];
+    for (int i = 0; i < 3; ++i) u.lpos[i] = p->light_position[i];
+    u.si = p->shadow_intensity;
+    for (int i = 0; i < 4; ++i) {
+        u.color[i] = p->color[i];
+        u.clear[i] = p->clear_color[i];
+    }
+    GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_phong, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, gbuffer->facing_dev, shadow_dev,
+                       reinterpret_cast<const float4*>(color_dev), u, reinterpret_cast<float4*>(rgba_out_dev));
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
 }  // extern "C"

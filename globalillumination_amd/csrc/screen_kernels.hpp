@@ -480,6 +480,103 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const fl
     soft_pixel<NT>(gb, maps, smw, smh, trans, u, out, x, y);
 }
 
+// ---------------------------------------------------------------------------
+// Deferred Phong shading: PhongShading.frag:11-46 (ShadowMapping and
+// SoftShadowMapping share it; the shadow R channel is the hard or soft pass's
+// output).
+// ---------------------------------------------------------------------------
+struct PhongUniforms {
+    Mat4 mv;
+    Mat3 nm;
+    float lpos[3];
+    float si;
+    float color[4];  // G-buffer colour of every pixel when no colour plane is given
+    float clear[4];  // background: shadeScene's glClearColor (main.cpp:452)
+};
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) {
+    return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// `pow(x, 0.3 * shininess)` with shininess 10: the FP32 exponent is exactly 3.
+// GLSL leaves pow's precision to the implementation; this build evaluates it
+// as (x * x) * x.
+__device__ __forceinline__ float pow3(float x) { return (x * x) * x; }
+
+__device__ __forceinline__ float4 phong_pixel(const GBufferView& gb, const float* __restrict__ facing,
+                                              const float* __restrict__ shadow, const float4* __restrict__ color,
+                                              const PhongUniforms& u, size_t k) {
+    const size_t plane = (size_t)gb.w * gb.h;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) return make_float4(u.clear[0], u.clear[1], u.clear[2], u.clear[3]);  // :40 discard
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    // the stored normal is negated on back faces; PhongShading reads it raw (:42)
+    const bool front = facing[k] != 0.0f;
+    float nr[3] = {gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]};
+    if (!front) {
+        nr[0] = -nr[0];
+        nr[1] = -nr[1];
+        nr[2] = -nr[2];
+    }
+    const float sh = shadow[k];
+    const float specShadow = sh - u.si;  // :18
+    float v[4];
+    mat4_vec4(u.mv.m, vx, vy, vz, 1.0f, v);  // :20
+    float n3[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {  // :21 normalMatrix * normal.xyz
+        float acc = u.nm.m[0 * 3 + i] * nr[0];
+        acc = acc + u.nm.m[1 * 3 + i] * nr[1];
+        acc = acc + u.nm.m[2 * 3 + i] * nr[2];
+        n3[i] = acc;
+    }
+    float N[3], L[3];
+    normalize3(n3, N);
+    const float ld[3] = {u.lpos[0] - v[0], u.lpos[1] - v[1], u.lpos[2] - v[2]};
+    normalize3(ld, L);  // :23
+    // :24 E = normalize(-vertex) normalizes all four components, then keeps xyz
+    const float e4[4] = {-v[0], -v[1], -v[2], -v[3]};
+    const float e_sqr = ((e4[0] * e4[0] + e4[1] * e4[1]) + e4[2] * e4[2]) + e4[3] * e4[3];
+    const float e_inv = 1.0f / sqrtf(e_sqr);
+    const float E[3] = {e4[0] * e_inv, e4[1] * e_inv, e4[2] * e_inv};
+    // :25 R = normalize(-reflect(L, N)), reflect(I, N) = I - 2.0 * dot(N, I) * N
+    const float dNL = dot3(N, L);
+    const float two_d = 2.0f * dNL;
+    const float mr[3] = {-(L[0] - two_d * N[0]), -(L[1] - two_d * N[1]), -(L[2] - two_d * N[2])};
+    float R[3];
+    normalize3(mr, R);
+    const float diff = gmax(dNL, 0.0f);                 // :28 max(dot(normal.xyz, L), 0.0)
+    const float spec = pow3(gmax(dot3(R, E), 0.0f));     // :29
+    const float amb[4] = {0.4f, 0.4f, 0.4f, 1.0f}, ldif[4] = {0.5f, 0.5f, 0.5f, 1.0f},
+                lspe[4] = {0.25f, 0.25f, 0.25f, 1.0f};
+    float col[4];
+    if (color) {
+        const float4 c = color[k];
+        col[0] = c.x; col[1] = c.y; col[2] = c.z; col[3] = c.w;
+    } else {
+        col[0] = u.color[0]; col[1] = u.color[1]; col[2] = u.color[2]; col[3] = u.color[3];
+    }
+    float o[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float Idiff = ldif[c] * diff;
+        const float Ispec = (specShadow * lspe[c]) * spec;
+        o[c] = (sh * col[c]) * ((Idiff + Ispec) + amb[c]);  // :32
+    }
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_phong(GBufferView gb, const float* __restrict__ facing,
+                                                          const float* __restrict__ shadow,
+                                                          const float4* __restrict__ color, PhongUniforms u,
+                                                          float4* __restrict__ out) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t k = (size_t)y * gb.w + x;
+    out[k] = phong_pixel(gb, facing, shadow, color, u, k);
+}
+
 // Tap values of the PCF loop `for (w = -offset; w < offset (or <=); w += step)`
 // with step = 2*offset/kernelOrder, accumulated in FP32 exactly as the shader
 // loop does (AccurateSoftShadow.frag:101-109, Shadow.frag:93-99).

@@ -117,7 +117,9 @@ int HardShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight
     const int npx = windowWidth * windowHeight;
     GI_TRY(gpos_.allocate(bytes_of(npx, 3 * sizeof(float))));
     GI_TRY(gnrm_.allocate(bytes_of(npx, 3 * sizeof(float))));
+    GI_TRY(gfac_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(hard_.allocate(bytes_of(npx, sizeof(float))));
+    GI_TRY(rgba_.allocate(bytes_of(npx, 4 * sizeof(float))));
     size_t ws = gi_depth_pass_workspace_size(ntris_, 1, shadowMapWidth, shadowMapHeight);
     size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
     if (ws == 0 || wg == 0) return GI_ENOTSUP;
@@ -152,11 +154,9 @@ int HardShadowPipeline::renderGBuffer() {
     GI_TRY(cam.configureAmbient(windowWidth_, windowHeight_));
     cameraMVP_ = cam.mvp();
     cameraMV_ = mul(cam.view, cam.model);
-    const int npx = windowWidth_ * windowHeight_;
-    (void)npx;
     return gi_gbuffer_pass(pos_.as<float>(), nrm_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_,
                            cameraMVP_.data(), windowWidth_, windowHeight_, gpos_.as<float>(), gnrm_.as<float>(),
-                           ws_.get(), ws_.size(), stream);
+                           gfac_.as<float>(), ws_.get(), ws_.size(), stream);
 }
 
 // The shader computeHardShadows picks from the flags (main.cpp:400-414).
@@ -193,18 +193,48 @@ int HardShadowPipeline::computeHardShadows() {
     p.light_position[0] = -lightEye[0];
     p.light_position[1] = lightEye[1];
     p.light_position[2] = -lightEye[2];
-    gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_};
+    gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
     return gi_hard_shadow_pass(&g, depth_.as<float>(), &p, hard_.as<float>(), stream);
+}
+
+// configurePhong(lightEye rotated 180 degrees about y, cameraEye)
+// (MyGLGeometryViewer.cpp:108-134, main.cpp:309) with the shadow buffer a
+// screen pass wrote.
+static int phong(const gi_gbuffer& g, const Mat4& cameraMV, const Vec3& lightEye, float shadowIntensity,
+                 const float* shadow, float* rgba, hipStream_t stream) {
+    gi_phong_params p{};
+    std::memcpy(p.mv, cameraMV.data(), sizeof(p.mv));
+    normal_matrix(cameraMV, p.normal_matrix);
+    p.light_position[0] = -lightEye[0];
+    p.light_position[1] = lightEye[1];
+    p.light_position[2] = -lightEye[2];
+    p.shadow_intensity = shadowIntensity;
+    std::memcpy(p.color, kSceneColor, sizeof(p.color));
+    std::memcpy(p.clear_color, kClearColor, sizeof(p.clear_color));
+    return gi_phong_pass(&g, shadow, nullptr, &p, rgba, stream);
+}
+
+std::array<float, 9> HardShadowPipeline::normalMatrix() const {
+    std::array<float, 9> nm{};
+    normal_matrix(cameraMV_, nm.data());
+    return nm;
+}
+
+int HardShadowPipeline::shadeScene() {
+    gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
+    return phong(g, cameraMV_, lightEye, shadowParams.shadowIntensity, hard_.as<float>(), rgba_.as<float>(), stream);
 }
 
 int HardShadowPipeline::display() {
     GI_TRY(renderShadowMap());
     GI_TRY(renderGBuffer());
-    return computeHardShadows();
+    GI_TRY(computeHardShadows());
+    return shadeScene();
 }
 
 int HardShadowPipeline::download(std::vector<float>& hardShadow, std::vector<float>* depth,
-                                 std::vector<float>* gbufferPosition, std::vector<float>* gbufferNormal) {
+                                 std::vector<float>* gbufferPosition, std::vector<float>* gbufferNormal,
+                                 std::vector<float>* rgba) {
     const size_t npx = static_cast<size_t>(windowWidth_) * windowHeight_;
     hardShadow.resize(npx);
     GI_TRY(hip_ok(hipMemcpyAsync(hardShadow.data(), hard_.get(), npx * sizeof(float), hipMemcpyDeviceToHost,
@@ -223,6 +253,11 @@ int HardShadowPipeline::download(std::vector<float>& hardShadow, std::vector<flo
         gbufferNormal->resize(3 * npx);
         GI_TRY(hip_ok(hipMemcpyAsync(gbufferNormal->data(), gnrm_.get(), 3 * npx * sizeof(float),
                                      hipMemcpyDeviceToHost, stream)));
+    }
+    if (rgba) {
+        rgba->resize(4 * npx);
+        GI_TRY(hip_ok(hipMemcpyAsync(rgba->data(), rgba_.get(), 4 * npx * sizeof(float), hipMemcpyDeviceToHost,
+                                     stream)));
     }
     return hip_ok(hipStreamSynchronize(stream));
 }
@@ -259,7 +294,9 @@ int SoftShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight
     const int npx = windowWidth * windowHeight;
     GI_TRY(gpos_.allocate(bytes_of(npx, 3 * sizeof(float))));
     GI_TRY(gnrm_.allocate(bytes_of(npx, 3 * sizeof(float))));
+    GI_TRY(gfac_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(soft_.allocate(bytes_of(npx, sizeof(float))));
+    GI_TRY(rgba_.allocate(bytes_of(npx, 4 * sizeof(float))));
     size_t ws = gi_depth_pass_workspace_size(ntris_, maxSamples, smw_, smh_);
     size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
     if (ws == 0 || wg == 0) return GI_ENOTSUP;
@@ -274,9 +311,10 @@ int SoftShadowPipeline::renderGBuffer() {
     cam.up = cameraUp;
     GI_TRY(cam.configureAmbient(windowWidth_, windowHeight_));
     cameraMVP_ = cam.mvp();
+    cameraMV_ = mul(cam.view, cam.model);
     return gi_gbuffer_pass(pos_.as<float>(), nrm_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_,
                            cameraMVP_.data(), windowWidth_, windowHeight_, gpos_.as<float>(), gnrm_.as<float>(),
-                           ws_.get(), ws_.size(), stream);
+                           gfac_.as<float>(), ws_.get(), ws_.size(), stream);
 }
 
 int SoftShadowPipeline::renderMonteCarlo() {
@@ -304,7 +342,7 @@ int SoftShadowPipeline::renderMonteCarlo() {
     gi_raster_opts opts{4.0f, 20.0f, nullptr, nullptr};  // glPolygonOffset(4, 20), Soft main.cpp:295
     GI_TRY(gi_depth_pass(pos_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_, mvps_.as<float>(), n, smw_, smh_,
                          &opts, maps_.as<float>(), ws_.get(), ws_.size(), stream));
-    gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_};
+    gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_, nullptr};
     gi_soft_params sp{shadowIntensity, adaptiveSamplingLowerAccuracy ? 1 : 0, 3, 1.0f, 1};
     // columns 0-2 come from the last sample's B*lightMVP (MyGLGeometryViewer.cpp:168-170)
     GI_TRY(gi_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, &biased[16 * (n - 1)], trans_.as<float>(),
@@ -312,7 +350,18 @@ int SoftShadowPipeline::renderMonteCarlo() {
     return hip_ok(hipStreamSynchronize(stream));  // the trans upload's host buffer goes out of scope
 }
 
-int SoftShadowPipeline::download(std::vector<float>& softShadow, std::vector<float>* maps) {
+std::array<float, 9> SoftShadowPipeline::normalMatrix() const {
+    std::array<float, 9> nm{};
+    normal_matrix(cameraMV_, nm.data());
+    return nm;
+}
+
+int SoftShadowPipeline::deferredShading() {
+    gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
+    return phong(g, cameraMV_, lightEye, shadowIntensity, soft_.as<float>(), rgba_.as<float>(), stream);
+}
+
+int SoftShadowPipeline::download(std::vector<float>& softShadow, std::vector<float>* maps, std::vector<float>* rgba) {
     const size_t npx = static_cast<size_t>(windowWidth_) * windowHeight_;
     softShadow.resize(npx);
     GI_TRY(hip_ok(hipMemcpyAsync(softShadow.data(), soft_.get(), npx * sizeof(float), hipMemcpyDeviceToHost,
@@ -321,6 +370,11 @@ int SoftShadowPipeline::download(std::vector<float>& softShadow, std::vector<flo
         maps->resize(static_cast<size_t>(numberOfSamples) * smw_ * smh_);
         GI_TRY(hip_ok(hipMemcpyAsync(maps->data(), maps_.get(), maps->size() * sizeof(float),
                                      hipMemcpyDeviceToHost, stream)));
+    }
+    if (rgba) {
+        rgba->resize(4 * npx);
+        GI_TRY(hip_ok(hipMemcpyAsync(rgba->data(), rgba_.get(), 4 * npx * sizeof(float), hipMemcpyDeviceToHost,
+                                     stream)));
     }
     return hip_ok(hipStreamSynchronize(stream));
 }

@@ -67,6 +67,14 @@ struct GeometryViewer {
     Mat4 mvp() const;  // projection * view * model
 };
 
+// G-buffer colour of a mesh without per-vertex colours: GBuffer.frag:27 reads
+// gl_FrontLightModelProduct.sceneColor = emission (0) + the default material
+// ambient (0.2) times the light-model ambient (0.1, MyGLGeometryViewer.cpp:69),
+// alpha 1 (the default material diffuse alpha).
+constexpr float kSceneColor[4] = {0.2f * 0.1f, 0.2f * 0.1f, 0.2f * 0.1f, 1.0f};
+// shadeScene / deferredShading clear colour (main.cpp:452).
+constexpr float kClearColor[4] = {0.63f, 0.82f, 0.96f, 1.0f};
+
 class DeviceBuffer {
 public:
     DeviceBuffer() = default;
@@ -101,18 +109,21 @@ public:
     int renderShadowMap();     // main.cpp:350-361
     int renderGBuffer();       // main.cpp:363-372
     int computeHardShadows();  // main.cpp:400-414
+    int shadeScene();          // main.cpp:449-456: deferred Phong over the hard-shadow buffer
     int display();             // main.cpp:459-472 without the GL swap
     int download(std::vector<float>& hardShadow, std::vector<float>* depth = nullptr,
-                 std::vector<float>* gbufferPosition = nullptr, std::vector<float>* gbufferNormal = nullptr);
+                 std::vector<float>* gbufferPosition = nullptr, std::vector<float>* gbufferNormal = nullptr,
+                 std::vector<float>* rgba = nullptr);
     const Mat4& cameraMVP() const { return cameraMVP_; }
     const Mat4& cameraMV() const { return cameraMV_; }
+    std::array<float, 9> normalMatrix() const;  // inverseTranspose(mat3(MV)), configurePhong
 
 private:
     int technique() const;
     int windowWidth_ = 0, windowHeight_ = 0, nverts_ = 0, ntris_ = 0;
     bool ownStream_ = false;
     Mat4 cameraMVP_{}, cameraMV_{};
-    DeviceBuffer pos_, nrm_, idx_, mvp_, depth_, gpos_, gnrm_, hard_, ws_;
+    DeviceBuffer pos_, nrm_, idx_, mvp_, depth_, gpos_, gnrm_, gfac_, hard_, rgba_, ws_;
 };
 
 // The Monte-Carlo area light of SoftShadowMapping (renderMonteCarlo,
@@ -136,16 +147,20 @@ public:
     // UniformSampledLightSource placement (eye and look-at shifted together,
     // UniformSampledLightSource.cpp:13-37), the N depth maps, then the screen pass.
     int renderMonteCarlo();
-    int download(std::vector<float>& softShadow, std::vector<float>* maps = nullptr);
+    int deferredShading();  // Soft main.cpp:1109-1117: Phong over the soft-shadow buffer
+    int download(std::vector<float>& softShadow, std::vector<float>* maps = nullptr,
+                 std::vector<float>* rgba = nullptr);
     const std::vector<Mat4>& lightMVPs() const { return lightMVPs_; }
     const Mat4& cameraMVP() const { return cameraMVP_; }
+    const Mat4& cameraMV() const { return cameraMV_; }
+    std::array<float, 9> normalMatrix() const;
 
 private:
     int windowWidth_ = 0, windowHeight_ = 0, smw_ = 0, smh_ = 0, maxSamples_ = 0, nverts_ = 0, ntris_ = 0;
     bool ownStream_ = false;
     std::vector<Mat4> lightMVPs_;
-    Mat4 cameraMVP_{};
-    DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, soft_, ws_;
+    Mat4 cameraMVP_{}, cameraMV_{};
+    DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, gfac_, soft_, rgba_, ws_;
 };
 
 }  // namespace gi::host

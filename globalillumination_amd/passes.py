@@ -186,6 +186,7 @@ def render_shadow_map(mesh: DeviceMesh, mvps: np.ndarray, width: int, height: in
 class GBuffer:
     position: torch.Tensor  # (3, H, W)
     normal: torch.Tensor    # (3, H, W), facing-flipped
+    facing: torch.Tensor | None = None  # (H, W): 1.0 front-facing, 0.0 back-facing or background
 
     @property
     def width(self) -> int:
@@ -198,7 +199,8 @@ class GBuffer:
     def to_c(self, with_normals: bool = True) -> _lib.GiGBuffer:
         return _lib.GiGBuffer(_dptr(self.position).value,
                               _dptr(self.normal).value if with_normals else None,
-                              self.width, self.height)
+                              self.width, self.height,
+                              _dptr(self.facing).value if with_normals and self.facing is not None else None)
 
 
 def render_gbuffer(mesh: DeviceMesh, mvp: np.ndarray, width: int, height: int, stream=None) -> GBuffer:
@@ -206,12 +208,13 @@ def render_gbuffer(mesh: DeviceMesh, mvp: np.ndarray, width: int, height: int, s
     dev = mesh.positions.device
     pos = torch.empty((3, height, width), dtype=torch.float32, device=dev)
     nrm = torch.empty((3, height, width), dtype=torch.float32, device=dev)
+    fac = torch.empty((height, width), dtype=torch.float32, device=dev)
     nbytes = _lib.lib().gi_gbuffer_pass_workspace_size(mesh.ntris, width, height)
     ws = _WS.get(nbytes, dev)
     _lib.call("gi_gbuffer_pass", _dptr(mesh.positions), _dptr(mesh.normals), mesh.nverts, _dptr(mesh.indices),
-              mesh.ntris, _f(mvp, 16), width, height, _dptr(pos), _dptr(nrm), _dptr(ws),
+              mesh.ntris, _f(mvp, 16), width, height, _dptr(pos), _dptr(nrm), _dptr(fac), _dptr(ws),
               ctypes.c_size_t(ws.numel()), _stream_handle(stream))
-    return GBuffer(pos, nrm)
+    return GBuffer(pos, nrm, fac)
 
 
 def compute_hard_shadows(gbuf: GBuffer, depth: torch.Tensor, params: ShadowParams,
@@ -223,6 +226,56 @@ def compute_hard_shadows(gbuf: GBuffer, depth: torch.Tensor, params: ShadowParam
     p = params.to_c()
     _lib.call("gi_hard_shadow_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out),
               _stream_handle(stream))
+    return out
+
+
+# G-buffer colour of a mesh without per-vertex colours: GBuffer.frag:27 reads
+# gl_FrontLightModelProduct.sceneColor, the default material ambient (0.2) times
+# the light-model ambient (0.1, MyGLGeometryViewer.cpp:69), alpha 1.
+SCENE_COLOR = tuple(float(np.float32(0.2) * np.float32(0.1)) for _ in range(3)) + (1.0,)
+CLEAR_COLOR = (0.63, 0.82, 0.96, 1.0)  # shadeScene / deferredShading, main.cpp:452
+
+
+@dataclass
+class PhongParams:
+    """Uniforms of PhongShading.frag:1-8 as displaySceneFromGBuffer sets them."""
+    MV: np.ndarray
+    normalMatrix: np.ndarray
+    lightPosition: tuple
+    shadowIntensity: float = 0.25
+    color: tuple = SCENE_COLOR
+    clearColor: tuple = CLEAR_COLOR
+
+    @classmethod
+    def from_views(cls, cam: View, light_eye, shadowIntensity: float = 0.25, **kw) -> "PhongParams":
+        MV, nm, lpos = phong_uniforms(cam, light_eye)
+        return cls(MV, nm, lpos, shadowIntensity, **kw)
+
+    def to_c(self) -> _lib.GiPhongParams:
+        p = _lib.GiPhongParams()
+        p.mv = _f(self.MV, 16)
+        p.normal_matrix = _f(self.normalMatrix, 9)
+        p.light_position = _f(self.lightPosition, 3)
+        p.shadow_intensity = float(self.shadowIntensity)
+        p.color = _f(self.color, 4)
+        p.clear_color = _f(self.clearColor, 4)
+        return p
+
+
+def shade_scene(gbuf: GBuffer, shadow: torch.Tensor, params: PhongParams, color: torch.Tensor | None = None,
+                out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """Deferred Phong shading (shadeScene main.cpp:449-456 / deferredShading
+    Soft main.cpp:1109-1117, PhongShading.frag): (H, W, 4) RGBA from the
+    G-buffer and a hard or soft shadow buffer; `color` is an optional
+    (H, W, 4) G-buffer colour plane."""
+    if gbuf.facing is None:
+        raise ValueError("shade_scene needs a G-buffer with its facing plane (render_gbuffer makes one)")
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width, 4), dtype=torch.float32, device=gbuf.position.device)
+    g = gbuf.to_c(with_normals=True)
+    p = params.to_c()
+    _lib.call("gi_phong_pass", ctypes.byref(g), _dptr(shadow), _dptr(color) if color is not None else None,
+              ctypes.byref(p), _dptr(out), _stream_handle(stream))
     return out
 
 

@@ -126,12 +126,16 @@ size_t gi_gbuffer_pass_workspace_size(int ntris, int width, int height);
 int gi_gbuffer_pass(const float* positions_dev, const float* normals_dev, int nverts,
                     const int32_t* indices_dev, int ntris, const float mvp[16],
                     int width, int height, float* position_out_dev, float* normal_out_dev,
-                    void* workspace_dev, size_t workspace_bytes, gi_stream stream);
+                    float* facing_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream);
+/* facing_out_dev (may be NULL): one plane, gl_FrontFacing as 1.0 / 0.0
+ * (GBuffer.frag:36), 0 for background; the deferred Phong pass reads it to
+ * recover the raw normal. */
 
 typedef struct gi_gbuffer {
     const float* position_dev; /* 3 planes of width*height */
     const float* normal_dev;   /* 3 planes, facing-flipped; may be NULL when unused */
     int width, height;
+    const float* facing_dev;   /* 1 plane, 1.0 front-facing / 0.0; may be NULL when unused */
 } gi_gbuffer;
 
 /* ------------------------------------------------------------------------ */
@@ -195,6 +199,32 @@ typedef struct gi_soft_params {
 int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples,
                         int smw, int smh, const float light_mvp[16], const float* trans_dev,
                         const gi_soft_params* params, float* out_dev, gi_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Deferred Phong shading                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* shadeScene / deferredShading (ShadowMapping/src/main.cpp:449-456,
+ * SoftShadowMapping/src/main.cpp:1109-1117) with PhongShading.frag:11-46:
+ * the shadow R channel (hard or soft pass output) modulates ambient + diffuse
+ * + (shadow - si)-weighted specular of the G-buffer colour.  mv, normal_matrix
+ * and light_position are configurePhong's uniforms (MyGLGeometryViewer.cpp:
+ * 108-134; the light is the world light rotated 180 degrees about y, as
+ * displaySceneFromGBuffer passes it, main.cpp:309). */
+typedef struct gi_phong_params {
+    float mv[16];
+    float normal_matrix[9];
+    float light_position[3];
+    float shadow_intensity;
+    float color[4];       /* G-buffer colour of every pixel when color_dev is NULL */
+    float clear_color[4]; /* background pixels (vertex.x == 0 is discarded) */
+} gi_phong_params;
+
+/* gbuffer: position, normal and facing planes.  shadow_dev: width*height R
+ * channel.  color_dev: RGBA per pixel (16-byte aligned) or NULL.
+ * rgba_out_dev: RGBA per pixel (16-byte aligned). */
+int gi_phong_pass(const gi_gbuffer* gbuffer, const float* shadow_dev, const float* color_dev,
+                  const gi_phong_params* params, float* rgba_out_dev, gi_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Differentiable area-light soft shadows                                     */

@@ -53,7 +53,8 @@ def lib() -> ctypes.CDLL:
             "or_depth_raster": [vp, i, vp, i, vp, i, i, f, f, vp],
             "or_depth_raster_brute": [vp, i, vp, i, vp, i, i, f, f, vp],
             "or_visibility_raster": [vp, i, vp, i, vp, i, i, vp],
-            "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp],
+            "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp, vp],
+            "or_phong": [vp, vp, vp, i, i, vp, vp, vp, vp],
             "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
             "or_diff_soft": [vp, i, i, vp, i, i, i, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp],
             "or_set_threads": [i], "or_get_threads": [],
@@ -207,10 +208,51 @@ def gbuffer(positions, normals, indices, mvp, w, h):
     m = _c32(mvp).reshape(-1)
     po = np.empty((3, h, w), np.float32)
     no = np.empty((3, h, w), np.float32)
-    rc = lib().or_gbuffer(_p(pos), _p(nrm), pos.shape[0], _p(idx), idx.shape[0], _p(m), w, h, _p(po), _p(no))
+    fo = np.empty((h, w), np.float32)
+    rc = lib().or_gbuffer(_p(pos), _p(nrm), pos.shape[0], _p(idx), idx.shape[0], _p(m), w, h, _p(po), _p(no),
+                          _p(fo))
     if rc:
         raise RuntimeError(f"or_gbuffer failed ({rc})")
-    return po, no
+    return po, no, fo
+
+
+class _PhongParams(ctypes.Structure):
+    _fields_ = [("mv", ctypes.c_float * 16), ("normal_matrix", ctypes.c_float * 9),
+                ("light_position", ctypes.c_float * 3), ("shadow_intensity", ctypes.c_float),
+                ("color", ctypes.c_float * 4), ("clear_color", ctypes.c_float * 4)]
+
+
+# gl_FrontLightModelProduct.sceneColor for the default material: ambient 0.2
+# times the light-model ambient 0.1 (ShadowMapping/src/Viewers/MyGLGeometryViewer.cpp:69)
+_SCENE = float(np.float32(0.2) * np.float32(0.1))
+
+
+def phong(position, normal, facing, shadow, mv, normal_matrix, light_position, shadow_intensity,
+          color=(_SCENE, _SCENE, _SCENE, 1.0), clear_color=(0.63, 0.82, 0.96, 1.0), color_plane=None) -> np.ndarray:
+    """PhongShading.frag over a G-buffer: (H, W, 4) RGBA.  `color` is the
+    G-buffer colour of every pixel unless `color_plane` (H, W, 4) is given."""
+    pos = _c32(position)
+    h, w = pos.shape[1], pos.shape[2]
+    nrm = _c32(normal)
+    fac = _c32(facing)
+    sh = _c32(shadow)
+    assert pos.shape == (3, h, w) and nrm.shape == (3, h, w) and fac.shape == (h, w) and sh.shape == (h, w)
+    cp = None if color_plane is None else _c32(color_plane)
+    if cp is not None:
+        assert cp.shape == (h, w, 4)
+    p = _PhongParams()
+    for name, val, n in (("mv", mv, 16), ("normal_matrix", normal_matrix, 9), ("light_position", light_position, 3),
+                         ("color", color, 4), ("clear_color", clear_color, 4)):
+        a = _c32(val).reshape(-1)
+        assert a.size == n, name
+        setattr(p, name, (ctypes.c_float * n)(*a.tolist()))
+    p.shadow_intensity = float(np.float32(shadow_intensity))
+    out = np.empty((h, w, 4), np.float32)
+    rc = lib().or_phong(_p(pos), _p(nrm), _p(fac), w, h, _p(sh), None if cp is None else _p(cp), ctypes.byref(p),
+                        _p(out))
+    if rc:
+        raise RuntimeError(f"or_phong failed ({rc})")
+    return out
 
 
 def _c64(a) -> np.ndarray:

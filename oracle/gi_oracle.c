@@ -913,7 +913,7 @@ int or_visibility_raster(const float* pos, int nv, const int* idx, int nt, const
  * of the object-space position and normal (FP64), gl_FrontFacing from the
  * window-space winding (CCW = front). */
 int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
-               const float mvp[16], int w, int h, float* pos_out, float* nrm_out)
+               const float mvp[16], int w, int h, float* pos_out, float* nrm_out, float* fac_out)
 {
     if (!nrm || !pos_out || !nrm_out) return -1;
     size_t npx = (size_t)w * (size_t)h;
@@ -928,6 +928,7 @@ int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int n
     for (long long k = 0; k < n; ++k) {
         uint64_t key = vis[k];
         float out[6] = {0, 0, 0, 0, 0, 0};
+        float facing = 0.0f;
         if (key != ~(uint64_t)0) {
             unsigned id = (unsigned)(key & 0xffffffffu);
             int tri = (int)(id >> 3), piece = (int)(id & 7u);
@@ -949,10 +950,12 @@ int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int n
                         out[q] = (float)(num / den);
                     }
                     if (!t.front) { out[3] = -out[3]; out[4] = -out[4]; out[5] = -out[5]; }
+                    facing = t.front ? 1.0f : 0.0f; /* GBuffer.frag:36 float(gl_FrontFacing) */
                 }
             }
         }
         for (int q = 0; q < 3; ++q) { pos_out[q * npx + k] = out[q]; nrm_out[q * npx + k] = out[3 + q]; }
+        if (fac_out) fac_out[k] = facing;
     }
     free(vis);
     return 0;
@@ -1036,6 +1039,80 @@ int or_diff_soft(const float* gpos, int gw, int gh, const double* maps, int nsam
             acc += si + (1.0 - si) * q;
         }
         out[k] = acc / nsamples;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deferred Phong shading: ShadowMapping/Shaders/GBuffer/PhongShading.frag   */
+/* ------------------------------------------------------------------------ */
+
+/* phong(vertex, normal, shadow), PhongShading.frag:11-35.  `nrm` is the raw
+ * G-buffer normal (GBuffer.frag:36 stores it unflipped). */
+static void phong_pixel(const or_phong_params* p, float vx, float vy, float vz, const float nrm[3],
+                        float shadow, const float col[4], float o[4])
+{
+    static const float amb[4] = {0.4f, 0.4f, 0.4f, 1.0f};   /* :14 */
+    static const float spe[4] = {0.25f, 0.25f, 0.25f, 1.0f}; /* :15 */
+    static const float dif[4] = {0.5f, 0.5f, 0.5f, 1.0f};    /* :16 */
+    float specShadow = shadow - p->shadow_intensity;           /* :18 */
+    float v[4];
+    m4v(p->mv, vx, vy, vz, 1.0f, v);                           /* :20 */
+    float n3[3], N[3];
+    for (int i = 0; i < 3; ++i) {                              /* :21 */
+        float acc = p->normal_matrix[0 * 3 + i] * nrm[0];
+        acc = acc + p->normal_matrix[1 * 3 + i] * nrm[1];
+        acc = acc + p->normal_matrix[2 * 3 + i] * nrm[2];
+        n3[i] = acc;
+    }
+    v3_normalize(n3, N);
+    float ld[3] = {p->light_position[0] - v[0], p->light_position[1] - v[1], p->light_position[2] - v[2]};
+    float L[3];
+    v3_normalize(ld, L);                                       /* :23 */
+    /* :24 normalize(-vertex) of the vec4 (w included), assigned to a vec3 */
+    float e[4] = {-v[0], -v[1], -v[2], -v[3]};
+    float esq = e[0] * e[0] + e[1] * e[1] + e[2] * e[2] + e[3] * e[3];
+    float einv = 1.0f / sqrtf(esq);
+    float E[3] = {e[0] * einv, e[1] * einv, e[2] * einv};
+    /* :25 reflect(I, N) = I - 2.0 * dot(N, I) * N (GLSL 1.20 section 8.4) */
+    float dNL = N[0] * L[0] + N[1] * L[1] + N[2] * L[2];
+    float r[3], R[3];
+    for (int i = 0; i < 3; ++i) r[i] = -(L[i] - (2.0f * dNL) * N[i]);
+    v3_normalize(r, R);
+    float diff = dNL < 0.0f ? 0.0f : dNL;                      /* :28 max(dot(n, L), 0) */
+    float dRE = R[0] * E[0] + R[1] * E[1] + R[2] * E[2];
+    float x = dRE < 0.0f ? 0.0f : dRE;
+    /* :29 pow(x, 0.3 * shininess): 0.3f * 10.0f rounds to exactly 3.0f; GLSL
+     * leaves pow's precision open, this restatement takes (x * x) * x. */
+    float spec = (x * x) * x;
+    for (int c = 0; c < 4; ++c) {
+        float Idiff = dif[c] * diff;
+        float Ispec = (specShadow * spe[c]) * spec;
+        o[c] = (shadow * col[c]) * ((Idiff + Ispec) + amb[c]); /* :32 */
+    }
+}
+
+int or_phong(const float* pos, const float* nrm, const float* fac, int w, int h, const float* shadow,
+             const float* color, const or_phong_params* p, float* out)
+{
+    if (!pos || !nrm || !fac || !shadow || !p || !out || w <= 0 || h <= 0) return -1;
+    size_t plane = (size_t)w * (size_t)h;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        float* o = out + 4 * k;
+        float vx = pos[k];
+        if (vx == 0.0f) { /* :40 discard: the clear colour stays */
+            for (int c = 0; c < 4; ++c) o[c] = p->clear_color[c];
+            continue;
+        }
+        /* the stored normal is flipped on back faces; undo it */
+        float sgn = fac[k] != 0.0f ? 1.0f : -1.0f;
+        float nr[3] = {sgn * nrm[k], sgn * nrm[plane + k], sgn * nrm[2 * plane + k]};
+        const float* col = color ? color + 4 * k : p->color;
+        phong_pixel(p, vx, pos[plane + k], pos[2 * plane + k], nr, shadow[k], col, o);
     }
     return 0;
 }

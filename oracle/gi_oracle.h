@@ -86,7 +86,22 @@ int or_visibility_raster(const float* pos, int nv, const int* idx, int nt,
  * GBuffer.frag:35-37.  pos_out / nrm_out: 3 planes each; normal pre-flipped
  * when back-facing; background = 0. */
 int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
-               const float mvp[16], int w, int h, float* pos_out, float* nrm_out);
+               const float mvp[16], int w, int h, float* pos_out, float* nrm_out, float* fac_out);
+
+/* PhongShading.frag:11-46.  pos/nrm: G-buffer planes (normal flipped on back
+ * faces), fac: 1.0 front / 0.0 back, shadow: w*h R channel, color: RGBA per
+ * pixel or NULL (p->color everywhere); out: RGBA per pixel. */
+typedef struct or_phong_params {
+    float mv[16];
+    float normal_matrix[9];
+    float light_position[3];
+    float shadow_intensity;
+    float color[4];
+    float clear_color[4];
+} or_phong_params;
+
+int or_phong(const float* pos, const float* nrm, const float* fac, int w, int h, const float* shadow,
+             const float* color, const or_phong_params* p, float* out);
 
 /* ---- differentiable soft shadows (continuous model, FP64) ------------------ */
 /* Texel depth of the continuous model: for every texel whose visibility key

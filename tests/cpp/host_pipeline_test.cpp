@@ -117,8 +117,8 @@ int main(int argc, char** argv) {
         p.RSMSS = t.rsmss;
         rc = hard.display();
         if (rc) return fail(t.name, rc);
-        std::vector<float> out, depth, gpos, gnrm;
-        rc = first ? hard.download(out, &depth, &gpos, &gnrm) : hard.download(out);
+        std::vector<float> out, depth, gpos, gnrm, rgba;
+        rc = first ? hard.download(out, &depth, &gpos, &gnrm, &rgba) : hard.download(out, nullptr, nullptr, nullptr, &rgba);
         if (rc) return fail("download", rc);
         if (first) {
             dumpv(dir, "depth.f32", depth);
@@ -127,9 +127,11 @@ int main(int argc, char** argv) {
             dump(dir + "/light_mvp.f32", p.lightMVP.data(), 64);
             dump(dir + "/camera_mvp.f32", hard.cameraMVP().data(), 64);
             dump(dir + "/camera_mv.f32", hard.cameraMV().data(), 64);
+            dump(dir + "/camera_nm.f32", hard.normalMatrix().data(), 36);
             first = false;
         }
         dumpv(dir, (std::string("hard_") + t.name + ".f32").c_str(), out);
+        dumpv(dir, (std::string("phong_") + t.name + ".f32").c_str(), rgba);
     }
 
     SoftShadowPipeline soft;
@@ -147,10 +149,15 @@ int main(int argc, char** argv) {
     if (rc) return fail("SoftShadowPipeline::renderGBuffer", rc);
     rc = soft.renderMonteCarlo();
     if (rc) return fail("renderMonteCarlo", rc);
-    std::vector<float> sout, maps;
-    rc = soft.download(sout, &maps);
+    rc = soft.deferredShading();
+    if (rc) return fail("deferredShading", rc);
+    std::vector<float> sout, maps, srgba;
+    rc = soft.download(sout, &maps, &srgba);
     if (rc) return fail("download", rc);
     dumpv(dir, "soft.f32", sout);
+    dumpv(dir, "soft_phong.f32", srgba);
+    dump(dir + "/soft_camera_mv.f32", soft.cameraMV().data(), 64);
+    dump(dir + "/soft_camera_nm.f32", soft.normalMatrix().data(), 36);
     dumpv(dir, "soft_maps.f32", maps);
     dump(dir + "/soft_light_mvps.f32", soft.lightMVPs().data(), soft.lightMVPs().size() * 64);
     std::printf("host pipeline ok: %d vertices, %d triangles\n", mesh.nverts(), mesh.ntris());

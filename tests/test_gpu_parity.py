@@ -122,9 +122,101 @@ def test_gbuffer_config1(gi, oracle_lib, cuda):
     dm = gi.DeviceMesh.from_mesh(wl.mesh)
     mvp = camera_mvp(wl)
     g = gi.render_gbuffer(dm, mvp, 800, 600)
-    pos, nrm = oracle_lib.gbuffer(wl.mesh.positions, wl.mesh.normals, wl.mesh.indices, mvp, 800, 600)
+    pos, nrm, _ = oracle_lib.gbuffer(wl.mesh.positions, wl.mesh.normals, wl.mesh.indices, mvp, 800, 600)
     assert bits_equal(_np(g.position), pos), mismatch_report(_np(g.position), pos, "gbuffer position")
     assert bits_equal(_np(g.normal), nrm), mismatch_report(_np(g.normal), nrm, "gbuffer normal")
+
+
+def test_gbuffer_facing_plane(gi, oracle_lib, cuda):
+    """float(gl_FrontFacing) per pixel on a scene seen from both sides of its
+    triangles (a soup: random winding)."""
+    mesh = shrink_soup(20_000)
+    wl = gi.scenes.config2(ntris=10, view_w=640, view_h=360, sm=512)
+    wl.mesh = mesh
+    dm = gi.DeviceMesh.from_mesh(mesh)
+    mvp = camera_mvp(wl)
+    g = gi.render_gbuffer(dm, mvp, 640, 360)
+    pos, nrm, fac = oracle_lib.gbuffer(mesh.positions, mesh.normals, mesh.indices, mvp, 640, 360)
+    assert bits_equal(_np(g.position), pos)
+    assert bits_equal(_np(g.normal), nrm)
+    assert bits_equal(_np(g.facing), fac), mismatch_report(_np(g.facing), fac, "gbuffer facing")
+    hit = pos[0] != 0
+    assert 0.01 < fac[hit].mean() < 0.99  # both sides are seen (the ground quad is most of the view)
+
+
+def _phong_case(gi, oracle_lib, g, shadow, wl, color_plane=None):
+    params = gi.PhongParams.from_views(wl.camera, wl.lights[0].eye, 0.25)
+    got = _np(gi.shade_scene(g, shadow, params, color=color_plane))
+    want = oracle_lib.phong(_np(g.position), _np(g.normal), _np(g.facing), _np(shadow), params.MV,
+                            params.normalMatrix, params.lightPosition, 0.25, color=params.color,
+                            clear_color=params.clearColor,
+                            color_plane=None if color_plane is None else _np(color_plane))
+    return got, want
+
+
+def test_phong_over_hard_shadows_config1(gi, oracle_lib, cuda):
+    """shadeScene: deferred Phong over the naive hard-shadow buffer."""
+    wl = gi.scenes.config1()
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)[:1]
+    depth = gi.render_shadow_map(dm, mvps, 1024, 1024)
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    params = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=1024, shadowMapHeight=1024, technique="naive",
+                             zbias=wl.zbias, preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos)
+    shadow = gi.compute_hard_shadows(g, depth[0], params)
+    got, want = _phong_case(gi, oracle_lib, g, shadow, wl)
+    assert bits_equal(got, want), mismatch_report(got, want, "phong (hard)")
+    bg = _np(g.position)[0] == 0
+    assert (got[bg] == np.float32(gi.passes.CLEAR_COLOR)).all()
+    assert (got[~bg, 0] > 0).mean() > 0.9
+
+
+def test_phong_over_soft_shadows_with_color_plane(gi, oracle_lib, cuda):
+    """deferredShading: Phong over the area-light soft-shadow buffer of a
+    back-and-front-facing soup, with a per-pixel colour plane."""
+    mesh = shrink_soup(20_000)
+    wl = gi.scenes.config3(nsamples=4, sm=512, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, 512, 512)
+    soft = gi.render_monte_carlo(g, maps, mvps, gi.SoftParams(pcf=True, kernelOrder=3))
+    gen = torch.Generator(device="cpu").manual_seed(11)
+    colors = torch.rand((wl.camera.height, wl.camera.width, 4), generator=gen).to("cuda")
+    got, want = _phong_case(gi, oracle_lib, g, soft, wl, colors)
+    assert bits_equal(got, want), mismatch_report(got, want, "phong (soft, colour plane)")
+    # the soup: back faces shade with the raw normal
+    dm2 = gi.DeviceMesh.from_mesh(mesh)
+    g2 = gi.render_gbuffer(dm2, camera_mvp(wl), wl.camera.width, wl.camera.height)
+    sh2 = torch.rand((wl.camera.height, wl.camera.width), generator=gen).to("cuda")
+    got, want = _phong_case(gi, oracle_lib, g2, sh2, wl)
+    assert bits_equal(got, want), mismatch_report(got, want, "phong (soup)")
+
+
+def test_phong_rejects(gi, cuda):
+    import ctypes
+
+    wl = gi.scenes.config1()
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 64, 48)
+    shadow = torch.ones((48, 64), device="cuda")
+    params = gi.PhongParams.from_views(wl.camera, wl.lights[0].eye)
+    out = torch.empty((48 * 64 * 4 + 1,), device="cuda")
+    c = g.to_c()
+    p = params.to_c()
+    L = gi.lib()
+    # misaligned output
+    assert L.gi_phong_pass(ctypes.byref(c), shadow.data_ptr(), None, ctypes.byref(p), out.data_ptr() + 4,
+                           None) == gi._lib.GI_EINVAL
+    # no facing plane
+    c.facing_dev = None
+    assert L.gi_phong_pass(ctypes.byref(c), shadow.data_ptr(), None, ctypes.byref(p), out.data_ptr(),
+                           None) == gi._lib.GI_EINVAL
+    c = g.to_c()
+    p.light_position[0] = float("nan")
+    assert L.gi_phong_pass(ctypes.byref(c), shadow.data_ptr(), None, ctypes.byref(p), out.data_ptr(),
+                           None) == gi._lib.GI_ENONFINITE
 
 
 def _hard_case(gi, oracle_lib, wl, technique, pre_eval, smw, smh, extra=None):

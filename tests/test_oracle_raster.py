@@ -223,7 +223,7 @@ def test_gbuffer_tilted_plane(oracle_lib, gi):
     px_world = 2 * 6.8 * np.tan(np.radians(30)) / H  # world size of a pixel at the far end
     for ccw, sign in [(True, 1.0), (False, -1.0)]:
         p_idx = idx if ccw else idx[:, [0, 2, 1]]
-        gp, gn = oracle_lib.gbuffer(pos, nrm, p_idx, mvp, W, H)
+        gp, gn, _ = oracle_lib.gbuffer(pos, nrm, p_idx, mvp, W, H)
         cover = gp[0] != 0
         assert cover.sum() > 100
         js, is_ = np.nonzero(cover)
