@@ -21,7 +21,7 @@ GI_EDEVICE = -3
 GI_ENOSPACE = -4
 GI_ENOTSUP = -5
 
-GI_NAIVE, GI_PCF, GI_RBSM, GI_RPCF, GI_RBSM_FILTERED = range(5)
+GI_NAIVE, GI_PCF, GI_RBSM, GI_RPCF, GI_RBSM_FILTERED, GI_RBSM_CONSERVATIVE, GI_RPCF_CONSERVATIVE = range(7)
 GI_MAX_SAMPLES = 4096
 GI_MAX_EDGE = 16384
 

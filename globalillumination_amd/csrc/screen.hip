@@ -43,13 +43,14 @@ extern "C" {
 int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_shadow_params* p,
                         float* out_dev, gi_stream stream) {
     if (!gbuffer_ok(gbuffer) || !depth_dev || !p || !out_dev) return GI_EINVAL;
-    if (p->technique < GI_NAIVE || p->technique > GI_RBSM_FILTERED) return GI_EINVAL;
+    if (p->technique < GI_NAIVE || p->technique > GI_RPCF_CONSERVATIVE) return GI_EINVAL;
     if (p->shadow_map_width <= 0 || p->shadow_map_height <= 0 || p->shadow_map_width > GI_MAX_EDGE ||
         p->shadow_map_height > GI_MAX_EDGE)
         return GI_EINVAL;
     if (!finite_all(p->light_mvp, 16) || !std::isfinite(p->shadow_intensity) || !std::isfinite(p->depth_threshold))
         return GI_ENONFINITE;
-    bool uses_loop = p->technique == GI_PCF || p->technique == GI_RPCF || p->technique == GI_RBSM_FILTERED;
+    bool uses_loop = p->technique == GI_PCF || p->technique == GI_RPCF || p->technique == GI_RBSM_FILTERED ||
+                     p->technique == GI_RPCF_CONSERVATIVE;
     // The PCF loops step by 2*penumbra/kernelOrder: a zero penumbra would never
     // advance, so it is rejected along with orders past the tap budget.
     if (uses_loop && (p->kernel_order <= 0 || p->penumbra_size <= 0 || p->kernel_order > 64 ||
@@ -84,7 +85,11 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
     case GI_PCF: hipLaunchKernelGGL(k_hard<GI_PCF>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_RBSM: hipLaunchKernelGGL(k_hard<GI_RBSM>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_RPCF: hipLaunchKernelGGL(k_hard<GI_RPCF>, grid, block, 0, s, gb, D, u, out_dev); break;
-    default: hipLaunchKernelGGL(k_hard<GI_RBSM_FILTERED>, grid, block, 0, s, gb, D, u, out_dev); break;
+    case GI_RBSM_FILTERED: hipLaunchKernelGGL(k_hard<GI_RBSM_FILTERED>, grid, block, 0, s, gb, D, u, out_dev); break;
+    case GI_RBSM_CONSERVATIVE:
+        hipLaunchKernelGGL(k_hard<GI_RBSM_CONSERVATIVE>, grid, block, 0, s, gb, D, u, out_dev);
+        break;
+    default: hipLaunchKernelGGL(k_hard<GI_RPCF_CONSERVATIVE>, grid, block, 0, s, gb, D, u, out_dev); break;
     }
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }

@@ -279,6 +279,123 @@ struct Rbsm {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Conservative revectorization: ConservativeSMSR.frag, the shader
+// computeHardShadows binds when ShadowParams.conservative is set
+// (ShadowMapping/src/main.cpp:406-408).  Binary neighbour tests and walks that
+// stop at the first umbra texel; the result is si or 1 per tap.
+// ---------------------------------------------------------------------------
+struct RbsmConservative {
+    DepthMap D;
+    float sx, sy;  // shadowMapStep
+    float thr;
+    int max_search;
+    float si;
+
+    __device__ __forceinline__ float tex(float s, float t) const { return D.fetch(s, t); }
+
+    // computeDiscontinuity :23-48, abs(dir - 1): 1 where the neighbour is in umbra.
+    // Returns left + right (in *lr) and the four-way sum.
+    __device__ float disc(float x, float y, float z, float* lr) const {
+        float xx = x - sx;
+        const float l = fabsf(le01(z, tex(xx, y)) - 1.0f);
+        xx = xx + 2.0f * sx;
+        const float r = fabsf(le01(z, tex(xx, y)) - 1.0f);
+        xx = xx - sx;
+        float yy = y + sy;
+        const float b = fabsf(le01(z, tex(xx, yy)) - 1.0f);
+        yy = yy - 2.0f * sy;
+        const float t = fabsf(le01(z, tex(xx, yy)) - 1.0f);
+        if (lr) {
+            lr[0] = l;
+            lr[1] = r;
+            lr[2] = b;
+            lr[3] = t;
+        }
+        return ((l + r) + b) + t;
+    }
+
+    // computeRelativeDistance(vec4, vec2, float) :50-86
+    __device__ float walk(float x, float y, float z, float dirx, float diry, float c) const {
+        const float stx = dirx * sx, sty = diry * sy;
+        float cx = x + stx, cy = y + sty, cz = z;
+        float found = 0.0f, dist = 0.0f;
+        for (int it = 0; it < max_search; ++it) {
+            const float d = tex(cx, cy);
+            if (fabsf(cz - d) < thr) cz = cz - thr;
+            if (le01(cz, d) == 0.0f) {  // isCenterUmbra
+                found = 1.0f;
+                break;
+            }
+            if (disc(cx, cy, cz, nullptr) == 0.0f) break;
+            dist = dist + 1.0f;
+            cx = cx + stx;
+            cy = cy + sty;
+        }
+        dist = dist + (1.0f - c);
+        return gmix(-dist, dist, found);
+    }
+
+    // normalizeRelativeDistance(vec2) :99-108
+    __device__ float norm(float a, float b) const {
+        float T = 1.0f;
+        if (a < 0.0f && b < 0.0f) T = 0.0f;
+        if (a > 0.0f && b > 0.0f) T = -2.0f;
+        const float len = gmin(fabsf(a) + fabsf(b), (float)max_search);
+        return fabsf(gmax(T * a, T * b)) / len;
+    }
+
+    // computeRelativeDistance(vec4, vec2) :88-97 -> normalizeRelativeDistance
+    // :110-118 -> revectorizeShadow :120-126
+    __device__ float revectorized(float x, float y, float z, float cx, float cy) const {
+        const float dl = walk(x, y, z, -1.0f, 0.0f, 1.0f - cx);
+        const float dr = walk(x, y, z, 1.0f, 0.0f, cx);
+        const float db = walk(x, y, z, 0.0f, -1.0f, 1.0f - cy);
+        const float dt = walk(x, y, z, 0.0f, 1.0f, cy);
+        const float rx = norm(dl, dr), ry = norm(db, dt);
+        if ((rx * ry > 0.0f) && (1.0f - rx > ry)) return si;
+        return 1.0f;
+    }
+
+    // computeShadowFromSMSR :128-144
+    __device__ float smsr(float x, float y, float z) const {
+        if (le01(z, tex(x, y)) == 0.0f) return si;
+        float lr[4];
+        if (disc(x, y, z, lr) == 0.0f) return 1.0f;
+        if (lr[0] + lr[1] == 2.0f || lr[2] + lr[3] == 2.0f) return si;
+        return revectorized(x, y, z, gfract(x * D.fw), gfract(y * D.fh));
+    }
+
+    // computeShadowFromRPCF :146-198
+    __device__ float rpcf(float x, float y, float z, int kernel_order, int penumbra) const {
+        const float incrWidth = 1.0f / D.fw;
+        const float incrHeight = 1.0f / D.fh;
+        float illum = 0.0f;
+        const float offset = (float)penumbra;
+        const float stepSize = 2.0f * offset / (float)kernel_order;
+        int count = 0;
+        for (float w = -offset; w <= offset; w += stepSize) {
+            for (float h = -offset; h <= offset; h += stepSize) {
+                const float px = x + w * incrWidth;
+                const float py = y + h * incrHeight;
+                const float shadow = (z <= tex(px, py)) ? 1.0f : si;
+                if (shadow == 1.0f) {
+                    float lr[4];
+                    disc(px, py, z, lr);
+                    if (lr[0] == 0.0f && lr[1] == 0.0f)
+                        illum = illum + 1.0f;
+                    else
+                        illum = illum + revectorized(px, py, z, gfract(px * D.fw), gfract(py * D.fh));
+                } else {
+                    illum = illum + si;
+                }
+                count++;
+            }
+        }
+        return illum / (float)count;
+    }
+};
+
 // Pixel index with an XCD-aware block order: hardware hands consecutive
 // workgroups to the 8 XCDs round-robin, so block b is remapped to make each
 // XCD's blocks one contiguous band of rows, keeping neighbouring pixels'
@@ -326,6 +443,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap
             } else {
                 shadow = pcf(D, u, n0, n1, n2);
             }
+        }
+    } else if constexpr (kTech == GI_RBSM_CONSERVATIVE || kTech == GI_RPCF_CONSERVATIVE) {
+        // ConservativeSMSR.frag:215-235 (no shadowCoord.w test)
+        RbsmConservative rb{D, u.stepx, u.stepy, u.thr, u.max_search, u.si};
+        if (shadow == 1.0f) {
+            if constexpr (kTech == GI_RBSM_CONSERVATIVE)
+                shadow = rb.smsr(n0, n1, n2);
+            else
+                shadow = rb.rpcf(n0, n1, n2, u.kernel_order, u.penumbra);
         }
     } else {
         Rbsm<kTech == GI_RBSM_FILTERED> rb{D, u.stepx, u.stepy, u.thr, u.max_search};

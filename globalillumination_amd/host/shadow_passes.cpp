@@ -162,8 +162,8 @@ int HardShadowPipeline::renderGBuffer() {
 // The shader computeHardShadows picks from the flags (main.cpp:400-414).
 int HardShadowPipeline::technique() const {
     const ShadowParams& p = shadowParams;
-    if (p.SMSR) return GI_RBSM;
-    if (p.RPCFPlusSMSR) return GI_RPCF;
+    if (p.SMSR) return p.conservative ? GI_RBSM_CONSERVATIVE : GI_RBSM;
+    if (p.RPCFPlusSMSR) return p.conservative ? GI_RPCF_CONSERVATIVE : GI_RPCF;
     if (p.RSMSS || p.RPCFPlusRSMSS) return GI_RBSM_FILTERED;
     if (p.naive) return GI_NAIVE;
     return GI_PCF;  // Shadow.frag falls through to PCF (:265-266)
@@ -171,7 +171,6 @@ int HardShadowPipeline::technique() const {
 
 int HardShadowPipeline::computeHardShadows() {
     const ShadowParams& sp = shadowParams;
-    if (sp.conservative) return GI_ENOTSUP;  // ConservativeSMSR.frag is not on this path
     if (!sp.naive && !sp.SMSR && !sp.RPCFPlusSMSR && !sp.RSMSS && !sp.RPCFPlusRSMSS && !sp.bilinearPCF)
         return GI_ENOTSUP;  // Shadow.frag's PCF reads an unset depth without bilinearPCF (:101-104)
     gi_shadow_params p{};

@@ -24,7 +24,8 @@ from . import _lib, camera
 from .scenes import Mesh, View
 
 TECHNIQUES = {"naive": _lib.GI_NAIVE, "pcf": _lib.GI_PCF, "rbsm": _lib.GI_RBSM,
-              "rpcf": _lib.GI_RPCF, "rbsm_filtered": _lib.GI_RBSM_FILTERED}
+              "rpcf": _lib.GI_RPCF, "rbsm_filtered": _lib.GI_RBSM_FILTERED,
+              "rbsm_conservative": _lib.GI_RBSM_CONSERVATIVE, "rpcf_conservative": _lib.GI_RPCF_CONSERVATIVE}
 
 
 def _stream_handle(stream=None) -> ctypes.c_void_p:

@@ -150,7 +150,11 @@ typedef enum gi_technique {
     GI_PCF = 1,           /* Shadow.frag bilinearPCF == 1, PCF() :86-116 */
     GI_RBSM = 2,          /* NonConservativeSMSR.frag SMSR == 1 :288-304 */
     GI_RPCF = 3,          /* NonConservativeSMSR.frag RPCFPlusSMSR == 1 :306-349 */
-    GI_RBSM_FILTERED = 4  /* FilteredRBSM.frag (RSMSS) :306-349 */
+    GI_RBSM_FILTERED = 4, /* FilteredRBSM.frag (RSMSS) :306-349 */
+    /* ConservativeSMSR.frag, bound instead of NonConservativeSMSR when
+     * ShadowParams.conservative is set (ShadowMapping/src/main.cpp:406-408) */
+    GI_RBSM_CONSERVATIVE = 5, /* SMSR == 1: computeShadowFromSMSR :128-144 */
+    GI_RPCF_CONSERVATIVE = 6  /* RPCFPlusSMSR == 1: computeShadowFromRPCF :146-198 */
 } gi_technique;
 
 /* Mirrors ShadowParams (ShadowMapping/include/Viewers/ShadowParams.h:6-37)

@@ -15,8 +15,9 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "libgi_oracle.so"
 
-NAIVE, PCF, RBSM, RPCF, RBSM_FILTERED = range(5)
-TECHNIQUES = {"naive": NAIVE, "pcf": PCF, "rbsm": RBSM, "rpcf": RPCF, "rbsm_filtered": RBSM_FILTERED}
+NAIVE, PCF, RBSM, RPCF, RBSM_FILTERED, RBSM_CONSERVATIVE, RPCF_CONSERVATIVE = range(7)
+TECHNIQUES = {"naive": NAIVE, "pcf": PCF, "rbsm": RBSM, "rpcf": RPCF, "rbsm_filtered": RBSM_FILTERED,
+              "rbsm_conservative": RBSM_CONSERVATIVE, "rpcf_conservative": RPCF_CONSERVATIVE}
 
 F16 = ctypes.c_float * 16
 

@@ -505,6 +505,118 @@ static float shadow_from_rpcf(const rb_ctx* c, int kernelOrder, int penumbraSize
     return illuminationCount / (float)count;
 }
 
+/* ------------------------------------------------------------------------ */
+/* ConservativeSMSR.frag (ShadowParams.conservative, main.cpp:406-408)       */
+/* ------------------------------------------------------------------------ */
+
+/* computeDiscontinuity :23-48: out[k] = |lit - 1| for left, right, bottom, top */
+static void cons_disc(const rb_ctx* c, float x, float y, float z, float out[4])
+{
+    float sx = c->stepx, sy = c->stepy;
+    x = x - sx;
+    out[0] = fabsf(cmp_le(z, rb_tex(c, x, y)) - 1.0f);
+    x = x + 2.0f * sx;
+    out[1] = fabsf(cmp_le(z, rb_tex(c, x, y)) - 1.0f);
+    x = x - sx;
+    y = y + sy;
+    out[2] = fabsf(cmp_le(z, rb_tex(c, x, y)) - 1.0f);
+    y = y - 2.0f * sy;
+    out[3] = fabsf(cmp_le(z, rb_tex(c, x, y)) - 1.0f);
+}
+
+static float sum4(const float v[4]) { return ((v[0] + v[1]) + v[2]) + v[3]; }
+
+/* computeRelativeDistance(vec4, vec2 dir, float c) :50-86 */
+static float cons_distance(const rb_ctx* c, float x, float y, float z, float dx, float dy, float sub)
+{
+    float stepx = dx * c->stepx, stepy = dy * c->stepy;
+    float found = 0.0f, distance = 0.0f;
+    x = x + stepx;
+    y = y + stepy;
+    for (int it = 0; it < c->maxSearch; ++it) {
+        float d = rb_tex(c, x, y);
+        if (fabsf(z - d) < c->thr) z = z - c->thr; /* depth-accuracy bias, kept for later steps */
+        if (!(cmp_le(z, d) != 0.0f)) { found = 1.0f; break; }
+        float disc[4];
+        cons_disc(c, x, y, z, disc);
+        if (sum4(disc) == 0.0f) break;
+        distance = distance + 1.0f;
+        x = x + stepx;
+        y = y + stepy;
+    }
+    distance = distance + (1.0f - sub);
+    return gmix(-distance, distance, found);
+}
+
+/* normalizeRelativeDistance(vec2) :99-108 */
+static float cons_normalize(const rb_ctx* c, float a, float b)
+{
+    float T = 1.0f;
+    if (a < 0.0f && b < 0.0f) T = 0.0f;
+    if (a > 0.0f && b > 0.0f) T = -2.0f;
+    float length = gmin(fabsf(a) + fabsf(b), (float)c->maxSearch);
+    return fabsf(gmax(T * a, T * b)) / length;
+}
+
+/* :88-97, :110-118, revectorizeShadow :120-126 */
+static float cons_revectorize(const rb_ctx* c, float x, float y, float z, float subx, float suby)
+{
+    float dl = cons_distance(c, x, y, z, -1.0f, 0.0f, 1.0f - subx);
+    float dr = cons_distance(c, x, y, z, 1.0f, 0.0f, subx);
+    float db = cons_distance(c, x, y, z, 0.0f, -1.0f, 1.0f - suby);
+    float dt = cons_distance(c, x, y, z, 0.0f, 1.0f, suby);
+    float rx = cons_normalize(c, dl, dr);
+    float ry = cons_normalize(c, db, dt);
+    return ((rx * ry > 0.0f) && (1.0f - rx > ry)) ? c->si : 1.0f;
+}
+
+/* computeShadowFromSMSR :128-144 */
+static float cons_smsr(const rb_ctx* c, float x, float y, float z)
+{
+    float d = rb_tex(c, x, y);
+    if (cmp_le(z, d) == 0.0f) return c->si;
+    float disc[4];
+    cons_disc(c, x, y, z, disc);
+    if (sum4(disc) == 0.0f) return 1.0f;
+    if ((disc[0] + disc[1]) == 2.0f || (disc[2] + disc[3]) == 2.0f) return c->si;
+    float subx = gfract(x * (float)c->W);
+    float suby = gfract(y * (float)c->H);
+    return cons_revectorize(c, x, y, z, subx, suby);
+}
+
+/* computeShadowFromRPCF :146-198 */
+static float cons_rpcf(const rb_ctx* c, int kernelOrder, int penumbraSize, float s, float t, float z)
+{
+    float incrWidth = 1.0f / (float)c->W;
+    float incrHeight = 1.0f / (float)c->H;
+    float offset = (float)penumbraSize;
+    float stepSize = 2.0f * offset / (float)kernelOrder;
+    float illum = 0.0f;
+    int count = 0;
+    for (float w = -offset; w <= offset; w += stepSize) {
+        for (float h = -offset; h <= offset; h += stepSize) {
+            float px = s + w * incrWidth, py = t + h * incrHeight;
+            float d = rb_tex(c, px, py);
+            float shadow = (z <= d) ? 1.0f : c->si;
+            if (shadow == 1.0f) {
+                float disc[4];
+                cons_disc(c, px, py, z, disc);
+                if (disc[0] == 0.0f && disc[1] == 0.0f) {
+                    illum = illum + 1.0f;
+                } else {
+                    float subx = gfract(px * (float)c->W);
+                    float suby = gfract(py * (float)c->H);
+                    illum = illum + cons_revectorize(c, px, py, z, subx, suby);
+                }
+            } else {
+                illum = illum + c->si;
+            }
+            count++;
+        }
+    }
+    return illum / (float)count;
+}
+
 static float hard_pixel(const or_params* p, const float* D, float vx, float vy, float vz,
                         float nx, float ny, float nz)
 {
@@ -535,8 +647,12 @@ static float hard_pixel(const or_params* p, const float* D, float vx, float vy, 
     c.si = p->shadow_intensity;
     c.filtered = (p->technique == OR_RBSM_FILTERED);
     if (shadow == 1.0f) {
-        if (p->technique == OR_RBSM) shadow = shadow_from_smsr(&c, n0, n1, n2);
-        else shadow = shadow_from_rpcf(&c, p->kernel_order, p->penumbra_size, n0, n1, n2);
+        switch (p->technique) {
+        case OR_RBSM: shadow = shadow_from_smsr(&c, n0, n1, n2); break;
+        case OR_RBSM_CONSERVATIVE: shadow = cons_smsr(&c, n0, n1, n2); break; /* ConservativeSMSR.frag:227-233 */
+        case OR_RPCF_CONSERVATIVE: shadow = cons_rpcf(&c, p->kernel_order, p->penumbra_size, n0, n1, n2); break;
+        default: shadow = shadow_from_rpcf(&c, p->kernel_order, p->penumbra_size, n0, n1, n2); break;
+        }
     }
     return shadow;
 }
@@ -546,7 +662,9 @@ int or_hard_shadow(const float* pos, const float* nrm, int w, int h, const float
 {
     if (!pos || !depth || !p || !out || w <= 0 || h <= 0) return -1;
     if (p->pre_evaluation && !nrm) return -1;
-    if (p->technique != OR_NAIVE && p->technique != OR_RBSM && (p->kernel_order <= 0 || p->penumbra_size <= 0))
+    if (p->technique < OR_NAIVE || p->technique > OR_RPCF_CONSERVATIVE) return -1;
+    if (p->technique != OR_NAIVE && p->technique != OR_RBSM && p->technique != OR_RBSM_CONSERVATIVE &&
+        (p->kernel_order <= 0 || p->penumbra_size <= 0))
         return -1; /* the PCF loop would not advance */
     size_t plane = (size_t)w * (size_t)h;
     long long n = (long long)plane;

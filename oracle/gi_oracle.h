@@ -39,7 +39,8 @@ void or_uniform_sampling(const float base[3], int size, int nsamples, int index,
 float or_fetch(const float* map, int w, int h, float s, float t);
 
 /* ---- per-technique screen passes (a8-a13) ---------------------------------- */
-enum { OR_NAIVE = 0, OR_PCF = 1, OR_RBSM = 2, OR_RPCF = 3, OR_RBSM_FILTERED = 4 };
+enum { OR_NAIVE = 0, OR_PCF = 1, OR_RBSM = 2, OR_RPCF = 3, OR_RBSM_FILTERED = 4,
+       OR_RBSM_CONSERVATIVE = 5, OR_RPCF_CONSERVATIVE = 6 };
 
 typedef struct or_params {
     float light_mvp[16];       /* B * lightMVP, column-major */

@@ -100,13 +100,15 @@ int main(int argc, char** argv) {
 
     struct Technique {
         const char* name;
-        bool naive, pcf, smsr, rpcf, rsmss;
+        bool naive, pcf, smsr, rpcf, rsmss, conservative;
     };
-    const Technique techs[] = {{"naive", true, false, false, false, false},
-                               {"pcf", false, true, false, false, false},
-                               {"rbsm", false, false, true, false, false},
-                               {"rpcf", false, false, false, true, false},
-                               {"rbsm_filtered", false, false, false, false, true}};
+    const Technique techs[] = {{"naive", true, false, false, false, false, false},
+                               {"pcf", false, true, false, false, false, false},
+                               {"rbsm", false, false, true, false, false, false},
+                               {"rpcf", false, false, false, true, false, false},
+                               {"rbsm_filtered", false, false, false, false, true, false},
+                               {"rbsm_conservative", false, false, true, false, false, true},
+                               {"rpcf_conservative", false, false, false, true, false, true}};
     bool first = true;
     for (const Technique& t : techs) {
         auto& p = hard.shadowParams;
@@ -115,6 +117,7 @@ int main(int argc, char** argv) {
         p.SMSR = t.smsr;
         p.RPCFPlusSMSR = t.rpcf;
         p.RSMSS = t.rsmss;
+        p.conservative = t.conservative;
         rc = hard.display();
         if (rc) return fail(t.name, rc);
         std::vector<float> out, depth, gpos, gnrm, rgba;

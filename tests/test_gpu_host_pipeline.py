@@ -42,7 +42,8 @@ def test_host_depth_and_gbuffer(dumps, oracle_lib):
     assert (load(dumps, "gbuf_nrm.f32").reshape(3, H, W).view(np.uint32) == gn.view(np.uint32)).all()
 
 
-@pytest.mark.parametrize("technique", ["naive", "pcf", "rbsm", "rpcf", "rbsm_filtered"])
+@pytest.mark.parametrize("technique", ["naive", "pcf", "rbsm", "rpcf", "rbsm_filtered", "rbsm_conservative",
+                                       "rpcf_conservative"])
 def test_host_compute_hard_shadows(dumps, oracle_lib, gi, technique):
     from globalillumination_amd import camera
 

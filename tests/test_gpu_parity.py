@@ -254,7 +254,7 @@ def test_hard_shadow_config1(gi, oracle_lib, cuda, technique, pre_eval):
         assert bits_equal(got, want), mismatch_report(got, want, "pcf")
 
 
-@pytest.mark.parametrize("technique", ["rbsm", "rpcf", "rbsm_filtered"])
+@pytest.mark.parametrize("technique", ["rbsm", "rpcf", "rbsm_filtered", "rbsm_conservative", "rpcf_conservative"])
 def test_rbsm_soup(gi, oracle_lib, cuda, technique):
     mesh = shrink_soup(30_000)
     wl = gi.scenes.config2(ntris=10, view_w=640, view_h=360, sm=1024)

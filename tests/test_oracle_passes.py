@@ -271,3 +271,105 @@ def test_loops_that_cannot_advance_are_rejected(oracle_lib):
     d = np.ones((4, 4), np.float32)
     with pytest.raises(RuntimeError):
         run_hard(oracle_lib, d, [(0.5, 0.5, 0.5)], technique="pcf", penumbra_size=0)
+
+
+# --------------------------------------------------------------------------
+# Conservative RBSM (ConservativeSMSR.frag)
+# --------------------------------------------------------------------------
+def test_conservative_rbsm_staircase_line(oracle_lib):
+    """Hand-derived for the 45-degree staircase (umbra where i + j <= 31): a lit
+    texel on the first lit diagonal (i + j = 32) has umbra left and below; its
+    left and bottom walks hit umbra at once (+fx, +fy), its right and top walks
+    stop after one step without a discontinuity (-(1-fx), -(1-fy)); so
+    r = (fx, fy) and the texel is shadowed where fx + fy < 1.  The conservative
+    edge is the line x + y = 33 in texel units, half a texel diagonal outside
+    the non-conservative 32.5."""
+    rng = np.random.default_rng(10)
+    pts = rbsm_points(rng, 3000)
+    d = staircase()
+    got = run_hard(oracle_lib, d, pts, technique="rbsm_conservative", si=0.0, max_search=16, depth_threshold=0.0)
+    s = pts[:, 0] * RW + pts[:, 1] * RW
+    far = np.abs(s - 33.0) > 0.05
+    np.testing.assert_array_equal(got[far], (s[far] >= 33.0).astype(np.float32))
+    nonc = run_hard(oracle_lib, d, pts, technique="rbsm", si=0.0, max_search=16, depth_threshold=0.0)
+    band = (s > 32.6) & (s < 32.95)
+    assert band.sum() > 50 and (nonc[band] == 1.0).all() and (got[band] == 0.0).all()
+
+
+def test_conservative_rbsm_only_grows_the_shadow(oracle_lib):
+    """Conservative: a texel the plain test shadows stays shadowed, so the
+    result never exceeds the naive one; values are si or 1."""
+    rng = np.random.default_rng(11)
+    d = (rng.random((RW, RW)) > 0.6).astype(np.float32)
+    pts = rng.uniform(0.05, 0.95, size=(3000, 3)).astype(np.float32)
+    pts[:, 2] = 0.5
+    got = run_hard(oracle_lib, d, pts, technique="rbsm_conservative", depth_threshold=0.0)
+    naive = run_hard(oracle_lib, d, pts, technique="naive")
+    assert (got <= naive).all()
+    assert set(np.unique(got).tolist()) <= {SI, 1.0}
+    assert (got < naive).mean() > 0.01
+
+
+def test_conservative_rbsm_axis_aligned_edge_and_uniform_maps(oracle_lib):
+    """A straight edge: the walks along it never find an end, T = 0 makes that
+    axis 0 and rx * ry = 0, so the result equals the naive test; RPCF equals
+    the plain average of its taps.  Uniform maps equal the naive test."""
+    n = 64
+    d = np.ones((n, n), np.float32)
+    d[:, :32] = 0.0
+    pts = np.array([((i + f) / n, (j + 0.5) / n, 0.5) for i in range(26, 38) for f in (0.05, 0.5, 0.95)
+                    for j in (20, 33, 43)], np.float32)
+    rb = run_hard(oracle_lib, d, pts, technique="rbsm_conservative", si=0.0, depth_threshold=0.0)
+    nv = run_hard(oracle_lib, d, pts, technique="naive", si=0.0)
+    np.testing.assert_array_equal(rb, nv)
+    rp = run_hard(oracle_lib, d, pts, technique="rpcf_conservative", si=0.0, kernel_order=3, penumbra_size=1)
+    taps = [-1.0, -1.0 / 3.0, 1.0 / 3.0, 1.0]
+    want = [np.mean([d[int(math.floor(y * n + b)), int(math.floor(x * n + a))] for a in taps for b in taps])
+            for x, y, _ in pts]
+    np.testing.assert_allclose(rp, want, atol=1e-6)
+    rng = np.random.default_rng(12)
+    upts = rng.uniform(0.1, 0.9, size=(100, 3)).astype(np.float32)
+    for val in (0.0, 1.0):
+        u = np.full((RW, RW), val, np.float32)
+        for tech in ("rbsm_conservative", "rpcf_conservative"):
+            got = run_hard(oracle_lib, u, upts, technique=tech, kernel_order=3, penumbra_size=1)
+            np.testing.assert_allclose(got, np.where(upts[:, 2] <= val, 1.0, SI), atol=1e-7)
+
+
+def test_conservative_rpcf_is_the_tap_mean_on_the_staircase(oracle_lib):
+    """On the staircase every lit texel next to umbra has a left or bottom
+    umbra neighbour and never umbra on both sides, so each RPCF tap takes the
+    same branch as the SMSR test at that tap: RPCF is the mean of SMSR over the
+    4 x 4 taps (coordinates s + w / W formed in FP32, as the loop forms them)."""
+    rng = np.random.default_rng(13)
+    pts = rbsm_points(rng, 300)
+    d = staircase()
+    rp = run_hard(oracle_lib, d, pts, technique="rpcf_conservative", max_search=16, depth_threshold=0.0,
+                  kernel_order=3, penumbra_size=1)
+    incr = np.float32(1.0) / np.float32(RW)
+    step = np.float32(2.0) * np.float32(1.0) / np.float32(3.0)
+    ws = []
+    w = np.float32(-1.0)
+    while w <= np.float32(1.0):
+        ws.append(w)
+        w = np.float32(w + step)
+    assert len(ws) == 4
+    taps = np.array([(np.float32(x + a * incr), np.float32(y + b * incr), z) for x, y, z in pts for a in ws
+                     for b in ws], np.float32)
+    sm = run_hard(oracle_lib, d, taps, technique="rbsm_conservative", max_search=16, depth_threshold=0.0)
+    want = sm.reshape(len(pts), 16).sum(axis=1, dtype=np.float32) / np.float32(16.0)
+    np.testing.assert_allclose(rp, want, atol=1e-6)
+
+
+def test_conservative_rbsm_transpose_symmetry(oracle_lib):
+    rng = np.random.default_rng(14)
+    d = staircase()
+    d[5:9, 20:25] = 0.0
+    d[20:27, 3:6] = 0.0
+    pts = np.array([((rng.integers(0, RW) + (rng.integers(0, 16) + 0.5) / 16) / RW,
+                     (rng.integers(0, RW) + (rng.integers(0, 16) + 0.5) / 16) / RW, 0.5) for _ in range(2000)],
+                   np.float32)
+    a = run_hard(oracle_lib, d, pts, technique="rbsm_conservative", depth_threshold=0.0)
+    b = run_hard(oracle_lib, np.ascontiguousarray(d.T), pts[:, [1, 0, 2]], technique="rbsm_conservative",
+                 depth_threshold=0.0)
+    np.testing.assert_array_equal(a, b)
