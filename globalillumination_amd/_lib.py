@@ -45,6 +45,21 @@ class GiGBuffer(ctypes.Structure):
                 ("facing_dev", ctypes.c_void_p)]
 
 
+class GiPcssParams(ctypes.Structure):
+    _fields_ = [("light_mvp", ctypes.c_float * 16),
+                ("shadow_map_width", ctypes.c_int),
+                ("shadow_map_height", ctypes.c_int),
+                ("blocker_search_size", ctypes.c_int),
+                ("kernel_size", ctypes.c_int),
+                ("light_source_radius", ctypes.c_int),
+                ("z_near", ctypes.c_int),
+                ("shadow_intensity", ctypes.c_float),
+                ("pre_evaluation", ctypes.c_int),
+                ("mv", ctypes.c_float * 16),
+                ("normal_matrix", ctypes.c_float * 9),
+                ("light_position", ctypes.c_float * 3)]
+
+
 class GiPhongParams(ctypes.Structure):
     _fields_ = [("mv", ctypes.c_float * 16),
                 ("normal_matrix", ctypes.c_float * 9),
@@ -145,6 +160,8 @@ _SIGS = {
     "gi_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_pcss_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiPcssParams),
+                                    ctypes.c_void_p, ctypes.c_void_p]),
     "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(GiPhongParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),

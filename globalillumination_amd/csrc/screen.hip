@@ -7,7 +7,10 @@
 //   Shadow.frag:222-273 (pre-evaluation, naive, PCF),
 //   RBSM/NonConservativeSMSR.frag:23-401 (binary RBSM, RPCF),
 //   RBSM/FilteredRBSM.frag:234-402 (fractional RBSM),
-//   SoftShadow/AccurateSoftShadow.frag:52-133 (area light).
+//   RBSM/ConservativeSMSR.frag:23-235 (conservative RBSM, RPCF),
+//   SoftShadow/AccurateSoftShadow.frag:52-133 (area light),
+//   SoftShadow/PlausibleSoftShadow.frag:166-632 (PCSS),
+//   GBuffer/PhongShading.frag:11-46 (deferred shading).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -135,6 +138,46 @@ int gi_phong_pass(const gi_gbuffer* gbuffer, const float* shadow_dev, const floa
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_phong, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, gbuffer->facing_dev, shadow_dev,
                        reinterpret_cast<const float4*>(color_dev), u, reinterpret_cast<float4*>(rgba_out_dev));
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_pcss_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_pcss_params* p, float* out_dev,
+                 gi_stream stream) {
+    if (!gbuffer_ok(gbuffer) || !depth_dev || !p || !out_dev) return GI_EINVAL;
+    if (p->shadow_map_width <= 0 || p->shadow_map_height <= 0 || p->shadow_map_width > GI_MAX_EDGE ||
+        p->shadow_map_height > GI_MAX_EDGE)
+        return GI_EINVAL;
+    // A size of 1 makes filterWidth 0 and every tap offset 0/0; sizes are
+    // bounded so the per-pixel loops stay within 64 x 64 taps.
+    if (p->blocker_search_size < 2 || p->blocker_search_size > 64 || p->kernel_size < 2 || p->kernel_size > 64)
+        return GI_EINVAL;
+    if (p->light_source_radius < 0 || p->light_source_radius > 4096) return GI_EINVAL;
+    if (!finite_all(p->light_mvp, 16) || !std::isfinite(p->shadow_intensity)) return GI_ENONFINITE;
+    if (p->pre_evaluation) {
+        if (!gbuffer->normal_dev) return GI_EINVAL;
+        if (!finite_all(p->mv, 16) || !finite_all(p->normal_matrix, 9) || !finite_all(p->light_position, 3))
+            return GI_ENONFINITE;
+    }
+    HardUniforms u{};
+    for (int i = 0; i < 16; ++i) { u.light_mvp.m[i] = p->light_mvp[i]; u.mv.m[i] = p->mv[i]; }
+    for (int i = 0; i < 9; ++i) u.nm.m[i] = p->normal_matrix[i];
+    for (int i = 0; i < 3; ++i) u.lpos[i] = p->light_position[i];
+    u.si = p->shadow_intensity;
+    u.pre_eval = p->pre_evaluation ? 1 : 0;
+    PcssUniforms q;
+    // `if (shadowMapWidth <= 1024.0)` (:171-173): radius over the map width, or over 1024
+    const float R = (float)p->light_source_radius;
+    q.bsw = (p->shadow_map_width <= 1024) ? R / (float)p->shadow_map_width : R / 1024.0f;
+    q.bfw = ((float)p->blocker_search_size - 1.0f) * 0.5f;
+    q.kfw = ((float)p->kernel_size - 1.0f) * 0.5f;
+    q.radius = R;
+    q.znear = (float)p->z_near;
+    q.ks = p->kernel_size;
+    GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
+    DepthMap D{depth_dev, p->shadow_map_width, p->shadow_map_height, (float)p->shadow_map_width,
+               (float)p->shadow_map_height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_pcss, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u, q, out_dev);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

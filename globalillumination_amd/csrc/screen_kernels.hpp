@@ -465,6 +465,94 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap
     out[k] = shadow;
 }
 
+// ---------------------------------------------------------------------------
+// Percentage-closer soft shadows: PlausibleSoftShadow.frag with PCSS == 1
+// (renderSoftShadows, SoftShadowMapping/src/main.cpp:925-1016).  One depth
+// map from the light centre; a blocker search sets the PCF kernel's width.
+// ---------------------------------------------------------------------------
+struct PcssUniforms {
+    float bsw;     // blockerSearchWidth (:171-173), formed on the host in FP32
+    float bfw;     // (blockerSearchSize - 1.0) * 0.5
+    float kfw;     // (kernelSize - 1.0) * 0.5
+    float radius;  // float(lightSourceRadius)
+    float znear;   // float(zNear), an int uniform
+    int ks;        // kernelSize
+};
+
+// The loops `for (int h = -filterWidth; h <= filterWidth; h++)` start at the
+// float bound converted to int (truncation toward zero) and compare as float.
+__device__ __forceinline__ int loop_start(float fw) { return (int)(-fw); }
+
+// computeAverageBlockerDepthBasedOnPCF :166-192
+__device__ __forceinline__ float pcss_blocker(const DepthMap& D, const PcssUniforms& q, float x, float y, float z) {
+    float avg = 0.0f;
+    int n = 0;
+    const int h0 = loop_start(q.bfw);
+    for (int h = h0; (float)h <= q.bfw; ++h) {
+        const float ty = y + ((float)h * q.bsw) / q.bfw;
+        for (int w = h0; (float)w <= q.bfw; ++w) {
+            const float d = D.fetch(x + ((float)w * q.bsw) / q.bfw, ty);
+            if (z > d) {
+                avg = avg + d;
+                ++n;
+            }
+        }
+    }
+    return n == 0 ? 1.0f : avg / (float)n;
+}
+
+// computePenumbraWidth :365-374
+__device__ __forceinline__ float pcss_penumbra(const PcssUniforms& q, float avg, float z) {
+    if (avg < 0.99f) return 0.0f;
+    const float pw = ((z - avg) / avg) * q.radius;
+    return (q.znear * pw) / z;
+}
+
+// PCF :376-398
+__device__ __forceinline__ float pcss_pcf(const DepthMap& D, const PcssUniforms& q, float si, float pw, float x,
+                                          float y, float z) {
+    const float stepSize = 2.0f * pw / (float)q.ks;
+    if (stepSize <= 0.0f || stepSize >= 1.0f) return 1.0f;
+    float illum = 0.0f;
+    const int h0 = loop_start(q.kfw);
+    for (int h = h0; (float)h <= q.kfw; ++h) {
+        const float ty = y + ((float)h * pw) / q.kfw;
+        for (int w = h0; (float)w <= q.kfw; ++w) {
+            const float d = D.fetch(x + ((float)w * pw) / q.kfw, ty);
+            illum = illum + ((z <= d) ? 1.0f : si);
+        }
+    }
+    return illum / (float)(q.ks * q.ks);
+}
+
+// main :605-632, percentageCloserSoftShadows :556-563
+__global__ __launch_bounds__(kBlockThreads) void k_pcss(GBufferView gb, DepthMap D, HardUniforms u, PcssUniforms q,
+                                                         float* __restrict__ out) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t k = (size_t)y * gb.w + x;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {  // discarded: the cleared target holds 0
+        out[k] = 0.0f;
+        return;
+    }
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    float sc[4];
+    mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
+    const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+    float shadow = 1.0f;
+    if (u.pre_eval)
+        shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+    if (sc[3] > 0.0f && shadow == 1.0f) {
+        const float avg = pcss_blocker(D, q, n0, n1, n2);
+        const float pw = pcss_penumbra(q, avg, n2);
+        shadow = pcss_pcf(D, q, u.si, pw, n0, n1, n2);
+    }
+    out[k] = shadow;
+}
+
 constexpr int kMaxTaps = 16;
 
 struct SoftUniforms {

@@ -349,6 +349,41 @@ int SoftShadowPipeline::renderMonteCarlo() {
     return hip_ok(hipStreamSynchronize(stream));  // the trans upload's host buffer goes out of scope
 }
 
+int SoftShadowPipeline::renderSoftShadows() {
+    // displaySceneFromLightPOV (:274-318): the light source's own view
+    GeometryViewer light;
+    light.eye = lightEye;
+    light.look = lightAt;
+    light.up = lightUp;
+    GI_TRY(light.configureAmbient(smw_, smh_));
+    lightMVP_ = light.mvp();
+    GI_TRY(hip_ok(hipMemcpyAsync(mvps_.get(), lightMVP_.data(), 16 * sizeof(float), hipMemcpyHostToDevice,
+                                 stream)));
+    gi_raster_opts opts{4.0f, 20.0f, nullptr, nullptr};  // glPolygonOffset(4, 20), :295
+    GI_TRY(gi_depth_pass(pos_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_, mvps_.as<float>(), 1, smw_, smh_,
+                         &opts, maps_.as<float>(), ws_.get(), ws_.size(), stream));
+    gi_pcss_params p{};
+    float B[16];
+    GI_TRY(gi_bias_matrix(0.0f, B));
+    GI_TRY(gi_mat4_mul(B, lightMVP_.data(), p.light_mvp));  // configureShadow :168-170
+    p.shadow_map_width = smw_;
+    p.shadow_map_height = smh_;
+    p.blocker_search_size = blockerSearchSize;
+    p.kernel_size = kernelSize;
+    p.light_source_radius = lightSize / 2;  // main.cpp:1614 (int division)
+    p.z_near = pcssZNear;
+    p.shadow_intensity = shadowIntensity;
+    p.pre_evaluation = preEvaluation ? 1 : 0;
+    // configurePhong(lightEye rotated 180 degrees about y) (displaySceneFromGBuffer :366-367, :418)
+    std::memcpy(p.mv, cameraMV_.data(), sizeof(p.mv));
+    normal_matrix(cameraMV_, p.normal_matrix);
+    p.light_position[0] = -lightEye[0];
+    p.light_position[1] = lightEye[1];
+    p.light_position[2] = -lightEye[2];
+    gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
+    return gi_pcss_pass(&g, maps_.as<float>(), &p, soft_.as<float>(), stream);
+}
+
 std::array<float, 9> SoftShadowPipeline::normalMatrix() const {
     std::array<float, 9> nm{};
     normal_matrix(cameraMV_, nm.data());

@@ -136,6 +136,13 @@ public:
     Vec3 lightEye{}, lightAt{}, lightUp{0.0f, 0.0f, 1.0f};
     int lightSize = 16;                            // LightSource::size (int), main.cpp:1598
     int numberOfSamples = 289;                     // main.cpp:1600
+    int blockerSearchSize = 7;                     // main.cpp:1612
+    int kernelSize = 15;                           // main.cpp:1613
+    // The int uniform zNear of PlausibleSoftShadow.frag.  The reference never
+    // uploads it for that shader, so it is 0 there and PCSS degenerates to
+    // "lit" (penumbra width 0); 1 is MyGLGeometryViewer's near plane.
+    int pcssZNear = 1;
+    bool preEvaluation = true;                     // PCSS runs the pre-evaluation (:614)
     Vec3 cameraEye{}, cameraAt{}, cameraUp{0.0f, 0.0f, 1.0f};
     float cameraFov = 45.0f;
     hipStream_t stream = nullptr;
@@ -147,6 +154,9 @@ public:
     // UniformSampledLightSource placement (eye and look-at shifted together,
     // UniformSampledLightSource.cpp:13-37), the N depth maps, then the screen pass.
     int renderMonteCarlo();
+    // renderSoftShadows with ShadowParams.PCSS (main.cpp:925-1016): the depth
+    // map of the light centre (into map 0) and PlausibleSoftShadow.frag.
+    int renderSoftShadows();
     int deferredShading();  // Soft main.cpp:1109-1117: Phong over the soft-shadow buffer
     int download(std::vector<float>& softShadow, std::vector<float>* maps = nullptr,
                  std::vector<float>* rgba = nullptr);
@@ -154,12 +164,13 @@ public:
     const Mat4& cameraMVP() const { return cameraMVP_; }
     const Mat4& cameraMV() const { return cameraMV_; }
     std::array<float, 9> normalMatrix() const;
+    const Mat4& lightMVP() const { return lightMVP_; }  // the PCSS light centre's P*V
 
 private:
     int windowWidth_ = 0, windowHeight_ = 0, smw_ = 0, smh_ = 0, maxSamples_ = 0, nverts_ = 0, ntris_ = 0;
     bool ownStream_ = false;
     std::vector<Mat4> lightMVPs_;
-    Mat4 cameraMVP_{}, cameraMV_{};
+    Mat4 cameraMVP_{}, cameraMV_{}, lightMVP_{};
     DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, gfac_, soft_, rgba_, ws_;
 };
 

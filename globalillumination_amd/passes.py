@@ -230,6 +230,57 @@ def compute_hard_shadows(gbuf: GBuffer, depth: torch.Tensor, params: ShadowParam
     return out
 
 
+@dataclass
+class PCSSParams:
+    """The ShadowParams fields PlausibleSoftShadow.frag reads with PCSS == 1
+    (SoftShadowMapping/src/main.cpp:1604-1614 defaults).  `zNear` is the int
+    uniform; the reference app never uploads it for this shader, so its value
+    there is 0 (penumbra 0, every tested pixel lit); 1 gives the intended
+    PCSS with MyGLGeometryViewer's near plane."""
+    lightMVP: np.ndarray                 # unbiased P*V*M of the light centre
+    shadowMapWidth: int
+    shadowMapHeight: int
+    blockerSearchSize: int = 7
+    kernelSize: int = 15
+    lightSourceRadius: int = 8           # LightSource::size / 2
+    zNear: int = 1
+    shadowIntensity: float = 0.25
+    zbias: float = 0.0
+    preEvaluation: bool = False
+    MV: np.ndarray = field(default_factory=lambda: np.eye(4, dtype=np.float32).reshape(-1))
+    normalMatrix: np.ndarray = field(default_factory=lambda: np.eye(3, dtype=np.float32).reshape(-1))
+    lightPosition: tuple = (0.0, 0.0, 0.0)
+
+    def to_c(self) -> _lib.GiPcssParams:
+        p = _lib.GiPcssParams()
+        p.light_mvp = _f(biased(self.lightMVP, self.zbias), 16)
+        p.shadow_map_width = int(self.shadowMapWidth)
+        p.shadow_map_height = int(self.shadowMapHeight)
+        p.blocker_search_size = int(self.blockerSearchSize)
+        p.kernel_size = int(self.kernelSize)
+        p.light_source_radius = int(self.lightSourceRadius)
+        p.z_near = int(self.zNear)
+        p.shadow_intensity = float(self.shadowIntensity)
+        p.pre_evaluation = 1 if self.preEvaluation else 0
+        p.mv = _f(self.MV, 16)
+        p.normal_matrix = _f(self.normalMatrix, 9)
+        p.light_position = _f(self.lightPosition, 3)
+        return p
+
+
+def render_soft_shadows(gbuf: GBuffer, depth: torch.Tensor, params: PCSSParams, out: torch.Tensor | None = None,
+                        stream=None) -> torch.Tensor:
+    """renderSoftShadows' screen pass with PCSS (PlausibleSoftShadow.frag):
+    blocker search, penumbra width, PCF over one depth map of the light
+    centre; returns the R channel."""
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=gbuf.position.device)
+    g = gbuf.to_c(with_normals=params.preEvaluation)
+    p = params.to_c()
+    _lib.call("gi_pcss_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out), _stream_handle(stream))
+    return out
+
+
 # G-buffer colour of a mesh without per-vertex colours: GBuffer.frag:27 reads
 # gl_FrontLightModelProduct.sceneColor, the default material ambient (0.2) times
 # the light-model ambient (0.1, MyGLGeometryViewer.cpp:69), alpha 1.

@@ -205,6 +205,37 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
                         const gi_soft_params* params, float* out_dev, gi_stream stream);
 
 /* ------------------------------------------------------------------------ */
+/* Percentage-closer soft shadows (plausible soft shadows, PCSS)             */
+/* ------------------------------------------------------------------------ */
+
+/* renderSoftShadows with ShadowParams.PCSS (SoftShadowMapping/src/main.cpp:
+ * 925-1016): one depth map from the light centre (gi_depth_pass with
+ * glPolygonOffset(4, 20), main.cpp:295), then PlausibleSoftShadow.frag:
+ * blocker search (:166-192), penumbra width (:365-374) and PCF (:376-398),
+ * main :605-632. */
+typedef struct gi_pcss_params {
+    float light_mvp[16];      /* B * lightMVP of the light centre (configureShadow :168-170) */
+    int shadow_map_width, shadow_map_height;
+    int blocker_search_size;  /* 7 in the reference (main.cpp:1612); 2..64 */
+    int kernel_size;          /* 15 (main.cpp:1613); 2..64 */
+    int light_source_radius;  /* LightSource::size / 2 (main.cpp:1614); 0..4096 */
+    int z_near;               /* the int uniform zNear.  The reference never uploads it for this
+                               * program (configureLinearization :88-97 runs only for the moment,
+                               * exponential and filter shaders), so it stays 0 there: the
+                               * penumbra width is then 0 and PCF returns 1.  Pass 0 to reproduce
+                               * that, or 1 (MyGLGeometryViewer's zNear) for the intended PCSS. */
+    float shadow_intensity;
+    int pre_evaluation;       /* computePreEvaluationBasedOnNormalOrientation, needs normals */
+    float mv[16];
+    float normal_matrix[9];
+    float light_position[3];
+} gi_pcss_params;
+
+/* out_dev: width*height R channel (0 for background). */
+int gi_pcss_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_pcss_params* params, float* out_dev,
+                 gi_stream stream);
+
+/* ------------------------------------------------------------------------ */
 /* Deferred Phong shading                                                     */
 /* ------------------------------------------------------------------------ */
 

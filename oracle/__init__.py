@@ -56,6 +56,7 @@ def lib() -> ctypes.CDLL:
             "or_visibility_raster": [vp, i, vp, i, vp, i, i, vp],
             "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp, vp],
             "or_phong": [vp, vp, vp, i, i, vp, vp, vp, vp],
+            "or_pcss": [vp, vp, i, i, vp, vp, i, i, i, i, vp],
             "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
             "or_diff_soft": [vp, i, i, vp, i, i, i, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp],
             "or_set_threads": [i], "or_get_threads": [],
@@ -159,6 +160,22 @@ def hard_shadow(pos: np.ndarray, nrm: np.ndarray | None, depth: np.ndarray, para
                               ctypes.byref(params), _p(out))
     if rc:
         raise RuntimeError(f"or_hard_shadow failed ({rc})")
+    return out
+
+
+def pcss(pos, nrm, depth, params, blocker_search_size=7, kernel_size=15, light_source_radius=8, z_near=1):
+    """PlausibleSoftShadow.frag with PCSS == 1 over a G-buffer; `params` from
+    make_params (light_mvp biased, map size, shadow intensity, pre-evaluation)."""
+    pos = _c32(pos)
+    h, w = pos.shape[1], pos.shape[2]
+    d = _c32(depth)
+    assert d.shape == (params.sm_height, params.sm_width)
+    n = None if nrm is None else _c32(nrm)
+    out = np.empty((h, w), np.float32)
+    rc = lib().or_pcss(_p(pos), None if n is None else _p(n), w, h, _p(d), ctypes.byref(params),
+                       int(blocker_search_size), int(kernel_size), int(light_source_radius), int(z_near), _p(out))
+    if rc:
+        raise RuntimeError(f"or_pcss failed ({rc})")
     return out
 
 

@@ -1234,3 +1234,75 @@ int or_phong(const float* pos, const float* nrm, const float* fac, int w, int h,
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* PCSS: SoftShadowMapping/Shaders/SoftShadow/PlausibleSoftShadow.frag        */
+/* ------------------------------------------------------------------------ */
+
+/* `for (int h = -filterWidth; h <= filterWidth; h++)` with a float bound: the
+ * start is the bound converted to int (C truncation), the test is in float. */
+static float pcss_pixel(const float* D, int W, int H, const or_params* p, int bss, int ks, int radius, int znear,
+                        float x, float y, float z)
+{
+    /* computeAverageBlockerDepthBasedOnPCF :166-192 */
+    float bsw = (W <= 1024) ? (float)radius / (float)W : (float)radius / 1024.0f;
+    float fw = ((float)bss - 1.0f) * 0.5f;
+    float sum = 0.0f;
+    int blockers = 0;
+    for (int hh = (int)(-fw); (float)hh <= fw; hh++) {
+        for (int ww = (int)(-fw); (float)ww <= fw; ww++) {
+            float ox = ((float)ww * bsw) / fw, oy = ((float)hh * bsw) / fw;
+            float d = or_fetch(D, W, H, x + ox, y + oy);
+            if (z > d) { sum = sum + d; blockers++; }
+        }
+    }
+    float avg = blockers == 0 ? 1.0f : sum / (float)blockers;
+    /* computePenumbraWidth :365-374 */
+    float pw = 0.0f;
+    if (!(avg < 0.99f)) {
+        float width = ((z - avg) / avg) * (float)radius;
+        pw = ((float)znear * width) / z;
+    }
+    /* PCF :376-398 */
+    float stepSize = 2.0f * pw / (float)ks;
+    if (stepSize <= 0.0f || stepSize >= 1.0f) return 1.0f;
+    float kfw = ((float)ks - 1.0f) * 0.5f;
+    float illum = 0.0f;
+    for (int hh = (int)(-kfw); (float)hh <= kfw; hh++) {
+        for (int ww = (int)(-kfw); (float)ww <= kfw; ww++) {
+            float ox = ((float)ww * pw) / kfw, oy = ((float)hh * pw) / kfw;
+            float d = or_fetch(D, W, H, x + ox, y + oy);
+            if (z <= d) illum = illum + 1.0f;
+            else illum = illum + p->shadow_intensity;
+        }
+    }
+    return illum / (float)(ks * ks);
+}
+
+int or_pcss(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
+            int blocker_search_size, int kernel_size, int light_source_radius, int z_near, float* out)
+{
+    if (!pos || !depth || !p || !out || w <= 0 || h <= 0) return -1;
+    if (p->pre_evaluation && !nrm) return -1;
+    if (blocker_search_size < 2 || kernel_size < 2) return -1; /* filterWidth 0: taps at 0/0 */
+    size_t plane = (size_t)w * (size_t)h;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 256) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { out[k] = 0.0f; continue; } /* :609 discard */
+        float sc[4];
+        m4v(p->light_mvp, vx, vy, vz, 1.0f, sc);
+        float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        float shadow = 1.0f;
+        if (p->pre_evaluation)
+            shadow = pre_evaluation(p, vx, vy, vz, nrm[k], nrm[plane + k], nrm[2 * plane + k]);
+        if (sc[3] > 0.0f && shadow == 1.0f) /* :616-627 */
+            shadow = pcss_pixel(depth, p->sm_width, p->sm_height, p, blocker_search_size, kernel_size,
+                                light_source_radius, z_near, n0, n1, n2);
+        out[k] = shadow;
+    }
+    return 0;
+}

@@ -89,6 +89,13 @@ int or_visibility_raster(const float* pos, int nv, const int* idx, int nt,
 int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
                const float mvp[16], int w, int h, float* pos_out, float* nrm_out, float* fac_out);
 
+/* PlausibleSoftShadow.frag with PCSS == 1 (:166-192, :365-398, :556-563,
+ * main :605-632).  pos/nrm as or_hard_shadow; p->light_mvp, mv,
+ * normal_matrix, light_position, shadow_intensity, pre_evaluation,
+ * sm_width/sm_height are read from `p`. */
+int or_pcss(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
+            int blocker_search_size, int kernel_size, int light_source_radius, int z_near, float* out);
+
 /* PhongShading.frag:11-46.  pos/nrm: G-buffer planes (normal flipped on back
  * faces), fac: 1.0 front / 0.0 back, shadow: w*h R channel, color: RGBA per
  * pixel or NULL (p->color everywhere); out: RGBA per pixel. */

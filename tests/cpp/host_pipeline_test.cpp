@@ -163,6 +163,20 @@ int main(int argc, char** argv) {
     dump(dir + "/soft_camera_nm.f32", soft.normalMatrix().data(), 36);
     dumpv(dir, "soft_maps.f32", maps);
     dump(dir + "/soft_light_mvps.f32", soft.lightMVPs().data(), soft.lightMVPs().size() * 64);
+
+    // renderSoftShadows (PCSS) from a far light: window depths above the
+    // shader's 0.99 blocker cut-off
+    soft.lightEye = {60.0f, 120.0f, 60.0f};
+    soft.lightSize = 16;
+    rc = soft.renderSoftShadows();
+    if (rc) return fail("renderSoftShadows", rc);
+    std::vector<float> pcss;
+    rc = soft.download(pcss, &maps);
+    if (rc) return fail("download", rc);
+    maps.resize(static_cast<size_t>(SM) * SM);
+    dumpv(dir, "pcss.f32", pcss);
+    dumpv(dir, "pcss_map.f32", maps);
+    dump(dir + "/pcss_light_mvp.f32", soft.lightMVP().data(), 64);
     std::printf("host pipeline ok: %d vertices, %d triangles\n", mesh.nverts(), mesh.ntris());
     return 0;
 }

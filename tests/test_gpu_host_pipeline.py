@@ -114,3 +114,33 @@ def test_host_render_monte_carlo(dumps, oracle_lib, gi):
     want_rgba = oracle_lib.phong(gp, gn, fac, want, mv, nm, (-4.0, 8.0, -4.0), 0.25)
     got_rgba = load(dumps, "soft_phong.f32").reshape(H, W, 4)
     assert (got_rgba.view(np.uint32) == want_rgba.view(np.uint32)).all()
+
+
+def test_host_render_soft_shadows_pcss(dumps, oracle_lib, gi):
+    """SoftShadowPipeline::renderSoftShadows: light-centre depth map with
+    glPolygonOffset(4, 20), then PCSS (blocker search 7, kernel 15, radius
+    16/2, zNear 1) with the pre-evaluation."""
+    from globalillumination_amd import camera
+
+    pos, nrm = load(dumps, "mesh_pos.f32"), load(dumps, "mesh_nrm.f32")
+    idx = load(dumps, "mesh_idx.i32", np.int32)
+    light = load(dumps, "pcss_light_mvp.f32")
+    want_light = oracle_lib.mat4_mul(oracle_lib.perspective(45.0, 1.0, 1.0, 1000.0),
+                                     oracle_lib.lookat((60.0, 120.0, 60.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0)))
+    assert (light.view(np.uint32) == want_light.view(np.uint32)).all()
+    depth = load(dumps, "pcss_map.f32").reshape(SM, SM)
+    want_depth = oracle_lib.depth_raster(pos, idx, light, SM, SM, (4.0, 20.0))
+    assert (depth.view(np.uint32) == want_depth.view(np.uint32)).all()
+    cam = oracle_lib.mat4_mul(oracle_lib.perspective(60.0, W / H, 1.0, 1000.0),
+                              oracle_lib.lookat((0, 3, 6), (0, 0, 0), (0, 0, 1)))
+    gp, gn, _ = oracle_lib.gbuffer(pos, nrm, idx, cam, W, H)
+    mv = load(dumps, "soft_camera_mv.f32")
+    nm = load(dumps, "soft_camera_nm.f32")
+    p = oracle_lib.make_params(camera.mat4_mul(camera.bias_matrix(0.0), light), SM, SM, "naive",
+                               shadow_intensity=0.25, pre_evaluation=True, mv=mv, normal_matrix=nm,
+                               light_position=(-60.0, 120.0, -60.0))
+    want = oracle_lib.pcss(gp, gn, depth, p, blocker_search_size=7, kernel_size=15, light_source_radius=8, z_near=1)
+    got = load(dumps, "pcss.f32").reshape(H, W)
+    assert (got.view(np.uint32) == want.view(np.uint32)).all()
+    geo = gp[0] != 0
+    assert (want[geo] < 1.0).mean() > 0.01  # the sphere's shadow is there

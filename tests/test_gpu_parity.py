@@ -329,3 +329,59 @@ def test_soft_shadow_full_size_properties(gi, cuda):
         acc += part * 8
     mean = acc / 64
     assert float((mean[~bg] - fg).abs().max()) <= 1e-5
+
+
+def _pcss_scene(gi, sm):
+    """The area-light scene lit from 40x further away with a 5-degree light
+    frustum: window depths near 0.997, above PCSS's 0.99 blocker cut-off."""
+    wl = gi.scenes.config3(nsamples=1, view_w=640, view_h=360, sm=sm)
+    wl.lights = [gi.scenes.View((120.0, 280.0, 80.0), (0.0, 0.0, 0.0), sm, sm, fov=5.0)]
+    return wl
+
+
+@pytest.mark.parametrize("sm,radius,znear,pre_eval", [(1024, 8, 1, True), (2048, 8, 1, False),
+                                                      (2048, 256, 1, True), (1024, 64, 0, True)])
+def test_pcss_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval):
+    wl = _pcss_scene(gi, sm)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)
+    depth = gi.render_shadow_map(dm, mvps, sm, sm, polygon_offset=(4.0, 20.0))
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    params = gi.PCSSParams(lightMVP=mvps[0], shadowMapWidth=sm, shadowMapHeight=sm, lightSourceRadius=radius,
+                           zNear=znear, preEvaluation=pre_eval, MV=MV, normalMatrix=nm, lightPosition=lpos)
+    got = _np(gi.render_soft_shadows(g, depth[0], params))
+    op = oracle_lib.make_params(gi.biased(mvps[0]), sm, sm, "naive", shadow_intensity=0.25,
+                                pre_evaluation=pre_eval, mv=MV, normal_matrix=nm, light_position=lpos)
+    want = oracle_lib.pcss(_np(g.position), _np(g.normal) if pre_eval else None, _np(depth[0]), op,
+                           blocker_search_size=7, kernel_size=15, light_source_radius=radius, z_near=znear)
+    assert bits_equal(got, want), mismatch_report(got, want, f"pcss sm={sm} radius={radius} zNear={znear}")
+    geo = _np(g.position)[0] != 0
+    if znear == 0 and not pre_eval:
+        assert (want[geo] == 1.0).all()
+    if radius == 256:
+        pen = (want > 0.25) & (want < 1.0)
+        assert pen.mean() > 0.001  # a penumbra exists
+
+
+def test_pcss_rejects(gi, cuda):
+    import ctypes
+
+    wl = gi.scenes.config1()
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 64, 48)
+    depth = torch.ones((64, 64), device="cuda")
+    out = torch.empty((48, 64), device="cuda")
+    L = gi.lib()
+    for field, val in (("kernel_size", 1), ("blocker_search_size", 1), ("kernel_size", 65),
+                       ("light_source_radius", -1)):
+        p = gi.PCSSParams(lightMVP=light_mvps(wl)[0], shadowMapWidth=64, shadowMapHeight=64).to_c()
+        setattr(p, field, val)
+        c = g.to_c(with_normals=False)
+        assert L.gi_pcss_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
+            gi._lib.GI_EINVAL, field
+    p = gi.PCSSParams(lightMVP=light_mvps(wl)[0], shadowMapWidth=64, shadowMapHeight=64).to_c()
+    p.light_mvp[3] = float("inf")
+    c = g.to_c(with_normals=False)
+    assert L.gi_pcss_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
+        gi._lib.GI_ENONFINITE
