@@ -20,6 +20,7 @@ GI_ENONFINITE = -2
 GI_EDEVICE = -3
 GI_ENOSPACE = -4
 GI_ENOTSUP = -5
+GI_EIO = -6
 
 GI_NAIVE, GI_PCF, GI_RBSM, GI_RPCF, GI_RBSM_FILTERED, GI_RBSM_CONSERVATIVE, GI_RPCF_CONSERVATIVE = range(7)
 GI_MAX_SAMPLES = 4096
@@ -43,6 +44,13 @@ class GiGBuffer(ctypes.Structure):
                 ("width", ctypes.c_int),
                 ("height", ctypes.c_int),
                 ("facing_dev", ctypes.c_void_p)]
+
+
+class GiSceneInfo(ctypes.Structure):
+    _fields_ = [("nverts", ctypes.c_int), ("ntris", ctypes.c_int), ("has_colors", ctypes.c_int),
+                ("ntextures", ctypes.c_int), ("camera_eye", ctypes.c_float * 3), ("camera_at", ctypes.c_float * 3),
+                ("light_eye", ctypes.c_float * 3), ("light_at", ctypes.c_float * 3),
+                ("depth_threshold", ctypes.c_float), ("hsm_alpha", ctypes.c_float), ("hsm_beta", ctypes.c_float)]
 
 
 class GiPcssParams(ctypes.Structure):
@@ -160,6 +168,13 @@ _SIGS = {
     "gi_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_scene_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "gi_obj_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "gi_scene_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GiSceneInfo)]),
+    "gi_scene_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]),
+    "gi_scene_texture_path": (ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_int]),
+    "gi_scene_free": (None, [ctypes.c_void_p]),
     "gi_pcss_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiPcssParams),
                                     ctypes.c_void_p, ctypes.c_void_p]),
     "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,

@@ -22,7 +22,7 @@ BUILD = PKG / "_build"
 LIB = PKG / "libgi.so"
 
 HIP_SOURCES = ["raster.hip", "screen.hip", "diff.hip", "batch.hip"]
-CXX_SOURCES = ["gi_math.cpp"]
+CXX_SOURCES = [CSRC / "gi_math.cpp", PKG / "host" / "scene_loader.cpp"]
 
 ARCH = os.environ.get("GI_OFFLOAD_ARCH", "gfx950")
 
@@ -73,7 +73,7 @@ def build(verbose: bool = False) -> Path:
     for src in CXX_SOURCES:
         obj = BUILD / (Path(src).stem + ".cpp.o")
         objs.append(obj)
-        jobs.append(["g++", *cxx_flags(), "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append(["g++", *cxx_flags(), "-c", str(src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         list(ex.map(lambda c: _run(c, verbose), jobs))
     tmp = LIB.with_suffix(".so.tmp")

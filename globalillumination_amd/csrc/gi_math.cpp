@@ -46,6 +46,7 @@ const char* gi_status_string(int status) {
     case GI_EDEVICE: return "HIP device error";
     case GI_ENOSPACE: return "workspace too small";
     case GI_ENOTSUP: return "size not supported";
+    case GI_EIO: return "file cannot be opened or parsed";
     default: return "unknown status";
     }
 }

@@ -366,3 +366,61 @@ def config5(nsamples: int = 16, view: int = 512, sm: int = 1024, freq: int = 50)
     cam = View((0.0, 3.0, 6.0), (0.0, 1.0, 0.0), view, view, fov=60.0)
     return Workload("cfg5_diff", mesh, cam, lights, "soft",
                     extra={"offsets": offs, "side": 0.5, "center": (3.0, 7.0, 2.0), "at": (0.0, 0.0, 0.0)})
+
+
+def _load(handle, name: str, window=(1024, 1024), sm: int = 2048) -> Workload:
+    import ctypes
+
+    from . import _lib
+
+    L = _lib.lib()
+    try:
+        info = _lib.GiSceneInfo()
+        _lib.call("gi_scene_get_info", handle, ctypes.byref(info))
+        nv, nt = int(info.nverts), int(info.ntris)
+        pos = np.empty((nv, 3), np.float32)
+        nrm = np.empty((nv, 3), np.float32)
+        idx = np.empty((nt, 3), np.int32)
+        col = np.empty((nv, 3), np.float32)
+        _lib.call("gi_scene_copy", handle, pos.ctypes.data, nrm.ctypes.data, idx.ctypes.data, col.ctypes.data)
+        textures = [L.gi_scene_texture_path(handle, i).decode() for i in range(int(info.ntextures))]
+    finally:
+        L.gi_scene_free(handle)
+    cam = View(tuple(info.camera_eye), tuple(info.camera_at), int(window[0]), int(window[1]), fov=45.0,
+               up=(0.0, 0.0, 1.0))
+    light = View(tuple(info.light_eye), tuple(info.light_at), int(sm), int(sm), fov=45.0, up=(0.0, 0.0, 1.0))
+    extra = {"depth_threshold": float(info.depth_threshold), "hsm": (float(info.hsm_alpha), float(info.hsm_beta)),
+             "colors": col if info.has_colors else None, "textures": textures}
+    return Workload(name, Mesh(pos, nrm, idx), cam, [light], "hard", extra=extra)
+
+
+def load_scene(config_path, base_dir=None, window=(1024, 1024), sm: int = 2048) -> Workload:
+    """A reference scene configuration (Configs/<name>.txt, format of
+    ShadowMapping/src/IO/SceneLoader.cpp:12-91) read by libgi's loader:
+    the merged mesh (OBJ files with recomputed normals, scaled, translated
+    and rotated in file order), the camera (`ve`/`va`) and light (`le`/`la`)
+    with up (0, 0, 1) and fov 45 (ShadowMapping/src/main.cpp:863-867), the
+    depth threshold and per-vertex colours.  Relative paths resolve against
+    `base_dir` (the reference's working directory is its repository root)."""
+    import ctypes
+    from pathlib import Path
+
+    from . import _lib
+
+    h = ctypes.c_void_p()
+    base = None if base_dir is None else str(base_dir).encode()
+    _lib.call("gi_scene_load", str(config_path).encode(), base, ctypes.byref(h))
+    return _load(h, Path(str(config_path)).stem, window, sm)
+
+
+def load_obj(obj_path) -> Mesh:
+    """One OBJ file as the loader's `o` line reads it (Mesh::loadOBJFile +
+    computeNormals, ShadowMapping/src/Mesh.cpp:195-313)."""
+    import ctypes
+
+    from . import _lib
+
+    h = ctypes.c_void_p()
+    _lib.call("gi_obj_load", str(obj_path).encode(), ctypes.byref(h))
+    return _load(h, "obj").mesh
+

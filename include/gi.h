@@ -39,7 +39,8 @@ typedef enum gi_status {
     GI_ENONFINITE = -2,  /* a matrix or scalar parameter is NaN / Inf */
     GI_EDEVICE = -3,     /* a HIP call failed */
     GI_ENOSPACE = -4,    /* workspace smaller than the matching *_workspace_size() */
-    GI_ENOTSUP = -5      /* size outside what the kernels support */
+    GI_ENOTSUP = -5,     /* size outside what the kernels support */
+    GI_EIO = -6          /* a scene or OBJ file cannot be opened or parsed (gi_scene.h) */
 } gi_status;
 
 typedef void* gi_stream; /* hipStream_t */

@@ -385,3 +385,64 @@ def test_pcss_rejects(gi, cuda):
     c = g.to_c(with_normals=False)
     assert L.gi_pcss_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
         gi._lib.GI_ENONFINITE
+
+
+def _write_obj(path, mesh):
+    lines = ["v " + " ".join(repr(float(c)) for c in p) for p in mesh.positions]
+    lines += ["f " + " ".join(str(int(i) + 1) for i in t) for t in mesh.indices]
+    path.write_text("\n".join(lines) + "\n")
+
+
+def test_loaded_scene_through_the_pipeline(gi, oracle_lib, cuda, tmp_path):
+    """A scene configuration in the reference's format (OBJ files, s/t/r/c
+    lines, camera, light and depth threshold) loaded by libgi's loader, then
+    the ShadowMapping frame on it: depth map with glPolygonOffset(4, 20),
+    G-buffer, hard shadows (naive and conservative RBSM) and Phong, each
+    bit-identical to the oracle on the loaded mesh."""
+    sphere = gi.scenes.displaced_icosphere(12, seed=7)
+    _write_obj(tmp_path / "sphere.obj", sphere)
+    quad = gi.scenes.ground_quad(1.0, 1.0, 0.0)
+    _write_obj(tmp_path / "plane.obj", quad)
+    (tmp_path / "scene.txt").write_text("""o sphere.obj
+s 4.0 4.0 4.0
+r 0.0 30.0 0.0
+t 0.0 4.0 10.0
+c 0.75 0.6 0.0
++
+o plane.obj
+s 60.0 1.0 40.0
+t 0.0 -8.0 0.0
++
+ve 0.0 41.0 -50.0
+va 0.0 16.0 -10.0
+le 10.0 130.0 100.0
+la 0.0 0.0 0.0
+d 0.000025
+""")
+    wl = gi.scenes.load_scene(tmp_path / "scene.txt", base_dir=tmp_path, window=(640, 480), sm=1024)
+    assert wl.mesh.ntris == sphere.ntris + quad.ntris
+    print(f"loaded scene: {wl.mesh.nverts} vertices, {wl.mesh.ntris} triangles", flush=True)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    mvps = light_mvps(wl)
+    depth = gi.render_shadow_map(dm, mvps, 1024, 1024, polygon_offset=(4.0, 20.0))
+    want_depth = oracle_lib.depth_raster(wl.mesh.positions, wl.mesh.indices, mvps[0], 1024, 1024, (4.0, 20.0))
+    assert bits_equal(_np(depth[0]), want_depth)
+    cam = camera_mvp(wl)
+    g = gi.render_gbuffer(dm, cam, 640, 480)
+    pos, nrm, fac = oracle_lib.gbuffer(wl.mesh.positions, wl.mesh.normals, wl.mesh.indices, cam, 640, 480)
+    assert bits_equal(_np(g.position), pos) and bits_equal(_np(g.normal), nrm) and bits_equal(_np(g.facing), fac)
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    for tech in ("naive", "rbsm_conservative"):
+        params = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=1024, shadowMapHeight=1024, technique=tech,
+                                 depthThreshold=wl.extra["depth_threshold"], preEvaluation=True, MV=MV,
+                                 normalMatrix=nm, lightPosition=lpos)
+        got = gi.compute_hard_shadows(g, depth[0], params)
+        op = oracle_lib.make_params(gi.biased(mvps[0]), 1024, 1024, tech, depth_threshold=params.depthThreshold,
+                                    shadow_intensity=0.25, pre_evaluation=True, mv=MV, normal_matrix=nm,
+                                    light_position=lpos)
+        want = oracle_lib.hard_shadow(pos, nrm, want_depth, op)
+        assert bits_equal(_np(got), want), mismatch_report(_np(got), want, f"loaded scene {tech}")
+    assert (want[pos[0] != 0] == 0.25).mean() > 0.01  # the sphere shadows the plane
+    rgba = gi.shade_scene(g, got, gi.PhongParams.from_views(wl.camera, wl.lights[0].eye))
+    want_rgba = oracle_lib.phong(pos, nrm, fac, want, MV, nm, lpos, 0.25)
+    assert bits_equal(_np(rgba), want_rgba)
