@@ -20,9 +20,10 @@
 // three sums into vertex and matrix gradients.
 //
 // Forward:  visibility raster, k_diff_setup, k_diff_depth, k_diff_shade.
-// Backward: the same first three (recomputed, nothing is kept from the
-//           forward pass), then k_diff_shade_bwd (pixel adjoint: dL/d(B*M'_i)
-//           per block, dL/dd per texel by scatter-add), k_diff_texel_bwd,
+// Backward: the visibility raster and k_diff_setup again (recomputed, nothing
+//           is kept from the forward pass), then k_diff_shade_bwd (pixel
+//           adjoint, tap depths from keys and planes: dL/d(B*M'_i) per block,
+//           dL/dd per texel by scatter-add), k_diff_texel_bwd,
 //           k_diff_tri_bwd, and k_diff_reduce (fixed-order reduction of the
 //           per-block matrix partials).
 #include <hip/hip_runtime.h>
@@ -43,6 +44,7 @@ constexpr int kGatherMax = 256;    // bbox texels up to which a triangle gathers
 constexpr int kLargeSlots = 4;     // large triangles a texel block sums in LDS
 constexpr int kMapsPerThread = 1;  // maps a triangle-adjoint thread walks (vertex gradients summed over them)
 constexpr int kGatherBatch = 8;    // bounding-box texels whose keys a gather loads together
+constexpr int kBwdPixPerThread = 4;  // pixels a pixel-adjoint thread sums before the block reduction
 
 // Per-block sum of one large triangle's G * lambda_j (tri = -1: unused).
 struct LargePart {
@@ -173,6 +175,24 @@ __device__ __forceinline__ int key_tri(unsigned long long key, int ntris) {
     return t < ntris ? t : -1;
 }
 
+// Depth of texel (kx, ky) of one map from its visibility key and the plane
+// record of the triangle the key names; 1 for background.
+__device__ __forceinline__ double texel_depth(const unsigned long long* __restrict__ vk,
+                                              const TriPlane* __restrict__ pl, int ntris, uint32_t smw, uint32_t kx,
+                                              uint32_t ky) {
+    const int t = key_tri(vk[(size_t)ky * smw + kx], ntris);
+    double d = 1.0;
+    if (t >= 0) {
+        const TriPlane& p = pl[t];
+        if (p.X0 == p.X0) {
+            const double dx = ((double)kx + 0.5) - p.X0, dy = ((double)ky + 0.5) - p.Y0;
+            const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
+            d = p.Z0 + l1 * p.dZ1 + l2 * p.dZ2;
+        }
+    }
+    return d;
+}
+
 __global__ __launch_bounds__(kThreads) void k_diff_depth(const unsigned long long* __restrict__ vis,
                                                          const TriPlane* __restrict__ planes, int ntris, DiffCfg c,
                                                          double* __restrict__ depth) {
@@ -180,18 +200,9 @@ __global__ __launch_bounds__(kThreads) void k_diff_depth(const unsigned long lon
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const int map = blockIdx.y;
     if (k >= plane) return;
-    const int t = key_tri(vis[(size_t)map * plane + k], ntris);
-    double d = 1.0;
-    if (t >= 0) {
-        const TriPlane& p = planes[(size_t)map * ntris + t];
-        if (p.X0 == p.X0) {
-            const uint32_t ky = k / (uint32_t)c.smw, kx = k - ky * (uint32_t)c.smw;
-            const double dx = ((double)kx + 0.5) - p.X0, dy = ((double)ky + 0.5) - p.Y0;
-            const double l1 = p.a1 * dx + p.b1 * dy, l2 = p.a2 * dx + p.b2 * dy;
-            d = p.Z0 + l1 * p.dZ1 + l2 * p.dZ2;
-        }
-    }
-    depth[(size_t)map * plane + k] = d;
+    const uint32_t ky = k / (uint32_t)c.smw, kx = k - ky * (uint32_t)c.smw;
+    depth[(size_t)map * plane + k] =
+        texel_depth(vis + (size_t)map * plane, planes + (size_t)map * ntris, ntris, (uint32_t)c.smw, kx, ky);
 }
 
 struct Tap {
@@ -275,56 +286,65 @@ __device__ __forceinline__ void block_sum16(const double v[16], double* __restri
     __syncthreads();
 }
 
-// Pixel adjoint.  grid (pixel blocks, S).
+// Pixel adjoint.  grid (pixel blocks of kBwdPixPerThread * kThreads pixels, S):
+// each thread adds the matrix terms of kBwdPixPerThread pixels (strided by
+// the block size, so every step reads coalesced) before the one block
+// reduction.  The tap depths are evaluated from the recomputed visibility
+// keys and plane records (texel_depth, the arithmetic k_diff_depth stores in
+// the forward pass), so backward never materialises the depth maps.
 __global__ __launch_bounds__(kThreads) void k_diff_shade_bwd(const float* __restrict__ gpos,
                                                              const double* __restrict__ biased,
-                                                             const double* __restrict__ depth,
+                                                             const unsigned long long* __restrict__ vis,
+                                                             const TriPlane* __restrict__ planes, int ntris,
                                                              const float* __restrict__ grad_out, DiffCfg c,
                                                              float* __restrict__ grad_depth,
                                                              double* __restrict__ partial) {
     const size_t plane = (size_t)c.gw * c.gh;
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = blockIdx.y;
     double gm[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) gm[e] = 0.0;
-    if (k < plane && gpos[k] != 0.0f) {
-        const double x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
-        const double gq = (double)grad_out[k] / (double)c.S * (1.0 - c.si);
-        double m[16];
-        load_m(biased + 16 * i, m);
-        const Tap t = project(m, x, y, z, c);
-        const size_t msz = (size_t)c.smw * c.smh;
-        const double* d = depth + (size_t)i * msz;
-        const double su = smoothw(t.fu), sv = smoothw(t.fv);
-        const double wu[2] = {1.0 - su, su}, wv[2] = {1.0 - sv, sv};
-        const double dwu[2] = {-dsmoothw(t.fu), dsmoothw(t.fu)}, dwv[2] = {-dsmoothw(t.fv), dsmoothw(t.fv)};
-        double gz = 0.0, gfu = 0.0, gfv = 0.0;
+    double m[16];
+    load_m(biased + 16 * i, m);
+    for (int p = 0; p < kBwdPixPerThread; ++p) {
+        const size_t k = ((size_t)blockIdx.x * kBwdPixPerThread + p) * blockDim.x + threadIdx.x;
+        if (k < plane && gpos[k] != 0.0f) {
+            const double x = gpos[k], y = gpos[plane + k], z = gpos[2 * plane + k];
+            const double gq = (double)grad_out[k] / (double)c.S * (1.0 - c.si);
+            const Tap t = project(m, x, y, z, c);
+            const size_t msz = (size_t)c.smw * c.smh;
+            const unsigned long long* vk = vis + (size_t)i * msz;
+            const TriPlane* pl = planes + (size_t)i * ntris;
+            const double su = smoothw(t.fu), sv = smoothw(t.fv);
+            const double wu[2] = {1.0 - su, su}, wv[2] = {1.0 - sv, sv};
+            const double dwu[2] = {-dsmoothw(t.fu), dsmoothw(t.fu)}, dwv[2] = {-dsmoothw(t.fv), dsmoothw(t.fv)};
+            double gz = 0.0, gfu = 0.0, gfv = 0.0;
 #pragma unroll
-        for (int db = 0; db < 2; ++db)
+            for (int db = 0; db < 2; ++db)
 #pragma unroll
-            for (int da = 0; da < 2; ++da) {
-                const int a = t.a0 + da, b = t.b0 + db;
-                const bool in = a >= 0 && a < c.smw && b >= 0 && b < c.smh;
-                const double dd = in ? d[(size_t)b * c.smw + a] : 0.0;
-                const double sg = sigmoid_d((dd - t.z + c.bias) / c.tau);
-                const double ds = sg * (1.0 - sg) / c.tau;
-                const double w = wu[da] * wv[db];
-                if (in && gq != 0.0)
-                    unsafeAtomicAdd(&grad_depth[(size_t)i * msz + (size_t)b * c.smw + a], (float)(gq * w * ds));
-                gz -= w * ds;
-                gfu += dwu[da] * wv[db] * sg;
-                gfv += wu[da] * dwv[db] * sg;
-            }
-        const double gs = gq * gfu * (double)c.smw, gt = gq * gfv * (double)c.smh;
-        gz *= gq;
-        const double iw = 1.0 / t.cw;
-        const double gc[4] = {gs * iw, gt * iw, gz * iw, -(gs * t.cx + gt * t.cy + gz * t.cz) * iw * iw};
-        const double xp[4] = {x, y, z, 1.0};
+                for (int da = 0; da < 2; ++da) {
+                    const int a = t.a0 + da, b = t.b0 + db;
+                    const bool in = a >= 0 && a < c.smw && b >= 0 && b < c.smh;
+                    const double dd = in ? texel_depth(vk, pl, ntris, (uint32_t)c.smw, (uint32_t)a, (uint32_t)b) : 0.0;
+                    const double sg = sigmoid_d((dd - t.z + c.bias) / c.tau);
+                    const double ds = sg * (1.0 - sg) / c.tau;
+                    const double w = wu[da] * wv[db];
+                    if (in && gq != 0.0)
+                        unsafeAtomicAdd(&grad_depth[(size_t)i * msz + (size_t)b * c.smw + a], (float)(gq * w * ds));
+                    gz -= w * ds;
+                    gfu += dwu[da] * wv[db] * sg;
+                    gfv += wu[da] * dwv[db] * sg;
+                }
+            const double gs = gq * gfu * (double)c.smw, gt = gq * gfv * (double)c.smh;
+            gz *= gq;
+            const double iw = 1.0 / t.cw;
+            const double gc[4] = {gs * iw, gt * iw, gz * iw, -(gs * t.cx + gt * t.cy + gz * t.cz) * iw * iw};
+            const double xp[4] = {x, y, z, 1.0};
 #pragma unroll
-        for (int col = 0; col < 4; ++col)
+            for (int col = 0; col < 4; ++col)
 #pragma unroll
-            for (int row = 0; row < 4; ++row) gm[col * 4 + row] = gc[row] * xp[col];
+                for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * xp[col];
+        }
     }
     block_sum16(gm, partial + ((size_t)i * gridDim.x + blockIdx.x) * 16);
 }
@@ -657,12 +677,14 @@ struct DiffWs {
     unsigned char* touched;
     void* raster;
     size_t raster_bytes;
-    int px_blocks, tri_blocks, tx_blocks;
+    int px_blocks, pxb_blocks, tri_blocks, tx_blocks;
 };
 
 size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* ws, char* base) {
     const size_t msz = (size_t)smw * smh;
     const int px_blocks = (int)(((size_t)gw * gh + kThreads - 1) / kThreads);
+    const int pxb_blocks = (int)(((size_t)gw * gh + (size_t)kThreads * kBwdPixPerThread - 1) /
+                                 ((size_t)kThreads * kBwdPixPerThread));
     const int tri_blocks = (ntris + kThreads - 1) / kThreads;
     const int tx_blocks = (int)((msz + (size_t)kThreads * kTexPerThread - 1) / ((size_t)kThreads * kTexPerThread));
     size_t off = 0;
@@ -694,6 +716,7 @@ size_t diff_layout(int ntris, int S, int smw, int smh, int gw, int gh, DiffWs* w
         ws->raster = base + o_r;
         ws->raster_bytes = rbytes;
         ws->px_blocks = px_blocks;
+        ws->pxb_blocks = pxb_blocks;
         ws->tri_blocks = tri_blocks;
         ws->tx_blocks = tx_blocks;
     }
@@ -723,9 +746,9 @@ int check_common(const float* positions_dev, int nverts, const int32_t* indices_
     return GI_OK;
 }
 
-// Visibility raster, per-triangle planes and texel depths.
+// Visibility raster, per-triangle planes and (forward) texel depths.
 int recompute(const float* pos, int nverts, const int32_t* idx, int ntris, const double* mvps, const DiffCfg& c,
-              const DiffWs& w, hipStream_t s) {
+              const DiffWs& w, hipStream_t s, bool with_depth) {
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((16 * c.S + 255) / 256)), dim3(256), 0, s, mvps, w.mvps32, 16 * c.S);
     int rc = internal::vis_raster(pos, nverts, idx, ntris, w.mvps32, c.S, c.smw, c.smh, w.vis, w.raster,
                                   w.raster_bytes, s);
@@ -734,8 +757,9 @@ int recompute(const float* pos, int nverts, const int32_t* idx, int ntris, const
         hipLaunchKernelGGL(k_diff_setup, dim3((unsigned)w.tri_blocks, c.S), dim3(kThreads), 0, s, pos, idx, ntris,
                            mvps, c, w.planes, w.large);
     const size_t msz = (size_t)c.smw * c.smh;
-    hipLaunchKernelGGL(k_diff_depth, dim3((unsigned)((msz + kThreads - 1) / kThreads), c.S), dim3(kThreads), 0, s,
-                       w.vis, w.planes, ntris, c, w.depth);
+    if (with_depth)
+        hipLaunchKernelGGL(k_diff_depth, dim3((unsigned)((msz + kThreads - 1) / kThreads), c.S), dim3(kThreads), 0, s,
+                           w.vis, w.planes, ntris, c, w.depth);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
@@ -767,7 +791,7 @@ int gi_diff_soft_forward(const float* positions_dev, int nverts, const int32_t* 
     if (need == 0) return GI_ENOTSUP;
     if (workspace_bytes < need) return GI_ENOSPACE;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    rc = recompute(positions_dev, nverts, indices_dev, ntris, mvps_dev, c, w, s);
+    rc = recompute(positions_dev, nverts, indices_dev, ntris, mvps_dev, c, w, s, true);
     if (rc != GI_OK) return rc;
     hipLaunchKernelGGL(k_diff_shade, dim3(w.px_blocks), dim3(kThreads), 0, s, gbuffer->position_dev, biased_mvps_dev,
                        w.depth, c, out_dev);
@@ -789,14 +813,15 @@ int gi_diff_soft_backward(const float* positions_dev, int nverts, const int32_t*
     if (need == 0) return GI_ENOTSUP;
     if (workspace_bytes < need) return GI_ENOSPACE;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // the forward intermediates are recomputed, not stored
-    rc = recompute(positions_dev, nverts, indices_dev, ntris, mvps_dev, c, w, s);
+    // the forward intermediates are recomputed, not stored; the pixel adjoint
+    // evaluates the texel depths it reads from the keys and planes
+    rc = recompute(positions_dev, nverts, indices_dev, ntris, mvps_dev, c, w, s, false);
     if (rc != GI_OK) return rc;
     const size_t msz = (size_t)smw * smh * nsamples;
     hipLaunchKernelGGL(k_fill_f32, dim3(2048), dim3(kThreads), 0, s, w.grad_depth, msz, 0.0f);
-    hipLaunchKernelGGL(k_diff_shade_bwd, dim3(w.px_blocks, nsamples), dim3(kThreads), 0, s, gbuffer->position_dev,
-                       biased_mvps_dev, w.depth, grad_out_dev, c, w.grad_depth, w.part_px);
-    hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_px, w.px_blocks,
+    hipLaunchKernelGGL(k_diff_shade_bwd, dim3(w.pxb_blocks, nsamples), dim3(kThreads), 0, s, gbuffer->position_dev,
+                       biased_mvps_dev, w.vis, w.planes, ntris, grad_out_dev, c, w.grad_depth, w.part_px);
+    hipLaunchKernelGGL(k_diff_reduce, dim3(nsamples), dim3(kThreads), 0, s, w.part_px, w.pxb_blocks,
                        grad_biased_mvps_dev);
     if (ntris > 0) {
         hipLaunchKernelGGL(k_fill_f64, dim3(256), dim3(kThreads), 0, s, w.wsum, (size_t)3 * ntris * nsamples, 0.0);

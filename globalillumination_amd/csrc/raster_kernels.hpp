@@ -655,6 +655,26 @@ __device__ __forceinline__ void plot(const Setup& s, const E e[3], int i, int j,
     store_min<kVis>(s, plane_bits(s, i * 256 + 128 - s.X[0], j * 256 + 128 - s.Y[0]), slot, tile);
 }
 
+// True when every pixel centre of [rx0, rx1] x [ry0, ry1] passes plot()'s
+// three edge tests.  Each edge value is affine in the pixel indices, so its
+// minimum over the rectangle is at a corner, and the test `e + tl > 0` holds
+// everywhere once it holds at the four corners.
+__device__ __forceinline__ bool rect_inside(const Setup& s, int rx0, int rx1, int ry0, int ry1) {
+    const long long cx[2] = {(long long)rx0 * 256 + 128, (long long)rx1 * 256 + 128};
+    const long long cy[2] = {(long long)ry0 * 256 + 128, (long long)ry1 * 256 + 128};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int a = (k + 1) % 3, b = (k + 2) % 3;
+        const long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
+        const long long tl = (s.tl >> k) & 1;
+        // the corner minimising ex * Py - ey * Px
+        const long long px = ey > 0 ? cx[1] : cx[0];
+        const long long py = ex > 0 ? cy[0] : cy[1];
+        if (ex * (py - s.Y[a]) - ey * (px - s.X[a]) + tl <= 0) return false;
+    }
+    return true;
+}
+
 // One thread covers the whole (small) rectangle, row by row.  When every
 // edge value inside the triangle's bounding box fits in 32 bits, the exact
 // stepping runs in int32.
@@ -706,6 +726,13 @@ __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0
     int j = y0 + tid / TS;
     if (j < ry0) j += ((ry0 - j + kRowStep - 1) / kRowStep) * kRowStep;
     if (j > ry1) return;
+    if (rect_inside(s, rx0, rx1, ry0, ry1)) {
+        // every pixel centre of the rectangle passes the three edge tests
+        for (; j <= ry1; j += kRowStep)
+            store_min<kVis>(s, plane_bits(s, i * 256 + 128 - s.X[0], j * 256 + 128 - s.Y[0]),
+                            (j - y0) * TS + (i - x0), tile);
+        return;
+    }
     Edges E;
     edges_at(s, i, j, E);
     long long e[3] = {E.e[0], E.e[1], E.e[2]};
