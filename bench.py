@@ -1,11 +1,12 @@
 """Shadow-pipeline benchmark: shaded Mpix/s of the 1080p, 64-sample area-light
 frame (64 light-view depth maps of 2048^2 + the soft-shadow screen pass with
-3x3 PCF per sample) on one MI355X.
+3x3 PCF per sample + deferred Phong shading) on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cfg3]
 
-Prints one JSON line.  A step is one whole frame: every depth map is
-rasterized and the screen pass runs, with the mesh, G-buffer and light
+Prints one JSON line.  A step is one whole frame (renderMonteCarlo then
+deferredShading): every depth map is rasterized, the soft-shadow screen pass
+runs and the frame is Phong-shaded, with the mesh, G-buffer and light
 matrices already resident in HBM.
 """
 from __future__ import annotations
@@ -96,9 +97,10 @@ def tap_offsets(kernel_order, penumbra, inclusive):
     return out
 
 
-def cpu_baseline(wl, mvps, g_pos, maps_np, smw, samples, gi):
+def cpu_baseline(wl, mvps, g_pos, g_nrm, g_fac, phong, smw, samples, gi):
     """The oracle (FP32 C port of the reference passes, OpenMP) on a bounded
-    sample of the frame: `samples` light samples, full 1080p view."""
+    sample of the frame: `samples` light samples, full 1080p view, plus the
+    Phong pass once (it runs once per frame)."""
     import numpy as np
 
     import oracle
@@ -114,12 +116,18 @@ def cpu_baseline(wl, mvps, g_pos, maps_np, smw, samples, gi):
     oracle.soft_shadow(g_pos, maps, common, trans, si=0.25, pcf=True, kernel_order=3, penumbra=1.0,
                        inclusive=False)
     t_screen = time.perf_counter() - t0
-    per_frame = (t_depth + t_screen) * (len(mvps) / samples)
+    shadow = np.ones(g_pos.shape[1:], np.float32)
+    t0 = time.perf_counter()
+    oracle.phong(g_pos, g_nrm, g_fac, shadow, phong.MV, phong.normalMatrix, phong.lightPosition,
+                 phong.shadowIntensity, color=phong.color, clear_color=phong.clearColor)
+    t_phong = time.perf_counter() - t0
+    per_frame = (t_depth + t_screen) * (len(mvps) / samples) + t_phong
     npx = g_pos.shape[1] * g_pos.shape[2]
     return {"value": npx / per_frame / 1e6, "unit": "Mpix/s", "cores": cores, "kind": "port",
             "sample": f"{samples} of {len(mvps)} light samples (depth maps + screen pass) on the full "
-                      f"{g_pos.shape[2]}x{g_pos.shape[1]} view, scaled x{len(mvps) // samples}",
-            "seconds": round(t_depth + t_screen, 3)}
+                      f"{g_pos.shape[2]}x{g_pos.shape[1]} view, scaled x{len(mvps) // samples}, plus one "
+                      f"Phong pass",
+            "seconds": round(t_depth + t_screen + t_phong, 3)}
 
 
 def diff_timing(gi, torch, steps, warmup):
@@ -259,6 +267,9 @@ def main():
     common, trans = gi.soft_uniforms(mvps, wl.zbias)
     trans_dev = torch.from_numpy(trans).to(dev)
     mvps_dev = torch.from_numpy(np.ascontiguousarray(mvps, dtype=np.float32)).to(dev)
+    # deferredShading: Phong with the centre light rotated 180 degrees about y
+    phong = gi.PhongParams.from_views(wl.camera, wl.extra["center"], 0.25)
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
 
     def frame(ev=None):
@@ -270,12 +281,15 @@ def main():
         passes.render_monte_carlo(g, maps, mvps, params, out=out, trans_dev=trans_dev)
         if ev:
             ev[2].record(stream)
+        passes.shade_scene(g, out, phong, out=rgba)
+        if ev:
+            ev[5].record(stream)
 
     for _ in range(args.warmup):
         frame()
     torch.cuda.synchronize()
-    # events: frame start, depth done, screen done, raster kernel begin / end
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(args.steps)]
+    # events: frame start, depth done, screen done, raster kernel begin / end, shading done
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(args.steps)]
     for e in evs:
         for x in e:
             x.record(stream)  # creates the underlying hipEvent
@@ -289,6 +303,7 @@ def main():
     depth_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     screen_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
     raster_ms = float(np.mean([e[3].elapsed_time(e[4]) for e in evs]))
+    phong_ms = float(np.mean([e[2].elapsed_time(e[5]) for e in evs]))
     value = W * H / (ms_step * 1e-3) / 1e6
 
     # Algorithmic bytes per launch.  Screen pass (k_soft): G-buffer position
@@ -317,19 +332,22 @@ def main():
                             "frac": round(screen_gbs / HBM_PEAK_GBS, 4), "bytes": int(screen_bytes),
                             "unique_texels": int(texels), "traffic": measured_traffic("k_soft")},
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
-                           "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)}}
+                           "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)},
+            "phong_pass": {"kernel": "k_phong", "ms": round(phong_ms, 4), "bytes": int(npx * 48),
+                           "GB/s": round(npx * 48 / (phong_ms * 1e-3) / 1e9, 1)}}
     diff = None if args.no_diff else diff_timing(gi, torch, args.diff_steps, 2)
     batched = None if args.no_batch else batch_timing(gi, torch, args.batch_steps)
     cpu = None
     if not args.no_cpu_baseline:
-        maps_np = None
-        cpu = cpu_baseline(wl, mvps, g.position.cpu().numpy(), maps_np, smw, args.cpu_samples, gi)
+        cpu = cpu_baseline(wl, mvps, g.position.cpu().numpy(), g.normal.cpu().numpy(), g.facing.cpu().numpy(),
+                           phong, smw, args.cpu_samples, gi)
     line = {"metric": "shaded Mpix/s (1080p, 64-sample soft shadow)", "value": round(value, 2),
             "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(ms_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic",
             "config": {"workload": "area-light soft shadows: 64 samples, 2048^2 FP32 depth maps, 1920x1080 "
-                                   "view, 3x3 PCF per sample, 100,822-triangle displaced icosphere + ground",
+                                   "view, 3x3 PCF per sample, deferred Phong, 100,822-triangle displaced "
+                                   "icosphere + ground",
                        "global_batch": 1, "samples": S, "shadow_map": smw, "view": [W, H],
                        "triangles": wl.mesh.ntris},
             "roofline": roof, "cpu_baseline": cpu, "differentiable": diff, "batched": batched}

@@ -327,7 +327,8 @@ def config3(nsamples: int = 64, view_w: int = 1920, view_h: int = 1080, sm: int 
     lights = area_light_samples((3.0, 7.0, 2.0), (0.0, 0.0, 0.0), 0.5, offs, sm, sm)
     cam = View((0.0, 3.0, 6.0), (0.0, 1.0, 0.0), view_w, view_h, fov=60.0)
     return Workload("cfg3_soft64", mesh, cam, lights, "soft", zbias=0.0,
-                    extra={"offsets": offs, "side": 0.5, "kernel_order": 3, "penumbra": 1.0})
+                    extra={"offsets": offs, "side": 0.5, "kernel_order": 3, "penumbra": 1.0,
+                           "center": (3.0, 7.0, 2.0), "at": (0.0, 0.0, 0.0)})
 
 
 def config4(n: int = 32, seed: int = 4, map_sizes=(512, 1024, 2048, 4096),
