@@ -175,6 +175,12 @@ _SIGS = {
                                      ctypes.c_void_p]),
     "gi_scene_texture_path": (ctypes.c_char_p, [ctypes.c_void_p, ctypes.c_int]),
     "gi_scene_free": (None, [ctypes.c_void_p]),
+    "gi_quadtree_node_test": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float),
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_adaptive_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
+                                                    ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_pcss_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiPcssParams),
                                     ctypes.c_void_p, ctypes.c_void_p]),
     "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,

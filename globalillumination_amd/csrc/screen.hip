@@ -115,6 +115,49 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
+int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
+                                 const float light_mvp[16], const float* trans_dev, const gi_soft_params* p,
+                                 float* out_dev, gi_stream stream) {
+    SoftUniforms u;
+    int rc = soft_setup(gbuffer, maps_dev, nsamples, smw, smh, light_mvp, trans_dev, p, out_dev, u);
+    if (rc != GI_OK) return rc;
+    GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const float4* tr = reinterpret_cast<const float4*>(trans_dev);
+    dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
+    switch (soft_variant(u)) {
+    case 3: hipLaunchKernelGGL((k_soft<3, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    case 4: hipLaunchKernelGGL((k_soft<4, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    default: hipLaunchKernelGGL((k_soft<0, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    }
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
+                          const int map_index[4], const float biased_mvps[64], int div, float* image_dev,
+                          unsigned* count_dev, gi_stream stream) {
+    if (!gbuffer_ok(gbuffer) || !maps_dev || !map_index || !biased_mvps || !image_dev || !count_dev) return GI_EINVAL;
+    if (nmaps <= 0 || smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE) return GI_EINVAL;
+    if (div < 1 || div > gbuffer->width || div > gbuffer->height) return GI_EINVAL;
+    for (int i = 0; i < 4; ++i)
+        if (map_index[i] < 0 || map_index[i] >= nmaps) return GI_EINVAL;
+    if (!finite_all(biased_mvps, 64)) return GI_ENONFINITE;
+    QtUniforms u;
+    for (int i = 0; i < 4; ++i) {
+        for (int e = 0; e < 16; ++e) u.m[i].m[e] = biased_mvps[16 * i + e];
+        u.map[i] = map_index[i];
+    }
+    GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_qt_reproject, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, maps_dev, smw, smh, u,
+                       image_dev, count_dev);
+    // glViewport(0, 0, windowWidth/div, windowHeight/div) (main.cpp:559)
+    const int wd = gb.w / div, hd = gb.h / div;
+    const unsigned blocks = (unsigned)(((size_t)wd * hd + 255) / 256);
+    hipLaunchKernelGGL(k_qt_evaluate, dim3(blocks), dim3(256), 0, s, image_dev, gb.w, gb.h, wd, hd, count_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
 int gi_phong_pass(const gi_gbuffer* gbuffer, const float* shadow_dev, const float* color_dev,
                   const gi_phong_params* p, float* rgba_out_dev, gi_stream stream) {
     if (!gbuffer_ok(gbuffer) || !gbuffer->normal_dev || !gbuffer->facing_dev || !shadow_dev || !p || !rgba_out_dev)

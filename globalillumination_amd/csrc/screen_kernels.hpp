@@ -647,8 +647,21 @@ __device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms
     }
 }
 
+// adaptiveSampling == 1 (AccurateSoftShadow.frag:79-87): trans.w carries the
+// sample's quad-tree level as level * 10000; the sample weighs
+// 1 / (2^level + 1)^2 (level clamped to 4) and projects with the remainder.
+__device__ __forceinline__ float adaptive_weight(float4& T) {
+    int level = (int)(T.w / 10000.0f);
+    if (level > 4) level = 4;
+    T.w = T.w - (float)(level * 10000);
+    float p = 1.0f;  // myPow(2.0, level)
+    for (int e = 1; e <= level; ++e) p = p * 2.0f;
+    const float f = p + 1.0f;
+    return 1.0f / (f * f);
+}
+
 // AccurateSoftShadow.frag:52-133 for pixel (x, y): samples in order 0..N-1.
-template <int NT>
+template <int NT, bool kAdaptive = false>
 __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* __restrict__ maps, int smw, int smh,
                                            const float4* __restrict__ trans, const SoftUniforms& u,
                                            float* __restrict__ out, int x, int y) {
@@ -676,22 +689,110 @@ __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* _
 #pragma unroll 4
     for (int i = 0; i < u.nsamples; ++i) {
         D.texels = maps + (size_t)i * mapsz;
-        acc = acc + soft_term<NT>(D, u, cm, trans[i], incrWidth, incrHeight);
-        count = count + 1.0f;
+        if constexpr (kAdaptive) {
+            float4 T = trans[i];
+            const float af = adaptive_weight(T);
+            acc = acc + soft_term<NT>(D, u, cm, T, incrWidth, incrHeight) * af;
+            count = count + af;
+        } else {
+            acc = acc + soft_term<NT>(D, u, cm, trans[i], incrWidth, incrHeight);
+            count = count + 1.0f;
+        }
     }
     out[k] = acc / count;
 }
 
 // One thread per pixel.  Blocks are ordered XCD-aware (pixel_of_block) so each
 // XCD's L2 serves one contiguous band of the view.
-template <int NT>
+template <int NT, bool kAdaptive = false>
 __global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
                                                             int smh, const float4* __restrict__ trans,
                                                             SoftUniforms u, float* __restrict__ out) {
     int x, y;
     pixel_of_block(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
-    soft_pixel<NT>(gb, maps, smw, smh, trans, u, out, x, y);
+    soft_pixel<NT, kAdaptive>(gb, maps, smw, smh, trans, u, out, x, y);
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive light sampling, one quad-tree node's test (evaluateSubAreaLightSource,
+// SoftShadowMapping/src/main.cpp:440-571).
+// ---------------------------------------------------------------------------
+struct QtUniforms {
+    Mat4 m[4];   // B * lightMVP of the node's four corner samples
+    int map[4];  // their slots in the map array (shadowMapIndices)
+};
+
+constexpr float kQtBackground = 0.63f;  // the reprojection target's clear colour R (main.cpp:511)
+
+// QuadTree/Reprojection.frag:8-24: how many of the four corner maps light
+// the pixel (0..4); background pixels keep the clear colour.  Block (0, 0)
+// also clears the node's pass counter for k_qt_evaluate.
+__global__ __launch_bounds__(kBlockThreads) void k_qt_reproject(GBufferView gb, const float* __restrict__ maps,
+                                                                 int smw, int smh, QtUniforms u,
+                                                                 float* __restrict__ img,
+                                                                 unsigned* __restrict__ count) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *count = 0u;
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t k = (size_t)y * gb.w + x;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {
+        img[k] = kQtBackground;
+        return;
+    }
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    const size_t mapsz = (size_t)smw * smh;
+    float sh[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float sc[4];
+        mat4_vec4(u.m[i].m, vx, vy, vz, 1.0f, sc);
+        const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        DepthMap D{maps + (size_t)u.map[i] * mapsz, smw, smh, (float)smw, (float)smh};
+        sh[i] = (n2 <= D.fetch(n0, n1)) ? 1.0f : 0.0f;
+    }
+    img[k] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+// Texel of the reprojection image that GL_NEAREST samples for reduced-grid
+// coordinate c (texture coordinate (c + 0.5) / nd over an n-texel axis),
+// or -1 outside [0, n) (CLAMP_TO_BORDER reads 0 there).  Exact arithmetic:
+// floor((c + 0.5) * n / nd).
+__device__ __forceinline__ int qt_texel(int c, int n, int nd) {
+    if (c < 0) return -1;
+    const long long a = ((2ll * c + 1) * n) / (2ll * nd);
+    return a < n ? (int)a : -1;
+}
+
+// QuadTree/Evaluation.frag:6-22 over the (W/div) x (H/div) viewport: a
+// penumbra pixel (1..3 lit corners) whose 3x3 neighbourhood all agree passes;
+// passes are counted on the device (the reference's GL_SAMPLES_PASSED query).
+__global__ __launch_bounds__(256) void k_qt_evaluate(const float* __restrict__ img, int w, int h, int wd, int hd,
+                                                     unsigned* __restrict__ count) {
+    __shared__ unsigned block_count;
+    if (threadIdx.x == 0) block_count = 0u;
+    __syncthreads();
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    bool pass = false;
+    if (id < wd * hd) {
+        const int i = id % wd, j = id / wd;
+        auto tex = [&](int ci, int cj) -> float {
+            const int a = qt_texel(ci, w, wd), b = qt_texel(cj, h, hd);
+            return (a < 0 || b < 0) ? 0.0f : img[(size_t)b * w + a];
+        };
+        const float sum = tex(i, j);
+        pass = !(sum == kQtBackground || sum == 0.0f || sum == 4.0f || sum == 5.0f);
+        for (int dy = -1; dy <= 1 && pass; ++dy)
+            for (int dx = -1; dx <= 1 && pass; ++dx)
+                if (sum != tex(i + dx, j + dy)) pass = false;
+    }
+    const unsigned long long b = __ballot(pass);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&block_count, (unsigned)__popcll(b));
+    __syncthreads();
+    if (threadIdx.x == 0 && block_count) atomicAdd(count, block_count);
 }
 
 // ---------------------------------------------------------------------------

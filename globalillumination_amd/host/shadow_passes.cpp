@@ -2,6 +2,7 @@
 #include "shadow_passes.hpp"
 
 #include <cmath>
+#include <memory>
 #include <cstring>
 
 namespace gi::host {
@@ -296,6 +297,8 @@ int SoftShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight
     GI_TRY(gfac_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(soft_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(rgba_.allocate(bytes_of(npx, 4 * sizeof(float))));
+    GI_TRY(qtImg_.allocate(bytes_of(npx, sizeof(float))));
+    GI_TRY(qtCount_.allocate(256));
     size_t ws = gi_depth_pass_workspace_size(ntris_, maxSamples, smw_, smh_);
     size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
     if (ws == 0 || wg == 0) return GI_ENOTSUP;
@@ -382,6 +385,190 @@ int SoftShadowPipeline::renderSoftShadows() {
     p.light_position[2] = -lightEye[2];
     gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
     return gi_pcss_pass(&g, maps_.as<float>(), &p, soft_.as<float>(), stream);
+}
+
+// ---------------------------------------------------------------------------
+// Adaptive light sampling (renderAdaptiveLightSourceSampling, main.cpp:813-892)
+// ---------------------------------------------------------------------------
+
+// QuadTreeLightSource (src/Scene/LightSource/QuadTreeLightSource.cpp): a node
+// holds the centre ("reference sample") of its square of the light; its four
+// corner samples and its children's centres are offsets of that centre.
+struct SoftShadowPipeline::QuadNode {
+    Vec3 refEye{}, refAt{};
+    int level = 0;
+    bool hasChildren = false;
+    std::unique_ptr<QuadNode> nodes[4];
+
+    // computeUniformSamples (levelShift 1) / computeReferenceSamples (levelShift 2)
+    static Vec3 offset(Vec3 v, int level, int size, int levelShift, int index) {
+        const int div = static_cast<int>(std::pow(2.0f, static_cast<float>(level + levelShift)));
+        const float step = static_cast<float>(size) / static_cast<float>(div);
+        v[0] = v[0] + (-step + (2.0f * step) * static_cast<float>(index % 2));
+        v[1] = v[1] + (-step + (2.0f * step) * static_cast<float>(index / 2));
+        return v;
+    }
+    void subdivide(int size) {  // QuadTreeLightSource::subdivide
+        for (int n = 0; n < 4; ++n) {
+            nodes[n] = std::make_unique<QuadNode>();
+            nodes[n]->refEye = offset(refEye, level, size, 2, n);
+            nodes[n]->refAt = offset(refAt, level, size, 2, n);
+            nodes[n]->level = level + 1;
+        }
+        hasChildren = true;
+    }
+};
+
+namespace {
+
+// computeChildIndices (main.cpp:426-438): node indices are row-major in the
+// level's 2^level x 2^level grid.
+void child_indices(int level, int nodeIndex, int child[4]) {
+    const int div = 1 << level;
+    const int x = nodeIndex % div, y = nodeIndex / div;
+    const int base = 2 * x + (1 << (level + 2)) * y;
+    child[0] = base;
+    child[1] = base + 1;
+    child[2] = base + (1 << (level + 1));
+    child[3] = base + (1 << (level + 1)) + 1;
+}
+
+// The node's corner `index` on the 17 x 17 grid of samples (main.cpp:447-448).
+void corner(int level, int nodeIndex, int index, int& x, int& y) {
+    const int childFactor = 16 >> level, parentFactor = 1 << level;
+    x = (index % 2) * childFactor + (nodeIndex % parentFactor) * childFactor;
+    y = (index / 2) * childFactor + (nodeIndex / parentFactor) * childFactor;
+}
+
+}  // namespace
+
+// displaySceneFromLightPOV for one light sample, into the next map slot.
+int SoftShadowPipeline::renderLightSample(const Vec3& eye, const Vec3& at) {
+    if (adaptiveSamples_ >= maxSamples_) return GI_ENOSPACE;
+    GeometryViewer light;
+    light.eye = eye;
+    light.look = at;
+    light.up = lightUp;
+    GI_TRY(light.configureAmbient(smw_, smh_));
+    lightMVP_ = light.mvp();
+    const int n = adaptiveSamples_;
+    lightMVPs_.push_back(lightMVP_);
+    GI_TRY(hip_ok(hipMemcpyAsync(mvps_.as<float>() + 16 * (size_t)n, lightMVP_.data(), 16 * sizeof(float),
+                                 hipMemcpyHostToDevice, stream)));
+    gi_raster_opts opts{4.0f, 20.0f, nullptr, nullptr};  // glPolygonOffset(4, 20), :295
+    GI_TRY(gi_depth_pass(pos_.as<float>(), nverts_, idx_.as<int32_t>(), ntris_, mvps_.as<float>() + 16 * (size_t)n,
+                         1, smw_, smh_, &opts, maps_.as<float>() + (size_t)n * smw_ * smh_, ws_.get(), ws_.size(),
+                         stream));
+    accFactor_.push_back(0.0f);
+    ++adaptiveSamples_;
+    return GI_OK;
+}
+
+// evaluateSubAreaLightSource (main.cpp:440-571) without the revectorization branch.
+int SoftShadowPipeline::evaluateSubAreaLightSource(const QuadNode& node, int nodeIndex, bool& subdivide) {
+    int local[4];
+    for (int index = 0; index < 4; ++index) {
+        int x, y;
+        corner(node.level, nodeIndex, index, x, y);
+        if (qtRendered_[x][y]) {
+            local[index] = qtHash_[x][y];
+            continue;
+        }
+        qtRendered_[x][y] = true;
+        qtHash_[x][y] = adaptiveSamples_;
+        local[index] = adaptiveSamples_;
+        GI_TRY(renderLightSample(QuadNode::offset(node.refEye, node.level, lightSize, 1, index),
+                                 QuadNode::offset(node.refAt, node.level, lightSize, 1, index)));
+    }
+    // configureShadow with quadTreeEvaluation: B * lightMVPs[hash] (MyGLGeometryViewer.cpp:173-183)
+    float B[16], biased[64];
+    GI_TRY(gi_bias_matrix(0.0f, B));
+    for (int i = 0; i < 4; ++i) GI_TRY(gi_mat4_mul(B, lightMVPs_[local[i]].data(), biased + 16 * i));
+    gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_, nullptr};
+    const int div = adaptiveSamplingLowerAccuracy ? 4 : 1;  // main.cpp:554-557
+    GI_TRY(gi_quadtree_node_test(&g, maps_.as<float>(), adaptiveSamples_, smw_, smh_, local, biased, div,
+                                 qtImg_.as<float>(), qtCount_.as<unsigned>(), stream));
+    unsigned count = 0;
+    GI_TRY(hip_ok(hipMemcpyAsync(&count, qtCount_.get(), sizeof(count), hipMemcpyDeviceToHost, stream)));
+    GI_TRY(hip_ok(hipStreamSynchronize(stream)));  // the occlusion query's GL_QUERY_RESULT
+    nodeTests_.push_back({nodeIndex, node.level, count});
+    subdivide = count > 0;
+    return GI_OK;
+}
+
+// evaluateAreaLightSource (main.cpp:573-620), refinement only.
+int SoftShadowPipeline::evaluateAreaLightSource(QuadNode& node, int nodeIndex) {
+    if (node.level == maxLevel) return GI_OK;
+    int child[4];
+    child_indices(node.level, nodeIndex, child);
+    bool subdivide = false;
+    GI_TRY(evaluateSubAreaLightSource(node, nodeIndex, subdivide));
+    if (subdivide) {
+        node.subdivide(lightSize);
+        for (int c = 0; c < 4; ++c) GI_TRY(evaluateAreaLightSource(*node.nodes[c], child[c]));
+    }
+    return GI_OK;
+}
+
+// renderAreaLightSource / renderSubAreaLightSource (main.cpp:622-694): each
+// leaf's corners; a map already rendered takes the leaf's level.
+int SoftShadowPipeline::renderAreaLightSource(const QuadNode& node, int nodeIndex) {
+    if (node.hasChildren) {
+        int child[4];
+        child_indices(node.level, nodeIndex, child);
+        for (int c = 0; c < 4; ++c) GI_TRY(renderAreaLightSource(*node.nodes[c], child[c]));
+        return GI_OK;
+    }
+    for (int sample = 0; sample < 4; ++sample) {
+        int x, y;
+        corner(node.level, nodeIndex, sample, x, y);
+        if (qtRendered_[x][y]) {
+            accFactor_[(size_t)qtHash_[x][y]] = static_cast<float>(node.level);
+            continue;
+        }
+        qtRendered_[x][y] = true;
+        qtHash_[x][y] = adaptiveSamples_;
+        GI_TRY(renderLightSample(QuadNode::offset(node.refEye, node.level, lightSize, 1, sample),
+                                 QuadNode::offset(node.refAt, node.level, lightSize, 1, sample)));
+        accFactor_.back() = static_cast<float>(node.level);
+    }
+    return GI_OK;
+}
+
+int SoftShadowPipeline::renderAdaptiveLightSourceSampling() {
+    if (maxLevel < 0 || maxLevel > 4) return GI_EINVAL;  // the 17 x 17 sample grid
+    for (auto& row : qtRendered_)
+        for (bool& r : row) r = false;
+    for (auto& row : qtHash_)
+        for (int& h : row) h = -1;
+    lightMVPs_.clear();
+    nodeTests_.clear();
+    accFactor_.clear();
+    adaptiveSamples_ = 0;
+    QuadNode root;  // buildFirstLevel: level 0, centred on the light
+    root.refEye = lightEye;
+    root.refAt = lightAt;
+    GI_TRY(evaluateAreaLightSource(root, root.level));
+    GI_TRY(renderAreaLightSource(root, root.level));
+    const int n = adaptiveSamples_;
+    // configureShadow (MyGLGeometryViewer.cpp:184-194): column 3 of B * lightMVPs[i],
+    // w += accFactor * 10000; columns 0-2 from B * the last light rendered
+    std::vector<float> trans(static_cast<size_t>(n) * 4);
+    float B[16], biased[16], common[16];
+    GI_TRY(gi_bias_matrix(0.0f, B));
+    for (int i = 0; i < n; ++i) {
+        GI_TRY(gi_mat4_mul(B, lightMVPs_[i].data(), biased));
+        for (int r = 0; r < 4; ++r) trans[4 * i + r] = biased[12 + r];
+        trans[4 * i + 3] = trans[4 * i + 3] + accFactor_[i] * 10000.0f;
+    }
+    GI_TRY(gi_mat4_mul(B, lightMVP_.data(), common));
+    GI_TRY(hip_ok(hipMemcpyAsync(trans_.get(), trans.data(), trans.size() * sizeof(float), hipMemcpyHostToDevice,
+                                 stream)));
+    gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_, nullptr};
+    gi_soft_params sp{shadowIntensity, adaptiveSamplingLowerAccuracy ? 1 : 0, 3, 1.0f, 1};
+    GI_TRY(gi_adaptive_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, common, trans_.as<float>(), &sp,
+                                        soft_.as<float>(), stream));
+    return hip_ok(hipStreamSynchronize(stream));  // the host trans buffer goes out of scope
 }
 
 std::array<float, 9> SoftShadowPipeline::normalMatrix() const {

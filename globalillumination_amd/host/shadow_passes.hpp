@@ -142,6 +142,7 @@ public:
     // uploads it for that shader, so it is 0 there and PCSS degenerates to
     // "lit" (penumbra width 0); 1 is MyGLGeometryViewer's near plane.
     int pcssZNear = 1;
+    int maxLevel = 4;                              // main.cpp:190
     bool preEvaluation = true;                     // PCSS runs the pre-evaluation (:614)
     Vec3 cameraEye{}, cameraAt{}, cameraUp{0.0f, 0.0f, 1.0f};
     float cameraFov = 45.0f;
@@ -157,6 +158,22 @@ public:
     // renderSoftShadows with ShadowParams.PCSS (main.cpp:925-1016): the depth
     // map of the light centre (into map 0) and PlausibleSoftShadow.frag.
     int renderSoftShadows();
+    // renderAdaptiveLightSourceSampling (main.cpp:813-892) without the
+    // revectorization variant: the quad tree over the light is refined where
+    // a node's four corner maps disagree on a penumbra pixel
+    // (evaluateAreaLightSource :573-620), every leaf's corner maps are
+    // rendered (renderAreaLightSource :622-694), and the weighted
+    // AccurateSoftShadow pass averages them.  Needs maxSamples >= 289.
+    int renderAdaptiveLightSourceSampling();
+    // The adaptive frame's record: node tests in evaluation order
+    // (node index, level, passing pixels) and each map's quad-tree level.
+    struct NodeTest {
+        int nodeIndex, level;
+        unsigned count;
+    };
+    const std::vector<NodeTest>& nodeTests() const { return nodeTests_; }
+    const std::vector<float>& accFactors() const { return accFactor_; }
+    int adaptiveSamples() const { return adaptiveSamples_; }
     int deferredShading();  // Soft main.cpp:1109-1117: Phong over the soft-shadow buffer
     int download(std::vector<float>& softShadow, std::vector<float>* maps = nullptr,
                  std::vector<float>* rgba = nullptr);
@@ -171,7 +188,18 @@ private:
     bool ownStream_ = false;
     std::vector<Mat4> lightMVPs_;
     Mat4 cameraMVP_{}, cameraMV_{}, lightMVP_{};
-    DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, gfac_, soft_, rgba_, ws_;
+    DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, gfac_, soft_, rgba_, ws_, qtImg_, qtCount_;
+    // adaptive sampling state (renderAdaptiveLightSourceSampling)
+    struct QuadNode;
+    int renderLightSample(const Vec3& eye, const Vec3& at);
+    int evaluateSubAreaLightSource(const QuadNode& node, int nodeIndex, bool& subdivide);
+    int evaluateAreaLightSource(QuadNode& node, int nodeIndex);
+    int renderAreaLightSource(const QuadNode& node, int nodeIndex);
+    bool qtRendered_[17][17]{};
+    int qtHash_[17][17]{};
+    std::vector<NodeTest> nodeTests_;
+    std::vector<float> accFactor_;
+    int adaptiveSamples_ = 0;
 };
 
 }  // namespace gi::host

@@ -370,3 +370,50 @@ def render_monte_carlo(gbuf: GBuffer, maps: torch.Tensor, light_mvps: np.ndarray
     _lib.call("gi_soft_shadow_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, _f(common, 16),
               _dptr(trans_dev), ctypes.byref(p), _dptr(out), _stream_handle(stream))
     return out
+
+
+def adaptive_trans(light_mvps: np.ndarray, levels, zbias: float = 0.0) -> tuple:
+    """configureShadow for adaptive sampling (MyGLGeometryViewer.cpp:184-194):
+    the soft uniforms with each sample's quad-tree level added to w as
+    level * 10000 (FP32)."""
+    common, trans = soft_uniforms(light_mvps, zbias)
+    lv = np.asarray(levels, np.float32).reshape(-1)
+    if lv.shape[0] != trans.shape[0]:
+        raise ValueError("one level per light sample is required")
+    trans = trans.copy()
+    trans[:, 3] = trans[:, 3] + lv * np.float32(10000.0)
+    return common, trans
+
+
+def render_adaptive(gbuf: GBuffer, maps: torch.Tensor, common: np.ndarray, trans: np.ndarray, params: SoftParams,
+                    out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """Final pass of renderAdaptiveLightSourceSampling: AccurateSoftShadow.frag
+    with adaptiveSampling == 1 (`common`, `trans` from adaptive_trans)."""
+    S, smh, smw = (int(x) for x in maps.shape)
+    trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, 4)
+    if trans.shape[0] != S:
+        raise ValueError("one trans row per depth map is required")
+    trans_dev = torch.from_numpy(trans).to(maps.device)
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=maps.device)
+    g = gbuf.to_c(with_normals=False)
+    p = params.to_c()
+    _lib.call("gi_adaptive_soft_shadow_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, _f(common, 16),
+              _dptr(trans_dev), ctypes.byref(p), _dptr(out), _stream_handle(stream))
+    return out
+
+
+def quadtree_node_test(gbuf: GBuffer, maps: torch.Tensor, map_index, biased_mvps: np.ndarray, div: int = 1,
+                       image: torch.Tensor | None = None, stream=None) -> tuple:
+    """One quad-tree node's test (QuadTree/Reprojection.frag +
+    Evaluation.frag): (reprojection image, passing pixel count).  Reads the
+    count back, as the reference reads its occlusion query."""
+    S, smh, smw = (int(x) for x in maps.shape)
+    if image is None:
+        image = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=maps.device)
+    count = torch.empty((1,), dtype=torch.int32, device=maps.device)
+    idx = (ctypes.c_int * 4)(*[int(i) for i in map_index])
+    g = gbuf.to_c(with_normals=False)
+    _lib.call("gi_quadtree_node_test", ctypes.byref(g), _dptr(maps), S, smw, smh, idx, _f(biased_mvps, 64), int(div),
+              _dptr(image), _dptr(count), _stream_handle(stream))
+    return image, int(count.item())

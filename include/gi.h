@@ -206,6 +206,33 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
                         const gi_soft_params* params, float* out_dev, gi_stream stream);
 
 /* ------------------------------------------------------------------------ */
+/* Adaptive area-light sampling (quad tree over the light)                    */
+/* ------------------------------------------------------------------------ */
+
+/* One quad-tree node's test in renderAdaptiveLightSourceSampling
+ * (evaluateSubAreaLightSource, SoftShadowMapping/src/main.cpp:440-571):
+ * QuadTree/Reprojection.frag:8-24 over the G-buffer with the node's four
+ * corner maps (slots map_index[] of maps_dev, nmaps maps of smw x smh;
+ * biased_mvps: B * lightMVP of each corner, 4 x 16 floats) into image_dev
+ * (width*height floats: lit-corner count, 0.63 for background), then
+ * QuadTree/Evaluation.frag:6-22 on the (width/div) x (height/div) viewport;
+ * *count_dev receives the number of passing pixels, which the reference reads
+ * back from a GL_SAMPLES_PASSED query (the node subdivides when it is > 0).
+ * div: 1, or 4 with adaptiveSamplingLowerAccuracy (main.cpp:554-557). */
+int gi_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
+                          const int map_index[4], const float biased_mvps[64], int div, float* image_dev,
+                          unsigned* count_dev, gi_stream stream);
+
+/* The final pass of renderAdaptiveLightSourceSampling (main.cpp:868-876):
+ * AccurateSoftShadow.frag with adaptiveSampling == 1.  Arguments as
+ * gi_soft_shadow_pass; trans_dev[4*i+3] carries w + level * 10000 for sample
+ * i (configureShadow, MyGLGeometryViewer.cpp:192), and each sample weighs
+ * 1 / (2^min(level, 4) + 1)^2 in the average. */
+int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
+                                 const float light_mvp[16], const float* trans_dev, const gi_soft_params* params,
+                                 float* out_dev, gi_stream stream);
+
+/* ------------------------------------------------------------------------ */
 /* Percentage-closer soft shadows (plausible soft shadows, PCSS)             */
 /* ------------------------------------------------------------------------ */
 

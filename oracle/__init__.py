@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
             "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp, vp],
             "or_phong": [vp, vp, vp, i, i, vp, vp, vp, vp],
             "or_pcss": [vp, vp, i, i, vp, vp, i, i, i, i, vp],
+            "or_soft_shadow_adaptive": [vp, i, i, vp, i, i, i, vp, vp, ctypes.c_float, i, i, ctypes.c_float, i, vp],
+            "or_qt_node_test": [vp, i, i, vp, i, i, vp, vp, i, vp, vp],
             "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
             "or_diff_soft": [vp, i, i, vp, i, i, i, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp],
             "or_set_threads": [i], "or_get_threads": [],
@@ -191,6 +193,38 @@ def soft_shadow(pos: np.ndarray, maps: np.ndarray, light_mvp_biased, trans, si=0
     if rc:
         raise RuntimeError(f"or_soft_shadow failed ({rc})")
     return out
+
+
+def soft_shadow_adaptive(pos: np.ndarray, maps: np.ndarray, light_mvp_biased, trans, si=0.25, pcf=False,
+                         kernel_order=3, penumbra=1.0, inclusive=True) -> np.ndarray:
+    """AccurateSoftShadow.frag with adaptiveSampling == 1; trans[:, 3] holds
+    w + level * 10000 (float32)."""
+    pos, maps = _c32(pos), _c32(maps)
+    _, h, w = pos.shape
+    S, smh, smw = maps.shape
+    m, t = _c32(light_mvp_biased).reshape(-1), _c32(trans).reshape(-1)
+    out = np.empty((h, w), np.float32)
+    rc = lib().or_soft_shadow_adaptive(_p(pos), w, h, _p(maps), S, smw, smh, _p(m), _p(t), si, 1 if pcf else 0,
+                                       kernel_order, penumbra, 1 if inclusive else 0, _p(out))
+    if rc:
+        raise RuntimeError(f"or_soft_shadow_adaptive failed ({rc})")
+    return out
+
+
+def qt_node_test(pos: np.ndarray, maps: np.ndarray, idx, mvps_biased, div=1):
+    """One quad-tree node's reprojection image and pass count."""
+    pos, maps = _c32(pos), _c32(maps)
+    _, h, w = pos.shape
+    S, smh, smw = maps.shape
+    ix = np.ascontiguousarray(idx, dtype=np.int32).reshape(4)
+    assert (ix >= 0).all() and (ix < S).all()
+    m = _c32(mvps_biased).reshape(64)
+    img = np.empty((h, w), np.float32)
+    cnt = ctypes.c_uint(0)
+    rc = lib().or_qt_node_test(_p(pos), w, h, _p(maps), smw, smh, _p(ix), _p(m), int(div), _p(img), ctypes.byref(cnt))
+    if rc:
+        raise RuntimeError(f"or_qt_node_test failed ({rc})")
+    return img, int(cnt.value)
 
 
 def _mesh_args(positions, indices):

@@ -684,10 +684,10 @@ int or_hard_shadow(const float* pos, const float* nrm, int w, int h, const float
 /* ------------------------------------------------------------------------ */
 /* Soft shadows: AccurateSoftShadow.frag:52-133 (a13)                         */
 /* ------------------------------------------------------------------------ */
-int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsamples,
-                   int smw, int smh, const float m[16], const float* trans,
-                   float si, int pcf_mode, int kernel_order, float penumbra,
-                   int inclusive, float* out)
+static int soft_shadow_impl(const float* pos, int w, int h, const float* maps, int nsamples,
+                            int smw, int smh, const float m[16], const float* trans,
+                            float si, int pcf_mode, int kernel_order, float penumbra,
+                            int inclusive, int adaptive, float* out)
 {
     if (!pos || !maps || !trans || !out || w <= 0 || h <= 0 || nsamples <= 0) return -1;
     if (pcf_mode && (kernel_order <= 0 || !(penumbra > 0.0f))) return -1; /* loop would not advance */
@@ -711,12 +711,24 @@ int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsampl
         float accShadow = 0.0f, count = 0.0f;
         for (int idx = 0; idx < nsamples; ++idx) {
             const float* T = trans + 4 * idx;
-            float cx = cm[0] + T[0], cy = cm[1] + T[1], cz = cm[2] + T[2], cw = cm[3] + T[3];
+            float accFactor = 1.0f, Tw = T[3];
+            if (adaptive) {
+                /* AccurateSoftShadow.frag:79-87: level = int(w / 10000), at most 4;
+                 * w - level * 10000; weight 1 / (myPow(2, level) + 1)^2 */
+                int level = (int)(T[3] / 10000.0f);
+                if (level > 4) level = 4;
+                Tw = T[3] - (float)(level * 10000);
+                float pw = 1.0f;
+                for (int e = 1; e <= level; ++e) pw = pw * 2.0f;
+                accFactor = pw + 1.0f;
+                accFactor = 1.0f / (accFactor * accFactor);
+            }
+            float cx = cm[0] + T[0], cy = cm[1] + T[1], cz = cm[2] + T[2], cw = cm[3] + Tw;
             float sx = cx / cw, sy = cy / cw, sz = cz / cw;
             const float* D = maps + (size_t)idx * mapsz;
             if (pcf_mode == 0) {
                 float d = or_fetch(D, smw, smh, sx, sy);
-                accShadow = accShadow + ((sz <= d) ? 1.0f : si);
+                accShadow = accShadow + ((sz <= d) ? 1.0f : si) * accFactor;
             } else {
                 float incrWidth = 1.0f / (float)smw;
                 float incrHeight = 1.0f / (float)smh;
@@ -732,12 +744,85 @@ int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsampl
                         PCFCount++;
                     }
                 }
-                accShadow = accShadow + illuminationCount / (float)PCFCount;
+                accShadow = accShadow + (illuminationCount / (float)PCFCount) * accFactor;
             }
-            count = count + 1.0f;
+            count = count + accFactor;
         }
         out[k] = accShadow / count;
     }
+    return 0;
+}
+
+int or_soft_shadow(const float* pos, int w, int h, const float* maps, int nsamples,
+                   int smw, int smh, const float m[16], const float* trans,
+                   float si, int pcf_mode, int kernel_order, float penumbra,
+                   int inclusive, float* out)
+{
+    return soft_shadow_impl(pos, w, h, maps, nsamples, smw, smh, m, trans, si, pcf_mode, kernel_order, penumbra,
+                            inclusive, 0, out);
+}
+
+int or_soft_shadow_adaptive(const float* pos, int w, int h, const float* maps, int nsamples,
+                            int smw, int smh, const float m[16], const float* trans,
+                            float si, int pcf_mode, int kernel_order, float penumbra,
+                            int inclusive, float* out)
+{
+    return soft_shadow_impl(pos, w, h, maps, nsamples, smw, smh, m, trans, si, pcf_mode, kernel_order, penumbra,
+                            inclusive, 1, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adaptive sampling: one quad-tree node's test                               */
+/* (evaluateSubAreaLightSource, SoftShadowMapping/src/main.cpp:440-571)      */
+/* ------------------------------------------------------------------------ */
+
+/* GL_NEAREST texel for reduced-grid coordinate c: texture coordinate
+ * (c + 0.5) / nd on an n-texel axis, floor((c + 0.5) * n / nd) in exact
+ * arithmetic; -1 outside (CLAMP_TO_BORDER, border 0). */
+static int qt_texel(int c, int n, int nd)
+{
+    if (c < 0) return -1;
+    long long a = ((2LL * c + 1) * (long long)n) / (2LL * nd);
+    return a < n ? (int)a : -1;
+}
+
+int or_qt_node_test(const float* pos, int w, int h, const float* maps, int smw, int smh, const int idx[4],
+                    const float mvps[64], int div, float* img, unsigned* count)
+{
+    if (!pos || !maps || !idx || !mvps || !img || !count || div < 1 || div > w || div > h) return -1;
+    size_t plane = (size_t)w * (size_t)h, mapsz = (size_t)smw * (size_t)smh;
+    /* QuadTree/Reprojection.frag:8-24 */
+    for (size_t k = 0; k < plane; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { img[k] = 0.63f; continue; } /* discarded: clear colour R (main.cpp:511) */
+        float shadow[4];
+        for (int i = 0; i < 4; ++i) {
+            float sc[4];
+            m4v(mvps + 16 * i, vx, vy, vz, 1.0f, sc);
+            float s = sc[0] / sc[3], t = sc[1] / sc[3], z = sc[2] / sc[3];
+            shadow[i] = (z <= or_fetch(maps + (size_t)idx[i] * mapsz, smw, smh, s, t)) ? 1.0f : 0.0f;
+        }
+        img[k] = shadow[0] + shadow[1] + shadow[2] + shadow[3];
+    }
+    /* QuadTree/Evaluation.frag:6-22 on the (w/div) x (h/div) viewport */
+    int wd = w / div, hd = h / div;
+    unsigned passed = 0;
+    for (int j = 0; j < hd; ++j) {
+        for (int i = 0; i < wd; ++i) {
+            int a = qt_texel(i, w, wd), b = qt_texel(j, h, hd);
+            float sum = (a < 0 || b < 0) ? 0.0f : img[(size_t)b * w + a];
+            if (sum == 0.63f || sum == 0.0f || sum == 4.0f || sum == 5.0f) continue;
+            int ok = 1;
+            for (int y = -1; y <= 1 && ok; ++y)
+                for (int x = -1; x <= 1 && ok; ++x) {
+                    int na = qt_texel(i + x, w, wd), nb = qt_texel(j + y, h, hd);
+                    float nsum = (na < 0 || nb < 0) ? 0.0f : img[(size_t)nb * w + na];
+                    if (sum != nsum) ok = 0;
+                }
+            passed += (unsigned)ok;
+        }
+    }
+    *count = passed;
     return 0;
 }
 

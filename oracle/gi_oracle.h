@@ -89,6 +89,21 @@ int or_visibility_raster(const float* pos, int nv, const int* idx, int nt,
 int or_gbuffer(const float* pos, const float* nrm, int nv, const int* idx, int nt,
                const float mvp[16], int w, int h, float* pos_out, float* nrm_out, float* fac_out);
 
+/* AccurateSoftShadow.frag with adaptiveSampling == 1 (:79-87): trans[4i+3]
+ * carries w + level * 10000; samples weigh 1 / (2^min(level,4) + 1)^2. */
+int or_soft_shadow_adaptive(const float* pos, int w, int h, const float* maps, int nsamples,
+                            int smw, int smh, const float light_mvp[16], const float* trans,
+                            float si, int pcf_mode, int kernel_order, float penumbra,
+                            int inclusive, float* out);
+
+/* One quad-tree node's test of renderAdaptiveLightSourceSampling:
+ * QuadTree/Reprojection.frag into img (w*h; 0.63 for background) with the
+ * maps idx[0..3] and their B*lightMVP matrices mvps (4 x 16), then
+ * QuadTree/Evaluation.frag on the (w/div) x (h/div) viewport; *count = the
+ * passing pixels (the reference's GL_SAMPLES_PASSED). */
+int or_qt_node_test(const float* pos, int w, int h, const float* maps, int smw, int smh, const int idx[4],
+                    const float mvps[64], int div, float* img, unsigned* count);
+
 /* PlausibleSoftShadow.frag with PCSS == 1 (:166-192, :365-398, :556-563,
  * main :605-632).  pos/nrm as or_hard_shadow; p->light_mvp, mv,
  * normal_matrix, light_position, shadow_intensity, pre_evaluation,

@@ -146,7 +146,7 @@ int main(int argc, char** argv) {
     soft.cameraEye = {0.0f, 3.0f, 6.0f};
     soft.cameraAt = {0.0f, 0.0f, 0.0f};
     soft.cameraFov = 60.0f;
-    rc = soft.init(mesh, W, H, SM, SM, 9);
+    rc = soft.init(mesh, W, H, SM, SM, 289);  // 289: adaptive sampling's 17 x 17 grid
     if (rc) return fail("SoftShadowPipeline::init", rc);
     rc = soft.renderGBuffer();
     if (rc) return fail("SoftShadowPipeline::renderGBuffer", rc);
@@ -177,6 +177,29 @@ int main(int argc, char** argv) {
     dumpv(dir, "pcss.f32", pcss);
     dumpv(dir, "pcss_map.f32", maps);
     dump(dir + "/pcss_light_mvp.f32", soft.lightMVP().data(), 64);
+
+    // renderAdaptiveLightSourceSampling: a 2 x 2 light at (4, 8, 4), maximum
+    // level 4, exact evaluation (div 1) and the lower-accuracy variant
+    // (4x4 PCF per sample, evaluation on a quarter-resolution grid)
+    soft.lightEye = {4.0f, 8.0f, 4.0f};
+    soft.lightSize = 2;
+    for (int lower = 0; lower < 2; ++lower) {
+        soft.adaptiveSamplingLowerAccuracy = lower != 0;
+        rc = soft.renderAdaptiveLightSourceSampling();
+        if (rc) return fail("renderAdaptiveLightSourceSampling", rc);
+        std::vector<float> ad;
+        rc = soft.download(ad);
+        if (rc) return fail("download", rc);
+        const std::string tag = lower ? "adaptive_lower" : "adaptive";
+        dumpv(dir, (tag + "_soft.f32").c_str(), ad);
+        dumpv(dir, (tag + "_acc.f32").c_str(), soft.accFactors());
+        dump(dir + "/" + tag + "_mvps.f32", soft.lightMVPs().data(), soft.lightMVPs().size() * 64);
+        std::vector<int32_t> tests;
+        for (const auto& t : soft.nodeTests())
+            tests.insert(tests.end(), {t.nodeIndex, t.level, static_cast<int32_t>(t.count)});
+        dumpv(dir, (tag + "_tests.i32").c_str(), tests);
+        std::printf("%s: %d samples, %zu node tests\n", tag.c_str(), soft.adaptiveSamples(), soft.nodeTests().size());
+    }
     std::printf("host pipeline ok: %d vertices, %d triangles\n", mesh.nverts(), mesh.ntris());
     return 0;
 }

@@ -1,4 +1,4 @@
-"""The C ABI boundary (include/gi.h): the library loads, exports every
+"""The C ABI boundary (include/*.h): the library loads, exports every
 declared entry point, and rejects bad arguments before touching the GPU."""
 import ctypes
 import re
@@ -11,7 +11,7 @@ ROOT = Path(__file__).resolve().parent.parent
 
 
 def declared_functions():
-    text = (ROOT / "include" / "gi.h").read_text()
+    text = "\n".join(h.read_text() for h in sorted((ROOT / "include").glob("*.h")))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(gi_[a-z0-9_]+)\s*\(", text)))
 

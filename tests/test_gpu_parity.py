@@ -446,3 +446,37 @@ d 0.000025
     rgba = gi.shade_scene(g, got, gi.PhongParams.from_views(wl.camera, wl.lights[0].eye))
     want_rgba = oracle_lib.phong(pos, nrm, fac, want, MV, nm, lpos, 0.25)
     assert bits_equal(_np(rgba), want_rgba)
+
+
+@pytest.mark.parametrize("div", [1, 4])
+def test_quadtree_node_test_matches_oracle(gi, oracle_lib, cuda, div):
+    """One quad-tree node's test on the area-light scene: the four corner
+    maps of a 0.5-wide light, the reprojection image and the pass count."""
+    wl = gi.scenes.config3(nsamples=4, view_w=640, view_h=360, sm=1024, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 640, 360)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, 1024, 1024, polygon_offset=(4.0, 20.0))
+    biased = np.stack([gi.biased(m) for m in mvps])
+    img, count = gi.passes.quadtree_node_test(g, maps, [0, 1, 2, 3], biased, div)
+    want_img, want_count = oracle_lib.qt_node_test(_np(g.position), _np(maps), [0, 1, 2, 3], biased, div)
+    assert bits_equal(_np(img), want_img)
+    assert count == want_count and count > 0
+
+
+@pytest.mark.parametrize("pcf", [False, True])
+def test_adaptive_soft_matches_oracle(gi, oracle_lib, cuda, pcf):
+    """The weighted AccurateSoftShadow pass with mixed quad-tree levels
+    (0-4 and one clamped level 6)."""
+    wl = gi.scenes.config3(nsamples=9, view_w=640, view_h=360, sm=1024, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 640, 360)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, 1024, 1024)
+    levels = [0, 1, 2, 3, 4, 1, 6, 2, 0]
+    common, trans = gi.passes.adaptive_trans(mvps, levels)
+    params = gi.SoftParams(pcf=pcf, kernelOrder=3, penumbra=1.0, inclusive=True)
+    got = _np(gi.passes.render_adaptive(g, maps, common, trans, params))
+    want = oracle_lib.soft_shadow_adaptive(_np(g.position), _np(maps), common, trans, si=0.25, pcf=pcf,
+                                           kernel_order=3, penumbra=1.0, inclusive=True)
+    assert bits_equal(got, want), mismatch_report(got, want, f"adaptive soft pcf={pcf}")
