@@ -297,6 +297,18 @@ def test_soft_shadow_area_light(gi, oracle_lib, cuda, pcf, inclusive, order, pen
     assert vis.min() >= 0.25 - 1e-7 and vis.max() <= 1.0 + 1e-7
 
 
+@pytest.mark.parametrize("si", [0.3, 0.0, 1.0, 0.5 + 2.0 ** -13, 0.375])
+def test_soft_shadow_intensities(gi, oracle_lib, cuda, si):
+    """Shadow intensities other than 0.25, including ones whose tap sums
+    round in FP32 (0.3, 0.5 + 2^-13): the kernel adds the taps in the loop's
+    order, bit-identical to the oracle's sequential sum."""
+    wl = gi.scenes.config3(nsamples=9, view_w=480, view_h=270, sm=1024, freq=40)
+    for inclusive in (True, False):
+        params = gi.SoftParams(shadowIntensity=si, pcf=True, inclusive=inclusive, kernelOrder=3, penumbra=1.0)
+        got, want = _soft_case(gi, oracle_lib, wl, 1024, params)
+        assert bits_equal(got, want), mismatch_report(got, want, f"soft si={si} inclusive={inclusive}")
+
+
 def test_soft_shadow_map_border(gi, oracle_lib, cuda):
     """A light frustum narrower than the scene: taps fall off the map edge and
     read the border (depth 0, shadowed)."""
