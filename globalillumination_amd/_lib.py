@@ -68,6 +68,12 @@ class GiPcssParams(ctypes.Structure):
                 ("light_position", ctypes.c_float * 3)]
 
 
+class GiRbssmParams(ctypes.Structure):
+    _fields_ = [("pcss", GiPcssParams),
+                ("max_search", ctypes.c_int),
+                ("depth_threshold", ctypes.c_float)]
+
+
 class GiPhongParams(ctypes.Structure):
     _fields_ = [("mv", ctypes.c_float * 16),
                 ("normal_matrix", ctypes.c_float * 9),
@@ -183,6 +189,8 @@ _SIGS = {
                                                     ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_pcss_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiPcssParams),
                                     ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_rbssm_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiRbssmParams),
+                                     ctypes.c_void_p, ctypes.c_void_p]),
     "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(GiPhongParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),

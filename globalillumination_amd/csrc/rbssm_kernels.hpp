@@ -1,0 +1,511 @@
+// rbssm_kernels.hpp — revectorization-based soft shadows (RBSSM) for gfx950.
+//
+// SoftShadowMapping/Shaders/SoftShadow/RBSSM.frag: a PCSS-sized kernel
+// (blocker search and penumbra width as PlausibleSoftShadow.frag) whose taps
+// that sit on a shadow-map discontinuity are revectorized: the discontinuity
+// is oriented by four edge walks (computeDiscontinuityLength :193-253),
+// normalized (:255-309) and filled by smoothONDS (:311-1117).  Every branch,
+// coordinate step and `mix`/`step` is evaluated as the shader writes it, in
+// FP32; the neighbour tests are the RBSM ones (Rbsm<false>::disc_f / disc_v /
+// discontinuity, identical in RBSSM.frag :37-186, :1137-1147).
+#pragma once
+
+namespace gi {
+namespace {
+
+__device__ __forceinline__ float gstep(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
+__device__ __forceinline__ float fbool(bool b) { return b ? 1.0f : 0.0f; }
+
+struct Rbssm {
+    Rbsm<false> rb;  // shadow map, shadowMapStep, depthThreshold, maxSearch
+    mutable float newDepth;
+
+    __device__ __forceinline__ float sx() const { return rb.sx; }
+    __device__ __forceinline__ float sy() const { return rb.sy; }
+    __device__ __forceinline__ float tex(float s, float t) const { return rb.tex(s, t); }
+    // getDisc(coord, vec2 dir, float discType) :63-101
+    __device__ __forceinline__ bool discf(float x, float y, float z, float dx, float dy, float type) const {
+        return rb.disc_f(x, y, z, dx, dy, type);
+    }
+    // getDisc(coord, vec2 dir, vec4 discType) :103-186 (writes newDepth)
+    __device__ __forceinline__ bool discv(float x, float y, float z, float dx, float dy, float r, float g,
+                                          float b) const {
+        float nd;
+        const bool h = rb.disc_v(x, y, z, dx, dy, r, g, b, nd);
+        newDepth = nd;
+        return h;
+    }
+
+    static __device__ __forceinline__ float compress(float nd) { return -2.0f - ((0.5f - nd) * 2.0f); }
+    static __device__ __forceinline__ float decompress(float nd) { return 0.5f - ((nd + 2.0f) * -1.0f) / 2.0f; }
+
+    // computeDiscontinuityLength :193-253
+    __device__ float length(float ir, float ig, float ib, float x, float y, float z, float dx, float dy) const {
+        float foundEdgeEnd = 0.0f;
+        if (dx == 0.0f && ir == 0.0f && ig != 0.0f) return -1.0f;
+        if (dy == 0.0f && ir != 0.0f && ig == 0.0f) return -1.0f;
+        if (((0.5f - ir) * 8.0f - 1.0f) == dx) return 1.0f;
+        if (((ig - 0.5f) * 8.0f + 1.0f) == dy) return 1.0f;
+        float dist = 1.0f;
+        const float stx = dx * sx(), sty = dy * sy();
+        float cx = x + stx, cy = y + sty, cz = z;
+        for (int it = 0; it < rb.max_search; ++it) {
+            const float d = tex(cx, cy);
+            if (ib == 0.0f)
+                if (fabsf(cz - d) < rb.thr) cz = cz - rb.thr;
+            const float center = le01(cz, d);
+            if (fabsf(center - ib) == 0.0f) {
+                foundEdgeEnd = discf(cx, cy, cz, 0.0f, 0.0f, ib) ? 1.0f : 0.0f;
+                break;
+            }
+            if (!discv(cx, cy, cz, dx, dy, ir, ig, ib)) break;
+            dist = dist + 1.0f;
+            cx = cx + stx;
+            cy = cy + sty;
+            if (ib == 1.0f) cz = newDepth;
+        }
+        return gmix(-dist, dist, foundEdgeEnd);
+    }
+
+    // normalizeDiscontinuitySpace :255-280
+    __device__ float normalize(float a, float b, float sub) const {
+        if (a < 0.0f && b < 0.0f) return -1.0f;
+        const float edgeLength = gmin(fabsf(a) + fabsf(b) - 1.0f, (float)rb.max_search);
+        float nd = 1.0f - gmax(a, b) / edgeLength;
+        if (a == b)
+            nd = nd + gmix(sub / edgeLength, (1.0f - sub) / edgeLength, gstep(0.5f, sub));
+        else if (a == gmax(a, b))
+            nd = nd + (1.0f - sub) / edgeLength;
+        else
+            nd = nd + sub / edgeLength;
+        if (a > 0.0f && b > 0.0f) return compress(nd);
+        return nd;
+    }
+
+    // smoothONDS :311-1117 (lightCoord, normalizedDiscontinuity and
+    // discontinuity are the shader's by-value copies)
+    __device__ float smooth(float lx, float ly, float lz, float nx, float ny, float dr, float dg, float db, float ux,
+                            float uy) const {
+        const float sxx = sx(), syy = sy();
+        if (db == 0.0f) {
+            // positive entering discontinuity on both axes, on both sides of one axis
+            if (nx <= -2.0f && ny <= -2.0f && (dr == 0.75f || dg == 0.75f)) {
+                bool left = true, right = true, bottom = true, top = true;
+                if (dr == 0.75f) {
+                    if (dg == 0.0f) {
+                        ly = ly + syy;
+                        top = discf(lx, ly, lz, 0.0f, 1.0f, db);
+                        ly = ly - 2.0f * syy;
+                        bottom = discf(lx, ly, lz, 0.0f, 1.0f, db);
+                        ly = ly + syy;
+                        if (top && bottom) return 0.0f;
+                        ly = ly + gmix(-syy, syy, gstep(1.0f, fbool(top)));
+                        left = discv(lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                        right = !left;
+                    } else {
+                        if (dg == 0.5f)
+                            top = false;
+                        else if (dg == 0.25f)
+                            bottom = false;
+                        ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * syy;
+                        left = discv(lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                        right = discv(lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                        if (left && right) {
+                            if (ny <= -2.0f) ny = decompress(ny) + 0.5f;
+                            const float a = gclamp(ux - (ny - 1.0f), 0.0f, 1.0f);
+                            const float b = gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f);
+                            const float c = gmix(1.0f - uy, uy, gstep(dg, 1.0f));
+                            return gmin(gmin(a, b), c);
+                        }
+                    }
+                }
+                if (dg == 0.75f) {
+                    if (dr == 0.0f) {
+                        lx = lx - sxx;
+                        left = discf(lx, ly, lz, 1.0f, 0.0f, db);
+                        lx = lx + 2.0f * sxx;
+                        right = discf(lx, ly, lz, 1.0f, 0.0f, db);
+                        lx = lx - sxx;
+                        if (left && right) return 0.0f;
+                        lx = lx + gmix(sxx, -sxx, gstep(1.0f, fbool(left)));
+                        bottom = discv(lx, ly, lz, 0.0f, 0.25f, 0.5f, 0.0f, 0.0f);
+                        top = !bottom;
+                    } else {
+                        if (dr == 0.5f)
+                            right = false;
+                        else if (dr == 0.25f)
+                            left = false;
+                        lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                        bottom = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                        top = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                        if (bottom && top) {
+                            if (nx <= -2.0f) nx = decompress(nx) + 0.5f;
+                            const float a = gclamp(uy - (nx - 1.0f), 0.0f, 1.0f);
+                            const float b = gclamp((1.0f - uy) - (nx - 1.0f), 0.0f, 1.0f);
+                            const float c = gmix(1.0f - ux, ux, gstep(dr, 1.0f));
+                            return gmin(gmin(a, b), c);
+                        }
+                    }
+                }
+                if (!left && !bottom)
+                    return gclamp((1.0f - ux) - (1.0f - uy), 0.0f, 1.0f);
+                else if (!right && !bottom)
+                    return gclamp(ux - (1.0f - uy), 0.0f, 1.0f);
+                else if (!left && !top)
+                    return gclamp((1.0f - uy) - ux, 0.0f, 1.0f);
+                else if (!right && !top)
+                    return gclamp((1.0f - uy) - (1.0f - ux), 0.0f, 1.0f);
+            }
+
+            if (dr == 0.75f || dg == 0.75f) {
+                // entering left and right discontinuity
+                if (dr == 0.75f && dg != 0.0f) {
+                    ly = ly + ((dg - 0.75f) * 4.0f) * syy;
+                    const bool left = discv(lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    const bool right = discv(lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    if (!left && !right) return gclamp(1.0f - ny, 0.0f, 1.0f);
+                    const float sub = gmix(1.0f - uy, uy, gstep(1.0f, dg));
+                    const float a =
+                        gmix(sub, gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(right)));
+                    const float b = gmix(sub, gclamp(ux - (ny - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(left)));
+                    return gmin(a, b);
+                }
+                // entering left and right discontinuity only
+                if (dr == 0.75f && dg == 0.0f) {
+                    bool topLeft, topRight, bottomLeft, bottomRight;
+                    const float ty = ly + syy;
+                    const bool topCenter = !(le01(lz, tex(lx, ty)) != 0.0f);
+                    topLeft = discv(lx, ty, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    topRight = discv(lx, ty, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    const float by = ly - syy;
+                    const bool bottomCenter = !(le01(lz, tex(lx, by)) != 0.0f);
+                    bottomLeft = discv(lx, by, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    bottomRight = discv(lx, by, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    if (topCenter) topLeft = topRight = true;
+                    if (bottomCenter) bottomLeft = bottomRight = true;
+                    if (ny <= -2.0f) {
+                        ny = decompress(ny) + 0.5f;
+                        const float a = gclamp(ux - (ny - 1.0f), 0.0f, 1.0f);
+                        const float b = gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f);
+                        if (!bottomRight || !topRight) return a;
+                        else if (!bottomLeft || !topLeft) return b;
+                        else return gmin(a, b);
+                    }
+                    if ((!bottomRight && !bottomLeft) || (!topRight && !topLeft))
+                        return gclamp(1.0f - ny, 0.0f, 1.0f);
+                    else if (!bottomRight || !topRight)
+                        return gclamp(ux - (ny - 1.0f), 0.0f, 1.0f);
+                    else
+                        return gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f);
+                }
+                // entering top and bottom discontinuity
+                if (dr != 0.0f && dg == 0.75f) {
+                    lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                    const bool bottom = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                    const bool top = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    if (!bottom && !top) return gclamp(1.0f - nx, 0.0f, 1.0f);
+                    const float sub = gmix(ux, 1.0f - ux, gstep(dr, 0.25f));
+                    const float a = gmix(sub, gclamp(uy - (nx - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(top)));
+                    const float b =
+                        gmix(sub, gclamp((1.0f - uy) - (nx - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(bottom)));
+                    return gmin(a, b);
+                }
+                // entering top and bottom discontinuity only
+                if (dr == 0.0f && dg == 0.75f) {
+                    bool topLeft, topRight, bottomLeft, bottomRight;
+                    const float rx = lx + sxx;
+                    const bool rightCenter = !(le01(lz, tex(rx, ly)) != 0.0f);
+                    bottomRight = discv(rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                    topRight = discv(rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    const float lxx = lx - sxx;
+                    const bool leftCenter = !(le01(lz, tex(lxx, ly)) != 0.0f);
+                    bottomLeft = discv(lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                    topLeft = discv(lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    if (rightCenter) bottomRight = topRight = true;
+                    if (leftCenter) topLeft = bottomLeft = true;
+                    if (nx <= -2.0f) {
+                        nx = decompress(nx) + 0.5f;
+                        const float a = gclamp(uy - (nx - 1.0f), 0.0f, 1.0f);
+                        const float b = gclamp((1.0f - uy) - (nx - 1.0f), 0.0f, 1.0f);
+                        if (!bottomRight || !bottomLeft) return a;
+                        else if (!topRight || !topLeft) return b;
+                        else return gmin(a, b);
+                    }
+                    if ((!bottomRight && !topRight) || (!bottomLeft && !topLeft))
+                        return gclamp(1.0f - nx, 0.0f, 1.0f);
+                    else if (!bottomRight || !bottomLeft)
+                        return gclamp(uy - (nx - 1.0f), 0.0f, 1.0f);
+                    else
+                        return gclamp((1.0f - uy) - (nx - 1.0f), 0.0f, 1.0f);
+                }
+            }
+
+            // discontinuity in both axes (corner)
+            if (dr > 0.0f && dg > 0.0f) {
+                lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                const bool horizontal = discf(lx, ly, lz, 1.0f, 0.0f, 0.0f);
+                lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * syy;
+                const bool vertical = discf(lx, ly, lz, 0.0f, 1.0f, 0.0f);
+                if (horizontal && !vertical) {
+                    dr = 0.0f;
+                } else if (!horizontal && vertical) {
+                    dg = 0.0f;
+                } else if (!horizontal && !vertical) {
+                    if (dg == 0.5f) return gclamp((1.0f - nx) - (uy - 1.0f), 0.0f, 1.0f);
+                    else if (dg == 0.25f) return gclamp((1.0f - nx) + uy, 0.0f, 1.0f);
+                } else {
+                    float a, b;
+                    if (dr == 0.5f && dg == 0.5f) {
+                        a = gmix(1.0f - uy, gclamp(1.0f - (nx - (1.0f - uy)), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(ux, gclamp(1.0f - (ny - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmin(a, b);
+                    } else if (dr == 0.5f && dg == 0.25f) {
+                        a = gmix(uy, gclamp(1.0f - (nx - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(ux, gclamp(1.0f - (ny - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmin(a, b);
+                    } else if (dr == 0.25f && dg == 0.5f) {
+                        a = gmix(1.0f - uy, gclamp(1.0f - (nx - (1.0f - uy)), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(1.0f - ux, gclamp(1.0f - (ny - (1.0f - ux)), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmin(a, b);
+                    } else if (dr == 0.25f && dg == 0.25f) {
+                        a = gmix(uy, gclamp(1.0f - (nx - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(1.0f - ux, gclamp(1.0f - (ny - (1.0f - ux)), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmin(a, b);
+                    }
+                }
+            }
+
+            if (nx <= -2.0f) return gmix(1.0f - uy, uy, gstep(dg, 0.25f));
+            if (ny <= -2.0f) return gmix(ux, 1.0f - ux, gstep(dr, 0.25f));
+            if (dg > 0.0f) {
+                if (dg == 0.5f) return gclamp((1.0f - uy) - (nx - 1.0f), 0.0f, 1.0f);
+                else return gclamp(uy - (nx - 1.0f), 0.0f, 1.0f);
+            }
+            if (dr > 0.0f) {
+                if (dr == 0.5f) return gclamp(ux - (ny - 1.0f), 0.0f, 1.0f);
+                else return gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f);
+            }
+        } else {  // exiting discontinuity
+            if (dr == 0.75f || dg == 0.75f) {
+                // exiting left and right discontinuity only
+                if (dr == 0.75f && dg == 0.0f) {
+                    float rx = lx - sxx, ry = ly;
+                    bool left = discv(rx, ry, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    rx = lx + sxx;
+                    bool right = discv(rx, ry, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    if (left && right) {
+                        return gclamp(ny, 0.0f, 1.0f);
+                    } else if (left || right) {
+                        if (!left) return gmix(1.0f - ux, gclamp(ny - ux, 0.0f, 1.0f), gstep(-2.0f, ny));
+                        else return gmix(ux, gclamp(ny - (1.0f - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                    } else {
+                        // count 0: mult 1, even -> one row down
+                        ry = ly + 1.0f * syy;
+                        rx = lx - sxx;
+                        left = discf(rx, ry, lz, 1.0f, 0.0f, 0.0f);
+                        rx = lx + sxx;
+                        right = discf(rx, ry, lz, 1.0f, 0.0f, 0.0f);
+                        if (left && right) {
+                            return gclamp(ny, 0.0f, 1.0f);
+                        } else {
+                            if (!left) return gmix(1.0f - ux, gclamp(ny - ux, 0.0f, 1.0f), gstep(-2.0f, ny));
+                            else return gmix(ux, gclamp(ny - (1.0f - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        }
+                    }
+                }
+                // exiting left and right discontinuity
+                if (dr == 0.75f && dg != 0.0f) {
+                    ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * syy;
+                    bool left = discv(lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    bool right = discv(lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    float a, b;
+                    if (left && right) {
+                        return gclamp(ny, 0.0f, 1.0f);
+                    } else if (left || right) {
+                        if (left) a = gmix(uy, 1.0f - uy, gstep(dg, 0.25f));
+                        else a = gmix(1.0f - ux, gclamp(ny - ux, 0.0f, 1.0f), gstep(-2.0f, ny));
+                        if (right) b = gmix(uy, 1.0f - uy, gstep(dg, 0.25f));
+                        else b = gmix(ux, gclamp(ny - (1.0f - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmax(a, b);
+                    } else {
+                        if (dg == 0.75f) return 0.0f;
+                        left = discf(lx - sxx, ly, lz, 1.0f, 0.0f, 0.0f);
+                        right = discf(lx + sxx, ly, lz, 1.0f, 0.0f, 0.0f);
+                        if (left && right) {
+                            return gclamp(ny, 0.0f, 1.0f);
+                        } else {
+                            a = gmix(1.0f - ux, ux, gstep(1.0f, fbool(left)));
+                            b = gmix(uy, 1.0f - uy, gstep(dg, 0.25f));
+                            return gmax(a, b);
+                        }
+                    }
+                }
+                // exiting top and bottom discontinuity only
+                if (dr == 0.0f && dg == 0.75f) {
+                    float rx = lx, ry = ly - syy;
+                    bool top = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
+                    ry = ly + syy;
+                    bool bottom = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
+                    if (bottom && top) {
+                        return gclamp(nx, 0.0f, 1.0f);
+                    } else if (bottom || top) {
+                        if (!top) return gmix(1.0f - uy, gclamp(nx - uy, 0.0f, 1.0f), gstep(-2.0f, nx));
+                        else return gmix(uy, gclamp(nx - (1.0f - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                    } else {
+                        rx = lx + 1.0f * sxx;
+                        ry = ly - syy;
+                        top = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
+                        ry = ly + syy;
+                        bottom = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
+                        if (top && bottom) {
+                            return gclamp(nx, 0.0f, 1.0f);
+                        } else {
+                            if (!top) return gmix(1.0f - uy, gclamp(nx - uy, 0.0f, 1.0f), gstep(-2.0f, nx));
+                            else return gmix(uy, gclamp(nx - (1.0f - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        }
+                    }
+                }
+                // exiting top and bottom discontinuity
+                if (dr != 0.0f && dg == 0.75f) {
+                    lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                    bool bottom = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                    bool top = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    float a, b;
+                    if (bottom && top) {
+                        return gclamp(nx, 0.0f, 1.0f);
+                    } else if (bottom || top) {
+                        if (top) a = gmix(1.0f - ux, ux, gstep(dr, 0.25f));
+                        else a = gmix(1.0f - uy, gclamp(nx - uy, 0.0f, 1.0f), gstep(-2.0f, nx));
+                        if (bottom) b = gmix(1.0f - ux, ux, gstep(dr, 0.25f));
+                        else b = gmix(uy, gclamp(nx - (1.0f - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        return gmax(a, b);
+                    } else {
+                        if (dr == 0.75f) return 0.0f;
+                        top = discf(lx, ly - syy, lz, 0.0f, 1.0f, 0.0f);
+                        bottom = discf(lx, ly + syy, lz, 0.0f, 1.0f, 0.0f);
+                        if (top && bottom) {
+                            return gclamp(nx, 0.0f, 1.0f);
+                        } else {
+                            a = gmix(1.0f - uy, uy, gstep(1.0f, fbool(top)));
+                            b = gmix(1.0f - ux, ux, gstep(dr, 0.25f));
+                            return gmax(a, b);
+                        }
+                    }
+                }
+            }
+            // discontinuity in both axes (corner)
+            if (dr > 0.0f && dg > 0.0f) {
+                lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                const bool horizontal = discf(lx, ly, lz, 1.0f, 0.0f, db);
+                lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * syy;
+                const bool vertical = discf(lx, ly, lz, 0.0f, 1.0f, db);
+                if (horizontal && !vertical) {
+                    dr = 0.0f;
+                } else if (!horizontal && vertical) {
+                    dg = 0.0f;
+                } else if (!horizontal && !vertical) {
+                    if (dg == 0.5f) return gclamp(uy - (1.0f - nx), 0.0f, 1.0f);
+                    else if (dg == 0.25f) return gclamp((1.0f - uy) - (1.0f - nx), 0.0f, 1.0f);
+                } else {
+                    float a, b;
+                    if (dr == 0.5f && dg == 0.5f) {
+                        a = gmix(uy, gclamp(nx - (1.0f - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(1.0f - ux, gclamp(ny - ux, 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmax(a, b);
+                    } else if (dr == 0.5f && dg == 0.25f) {
+                        a = gmix(1.0f - uy, gclamp(nx - uy, 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(1.0f - ux, gclamp(ny - ux, 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmax(a, b);
+                    } else if (dr == 0.25f && dg == 0.5f) {
+                        a = gmix(uy, gclamp(nx - (1.0f - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(ux, gclamp(ny - (1.0f - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmax(a, b);
+                    } else if (dr == 0.25f && dg == 0.25f) {
+                        a = gmix(1.0f - uy, gclamp(nx - uy, 0.0f, 1.0f), gstep(-2.0f, nx));
+                        b = gmix(ux, gclamp(ny - (1.0f - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
+                        return gmax(a, b);
+                    }
+                }
+            }
+            if (nx <= -2.0f) return gmix(uy, 1.0f - uy, gstep(dg, 0.25f));
+            if (ny <= -2.0f) return gmix(1.0f - ux, ux, gstep(dr, 0.25f));
+            if (dg > 0.0f) {
+                if (dg == 0.5f) return gclamp(nx - (1.0f - uy), 0.0f, 1.0f);
+                else return gclamp(nx - uy, 0.0f, 1.0f);
+            }
+            if (dr > 0.0f) {
+                if (dr == 0.5f) return gclamp(ny - ux, 0.0f, 1.0f);
+                else return gclamp(ny - (1.0f - ux), 0.0f, 1.0f);
+            }
+        }
+        return 1.0f - db;
+    }
+
+    // RBSSM :1202-1339 over the penumbra kernel
+    __device__ float kernel(const PcssUniforms& q, float si, float pw, float x, float y, float z) const {
+        float illum = 0.0f;
+        const int h0 = loop_start(q.kfw);
+        for (int h = h0; (float)h <= q.kfw; ++h) {
+            const float ty = y + ((float)h * pw) / q.kfw;
+            for (int w = h0; (float)w <= q.kfw; ++w) {
+                const float tx = x + ((float)w * pw) / q.kfw;
+                const float ux = gfract(tx * rb.D.fw), uy = gfract(ty * rb.D.fh);
+                const float d = tex(tx, ty);
+                float dr, dg, db;
+                rb.discontinuity(tx, ty, z, d, dr, dg, db);
+                if (dr > 0.0f || dg > 0.0f) {
+                    const float left = length(dr, dg, db, tx, ty, z, -1.0f, 0.0f);
+                    const float right = length(dr, dg, db, tx, ty, z, 1.0f, 0.0f);
+                    const float down = length(dr, dg, db, tx, ty, z, 0.0f, -1.0f);
+                    const float up = length(dr, dg, db, tx, ty, z, 0.0f, 1.0f);
+                    const float nx = normalize(left, right, ux), ny = normalize(down, up, uy);
+                    float fill = smooth(tx, ty, z, nx, ny, dr, dg, db, ux, uy);
+                    fill = gmix(fill, 1.0f, si);
+                    illum = illum + fill;
+                } else {
+                    illum = illum + ((z <= d) ? 1.0f : si);
+                }
+            }
+        }
+        return illum / (float)(q.ks * q.ks);
+    }
+};
+
+// main :1362-1378 with revectorizationBasedShadowMappingSmoothing :1341-1360
+__global__ __launch_bounds__(kBlockThreads) void k_rbssm(GBufferView gb, DepthMap D, HardUniforms u, PcssUniforms q,
+                                                          float* __restrict__ out) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t k = (size_t)y * gb.w + x;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {
+        out[k] = 0.0f;
+        return;
+    }
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    float sc[4];
+    mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
+    const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+    float shadow = 1.0f;
+    if (u.pre_eval)
+        shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+    if (sc[3] > 0.0f && shadow == 1.0f) {
+        const float avg = pcss_blocker(D, q, n0, n1, n2);
+        const float pw = pcss_penumbra(q, avg, n2);
+        const float stepSize = 2.0f * pw / (float)q.ks;
+        if (stepSize <= 0.0f || stepSize >= 1.0f) {
+            shadow = 1.0f;
+        } else {
+            Rbssm r{Rbsm<false>{D, u.stepx, u.stepy, u.thr, u.max_search}, 0.0f};
+            shadow = r.kernel(q, u.si, pw, n0, n1, n2);
+        }
+    }
+    out[k] = shadow;
+}
+
+}  // namespace
+}  // namespace gi

@@ -19,6 +19,7 @@
 #include "../../include/gi.h"
 #include "gi_device.hpp"
 #include "screen_kernels.hpp"
+#include "rbssm_kernels.hpp"
 
 namespace gi {
 namespace {
@@ -221,6 +222,53 @@ int gi_pcss_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_pcs
                (float)p->shadow_map_height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_pcss, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u, q, out_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_rbssm_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_rbssm_params* p, float* out_dev,
+                  gi_stream stream) {
+    if (!p) return GI_EINVAL;
+    const gi_pcss_params& c = p->pcss;
+    if (!gbuffer_ok(gbuffer) || !depth_dev || !out_dev) return GI_EINVAL;
+    if (c.shadow_map_width <= 0 || c.shadow_map_height <= 0 || c.shadow_map_width > GI_MAX_EDGE ||
+        c.shadow_map_height > GI_MAX_EDGE)
+        return GI_EINVAL;
+    // per-pixel work is kernel_size^2 taps x 4 walks x max_search steps: both are bounded
+    if (c.blocker_search_size < 2 || c.blocker_search_size > 64 || c.kernel_size < 2 || c.kernel_size > 31)
+        return GI_EINVAL;
+    if (c.light_source_radius < 0 || c.light_source_radius > 4096) return GI_EINVAL;
+    if (p->max_search < 0 || p->max_search > 64) return GI_EINVAL;
+    if (!finite_all(c.light_mvp, 16) || !std::isfinite(c.shadow_intensity) || !std::isfinite(p->depth_threshold))
+        return GI_ENONFINITE;
+    if (c.pre_evaluation) {
+        if (!gbuffer->normal_dev) return GI_EINVAL;
+        if (!finite_all(c.mv, 16) || !finite_all(c.normal_matrix, 9) || !finite_all(c.light_position, 3))
+            return GI_ENONFINITE;
+    }
+    HardUniforms u{};
+    for (int i = 0; i < 16; ++i) { u.light_mvp.m[i] = c.light_mvp[i]; u.mv.m[i] = c.mv[i]; }
+    for (int i = 0; i < 9; ++i) u.nm.m[i] = c.normal_matrix[i];
+    for (int i = 0; i < 3; ++i) u.lpos[i] = c.light_position[i];
+    u.si = c.shadow_intensity;
+    u.pre_eval = c.pre_evaluation ? 1 : 0;
+    u.max_search = p->max_search;
+    u.thr = p->depth_threshold;
+    // glUniform2f(shadowMapStep, 1.0/W, 1.0/H) (MyGLGeometryViewer.cpp:226-227)
+    u.stepx = (float)(1.0 / (double)c.shadow_map_width);
+    u.stepy = (float)(1.0 / (double)c.shadow_map_height);
+    PcssUniforms q;
+    const float R = (float)c.light_source_radius;
+    q.bsw = (c.shadow_map_width <= 1024) ? R / (float)c.shadow_map_width : R / 1024.0f;
+    q.bfw = ((float)c.blocker_search_size - 1.0f) * 0.5f;
+    q.kfw = ((float)c.kernel_size - 1.0f) * 0.5f;
+    q.radius = R;
+    q.znear = (float)c.z_near;
+    q.ks = c.kernel_size;
+    GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
+    DepthMap D{depth_dev, c.shadow_map_width, c.shadow_map_height, (float)c.shadow_map_width,
+               (float)c.shadow_map_height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_rbssm, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u, q, out_dev);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

@@ -384,6 +384,10 @@ int SoftShadowPipeline::renderSoftShadows() {
     p.light_position[1] = lightEye[1];
     p.light_position[2] = -lightEye[2];
     gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
+    if (rbssm) {
+        gi_rbssm_params r{p, maxSearch, depthThreshold};
+        return gi_rbssm_pass(&g, maps_.as<float>(), &r, soft_.as<float>(), stream);
+    }
     return gi_pcss_pass(&g, maps_.as<float>(), &p, soft_.as<float>(), stream);
 }
 

@@ -142,6 +142,12 @@ public:
     // uploads it for that shader, so it is 0 there and PCSS degenerates to
     // "lit" (penumbra width 0); 1 is MyGLGeometryViewer's near plane.
     int pcssZNear = 1;
+    // ShadowParams.RBSSM (menu entry 5, main.cpp:1451-1455, kernelSize 7):
+    // renderSoftShadows runs RBSSM.frag instead of PlausibleSoftShadow.frag
+    // (main.cpp:1012-1013) with the edge walk of maxSearch texels.
+    bool rbssm = false;
+    int maxSearch = 16;                            // main.cpp:1617
+    float depthThreshold = 0.0f;                   // SceneLoader `d`, main.cpp:1618
     int maxLevel = 4;                              // main.cpp:190
     bool preEvaluation = true;                     // PCSS runs the pre-evaluation (:614)
     Vec3 cameraEye{}, cameraAt{}, cameraUp{0.0f, 0.0f, 1.0f};
@@ -155,8 +161,9 @@ public:
     // UniformSampledLightSource placement (eye and look-at shifted together,
     // UniformSampledLightSource.cpp:13-37), the N depth maps, then the screen pass.
     int renderMonteCarlo();
-    // renderSoftShadows with ShadowParams.PCSS (main.cpp:925-1016): the depth
-    // map of the light centre (into map 0) and PlausibleSoftShadow.frag.
+    // renderSoftShadows with ShadowParams.PCSS or .RBSSM (main.cpp:925-1016):
+    // the depth map of the light centre (into map 0), then
+    // PlausibleSoftShadow.frag or RBSSM.frag.
     int renderSoftShadows();
     // renderAdaptiveLightSourceSampling (main.cpp:813-892) without the
     // revectorization variant: the quad tree over the light is refined where

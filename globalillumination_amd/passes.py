@@ -281,6 +281,37 @@ def render_soft_shadows(gbuf: GBuffer, depth: torch.Tensor, params: PCSSParams, 
     return out
 
 
+@dataclass
+class RBSSMParams(PCSSParams):
+    """The ShadowParams fields RBSSM.frag reads (SoftShadowMapping/src/main.cpp:
+    1452-1454 selects it with kernelSize 7; maxSearch 16 at :1617, the
+    depth threshold is the scene's `d` line).  As with PCSS, the app never
+    uploads zNear, so its value there is 0 and every tested pixel is lit."""
+    kernelSize: int = 7
+    maxSearch: int = 16
+    depthThreshold: float = 0.0
+
+    def to_c(self) -> _lib.GiRbssmParams:
+        p = _lib.GiRbssmParams()
+        p.pcss = PCSSParams.to_c(self)
+        p.max_search = int(self.maxSearch)
+        p.depth_threshold = float(self.depthThreshold)
+        return p
+
+
+def render_rbssm(gbuf: GBuffer, depth: torch.Tensor, params: RBSSMParams, out: torch.Tensor | None = None,
+                 stream=None) -> torch.Tensor:
+    """renderSoftShadows' screen pass with RBSSM (RBSSM.frag): PCSS's blocker
+    search and penumbra width, then a kernelSize^2 kernel whose taps on a
+    shadow-map discontinuity are revectorized; returns the R channel."""
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=gbuf.position.device)
+    g = gbuf.to_c(with_normals=params.preEvaluation)
+    p = params.to_c()
+    _lib.call("gi_rbssm_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out), _stream_handle(stream))
+    return out
+
+
 # G-buffer colour of a mesh without per-vertex colours: GBuffer.frag:27 reads
 # gl_FrontLightModelProduct.sceneColor, the default material ambient (0.2) times
 # the light-model ambient (0.1, MyGLGeometryViewer.cpp:69), alpha 1.

@@ -263,6 +263,23 @@ typedef struct gi_pcss_params {
 int gi_pcss_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_pcss_params* params, float* out_dev,
                  gi_stream stream);
 
+/* renderSoftShadows with ShadowParams.RBSSM (SoftShadowMapping/src/main.cpp:
+ * 1012): RBSSM.frag, PCSS's blocker search and penumbra width with a
+ * kernelSize x kernelSize kernel whose taps on a shadow-map discontinuity are
+ * revectorized (computeDiscontinuityLength :193-253, normalizeDiscontinuitySpace
+ * :255-309, smoothONDS :311-1117, RBSSM :1202-1339, main :1362-1378).  The
+ * pcss fields keep their meaning (kernel_size 7 in the reference's menu,
+ * main.cpp:1454; z_near as documented there, 0 in the reference app). */
+typedef struct gi_rbssm_params {
+    gi_pcss_params pcss;
+    int max_search;         /* edge walk length: 16 (main.cpp:1617); 0..64 */
+    float depth_threshold;  /* SceneLoader `d` (main.cpp:1618) */
+} gi_rbssm_params;
+
+/* out_dev: width*height R channel (0 for background).  kernel_size 2..31. */
+int gi_rbssm_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_rbssm_params* params, float* out_dev,
+                  gi_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Deferred Phong shading                                                     */
 /* ------------------------------------------------------------------------ */

@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
             "or_gbuffer": [vp, vp, i, vp, i, vp, i, i, vp, vp, vp],
             "or_phong": [vp, vp, vp, i, i, vp, vp, vp, vp],
             "or_pcss": [vp, vp, i, i, vp, vp, i, i, i, i, vp],
+            "or_rbssm": [vp, vp, i, i, vp, vp, i, i, i, i, vp],
+            "or_rbssm_tap": [vp, vp, f, f, f, vp],
             "or_soft_shadow_adaptive": [vp, i, i, vp, i, i, i, vp, vp, ctypes.c_float, i, i, ctypes.c_float, i, vp],
             "or_qt_node_test": [vp, i, i, vp, i, i, vp, vp, i, vp, vp],
             "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
@@ -163,6 +165,35 @@ def hard_shadow(pos: np.ndarray, nrm: np.ndarray | None, depth: np.ndarray, para
     if rc:
         raise RuntimeError(f"or_hard_shadow failed ({rc})")
     return out
+
+
+def rbssm(pos, nrm, depth, params, blocker_search_size=7, kernel_size=7, light_source_radius=8, z_near=1):
+    """RBSSM.frag over a G-buffer; `params` from make_params (light_mvp biased,
+    map size, shadow intensity, pre-evaluation, max_search, depth_threshold)."""
+    pos = _c32(pos)
+    h, w = pos.shape[1], pos.shape[2]
+    d = _c32(depth)
+    assert d.shape == (params.sm_height, params.sm_width)
+    n = None if nrm is None else _c32(nrm)
+    out = np.empty((h, w), np.float32)
+    rc = lib().or_rbssm(_p(pos), None if n is None else _p(n), w, h, _p(d), ctypes.byref(params),
+                        int(blocker_search_size), int(kernel_size), int(light_source_radius), int(z_near), _p(out))
+    if rc:
+        raise RuntimeError(f"or_rbssm failed ({rc})")
+    return out
+
+
+def rbssm_tap(depth, params, x, y, z):
+    """One RBSSM tap: dict of the discontinuity (r, g, b), the four edge-walk
+    lengths, the normalized discontinuity and smoothONDS's fill."""
+    d = _c32(depth)
+    assert d.shape == (params.sm_height, params.sm_width)
+    out = np.empty(10, np.float32)
+    rc = lib().or_rbssm_tap(_p(d), ctypes.byref(params), float(x), float(y), float(z), _p(out))
+    if rc:
+        raise RuntimeError(f"or_rbssm_tap failed ({rc})")
+    keys = ("r", "g", "b", "left", "right", "down", "up", "nx", "ny", "fill")
+    return {k: float(v) for k, v in zip(keys, out)}
 
 
 def pcss(pos, nrm, depth, params, blocker_search_size=7, kernel_size=15, light_source_radius=8, z_near=1):

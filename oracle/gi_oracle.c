@@ -1391,3 +1391,472 @@ int or_pcss(const float* pos, const float* nrm, int w, int h, const float* depth
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* RBSSM: SoftShadowMapping/Shaders/SoftShadow/RBSSM.frag                     */
+/* ------------------------------------------------------------------------ */
+
+static inline float gl_step(float edge, float x) { return x < edge ? 0.0f : 1.0f; }
+
+typedef struct rbssm_ctx {
+    rb_ctx rb;
+    float newDepth; /* the shader global written by getDisc(vec4 discType) */
+} rbssm_ctx;
+
+static int ss_discf(rbssm_ctx* c, float x, float y, float z, float dx, float dy, float type)
+{
+    return get_disc_f(&c->rb, x, y, z, dx, dy, type);
+}
+
+static int ss_discv(rbssm_ctx* c, float x, float y, float z, float dx, float dy, float r, float g, float b)
+{
+    return get_disc_v(&c->rb, x, y, z, dx, dy, r, g, b, &c->newDepth);
+}
+
+static float ss_compress(float nd) { return -2.0f - ((0.5f - nd) * 2.0f); }          /* :25-29 */
+static float ss_decompress(float nd) { return 0.5f - ((nd + 2.0f) * -1.0f) / 2.0f; } /* :31-35 */
+
+/* computeDiscontinuityLength :193-253 */
+static float ss_length(rbssm_ctx* c, float ir, float ig, float ib, float x, float y, float z, float dx, float dy)
+{
+    float found = 0.0f;
+    if (dx == 0.0f && ir == 0.0f && ig != 0.0f) return -1.0f;
+    if (dy == 0.0f && ir != 0.0f && ig == 0.0f) return -1.0f;
+    if (((0.5f - ir) * 8.0f - 1.0f) == dx) return 1.0f;
+    if (((ig - 0.5f) * 8.0f + 1.0f) == dy) return 1.0f;
+    float dist = 1.0f;
+    float stx = dx * c->rb.stepx, sty = dy * c->rb.stepy;
+    float cx = x + stx, cy = y + sty, cz = z;
+    for (int it = 0; it < c->rb.maxSearch; ++it) {
+        float d = rb_tex(&c->rb, cx, cy);
+        if (ib == 0.0f && fabsf(cz - d) < c->rb.thr) cz = cz - c->rb.thr;
+        float center = cmp_le(cz, d);
+        if (fabsf(center - ib) == 0.0f) {
+            found = ss_discf(c, cx, cy, cz, 0.0f, 0.0f, ib) ? 1.0f : 0.0f;
+            break;
+        }
+        if (!ss_discv(c, cx, cy, cz, dx, dy, ir, ig, ib)) break;
+        dist = dist + 1.0f;
+        cx = cx + stx;
+        cy = cy + sty;
+        if (ib == 1.0f) cz = c->newDepth;
+    }
+    return gmix(-dist, dist, found);
+}
+
+/* normalizeDiscontinuitySpace :255-280 */
+static float ss_normalize(const rbssm_ctx* c, float a, float b, float sub)
+{
+    if (a < 0.0f && b < 0.0f) return -1.0f;
+    float len = gmin(fabsf(a) + fabsf(b) - 1.0f, (float)c->rb.maxSearch);
+    float nd = 1.0f - gmax(a, b) / len;
+    if (a == b) nd = nd + gmix(sub / len, (1.0f - sub) / len, gl_step(0.5f, sub));
+    else if (a == gmax(a, b)) nd = nd + (1.0f - sub) / len;
+    else nd = nd + sub / len;
+    if (a > 0.0f && b > 0.0f) return ss_compress(nd);
+    return nd;
+}
+
+#define CL01(v) gclamp((v), 0.0f, 1.0f)
+
+/* smoothONDS :311-1117, with its arguments as by-value copies */
+static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, float ny, float dr, float dg, float db,
+                       float ux, float uy)
+{
+    const float sx = c->rb.stepx, sy = c->rb.stepy;
+    if (db == 0.0f) {
+        /* :316-421 positive entering discontinuity on both axes */
+        if (nx <= -2.0f && ny <= -2.0f && (dr == 0.75f || dg == 0.75f)) {
+            int left = 1, right = 1, bottom = 1, top = 1;
+            if (dr == 0.75f) {
+                if (dg == 0.0f) {
+                    ly = ly + sy;
+                    top = ss_discf(c, lx, ly, lz, 0.0f, 1.0f, db);
+                    ly = ly - 2.0f * sy;
+                    bottom = ss_discf(c, lx, ly, lz, 0.0f, 1.0f, db);
+                    ly = ly + sy;
+                    if (top && bottom) return 0.0f;
+                    ly = ly + gmix(-sy, sy, gl_step(1.0f, top ? 1.0f : 0.0f));
+                    left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    right = !left;
+                } else {
+                    if (dg == 0.5f) top = 0;
+                    else if (dg == 0.25f) bottom = 0;
+                    ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * sy;
+                    left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    right = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    if (left && right) {
+                        if (ny <= -2.0f) ny = ss_decompress(ny) + 0.5f;
+                        float a = CL01(ux - (ny - 1.0f));
+                        float b = CL01((1.0f - ux) - (ny - 1.0f));
+                        float cc = gmix(1.0f - uy, uy, gl_step(dg, 1.0f));
+                        return gmin(gmin(a, b), cc);
+                    }
+                }
+            }
+            if (dg == 0.75f) {
+                if (dr == 0.0f) {
+                    lx = lx - sx;
+                    left = ss_discf(c, lx, ly, lz, 1.0f, 0.0f, db);
+                    lx = lx + 2.0f * sx;
+                    right = ss_discf(c, lx, ly, lz, 1.0f, 0.0f, db);
+                    lx = lx - sx;
+                    if (left && right) return 0.0f;
+                    lx = lx + gmix(sx, -sx, gl_step(1.0f, left ? 1.0f : 0.0f));
+                    bottom = ss_discv(c, lx, ly, lz, 0.0f, 0.25f, 0.5f, 0.0f, 0.0f);
+                    top = !bottom;
+                } else {
+                    if (dr == 0.5f) right = 0;
+                    else if (dr == 0.25f) left = 0;
+                    lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
+                    bottom = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                    top = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    if (bottom && top) {
+                        if (nx <= -2.0f) nx = ss_decompress(nx) + 0.5f;
+                        float a = CL01(uy - (nx - 1.0f));
+                        float b = CL01((1.0f - uy) - (nx - 1.0f));
+                        float cc = gmix(1.0f - ux, ux, gl_step(dr, 1.0f));
+                        return gmin(gmin(a, b), cc);
+                    }
+                }
+            }
+            if (!left && !bottom) return CL01((1.0f - ux) - (1.0f - uy));
+            else if (!right && !bottom) return CL01(ux - (1.0f - uy));
+            else if (!left && !top) return CL01((1.0f - uy) - ux);
+            else if (!right && !top) return CL01((1.0f - uy) - (1.0f - ux));
+        }
+        if (dr == 0.75f || dg == 0.75f) {
+            if (dr == 0.75f && dg != 0.0f) { /* :429-455 */
+                ly = ly + ((dg - 0.75f) * 4.0f) * sy;
+                int left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                int right = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                if (!left && !right) return CL01(1.0f - ny);
+                float sub = gmix(1.0f - uy, uy, gl_step(1.0f, dg));
+                float a = gmix(sub, CL01((1.0f - ux) - (ny - 1.0f)), gl_step(1.0f, right ? 1.0f : 0.0f));
+                float b = gmix(sub, CL01(ux - (ny - 1.0f)), gl_step(1.0f, left ? 1.0f : 0.0f));
+                return gmin(a, b);
+            }
+            if (dr == 0.75f && dg == 0.0f) { /* :458-545 */
+                float ty = ly + sy;
+                int topCenter = !(cmp_le(lz, rb_tex(&c->rb, lx, ty)) != 0.0f);
+                int topLeft = ss_discv(c, lx, ty, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                int topRight = ss_discv(c, lx, ty, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                float by = ly - sy;
+                int bottomCenter = !(cmp_le(lz, rb_tex(&c->rb, lx, by)) != 0.0f);
+                int bottomLeft = ss_discv(c, lx, by, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                int bottomRight = ss_discv(c, lx, by, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                if (topCenter) { topLeft = 1; topRight = 1; }
+                if (bottomCenter) { bottomLeft = 1; bottomRight = 1; }
+                if (ny <= -2.0f) {
+                    ny = ss_decompress(ny) + 0.5f;
+                    float a = CL01(ux - (ny - 1.0f));
+                    float b = CL01((1.0f - ux) - (ny - 1.0f));
+                    if (!bottomRight || !topRight) return a;
+                    else if (!bottomLeft || !topLeft) return b;
+                    else return gmin(a, b);
+                }
+                if ((!bottomRight && !bottomLeft) || (!topRight && !topLeft)) return CL01(1.0f - ny);
+                else if (!bottomRight || !topRight) return CL01(ux - (ny - 1.0f));
+                else return CL01((1.0f - ux) - (ny - 1.0f));
+            }
+            if (dr != 0.0f && dg == 0.75f) { /* :548-575 */
+                lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
+                int bottom = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                int top = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                if (!bottom && !top) return CL01(1.0f - nx);
+                float sub = gmix(ux, 1.0f - ux, gl_step(dr, 0.25f));
+                float a = gmix(sub, CL01(uy - (nx - 1.0f)), gl_step(1.0f, top ? 1.0f : 0.0f));
+                float b = gmix(sub, CL01((1.0f - uy) - (nx - 1.0f)), gl_step(1.0f, bottom ? 1.0f : 0.0f));
+                return gmin(a, b);
+            }
+            if (dr == 0.0f && dg == 0.75f) { /* :578-662 */
+                float rx = lx + sx;
+                int rightCenter = !(cmp_le(lz, rb_tex(&c->rb, rx, ly)) != 0.0f);
+                int bottomRight = ss_discv(c, rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                int topRight = ss_discv(c, rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                float lxx = lx - sx;
+                int leftCenter = !(cmp_le(lz, rb_tex(&c->rb, lxx, ly)) != 0.0f);
+                int bottomLeft = ss_discv(c, lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                int topLeft = ss_discv(c, lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                if (rightCenter) { bottomRight = 1; topRight = 1; }
+                if (leftCenter) { topLeft = 1; bottomLeft = 1; }
+                if (nx <= -2.0f) {
+                    nx = ss_decompress(nx) + 0.5f;
+                    float a = CL01(uy - (nx - 1.0f));
+                    float b = CL01((1.0f - uy) - (nx - 1.0f));
+                    if (!bottomRight || !bottomLeft) return a;
+                    else if (!topRight || !topLeft) return b;
+                    else return gmin(a, b);
+                }
+                if ((!bottomRight && !topRight) || (!bottomLeft && !topLeft)) return CL01(1.0f - nx);
+                else if (!bottomRight || !bottomLeft) return CL01(uy - (nx - 1.0f));
+                else return CL01((1.0f - uy) - (nx - 1.0f));
+            }
+        }
+        if (dr > 0.0f && dg > 0.0f) { /* :667-722 corner */
+            lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
+            int horizontal = ss_discf(c, lx, ly, lz, 1.0f, 0.0f, 0.0f);
+            lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sx;
+            ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * sy;
+            int vertical = ss_discf(c, lx, ly, lz, 0.0f, 1.0f, 0.0f);
+            if (horizontal && !vertical) {
+                dr = 0.0f;
+            } else if (!horizontal && vertical) {
+                dg = 0.0f;
+            } else if (!horizontal && !vertical) {
+                if (dg == 0.5f) return CL01((1.0f - nx) - (uy - 1.0f));
+                else if (dg == 0.25f) return CL01((1.0f - nx) + uy);
+            } else {
+                float a, b;
+                if (dr == 0.5f && dg == 0.5f) {
+                    a = gmix(1.0f - uy, CL01(1.0f - (nx - (1.0f - uy))), gl_step(-2.0f, nx));
+                    b = gmix(ux, CL01(1.0f - (ny - ux)), gl_step(-2.0f, ny));
+                    return gmin(a, b);
+                } else if (dr == 0.5f && dg == 0.25f) {
+                    a = gmix(uy, CL01(1.0f - (nx - uy)), gl_step(-2.0f, nx));
+                    b = gmix(ux, CL01(1.0f - (ny - ux)), gl_step(-2.0f, ny));
+                    return gmin(a, b);
+                } else if (dr == 0.25f && dg == 0.5f) {
+                    a = gmix(1.0f - uy, CL01(1.0f - (nx - (1.0f - uy))), gl_step(-2.0f, nx));
+                    b = gmix(1.0f - ux, CL01(1.0f - (ny - (1.0f - ux))), gl_step(-2.0f, ny));
+                    return gmin(a, b);
+                } else if (dr == 0.25f && dg == 0.25f) {
+                    a = gmix(uy, CL01(1.0f - (nx - uy)), gl_step(-2.0f, nx));
+                    b = gmix(1.0f - ux, CL01(1.0f - (ny - (1.0f - ux))), gl_step(-2.0f, ny));
+                    return gmin(a, b);
+                }
+            }
+        }
+        if (nx <= -2.0f) return gmix(1.0f - uy, uy, gl_step(dg, 0.25f));
+        if (ny <= -2.0f) return gmix(ux, 1.0f - ux, gl_step(dr, 0.25f));
+        if (dg > 0.0f) return dg == 0.5f ? CL01((1.0f - uy) - (nx - 1.0f)) : CL01(uy - (nx - 1.0f));
+        if (dr > 0.0f) return dr == 0.5f ? CL01(ux - (ny - 1.0f)) : CL01((1.0f - ux) - (ny - 1.0f));
+    } else { /* :758-1113 exiting discontinuity */
+        if (dr == 0.75f || dg == 0.75f) {
+            if (dr == 0.75f && dg == 0.0f) { /* :763-818 */
+                int left = ss_discv(c, lx - sx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                int right = ss_discv(c, lx + sx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                if (left && right) return CL01(ny);
+                if (left || right) {
+                    if (!left) return gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
+                    return gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
+                }
+                float ry = ly + 1.0f * sy; /* count 0: mult 1, even */
+                left = ss_discf(c, lx - sx, ry, lz, 1.0f, 0.0f, 0.0f);
+                right = ss_discf(c, lx + sx, ry, lz, 1.0f, 0.0f, 0.0f);
+                if (left && right) return CL01(ny);
+                if (!left) return gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
+                return gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
+            }
+            if (dr == 0.75f && dg != 0.0f) { /* :821-878 */
+                ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * sy;
+                int left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                int right = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                float a, b;
+                if (left && right) return CL01(ny);
+                if (left || right) {
+                    if (left) a = gmix(uy, 1.0f - uy, gl_step(dg, 0.25f));
+                    else a = gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
+                    if (right) b = gmix(uy, 1.0f - uy, gl_step(dg, 0.25f));
+                    else b = gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
+                    return gmax(a, b);
+                }
+                if (dg == 0.75f) return 0.0f;
+                left = ss_discf(c, lx - sx, ly, lz, 1.0f, 0.0f, 0.0f);
+                right = ss_discf(c, lx + sx, ly, lz, 1.0f, 0.0f, 0.0f);
+                if (left && right) return CL01(ny);
+                a = gmix(1.0f - ux, ux, gl_step(1.0f, left ? 1.0f : 0.0f));
+                b = gmix(uy, 1.0f - uy, gl_step(dg, 0.25f));
+                return gmax(a, b);
+            }
+            if (dr == 0.0f && dg == 0.75f) { /* :881-936 */
+                int top = ss_discf(c, lx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
+                int bottom = ss_discf(c, lx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                if (bottom && top) return CL01(nx);
+                if (bottom || top) {
+                    if (!top) return gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
+                    return gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
+                }
+                float rx = lx + 1.0f * sx; /* count 0: mult 1, even */
+                top = ss_discf(c, rx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
+                bottom = ss_discf(c, rx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                if (top && bottom) return CL01(nx);
+                if (!top) return gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
+                return gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
+            }
+            if (dr != 0.0f && dg == 0.75f) { /* :939-997 */
+                lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
+                int bottom = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                int top = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                float a, b;
+                if (bottom && top) return CL01(nx);
+                if (bottom || top) {
+                    if (top) a = gmix(1.0f - ux, ux, gl_step(dr, 0.25f));
+                    else a = gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
+                    if (bottom) b = gmix(1.0f - ux, ux, gl_step(dr, 0.25f));
+                    else b = gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
+                    return gmax(a, b);
+                }
+                if (dr == 0.75f) return 0.0f;
+                top = ss_discf(c, lx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
+                bottom = ss_discf(c, lx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                if (top && bottom) return CL01(nx);
+                a = gmix(1.0f - uy, uy, gl_step(1.0f, top ? 1.0f : 0.0f));
+                b = gmix(1.0f - ux, ux, gl_step(dr, 0.25f));
+                return gmax(a, b);
+            }
+        }
+        if (dr > 0.0f && dg > 0.0f) { /* :1002-1061 corner */
+            lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sx;
+            int horizontal = ss_discf(c, lx, ly, lz, 1.0f, 0.0f, db);
+            lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sx;
+            ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * sy;
+            int vertical = ss_discf(c, lx, ly, lz, 0.0f, 1.0f, db);
+            if (horizontal && !vertical) {
+                dr = 0.0f;
+            } else if (!horizontal && vertical) {
+                dg = 0.0f;
+            } else if (!horizontal && !vertical) {
+                if (dg == 0.5f) return CL01(uy - (1.0f - nx));
+                else if (dg == 0.25f) return CL01((1.0f - uy) - (1.0f - nx));
+            } else {
+                float a, b;
+                if (dr == 0.5f && dg == 0.5f) {
+                    a = gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
+                    b = gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
+                    return gmax(a, b);
+                } else if (dr == 0.5f && dg == 0.25f) {
+                    a = gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
+                    b = gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
+                    return gmax(a, b);
+                } else if (dr == 0.25f && dg == 0.5f) {
+                    a = gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
+                    b = gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
+                    return gmax(a, b);
+                } else if (dr == 0.25f && dg == 0.25f) {
+                    a = gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
+                    b = gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
+                    return gmax(a, b);
+                }
+            }
+        }
+        if (nx <= -2.0f) return gmix(uy, 1.0f - uy, gl_step(dg, 0.25f));
+        if (ny <= -2.0f) return gmix(1.0f - ux, ux, gl_step(dr, 0.25f));
+        if (dg > 0.0f) return dg == 0.5f ? CL01(nx - (1.0f - uy)) : CL01(nx - uy);
+        if (dr > 0.0f) return dr == 0.5f ? CL01(ny - ux) : CL01(ny - (1.0f - ux));
+    }
+    return 1.0f - db;
+}
+
+#undef CL01
+
+/* revectorizationBasedShadowMappingSmoothing :1341-1360 with RBSSM :1202-1339 */
+static float rbssm_pixel(rbssm_ctx* c, const or_params* p, int bss, int ks, int radius, int znear, float x, float y,
+                         float z)
+{
+    const float* D = c->rb.D;
+    int W = c->rb.W, H = c->rb.H;
+    /* computeAverageBlockerDepthBasedOnPCF :1149-1176 (as PlausibleSoftShadow.frag) */
+    float bsw = (W <= 1024) ? (float)radius / (float)W : (float)radius / 1024.0f;
+    float fw = ((float)bss - 1.0f) * 0.5f;
+    float sum = 0.0f;
+    int blockers = 0;
+    for (int hh = (int)(-fw); (float)hh <= fw; hh++)
+        for (int ww = (int)(-fw); (float)ww <= fw; ww++) {
+            float d = or_fetch(D, W, H, x + ((float)ww * bsw) / fw, y + ((float)hh * bsw) / fw);
+            if (z > d) { sum = sum + d; blockers++; }
+        }
+    float avg = blockers == 0 ? 1.0f : sum / (float)blockers;
+    float pw = 0.0f; /* computePenumbraWidth :1178-1187 */
+    if (!(avg < 0.99f)) pw = ((float)znear * (((z - avg) / avg) * (float)radius)) / z;
+    float stepSize = 2.0f * pw / (float)ks;
+    if (stepSize <= 0.0f || stepSize >= 1.0f) return 1.0f;
+    float kfw = ((float)ks - 1.0f) * 0.5f;
+    float illum = 0.0f;
+    for (int hh = (int)(-kfw); (float)hh <= kfw; hh++) {
+        float ty = y + ((float)hh * pw) / kfw;
+        for (int ww = (int)(-kfw); (float)ww <= kfw; ww++) {
+            float tx = x + ((float)ww * pw) / kfw;
+            float ux = gfract(tx * (float)W), uy = gfract(ty * (float)H);
+            float d = or_fetch(D, W, H, tx, ty);
+            float dr, dg, db;
+            compute_discontinuity(&c->rb, tx, ty, z, d, &dr, &dg, &db);
+            if (dr > 0.0f || dg > 0.0f) {
+                float l = ss_length(c, dr, dg, db, tx, ty, z, -1.0f, 0.0f);
+                float r = ss_length(c, dr, dg, db, tx, ty, z, 1.0f, 0.0f);
+                float dn = ss_length(c, dr, dg, db, tx, ty, z, 0.0f, -1.0f);
+                float up = ss_length(c, dr, dg, db, tx, ty, z, 0.0f, 1.0f);
+                float nx = ss_normalize(c, l, r, ux), ny = ss_normalize(c, dn, up, uy);
+                float fill = ss_smooth(c, tx, ty, z, nx, ny, dr, dg, db, ux, uy);
+                fill = gmix(fill, 1.0f, p->shadow_intensity);
+                illum = illum + fill;
+            } else {
+                illum = illum + ((z <= d) ? 1.0f : p->shadow_intensity);
+            }
+        }
+    }
+    return illum / (float)(ks * ks);
+}
+
+int or_rbssm(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
+             int blocker_search_size, int kernel_size, int light_source_radius, int z_near, float* out)
+{
+    if (!pos || !depth || !p || !out || w <= 0 || h <= 0) return -1;
+    if (p->pre_evaluation && !nrm) return -1;
+    if (blocker_search_size < 2 || kernel_size < 2) return -1;
+    size_t plane = (size_t)w * (size_t)h;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { out[k] = 0.0f; continue; }
+        float sc[4];
+        m4v(p->light_mvp, vx, vy, vz, 1.0f, sc);
+        float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        float shadow = 1.0f;
+        if (p->pre_evaluation)
+            shadow = pre_evaluation(p, vx, vy, vz, nrm[k], nrm[plane + k], nrm[2 * plane + k]);
+        if (sc[3] > 0.0f && shadow == 1.0f) {
+            rbssm_ctx c;
+            c.rb.D = depth; c.rb.W = p->sm_width; c.rb.H = p->sm_height;
+            c.rb.stepx = (float)(1.0 / (double)p->sm_width);
+            c.rb.stepy = (float)(1.0 / (double)p->sm_height);
+            c.rb.thr = p->depth_threshold;
+            c.rb.maxSearch = p->max_search;
+            c.rb.si = p->shadow_intensity;
+            c.rb.filtered = 0;
+            c.newDepth = 0.0f;
+            shadow = rbssm_pixel(&c, p, blocker_search_size, kernel_size, light_source_radius, z_near, n0, n1, n2);
+        }
+        out[k] = shadow;
+    }
+    return 0;
+}
+
+int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float z, float* out)
+{
+    if (!depth || !p || !out) return -1;
+    rbssm_ctx c;
+    c.rb.D = depth; c.rb.W = p->sm_width; c.rb.H = p->sm_height;
+    c.rb.stepx = (float)(1.0 / (double)p->sm_width);
+    c.rb.stepy = (float)(1.0 / (double)p->sm_height);
+    c.rb.thr = p->depth_threshold;
+    c.rb.maxSearch = p->max_search;
+    c.rb.si = p->shadow_intensity;
+    c.rb.filtered = 0;
+    c.newDepth = 0.0f;
+    float ux = gfract(x * (float)c.rb.W), uy = gfract(y * (float)c.rb.H);
+    float dr, dg, db;
+    compute_discontinuity(&c.rb, x, y, z, rb_tex(&c.rb, x, y), &dr, &dg, &db);
+    out[0] = dr; out[1] = dg; out[2] = db;
+    out[3] = ss_length(&c, dr, dg, db, x, y, z, -1.0f, 0.0f);
+    out[4] = ss_length(&c, dr, dg, db, x, y, z, 1.0f, 0.0f);
+    out[5] = ss_length(&c, dr, dg, db, x, y, z, 0.0f, -1.0f);
+    out[6] = ss_length(&c, dr, dg, db, x, y, z, 0.0f, 1.0f);
+    float nx = ss_normalize(&c, out[3], out[4], ux), ny = ss_normalize(&c, out[5], out[6], uy);
+    out[7] = nx; out[8] = ny;
+    out[9] = ss_smooth(&c, x, y, z, nx, ny, dr, dg, db, ux, uy);
+    return 0;
+}

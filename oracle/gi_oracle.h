@@ -9,6 +9,11 @@
  *
  * Every function cites the reference file:line it restates.  Paths are relative
  * to the reference repository root (MarcioCerqueira/GlobalIllumination).
+ *
+ * Parity unpinned: or_rbssm's smoothONDS fill at edge ends, corners and
+ * one-texel features (RBSSM.frag:311-1117) has no closed form to check it
+ * against; its straight-edge branches, edge walks and normalization are
+ * pinned, the rest is a branch-by-branch transcription.
  */
 #ifndef GI_ORACLE_H
 #define GI_ORACLE_H
@@ -110,6 +115,18 @@ int or_qt_node_test(const float* pos, int w, int h, const float* maps, int smw, 
  * sm_width/sm_height are read from `p`. */
 int or_pcss(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
             int blocker_search_size, int kernel_size, int light_source_radius, int z_near, float* out);
+
+/* RBSSM.frag (main :1362-1378): PCSS's blocker search and penumbra width,
+ * then the kernel_size^2 kernel with revectorized discontinuity taps.
+ * p->max_search and p->depth_threshold are the edge walks' parameters. */
+int or_rbssm(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
+             int blocker_search_size, int kernel_size, int light_source_radius, int z_near, float* out);
+
+/* One RBSSM tap at shadow-map coordinate (x, y) and receiver depth z:
+ * out[0..2] computeDiscontinuity's (r, g, b), out[3..6] the edge-walk lengths
+ * left/right/down/up, out[7..8] the normalized discontinuity (x, y),
+ * out[9] smoothONDS's fill before the shadow-intensity mix. */
+int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float z, float* out);
 
 /* PhongShading.frag:11-46.  pos/nrm: G-buffer planes (normal flipped on back
  * faces), fac: 1.0 front / 0.0 back, shadow: w*h R channel, color: RGBA per

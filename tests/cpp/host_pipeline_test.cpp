@@ -178,6 +178,19 @@ int main(int argc, char** argv) {
     dumpv(dir, "pcss_map.f32", maps);
     dump(dir + "/pcss_light_mvp.f32", soft.lightMVP().data(), 64);
 
+    // the same frame with ShadowParams.RBSSM: kernel 7, maxSearch 16
+    soft.rbssm = true;
+    soft.kernelSize = 7;
+    soft.depthThreshold = 2.5e-5f;
+    rc = soft.renderSoftShadows();
+    if (rc) return fail("renderSoftShadows(RBSSM)", rc);
+    std::vector<float> rbssm;
+    rc = soft.download(rbssm);
+    if (rc) return fail("download", rc);
+    dumpv(dir, "rbssm.f32", rbssm);
+    soft.rbssm = false;
+    soft.kernelSize = 15;
+
     // renderAdaptiveLightSourceSampling: a 2 x 2 light at (4, 8, 4), maximum
     // level 4, exact evaluation (div 1) and the lower-accuracy variant
     // (4x4 PCF per sample, evaluation on a quarter-resolution grid)

@@ -119,7 +119,8 @@ def test_host_render_monte_carlo(dumps, oracle_lib, gi):
 def test_host_render_soft_shadows_pcss(dumps, oracle_lib, gi):
     """SoftShadowPipeline::renderSoftShadows: light-centre depth map with
     glPolygonOffset(4, 20), then PCSS (blocker search 7, kernel 15, radius
-    16/2, zNear 1) with the pre-evaluation."""
+    16/2, zNear 1) with the pre-evaluation, and RBSSM (kernel 7) over the
+    same map."""
     from globalillumination_amd import camera
 
     pos, nrm = load(dumps, "mesh_pos.f32"), load(dumps, "mesh_nrm.f32")
@@ -144,6 +145,14 @@ def test_host_render_soft_shadows_pcss(dumps, oracle_lib, gi):
     assert (got.view(np.uint32) == want.view(np.uint32)).all()
     geo = gp[0] != 0
     assert (want[geo] < 1.0).mean() > 0.01  # the sphere's shadow is there
+    # ShadowParams.RBSSM on the same map: kernel 7, maxSearch 16, threshold 2.5e-5
+    p = oracle_lib.make_params(camera.mat4_mul(camera.bias_matrix(0.0), light), SM, SM, "naive",
+                               shadow_intensity=0.25, max_search=16, depth_threshold=2.5e-5, pre_evaluation=True,
+                               mv=mv, normal_matrix=nm, light_position=(-60.0, 120.0, -60.0))
+    want = oracle_lib.rbssm(gp, gn, depth, p, blocker_search_size=7, kernel_size=7, light_source_radius=8, z_near=1)
+    got = load(dumps, "rbssm.f32").reshape(H, W)
+    assert (got.view(np.uint32) == want.view(np.uint32)).all()
+    assert (want[geo] < 1.0).mean() > 0.01
 
 
 # --------------------------------------------------------------------------

@@ -399,6 +399,70 @@ def test_pcss_rejects(gi, cuda):
         gi._lib.GI_ENONFINITE
 
 
+# depth thresholds: the `d` values of the reference's scene configurations
+@pytest.mark.parametrize("sm,radius,znear,pre_eval,max_search,thr", [(1024, 8, 1, True, 16, 2.5e-6),
+                                                                     (2048, 64, 1, False, 16, 2.5e-5),
+                                                                     (2048, 256, 1, True, 16, 2.5e-6),
+                                                                     (1024, 64, 1, True, 4, 2.5e-5),
+                                                                     (1024, 64, 0, True, 16, 0.0)])
+def test_rbssm_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval, max_search, thr):
+    wl = _pcss_scene(gi, sm)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)
+    depth = gi.render_shadow_map(dm, mvps, sm, sm, polygon_offset=(4.0, 20.0))
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    params = gi.RBSSMParams(lightMVP=mvps[0], shadowMapWidth=sm, shadowMapHeight=sm, lightSourceRadius=radius,
+                            zNear=znear, preEvaluation=pre_eval, MV=MV, normalMatrix=nm, lightPosition=lpos,
+                            maxSearch=max_search, depthThreshold=thr)
+    print(f"rbssm sm={sm} radius={radius} zNear={znear} maxSearch={max_search}", flush=True)
+    got = _np(gi.render_rbssm(g, depth[0], params))
+    op = oracle_lib.make_params(gi.biased(mvps[0]), sm, sm, "naive", shadow_intensity=0.25, max_search=max_search,
+                                depth_threshold=thr, pre_evaluation=pre_eval, mv=MV, normal_matrix=nm,
+                                light_position=lpos)
+    want = oracle_lib.rbssm(_np(g.position), _np(g.normal) if pre_eval else None, _np(depth[0]), op,
+                            blocker_search_size=7, kernel_size=7, light_source_radius=radius, z_near=znear)
+    assert bits_equal(got, want), mismatch_report(got, want, f"rbssm sm={sm} radius={radius} zNear={znear}")
+    geo = _np(g.position)[0] != 0
+    if znear == 0:
+        lit = geo if not pre_eval else geo & (want != 0.25)
+        assert (want[lit] == 1.0).all()
+    if radius >= 64 and znear == 1:
+        pen = (want > 0.25) & (want < 1.0)
+        assert pen.mean() > 0.001  # a penumbra exists
+        # revectorized taps differ from PCF over the same taps somewhere
+        pcf = oracle_lib.pcss(_np(g.position), _np(g.normal) if pre_eval else None, _np(depth[0]), op,
+                              blocker_search_size=7, kernel_size=7, light_source_radius=radius, z_near=znear)
+        assert (pcf != want).any()
+
+
+def test_rbssm_rejects(gi, cuda):
+    import ctypes
+
+    wl = gi.scenes.config1()
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 64, 48)
+    depth = torch.ones((64, 64), device="cuda")
+    out = torch.empty((48, 64), device="cuda")
+    L = gi.lib()
+    base = gi.RBSSMParams(lightMVP=light_mvps(wl)[0], shadowMapWidth=64, shadowMapHeight=64)
+    for field, val in (("kernel_size", 1), ("kernel_size", 32), ("blocker_search_size", 1),
+                       ("light_source_radius", -1), ("max_search", -1), ("max_search", 65)):
+        p = base.to_c()
+        if field == "max_search":
+            p.max_search = val
+        else:
+            setattr(p.pcss, field, val)
+        c = g.to_c(with_normals=False)
+        assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
+            gi._lib.GI_EINVAL, (field, val)
+    p = base.to_c()
+    p.depth_threshold = float("nan")
+    c = g.to_c(with_normals=False)
+    assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
+        gi._lib.GI_ENONFINITE
+
+
 def _write_obj(path, mesh):
     lines = ["v " + " ".join(repr(float(c)) for c in p) for p in mesh.positions]
     lines += ["f " + " ".join(str(int(i) + 1) for i in t) for t in mesh.indices]
