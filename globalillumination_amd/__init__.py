@@ -7,11 +7,12 @@ ABI, and pass functions that mirror the reference's per-technique passes.
 """
 from . import _lib, camera, diff, scenes
 from ._lib import GiError, lib
-from .passes import (DeviceMesh, GBuffer, PCSSParams, PhongParams, RBSSMParams, ShadowParams, SoftParams, biased,
+from .passes import (DeviceMesh, GBuffer, PCSSParams, PhongParams, RBSSMParams, RevSoftParams, ShadowParams,
+                     SoftParams, biased,
                      compute_hard_shadows, phong_uniforms, render_gbuffer, render_monte_carlo, render_rbssm,
                      render_shadow_map, render_soft_shadows, shade_scene, soft_uniforms, view_matrices)
 
 __all__ = ["_lib", "camera", "diff", "scenes", "GiError", "lib", "DeviceMesh", "GBuffer", "PCSSParams",
-           "PhongParams", "RBSSMParams", "ShadowParams", "SoftParams", "biased", "compute_hard_shadows", "phong_uniforms",
+           "PhongParams", "RBSSMParams", "RevSoftParams", "ShadowParams", "SoftParams", "biased", "compute_hard_shadows", "phong_uniforms",
            "render_gbuffer", "render_monte_carlo", "render_rbssm", "render_shadow_map", "render_soft_shadows", "shade_scene",
            "soft_uniforms", "view_matrices"]

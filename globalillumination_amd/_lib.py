@@ -25,6 +25,7 @@ GI_EIO = -6
 GI_NAIVE, GI_PCF, GI_RBSM, GI_RPCF, GI_RBSM_FILTERED, GI_RBSM_CONSERVATIVE, GI_RPCF_CONSERVATIVE = range(7)
 GI_MAX_SAMPLES = 4096
 GI_MAX_EDGE = 16384
+GI_REV_DISC_LAYERS = 50  # the reference's discontinuity array depth (main.cpp:1635)
 
 F16 = ctypes.c_float * 16
 F9 = ctypes.c_float * 9
@@ -72,6 +73,16 @@ class GiRbssmParams(ctypes.Structure):
     _fields_ = [("pcss", GiPcssParams),
                 ("max_search", ctypes.c_int),
                 ("depth_threshold", ctypes.c_float)]
+
+
+class GiRevSoftParams(ctypes.Structure):
+    _fields_ = [("shadow_intensity", ctypes.c_float),
+                ("rpcf", ctypes.c_int),
+                ("max_search", ctypes.c_int),
+                ("depth_threshold", ctypes.c_float),
+                ("mv", ctypes.c_float * 16),
+                ("normal_matrix", ctypes.c_float * 9),
+                ("light_position", ctypes.c_float * 3)]
 
 
 class GiPhongParams(ctypes.Structure):
@@ -191,6 +202,20 @@ _SIGS = {
                                     ctypes.c_void_p, ctypes.c_void_p]),
     "gi_rbssm_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiRbssmParams),
                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_rev_discontinuity_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "gi_rev_quadtree_node_test": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
+                                                 ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]),
+    "gi_rev_adaptive_soft_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
+                                                        ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                        ctypes.c_void_p, ctypes.POINTER(GiRevSoftParams),
+                                                        ctypes.c_void_p, ctypes.c_void_p]),
     "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(GiPhongParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),

@@ -19,6 +19,7 @@ __device__ __forceinline__ float fbool(bool b) { return b ? 1.0f : 0.0f; }
 struct Rbssm {
     Rbsm<false> rb;  // shadow map, shadowMapStep, depthThreshold, maxSearch
     mutable float newDepth;
+    int cap;         // scan bound of the accurate smoothONDS: 4 (W + H) + 16
 
     __device__ __forceinline__ float sx() const { return rb.sx; }
     __device__ __forceinline__ float sy() const { return rb.sy; }
@@ -82,12 +83,24 @@ struct Rbssm {
         return nd;
     }
 
-    // smoothONDS :311-1117 (lightCoord, normalizedDiscontinuity and
-    // discontinuity are the shader's by-value copies)
+    // smoothONDS, RBSSM.frag:311-1117 (kAcc = false) and
+    // RevectorizationBasedAccurateSoftShadow.frag:320-1138 (kAcc = true).  The
+    // two differ in three early exits and in eight scans that RBSSM.frag runs
+    // once (their `while` lines are commented out there) and the accurate
+    // shader repeats until the scan condition fails.  Each scan moves at least
+    // one texel per step and ends at the map border (depth 0) for any finite
+    // depth; `cap` steps (4 (W + H) + 16) bound it for non-finite depths.
+    // lightCoord, normalizedDiscontinuity and discontinuity are by-value
+    // copies, as in the shader.
+    template <bool kAcc>
     __device__ float smooth(float lx, float ly, float lz, float nx, float ny, float dr, float dg, float db, float ux,
                             float uy) const {
         const float sxx = sx(), syy = sy();
         if (db == 0.0f) {
+            if (kAcc) {
+                if (nx == -1.0f && ny == -1.0f) return 1.0f;
+                if (dr == 0.75f && dg == 0.75f) return 0.0f;
+            }
             // positive entering discontinuity on both axes, on both sides of one axis
             if (nx <= -2.0f && ny <= -2.0f && (dr == 0.75f || dg == 0.75f)) {
                 bool left = true, right = true, bottom = true, top = true;
@@ -158,11 +171,16 @@ struct Rbssm {
             }
 
             if (dr == 0.75f || dg == 0.75f) {
-                // entering left and right discontinuity
+                // entering left and right discontinuity: scan while umbra on both sides
                 if (dr == 0.75f && dg != 0.0f) {
-                    ly = ly + ((dg - 0.75f) * 4.0f) * syy;
-                    const bool left = discv(lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                    const bool right = discv(lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    bool left, right;
+                    int guard = 0;
+                    do {
+                        ly = ly + ((dg - 0.75f) * 4.0f) * syy;
+                        left = discv(lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                        right = discv(lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                        if (!kAcc || lz > tex(lx, ly)) break;
+                    } while (left && right && ++guard < cap);
                     if (!left && !right) return gclamp(1.0f - ny, 0.0f, 1.0f);
                     const float sub = gmix(1.0f - uy, uy, gstep(1.0f, dg));
                     const float a =
@@ -170,17 +188,28 @@ struct Rbssm {
                     const float b = gmix(sub, gclamp(ux - (ny - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(left)));
                     return gmin(a, b);
                 }
-                // entering left and right discontinuity only
+                // entering left and right discontinuity only: scan up and down for the ends
                 if (dr == 0.75f && dg == 0.0f) {
-                    bool topLeft, topRight, bottomLeft, bottomRight;
-                    const float ty = ly + syy;
-                    const bool topCenter = !(le01(lz, tex(lx, ty)) != 0.0f);
-                    topLeft = discv(lx, ty, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                    topRight = discv(lx, ty, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
-                    const float by = ly - syy;
-                    const bool bottomCenter = !(le01(lz, tex(lx, by)) != 0.0f);
-                    bottomLeft = discv(lx, by, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                    bottomRight = discv(lx, by, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    bool topLeft = true, topRight = true, bottomLeft = true, bottomRight = true;
+                    bool topCenter = false, bottomCenter = false;
+                    float ty = ly, by = ly;
+                    int guard = 0;
+                    do {
+                        if (!topCenter) {
+                            ty = ty + syy;
+                            topCenter = !(le01(lz, tex(lx, ty)) != 0.0f);
+                            topLeft = discv(lx, ty, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                            topRight = discv(lx, ty, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                        }
+                        if (!bottomCenter) {
+                            by = by - syy;
+                            bottomCenter = !(le01(lz, tex(lx, by)) != 0.0f);
+                            bottomLeft = discv(lx, by, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                            bottomRight = discv(lx, by, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                        }
+                        if (!kAcc) break;
+                    } while (topLeft && topRight && bottomLeft && bottomRight && (!topCenter || !bottomCenter) &&
+                             ++guard < cap);
                     if (topCenter) topLeft = topRight = true;
                     if (bottomCenter) bottomLeft = bottomRight = true;
                     if (ny <= -2.0f) {
@@ -198,11 +227,16 @@ struct Rbssm {
                     else
                         return gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f);
                 }
-                // entering top and bottom discontinuity
+                // entering top and bottom discontinuity: scan while umbra above and below
                 if (dr != 0.0f && dg == 0.75f) {
-                    lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
-                    const bool bottom = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
-                    const bool top = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    bool bottom, top;
+                    int guard = 0;
+                    do {
+                        lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                        bottom = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                        top = discv(lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                        if (!kAcc || lz > tex(lx, ly)) break;
+                    } while (bottom && top && ++guard < cap);
                     if (!bottom && !top) return gclamp(1.0f - nx, 0.0f, 1.0f);
                     const float sub = gmix(ux, 1.0f - ux, gstep(dr, 0.25f));
                     const float a = gmix(sub, gclamp(uy - (nx - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(top)));
@@ -210,17 +244,28 @@ struct Rbssm {
                         gmix(sub, gclamp((1.0f - uy) - (nx - 1.0f), 0.0f, 1.0f), gstep(1.0f, fbool(bottom)));
                     return gmin(a, b);
                 }
-                // entering top and bottom discontinuity only
+                // entering top and bottom discontinuity only: scan right and left for the ends
                 if (dr == 0.0f && dg == 0.75f) {
-                    bool topLeft, topRight, bottomLeft, bottomRight;
-                    const float rx = lx + sxx;
-                    const bool rightCenter = !(le01(lz, tex(rx, ly)) != 0.0f);
-                    bottomRight = discv(rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
-                    topRight = discv(rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
-                    const float lxx = lx - sxx;
-                    const bool leftCenter = !(le01(lz, tex(lxx, ly)) != 0.0f);
-                    bottomLeft = discv(lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
-                    topLeft = discv(lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    bool topLeft = true, topRight = true, bottomLeft = true, bottomRight = true;
+                    bool leftCenter = false, rightCenter = false;
+                    float rx = lx, lxx = lx;
+                    int guard = 0;
+                    do {
+                        if (!rightCenter) {
+                            rx = rx + sxx;
+                            rightCenter = !(le01(lz, tex(rx, ly)) != 0.0f);
+                            bottomRight = discv(rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                            topRight = discv(rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                        }
+                        if (!leftCenter) {
+                            lxx = lxx - sxx;
+                            leftCenter = !(le01(lz, tex(lxx, ly)) != 0.0f);
+                            bottomLeft = discv(lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                            topLeft = discv(lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                        }
+                        if (!kAcc) break;
+                    } while (topLeft && topRight && bottomLeft && bottomRight && (!rightCenter || !leftCenter) &&
+                             ++guard < cap);
                     if (rightCenter) bottomRight = topRight = true;
                     if (leftCenter) topLeft = bottomLeft = true;
                     if (nx <= -2.0f) {
@@ -287,25 +332,28 @@ struct Rbssm {
                 else return gclamp((1.0f - ux) - (ny - 1.0f), 0.0f, 1.0f);
             }
         } else {  // exiting discontinuity
+            if (kAcc && dr == 0.75f && dg == 0.75f) return 1.0f;
             if (dr == 0.75f || dg == 0.75f) {
                 // exiting left and right discontinuity only
                 if (dr == 0.75f && dg == 0.0f) {
-                    float rx = lx - sxx, ry = ly;
-                    bool left = discv(rx, ry, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                    rx = lx + sxx;
-                    bool right = discv(rx, ry, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    bool left = discv(lx - sxx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    bool right = discv(lx + sxx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
                     if (left && right) {
                         return gclamp(ny, 0.0f, 1.0f);
                     } else if (left || right) {
                         if (!left) return gmix(1.0f - ux, gclamp(ny - ux, 0.0f, 1.0f), gstep(-2.0f, ny));
                         else return gmix(ux, gclamp(ny - (1.0f - ux), 0.0f, 1.0f), gstep(-2.0f, ny));
                     } else {
-                        // count 0: mult 1, even -> one row down
-                        ry = ly + 1.0f * syy;
-                        rx = lx - sxx;
-                        left = discf(rx, ry, lz, 1.0f, 0.0f, 0.0f);
-                        rx = lx + sxx;
-                        right = discf(rx, ry, lz, 1.0f, 0.0f, 0.0f);
+                        // rows +1, -1, +2, -2, ... until umbra beside the column
+                        int count = 0;
+                        do {
+                            const int mult = count / 2 + 1;
+                            const float ry = (count % 2 == 0) ? ly + (float)mult * syy : ly - (float)mult * syy;
+                            left = discf(lx - sxx, ry, lz, 1.0f, 0.0f, 0.0f);
+                            right = discf(lx + sxx, ry, lz, 1.0f, 0.0f, 0.0f);
+                            if (!kAcc || lz <= tex(lx, ly)) break;
+                            ++count;
+                        } while (!left && !right && count < cap);
                         if (left && right) {
                             return gclamp(ny, 0.0f, 1.0f);
                         } else {
@@ -330,8 +378,13 @@ struct Rbssm {
                         return gmax(a, b);
                     } else {
                         if (dg == 0.75f) return 0.0f;
-                        left = discf(lx - sxx, ly, lz, 1.0f, 0.0f, 0.0f);
-                        right = discf(lx + sxx, ly, lz, 1.0f, 0.0f, 0.0f);
+                        int guard = 0;
+                        do {
+                            left = discf(lx - sxx, ly, lz, 1.0f, 0.0f, 0.0f);
+                            right = discf(lx + sxx, ly, lz, 1.0f, 0.0f, 0.0f);
+                            if (!kAcc || lz <= tex(lx, ly)) break;
+                            ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * syy;
+                        } while (!left && !right && ++guard < cap);
                         if (left && right) {
                             return gclamp(ny, 0.0f, 1.0f);
                         } else {
@@ -343,21 +396,24 @@ struct Rbssm {
                 }
                 // exiting top and bottom discontinuity only
                 if (dr == 0.0f && dg == 0.75f) {
-                    float rx = lx, ry = ly - syy;
-                    bool top = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
-                    ry = ly + syy;
-                    bool bottom = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
+                    bool top = discf(lx, ly - syy, lz, 0.0f, 1.0f, 0.0f);
+                    bool bottom = discf(lx, ly + syy, lz, 0.0f, 1.0f, 0.0f);
                     if (bottom && top) {
                         return gclamp(nx, 0.0f, 1.0f);
                     } else if (bottom || top) {
                         if (!top) return gmix(1.0f - uy, gclamp(nx - uy, 0.0f, 1.0f), gstep(-2.0f, nx));
                         else return gmix(uy, gclamp(nx - (1.0f - uy), 0.0f, 1.0f), gstep(-2.0f, nx));
                     } else {
-                        rx = lx + 1.0f * sxx;
-                        ry = ly - syy;
-                        top = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
-                        ry = ly + syy;
-                        bottom = discf(rx, ry, lz, 0.0f, 1.0f, 0.0f);
+                        // columns +1, -1, +2, -2, ... until umbra above or below
+                        int count = 0;
+                        do {
+                            const int mult = count / 2 + 1;
+                            const float rx = (count % 2 == 0) ? lx + (float)mult * sxx : lx - (float)mult * sxx;
+                            top = discf(rx, ly - syy, lz, 0.0f, 1.0f, 0.0f);
+                            bottom = discf(rx, ly + syy, lz, 0.0f, 1.0f, 0.0f);
+                            if (!kAcc || lz <= tex(lx, ly)) break;
+                            ++count;
+                        } while (!top && !bottom && count < cap);
                         if (top && bottom) {
                             return gclamp(nx, 0.0f, 1.0f);
                         } else {
@@ -382,8 +438,13 @@ struct Rbssm {
                         return gmax(a, b);
                     } else {
                         if (dr == 0.75f) return 0.0f;
-                        top = discf(lx, ly - syy, lz, 0.0f, 1.0f, 0.0f);
-                        bottom = discf(lx, ly + syy, lz, 0.0f, 1.0f, 0.0f);
+                        int guard = 0;
+                        do {
+                            top = discf(lx, ly - syy, lz, 0.0f, 1.0f, 0.0f);
+                            bottom = discf(lx, ly + syy, lz, 0.0f, 1.0f, 0.0f);
+                            if (!kAcc || lz <= tex(lx, ly)) break;
+                            lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sxx;
+                        } while (!top && !bottom && ++guard < cap);
                         if (top && bottom) {
                             return gclamp(nx, 0.0f, 1.0f);
                         } else {
@@ -443,6 +504,19 @@ struct Rbssm {
         return 1.0f - db;
     }
 
+    // RSMSS, RevectorizationBasedAccurateSoftShadow.frag:1175-1186: one tap on
+    // a discontinuity, oriented, normalized and filled by the accurate
+    // smoothONDS, then mixed with the shadow intensity.
+    __device__ float rsmss(float x, float y, float z, float dr, float dg, float db, float si) const {
+        const float ux = gfract(x * rb.D.fw), uy = gfract(y * rb.D.fh);
+        const float left = length(dr, dg, db, x, y, z, -1.0f, 0.0f);
+        const float right = length(dr, dg, db, x, y, z, 1.0f, 0.0f);
+        const float down = length(dr, dg, db, x, y, z, 0.0f, -1.0f);
+        const float up = length(dr, dg, db, x, y, z, 0.0f, 1.0f);
+        const float nx = normalize(left, right, ux), ny = normalize(down, up, uy);
+        return gmix(smooth<true>(x, y, z, nx, ny, dr, dg, db, ux, uy), 1.0f, si);
+    }
+
     // RBSSM :1202-1339 over the penumbra kernel
     __device__ float kernel(const PcssUniforms& q, float si, float pw, float x, float y, float z) const {
         float illum = 0.0f;
@@ -461,7 +535,7 @@ struct Rbssm {
                     const float down = length(dr, dg, db, tx, ty, z, 0.0f, -1.0f);
                     const float up = length(dr, dg, db, tx, ty, z, 0.0f, 1.0f);
                     const float nx = normalize(left, right, ux), ny = normalize(down, up, uy);
-                    float fill = smooth(tx, ty, z, nx, ny, dr, dg, db, ux, uy);
+                    float fill = smooth<false>(tx, ty, z, nx, ny, dr, dg, db, ux, uy);
                     fill = gmix(fill, 1.0f, si);
                     illum = illum + fill;
                 } else {
@@ -500,9 +574,210 @@ __global__ __launch_bounds__(kBlockThreads) void k_rbssm(GBufferView gb, DepthMa
         if (stepSize <= 0.0f || stepSize >= 1.0f) {
             shadow = 1.0f;
         } else {
-            Rbssm r{Rbsm<false>{D, u.stepx, u.stepy, u.thr, u.max_search}, 0.0f};
+            Rbssm r{Rbsm<false>{D, u.stepx, u.stepy, u.thr, u.max_search}, 0.0f, 0};
             shadow = r.kernel(q, u.si, pw, n0, n1, n2);
         }
+    }
+    out[k] = shadow;
+}
+
+// ---------------------------------------------------------------------------
+// Revectorization-based adaptive sampling (ShadowParams.
+// revectorizationBasedAdaptiveSampling, SoftShadowMapping/src/main.cpp:440-571
+// and :813-892).
+// ---------------------------------------------------------------------------
+
+// Discontinuity.frag writes (r, g, b, 1) into a GL_RGBA8 layer; r and g are
+// multiples of 0.25, b is 0 or 1.  One byte holds the same information:
+// bits 0-1 r * 4, bits 2-3 g * 4, bit 4 b, bit 5 the alpha of an evaluated
+// pixel.  Background pixels keep the layer's clear colour (1, 1, 1, 1).
+constexpr unsigned char kDiscBackground = 0xFF;
+
+__device__ __forceinline__ unsigned char disc_encode(float r, float g, float b) {
+    return (unsigned char)((int)(r * 4.0f) | ((int)(g * 4.0f) << 2) | (b != 0.0f ? 16 : 0) | 32);
+}
+
+// correctDiscontinuity (RevectorizationBasedAccurateSoftShadow.frag:1188-1205)
+// restores the exact quarters from the 8-bit values; from the byte code they
+// are exact already.
+__device__ __forceinline__ void disc_decode(unsigned char c, float& r, float& g, float& b) {
+    r = (float)(c & 3) * 0.25f;
+    g = (float)((c >> 2) & 3) * 0.25f;
+    b = (float)((c >> 4) & 1);
+}
+
+// Discontinuity.frag:66-88: the discontinuity of the G-buffer pixel in one
+// corner map (computeDiscontinuity when the pre-evaluation lets it through and
+// w > 0, else 0).
+__global__ __launch_bounds__(kBlockThreads) void k_rev_discontinuity(GBufferView gb, DepthMap D, HardUniforms u,
+                                                                      unsigned char* __restrict__ layer) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t k = (size_t)y * gb.w + x;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {
+        layer[k] = kDiscBackground;
+        return;
+    }
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    float sc[4];
+    mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
+    const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+    const float shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+    unsigned char code = 0;
+    if (sc[3] > 0.0f && shadow == 1.0f) {
+        const Rbsm<false> rb{D, u.stepx, u.stepy, u.thr, u.max_search};
+        float dr, dg, db;
+        rb.discontinuity(n0, n1, n2, D.fetch(n0, n1), dr, dg, db);
+        code = disc_encode(dr, dg, db);
+    }
+    layer[k] = code;
+}
+
+struct RevQtUniforms {
+    QtUniforms q;
+    int level;        // quadTreeLevel of the node under test
+    int disc_layers;  // layers of the discontinuity array; reads clamp to the last
+};
+
+// QuadTree/RevectorizationBasedReprojection.frag:14-51: lit-corner count as
+// Reprojection.frag, forced to 5 (never a penumbra pixel for Evaluation.frag)
+// where two or more corner maps see a discontinuity at the pixel, and the
+// visibility classification (0 umbra, 1 lit, 0.5 penumbra) started at level
+// 0 and demoted to 0.5 where a deeper node disagrees.  Each pixel reads and
+// writes only its own classification, so the update is in place (the
+// reference renders a copy and copies it back).  Block (0, 0) clears the
+// node's pass counter for k_qt_evaluate.
+__global__ __launch_bounds__(kBlockThreads) void k_rev_qt_reproject(GBufferView gb, const float* __restrict__ maps,
+                                                                     int smw, int smh, RevQtUniforms u,
+                                                                     const unsigned char* __restrict__ disc,
+                                                                     float* __restrict__ img,
+                                                                     float* __restrict__ vis,
+                                                                     unsigned* __restrict__ count) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *count = 0u;
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t k = (size_t)y * gb.w + x;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {  // both targets keep the clear colour
+        img[k] = kQtBackground;
+        vis[k] = kQtBackground;
+        return;
+    }
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    const size_t mapsz = (size_t)smw * smh;
+    float sh[4];
+    int ndisc = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float sc[4];
+        mat4_vec4(u.q.m[i].m, vx, vy, vz, 1.0f, sc);
+        const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        DepthMap D{maps + (size_t)u.q.map[i] * mapsz, smw, smh, (float)smw, (float)smh};
+        sh[i] = (n2 <= D.fetch(n0, n1)) ? 1.0f : 0.0f;
+        const int layer = u.q.map[i] < u.disc_layers ? u.q.map[i] : u.disc_layers - 1;
+        if ((disc[(size_t)layer * plane + k] & 15) != 0) ++ndisc;
+    }
+    float sum = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    if (ndisc >= 2) sum = 5.0f;
+    float v = vis[k];
+    if (u.level == 0) {
+        v = (sum == 0.0f || sum == 4.0f) ? sum / 4.0f : 0.5f;
+    } else if ((v == 0.0f && sum != 0.0f) || (v == 1.0f && sum != 4.0f)) {
+        v = 0.5f;
+    }
+    img[k] = sum;
+    vis[k] = v;
+}
+
+struct RevSoftUniforms {
+    float m[12];      // rows 0-3 of columns 0-2 of B*lightMVP
+    HardUniforms h;   // pre-evaluation (mv, nm, lpos, si), maxSearch, depthThreshold, shadowMapStep
+    int nsamples;
+    int disc_layers;
+    int cap;          // scan bound of the accurate smoothONDS
+};
+
+// RevectorizationBasedAccurateSoftShadow.frag:1252-1331.  Pixels the
+// pre-evaluation lets through take their classification (clamped to
+// [si, 1]); penumbra pixels (0.5) run every sample, weighted by its
+// quad-tree level.  kRpcf == false: one tap, revectorized where the stored
+// discontinuity of that sample is set; kRpcf == true: a 3 x 3 tap kernel,
+// each tap's discontinuity computed on the fly and revectorized.
+template <bool kRpcf>
+__global__ __launch_bounds__(kBlockThreads) void k_rev_soft(GBufferView gb, const float* __restrict__ maps, int smw,
+                                                             int smh, const float4* __restrict__ trans,
+                                                             const unsigned char* __restrict__ disc,
+                                                             const float* __restrict__ vis, RevSoftUniforms u,
+                                                             float* __restrict__ out) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t k = (size_t)y * gb.w + x;
+    const float vx = gb.pos[k];
+    if (vx == 0.0f) {  // discard: the cleared target holds 0
+        out[k] = 0.0f;
+        return;
+    }
+    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+    const float si = u.h.si;
+    float shadow = pre_evaluation(u.h, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+    if (shadow == 1.0f) shadow = gclamp(vis[k], si, 1.0f);
+    if (shadow == 0.5f) {
+        float cm[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float a = u.m[0 * 4 + i] * vx;
+            a = a + u.m[1 * 4 + i] * vy;
+            a = a + u.m[2 * 4 + i] * vz;
+            cm[i] = a;
+        }
+        const size_t mapsz = (size_t)smw * smh;
+        DepthMap D{maps, smw, smh, (float)smw, (float)smh};
+        const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
+        float acc = 0.0f, count = 0.0f;
+        for (int i = 0; i < u.nsamples; ++i) {
+            float4 T = trans[i];
+            const float af = adaptive_weight(T);
+            const float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
+            const float sx = cx / cw, sy = cy / cw, sz = cz / cw;
+            D.texels = maps + (size_t)i * mapsz;
+            const Rbssm r{Rbsm<false>{D, u.h.stepx, u.h.stepy, u.h.thr, u.h.max_search}, 0.0f, u.cap};
+            if constexpr (!kRpcf) {
+                const float d = D.fetch(sx, sy);
+                const int layer = i < u.disc_layers ? i : u.disc_layers - 1;
+                float dr, dg, db;
+                disc_decode(disc[(size_t)layer * plane + k], dr, dg, db);
+                if (dr > 0.0f || dg > 0.0f)
+                    acc = acc + r.rsmss(sx, sy, sz, dr, dg, db, si) * af;
+                else
+                    acc = acc + ((sz <= d) ? 1.0f : si) * af;
+            } else {
+                float illum = 0.0f;
+                int n = 0;
+                for (float w = -1.0f; w <= 1.0f; w += 1.0f) {
+                    for (float hh = -1.0f; hh <= 1.0f; hh += 1.0f) {
+                        const float tx = sx + w * incrWidth, ty = sy + hh * incrHeight;
+                        const float d = D.fetch(tx, ty);
+                        float dr, dg, db;
+                        r.rb.discontinuity(tx, ty, sz, d, dr, dg, db);
+                        if (dr > 0.0f || dg > 0.0f)
+                            illum = illum + r.rsmss(tx, ty, sz, dr, dg, db, si);
+                        else
+                            illum = illum + ((sz <= d) ? 1.0f : si);
+                        ++n;
+                    }
+                }
+                acc = acc + (illum / (float)n) * af;
+            }
+            count = count + af;
+        }
+        shadow = acc / count;
     }
     out[k] = shadow;
 }

@@ -10,7 +10,11 @@
 //   RBSM/ConservativeSMSR.frag:23-235 (conservative RBSM, RPCF),
 //   SoftShadow/AccurateSoftShadow.frag:52-133 (area light),
 //   SoftShadow/PlausibleSoftShadow.frag:166-632 (PCSS),
-//   GBuffer/PhongShading.frag:11-46 (deferred shading).
+//   GBuffer/PhongShading.frag:11-46 (deferred shading),
+//   SoftShadow/RBSSM.frag (revectorization-based soft shadows),
+//   HardShadow/Discontinuity.frag, QuadTree/RevectorizationBasedReprojection.frag
+//   and SoftShadow/RevectorizationBasedAccurateSoftShadow.frag (revectorization-
+//   based adaptive sampling).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -156,6 +160,104 @@ int gi_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, int 
     const int wd = gb.w / div, hd = gb.h / div;
     const unsigned blocks = (unsigned)(((size_t)wd * hd + 255) / 256);
     hipLaunchKernelGGL(k_qt_evaluate, dim3(blocks), dim3(256), 0, s, image_dev, gb.w, gb.h, wd, hd, count_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_rev_discontinuity_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
+                              int map_index, const float biased_mvp[16], const float mv[16],
+                              const float normal_matrix[9], const float light_position[3], float shadow_intensity,
+                              unsigned char* disc_dev, int disc_layers, gi_stream stream) {
+    if (!gbuffer_ok(gbuffer) || !gbuffer->normal_dev || !maps_dev || !biased_mvp || !mv || !normal_matrix ||
+        !light_position || !disc_dev)
+        return GI_EINVAL;
+    if (nmaps <= 0 || smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE) return GI_EINVAL;
+    if (map_index < 0 || map_index >= nmaps || disc_layers <= 0) return GI_EINVAL;
+    if (!finite_all(biased_mvp, 16) || !finite_all(mv, 16) || !finite_all(normal_matrix, 9) ||
+        !finite_all(light_position, 3) || !std::isfinite(shadow_intensity))
+        return GI_ENONFINITE;
+    if (map_index >= disc_layers) return GI_OK;  // no layer for this slot
+    HardUniforms u{};
+    for (int i = 0; i < 16; ++i) { u.light_mvp.m[i] = biased_mvp[i]; u.mv.m[i] = mv[i]; }
+    for (int i = 0; i < 9; ++i) u.nm.m[i] = normal_matrix[i];
+    for (int i = 0; i < 3; ++i) u.lpos[i] = light_position[i];
+    u.si = shadow_intensity;
+    u.stepx = (float)(1.0 / (double)smw);
+    u.stepy = (float)(1.0 / (double)smh);
+    GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
+    DepthMap D{maps_dev + (size_t)map_index * smw * smh, smw, smh, (float)smw, (float)smh};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_rev_discontinuity, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u,
+                       disc_dev + (size_t)map_index * gb.w * gb.h);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_rev_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
+                              const int map_index[4], const float biased_mvps[64], const unsigned char* disc_dev,
+                              int disc_layers, int level, int div, float* image_dev, float* visibility_dev,
+                              unsigned* count_dev, gi_stream stream) {
+    if (!gbuffer_ok(gbuffer) || !maps_dev || !map_index || !biased_mvps || !disc_dev || !image_dev ||
+        !visibility_dev || !count_dev)
+        return GI_EINVAL;
+    if (nmaps <= 0 || smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE) return GI_EINVAL;
+    if (div < 1 || div > gbuffer->width || div > gbuffer->height || level < 0 || disc_layers <= 0)
+        return GI_EINVAL;
+    for (int i = 0; i < 4; ++i)
+        if (map_index[i] < 0 || map_index[i] >= nmaps) return GI_EINVAL;
+    if (!finite_all(biased_mvps, 64)) return GI_ENONFINITE;
+    RevQtUniforms u;
+    for (int i = 0; i < 4; ++i) {
+        for (int e = 0; e < 16; ++e) u.q.m[i].m[e] = biased_mvps[16 * i + e];
+        u.q.map[i] = map_index[i];
+    }
+    u.level = level;
+    u.disc_layers = disc_layers;
+    GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_rev_qt_reproject, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, maps_dev, smw, smh,
+                       u, disc_dev, image_dev, visibility_dev, count_dev);
+    const int wd = gb.w / div, hd = gb.h / div;
+    const unsigned blocks = (unsigned)(((size_t)wd * hd + 255) / 256);
+    hipLaunchKernelGGL(k_qt_evaluate, dim3(blocks), dim3(256), 0, s, image_dev, gb.w, gb.h, wd, hd, count_dev);
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw,
+                                     int smh, const float light_mvp[16], const float* trans_dev,
+                                     const unsigned char* disc_dev, int disc_layers, const float* visibility_dev,
+                                     const gi_rev_soft_params* p, float* out_dev, gi_stream stream) {
+    if (!gbuffer_ok(gbuffer) || !gbuffer->normal_dev || !maps_dev || !light_mvp || !trans_dev || !disc_dev ||
+        !visibility_dev || !p || !out_dev)
+        return GI_EINVAL;
+    if (nsamples <= 0 || nsamples > GI_MAX_SAMPLES) return GI_EINVAL;
+    if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE || disc_layers <= 0) return GI_EINVAL;
+    if (p->max_search < 0 || p->max_search > 64) return GI_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(trans_dev) & 15u) != 0) return GI_EINVAL;
+    if (!finite_all(light_mvp, 16) || !std::isfinite(p->shadow_intensity) || !std::isfinite(p->depth_threshold) ||
+        !finite_all(p->mv, 16) || !finite_all(p->normal_matrix, 9) || !finite_all(p->light_position, 3))
+        return GI_ENONFINITE;
+    RevSoftUniforms u{};
+    for (int i = 0; i < 12; ++i) u.m[i] = light_mvp[i];
+    for (int i = 0; i < 16; ++i) u.h.mv.m[i] = p->mv[i];
+    for (int i = 0; i < 9; ++i) u.h.nm.m[i] = p->normal_matrix[i];
+    for (int i = 0; i < 3; ++i) u.h.lpos[i] = p->light_position[i];
+    u.h.si = p->shadow_intensity;
+    u.h.max_search = p->max_search;
+    u.h.thr = p->depth_threshold;
+    u.h.stepx = (float)(1.0 / (double)smw);
+    u.h.stepy = (float)(1.0 / (double)smh);
+    u.nsamples = nsamples;
+    u.disc_layers = disc_layers;
+    u.cap = 4 * (smw + smh) + 16;
+    GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const float4* tr = reinterpret_cast<const float4*>(trans_dev);
+    dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
+    if (p->rpcf)
+        hipLaunchKernelGGL(k_rev_soft<true>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, disc_dev,
+                           visibility_dev, u, out_dev);
+    else
+        hipLaunchKernelGGL(k_rev_soft<false>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, disc_dev,
+                           visibility_dev, u, out_dev);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

@@ -448,3 +448,95 @@ def quadtree_node_test(gbuf: GBuffer, maps: torch.Tensor, map_index, biased_mvps
     _lib.call("gi_quadtree_node_test", ctypes.byref(g), _dptr(maps), S, smw, smh, idx, _f(biased_mvps, 64), int(div),
               _dptr(image), _dptr(count), _stream_handle(stream))
     return image, int(count.item())
+
+
+# ---------------------------------------------------------------------------
+# Revectorization-based adaptive sampling (ShadowParams.
+# revectorizationBasedAdaptiveSampling, SoftShadowMapping/src/main.cpp:
+# 440-571, 813-892)
+# ---------------------------------------------------------------------------
+def rev_level_div(level: int, rpcf: bool) -> int:
+    """Evaluation grid divisor of a node test (main.cpp:535-549)."""
+    if level <= 1:
+        return 1
+    if level <= 3:
+        return 3 if rpcf else 2
+    return 6 if rpcf else 4
+
+
+def rev_discontinuity_layers(gbuf: GBuffer, layers: int = _lib.GI_REV_DISC_LAYERS) -> torch.Tensor:
+    """The discontinuity array: `layers` x H x W bytes, zero-filled (the
+    reference's layers start undefined and keep earlier frames' contents)."""
+    return torch.zeros((layers, gbuf.height, gbuf.width), dtype=torch.uint8, device=gbuf.position.device)
+
+
+def rev_discontinuity(gbuf: GBuffer, maps: torch.Tensor, map_index: int, biased_mvp: np.ndarray, MV: np.ndarray,
+                      normalMatrix: np.ndarray, lightPosition, shadowIntensity: float, disc: torch.Tensor,
+                      stream=None) -> None:
+    """Discontinuity.frag for corner map `map_index` into layer map_index of
+    `disc` (nothing is written past the array's last layer)."""
+    S, smh, smw = (int(x) for x in maps.shape)
+    g = gbuf.to_c(with_normals=True)
+    _lib.call("gi_rev_discontinuity_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, int(map_index),
+              _f(biased_mvp, 16), _f(MV, 16), _f(normalMatrix, 9), _f(lightPosition, 3), float(shadowIntensity),
+              _dptr(disc), int(disc.shape[0]), _stream_handle(stream))
+
+
+def rev_quadtree_node_test(gbuf: GBuffer, maps: torch.Tensor, map_index, biased_mvps: np.ndarray,
+                           disc: torch.Tensor, level: int, visibility: torch.Tensor, div: int = 1,
+                           image: torch.Tensor | None = None, stream=None) -> tuple:
+    """RevectorizationBasedReprojection.frag + Evaluation.frag: (image, pass
+    count); `visibility` (H, W) float32 is updated in place."""
+    S, smh, smw = (int(x) for x in maps.shape)
+    if image is None:
+        image = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=maps.device)
+    count = torch.empty((1,), dtype=torch.int32, device=maps.device)
+    idx = (ctypes.c_int * 4)(*[int(i) for i in map_index])
+    g = gbuf.to_c(with_normals=False)
+    _lib.call("gi_rev_quadtree_node_test", ctypes.byref(g), _dptr(maps), S, smw, smh, idx, _f(biased_mvps, 64),
+              _dptr(disc), int(disc.shape[0]), int(level), int(div), _dptr(image), _dptr(visibility), _dptr(count),
+              _stream_handle(stream))
+    return image, int(count.item())
+
+
+@dataclass
+class RevSoftParams:
+    """Uniforms of RevectorizationBasedAccurateSoftShadow.frag."""
+    MV: np.ndarray
+    normalMatrix: np.ndarray
+    lightPosition: tuple
+    shadowIntensity: float = 0.25
+    RPCF: bool = False
+    maxSearch: int = 16
+    depthThreshold: float = 0.0
+
+    def to_c(self) -> _lib.GiRevSoftParams:
+        p = _lib.GiRevSoftParams()
+        p.shadow_intensity = float(self.shadowIntensity)
+        p.rpcf = 1 if self.RPCF else 0
+        p.max_search = int(self.maxSearch)
+        p.depth_threshold = float(self.depthThreshold)
+        p.mv = _f(self.MV, 16)
+        p.normal_matrix = _f(self.normalMatrix, 9)
+        p.light_position = _f(self.lightPosition, 3)
+        return p
+
+
+def render_rev_adaptive(gbuf: GBuffer, maps: torch.Tensor, common: np.ndarray, trans: np.ndarray,
+                        disc: torch.Tensor, visibility: torch.Tensor, params: RevSoftParams,
+                        out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    """Final pass of the revectorization-based adaptive frame (`common`,
+    `trans` from adaptive_trans)."""
+    S, smh, smw = (int(x) for x in maps.shape)
+    trans = np.ascontiguousarray(trans, dtype=np.float32).reshape(-1, 4)
+    if trans.shape[0] != S:
+        raise ValueError("one trans row per depth map is required")
+    trans_dev = torch.from_numpy(trans).to(maps.device)
+    if out is None:
+        out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=maps.device)
+    g = gbuf.to_c(with_normals=True)
+    p = params.to_c()
+    _lib.call("gi_rev_adaptive_soft_shadow_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, _f(common, 16),
+              _dptr(trans_dev), _dptr(disc), int(disc.shape[0]), _dptr(visibility), ctypes.byref(p), _dptr(out),
+              _stream_handle(stream))
+    return out

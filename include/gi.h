@@ -232,6 +232,66 @@ int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_de
                                  const float light_mvp[16], const float* trans_dev, const gi_soft_params* params,
                                  float* out_dev, gi_stream stream);
 
+/* ---- revectorization-based adaptive sampling ---------------------------- */
+/* ShadowParams.revectorizationBasedAdaptiveSampling (menu entry 2,
+ * SoftShadowMapping/src/main.cpp:1411-1416).  Three passes replace the plain
+ * ones: every corner map rendered while the quad tree is evaluated gets a
+ * screen-space discontinuity layer, the node test also keeps a per-pixel
+ * visibility classification, and the final pass revectorizes the samples of
+ * penumbra pixels.  The discontinuity array holds disc_layers layers of
+ * width*height bytes; the reference allocates 50 (createRGBATextureArray,
+ * main.cpp:1635), so maps from slot 50 on get no layer and their reads land
+ * on the last layer.  Layers keep their contents from earlier frames. */
+#define GI_REV_DISC_LAYERS 50
+
+/* HardShadow/Discontinuity.frag:66-88 for corner map `map_index` (biased_mvp
+ * = B * its lightMVP): per G-buffer pixel, computeDiscontinuity where the
+ * pre-evaluation (mv, normal_matrix, light_position: the sample's eye as
+ * configurePhong receives it during evaluation, main.cpp:357-361) and w > 0
+ * let it through, else 0; background pixels keep the clear value.  Writes
+ * layer map_index of disc_dev when map_index < disc_layers; otherwise nothing
+ * (the reference's framebuffer is incomplete there).  The gbuffer needs its
+ * normals.  Byte code: bits 0-1 r*4, bits 2-3 g*4, bit 4 discType, bit 5
+ * set for an evaluated pixel, 0xFF for background. */
+int gi_rev_discontinuity_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
+                              int map_index, const float biased_mvp[16], const float mv[16],
+                              const float normal_matrix[9], const float light_position[3], float shadow_intensity,
+                              unsigned char* disc_dev, int disc_layers, gi_stream stream);
+
+/* The node test with revectorizationBasedQuadTreeEvaluation (main.cpp:
+ * 506-560): QuadTree/RevectorizationBasedReprojection.frag:14-51 (lit-corner
+ * count, 5 where two or more corner layers of disc_dev flag a discontinuity;
+ * visibility_dev, width*height floats, set at level 0 and demoted to 0.5 at
+ * deeper levels, in place), then Evaluation.frag on the (width/div) x
+ * (height/div) viewport as gi_quadtree_node_test.  div follows the level:
+ * 1 for levels 0-1, 2 (3 with RPCF) for 2-3, 4 (6 with RPCF) deeper
+ * (main.cpp:535-549). */
+int gi_rev_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
+                              const int map_index[4], const float biased_mvps[64], const unsigned char* disc_dev,
+                              int disc_layers, int level, int div, float* image_dev, float* visibility_dev,
+                              unsigned* count_dev, gi_stream stream);
+
+typedef struct gi_rev_soft_params {
+    float shadow_intensity;
+    int rpcf;               /* ShadowParams.RPCF: 3 x 3 revectorized taps per sample */
+    int max_search;         /* 16 (main.cpp:1617); 0..64 */
+    float depth_threshold;  /* SceneLoader `d` */
+    float mv[16];           /* pre-evaluation, as gi_pcss_params */
+    float normal_matrix[9];
+    float light_position[3];
+} gi_rev_soft_params;
+
+/* The final pass (main.cpp:868-871): RevectorizationBasedAccurateSoftShadow.frag
+ * :1252-1331.  Arguments as gi_adaptive_soft_shadow_pass plus the
+ * discontinuity layers and the visibility classification of the last node
+ * test; the gbuffer needs its normals.  A lit pixel takes its
+ * classification clamped to [si, 1]; pixels at 0.5 average every sample,
+ * revectorizing taps on a discontinuity (RSMSS :1175-1186). */
+int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw,
+                                     int smh, const float light_mvp[16], const float* trans_dev,
+                                     const unsigned char* disc_dev, int disc_layers, const float* visibility_dev,
+                                     const gi_rev_soft_params* params, float* out_dev, gi_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Percentage-closer soft shadows (plausible soft shadows, PCSS)             */
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,9 @@ def lib() -> ctypes.CDLL:
             "or_rbssm_tap": [vp, vp, f, f, f, vp],
             "or_soft_shadow_adaptive": [vp, i, i, vp, i, i, i, vp, vp, ctypes.c_float, i, i, ctypes.c_float, i, vp],
             "or_qt_node_test": [vp, i, i, vp, i, i, vp, vp, i, vp, vp],
+            "or_rev_discontinuity": [vp, vp, i, i, vp, vp, vp],
+            "or_rev_node_test": [vp, i, i, vp, i, i, vp, vp, vp, i, i, i, vp, vp, vp],
+            "or_rev_soft": [vp, vp, i, i, vp, i, i, i, vp, vp, vp, i, vp, vp, i, vp],
             "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
             "or_diff_soft": [vp, i, i, vp, i, i, i, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp],
             "or_set_threads": [i], "or_get_threads": [],
@@ -256,6 +259,71 @@ def qt_node_test(pos: np.ndarray, maps: np.ndarray, idx, mvps_biased, div=1):
     if rc:
         raise RuntimeError(f"or_qt_node_test failed ({rc})")
     return img, int(cnt.value)
+
+
+REV_DISC_LAYERS = 50  # the reference's discontinuity array depth
+
+
+def rev_discontinuity(pos, nrm, depth, params) -> np.ndarray:
+    """Discontinuity.frag for one map: (h, w) uint8 codes (bits 0-1 r*4,
+    2-3 g*4, 4 discType, 5 alpha; 0xFF background).  `params` from
+    make_params: light_mvp = B * the map's lightMVP, map size, max_search,
+    depth_threshold, the pre-evaluation's mv / normal_matrix /
+    light_position and shadow_intensity."""
+    pos, nrm, d = _c32(pos), _c32(nrm), _c32(depth)
+    _, h, w = pos.shape
+    assert nrm.shape == pos.shape and d.shape == (params.sm_height, params.sm_width)
+    out = np.empty((h, w), np.uint8)
+    rc = lib().or_rev_discontinuity(_p(pos), _p(nrm), w, h, _p(d), ctypes.byref(params),
+                                    out.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise RuntimeError(f"or_rev_discontinuity failed ({rc})")
+    return out
+
+
+def rev_node_test(pos, maps, idx, mvps_biased, disc, level, vis, div=1):
+    """RevectorizationBasedReprojection.frag + Evaluation.frag.  `disc`:
+    (L, h, w) uint8 layers; `vis` (h, w) float32 is updated in place.
+    Returns (image, pass count)."""
+    pos, maps = _c32(pos), _c32(maps)
+    _, h, w = pos.shape
+    S, smh, smw = maps.shape
+    ix = np.ascontiguousarray(idx, dtype=np.int32).reshape(4)
+    assert (ix >= 0).all() and (ix < S).all()
+    disc = np.ascontiguousarray(disc, dtype=np.uint8)
+    assert disc.ndim == 3 and disc.shape[1:] == (h, w)
+    assert vis.dtype == np.float32 and vis.shape == (h, w) and vis.flags.c_contiguous
+    m = _c32(mvps_biased).reshape(64)
+    img = np.empty((h, w), np.float32)
+    cnt = ctypes.c_uint(0)
+    rc = lib().or_rev_node_test(_p(pos), w, h, _p(maps), smw, smh, _p(ix), _p(m),
+                                disc.ctypes.data_as(ctypes.c_void_p), disc.shape[0], int(level), int(div), _p(img),
+                                _p(vis), ctypes.byref(cnt))
+    if rc:
+        raise RuntimeError(f"or_rev_node_test failed ({rc})")
+    return img, int(cnt.value)
+
+
+def rev_soft(pos, nrm, maps, light_mvp_biased, trans, disc, vis, params, rpcf=False) -> np.ndarray:
+    """RevectorizationBasedAccurateSoftShadow.frag: maps (N, smh, smw),
+    trans (N, 4) with w + level * 10000, disc (L, h, w) uint8, vis (h, w);
+    `params` supplies shadow_intensity, max_search, depth_threshold and the
+    pre-evaluation fields."""
+    pos, nrm, maps = _c32(pos), _c32(nrm), _c32(maps)
+    _, h, w = pos.shape
+    n, smh, smw = maps.shape
+    tr = _c32(trans).reshape(n, 4)
+    disc = np.ascontiguousarray(disc, dtype=np.uint8)
+    assert disc.ndim == 3 and disc.shape[1:] == (h, w)
+    vis = _c32(vis)
+    m = _c32(light_mvp_biased).reshape(16)
+    out = np.empty((h, w), np.float32)
+    rc = lib().or_rev_soft(_p(pos), _p(nrm), w, h, _p(maps), n, smw, smh, _p(m), _p(tr),
+                           disc.ctypes.data_as(ctypes.c_void_p), disc.shape[0], _p(vis), ctypes.byref(params),
+                           1 if rpcf else 0, _p(out))
+    if rc:
+        raise RuntimeError(f"or_rev_soft failed ({rc})")
+    return out
 
 
 def _mesh_args(positions, indices):

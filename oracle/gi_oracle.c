@@ -1401,6 +1401,8 @@ static inline float gl_step(float edge, float x) { return x < edge ? 0.0f : 1.0f
 typedef struct rbssm_ctx {
     rb_ctx rb;
     float newDepth; /* the shader global written by getDisc(vec4 discType) */
+    int acc;        /* 1: the accurate shader's smoothONDS */
+    int cap;        /* its scan bound */
 } rbssm_ctx;
 
 static int ss_discf(rbssm_ctx* c, float x, float y, float z, float dx, float dy, float type)
@@ -1459,13 +1461,24 @@ static float ss_normalize(const rbssm_ctx* c, float a, float b, float sub)
 
 #define CL01(v) gclamp((v), 0.0f, 1.0f)
 
-/* smoothONDS :311-1117, with its arguments as by-value copies */
+/* smoothONDS, RBSSM.frag:311-1117 (acc == 0) and
+ * RevectorizationBasedAccurateSoftShadow.frag:320-1138 (acc == 1), with the
+ * arguments as by-value copies.  The accurate shader adds three early exits
+ * and repeats eight scans that RBSSM.frag runs once (their `while` lines are
+ * commented out there); c->cap bounds each scan (4 (W + H) + 16 steps, never
+ * reached for finite depths: every step moves at least one texel and the map
+ * border, depth 0, ends the scan). */
 static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, float ny, float dr, float dg, float db,
                        float ux, float uy)
 {
     const float sx = c->rb.stepx, sy = c->rb.stepy;
+    const int acc = c->acc;
     if (db == 0.0f) {
-        /* :316-421 positive entering discontinuity on both axes */
+        if (acc) { /* :325-331 */
+            if (nx == -1.0f && ny == -1.0f) return 1.0f;
+            if (dr == 0.75f && dg == 0.75f) return 0.0f;
+        }
+        /* positive entering discontinuity on both axes */
         if (nx <= -2.0f && ny <= -2.0f && (dr == 0.75f || dg == 0.75f)) {
             int left = 1, right = 1, bottom = 1, top = 1;
             if (dr == 0.75f) {
@@ -1526,25 +1539,40 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
             else if (!right && !top) return CL01((1.0f - uy) - (1.0f - ux));
         }
         if (dr == 0.75f || dg == 0.75f) {
-            if (dr == 0.75f && dg != 0.0f) { /* :429-455 */
-                ly = ly + ((dg - 0.75f) * 4.0f) * sy;
-                int left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                int right = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+            if (dr == 0.75f && dg != 0.0f) { /* entering left and right: scan while umbra on both sides */
+                int left = 1, right = 1, guard = 0;
+                while (left && right) {
+                    ly = ly + ((dg - 0.75f) * 4.0f) * sy;
+                    left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                    right = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    if (!acc || lz > rb_tex(&c->rb, lx, ly)) break;
+                    if (++guard >= c->cap) break;
+                }
                 if (!left && !right) return CL01(1.0f - ny);
                 float sub = gmix(1.0f - uy, uy, gl_step(1.0f, dg));
                 float a = gmix(sub, CL01((1.0f - ux) - (ny - 1.0f)), gl_step(1.0f, right ? 1.0f : 0.0f));
                 float b = gmix(sub, CL01(ux - (ny - 1.0f)), gl_step(1.0f, left ? 1.0f : 0.0f));
                 return gmin(a, b);
             }
-            if (dr == 0.75f && dg == 0.0f) { /* :458-545 */
-                float ty = ly + sy;
-                int topCenter = !(cmp_le(lz, rb_tex(&c->rb, lx, ty)) != 0.0f);
-                int topLeft = ss_discv(c, lx, ty, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                int topRight = ss_discv(c, lx, ty, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
-                float by = ly - sy;
-                int bottomCenter = !(cmp_le(lz, rb_tex(&c->rb, lx, by)) != 0.0f);
-                int bottomLeft = ss_discv(c, lx, by, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
-                int bottomRight = ss_discv(c, lx, by, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+            if (dr == 0.75f && dg == 0.0f) { /* entering left and right only: scan up and down */
+                int topLeft = 1, topRight = 1, bottomLeft = 1, bottomRight = 1;
+                int topCenter = 0, bottomCenter = 0, guard = 0;
+                float ty = ly, by = ly;
+                while (topLeft && topRight && bottomLeft && bottomRight && (!topCenter || !bottomCenter)) {
+                    if (!topCenter) {
+                        ty = ty + sy;
+                        topCenter = !(cmp_le(lz, rb_tex(&c->rb, lx, ty)) != 0.0f);
+                        topLeft = ss_discv(c, lx, ty, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                        topRight = ss_discv(c, lx, ty, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    }
+                    if (!bottomCenter) {
+                        by = by - sy;
+                        bottomCenter = !(cmp_le(lz, rb_tex(&c->rb, lx, by)) != 0.0f);
+                        bottomLeft = ss_discv(c, lx, by, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
+                        bottomRight = ss_discv(c, lx, by, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
+                    }
+                    if (!acc || ++guard >= c->cap) break;
+                }
                 if (topCenter) { topLeft = 1; topRight = 1; }
                 if (bottomCenter) { bottomLeft = 1; bottomRight = 1; }
                 if (ny <= -2.0f) {
@@ -1559,25 +1587,40 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
                 else if (!bottomRight || !topRight) return CL01(ux - (ny - 1.0f));
                 else return CL01((1.0f - ux) - (ny - 1.0f));
             }
-            if (dr != 0.0f && dg == 0.75f) { /* :548-575 */
-                lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
-                int bottom = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
-                int top = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+            if (dr != 0.0f && dg == 0.75f) { /* entering top and bottom: scan while umbra above and below */
+                int bottom = 1, top = 1, guard = 0;
+                while (bottom && top) {
+                    lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
+                    bottom = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                    top = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    if (!acc || lz > rb_tex(&c->rb, lx, ly)) break;
+                    if (++guard >= c->cap) break;
+                }
                 if (!bottom && !top) return CL01(1.0f - nx);
                 float sub = gmix(ux, 1.0f - ux, gl_step(dr, 0.25f));
                 float a = gmix(sub, CL01(uy - (nx - 1.0f)), gl_step(1.0f, top ? 1.0f : 0.0f));
                 float b = gmix(sub, CL01((1.0f - uy) - (nx - 1.0f)), gl_step(1.0f, bottom ? 1.0f : 0.0f));
                 return gmin(a, b);
             }
-            if (dr == 0.0f && dg == 0.75f) { /* :578-662 */
-                float rx = lx + sx;
-                int rightCenter = !(cmp_le(lz, rb_tex(&c->rb, rx, ly)) != 0.0f);
-                int bottomRight = ss_discv(c, rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
-                int topRight = ss_discv(c, rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
-                float lxx = lx - sx;
-                int leftCenter = !(cmp_le(lz, rb_tex(&c->rb, lxx, ly)) != 0.0f);
-                int bottomLeft = ss_discv(c, lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
-                int topLeft = ss_discv(c, lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+            if (dr == 0.0f && dg == 0.75f) { /* entering top and bottom only: scan right and left */
+                int topLeft = 1, topRight = 1, bottomLeft = 1, bottomRight = 1;
+                int leftCenter = 0, rightCenter = 0, guard = 0;
+                float rx = lx, lxx = lx;
+                while (topLeft && topRight && bottomLeft && bottomRight && (!rightCenter || !leftCenter)) {
+                    if (!rightCenter) {
+                        rx = rx + sx;
+                        rightCenter = !(cmp_le(lz, rb_tex(&c->rb, rx, ly)) != 0.0f);
+                        bottomRight = ss_discv(c, rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                        topRight = ss_discv(c, rx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    }
+                    if (!leftCenter) {
+                        lxx = lxx - sx;
+                        leftCenter = !(cmp_le(lz, rb_tex(&c->rb, lxx, ly)) != 0.0f);
+                        bottomLeft = ss_discv(c, lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
+                        topLeft = ss_discv(c, lxx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
+                    }
+                    if (!acc || ++guard >= c->cap) break;
+                }
                 if (rightCenter) { bottomRight = 1; topRight = 1; }
                 if (leftCenter) { topLeft = 1; bottomLeft = 1; }
                 if (nx <= -2.0f) {
@@ -1593,7 +1636,7 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
                 else return CL01((1.0f - uy) - (nx - 1.0f));
             }
         }
-        if (dr > 0.0f && dg > 0.0f) { /* :667-722 corner */
+        if (dr > 0.0f && dg > 0.0f) { /* corner */
             lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
             int horizontal = ss_discf(c, lx, ly, lz, 1.0f, 0.0f, 0.0f);
             lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sx;
@@ -1631,9 +1674,10 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
         if (ny <= -2.0f) return gmix(ux, 1.0f - ux, gl_step(dr, 0.25f));
         if (dg > 0.0f) return dg == 0.5f ? CL01((1.0f - uy) - (nx - 1.0f)) : CL01(uy - (nx - 1.0f));
         if (dr > 0.0f) return dr == 0.5f ? CL01(ux - (ny - 1.0f)) : CL01((1.0f - ux) - (ny - 1.0f));
-    } else { /* :758-1113 exiting discontinuity */
+    } else { /* exiting discontinuity */
+        if (acc && dr == 0.75f && dg == 0.75f) return 1.0f; /* :777-779 */
         if (dr == 0.75f || dg == 0.75f) {
-            if (dr == 0.75f && dg == 0.0f) { /* :763-818 */
+            if (dr == 0.75f && dg == 0.0f) { /* exiting left and right only */
                 int left = ss_discv(c, lx - sx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
                 int right = ss_discv(c, lx + sx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
                 if (left && right) return CL01(ny);
@@ -1641,14 +1685,22 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
                     if (!left) return gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
                     return gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
                 }
-                float ry = ly + 1.0f * sy; /* count 0: mult 1, even */
-                left = ss_discf(c, lx - sx, ry, lz, 1.0f, 0.0f, 0.0f);
-                right = ss_discf(c, lx + sx, ry, lz, 1.0f, 0.0f, 0.0f);
+                /* rows +1, -1, +2, -2, ... */
+                int count = 0;
+                while (!left && !right) {
+                    int mult = count / 2 + 1;
+                    float ry = (count % 2 == 0) ? ly + (float)mult * sy : ly - (float)mult * sy;
+                    left = ss_discf(c, lx - sx, ry, lz, 1.0f, 0.0f, 0.0f);
+                    right = ss_discf(c, lx + sx, ry, lz, 1.0f, 0.0f, 0.0f);
+                    if (!acc || lz <= rb_tex(&c->rb, lx, ly)) break;
+                    count++;
+                    if (count >= c->cap) break;
+                }
                 if (left && right) return CL01(ny);
                 if (!left) return gmix(1.0f - ux, CL01(ny - ux), gl_step(-2.0f, ny));
                 return gmix(ux, CL01(ny - (1.0f - ux)), gl_step(-2.0f, ny));
             }
-            if (dr == 0.75f && dg != 0.0f) { /* :821-878 */
+            if (dr == 0.75f && dg != 0.0f) { /* exiting left and right */
                 ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * sy;
                 int left = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.5f, 0.0f, 0.0f);
                 int right = ss_discv(c, lx, ly, lz, 0.0f, 1.0f, 0.25f, 0.0f, 0.0f);
@@ -1662,14 +1714,20 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
                     return gmax(a, b);
                 }
                 if (dg == 0.75f) return 0.0f;
-                left = ss_discf(c, lx - sx, ly, lz, 1.0f, 0.0f, 0.0f);
-                right = ss_discf(c, lx + sx, ly, lz, 1.0f, 0.0f, 0.0f);
+                int guard = 0;
+                while (!left && !right) {
+                    left = ss_discf(c, lx - sx, ly, lz, 1.0f, 0.0f, 0.0f);
+                    right = ss_discf(c, lx + sx, ly, lz, 1.0f, 0.0f, 0.0f);
+                    if (!acc || lz <= rb_tex(&c->rb, lx, ly)) break;
+                    ly = ly + ((0.5f - dg) * 8.0f - 1.0f) * sy;
+                    if (++guard >= c->cap) break;
+                }
                 if (left && right) return CL01(ny);
                 a = gmix(1.0f - ux, ux, gl_step(1.0f, left ? 1.0f : 0.0f));
                 b = gmix(uy, 1.0f - uy, gl_step(dg, 0.25f));
                 return gmax(a, b);
             }
-            if (dr == 0.0f && dg == 0.75f) { /* :881-936 */
+            if (dr == 0.0f && dg == 0.75f) { /* exiting top and bottom only */
                 int top = ss_discf(c, lx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
                 int bottom = ss_discf(c, lx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
                 if (bottom && top) return CL01(nx);
@@ -1677,14 +1735,22 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
                     if (!top) return gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
                     return gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
                 }
-                float rx = lx + 1.0f * sx; /* count 0: mult 1, even */
-                top = ss_discf(c, rx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
-                bottom = ss_discf(c, rx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                /* columns +1, -1, +2, -2, ... */
+                int count = 0;
+                while (!top && !bottom) {
+                    int mult = count / 2 + 1;
+                    float rx = (count % 2 == 0) ? lx + (float)mult * sx : lx - (float)mult * sx;
+                    top = ss_discf(c, rx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
+                    bottom = ss_discf(c, rx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                    if (!acc || lz <= rb_tex(&c->rb, lx, ly)) break;
+                    count++;
+                    if (count >= c->cap) break;
+                }
                 if (top && bottom) return CL01(nx);
                 if (!top) return gmix(1.0f - uy, CL01(nx - uy), gl_step(-2.0f, nx));
                 return gmix(uy, CL01(nx - (1.0f - uy)), gl_step(-2.0f, nx));
             }
-            if (dr != 0.0f && dg == 0.75f) { /* :939-997 */
+            if (dr != 0.0f && dg == 0.75f) { /* exiting top and bottom */
                 lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
                 int bottom = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.5f, 0.0f);
                 int top = ss_discv(c, lx, ly, lz, 1.0f, 0.0f, 0.0f, 0.25f, 0.0f);
@@ -1698,15 +1764,21 @@ static float ss_smooth(rbssm_ctx* c, float lx, float ly, float lz, float nx, flo
                     return gmax(a, b);
                 }
                 if (dr == 0.75f) return 0.0f;
-                top = ss_discf(c, lx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
-                bottom = ss_discf(c, lx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                int guard = 0;
+                while (!top && !bottom) {
+                    top = ss_discf(c, lx, ly - sy, lz, 0.0f, 1.0f, 0.0f);
+                    bottom = ss_discf(c, lx, ly + sy, lz, 0.0f, 1.0f, 0.0f);
+                    if (!acc || lz <= rb_tex(&c->rb, lx, ly)) break;
+                    lx = lx - ((0.5f - dr) * 8.0f - 1.0f) * sx;
+                    if (++guard >= c->cap) break;
+                }
                 if (top && bottom) return CL01(nx);
                 a = gmix(1.0f - uy, uy, gl_step(1.0f, top ? 1.0f : 0.0f));
                 b = gmix(1.0f - ux, ux, gl_step(dr, 0.25f));
                 return gmax(a, b);
             }
         }
-        if (dr > 0.0f && dg > 0.0f) { /* :1002-1061 corner */
+        if (dr > 0.0f && dg > 0.0f) { /* corner */
             lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sx;
             int horizontal = ss_discf(c, lx, ly, lz, 1.0f, 0.0f, db);
             lx = lx + ((0.5f - dr) * 8.0f - 1.0f) * sx;
@@ -1828,6 +1900,8 @@ int or_rbssm(const float* pos, const float* nrm, int w, int h, const float* dept
             c.rb.si = p->shadow_intensity;
             c.rb.filtered = 0;
             c.newDepth = 0.0f;
+            c.acc = 0;
+            c.cap = 0;
             shadow = rbssm_pixel(&c, p, blocker_search_size, kernel_size, light_source_radius, z_near, n0, n1, n2);
         }
         out[k] = shadow;
@@ -1847,6 +1921,8 @@ int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float
     c.rb.si = p->shadow_intensity;
     c.rb.filtered = 0;
     c.newDepth = 0.0f;
+    c.acc = 0;
+    c.cap = 0;
     float ux = gfract(x * (float)c.rb.W), uy = gfract(y * (float)c.rb.H);
     float dr, dg, db;
     compute_discontinuity(&c.rb, x, y, z, rb_tex(&c.rb, x, y), &dr, &dg, &db);
@@ -1858,5 +1934,210 @@ int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float
     float nx = ss_normalize(&c, out[3], out[4], ux), ny = ss_normalize(&c, out[5], out[6], uy);
     out[7] = nx; out[8] = ny;
     out[9] = ss_smooth(&c, x, y, z, nx, ny, dr, dg, db, ux, uy);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Revectorization-based adaptive sampling (ShadowParams.                    */
+/* revectorizationBasedAdaptiveSampling, SoftShadowMapping/src/main.cpp)     */
+/* ------------------------------------------------------------------------ */
+
+/* Discontinuity.frag's (r, g, b, 1) as one byte: bits 0-1 r*4, bits 2-3 g*4,
+ * bit 4 b, bit 5 alpha; 0xFF is the layer's clear colour (1, 1, 1, 1). */
+static unsigned char rev_encode(float r, float g, float b)
+{
+    return (unsigned char)((int)(r * 4.0f) | ((int)(g * 4.0f) << 2) | (b != 0.0f ? 16 : 0) | 32);
+}
+
+static void rbssm_ctx_init(rbssm_ctx* c, const float* D, int smw, int smh, float thr, int max_search, float si,
+                           int acc)
+{
+    c->rb.D = D; c->rb.W = smw; c->rb.H = smh;
+    c->rb.stepx = (float)(1.0 / (double)smw);
+    c->rb.stepy = (float)(1.0 / (double)smh);
+    c->rb.thr = thr;
+    c->rb.maxSearch = max_search;
+    c->rb.si = si;
+    c->rb.filtered = 0;
+    c->newDepth = 0.0f;
+    c->acc = acc;
+    c->cap = 4 * (smw + smh) + 16;
+}
+
+/* HardShadow/Discontinuity.frag:66-88 */
+int or_rev_discontinuity(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
+                         unsigned char* out)
+{
+    if (!pos || !nrm || !depth || !p || !out || w <= 0 || h <= 0) return -1;
+    size_t plane = (size_t)w * (size_t)h;
+    for (size_t k = 0; k < plane; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { out[k] = 0xFF; continue; } /* discard: the cleared layer */
+        float sc[4];
+        m4v(p->light_mvp, vx, vy, vz, 1.0f, sc);
+        float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        float shadow = pre_evaluation(p, vx, vy, vz, nrm[k], nrm[plane + k], nrm[2 * plane + k]);
+        unsigned char code = 0;
+        if (sc[3] > 0.0f && shadow == 1.0f) {
+            rbssm_ctx c;
+            rbssm_ctx_init(&c, depth, p->sm_width, p->sm_height, p->depth_threshold, p->max_search,
+                           p->shadow_intensity, 1);
+            float dr, dg, db;
+            compute_discontinuity(&c.rb, n0, n1, n2, rb_tex(&c.rb, n0, n1), &dr, &dg, &db);
+            code = rev_encode(dr, dg, db);
+        }
+        out[k] = code;
+    }
+    return 0;
+}
+
+/* QuadTree/RevectorizationBasedReprojection.frag:14-51, then Evaluation.frag
+ * as or_qt_node_test.  vis is read and written in place. */
+int or_rev_node_test(const float* pos, int w, int h, const float* maps, int smw, int smh, const int idx[4],
+                     const float mvps[64], const unsigned char* disc, int disc_layers, int level, int div,
+                     float* img, float* vis, unsigned* count)
+{
+    if (!pos || !maps || !idx || !mvps || !disc || !img || !vis || !count || div < 1 || div > w || div > h ||
+        disc_layers <= 0 || level < 0)
+        return -1;
+    size_t plane = (size_t)w * (size_t)h, mapsz = (size_t)smw * (size_t)smh;
+    for (size_t k = 0; k < plane; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { img[k] = 0.63f; vis[k] = 0.63f; continue; } /* both targets' clear colour */
+        float shadow[4];
+        int ndisc = 0;
+        for (int i = 0; i < 4; ++i) {
+            float sc[4];
+            m4v(mvps + 16 * i, vx, vy, vz, 1.0f, sc);
+            float s = sc[0] / sc[3], t = sc[1] / sc[3], z = sc[2] / sc[3];
+            shadow[i] = (z <= or_fetch(maps + (size_t)idx[i] * mapsz, smw, smh, s, t)) ? 1.0f : 0.0f;
+            /* texture2DArray clamps the layer to the array's depth */
+            int layer = idx[i] < disc_layers ? idx[i] : disc_layers - 1;
+            unsigned char code = disc[(size_t)layer * plane + k];
+            float r = (float)(code & 3) / 4.0f, g = (float)((code >> 2) & 3) / 4.0f;
+            if ((r + g) > 0.0f) ndisc++;
+        }
+        float sum = shadow[0] + shadow[1] + shadow[2] + shadow[3];
+        if (ndisc >= 2) sum = 5.0f;
+        float v = vis[k];
+        if (level == 0) {
+            if (sum == 0.0f || sum == 4.0f) v = sum / 4.0f;
+            else v = 0.5f;
+        } else {
+            if ((v == 0.0f && sum != 0.0f) || (v == 1.0f && sum != 4.0f)) v = 0.5f;
+        }
+        img[k] = sum;
+        vis[k] = v;
+    }
+    int wd = w / div, hd = h / div;
+    unsigned passed = 0;
+    for (int j = 0; j < hd; ++j) {
+        for (int i = 0; i < wd; ++i) {
+            int a = qt_texel(i, w, wd), b = qt_texel(j, h, hd);
+            float sum = (a < 0 || b < 0) ? 0.0f : img[(size_t)b * w + a];
+            if (sum == 0.63f || sum == 0.0f || sum == 4.0f || sum == 5.0f) continue;
+            int ok = 1;
+            for (int y = -1; y <= 1 && ok; ++y)
+                for (int x = -1; x <= 1 && ok; ++x) {
+                    int na = qt_texel(i + x, w, wd), nb = qt_texel(j + y, h, hd);
+                    float nsum = (na < 0 || nb < 0) ? 0.0f : img[(size_t)nb * w + na];
+                    if (sum != nsum) ok = 0;
+                }
+            passed += (unsigned)ok;
+        }
+    }
+    *count = passed;
+    return 0;
+}
+
+/* RSMSS, RevectorizationBasedAccurateSoftShadow.frag:1175-1186 */
+static float rev_rsmss(rbssm_ctx* c, float x, float y, float z, float dr, float dg, float db)
+{
+    float ux = gfract(x * (float)c->rb.W), uy = gfract(y * (float)c->rb.H);
+    float l = ss_length(c, dr, dg, db, x, y, z, -1.0f, 0.0f);
+    float r = ss_length(c, dr, dg, db, x, y, z, 1.0f, 0.0f);
+    float dn = ss_length(c, dr, dg, db, x, y, z, 0.0f, -1.0f);
+    float up = ss_length(c, dr, dg, db, x, y, z, 0.0f, 1.0f);
+    float nx = ss_normalize(c, l, r, ux), ny = ss_normalize(c, dn, up, uy);
+    float fill = ss_smooth(c, x, y, z, nx, ny, dr, dg, db, ux, uy);
+    return gmix(fill, 1.0f, c->rb.si);
+}
+
+/* RevectorizationBasedAccurateSoftShadow.frag:1252-1331.  p supplies
+ * shadow_intensity, max_search, depth_threshold, mv, normal_matrix and
+ * light_position; m: B * lightMVP (columns 0-2 used). */
+int or_rev_soft(const float* pos, const float* nrm, int w, int h, const float* maps, int nsamples, int smw, int smh,
+                const float m[16], const float* trans, const unsigned char* disc, int disc_layers, const float* vis,
+                const or_params* p, int rpcf, float* out)
+{
+    if (!pos || !nrm || !maps || !trans || !disc || !vis || !p || !out || w <= 0 || h <= 0 || nsamples <= 0 ||
+        disc_layers <= 0)
+        return -1;
+    size_t plane = (size_t)w * (size_t)h, mapsz = (size_t)smw * (size_t)smh;
+    const float si = p->shadow_intensity;
+    long long n = (long long)plane;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 64) OMP_THREADS
+#endif
+    for (long long k = 0; k < n; ++k) {
+        float vx = pos[k], vy = pos[plane + k], vz = pos[2 * plane + k];
+        if (vx == 0.0f) { out[k] = 0.0f; continue; }
+        float shadow = pre_evaluation(p, vx, vy, vz, nrm[k], nrm[plane + k], nrm[2 * plane + k]);
+        if (shadow == 1.0f) shadow = gclamp(vis[k], si, 1.0f);
+        if (shadow == 0.5f) {
+            float cm[4];
+            for (int i = 0; i < 4; ++i) {
+                float a = m[0 * 4 + i] * vx;
+                a = a + m[1 * 4 + i] * vy;
+                a = a + m[2 * 4 + i] * vz;
+                cm[i] = a;
+            }
+            float accShadow = 0.0f, count = 0.0f;
+            for (int idx = 0; idx < nsamples; ++idx) {
+                const float* T = trans + 4 * idx;
+                int level = (int)(T[3] / 10000.0f);
+                if (level > 4) level = 4;
+                float wValue = T[3] - (float)(level * 10000);
+                float pw = 1.0f;
+                for (int e = 1; e <= level; ++e) pw = pw * 2.0f;
+                float accFactor = pw + 1.0f;
+                accFactor = 1.0f / (accFactor * accFactor);
+                float cx = cm[0] + T[0], cy = cm[1] + T[1], cz = cm[2] + T[2], cw = cm[3] + wValue;
+                float sx = cx / cw, sy = cy / cw, sz = cz / cw;
+                const float* D = maps + (size_t)idx * mapsz;
+                rbssm_ctx c;
+                rbssm_ctx_init(&c, D, smw, smh, p->depth_threshold, p->max_search, si, 1);
+                if (!rpcf) {
+                    float d = or_fetch(D, smw, smh, sx, sy);
+                    int layer = idx < disc_layers ? idx : disc_layers - 1;
+                    unsigned char code = disc[(size_t)layer * plane + k];
+                    float dr = (float)(code & 3) * 0.25f, dg = (float)((code >> 2) & 3) * 0.25f;
+                    float db = (float)((code >> 4) & 1);
+                    if (dr > 0.0f || dg > 0.0f) accShadow = accShadow + rev_rsmss(&c, sx, sy, sz, dr, dg, db) * accFactor;
+                    else accShadow = accShadow + ((sz <= d) ? 1.0f : si) * accFactor;
+                } else {
+                    float incrWidth = 1.0f / (float)smw, incrHeight = 1.0f / (float)smh;
+                    float offset = 1.0f, stepSize = 2.0f * offset / 2.0f;
+                    float illum = 0.0f;
+                    int pcfCount = 0;
+                    for (float ww = -offset; ww <= offset; ww += stepSize) {
+                        for (float hh = -offset; hh <= offset; hh += stepSize) {
+                            float tx = sx + ww * incrWidth, ty = sy + hh * incrHeight;
+                            float d = or_fetch(D, smw, smh, tx, ty);
+                            float dr, dg, db;
+                            compute_discontinuity(&c.rb, tx, ty, sz, d, &dr, &dg, &db);
+                            if (dr > 0.0f || dg > 0.0f) illum = illum + rev_rsmss(&c, tx, ty, sz, dr, dg, db);
+                            else illum = illum + ((sz <= d) ? 1.0f : si);
+                            pcfCount++;
+                        }
+                    }
+                    accShadow = accShadow + (illum / (float)pcfCount) * accFactor;
+                }
+                count = count + accFactor;
+            }
+            shadow = accShadow / count;
+        }
+        out[k] = shadow;
+    }
     return 0;
 }

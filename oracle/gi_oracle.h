@@ -128,6 +128,29 @@ int or_rbssm(const float* pos, const float* nrm, int w, int h, const float* dept
  * out[9] smoothONDS's fill before the shadow-intensity mix. */
 int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float z, float* out);
 
+/* Revectorization-based adaptive sampling.  Discontinuity layers are
+ * w*h bytes: bits 0-1 r*4, bits 2-3 g*4, bit 4 discType, bit 5 alpha, 0xFF
+ * for background (Discontinuity.frag's RGBA8 (r, g, b, 1) and the clear
+ * colour).  HardShadow/Discontinuity.frag:66-88 for one map: p->light_mvp
+ * (B * its lightMVP), sm_width/sm_height, depth_threshold, max_search, and
+ * the pre-evaluation's mv, normal_matrix, light_position, shadow_intensity. */
+int or_rev_discontinuity(const float* pos, const float* nrm, int w, int h, const float* depth, const or_params* p,
+                         unsigned char* out);
+
+/* QuadTree/RevectorizationBasedReprojection.frag:14-51 (vis read and
+ * written in place; layers clamp to disc_layers - 1) and Evaluation.frag on
+ * the (w/div) x (h/div) viewport; *count = passing pixels. */
+int or_rev_node_test(const float* pos, int w, int h, const float* maps, int smw, int smh, const int idx[4],
+                     const float mvps[64], const unsigned char* disc, int disc_layers, int level, int div,
+                     float* img, float* vis, unsigned* count);
+
+/* RevectorizationBasedAccurateSoftShadow.frag:1252-1331 (RSMSS :1175-1186,
+ * smoothONDS :320-1138 with its scans bounded by 4 (W + H) + 16 steps).
+ * rpcf: ShadowParams.RPCF. */
+int or_rev_soft(const float* pos, const float* nrm, int w, int h, const float* maps, int nsamples, int smw, int smh,
+                const float m[16], const float* trans, const unsigned char* disc, int disc_layers, const float* vis,
+                const or_params* p, int rpcf, float* out);
+
 /* PhongShading.frag:11-46.  pos/nrm: G-buffer planes (normal flipped on back
  * faces), fac: 1.0 front / 0.0 back, shadow: w*h R channel, color: RGBA per
  * pixel or NULL (p->color everywhere); out: RGBA per pixel. */

@@ -343,6 +343,134 @@ def test_soft_shadow_full_size_properties(gi, cuda):
     assert float((mean[~bg] - fg).abs().max()) <= 1e-5
 
 
+def _rev_frame(gi, nsamples=9):
+    """The area-light scene with nsamples corner maps, their discontinuity
+    layers (corner eye unrotated in the pre-evaluation, as during the quad-tree
+    evaluation) and the uniforms the final pass uses."""
+    wl = gi.scenes.config3(nsamples=nsamples, view_w=640, view_h=360, sm=1024, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 640, 360)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, 1024, 1024, polygon_offset=(4.0, 20.0))
+    biased = np.stack([gi.biased(m) for m in mvps])
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    return wl, g, mvps, maps, biased, MV, nm, lpos
+
+
+def test_rev_discontinuity_matches_oracle(gi, oracle_lib, cuda):
+    """Discontinuity.frag per corner map, and no write past the last layer."""
+    wl, g, mvps, maps, biased, MV, nm, _ = _rev_frame(gi, nsamples=4)
+    disc = torch.full((3, g.height, g.width), 7, dtype=torch.uint8, device="cuda")
+    gp, gn = _np(g.position), _np(g.normal)
+    for i in range(4):
+        eye = tuple(float(v) for v in wl.lights[i].eye)
+        print(f"rev discontinuity map {i}", flush=True)
+        gi.passes.rev_discontinuity(g, maps, i, biased[i], MV, nm, eye, 0.25, disc)
+        if i < 3:
+            op = oracle_lib.make_params(biased[i], 1024, 1024, "naive", shadow_intensity=0.25, max_search=16,
+                                        pre_evaluation=True, mv=MV, normal_matrix=nm, light_position=eye)
+            want = oracle_lib.rev_discontinuity(gp, gn, _np(maps[i]), op)
+            got = disc[i].cpu().numpy()
+            assert (got == want).all(), f"map {i}: {int((got != want).sum())} pixels differ"
+            assert ((want & 15) != 0).sum() > 100  # the occluder's edges are there
+    # slot 3 has no layer: layers 0-2 are as written above, nothing else changed
+    assert int((disc == 7).sum()) == 0
+
+
+def test_rev_node_test_matches_oracle(gi, oracle_lib, cuda):
+    """RevectorizationBasedReprojection.frag + Evaluation.frag at level 0 and
+    then level 2 (div 2) over the same visibility map, with the layer clamp
+    (a 3-layer array, corner slots up to 3)."""
+    wl, g, mvps, maps, biased, MV, nm, _ = _rev_frame(gi, nsamples=4)
+    gp, gn = _np(g.position), _np(g.normal)
+    disc = gi.passes.rev_discontinuity_layers(g, layers=3)
+    disc_np = np.zeros((3, g.height, g.width), np.uint8)
+    for i in range(3):
+        eye = tuple(float(v) for v in wl.lights[i].eye)
+        gi.passes.rev_discontinuity(g, maps, i, biased[i], MV, nm, eye, 0.25, disc)
+        op = oracle_lib.make_params(biased[i], 1024, 1024, "naive", shadow_intensity=0.25, max_search=16,
+                                    pre_evaluation=True, mv=MV, normal_matrix=nm, light_position=eye)
+        disc_np[i] = oracle_lib.rev_discontinuity(gp, gn, _np(maps[i]), op)
+    vis = torch.zeros((g.height, g.width), dtype=torch.float32, device="cuda")
+    vis_np = np.zeros((g.height, g.width), np.float32)
+    for level, order, div in ((0, [0, 1, 2, 3], 1), (2, [3, 2, 1, 0], 2)):
+        print(f"rev node test level {level}", flush=True)
+        img, count = gi.passes.rev_quadtree_node_test(g, maps, order, biased[order], disc, level, vis, div)
+        want_img, want_count = oracle_lib.rev_node_test(gp, _np(maps), order, biased[order], disc_np, level, vis_np,
+                                                        div)
+        assert bits_equal(_np(img), want_img)
+        assert bits_equal(_np(vis), vis_np)
+        assert count == want_count
+        assert (want_img == 5.0).sum() > 0 and (vis_np == 0.5).sum() > 0
+
+
+@pytest.mark.parametrize("rpcf", [False, True])
+def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf):
+    """RevectorizationBasedAccurateSoftShadow.frag with mixed quad-tree levels,
+    discontinuity layers for the first 6 of 9 maps (the rest read a
+    3-layer-clamped or stale layer as the reference does), and the visibility
+    classification of one level-0 node test."""
+    wl, g, mvps, maps, biased, MV, nm, lpos = _rev_frame(gi, nsamples=9)
+    gp, gn = _np(g.position), _np(g.normal)
+    layers = 6
+    disc = gi.passes.rev_discontinuity_layers(g, layers=layers)
+    disc_np = np.zeros((layers, g.height, g.width), np.uint8)
+    for i in range(layers):
+        eye = tuple(float(v) for v in wl.lights[i].eye)
+        gi.passes.rev_discontinuity(g, maps, i, biased[i], MV, nm, eye, 0.25, disc)
+        op = oracle_lib.make_params(biased[i], 1024, 1024, "naive", shadow_intensity=0.25, max_search=16,
+                                    pre_evaluation=True, mv=MV, normal_matrix=nm, light_position=eye)
+        disc_np[i] = oracle_lib.rev_discontinuity(gp, gn, _np(maps[i]), op)
+    vis = torch.zeros((g.height, g.width), dtype=torch.float32, device="cuda")
+    vis_np = np.zeros((g.height, g.width), np.float32)
+    gi.passes.rev_quadtree_node_test(g, maps, [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc, 0, vis)
+    oracle_lib.rev_node_test(gp, _np(maps), [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc_np, 0, vis_np)
+    levels = [0, 1, 2, 3, 4, 1, 6, 2, 0]
+    common, trans = gi.passes.adaptive_trans(mvps, levels)
+    params = gi.RevSoftParams(MV, nm, lpos, RPCF=rpcf, maxSearch=16, depthThreshold=2.5e-6)
+    print(f"rev soft rpcf={rpcf}", flush=True)
+    got = _np(gi.passes.render_rev_adaptive(g, maps, common, trans, disc, vis, params))
+    op = oracle_lib.make_params(common, 1024, 1024, "naive", shadow_intensity=0.25, max_search=16,
+                                depth_threshold=2.5e-6, pre_evaluation=True, mv=MV, normal_matrix=nm,
+                                light_position=lpos)
+    want = oracle_lib.rev_soft(gp, gn, _np(maps), common, trans, disc_np, vis_np, op, rpcf=rpcf)
+    assert bits_equal(got, want), mismatch_report(got, want, f"rev soft rpcf={rpcf}")
+    pen = (vis_np == 0.5) & (gp[0] != 0)
+    assert pen.sum() > 100
+    assert ((want > 0.25) & (want < 1.0)).sum() > 100
+
+
+def test_rev_rejects(gi, cuda):
+    import ctypes
+
+    wl, g, mvps, maps, biased, MV, nm, lpos = _rev_frame(gi, nsamples=4)
+    L = gi.lib()
+    disc = gi.passes.rev_discontinuity_layers(g, layers=4)
+    vis = torch.zeros((g.height, g.width), dtype=torch.float32, device="cuda")
+    out = torch.empty((g.height, g.width), dtype=torch.float32, device="cuda")
+    c = g.to_c(with_normals=True)
+    F16, F9, F3 = ctypes.c_float * 16, ctypes.c_float * 9, ctypes.c_float * 3
+    args = (F16(*biased[0].reshape(-1)), F16(*MV.reshape(-1)), F9(*nm.reshape(-1)), F3(*lpos))
+    for idx, layers in ((-1, 4), (4, 4), (0, 0)):
+        assert L.gi_rev_discontinuity_pass(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, idx, *args,
+                                           ctypes.c_float(0.25), disc.data_ptr(), layers, None) == gi._lib.GI_EINVAL
+    nan = F16(*([float("nan")] * 16))
+    assert L.gi_rev_discontinuity_pass(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, 0, nan, *args[1:],
+                                       ctypes.c_float(0.25), disc.data_ptr(), 4, None) == gi._lib.GI_ENONFINITE
+    cnt = torch.zeros((1,), dtype=torch.int32, device="cuda")
+    idx4 = (ctypes.c_int * 4)(0, 1, 2, 4)
+    m64 = (ctypes.c_float * 64)(*np.stack(biased).reshape(-1)[:64])
+    assert L.gi_rev_quadtree_node_test(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, idx4, m64, disc.data_ptr(),
+                                       4, 0, 1, out.data_ptr(), vis.data_ptr(), cnt.data_ptr(), None) == \
+        gi._lib.GI_EINVAL
+    common, trans = gi.passes.adaptive_trans(mvps, [0, 0, 0, 0])
+    tr = torch.from_numpy(np.ascontiguousarray(trans)).cuda()
+    p = gi.RevSoftParams(MV, nm, lpos, maxSearch=65).to_c()
+    assert L.gi_rev_adaptive_soft_shadow_pass(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, F16(*common),
+                                              tr.data_ptr(), disc.data_ptr(), 4, vis.data_ptr(), ctypes.byref(p),
+                                              out.data_ptr(), None) == gi._lib.GI_EINVAL
+
+
 def _pcss_scene(gi, sm):
     """The area-light scene lit from 40x further away with a 5-degree light
     frustum: window depths near 0.997, above PCSS's 0.99 blocker cut-off."""
