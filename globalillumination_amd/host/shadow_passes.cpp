@@ -299,6 +299,13 @@ int SoftShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight
     GI_TRY(rgba_.allocate(bytes_of(npx, 4 * sizeof(float))));
     GI_TRY(qtImg_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(qtCount_.allocate(256));
+    // revectorization-based adaptive sampling: the visibility map and the
+    // 50-layer discontinuity array (createRGBATextureArray, main.cpp:1635),
+    // whose layers persist across frames; they start at zero here
+    GI_TRY(qtVis_.allocate(bytes_of(npx, sizeof(float))));
+    GI_TRY(disc_.allocate(static_cast<size_t>(GI_REV_DISC_LAYERS) * npx));
+    GI_TRY(hip_ok(hipMemsetAsync(disc_.get(), 0, disc_.size(), stream)));
+    GI_TRY(hip_ok(hipMemsetAsync(qtVis_.get(), 0, qtVis_.size(), stream)));
     size_t ws = gi_depth_pass_workspace_size(ntris_, maxSamples, smw_, smh_);
     size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
     if (ws == 0 || wg == 0) return GI_ENOTSUP;
@@ -468,9 +475,21 @@ int SoftShadowPipeline::renderLightSample(const Vec3& eye, const Vec3& at) {
     return GI_OK;
 }
 
-// evaluateSubAreaLightSource (main.cpp:440-571) without the revectorization branch.
+// The evaluation grid divisor of a node test (main.cpp:533-557).
+static int node_test_div(bool revectorized, bool rpcf, bool lowerAccuracy, int level) {
+    if (revectorized) {
+        if (level <= 1) return 1;
+        if (level <= 3) return rpcf ? 3 : 2;
+        return rpcf ? 6 : 4;
+    }
+    return lowerAccuracy ? 4 : 1;
+}
+
+// evaluateSubAreaLightSource (main.cpp:440-571).
 int SoftShadowPipeline::evaluateSubAreaLightSource(const QuadNode& node, int nodeIndex, bool& subdivide) {
     int local[4];
+    float B[16], biased[64];
+    GI_TRY(gi_bias_matrix(0.0f, B));
     for (int index = 0; index < 4; ++index) {
         int x, y;
         corner(node.level, nodeIndex, index, x, y);
@@ -481,17 +500,33 @@ int SoftShadowPipeline::evaluateSubAreaLightSource(const QuadNode& node, int nod
         qtRendered_[x][y] = true;
         qtHash_[x][y] = adaptiveSamples_;
         local[index] = adaptiveSamples_;
-        GI_TRY(renderLightSample(QuadNode::offset(node.refEye, node.level, lightSize, 1, index),
-                                 QuadNode::offset(node.refAt, node.level, lightSize, 1, index)));
+        const Vec3 eye = QuadNode::offset(node.refEye, node.level, lightSize, 1, index);
+        GI_TRY(renderLightSample(eye, QuadNode::offset(node.refAt, node.level, lightSize, 1, index)));
+        if (revectorizationBasedAdaptiveSampling) {
+            // Discontinuity.frag into layer `slot` (:476-491); configurePhong
+            // receives the sample's eye as it is (no 180-degree turn while
+            // evaluating, :357-361)
+            float mvp[16], nm[9];
+            GI_TRY(gi_mat4_mul(B, lightMVP_.data(), mvp));
+            normal_matrix(cameraMV_, nm);
+            gi_gbuffer gn{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
+            GI_TRY(gi_rev_discontinuity_pass(&gn, maps_.as<float>(), adaptiveSamples_, smw_, smh_,
+                                             local[index], mvp, cameraMV_.data(), nm, eye.data(), shadowIntensity,
+                                             disc_.as<unsigned char>(), GI_REV_DISC_LAYERS, stream));
+        }
     }
     // configureShadow with quadTreeEvaluation: B * lightMVPs[hash] (MyGLGeometryViewer.cpp:173-183)
-    float B[16], biased[64];
-    GI_TRY(gi_bias_matrix(0.0f, B));
     for (int i = 0; i < 4; ++i) GI_TRY(gi_mat4_mul(B, lightMVPs_[local[i]].data(), biased + 16 * i));
     gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_, nullptr};
-    const int div = adaptiveSamplingLowerAccuracy ? 4 : 1;  // main.cpp:554-557
-    GI_TRY(gi_quadtree_node_test(&g, maps_.as<float>(), adaptiveSamples_, smw_, smh_, local, biased, div,
-                                 qtImg_.as<float>(), qtCount_.as<unsigned>(), stream));
+    const int div = node_test_div(revectorizationBasedAdaptiveSampling, RPCF, adaptiveSamplingLowerAccuracy,
+                                  node.level);
+    if (revectorizationBasedAdaptiveSampling)
+        GI_TRY(gi_rev_quadtree_node_test(&g, maps_.as<float>(), adaptiveSamples_, smw_, smh_, local, biased,
+                                         disc_.as<unsigned char>(), GI_REV_DISC_LAYERS, node.level, div,
+                                         qtImg_.as<float>(), qtVis_.as<float>(), qtCount_.as<unsigned>(), stream));
+    else
+        GI_TRY(gi_quadtree_node_test(&g, maps_.as<float>(), adaptiveSamples_, smw_, smh_, local, biased, div,
+                                     qtImg_.as<float>(), qtCount_.as<unsigned>(), stream));
     unsigned count = 0;
     GI_TRY(hip_ok(hipMemcpyAsync(&count, qtCount_.get(), sizeof(count), hipMemcpyDeviceToHost, stream)));
     GI_TRY(hip_ok(hipStreamSynchronize(stream)));  // the occlusion query's GL_QUERY_RESULT
@@ -568,10 +603,34 @@ int SoftShadowPipeline::renderAdaptiveLightSourceSampling() {
     GI_TRY(gi_mat4_mul(B, lightMVP_.data(), common));
     GI_TRY(hip_ok(hipMemcpyAsync(trans_.get(), trans.data(), trans.size() * sizeof(float), hipMemcpyHostToDevice,
                                  stream)));
-    gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_, nullptr};
-    gi_soft_params sp{shadowIntensity, adaptiveSamplingLowerAccuracy ? 1 : 0, 3, 1.0f, 1};
-    GI_TRY(gi_adaptive_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, common, trans_.as<float>(), &sp,
-                                        soft_.as<float>(), stream));
+    if (revectorizationBasedAdaptiveSampling) {
+        // RevectorizationBasedAccurateSoftShadow.frag (:868-871); the final
+        // rendering turns the light 180 degrees about y (:363-366)
+        gi_rev_soft_params rp{};
+        rp.shadow_intensity = shadowIntensity;
+        rp.rpcf = RPCF ? 1 : 0;
+        rp.max_search = maxSearch;
+        rp.depth_threshold = depthThreshold;
+        std::memcpy(rp.mv, cameraMV_.data(), sizeof(rp.mv));
+        normal_matrix(cameraMV_, rp.normal_matrix);
+        rp.light_position[0] = -lightEye[0];
+        rp.light_position[1] = lightEye[1];
+        rp.light_position[2] = -lightEye[2];
+        gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
+        GI_TRY(gi_rev_adaptive_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, common, trans_.as<float>(),
+                                                disc_.as<unsigned char>(), GI_REV_DISC_LAYERS, qtVis_.as<float>(),
+                                                &rp, soft_.as<float>(), stream));
+        // the next frame's RPCF (main.cpp:880-885)
+        if (RPCF && n <= 9)
+            RPCF = false;
+        else if (!RPCF && n >= 81)
+            RPCF = true;
+    } else {
+        gi_gbuffer g{gpos_.as<float>(), nullptr, windowWidth_, windowHeight_, nullptr};
+        gi_soft_params sp{shadowIntensity, adaptiveSamplingLowerAccuracy ? 1 : 0, 3, 1.0f, 1};
+        GI_TRY(gi_adaptive_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, common, trans_.as<float>(), &sp,
+                                            soft_.as<float>(), stream));
+    }
     return hip_ok(hipStreamSynchronize(stream));  // the host trans buffer goes out of scope
 }
 

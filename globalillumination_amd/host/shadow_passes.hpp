@@ -149,6 +149,12 @@ public:
     int maxSearch = 16;                            // main.cpp:1617
     float depthThreshold = 0.0f;                   // SceneLoader `d`, main.cpp:1618
     int maxLevel = 4;                              // main.cpp:190
+    // Menu entry 2 (main.cpp:1411-1416): discontinuity layers per evaluated
+    // corner map, the revectorized node test and final pass.  RPCF is the
+    // frame-to-frame state the reference keeps in ShadowParams.RPCF: after
+    // each frame it turns on at 81 or more samples and off at 9 or fewer.
+    bool revectorizationBasedAdaptiveSampling = false;
+    bool RPCF = false;
     bool preEvaluation = true;                     // PCSS runs the pre-evaluation (:614)
     Vec3 cameraEye{}, cameraAt{}, cameraUp{0.0f, 0.0f, 1.0f};
     float cameraFov = 45.0f;
@@ -165,12 +171,13 @@ public:
     // the depth map of the light centre (into map 0), then
     // PlausibleSoftShadow.frag or RBSSM.frag.
     int renderSoftShadows();
-    // renderAdaptiveLightSourceSampling (main.cpp:813-892) without the
-    // revectorization variant: the quad tree over the light is refined where
-    // a node's four corner maps disagree on a penumbra pixel
-    // (evaluateAreaLightSource :573-620), every leaf's corner maps are
-    // rendered (renderAreaLightSource :622-694), and the weighted
-    // AccurateSoftShadow pass averages them.  Needs maxSamples >= 289.
+    // renderAdaptiveLightSourceSampling (main.cpp:813-892): the quad tree
+    // over the light is refined where a node's four corner maps disagree on
+    // a penumbra pixel (evaluateAreaLightSource :573-620), every leaf's
+    // corner maps are rendered (renderAreaLightSource :622-694), and the
+    // weighted AccurateSoftShadow pass averages them (or, with
+    // revectorizationBasedAdaptiveSampling, the revectorized passes).
+    // Needs maxSamples >= 289.
     int renderAdaptiveLightSourceSampling();
     // The adaptive frame's record: node tests in evaluation order
     // (node index, level, passing pixels) and each map's quad-tree level.
@@ -196,6 +203,7 @@ private:
     std::vector<Mat4> lightMVPs_;
     Mat4 cameraMVP_{}, cameraMV_{}, lightMVP_{};
     DeviceBuffer pos_, nrm_, idx_, mvps_, trans_, maps_, gpos_, gnrm_, gfac_, soft_, rgba_, ws_, qtImg_, qtCount_;
+    DeviceBuffer qtVis_, disc_;  // revectorization-based adaptive sampling
     // adaptive sampling state (renderAdaptiveLightSourceSampling)
     struct QuadNode;
     int renderLightSample(const Vec3& eye, const Vec3& at);

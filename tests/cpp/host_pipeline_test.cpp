@@ -213,6 +213,31 @@ int main(int argc, char** argv) {
         dumpv(dir, (tag + "_tests.i32").c_str(), tests);
         std::printf("%s: %d samples, %zu node tests\n", tag.c_str(), soft.adaptiveSamples(), soft.nodeTests().size());
     }
+    // the revectorization-based variant (menu entry 2): three frames over the
+    // same discontinuity layers, the third with RPCF forced on
+    soft.revectorizationBasedAdaptiveSampling = true;
+    soft.adaptiveSamplingLowerAccuracy = true;
+    soft.maxSearch = 16;
+    soft.depthThreshold = 2.5e-6f;
+    for (int frame = 0; frame < 3; ++frame) {
+        if (frame == 2) soft.RPCF = true;
+        const int32_t rpcf = soft.RPCF ? 1 : 0;
+        rc = soft.renderAdaptiveLightSourceSampling();
+        if (rc) return fail("renderAdaptiveLightSourceSampling(revectorization)", rc);
+        std::vector<float> ad;
+        rc = soft.download(ad);
+        if (rc) return fail("download", rc);
+        const std::string tag = "rev" + std::to_string(frame);
+        dumpv(dir, (tag + "_soft.f32").c_str(), ad);
+        dumpv(dir, (tag + "_acc.f32").c_str(), soft.accFactors());
+        dump(dir + "/" + tag + "_mvps.f32", soft.lightMVPs().data(), soft.lightMVPs().size() * 64);
+        std::vector<int32_t> tests{rpcf};
+        for (const auto& t : soft.nodeTests())
+            tests.insert(tests.end(), {t.nodeIndex, t.level, static_cast<int32_t>(t.count)});
+        dumpv(dir, (tag + "_tests.i32").c_str(), tests);
+        std::printf("%s: %d samples, %zu node tests, RPCF %d\n", tag.c_str(), soft.adaptiveSamples(),
+                    soft.nodeTests().size(), rpcf);
+    }
     std::printf("host pipeline ok: %d vertices, %d triangles\n", mesh.nverts(), mesh.ntris());
     return 0;
 }
