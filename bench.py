@@ -50,6 +50,8 @@ def _parse():
     ap.add_argument("--diff-steps", type=int, default=10, help="timed steps of the differentiable pass")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames timing")
     ap.add_argument("--batch-steps", type=int, default=5, help="timed steps of the batched frames")
+    ap.add_argument("--no-passes", action="store_true", help="skip the other screen passes' timing")
+    ap.add_argument("--passes-steps", type=int, default=5, help="timed steps of each other screen pass")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -228,6 +230,92 @@ def batch_timing(gi, torch, steps):
             "tuning": tuning, "autotune_ms": round(tuned_ms, 3), "steps": steps}
 
 
+def passes_timing(gi, torch, steps):
+    """The other screen passes on one far-light frame: the area-light scene
+    (100,822 triangles) lit from 40x further away with a 5-degree light
+    frustum (window depths near 0.997, so PCSS and RBSSM find penumbrae),
+    1920x1080 view, 2048^2 maps.  Hard-shadow techniques, PCSS (radius 64,
+    kernel 15), RBSSM (radius 64, kernel 7, maxSearch 16), and the
+    revectorization-based adaptive passes over the 9 samples of a 3 x 3
+    light: one discontinuity layer, one level-0 node test, the final pass
+    (binary and RPCF).  HIP events on the current stream; inputs resident."""
+    import numpy as np
+
+    from globalillumination_amd import passes
+
+    wl = gi.scenes.config3(nsamples=9, sm=2048)
+    wl.lights = [gi.scenes.View((120.0, 280.0, 80.0), (0.0, 0.0, 0.0), 2048, 2048, fov=5.0)] + \
+        [gi.scenes.View(tuple(np.float32(v.eye) * np.float32(40.0)), v.at, 2048, 2048, fov=5.0) for v in wl.lights]
+    W, H = wl.camera.width, wl.camera.height
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, passes.view_matrices(wl.camera)[2], W, H)
+    mvps = np.stack([passes.view_matrices(v)[2] for v in wl.lights])
+    maps = passes.render_shadow_map(dm, mvps, 2048, 2048, polygon_offset=(4.0, 20.0))
+    MV, nm, lpos = passes.phong_uniforms(wl.camera, wl.lights[0].eye)
+    out = torch.empty((H, W), dtype=torch.float32, device=maps.device)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return round(e0.elapsed_time(e1) / steps, 4)
+
+    res = {"workload": "far-light frame: 100,822-triangle displaced icosphere + ground, 1920x1080 view, "
+                       "2048^2 maps (light 40x further, 5-degree frustum)", "steps": steps}
+    d0 = maps[0]
+    for tech in ("naive", "pcf", "rbsm", "rpcf", "rbsm_filtered", "rbsm_conservative", "rpcf_conservative"):
+        sp = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, technique=tech,
+                             preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos,
+                             depthThreshold=2.5e-5, maxSearch=16)
+        res[f"hard_{tech}_ms"] = timed(lambda: passes.compute_hard_shadows(g, d0, sp, out=out))
+    pc = gi.PCSSParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, lightSourceRadius=64,
+                       preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos)
+    res["pcss_ms"] = timed(lambda: passes.render_soft_shadows(g, d0, pc, out=out))
+    rb = gi.RBSSMParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, lightSourceRadius=64,
+                        preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos, maxSearch=16,
+                        depthThreshold=2.5e-5)
+    res["rbssm_ms"] = timed(lambda: passes.render_rbssm(g, d0, rb, out=out))
+    samp = maps[1:].contiguous()
+    biased = np.stack([passes.biased(m) for m in mvps[1:]])
+    disc = passes.rev_discontinuity_layers(g)
+    eyes = [tuple(float(x) for x in v.eye) for v in wl.lights[1:]]
+    res["rev_discontinuity_ms"] = timed(
+        lambda: passes.rev_discontinuity(g, samp, 0, biased[0], MV, nm, eyes[0], 0.25, disc))
+    for i in range(1, 9):
+        passes.rev_discontinuity(g, samp, i, biased[i], MV, nm, eyes[i], 0.25, disc)
+    vis = torch.zeros((H, W), dtype=torch.float32, device=maps.device)
+    img = torch.empty((H, W), dtype=torch.float32, device=maps.device)
+    corners = [0, 2, 6, 8]
+    _, count = passes.rev_quadtree_node_test(g, samp, corners, biased[corners], disc, 0, vis, image=img)
+    count_dev = torch.empty((1,), dtype=torch.int32, device=maps.device)
+    import ctypes
+
+    from globalillumination_amd import _lib
+    idx = (ctypes.c_int * 4)(*corners)
+    gc = g.to_c(with_normals=False)
+    m64 = passes._f(biased[corners], 64)
+
+    def node_test():
+        _lib.call("gi_rev_quadtree_node_test", ctypes.byref(gc), passes._dptr(samp), 9, 2048, 2048, idx, m64,
+                  passes._dptr(disc), int(disc.shape[0]), 0, 1, passes._dptr(img), passes._dptr(vis),
+                  passes._dptr(count_dev), passes._stream_handle(None))
+    res["rev_node_test_ms"] = timed(node_test)
+    common, trans = passes.adaptive_trans(mvps[1:], [1, 2, 1, 2, 1, 2, 1, 2, 1])
+    pen = float(((vis == 0.5) & (g.position[0] != 0)).float().mean())
+    res["rev_penumbra_pixel_fraction"] = round(pen, 4)
+    for rp in (False, True):
+        prm = gi.RevSoftParams(MV, nm, lpos, RPCF=rp, maxSearch=16, depthThreshold=2.5e-5)
+        res[f"rev_soft{'_rpcf' if rp else ''}_ms"] = timed(
+            lambda: passes.render_rev_adaptive(g, samp, common, trans, disc, vis, prm, out=out))
+    return res
+
+
 def main():
     args = _parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -275,7 +363,7 @@ def main():
     def frame(ev=None):
         if ev:
             ev[0].record(stream)
-        passes.render_shadow_map(dm, mvps, smw, smw, out=maps, raster_events=(ev[3], ev[4]) if ev else None)
+        passes.render_shadow_map(dm, mvps_dev, smw, smw, out=maps, raster_events=(ev[3], ev[4]) if ev else None)
         if ev:
             ev[1].record(stream)
         passes.render_monte_carlo(g, maps, mvps, params, out=out, trans_dev=trans_dev)
@@ -336,6 +424,7 @@ def main():
             "phong_pass": {"kernel": "k_phong", "ms": round(phong_ms, 4), "bytes": int(npx * 48),
                            "GB/s": round(npx * 48 / (phong_ms * 1e-3) / 1e9, 1)}}
     diff = None if args.no_diff else diff_timing(gi, torch, args.diff_steps, 2)
+    other = None if args.no_passes else passes_timing(gi, torch, args.passes_steps)
     batched = None if args.no_batch else batch_timing(gi, torch, args.batch_steps)
     cpu = None
     if not args.no_cpu_baseline:
@@ -350,7 +439,8 @@ def main():
                                    "icosphere + ground",
                        "global_batch": 1, "samples": S, "shadow_map": smw, "view": [W, H],
                        "triangles": wl.mesh.ntris},
-            "roofline": roof, "cpu_baseline": cpu, "differentiable": diff, "batched": batched}
+            "roofline": roof, "cpu_baseline": cpu, "differentiable": diff, "batched": batched,
+            "other_passes": other}
     print(json.dumps(line), flush=True)
 
 

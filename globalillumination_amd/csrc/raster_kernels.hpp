@@ -388,8 +388,17 @@ struct BinBufs {
     uint32_t* nclip;     // nmaps
     uint32_t* clip;      // nmaps x wcap: triangles crossing a clip plane
     uint32_t* entries;   // nmaps x cap
-    uint32_t cap, wcap;
+    uint32_t* tiles;     // nmaps x tstride: the count pass's tiles of each directly binned triangle
+    uint32_t cap, wcap, tstride;
 };
+
+// The tiles a directly binned triangle overlaps, as the count pass found
+// them, for the scatter pass: bits 0-15 the first tile of its tile box,
+// bits 16-17 the box width in tiles minus 1, bits 18-21 which of the box's
+// (at most 4) tiles it overlaps.  0: none (rejected, clipped, wide or empty).
+__device__ __forceinline__ uint32_t tiles_code(int base, int ntx, uint32_t mask) {
+    return mask ? ((uint32_t)base | ((uint32_t)(ntx - 1) << 16) | (mask << 18)) : 0u;
+}
 
 __device__ __forceinline__ void bin_tile(const BinBufs& b, int map, int ntiles, int tile, uint32_t entry,
                                          bool scatter) {
@@ -432,16 +441,31 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
     if (kScatter && b.overflow[map]) return;  // uniform per workgroup
     if (lds)
         for (int p = tid; p < ntiles; p += blockDim.x) hist[p] = 0;
-    float mvp[16];
-    mvps.load(map, mvp);
     // Up to wide_span<TS>() tiles per thread, kept in registers (static indices).
     constexpr int kWideSpan = wide_span<TS>();
+    // 64-texel tiles: the count pass stores each triangle's tiles and the
+    // scatter pass reads them back; smaller tiles (a wider span than the code
+    // holds) classify again in the scatter pass
+    constexpr bool kCache = kWideSpan <= 4;
     int tile_of[kWideSpan];
     uint32_t rank_of[kWideSpan];
 #pragma unroll
     for (int k = 0; k < kWideSpan; ++k) tile_of[k] = -1;
     const uint32_t entry = (uint32_t)tri * 8u;
-    if (tri < mesh.nt) {
+    if (kScatter && kCache) {
+        // the count pass classified this (triangle, map) already
+        if (tri < mesh.nt) {
+            const uint32_t code = b.tiles[(size_t)map * b.tstride + tri];
+            const int base = (int)(code & 0xFFFFu), ntx = (int)((code >> 16) & 3u) + 1;
+            const uint32_t mask = code >> 18;
+#pragma unroll
+            for (int k = 0; k < kWideSpan; ++k)
+                if (mask & (1u << k)) tile_of[k] = base + (k / ntx) * cfg.tiles_x + (k % ntx);
+        }
+    } else if (tri < mesh.nt) {
+        float mvp[16];
+        mvps.load(map, mvp);
+        uint32_t code = 0;
         Tri<0> t;
         bool ok;
         int cls = classify<0>(mesh, mvp, tri, cfg, t, ok);
@@ -458,6 +482,7 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
             if (span > kWideSpan) {
                 if (!kScatter) push_record(b, map, t, entry);
             } else {
+                uint32_t mask = 0;
 #pragma unroll
                 for (int k = 0; k < kWideSpan; ++k) {
                     if (k >= span) continue;
@@ -468,9 +493,12 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
                         if (!rect_overlap<0>(t, x0, x1, y0, y1)) continue;
                     }
                     tile_of[k] = ty * cfg.tiles_x + tx;
+                    mask |= 1u << k;
                 }
+                code = tiles_code(ty0 * cfg.tiles_x + tx0, ntx, mask);
             }
         }
+        if (!kScatter && kCache) b.tiles[(size_t)map * b.tstride + tri] = code;
     }
     if (!lds) {
 #pragma unroll
@@ -995,6 +1023,7 @@ inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws
     size_t o_wide = off; off = align256(off + sizeof(Setup) * (size_t)wcap * nmaps);
     size_t o_clip = off; off = align256(off + sizeof(uint32_t) * (size_t)wcap * nmaps);
     size_t o_entries = off; off = align256(off + sizeof(uint32_t) * (size_t)cap * nmaps);
+    size_t o_tiles = off; off = align256(off + sizeof(uint32_t) * (size_t)(ntris > 0 ? ntris : 0) * nmaps);
     size_t o_vis = off;
     if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
     if (ws) {
@@ -1008,6 +1037,8 @@ inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws
         b.nclip = reinterpret_cast<uint32_t*>(base + o_nclip);
         b.clip = reinterpret_cast<uint32_t*>(base + o_clip);
         b.entries = reinterpret_cast<uint32_t*>(base + o_entries);
+        b.tiles = reinterpret_cast<uint32_t*>(base + o_tiles);
+        b.tstride = (uint32_t)(ntris > 0 ? ntris : 0);
         b.cap = cap;
         b.wcap = wcap;
         ws->vis = vis ? reinterpret_cast<void*>(base + o_vis) : nullptr;

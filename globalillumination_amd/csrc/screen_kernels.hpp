@@ -611,15 +611,25 @@ __device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms
         inside = inside && (iu[NT - 1] - iu[0] <= NT - 1) && (iu[0] + NT - 1 < D.w);
         float illum = 0.0f;
         if (inside) {
-            Row<NT> rows[NT];
+            // each row is copied into its own small struct and then into
+            // scalars, so the rows stay in VGPRs (an array of rows filled by
+            // one memcpy each was placed in LDS by the compiler)
+            float rv[NT][NT];
 #pragma unroll
-            for (int b = 0; b < NT; ++b)
-                __builtin_memcpy(&rows[b], D.texels + (size_t)iv[b] * D.w + iu[0], sizeof(Row<NT>));
+            for (int b = 0; b < NT; ++b) {
+                Row<NT> r;
+                __builtin_memcpy(&r, D.texels + (size_t)iv[b] * D.w + iu[0], sizeof(Row<NT>));
+#pragma unroll
+                for (int q = 0; q < NT; ++q) rv[b][q] = r.v[q];
+            }
 #pragma unroll
             for (int a = 0; a < NT; ++a) {
+                const int k = iu[a] - iu[0];
 #pragma unroll
                 for (int b = 0; b < NT; ++b) {
-                    float d = pick<NT>(rows[b], iu[a] - iu[0]);
+                    float d = rv[b][0];
+#pragma unroll
+                    for (int q = 1; q < NT; ++q) d = (k == q) ? rv[b][q] : d;
                     illum = illum + ((sz <= d) ? 1.0f : u.si);
                 }
             }

@@ -164,11 +164,19 @@ def render_shadow_map(mesh: DeviceMesh, mvps: np.ndarray, width: int, height: in
     an (S, height, width) FP32 tensor (renderShadowMap, and renderMonteCarlo's
     per-sample loop SoftShadowMapping/src/main.cpp:774-793).  `raster_events`:
     optional pair of torch.cuda.Event (enable_timing) recorded around the
-    tile-raster kernel."""
-    m = np.ascontiguousarray(mvps, dtype=np.float32).reshape(-1, 16)
-    S = m.shape[0]
+    tile-raster kernel.  `mvps` may also be a device tensor of (S, 16) FP32
+    matrices already resident on the mesh's device: then nothing is copied
+    and the host does not wait on the stream."""
     dev = mesh.positions.device
-    mvps_dev = torch.from_numpy(m).to(dev)
+    if isinstance(mvps, torch.Tensor):
+        if mvps.device != dev or mvps.dtype != torch.float32 or not mvps.is_contiguous() or mvps.numel() % 16:
+            raise ValueError("device mvps must be a contiguous float32 (S, 16) tensor on the mesh's device")
+        mvps_dev = mvps
+        S = mvps.numel() // 16
+    else:
+        m = np.ascontiguousarray(mvps, dtype=np.float32).reshape(-1, 16)
+        S = m.shape[0]
+        mvps_dev = torch.from_numpy(m).to(dev)
     if out is None:
         out = torch.empty((S, height, width), dtype=torch.float32, device=dev)
     nbytes = _lib.lib().gi_depth_pass_workspace_size(mesh.ntris, S, width, height)

@@ -58,6 +58,20 @@ def test_depth_pass_batch_of_area_light_samples(gi, oracle_lib, cuda):
     assert bits_equal(got, want), mismatch_report(got, want, "area-light depth batch")
 
 
+def test_depth_pass_device_matrices(gi, cuda):
+    """Matrices handed over as a device tensor give the same maps as host
+    matrices; a wrongly shaped tensor is rejected."""
+    wl = gi.scenes.config3(nsamples=4, view_w=64, view_h=64, sm=256, freq=20)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    mvps = light_mvps(wl)
+    want = gi.render_shadow_map(dm, mvps, 256, 256)
+    dev = torch.from_numpy(np.ascontiguousarray(mvps, dtype=np.float32)).cuda()
+    got = gi.render_shadow_map(dm, dev, 256, 256)
+    assert bits_equal(_np(got), _np(want))
+    with pytest.raises(ValueError):
+        gi.render_shadow_map(dm, dev[:, :15].contiguous(), 256, 256)
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (33, 17), (100, 257)])
 def test_depth_pass_ragged_sizes(gi, oracle_lib, cuda, w, h):
     wl = gi.scenes.config1()
