@@ -297,9 +297,15 @@ Layout plan_layout(const gi_batch_instance* inst, int n) {
     L.bins.resize(n);
     for (int i = 0; i < n; ++i) {
         L.bins[i] = off;
-        // the 32-texel tile grid has the most tiles: its layout covers both edges
-        off = align256(off + layout(inst[i].ntris, inst[i].nsamples, inst[i].sm_width, inst[i].sm_height, false,
-                                    nullptr, nullptr, 32));
+        // one region serves both tile edges: the 32-texel grid has more tiles,
+        // the 64-texel one keeps a setup record per (triangle, map)
+        size_t need = 0;
+        for (int v = 0; v < 2; ++v) {
+            const size_t sz = layout(inst[i].ntris, inst[i].nsamples, inst[i].sm_width, inst[i].sm_height, false,
+                                     nullptr, nullptr, kTiles[v]);
+            if (sz > need) need = sz;
+        }
+        off = align256(off + need);
     }
     L.total = off;
     return L;

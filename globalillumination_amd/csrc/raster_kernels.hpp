@@ -389,6 +389,7 @@ struct BinBufs {
     uint32_t* clip;      // nmaps x wcap: triangles crossing a clip plane
     uint32_t* entries;   // nmaps x cap
     uint32_t* tiles;     // nmaps x tstride: the count pass's tiles of each directly binned triangle
+    Setup* setups;       // nmaps x tstride: its setup record (64-texel tiles only)
     uint32_t cap, wcap, tstride;
 };
 
@@ -496,6 +497,7 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
                     mask |= 1u << k;
                 }
                 code = tiles_code(ty0 * cfg.tiles_x + tx0, ntx, mask);
+                if (!kScatter && kCache && mask) b.setups[(size_t)map * b.tstride + tri] = to_setup(t, entry);
             }
         }
         if (!kScatter && kCache) b.tiles[(size_t)map * b.tstride + tri] = code;
@@ -808,6 +810,16 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
                     int rx0 = max(x0, w.px0), rx1 = min(x1, w.px1), ry0 = max(y0, w.py0), ry1 = min(y1, w.py1);
                     if (rx0 <= rx1 && ry0 <= ry1) raster_rect_self<TS, kVis>(w, x0, y0, rx0, rx1, ry0, ry1, tile);
                 }
+            } else if constexpr (!kFallback && wide_span<TS>() <= 4) {
+                // a directly binned triangle: the count pass stored its setup
+                const Setup s = b.setups[(size_t)map * b.tstride + (entry >> 3)];
+                int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
+                if (rx0 <= rx1 && ry0 <= ry1) {
+                    int slot = kRecQueue;
+                    if ((rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kSelfRasterMaxPx) slot = atomicAdd(&nrec, 1);
+                    if (slot < kRecQueue) rec[slot] = s;
+                    else raster_rect_self<TS, kVis>(s, x0, y0, rx0, rx1, ry0, ry1, tile);
+                }
             } else {
                 Tri<0> t;
                 bool ok;
@@ -1024,6 +1036,9 @@ inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws
     size_t o_clip = off; off = align256(off + sizeof(uint32_t) * (size_t)wcap * nmaps);
     size_t o_entries = off; off = align256(off + sizeof(uint32_t) * (size_t)cap * nmaps);
     size_t o_tiles = off; off = align256(off + sizeof(uint32_t) * (size_t)(ntris > 0 ? ntris : 0) * nmaps);
+    const bool records = 4 * (64 / ts) * (64 / ts) <= 4;  // the tile code's span (wide_span<ts>() <= 4)
+    size_t o_setups = off;
+    if (records) off = align256(off + sizeof(Setup) * (size_t)(ntris > 0 ? ntris : 0) * nmaps);
     size_t o_vis = off;
     if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
     if (ws) {
@@ -1038,6 +1053,7 @@ inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws
         b.clip = reinterpret_cast<uint32_t*>(base + o_clip);
         b.entries = reinterpret_cast<uint32_t*>(base + o_entries);
         b.tiles = reinterpret_cast<uint32_t*>(base + o_tiles);
+        b.setups = records ? reinterpret_cast<Setup*>(base + o_setups) : nullptr;
         b.tstride = (uint32_t)(ntris > 0 ? ntris : 0);
         b.cap = cap;
         b.wcap = wcap;
