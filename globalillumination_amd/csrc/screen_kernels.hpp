@@ -614,11 +614,14 @@ __device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms
             // each row is copied into its own small struct and then into
             // scalars, so the rows stay in VGPRs (an array of rows filled by
             // one memcpy each was placed in LDS by the compiler)
+            // inside the map the texel index is below W * H <= 2^28: 32-bit
+            // unsigned offsets from the map base (no 64-bit address math)
             float rv[NT][NT];
 #pragma unroll
             for (int b = 0; b < NT; ++b) {
                 Row<NT> r;
-                __builtin_memcpy(&r, D.texels + (size_t)iv[b] * D.w + iu[0], sizeof(Row<NT>));
+                const uint32_t off = (uint32_t)iv[b] * (uint32_t)D.w + (uint32_t)iu[0];
+                __builtin_memcpy(&r, D.texels + off, sizeof(Row<NT>));
 #pragma unroll
                 for (int q = 0; q < NT; ++q) rv[b][q] = r.v[q];
             }
