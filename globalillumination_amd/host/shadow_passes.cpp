@@ -505,7 +505,7 @@ int SoftShadowPipeline::evaluateSubAreaLightSource(const QuadNode& node, int nod
         if (revectorizationBasedAdaptiveSampling) {
             // Discontinuity.frag into layer `slot` (:476-491); configurePhong
             // receives the sample's eye as it is (no 180-degree turn while
-            // evaluating, :357-361)
+            // evaluating, :365-368)
             float mvp[16], nm[9];
             GI_TRY(gi_mat4_mul(B, lightMVP_.data(), mvp));
             normal_matrix(cameraMV_, nm);
@@ -605,7 +605,7 @@ int SoftShadowPipeline::renderAdaptiveLightSourceSampling() {
                                  stream)));
     if (revectorizationBasedAdaptiveSampling) {
         // RevectorizationBasedAccurateSoftShadow.frag (:868-871); the final
-        // rendering turns the light 180 degrees about y (:363-366)
+        // rendering turns the light 180 degrees about y (:365-368)
         gi_rev_soft_params rp{};
         rp.shadow_intensity = shadowIntensity;
         rp.rpcf = RPCF ? 1 : 0;

@@ -247,7 +247,7 @@ int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_de
 /* HardShadow/Discontinuity.frag:66-88 for corner map `map_index` (biased_mvp
  * = B * its lightMVP): per G-buffer pixel, computeDiscontinuity where the
  * pre-evaluation (mv, normal_matrix, light_position: the sample's eye as
- * configurePhong receives it during evaluation, main.cpp:357-361) and w > 0
+ * configurePhong receives it during evaluation, main.cpp:365-368) and w > 0
  * let it through, else 0; background pixels keep the clear value.  Writes
  * layer map_index of disc_dev when map_index < disc_layers; otherwise nothing
  * (the reference's framebuffer is incomplete there).  The gbuffer needs its
