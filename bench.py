@@ -417,8 +417,11 @@ def main():
             "traffic_source": str(TRAFFIC_SUMMARY.relative_to(ROOT)), "kernel": dom["kernel"],
             "algorithmic_bytes": int(dom["bytes"]), "ms": round(dom["ms"], 4),
             "screen_pass": {"kernel": "k_soft", "ms": round(screen_ms, 4), "GB/s": round(screen_gbs, 1),
+                            "taps_per_s": round(npx * S * len(taps) ** 2 / (screen_ms * 1e-3), -6),
                             "frac": round(screen_gbs / HBM_PEAK_GBS, 4), "bytes": int(screen_bytes),
-                            "unique_texels": int(texels), "traffic": measured_traffic("k_soft")},
+                            "unique_texels": int(texels), "traffic": measured_traffic("k_soft"),
+                            "traffic_GB/s": (round(measured_traffic("k_soft") / (screen_ms * 1e-3) / 1e9, 1)
+                                             if measured_traffic("k_soft") else None)},
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
                            "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)},
             "phong_pass": {"kernel": "k_phong", "ms": round(phong_ms, 4), "bytes": int(npx * 48),
