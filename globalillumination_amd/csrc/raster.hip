@@ -49,9 +49,11 @@ namespace gi {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Binning of every map (count, clip, scan, scatter) and the raster of the
+// maps whose lists overflowed the workspace.
 template <bool kVis>
-int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const RasterCfg& cfg, const Workspace& ws,
-               void* out, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
+int bin_stage(const MeshView& mesh, const MvpSource& mvps, int nmaps, const RasterCfg& cfg, const Workspace& ws,
+              void* out, hipStream_t s) {
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     const BinBufs& b = ws.bins;
     size_t nz = zero_words(nmaps, cfg);
@@ -70,13 +72,32 @@ int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Ras
         hipLaunchKernelGGL((k_bin<kTile, true>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
         hipLaunchKernelGGL((k_bin_wide<kTile, true>), gw, dim3(256), 0, s, cfg, b);
     }
-    if (ev_begin && hipEventRecord(ev_begin, s) != hipSuccess) return GI_EDEVICE;
-    hipLaunchKernelGGL((k_raster<kTile, kVis>), dim3(ntiles, nmaps), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b,
-                       out);
-    if (ev_end && hipEventRecord(ev_end, s) != hipSuccess) return GI_EDEVICE;
     hipLaunchKernelGGL((k_raster_fallback<kTile, kVis>), dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b,
                        out, nmaps);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+// Tile raster of maps [map_begin, map_begin + map_count) from the lists the
+// bin stage left in the workspace (maps that overflowed are skipped: the bin
+// stage rastered them).
+template <bool kVis>
+int raster_stage(const MeshView& mesh, const MvpSource& mvps, int map_begin, int map_count, const RasterCfg& cfg,
+                 const Workspace& ws, void* out, hipStream_t s, hipEvent_t ev_begin = nullptr,
+                 hipEvent_t ev_end = nullptr) {
+    const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    if (ev_begin && hipEventRecord(ev_begin, s) != hipSuccess) return GI_EDEVICE;
+    hipLaunchKernelGGL((k_raster<kTile, kVis>), dim3(ntiles, map_count), dim3(kRasterThreads), 0, s, mesh, mvps, cfg,
+                       ws.bins, out, map_begin);
+    if (ev_end && hipEventRecord(ev_end, s) != hipSuccess) return GI_EDEVICE;
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
+template <bool kVis>
+int run_raster(const MeshView& mesh, const MvpSource& mvps, int nmaps, const RasterCfg& cfg, const Workspace& ws,
+               void* out, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr) {
+    int rc = bin_stage<kVis>(mesh, mvps, nmaps, cfg, ws, out, s);
+    if (rc != GI_OK) return rc;
+    return raster_stage<kVis>(mesh, mvps, 0, nmaps, cfg, ws, out, s, ev_begin, ev_end);
 }
 
 }  // namespace
@@ -116,9 +137,21 @@ size_t gi_depth_pass_workspace_size(int ntris, int nmaps, int width, int height)
     return layout(ntris, nmaps, width, height, false, nullptr, nullptr);
 }
 
-int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
-                  const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
-                  float* depth_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+}  // extern "C"
+
+namespace {
+
+struct DepthCall {
+    MeshView mesh;
+    MvpSource mvps;
+    RasterCfg cfg;
+    Workspace ws;
+};
+
+// Argument checks and workspace layout shared by the depth-pass entry points.
+int depth_setup(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                float* depth_out_dev, void* workspace_dev, size_t workspace_bytes, DepthCall& d) {
     if (!positions_dev || (!indices_dev && ntris > 0) || !mvps_dev || !depth_out_dev || !workspace_dev)
         return GI_EINVAL;
     if (nverts <= 0 || ntris < 0 || nmaps <= 0) return GI_EINVAL;
@@ -128,16 +161,54 @@ int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices
     float off_f = opts ? opts->polygon_offset_factor : 0.0f;
     float off_u = opts ? opts->polygon_offset_units : 0.0f;
     if (!std::isfinite(off_f) || !std::isfinite(off_u)) return GI_ENONFINITE;
-    Workspace ws;
-    size_t need = layout(ntris, nmaps, width, height, false, &ws, reinterpret_cast<char*>(workspace_dev));
+    size_t need = layout(ntris, nmaps, width, height, false, &d.ws, reinterpret_cast<char*>(workspace_dev));
     if (workspace_bytes < need) return GI_ENOSPACE;
-    MeshView mesh{positions_dev, nullptr, indices_dev, ntris};
-    MvpSource mvps{};
-    mvps.dev = mvps_dev;
-    RasterCfg cfg{width, height, (width + kTile - 1) / kTile, (height + kTile - 1) / kTile, off_f, off_u};
-    return run_raster<false>(mesh, mvps, nmaps, cfg, ws, depth_out_dev, reinterpret_cast<hipStream_t>(stream),
+    d.mesh = MeshView{positions_dev, nullptr, indices_dev, ntris};
+    d.mvps = MvpSource{};
+    d.mvps.dev = mvps_dev;
+    d.cfg = RasterCfg{width, height, (width + kTile - 1) / kTile, (height + kTile - 1) / kTile, off_f, off_u};
+    return GI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                  const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                  float* depth_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+    DepthCall d;
+    int rc = depth_setup(positions_dev, nverts, indices_dev, ntris, mvps_dev, nmaps, width, height, opts,
+                         depth_out_dev, workspace_dev, workspace_bytes, d);
+    if (rc != GI_OK) return rc;
+    return run_raster<false>(d.mesh, d.mvps, nmaps, d.cfg, d.ws, depth_out_dev, reinterpret_cast<hipStream_t>(stream),
                              opts ? reinterpret_cast<hipEvent_t>(opts->raster_begin_event) : nullptr,
                              opts ? reinterpret_cast<hipEvent_t>(opts->raster_end_event) : nullptr);
+}
+
+int gi_depth_pass_bin(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                      const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                      float* depth_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
+    DepthCall d;
+    int rc = depth_setup(positions_dev, nverts, indices_dev, ntris, mvps_dev, nmaps, width, height, opts,
+                         depth_out_dev, workspace_dev, workspace_bytes, d);
+    if (rc != GI_OK) return rc;
+    return bin_stage<false>(d.mesh, d.mvps, nmaps, d.cfg, d.ws, depth_out_dev, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gi_depth_pass_raster(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                         const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                         int map_begin, int map_count, float* depth_out_dev, void* workspace_dev,
+                         size_t workspace_bytes, gi_stream stream) {
+    DepthCall d;
+    int rc = depth_setup(positions_dev, nverts, indices_dev, ntris, mvps_dev, nmaps, width, height, opts,
+                         depth_out_dev, workspace_dev, workspace_bytes, d);
+    if (rc != GI_OK) return rc;
+    if (map_begin < 0 || map_count <= 0 || map_begin > nmaps - map_count) return GI_EINVAL;
+    return raster_stage<false>(d.mesh, d.mvps, map_begin, map_count, d.cfg, d.ws, depth_out_dev,
+                               reinterpret_cast<hipStream_t>(stream),
+                               opts ? reinterpret_cast<hipEvent_t>(opts->raster_begin_event) : nullptr,
+                               opts ? reinterpret_cast<hipEvent_t>(opts->raster_end_event) : nullptr);
 }
 
 size_t gi_gbuffer_pass_workspace_size(int ntris, int width, int height) {

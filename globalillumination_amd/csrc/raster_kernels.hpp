@@ -889,11 +889,11 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
 // k_raster_fallback.
 template <int TS, bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
-                                                           void* out) {
+                                                           void* out, int map_base) {
     __shared__ typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
-    const int map = blockIdx.y;
+    const int map = map_base + (int)blockIdx.y;
     if (b.overflow[map]) return;
     raster_tile<TS, kVis, false>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
 }

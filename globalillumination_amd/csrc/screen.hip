@@ -120,6 +120,27 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
+int gi_soft_shadow_pass_chunk(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
+                              const float light_mvp[16], const float* trans_dev, const gi_soft_params* p,
+                              float* acc_dev, int first, int total_samples, float* out_dev, gi_stream stream) {
+    SoftUniforms u;
+    int rc = soft_setup(gbuffer, maps_dev, nsamples, smw, smh, light_mvp, trans_dev, p, out_dev, u);
+    if (rc != GI_OK) return rc;
+    if (!acc_dev || total_samples < 0 || total_samples > GI_MAX_SAMPLES) return GI_EINVAL;
+    if (total_samples > 0 && total_samples < nsamples) return GI_EINVAL;
+    SoftChunk ch{acc_dev, first ? 1 : 0, (float)total_samples};
+    GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const float4* tr = reinterpret_cast<const float4*>(trans_dev);
+    dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
+    switch (soft_variant(u)) {
+    case 3: hipLaunchKernelGGL(k_soft_chunk<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, ch, out_dev); break;
+    case 4: hipLaunchKernelGGL(k_soft_chunk<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, ch, out_dev); break;
+    default: hipLaunchKernelGGL(k_soft_chunk<0>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, ch, out_dev); break;
+    }
+    return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+}
+
 int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
                                  const float light_mvp[16], const float* trans_dev, const gi_soft_params* p,
                                  float* out_dev, gi_stream stream) {

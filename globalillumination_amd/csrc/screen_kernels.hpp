@@ -673,16 +673,26 @@ __device__ __forceinline__ float adaptive_weight(float4& T) {
     return 1.0f / (f * f);
 }
 
+// A run of consecutive samples of one Monte-Carlo frame (accFactor 1): the
+// running sum `acc` of the samples before the run is read (unless first) and
+// the sum after it stored, or, on the last run, divided by the frame's sample
+// count.  The additions are the single pass's, in the same order.
+struct SoftChunk {
+    float* acc;         // width*height running sums
+    int first;          // no samples before this run
+    float count_total;  // > 0 on the last run: the frame's samples (exact in FP32)
+};
+
 // AccurateSoftShadow.frag:52-133 for pixel (x, y): samples in order 0..N-1.
-template <int NT, bool kAdaptive = false>
+template <int NT, bool kAdaptive = false, bool kChunk = false>
 __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* __restrict__ maps, int smw, int smh,
                                            const float4* __restrict__ trans, const SoftUniforms& u,
-                                           float* __restrict__ out, int x, int y) {
+                                           float* __restrict__ out, int x, int y, const SoftChunk& ch = {}) {
     const size_t plane = (size_t)gb.w * gb.h;
     const size_t k = (size_t)y * gb.w + x;
     const float vx = gb.pos[k];
     if (vx == 0.0f) {  // Shadow.frag:244 background test; the cleared target holds 0
-        out[k] = 0.0f;
+        if (!kChunk || ch.count_total > 0.0f) out[k] = 0.0f;
         return;
     }
     const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
@@ -698,6 +708,9 @@ __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* _
     DepthMap D{maps, smw, smh, (float)smw, (float)smh};
     const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
     float acc = 0.0f, count = 0.0f;
+    if constexpr (kChunk) {
+        if (!ch.first) acc = ch.acc[k];
+    }
     // unrolled so several samples' gathers are in flight; the sum stays in order
 #pragma unroll 4
     for (int i = 0; i < u.nsamples; ++i) {
@@ -712,7 +725,14 @@ __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* _
             count = count + 1.0f;
         }
     }
-    out[k] = acc / count;
+    if constexpr (kChunk) {
+        if (ch.count_total > 0.0f)
+            out[k] = acc / ch.count_total;
+        else
+            ch.acc[k] = acc;
+    } else {
+        out[k] = acc / count;
+    }
 }
 
 // One thread per pixel.  Blocks are ordered XCD-aware (pixel_of_block) so each
@@ -725,6 +745,18 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const fl
     pixel_of_block(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     soft_pixel<NT, kAdaptive>(gb, maps, smw, smh, trans, u, out, x, y);
+}
+
+// One run of samples of a Monte-Carlo frame (SoftChunk).
+template <int NT>
+__global__ __launch_bounds__(kBlockThreads) void k_soft_chunk(GBufferView gb, const float* __restrict__ maps, int smw,
+                                                                  int smh, const float4* __restrict__ trans,
+                                                                  SoftUniforms u, SoftChunk ch,
+                                                                  float* __restrict__ out) {
+    int x, y;
+    pixel_of_block(gb.w, gb.h, x, y);
+    if (x >= gb.w || y >= gb.h) return;
+    soft_pixel<NT, false, true>(gb, maps, smw, smh, trans, u, out, x, y, ch);
 }
 
 // ---------------------------------------------------------------------------

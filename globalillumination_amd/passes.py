@@ -191,6 +191,45 @@ def render_shadow_map(mesh: DeviceMesh, mvps: np.ndarray, width: int, height: in
     return out
 
 
+class SplitDepthPass:
+    """gi_depth_pass in two steps (gi_depth_pass_bin, then
+    gi_depth_pass_raster over ranges of maps), with a workspace of its own so
+    that the steps may interleave with other passes on other streams.
+    `mvps`: (S, 16) unbiased matrices, host array or device tensor."""
+
+    def __init__(self, mesh: DeviceMesh, mvps, width: int, height: int, polygon_offset=(0.0, 0.0),
+                 out: torch.Tensor | None = None):
+        dev = mesh.positions.device
+        if isinstance(mvps, torch.Tensor):
+            if mvps.device != dev or mvps.dtype != torch.float32 or not mvps.is_contiguous() or mvps.numel() % 16:
+                raise ValueError("device mvps must be a contiguous float32 (S, 16) tensor on the mesh's device")
+            self.mvps_dev = mvps
+        else:
+            self.mvps_dev = torch.from_numpy(np.ascontiguousarray(mvps, dtype=np.float32).reshape(-1, 16)).to(dev)
+        self.mesh, self.S, self.width, self.height = mesh, self.mvps_dev.numel() // 16, int(width), int(height)
+        self.polygon_offset = (float(polygon_offset[0]), float(polygon_offset[1]))
+        self.out = out if out is not None else torch.empty((self.S, height, width), dtype=torch.float32, device=dev)
+        nbytes = _lib.lib().gi_depth_pass_workspace_size(mesh.ntris, self.S, self.width, self.height)
+        if nbytes == 0:
+            raise _lib.GiError("gi_depth_pass_workspace_size", _lib.GI_EINVAL, "bad size")
+        self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+
+    def _args(self, events=None):
+        ev = (None, None) if events is None else (events[0].cuda_event, events[1].cuda_event)
+        opts = _lib.GiRasterOpts(self.polygon_offset[0], self.polygon_offset[1], ev[0], ev[1])
+        m = self.mesh
+        return (_dptr(m.positions), m.nverts, _dptr(m.indices), m.ntris, _dptr(self.mvps_dev), self.S, self.width,
+                self.height, ctypes.byref(opts))
+
+    def bin(self, stream=None) -> None:
+        _lib.call("gi_depth_pass_bin", *self._args(), _dptr(self.out), _dptr(self.ws), ctypes.c_size_t(self.ws.numel()),
+                  _stream_handle(stream))
+
+    def raster(self, map_begin: int, map_count: int, stream=None, raster_events=None) -> None:
+        _lib.call("gi_depth_pass_raster", *self._args(raster_events), int(map_begin), int(map_count), _dptr(self.out),
+                  _dptr(self.ws), ctypes.c_size_t(self.ws.numel()), _stream_handle(stream))
+
+
 @dataclass
 class GBuffer:
     position: torch.Tensor  # (3, H, W)
@@ -409,6 +448,23 @@ def render_monte_carlo(gbuf: GBuffer, maps: torch.Tensor, light_mvps: np.ndarray
     _lib.call("gi_soft_shadow_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, _f(common, 16),
               _dptr(trans_dev), ctypes.byref(p), _dptr(out), _stream_handle(stream))
     return out
+
+
+def render_monte_carlo_chunk(gbuf: GBuffer, maps: torch.Tensor, common: np.ndarray, trans_dev: torch.Tensor,
+                             params: SoftParams, acc: torch.Tensor, first: bool, total_samples: int,
+                             out: torch.Tensor, stream=None) -> None:
+    """One run of consecutive samples of a renderMonteCarlo frame
+    (gi_soft_shadow_pass_chunk): `maps` (n, H, W) and `trans_dev` (n, 4) are
+    the run's slices, `common` the frame's soft uniform (soft_uniforms), `acc`
+    an (H, W) float32 running sum; total_samples > 0 on the last run."""
+    n, smh, smw = (int(x) for x in maps.shape)
+    if trans_dev.shape[0] != n:
+        raise ValueError("one trans row per depth map of the run is required")
+    g = gbuf.to_c(with_normals=False)
+    p = params.to_c()
+    _lib.call("gi_soft_shadow_pass_chunk", ctypes.byref(g), _dptr(maps), n, smw, smh, _f(common, 16),
+              _dptr(trans_dev), ctypes.byref(p), _dptr(acc), 1 if first else 0, int(total_samples), _dptr(out),
+              _stream_handle(stream))
 
 
 def adaptive_trans(light_mvps: np.ndarray, levels, zbias: float = 0.0) -> tuple:

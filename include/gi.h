@@ -111,6 +111,25 @@ int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices
                   const gi_raster_opts* opts, float* depth_out_dev,
                   void* workspace_dev, size_t workspace_bytes, gi_stream stream);
 
+/* gi_depth_pass in two steps, so that the tile raster can run a range of
+ * maps at a time: a frame can then be rastered and shaded in runs of maps
+ * (gi_soft_shadow_pass_chunk) with one run's maps in memory, or with a run's
+ * screen pass on another stream.  gi_depth_pass_bin bins all
+ * `nmaps` maps into the workspace and rasters any map whose lists overflowed
+ * it; gi_depth_pass_raster then rasters maps [map_begin, map_begin +
+ * map_count) from those lists.  Both take gi_depth_pass's arguments (the
+ * same workspace, untouched in between); calling raster over every map once
+ * after bin gives gi_depth_pass's maps bit for bit.  opts' events, when set,
+ * are recorded around the raster step's kernel.  No counterpart in the
+ * reference, which rasters each map in its own GL pass. */
+int gi_depth_pass_bin(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                      const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                      float* depth_out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream);
+int gi_depth_pass_raster(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                         const float* mvps_dev, int nmaps, int width, int height, const gi_raster_opts* opts,
+                         int map_begin, int map_count, float* depth_out_dev, void* workspace_dev,
+                         size_t workspace_bytes, gi_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Camera G-buffer (input format of the screen passes, a6)                   */
 /* ------------------------------------------------------------------------ */
@@ -222,6 +241,21 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
 int gi_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, int nmaps, int smw, int smh,
                           const int map_index[4], const float biased_mvps[64], int div, float* image_dev,
                           unsigned* count_dev, gi_stream stream);
+
+/* gi_soft_shadow_pass over a run of consecutive samples of one frame, so that
+ * a frame can be shaded in runs while later samples' maps are still being
+ * rastered.  maps_dev / trans_dev point at the run's first map and trans row,
+ * nsamples is the run's length; light_mvp and params as gi_soft_shadow_pass.
+ * acc_dev (width*height floats, caller-owned) carries each pixel's running
+ * sum from run to run: `first` starts it at 0, otherwise it is read; the
+ * last run passes the frame's sample count in total_samples (> 0) and writes
+ * out_dev = sum / total_samples, the earlier runs pass 0 and store the sum.
+ * The additions are gi_soft_shadow_pass's in the same order, so the runs
+ * give its output bit for bit.  No counterpart in the reference, whose
+ * AccurateSoftShadow.frag loops over every sample in one draw. */
+int gi_soft_shadow_pass_chunk(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw, int smh,
+                              const float light_mvp[16], const float* trans_dev, const gi_soft_params* params,
+                              float* acc_dev, int first, int total_samples, float* out_dev, gi_stream stream);
 
 /* The final pass of renderAdaptiveLightSourceSampling (main.cpp:868-876):
  * AccurateSoftShadow.frag with adaptiveSampling == 1.  Arguments as

@@ -72,6 +72,81 @@ def test_depth_pass_device_matrices(gi, cuda):
         gi.render_shadow_map(dm, dev[:, :15].contiguous(), 256, 256)
 
 
+def test_depth_pass_split_matches_single_call(gi, cuda):
+    """gi_depth_pass_bin, then gi_depth_pass_raster over uneven ranges of
+    maps, gives gi_depth_pass's maps bit for bit; bad ranges are rejected."""
+    wl = gi.scenes.config3(nsamples=9, view_w=64, view_h=64, sm=1024, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    mvps = light_mvps(wl)
+    want = gi.render_shadow_map(dm, mvps, 1024, 1024, polygon_offset=(4.0, 20.0))
+    sp = gi.passes.SplitDepthPass(dm, mvps, 1024, 1024, polygon_offset=(4.0, 20.0))
+    sp.out.fill_(-1.0)
+    sp.bin()
+    for b, n in ((0, 2), (2, 5), (7, 1), (8, 1)):
+        sp.raster(b, n)
+    assert bits_equal(_np(sp.out), _np(want))
+    for b, n in ((-1, 2), (8, 2), (0, 0)):
+        with pytest.raises(gi.GiError):
+            sp.raster(b, n)
+
+
+@pytest.mark.parametrize("pcf", [False, True])
+def test_soft_pass_runs_match_single_pass(gi, cuda, pcf):
+    """gi_soft_shadow_pass_chunk over runs of 5, 7 and 4 samples carries the
+    running sums and gives gi_soft_shadow_pass's output bit for bit."""
+    wl = gi.scenes.config3(nsamples=16, view_w=640, view_h=360, sm=1024, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 640, 360)
+    mvps = light_mvps(wl)
+    maps = gi.render_shadow_map(dm, mvps, 1024, 1024)
+    params = gi.SoftParams(pcf=pcf, kernelOrder=3, penumbra=1.0, inclusive=False)
+    want = _np(gi.render_monte_carlo(g, maps, mvps, params))
+    common, trans = gi.soft_uniforms(mvps)
+    trans_dev = torch.from_numpy(trans).cuda()
+    acc = torch.full((360, 640), 123.0, device="cuda")
+    out = torch.full((360, 640), -7.0, device="cuda")
+    b = 0
+    for n in (5, 7, 4):
+        gi.passes.render_monte_carlo_chunk(g, maps[b:b + n], common, trans_dev[b:b + n], params, acc, b == 0,
+                                           16 if b + n == 16 else 0, out)
+        b += n
+    assert bits_equal(_np(out), want)
+
+
+def test_frame_on_two_streams_matches_sequential(gi, cuda):
+    """The frame in runs of 4 maps: bin and rasters on one stream, each run's
+    soft pass on a second stream after that run's raster; same bits as the
+    single depth pass and soft pass."""
+    wl = gi.scenes.config3(nsamples=16, view_w=640, view_h=360, sm=1024, freq=40)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 640, 360)
+    mvps = light_mvps(wl)
+    params = gi.SoftParams(pcf=True, kernelOrder=3, penumbra=1.0, inclusive=False)
+    want_maps = gi.render_shadow_map(dm, mvps, 1024, 1024)
+    want = _np(gi.render_monte_carlo(g, want_maps, mvps, params))
+    common, trans = gi.soft_uniforms(mvps)
+    trans_dev = torch.from_numpy(trans).cuda()
+    acc = torch.empty((360, 640), device="cuda")
+    out = torch.empty((360, 640), device="cuda")
+    sa, sb = torch.cuda.current_stream(), torch.cuda.Stream()
+    sp = gi.passes.SplitDepthPass(dm, mvps, 1024, 1024)
+    for rep in range(3):
+        out.fill_(-1.0)
+        sb.wait_stream(sa)
+        sp.bin(stream=sa)
+        for c in range(4):
+            sp.raster(4 * c, 4, stream=sa)
+            ev = torch.cuda.Event()
+            ev.record(sa)
+            sb.wait_event(ev)
+            gi.passes.render_monte_carlo_chunk(g, sp.out[4 * c:4 * c + 4], common, trans_dev[4 * c:4 * c + 4], params,
+                                               acc, c == 0, 16 if c == 3 else 0, out, stream=sb)
+        sa.wait_stream(sb)
+        torch.cuda.synchronize()
+        assert bits_equal(_np(sp.out), _np(want_maps)), rep
+        assert bits_equal(_np(out), want), rep
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (33, 17), (100, 257)])
 def test_depth_pass_ragged_sizes(gi, oracle_lib, cuda, w, h):
     wl = gi.scenes.config1()
