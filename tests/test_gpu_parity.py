@@ -482,7 +482,8 @@ def test_rev_node_test_matches_oracle(gi, oracle_lib, cuda):
         disc_np[i] = oracle_lib.rev_discontinuity(gp, gn, _np(maps[i]), op)
     vis = torch.zeros((g.height, g.width), dtype=torch.float32, device="cuda")
     vis_np = np.zeros((g.height, g.width), np.float32)
-    for level, order, div in ((0, [0, 1, 2, 3], 1), (2, [3, 2, 1, 0], 2)):
+    for level, order, div in ((0, [0, 1, 2, 3], 1), (2, [3, 2, 1, 0], 2), (3, [1, 3, 0, 2], 3),
+                              (4, [2, 0, 3, 1], 6)):
         print(f"rev node test level {level}", flush=True)
         img, count = gi.passes.rev_quadtree_node_test(g, maps, order, biased[order], disc, level, vis, div)
         want_img, want_count = oracle_lib.rev_node_test(gp, _np(maps), order, biased[order], disc_np, level, vis_np,
@@ -493,8 +494,8 @@ def test_rev_node_test_matches_oracle(gi, oracle_lib, cuda):
         assert (want_img == 5.0).sum() > 0 and (vis_np == 0.5).sum() > 0
 
 
-@pytest.mark.parametrize("rpcf", [False, True])
-def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf):
+@pytest.mark.parametrize("rpcf,si", [(False, 0.25), (True, 0.25), (False, 0.5)])
+def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si):
     """RevectorizationBasedAccurateSoftShadow.frag with mixed quad-tree levels,
     discontinuity layers for the first 6 of 9 maps (the rest read a
     3-layer-clamped or stale layer as the reference does), and the visibility
@@ -516,17 +517,17 @@ def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf):
     oracle_lib.rev_node_test(gp, _np(maps), [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc_np, 0, vis_np)
     levels = [0, 1, 2, 3, 4, 1, 6, 2, 0]
     common, trans = gi.passes.adaptive_trans(mvps, levels)
-    params = gi.RevSoftParams(MV, nm, lpos, RPCF=rpcf, maxSearch=16, depthThreshold=2.5e-6)
-    print(f"rev soft rpcf={rpcf}", flush=True)
+    params = gi.RevSoftParams(MV, nm, lpos, shadowIntensity=si, RPCF=rpcf, maxSearch=16, depthThreshold=2.5e-6)
+    print(f"rev soft rpcf={rpcf} si={si}", flush=True)
     got = _np(gi.passes.render_rev_adaptive(g, maps, common, trans, disc, vis, params))
-    op = oracle_lib.make_params(common, 1024, 1024, "naive", shadow_intensity=0.25, max_search=16,
+    op = oracle_lib.make_params(common, 1024, 1024, "naive", shadow_intensity=si, max_search=16,
                                 depth_threshold=2.5e-6, pre_evaluation=True, mv=MV, normal_matrix=nm,
                                 light_position=lpos)
     want = oracle_lib.rev_soft(gp, gn, _np(maps), common, trans, disc_np, vis_np, op, rpcf=rpcf)
-    assert bits_equal(got, want), mismatch_report(got, want, f"rev soft rpcf={rpcf}")
+    assert bits_equal(got, want), mismatch_report(got, want, f"rev soft rpcf={rpcf} si={si}")
     pen = (vis_np == 0.5) & (gp[0] != 0)
     assert pen.sum() > 100
-    assert ((want > 0.25) & (want < 1.0)).sum() > 100
+    assert ((want > si) & (want < 1.0)).sum() > 100
 
 
 def test_rev_rejects(gi, cuda):
@@ -651,6 +652,31 @@ def test_rbssm_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval,
         pcf = oracle_lib.pcss(_np(g.position), _np(g.normal) if pre_eval else None, _np(depth[0]), op,
                               blocker_search_size=7, kernel_size=7, light_source_radius=radius, z_near=znear)
         assert (pcf != want).any()
+
+
+@pytest.mark.parametrize("ks,smw,smh", [(15, 1024, 1024), (4, 1024, 1024), (7, 1536, 1024)])
+def test_rbssm_kernel_sizes_and_map_shapes(gi, oracle_lib, cuda, ks, smw, smh):
+    """RBSSM with the reference's other kernel sizes (15; an even size runs
+    int(-1.5) = -1 .. 1 and divides by 16) and a non-square map."""
+    wl = gi.scenes.config3(nsamples=1, view_w=480, view_h=270, sm=smw, freq=71)
+    wl.lights = [gi.scenes.View((120.0, 280.0, 80.0), (0.0, 0.0, 0.0), smw, smh, fov=5.0)]
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), wl.camera.width, wl.camera.height)
+    mvps = light_mvps(wl)
+    depth = gi.render_shadow_map(dm, mvps, smw, smh, polygon_offset=(4.0, 20.0))
+    MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
+    params = gi.RBSSMParams(lightMVP=mvps[0], shadowMapWidth=smw, shadowMapHeight=smh, lightSourceRadius=64,
+                            kernelSize=ks, preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos,
+                            maxSearch=16, depthThreshold=2.5e-5)
+    print(f"rbssm ks={ks} map {smw}x{smh}", flush=True)
+    got = _np(gi.render_rbssm(g, depth[0], params))
+    op = oracle_lib.make_params(gi.biased(mvps[0]), smw, smh, "naive", shadow_intensity=0.25, max_search=16,
+                                depth_threshold=2.5e-5, pre_evaluation=True, mv=MV, normal_matrix=nm,
+                                light_position=lpos)
+    want = oracle_lib.rbssm(_np(g.position), _np(g.normal), _np(depth[0]), op, blocker_search_size=7,
+                            kernel_size=ks, light_source_radius=64, z_near=1)
+    assert bits_equal(got, want), mismatch_report(got, want, f"rbssm ks={ks} {smw}x{smh}")
+    assert ((want > 0.25) & (want < 1.0)).sum() > 100
 
 
 def test_rbssm_rejects(gi, cuda):
