@@ -334,6 +334,52 @@ __device__ __forceinline__ Setup to_setup(const Tri<0>& t, uint32_t id) {
     return s;
 }
 
+// The count pass's record of a directly binned triangle, 48 bytes: the list
+// entry supplies the id, and the tie bits follow from the edges.  Pixel
+// bounds fit 16 bits (map edges are at most GI_MAX_EDGE).
+struct alignas(16) SetupRec {
+    int X[3], Y[3];
+    float z0, A, B, zoff;
+    uint32_t bx, by;  // px0 | px1 << 16, py0 | py1 << 16
+};
+static_assert(sizeof(SetupRec) == 48, "three 16-byte loads");
+static_assert(GI_MAX_EDGE <= 65536, "pixel bounds are packed in 16 bits");
+
+__device__ __forceinline__ SetupRec to_rec(const Tri<0>& t) {
+    SetupRec r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        r.X[i] = (int)t.X[i];
+        r.Y[i] = (int)t.Y[i];
+    }
+    r.z0 = t.z0; r.A = t.A; r.B = t.B; r.zoff = t.zoff;
+    r.bx = (uint32_t)t.px0 | ((uint32_t)t.px1 << 16);
+    r.by = (uint32_t)t.py0 | ((uint32_t)t.py1 << 16);
+    return r;
+}
+
+// Same tie rule as setup_piece, on the same 24.8 coordinates.
+__device__ __forceinline__ Setup from_rec(const SetupRec& r, uint32_t id) {
+    Setup s;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        s.X[i] = r.X[i];
+        s.Y[i] = r.Y[i];
+    }
+    s.px0 = (int)(r.bx & 0xFFFFu); s.px1 = (int)(r.bx >> 16);
+    s.py0 = (int)(r.by & 0xFFFFu); s.py1 = (int)(r.by >> 16);
+    s.tl = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        int a = (i + 1) % 3, b = (i + 2) % 3;
+        long long dy = (long long)r.Y[b] - r.Y[a], dx = (long long)r.X[b] - r.X[a];
+        s.tl |= (dy < 0 || (dy == 0 && dx < 0)) ? (1 << i) : 0;
+    }
+    s.id = id;
+    s.z0 = r.z0; s.A = r.A; s.B = r.B; s.zoff = r.zoff;
+    return s;
+}
+
 // Same arithmetic as eval_pixel on the compact record.
 __device__ __forceinline__ bool eval_setup(const Setup& s, int i, int j, float& zout) {
     const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
@@ -389,7 +435,7 @@ struct BinBufs {
     uint32_t* clip;      // nmaps x wcap: triangles crossing a clip plane
     uint32_t* entries;   // nmaps x cap
     uint32_t* tiles;     // nmaps x tstride: the count pass's tiles of each directly binned triangle
-    Setup* setups;       // nmaps x tstride: its setup record (64-texel tiles only)
+    SetupRec* setups;    // nmaps x tstride: its setup record (64-texel tiles only)
     uint32_t cap, wcap, tstride;
 };
 
@@ -497,7 +543,7 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
                     mask |= 1u << k;
                 }
                 code = tiles_code(ty0 * cfg.tiles_x + tx0, ntx, mask);
-                if (!kScatter && kCache && mask) b.setups[(size_t)map * b.tstride + tri] = to_setup(t, entry);
+                if (!kScatter && kCache && mask) b.setups[(size_t)map * b.tstride + tri] = to_rec(t);
             }
         }
         if (!kScatter && kCache) b.tiles[(size_t)map * b.tstride + tri] = code;
@@ -812,7 +858,7 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
                 }
             } else if constexpr (!kFallback && wide_span<TS>() <= 4) {
                 // a directly binned triangle: the count pass stored its setup
-                const Setup s = b.setups[(size_t)map * b.tstride + (entry >> 3)];
+                const Setup s = from_rec(b.setups[(size_t)map * b.tstride + (entry >> 3)], entry);
                 int rx0 = max(x0, s.px0), rx1 = min(x1, s.px1), ry0 = max(y0, s.py0), ry1 = min(y1, s.py1);
                 if (rx0 <= rx1 && ry0 <= ry1) {
                     int slot = kRecQueue;
@@ -1038,7 +1084,7 @@ inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws
     size_t o_tiles = off; off = align256(off + sizeof(uint32_t) * (size_t)(ntris > 0 ? ntris : 0) * nmaps);
     const bool records = 4 * (64 / ts) * (64 / ts) <= 4;  // the tile code's span (wide_span<ts>() <= 4)
     size_t o_setups = off;
-    if (records) off = align256(off + sizeof(Setup) * (size_t)(ntris > 0 ? ntris : 0) * nmaps);
+    if (records) off = align256(off + sizeof(SetupRec) * (size_t)(ntris > 0 ? ntris : 0) * nmaps);
     size_t o_vis = off;
     if (vis) off = align256(off + sizeof(unsigned long long) * (size_t)w * h);
     if (ws) {
@@ -1053,7 +1099,7 @@ inline size_t layout(int ntris, int nmaps, int w, int h, bool vis, Workspace* ws
         b.clip = reinterpret_cast<uint32_t*>(base + o_clip);
         b.entries = reinterpret_cast<uint32_t*>(base + o_entries);
         b.tiles = reinterpret_cast<uint32_t*>(base + o_tiles);
-        b.setups = records ? reinterpret_cast<Setup*>(base + o_setups) : nullptr;
+        b.setups = records ? reinterpret_cast<SetupRec*>(base + o_setups) : nullptr;
         b.tstride = (uint32_t)(ntris > 0 ? ntris : 0);
         b.cap = cap;
         b.wcap = wcap;
