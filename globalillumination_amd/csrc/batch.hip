@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void kb_bin(RasterTable t) {
     }
 }
 
-__global__ __launch_bounds__(256) void kb_clip(RasterTable t) {
+__global__ __launch_bounds__(kClipThreads) void kb_clip(RasterTable t) {
     const uint32_t* pre = t.prefix + kStClip * (t.njobs + 1);
     const uint32_t total = pre[t.njobs];
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -363,7 +363,8 @@ int run_depth(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
     if ((g = grid_of(R.total[kStFill], cap))) hipLaunchKernelGGL(kb_fill, dim3(g), dim3(256), 0, s, tab);
 #define GI_DEPTH_STAGES(TS)                                                                                        \
     if ((g = grid_of(R.total[kStBin], cap))) hipLaunchKernelGGL((kb_bin<TS, false>), dim3(g), dim3(256), 0, s, tab); \
-    if ((g = grid_of(R.total[kStClip], cap))) hipLaunchKernelGGL(kb_clip, dim3(g), dim3(256), 0, s, tab);            \
+    if ((g = grid_of(R.total[kStClip], cap)))                                                                       \
+        hipLaunchKernelGGL(kb_clip, dim3(g), dim3(kClipThreads), 0, s, tab);                                        \
     if ((g = grid_of(R.total[kStWide], cap)))                                                                       \
         hipLaunchKernelGGL((kb_bin_wide<TS, false>), dim3(g), dim3(256), 0, s, tab);                                \
     if ((g = grid_of(R.total[kStScan], cap))) hipLaunchKernelGGL(kb_scan, dim3(g), dim3(kScanThreads), 0, s, tab);   \

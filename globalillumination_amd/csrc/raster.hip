@@ -64,7 +64,7 @@ int bin_stage(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Rast
     dim3 gw(kWideBlocks, nmaps);
     if (mesh.nt > 0) {
         hipLaunchKernelGGL((k_bin<kTile, false>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
-        hipLaunchKernelGGL(k_clip, dim3(kClipBlocks, nmaps), dim3(256), 0, s, mesh, mvps, cfg, b);
+        hipLaunchKernelGGL(k_clip, dim3(kClipBlocks, nmaps), dim3(kClipThreads), 0, s, mesh, mvps, cfg, b);
         hipLaunchKernelGGL((k_bin_wide<kTile, false>), gw, dim3(256), 0, s, cfg, b);
     }
     hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, b, ntiles);

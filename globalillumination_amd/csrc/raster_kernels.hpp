@@ -193,15 +193,13 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
     return true;
 }
 
-// Clips a triangle that crosses a clip plane and sets up fan piece `piece`.
-// Returns the number of fan pieces; `ok` tells whether `piece` is non-empty.
+// Sutherland-Hodgman over the six clip planes, from the triangle in
+// poly[0..2]; `poly` and `tmp` hold kMaxPoly vertices each and swap roles
+// after every plane.  Sets up fan piece `piece` of the result and returns the
+// number of fan pieces; `ok` tells whether `piece` is non-empty.
 template <int NA>
-__device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
-                                           Tri<NA>& t, bool& ok) {
-    CV<NA> poly[kMaxPoly], tmp[kMaxPoly];
-    poly[0] = v[0];
-    poly[1] = v[1];
-    poly[2] = v[2];
+__device__ __forceinline__ int clip_fan(CV<NA>* poly, CV<NA>* tmp, int piece, const RasterCfg& cfg, Tri<NA>& t,
+                                        bool& ok) {
     int n = 3;
     ok = false;
     for (int p = 0; p < 6; ++p) {
@@ -219,10 +217,23 @@ __device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const R
         }
         n = m;
         if (n < 3) return 0;
-        for (int k = 0; k < n; ++k) poly[k] = tmp[k];
+        CV<NA>* x = poly;
+        poly = tmp;
+        tmp = x;
     }
     if (piece + 2 < n) ok = setup_piece<NA>(poly[0], poly[piece + 1], poly[piece + 2], cfg, t);
     return n - 2;
+}
+
+// clip_fan with the polygons in private memory.
+template <int NA>
+__device__ __noinline__ int clip_and_setup(const CV<NA> v[3], int piece, const RasterCfg& cfg,
+                                           Tri<NA>& t, bool& ok) {
+    CV<NA> poly[kMaxPoly], tmp[kMaxPoly];
+    poly[0] = v[0];
+    poly[1] = v[1];
+    poly[2] = v[2];
+    return clip_fan<NA>(poly, tmp, piece, cfg, t, ok);
 }
 
 // Number of fan pieces of `tri` (0 when trivially rejected) and the setup of
@@ -422,7 +433,8 @@ constexpr int wide_span() {
 }
 constexpr uint32_t kWideBit = 0x80000000u;
 constexpr int kWideBlocks = 64;         // workgroups per map binning wide pieces
-constexpr int kClipBlocks = 4;          // workgroups per map clipping listed triangles
+constexpr int kClipBlocks = 16;         // workgroups per map clipping listed triangles
+constexpr int kClipThreads = 64;        // threads per clipping workgroup
 
 struct BinBufs {
     uint32_t* counts;    // nmaps x ntiles (scatter cursors after k_scan)
@@ -589,25 +601,30 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
 // `part` of `nparts` takes every nparts-th stride of the map's items.
 constexpr int kMaxPieces = kMaxPoly - 2;
 
+// Each thread keeps its two polygons in LDS (kClipThreads x 360 bytes):
+// private arrays indexed at run time would live in scratch memory, and the
+// clip's chain of dependent reads and writes would wait on each one.
 __device__ __forceinline__ void clip_body(const MeshView& mesh, const MvpSource& mvps, const RasterCfg& cfg,
                                           const BinBufs& b, int part, int nparts, int map) {
+    __shared__ CV<0> pool[kClipThreads][2][kMaxPoly];
     const uint32_t n = min(b.nclip[map], b.wcap) * (uint32_t)kMaxPieces;
     float mvp[16];
     mvps.load(map, mvp);
-    for (uint32_t item = part * kRasterThreads + threadIdx.x; item < n; item += nparts * kRasterThreads) {
+    CV<0>* poly = pool[threadIdx.x][0];
+    CV<0>* tmp = pool[threadIdx.x][1];
+    for (uint32_t item = part * kClipThreads + threadIdx.x; item < n; item += nparts * kClipThreads) {
         const uint32_t c = item / kMaxPieces;
         const int piece = (int)(item % kMaxPieces);
         const int tri = (int)b.clip[(size_t)map * b.wcap + c];
-        CV<0> v[3];
-        load_tri<0>(mesh, mvp, tri, v);
+        load_tri<0>(mesh, mvp, tri, poly);
         Tri<0> t;
         bool ok;
-        clip_and_setup<0>(v, piece, cfg, t, ok);
+        clip_fan<0>(poly, tmp, piece, cfg, t, ok);
         if (ok) push_record(b, map, t, (uint32_t)tri * 8u + (uint32_t)piece);
     }
 }
 
-__global__ __launch_bounds__(256) void k_clip(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
+__global__ __launch_bounds__(kClipThreads) void k_clip(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b) {
     clip_body(mesh, mvps, cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
