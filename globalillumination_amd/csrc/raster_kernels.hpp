@@ -968,6 +968,11 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mes
     __shared__ typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
+    // The usual case is no overflowed map: the threads read the flags
+    // together and the workgroup returns after one barrier.
+    bool any = false;
+    for (int m = (int)threadIdx.x; m < nmaps; m += (int)blockDim.x) any |= b.overflow[m] != 0;
+    if (!__syncthreads_or(any)) return;
     for (int map = 0; map < nmaps; ++map) {
         if (!b.overflow[map]) continue;
         raster_tile<TS, kVis, true>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);

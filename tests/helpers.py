@@ -37,3 +37,35 @@ def shrink_soup(n: int, seed: int = 1):
 
     soup = scenes.triangle_soup(n, seed=seed)
     return scenes.Mesh.concat(soup, scenes.ground_quad(40.0, 40.0, 0.0))
+
+
+def clip_heavy_mesh(n_small: int = 8000, n_clip: int = 200, seed: int = 3):
+    """n_small tiny triangles under a light at (0, 10, 0.1) and n_clip large
+    ones around the light's eye, most of which straddle the near plane or the
+    guard band and are cut into fan pieces by the clip kernel."""
+    from globalillumination_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    eye = np.array([0.0, 10.0, 0.1], np.float32)
+    big = eye + rng.uniform(-15, 15, size=(n_clip, 3, 3))
+    cen = np.c_[rng.uniform(-3, 3, n_small), rng.uniform(0, 2, n_small), rng.uniform(-3, 3, n_small)]
+    small = cen[:, None, :] + rng.uniform(-0.05, 0.05, size=(n_small, 3, 3))
+    pos = np.concatenate([small, big]).astype(np.float32)
+    n = n_small + n_clip
+    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
+    mesh = scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    light = scenes.View((0.0, 10.0, 0.1), (0.0, 0.0, 0.0), 128, 128, fov=60.0)
+    return mesh, light
+
+
+def clipped_count(mesh, mvp, guard=4.0):
+    """Triangles the depth pass lists for clipping: not outside any of the six
+    planes (z + w, w - z, x + g w, g w - x, y + g w, g w - y >= 0, g the guard
+    band) and not inside all of them."""
+    M = np.asarray(mvp, np.float64).reshape(4, 4).T
+    h = np.c_[mesh.positions.astype(np.float64), np.ones(mesh.nverts)] @ M.T
+    x, y, z, w = h.T
+    d = np.stack([z + w, w - z, x + guard * w, guard * w - x, y + guard * w, guard * w - y], 1)[mesh.indices]
+    reject = (d < 0).all(axis=1).any(axis=1)
+    inside = (d >= 0).all(axis=(1, 2))
+    return int((~reject & ~inside).sum())

@@ -154,6 +154,34 @@ def test_batch_rejects_bad_plans(small, gi):
     b.close()
 
 
+def test_batch_clip_heavy(cuda, gi):
+    """The batch's clip stage on the clip-heavy scene of the depth-pass
+    parity tests (more clipped triangles than one pass of its threads
+    covers), in two instances with two maps each."""
+    from types import SimpleNamespace
+
+    from globalillumination_amd import scenes
+    from globalillumination_amd.batch import Batch
+    from globalillumination_amd.passes import SoftParams
+    from helpers import clip_heavy_mesh
+
+    mesh, light = clip_heavy_mesh()
+    light2 = scenes.View((1.0, 9.0, -0.5), (0.0, 0.0, 0.0), 128, 128, fov=75.0)
+    cam = scenes.View((0.0, 6.0, 8.0), (0.0, 0.0, 0.0), 96, 64)
+    wls = [SimpleNamespace(mesh=mesh, lights=[light, light2], camera=cam),
+           SimpleNamespace(mesh=mesh, lights=[light2, light], camera=cam)]
+    insts = _instances(wls, [SoftParams(pcf=True, kernelOrder=3, penumbra=1.0), SoftParams(pcf=False)])
+    maps, outs = _reference(insts)
+    print("batch clip-heavy: 2 instances x 2 maps of 128^2", flush=True)
+    for tile in (64, 32):
+        b = Batch(insts)
+        b.set_tuning(tile, 2, 0, 2)
+        _clear(insts)
+        b.run()
+        _same(insts, maps, outs, f"clip-heavy, tile {tile}")
+        b.close()
+
+
 def test_batch_full_workload(cuda, gi):
     """32 instances: maps 512^2-4096^2, views 640x480-1920x1080, 5k-300k
     triangles, 1-16 samples (seed 4), autotuned, through a graph."""

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import bits_equal, camera_mvp, light_mvps, mismatch_report, shrink_soup
+from helpers import bits_equal, camera_mvp, clip_heavy_mesh, clipped_count, light_mvps, mismatch_report, shrink_soup
 
 pytestmark = pytest.mark.gpu
 
@@ -194,6 +194,24 @@ def test_depth_pass_wide_record_overflow(gi, oracle_lib, cuda):
     mvps = np.stack([gi.passes.view_matrices(light)[2]])
     got, want = _depth_both(gi, oracle_lib, mesh, mvps, 512, 512)
     assert bits_equal(got, want), mismatch_report(got, want, "wide record overflow")
+
+
+def test_depth_pass_clip_heavy(gi, oracle_lib, cuda):
+    # 156 of the 200 large triangles are clipped, more than one pass of the
+    # clip kernel's 16 x 64 threads (7 fan pieces each) covers.  The workspace
+    # holds every list without falling back: at most 4 tiles per piece and
+    # 7 pieces per clipped triangle fit 4 entries per triangle plus 1024, and
+    # 200 x 7 setup records fit 1024 + 8200 / 16.
+    mesh, light = clip_heavy_mesh()
+    mvp = gi.passes.view_matrices(light)[2]
+    n = clipped_count(mesh, mvp)
+    assert n * 7 > 16 * 64, n
+    light2 = gi.scenes.View((1.0, 9.0, -0.5), (0.0, 0.0, 0.0), 128, 128, fov=75.0)
+    mvps = np.stack([mvp, gi.passes.view_matrices(light2)[2]])
+    print(f"clip-heavy depth: {mesh.ntris} triangles, {n} clipped in map 0, 2 maps of 128^2", flush=True)
+    got, want = _depth_both(gi, oracle_lib, mesh, mvps, 128, 128)
+    assert bits_equal(got, want), mismatch_report(got, want, "clip-heavy depth")
+    assert (want < 1.0).mean() > 0.5
 
 
 def test_depth_pass_deterministic(gi, cuda):
