@@ -426,6 +426,22 @@ def main():
                            "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)},
             "phong_pass": {"kernel": "k_phong", "ms": round(phong_ms, 4), "bytes": int(npx * 48),
                            "GB/s": round(npx * 48 / (phong_ms * 1e-3) / 1e9, 1)}}
+    # The reference's own soft-shadow PCF loop (AccurateSoftShadow.frag:101-119,
+    # `<=`) takes 4 x 4 taps per sample where the workload names 3 x 3: the
+    # screen pass in that mode, over the frame's maps, and the frame it gives.
+    params16 = gi.SoftParams(pcf=True, kernelOrder=3, penumbra=1.0, inclusive=True)
+    ev16 = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    passes.render_monte_carlo(g, maps, mvps, params16, out=out, trans_dev=trans_dev)
+    for a0, a1 in ev16:
+        a0.record(stream)
+        passes.render_monte_carlo(g, maps, mvps, params16, out=out, trans_dev=trans_dev)
+        a1.record(stream)
+    torch.cuda.synchronize()
+    screen16_ms = float(np.mean([a0.elapsed_time(a1) for a0, a1 in ev16]))
+    ref_loop = {"taps_per_sample": 16, "screen_ms": round(screen16_ms, 4),
+                "taps_per_s": round(npx * S * 16 / (screen16_ms * 1e-3), -6),
+                "frame_ms_events": round(depth_ms + screen16_ms + phong_ms, 4),
+                "frame_Mpix_per_s_events": round(npx / ((depth_ms + screen16_ms + phong_ms) * 1e-3) / 1e6, 1)}
     diff = None if args.no_diff else diff_timing(gi, torch, args.diff_steps, 2)
     other = None if args.no_passes else passes_timing(gi, torch, args.passes_steps)
     batched = None if args.no_batch else batch_timing(gi, torch, args.batch_steps)
@@ -442,8 +458,8 @@ def main():
                                    "icosphere + ground",
                        "global_batch": 1, "samples": S, "shadow_map": smw, "view": [W, H],
                        "triangles": wl.mesh.ntris},
-            "roofline": roof, "cpu_baseline": cpu, "differentiable": diff, "batched": batched,
-            "other_passes": other}
+            "roofline": roof, "cpu_baseline": cpu, "reference_pcf_loop": ref_loop, "differentiable": diff,
+            "batched": batched, "other_passes": other}
     print(json.dumps(line), flush=True)
 
 
