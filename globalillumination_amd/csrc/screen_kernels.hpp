@@ -560,6 +560,10 @@ struct SoftUniforms {
     float si;
     int nsamples;
     int pcf;
+    // si lies in [0, 1] on a 2^-16 grid: every partial sum of up to 16 tap
+    // values (1 or si) is then a multiple of 2^-16 below 32, exact in FP32, so
+    // a sample's tap sum equals lit + (taps - lit) * si in any order
+    int exact_sum;
     int ntaps;             // taps per axis of the PCF loop
     float taps[kMaxTaps];  // loop values w of `for (w = -offset; w </<= offset; w += step)`
     float tapw[kMaxTaps];  // taps[a] * (1/W) and taps[a] * (1/H): the shader's products, in FP32
@@ -625,15 +629,38 @@ __device__ __forceinline__ float soft_term(const DepthMap& D, const SoftUniforms
 #pragma unroll
                 for (int q = 0; q < NT; ++q) rv[b][q] = r.v[q];
             }
+            if (u.exact_sum) {
+                // each texel is tested once; tap column a reads row element
+                // iu[a] - iu[0], so each element's count is weighted by the
+                // number of tap columns that read it
+                int weight[NT];
 #pragma unroll
-            for (int a = 0; a < NT; ++a) {
-                const int k = iu[a] - iu[0];
+                for (int q = 0; q < NT; ++q) weight[q] = 0;
 #pragma unroll
-                for (int b = 0; b < NT; ++b) {
-                    float d = rv[b][0];
+                for (int a = 0; a < NT; ++a) {
 #pragma unroll
-                    for (int q = 1; q < NT; ++q) d = (k == q) ? rv[b][q] : d;
-                    illum = illum + ((sz <= d) ? 1.0f : u.si);
+                    for (int q = 0; q < NT; ++q) weight[q] += (iu[a] - iu[0] == q) ? 1 : 0;
+                }
+                int lit = 0;
+#pragma unroll
+                for (int q = 0; q < NT; ++q) {
+                    int col = 0;
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) col += (sz <= rv[b][q]) ? 1 : 0;
+                    lit += weight[q] * col;
+                }
+                illum = (float)lit + (float)(NT * NT - lit) * u.si;
+            } else {
+#pragma unroll
+                for (int a = 0; a < NT; ++a) {
+                    const int k = iu[a] - iu[0];
+#pragma unroll
+                    for (int b = 0; b < NT; ++b) {
+                        float d = rv[b][0];
+#pragma unroll
+                        for (int q = 1; q < NT; ++q) d = (k == q) ? rv[b][q] : d;
+                        illum = illum + ((sz <= d) ? 1.0f : u.si);
+                    }
                 }
             }
         } else {
@@ -976,6 +1003,10 @@ inline int soft_setup(const gi_gbuffer* gbuffer, const float* maps_dev, int nsam
     u.si = p->shadow_intensity;
     u.nsamples = nsamples;
     u.pcf = p->pcf ? 1 : 0;
+    {
+        const float g = u.si * 65536.0f;  // exact: a power-of-two scale of a value in [0, 1]
+        u.exact_sum = (u.si >= 0.0f && u.si <= 1.0f && g == std::floor(g)) ? 1 : 0;
+    }
     u.ntaps = 1;
     for (int i = 0; i < kMaxTaps; ++i) u.taps[i] = u.tapw[i] = u.taph[i] = 0.0f;
     if (p->pcf) {
