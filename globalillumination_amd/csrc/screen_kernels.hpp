@@ -396,23 +396,33 @@ struct RbsmConservative {
     }
 };
 
-// Pixel index with an XCD-aware block order: hardware hands consecutive
-// workgroups to the 8 XCDs round-robin, so block b is remapped to make each
-// XCD's blocks one contiguous band of rows, keeping neighbouring pixels'
-// shadow-map texels in the same L2.
+// Pixel index with an XCD-aware block order.  Hardware hands consecutive
+// workgroups to the 8 XCDs round-robin; block b is remapped so that each XCD
+// drains one contiguous range of a strip-major order: vertical strips of
+// ceil(bx_n / 8) block columns, row-major inside each strip.  Neighbouring
+// pixels' shadow-map texels then stay in one XCD's L2, and every XCD's strip
+// crosses the whole image height, which balances background (cheap) and lit
+// (costly) pixels across the XCDs better than horizontal bands do.
 __device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
     const int bx_n = (w + kBlockX - 1) / kBlockX;
-    const int nblocks = bx_n * (int)gridDim.y;
-    int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
-    int per = nblocks / 8;
-    int rem = nblocks - per * 8;
-    int xcd = b % 8, slot = b / 8;
+    const int by_n = (int)gridDim.y;
+    const int nblocks = bx_n * by_n;
+    const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+    const int per = nblocks / 8;
+    const int rem = nblocks - per * 8;
+    const int xcd = b % 8, slot = b / 8;
     int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
     if (lin >= nblocks) lin = b;
+    const int strip = (bx_n + 7) / 8;
+    const int nstrips = (bx_n + strip - 1) / strip;
+    int k = lin / (strip * by_n);
+    if (k > nstrips - 1) k = nstrips - 1;
+    const int q = lin - k * strip * by_n;
+    const int wk = min(strip, bx_n - k * strip);
     int px, py;
     wave_tile_xy((int)threadIdx.x, px, py);
-    x = (lin % bx_n) * kBlockX + px;
-    y = (lin / bx_n) * kBlockY + py;
+    x = (k * strip + q % wk) * kBlockX + px;
+    y = (q / wk) * kBlockY + py;
 }
 
 template <int kTech>
