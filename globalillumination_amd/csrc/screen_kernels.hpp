@@ -403,6 +403,20 @@ struct RbsmConservative {
 // pixels' shadow-map texels then stay in one XCD's L2, and every XCD's strip
 // crosses the whole image height, which balances background (cheap) and lit
 // (costly) pixels across the XCDs better than horizontal bands do.
+// Block (bx, by) at position lin of the strip-major order of a bx_n x by_n
+// grid: vertical strips of ceil(bx_n / 8) block columns, row-major inside
+// each strip.
+__device__ __forceinline__ void strip_block(int lin, int bx_n, int by_n, int& bx, int& by) {
+    const int strip = (bx_n + 7) / 8;
+    const int nstrips = (bx_n + strip - 1) / strip;
+    int k = lin / (strip * by_n);
+    if (k > nstrips - 1) k = nstrips - 1;
+    const int q = lin - k * strip * by_n;
+    const int wk = min(strip, bx_n - k * strip);
+    bx = k * strip + q % wk;
+    by = q / wk;
+}
+
 __device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
     const int bx_n = (w + kBlockX - 1) / kBlockX;
     const int by_n = (int)gridDim.y;
@@ -413,16 +427,11 @@ __device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
     const int xcd = b % 8, slot = b / 8;
     int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
     if (lin >= nblocks) lin = b;
-    const int strip = (bx_n + 7) / 8;
-    const int nstrips = (bx_n + strip - 1) / strip;
-    int k = lin / (strip * by_n);
-    if (k > nstrips - 1) k = nstrips - 1;
-    const int q = lin - k * strip * by_n;
-    const int wk = min(strip, bx_n - k * strip);
-    int px, py;
+    int bx, by, px, py;
+    strip_block(lin, bx_n, by_n, bx, by);
     wave_tile_xy((int)threadIdx.x, px, py);
-    x = (k * strip + q % wk) * kBlockX + px;
-    y = (q / wk) * kBlockY + py;
+    x = bx * kBlockX + px;
+    y = by * kBlockY + py;
 }
 
 template <int kTech>
