@@ -59,6 +59,7 @@ def lib() -> ctypes.CDLL:
             "or_pcss": [vp, vp, i, i, vp, vp, i, i, i, i, vp],
             "or_rbssm": [vp, vp, i, i, vp, vp, i, i, i, i, vp],
             "or_rbssm_tap": [vp, vp, f, f, f, vp],
+            "or_rbsm_tap": [vp, vp, i, f, f, f, vp],
             "or_soft_shadow_adaptive": [vp, i, i, vp, i, i, i, vp, vp, ctypes.c_float, i, i, ctypes.c_float, i, vp],
             "or_qt_node_test": [vp, i, i, vp, i, i, vp, vp, i, vp, vp],
             "or_rev_discontinuity": [vp, vp, i, i, vp, vp, vp],
@@ -66,6 +67,8 @@ def lib() -> ctypes.CDLL:
             "or_rev_soft": [vp, vp, i, i, vp, i, i, i, vp, vp, vp, i, vp, vp, i, vp],
             "or_diff_depth": [vp, i, vp, i, vp, i, i, vp, vp],
             "or_diff_soft": [vp, i, i, vp, i, i, i, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, vp],
+            "or_diff_adjoint": [vp, i, vp, i, vp, vp, vp, i, i, i, vp, i, i, ctypes.c_double, ctypes.c_double,
+                                ctypes.c_double, vp, vp, vp, vp],
             "or_set_threads": [i], "or_get_threads": [],
         }
         for name, args in sig.items():
@@ -196,6 +199,21 @@ def rbssm_tap(depth, params, x, y, z):
     if rc:
         raise RuntimeError(f"or_rbssm_tap failed ({rc})")
     keys = ("r", "g", "b", "left", "right", "down", "up", "nx", "ny", "fill")
+    return {k: float(v) for k, v in zip(keys, out)}
+
+
+def rbsm_tap(depth, params, x, y, z, filtered=False):
+    """One RBSM tap: dict of the discontinuity (r, g, b), the four edge-walk
+    lengths, the two relative positions and revectorizeShadow's value (before
+    the mix with the shadow intensity; the plain test without a
+    discontinuity)."""
+    d = _c32(depth)
+    assert d.shape == (params.sm_height, params.sm_width)
+    out = np.empty(10, np.float32)
+    rc = lib().or_rbsm_tap(_p(d), ctypes.byref(params), int(bool(filtered)), float(x), float(y), float(z), _p(out))
+    if rc:
+        raise RuntimeError(f"or_rbsm_tap failed ({rc})")
+    keys = ("r", "g", "b", "left", "right", "down", "up", "rx", "ry", "value")
     return {k: float(v) for k, v in zip(keys, out)}
 
 
@@ -437,3 +455,29 @@ def diff_soft(gpos: np.ndarray, maps64: np.ndarray, biased_mvps64, tau, bias, si
     if rc:
         raise RuntimeError(f"or_diff_soft failed ({rc})")
     return out
+
+
+def diff_adjoint(positions, indices, mvps64, biased_mvps64, vis: np.ndarray, gpos: np.ndarray, tau, bias, si,
+                 grad_out) -> tuple:
+    """Float64 reverse-mode derivative of diff_depth + diff_soft for
+    L = sum(grad_out * S), the triangle of every texel held fixed by vis
+    (S x h x w).  mvps64 / biased_mvps64: (S, 16) column-major.  Returns
+    (dL/dpositions (nv, 3), dL/dM'_i (S, 16), dL/d(B*M'_i) (S, 16)), the
+    matrices column-major."""
+    pos = _c64(positions).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+    v = np.ascontiguousarray(vis, dtype=np.uint64)
+    S, h, w = v.shape
+    m = _c64(mvps64).reshape(S, 16)
+    mb = _c64(biased_mvps64).reshape(S, 16)
+    g = _c32(gpos)
+    _, gh, gw = g.shape
+    go = _c64(grad_out).reshape(gh, gw)
+    gp = np.empty_like(pos)
+    gm = np.empty((S, 16), np.float64)
+    gb = np.empty((S, 16), np.float64)
+    rc = lib().or_diff_adjoint(_p(pos), pos.shape[0], _p(idx), idx.shape[0], _p(m), _p(mb), _p(v), S, w, h, _p(g),
+                               gw, gh, float(tau), float(bias), float(si), _p(go), _p(gp), _p(gm), _p(gb))
+    if rc:
+        raise RuntimeError(f"or_diff_adjoint failed ({rc})")
+    return gp, gm, gb

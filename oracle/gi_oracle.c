@@ -1246,6 +1246,174 @@ int or_diff_soft(const float* gpos, int gw, int gh, const double* maps, int nsam
     return 0;
 }
 
+/* Reverse-mode derivative of the continuous model above (or_diff_depth +
+ * or_diff_soft), in FP64 and in a fixed order, for L = sum_k g_k * S_k:
+ *   grad_pos     (nv x 3): dL/d(vertex positions), through the texel depths;
+ *   grad_mvps    (S x 16, column-major): dL/dM'_i through the texel depths;
+ *   grad_biased  (S x 16, column-major): dL/d(B*M'_i) through the receivers'
+ *                projections.
+ * The texel-depth part differentiates or_diff_depth's own expression
+ *   l1 = ((xc-X0)(Y2-Y0) - (X2-X0)(yc-Y0)) / D,
+ *   l2 = ((X1-X0)(yc-Y0) - (xc-X0)(Y1-Y0)) / D,
+ *   d  = Z0 + l1 (Z1-Z0) + l2 (Z2-Z0)
+ * term by term (chain rule through D, the two numerators and the window
+ * transform), with the triangle of every texel held fixed by `vis`.  Maps are
+ * independent; each map's vertex gradient is kept apart and the maps are
+ * added in index order, so the result does not depend on the thread count.
+ * TEST INFRASTRUCTURE: the checker of the adjoint kernels, itself checked
+ * against central finite differences of or_diff_depth + or_diff_soft. */
+int or_diff_adjoint(const double* pos, int nv, const int* idx, int nt, const double* mvps,
+                    const double* biased_mvps, const uint64_t* vis, int nsamples, int w, int h,
+                    const float* gpos, int gw, int gh, double tau, double bias, double si,
+                    const double* grad_out, double* grad_pos, double* grad_mvps, double* grad_biased)
+{
+    if (!pos || !idx || !mvps || !biased_mvps || !vis || !gpos || !grad_out || !grad_pos || !grad_mvps ||
+        !grad_biased || nv <= 0 || nt < 0 || nsamples <= 0 || w <= 0 || h <= 0 || gw <= 0 || gh <= 0 ||
+        !(tau > 0.0))
+        return -1;
+    const size_t msz = (size_t)w * h, plane = (size_t)gw * gh;
+    double* gv_map = (double*)calloc((size_t)nsamples * nv * 3, sizeof(double));
+    if (!gv_map) return -1;
+    int fail = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) OMP_THREADS
+#endif
+    for (int i = 0; i < nsamples; ++i) {
+        double* D = (double*)malloc(msz * sizeof(double));
+        double* G = (double*)calloc(msz, sizeof(double));
+        double* tg = (double*)calloc((size_t)(nt > 0 ? nt : 1) * 9, sizeof(double)); /* dL/d(X,Y,Z)_j */
+        if (!D || !G || !tg) {
+            free(D); free(G); free(tg);
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+            fail = 1;
+            continue;
+        }
+        const double* M = mvps + 16 * (size_t)i;
+        const double* Mb = biased_mvps + 16 * (size_t)i;
+        const uint64_t* vk = vis + (size_t)i * msz;
+        or_diff_depth(pos, nv, idx, nt, M, w, h, vk, D);
+        /* receivers: dL/dd per texel and dL/d(B*M'_i) */
+        double gmb[16] = {0};
+        for (size_t k = 0; k < plane; ++k) {
+            if (gpos[k] == 0.0f || grad_out[k] == 0.0) continue;
+            const double p[4] = {gpos[k], gpos[plane + k], gpos[2 * plane + k], 1.0};
+            double c[4];
+            m4v_d(Mb, p[0], p[1], p[2], 1.0, c);
+            const double s = c[0] / c[3], t = c[1] / c[3], zz = c[2] / c[3];
+            const double u = s * w - 0.5, v = t * h - 0.5;
+            const double a0 = floor(u), b0 = floor(v);
+            const double fu = u - a0, fv = v - b0;
+            const double wu[2] = {1.0 - smooth_w(fu), smooth_w(fu)}, wv[2] = {1.0 - smooth_w(fv), smooth_w(fv)};
+            const double dwu[2] = {-6.0 * fu * (1.0 - fu), 6.0 * fu * (1.0 - fu)};
+            const double dwv[2] = {-6.0 * fv * (1.0 - fv), 6.0 * fv * (1.0 - fv)};
+            const double gq = grad_out[k] * (1.0 - si) / nsamples; /* dL/dq_i */
+            double g_z = 0.0, g_fu = 0.0, g_fv = 0.0;
+            for (int db = 0; db < 2; ++db)
+                for (int da = 0; da < 2; ++da) {
+                    const long long a = (long long)a0 + da, b = (long long)b0 + db;
+                    const int in = a >= 0 && a < w && b >= 0 && b < h;
+                    const double d = in ? D[(size_t)b * w + a] : 0.0;
+                    const double sg = sigmoid_d((d - zz + bias) / tau);
+                    const double dsg = sg * (1.0 - sg) / tau; /* d sigmoid / d d */
+                    if (in) G[(size_t)b * w + a] += gq * wu[da] * wv[db] * dsg;
+                    g_z -= gq * wu[da] * wv[db] * dsg;
+                    g_fu += gq * dwu[da] * wv[db] * sg;
+                    g_fv += gq * wu[da] * dwv[db] * sg;
+                }
+            /* u = s*w - .5 and fu = u - floor(u): d fu / d s = w */
+            const double g_s = g_fu * w, g_t = g_fv * h;
+            const double gc[4] = {g_s / c[3], g_t / c[3], g_z / c[3],
+                                  -(g_s * c[0] + g_t * c[1] + g_z * c[2]) / (c[3] * c[3])};
+            for (int col = 0; col < 4; ++col)
+                for (int row = 0; row < 4; ++row) gmb[col * 4 + row] += gc[row] * p[col];
+        }
+        memcpy(grad_biased + 16 * (size_t)i, gmb, sizeof gmb);
+        /* texel depths: dL/d(window X, Y, Z) of each triangle's vertices */
+        for (size_t k = 0; k < msz; ++k) {
+            const double g = G[k];
+            const uint64_t key = vk[k];
+            if (g == 0.0 || key == ~(uint64_t)0) continue;
+            const int tri = (int)((key & 0xffffffffu) >> 3);
+            if (tri >= nt) continue;
+            double X[3], Y[3], Z[3];
+            int ok = 1;
+            for (int j = 0; j < 3; ++j) {
+                const double* vp = pos + 3 * (size_t)idx[3 * tri + j];
+                double c[4];
+                m4v_d(M, vp[0], vp[1], vp[2], 1.0, c);
+                if (!(c[3] > 0.0)) ok = 0;
+                X[j] = (c[0] / c[3]) * (0.5 * w) + 0.5 * w;
+                Y[j] = (c[1] / c[3]) * (0.5 * h) + 0.5 * h;
+                Z[j] = (c[2] / c[3]) * 0.5 + 0.5;
+            }
+            const double e1x = X[1] - X[0], e1y = Y[1] - Y[0], e2x = X[2] - X[0], e2y = Y[2] - Y[0];
+            const double Dt = e1x * e2y - e2x * e1y;
+            if (!ok || Dt == 0.0) continue;
+            const double xc = (double)(k % w) + 0.5, yc = (double)(k / w) + 0.5;
+            const double A = xc - X[0], B = yc - Y[0];
+            const double n1 = A * e2y - e2x * B, n2 = e1x * B - A * e1y;
+            const double l1 = n1 / Dt, l2 = n2 / Dt;
+            const double dz1 = Z[1] - Z[0], dz2 = Z[2] - Z[0];
+            /* d = Z0 + l1*dz1 + l2*dz2 */
+            const double g_l1 = g * dz1, g_l2 = g * dz2;
+            const double g_dz1 = g * l1, g_dz2 = g * l2;
+            const double g_n1 = g_l1 / Dt, g_n2 = g_l2 / Dt;
+            const double g_D = -(g_l1 * l1 + g_l2 * l2) / Dt;
+            double g_A = 0.0, g_B = 0.0, g_e1x = 0.0, g_e1y = 0.0, g_e2x = 0.0, g_e2y = 0.0;
+            /* n1 = A*e2y - e2x*B */
+            g_A += g_n1 * e2y; g_e2y += g_n1 * A; g_e2x -= g_n1 * B; g_B -= g_n1 * e2x;
+            /* n2 = e1x*B - A*e1y */
+            g_e1x += g_n2 * B; g_B += g_n2 * e1x; g_A -= g_n2 * e1y; g_e1y -= g_n2 * A;
+            /* Dt = e1x*e2y - e2x*e1y */
+            g_e1x += g_D * e2y; g_e2y += g_D * e1x; g_e2x -= g_D * e1y; g_e1y -= g_D * e2x;
+            double* r = tg + 9 * (size_t)tri; /* X0 X1 X2 Y0 Y1 Y2 Z0 Z1 Z2 */
+            r[0] += -g_A - g_e1x - g_e2x; r[1] += g_e1x; r[2] += g_e2x;
+            r[3] += -g_B - g_e1y - g_e2y; r[4] += g_e1y; r[5] += g_e2y;
+            r[6] += g - g_dz1 - g_dz2;    r[7] += g_dz1; r[8] += g_dz2;
+        }
+        /* window coordinates -> clip coordinates -> M'_i and the vertices */
+        double gm[16] = {0};
+        double* gv = gv_map + (size_t)i * nv * 3;
+        for (int tri = 0; tri < nt; ++tri) {
+            const double* r = tg + 9 * (size_t)tri;
+            int any = 0;
+            for (int q = 0; q < 9; ++q) any |= r[q] != 0.0;
+            if (!any) continue;
+            for (int j = 0; j < 3; ++j) {
+                const int vi = idx[3 * tri + j];
+                const double* vp = pos + 3 * (size_t)vi;
+                double c[4];
+                m4v_d(M, vp[0], vp[1], vp[2], 1.0, c);
+                /* X = (c0/c3)(w/2) + w/2, Y likewise, Z = (c2/c3)/2 + 1/2 */
+                const double gX = r[j] * 0.5 * w, gY = r[3 + j] * 0.5 * h, gZ = r[6 + j] * 0.5;
+                const double gc[4] = {gX / c[3], gY / c[3], gZ / c[3],
+                                      -(gX * c[0] + gY * c[1] + gZ * c[2]) / (c[3] * c[3])};
+                const double pv[4] = {vp[0], vp[1], vp[2], 1.0};
+                for (int col = 0; col < 4; ++col)
+                    for (int row = 0; row < 4; ++row) gm[col * 4 + row] += gc[row] * pv[col];
+                for (int a = 0; a < 3; ++a) {
+                    double s = 0.0;
+                    for (int row = 0; row < 4; ++row) s += M[a * 4 + row] * gc[row];
+                    gv[3 * (size_t)vi + a] += s;
+                }
+            }
+        }
+        memcpy(grad_mvps + 16 * (size_t)i, gm, sizeof gm);
+        free(D); free(G); free(tg);
+    }
+    if (!fail) {
+        for (size_t q = 0; q < (size_t)nv * 3; ++q) {
+            double s = 0.0;
+            for (int i = 0; i < nsamples; ++i) s += gv_map[(size_t)i * nv * 3 + q];
+            grad_pos[q] = s;
+        }
+    }
+    free(gv_map);
+    return fail ? -1 : 0;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Deferred Phong shading: ShadowMapping/Shaders/GBuffer/PhongShading.frag   */
 /* ------------------------------------------------------------------------ */
@@ -1934,6 +2102,44 @@ int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float
     float nx = ss_normalize(&c, out[3], out[4], ux), ny = ss_normalize(&c, out[5], out[6], uy);
     out[7] = nx; out[8] = ny;
     out[9] = ss_smooth(&c, x, y, z, nx, ny, dr, dg, db, ux, uy);
+    return 0;
+}
+
+/* One RBSM tap at texture coordinate (x, y), depth z, as every RBSM pass
+ * evaluates it (shadow_from_smsr's branches, shared by the RPCF loops):
+ * out = {r, g, b, left, right, down, up, rel_x, rel_y, value}, where value is
+ * revectorizeShadow's result before the mix with shadowIntensity (binary
+ * NonConservativeSMSR.frag:234-274, or continuous FilteredRBSM.frag:234-272
+ * when `filtered`); without a discontinuity value is the plain test and the
+ * walks are 0.  Walks are orientateDS's four lengths (:245-254). */
+int or_rbsm_tap(const float* depth, const or_params* p, int filtered, float x, float y, float z, float* out)
+{
+    if (!depth || !p || !out || p->sm_width <= 0 || p->sm_height <= 0) return -1;
+    rb_ctx c;
+    c.D = depth; c.W = p->sm_width; c.H = p->sm_height;
+    c.stepx = (float)(1.0 / (double)p->sm_width);
+    c.stepy = (float)(1.0 / (double)p->sm_height);
+    c.thr = p->depth_threshold;
+    c.maxSearch = p->max_search;
+    c.si = p->shadow_intensity;
+    c.filtered = filtered ? 1 : 0;
+    float dC = rb_tex(&c, x, y);
+    float dr, dg, db;
+    compute_discontinuity(&c, x, y, z, dC, &dr, &dg, &db);
+    float shadow = cmp_le(z, dC);
+    out[0] = dr; out[1] = dg; out[2] = db;
+    for (int k = 3; k < 9; ++k) out[k] = 0.0f;
+    out[9] = shadow;
+    if (dr > 0.0f || dg > 0.0f) {
+        float subx = gfract(x * (float)c.W), suby = gfract(y * (float)c.H);
+        out[3] = disc_length(&c, dr, dg, db, x, y, z, -1.0f, 0.0f, 1.0f - subx);
+        out[4] = disc_length(&c, dr, dg, db, x, y, z, 1.0f, 0.0f, subx);
+        out[5] = disc_length(&c, dr, dg, db, x, y, z, 0.0f, -1.0f, 1.0f - suby);
+        out[6] = disc_length(&c, dr, dg, db, x, y, z, 0.0f, 1.0f, suby);
+        out[7] = rel_position(&c, out[3], out[4], shadow);
+        out[8] = rel_position(&c, out[5], out[6], shadow);
+        out[9] = revectorize(&c, out[7], out[8], shadow);
+    }
     return 0;
 }
 

@@ -127,6 +127,10 @@ int or_rbssm(const float* pos, const float* nrm, int w, int h, const float* dept
  * left/right/down/up, out[7..8] the normalized discontinuity (x, y),
  * out[9] smoothONDS's fill before the shadow-intensity mix. */
 int or_rbssm_tap(const float* depth, const or_params* p, float x, float y, float z, float* out);
+/* One RBSM tap: out = {r, g, b, left, right, down, up, rel_x, rel_y, value}
+ * (binary NonConservativeSMSR or, with `filtered`, FilteredRBSM's continuous
+ * revectorizeShadow; value before the mix with shadowIntensity). */
+int or_rbsm_tap(const float* depth, const or_params* p, int filtered, float x, float y, float z, float* out);
 
 /* Revectorization-based adaptive sampling.  Discontinuity layers are
  * w*h bytes: bits 0-1 r*4, bits 2-3 g*4, bit 4 discType, bit 5 alpha, 0xFF
@@ -181,6 +185,17 @@ int or_diff_depth(const double* pos, int nv, const int* idx, int nt, const doubl
  * depth 0.  Background pixels (x == 0) give 0.  maps: N x h x w (FP64). */
 int or_diff_soft(const float* gpos, int gw, int gh, const double* maps, int nsamples, int w, int h,
                  const double* biased_mvps, double tau, double bias, double si, double* out);
+
+/* Reverse-mode derivative of or_diff_depth + or_diff_soft (FP64) for
+ * L = sum_k grad_out[k] * S[k], the triangle of every texel held fixed by
+ * vis (nsamples x h x w keys of or_visibility_raster).  mvps / biased_mvps:
+ * nsamples x 16 column-major (M'_i and B*M'_i).  Outputs: grad_pos (nv x 3),
+ * grad_mvps = dL/dM'_i through the texel depths and grad_biased =
+ * dL/d(B*M'_i) through the receivers (nsamples x 16, column-major). */
+int or_diff_adjoint(const double* pos, int nv, const int* idx, int nt, const double* mvps,
+                    const double* biased_mvps, const uint64_t* vis, int nsamples, int w, int h,
+                    const float* gpos, int gw, int gh, double tau, double bias, double si,
+                    const double* grad_out, double* grad_pos, double* grad_mvps, double* grad_biased);
 
 /* threads used by the OpenMP loops (0 = runtime default) */
 void or_set_threads(int n);

@@ -373,3 +373,224 @@ def test_conservative_rbsm_transpose_symmetry(oracle_lib):
     b = run_hard(oracle_lib, np.ascontiguousarray(d.T), pts[:, [1, 0, 2]], technique="rbsm_conservative",
                  depth_threshold=0.0)
     np.testing.assert_array_equal(a, b)
+
+
+# --------------------------------------------------------------------------
+# RBSM taps traced by hand through the shader (or_rbsm_tap returns the walks,
+# the relative positions and revectorizeShadow's value of one tap)
+# --------------------------------------------------------------------------
+def _tap_params(oracle_lib, d, thr=0.0, max_search=16):
+    h, w = d.shape
+    return oracle_lib.make_params(np.eye(4, dtype=np.float32).reshape(-1), w, h, "naive", shadow_intensity=0.0,
+                                  max_search=max_search, depth_threshold=thr)
+
+
+def _subs(rng, n):
+    """Sub-texel positions away from 0 (where rx * ry == 0 takes its own branch)."""
+    return rng.uniform(0.02, 0.98, size=(n, 2))
+
+
+def test_filtered_rbsm_staircase_coverage_closed_form(oracle_lib):
+    """45-degree staircase (umbra where i + j <= 31), receiver 0.5, tap in
+    texel (i, j) at sub-texel (fx, fy), traced through NonConservativeSMSR's
+    walks (FilteredRBSM.frag shares them):
+    * first lit diagonal (i + j = 32): r = .5, g = .25, b = 0; the left and
+      down walks step into umbra and find an end at once (+fx, +fy), the right
+      and up walks lose the discontinuity after one step (-(1-fx), -(1-fy));
+      T = 1 on both axes, so rel = (fx, fy);
+    * last umbra diagonal (i + j = 31): r = .25, g = .5, b = 1; walks
+      (-fx, 1-fx, -fy, 1-fy), so rel = (1-fx, 1-fy).
+    FilteredRBSM.frag:266-272 (same-sign branch, clamp(1-s + (2s-1)(rx+ry)))
+    then gives min(fx+fy, 1) on the lit diagonal and max(fx+fy-1, 0) on the
+    umbra one: both are clamp(x + y - 32, 0, 1) in texel units, the coverage
+    of a unit-wide linear ramp centred on x + y = 32.5.  The binary shader
+    thresholds the same sum at 1/2: lit iff x + y >= 32.5.  Off the two
+    diagonals there is no discontinuity and the tap keeps its hard value."""
+    d = staircase()
+    p = _tap_params(oracle_lib, d)
+    rng = np.random.default_rng(21)
+    for fx, fy in _subs(rng, 40):
+        for i, j in ((14, 18), (20, 12), (9, 23)):       # i + j = 32, lit
+            for di, want_b in ((0, 0.0), (-1, 1.0)):     # and the umbra texel left of it
+                x = np.float32((i + di + fx) / RW)
+                y = np.float32((j + fy) / RW)
+                xs, ys = float(x) * RW, float(y) * RW     # the tap's texel coordinates in FP32
+                for filtered in (True, False):
+                    t = oracle_lib.rbsm_tap(d, p, x, y, 0.5, filtered=filtered)
+                    ufx, ufy = xs - math.floor(xs), ys - math.floor(ys)
+                    assert t["b"] == want_b
+                    if want_b == 0.0:
+                        assert (t["r"], t["g"]) == (0.5, 0.25)
+                        np.testing.assert_allclose([t["left"], t["right"], t["down"], t["up"]],
+                                                   [ufx, -(1 - ufx), ufy, -(1 - ufy)], atol=2e-6)
+                        np.testing.assert_allclose([t["rx"], t["ry"]], [ufx, ufy], atol=2e-6)
+                    else:
+                        assert (t["r"], t["g"]) == (0.25, 0.5)
+                        np.testing.assert_allclose([t["left"], t["right"], t["down"], t["up"]],
+                                                   [-ufx, 1 - ufx, -ufy, 1 - ufy], atol=2e-6)
+                        np.testing.assert_allclose([t["rx"], t["ry"]], [1 - ufx, 1 - ufy], atol=2e-6)
+                    ramp = min(max(xs + ys - 32.0, 0.0), 1.0)
+                    if filtered:
+                        assert abs(t["value"] - ramp) <= 4e-6, (i + di, j, fx, fy, t)
+                    elif abs(xs + ys - 32.5) > 1e-5:
+                        assert t["value"] == (1.0 if xs + ys >= 32.5 else 0.0)
+    # two texels away from the diagonals: no discontinuity, hard values
+    for i, j, want in ((13, 17, 0.0), (15, 18, 1.0)):
+        t = oracle_lib.rbsm_tap(d, p, (i + 0.4) / RW, (j + 0.6) / RW, 0.5, filtered=True)
+        assert (t["r"], t["g"], t["value"]) == (0.0, 0.0, want)
+
+
+def _notch():
+    """Umbra rows 0..20 except a lit notch at (10..12, 20); lit above."""
+    d = np.ones((RW, RW), np.float32)
+    d[:21, :] = 0.0
+    d[20, 10:13] = 1.0
+    return d
+
+
+def test_filtered_rbsm_notch_takes_the_both_ends_branch(oracle_lib):
+    """Middle texel (11, 20) of a three-texel lit notch in an umbra edge, traced
+    by hand: umbra only below it, so r = 0, g = .25, b = 0.  The left walk
+    crosses lit (10, 20), whose lower neighbour is umbra, and stops in umbra
+    texel 9 with an end found: +(1 + fx); the right walk likewise
+    +(1 + (1 - fx)); the down walk finds umbra at once (+fy); the up walk has
+    no discontinuity to follow (-(1 - fy)).  Both x walks positive: T = -2 and
+    edgeLength = 3, so FilteredRBSM.frag:234-243 gives
+    rel_x = -2 |max(-2(1+fx), -2(2-fx))| / 3 = -4 min(1+fx, 2-fx) / 3, rel_y = fy
+    (T = 1).  The signs differ, so revectorizeShadow returns
+    (1 - s) + (2s - 1) max(rx, ry) = fy for the lit tap (s = 1).  The binary
+    shader scales |max(T dx, T dy)| by max(T, 2s-1) = 1 instead, so
+    rel_x = +2 min(1+fx, 2-fx) / 3: both positive, and the tap stays lit."""
+    d = _notch()
+    p = _tap_params(oracle_lib, d)
+    rng = np.random.default_rng(22)
+    for fx, fy in _subs(rng, 40):
+        x, y = np.float32((11 + fx) / RW), np.float32((20 + fy) / RW)
+        xs, ys = float(x) * RW, float(y) * RW
+        ufx, ufy = xs - 11.0, ys - 20.0
+        ft = oracle_lib.rbsm_tap(d, p, x, y, 0.5, filtered=True)
+        bt = oracle_lib.rbsm_tap(d, p, x, y, 0.5, filtered=False)
+        for t in (ft, bt):
+            assert (t["r"], t["g"], t["b"]) == (0.0, 0.25, 0.0)
+            np.testing.assert_allclose([t["left"], t["right"], t["down"], t["up"]],
+                                       [1 + ufx, 2 - ufx, ufy, -(1 - ufy)], atol=4e-6)
+            assert abs(t["ry"] - ufy) <= 4e-6
+        m = min(1 + ufx, 2 - ufx)
+        assert abs(ft["rx"] + 4.0 * m / 3.0) <= 4e-6
+        assert abs(bt["rx"] - 2.0 * m / 3.0) <= 4e-6
+        assert abs(ft["value"] - ufy) <= 4e-6, (fx, fy, ft)
+        assert bt["value"] == 1.0
+
+
+def _step_map(umbra_right: bool):
+    """Vertical step at column 16 with both sides within a depth threshold
+    0.01 of the receiver 0.5: columns < 16 at 0.504 (lit), >= 16 at 0.495
+    (umbra), or swapped."""
+    d = np.full((RW, RW), 0.504, np.float32)
+    if umbra_right:
+        d[:, 16:] = 0.495
+    else:
+        d[:, :16] = 0.495
+    return d
+
+
+def test_rbsm_depth_threshold_carries_new_depth(oracle_lib):
+    """Umbra tap in column 16 beside lit column 15, depthThreshold 0.01,
+    traced through NonConservativeSMSR.frag:96-232: r = .5, g = 0, b = 1.
+    The down walk steps to (16, 15), still umbra; getDisc(vec4) reads the
+    left neighbour 0.504, within the threshold of 0.5, so it lowers the depth
+    to 0.49, records it in newDepth, and the left test is lit: the edge goes
+    on.  The walk then carries newDepth (:221-222): at (16, 14), 0.49 <= 0.495
+    turns the texel lit, which is the end of the edge: +(1 + fy); the up walk
+    likewise +(1 + (1 - fy)).  The left walk meets lit column 15 at once
+    (+fx); the right walk has no discontinuity to follow (-(1 - fx)).  T = -2
+    on y with edgeLength 3: binary rel_y = max(-2, -1) * 2 min(1+fy, 2-fy) / 3,
+    filtered rel_y = -4 min(1+fy, 2-fy) / 3; rel_x = fx.  Binary: |rx| + |ry|
+    > 1/2 shadows the tap (0); filtered: signs differ, so
+    1 - max(rx, ry) = 1 - fx.
+    Without the carry the walk keeps depth 0.5, never sees the texel turn lit
+    and runs maxSearch steps down to the map border: -(16 + fy)."""
+    d = _step_map(umbra_right=True)
+    p = _tap_params(oracle_lib, d, thr=0.01)
+    rng = np.random.default_rng(23)
+    for fx, fy in _subs(rng, 30):
+        x, y = np.float32((16 + fx) / RW), np.float32((16 + fy) / RW)
+        ufx, ufy = float(x) * RW - 16.0, float(y) * RW - 16.0
+        m = min(1 + ufy, 2 - ufy)
+        for filtered in (False, True):
+            t = oracle_lib.rbsm_tap(d, p, x, y, 0.5, filtered=filtered)
+            assert (t["r"], t["g"], t["b"]) == (0.5, 0.0, 1.0)
+            np.testing.assert_allclose([t["left"], t["right"], t["down"], t["up"]],
+                                       [ufx, -(1 - ufx), 1 + ufy, 2 - ufy], atol=4e-6)
+            assert abs(t["rx"] - ufx) <= 4e-6
+            if filtered:
+                assert abs(t["ry"] + 4.0 * m / 3.0) <= 4e-6
+                assert abs(t["value"] - (1.0 - ufx)) <= 4e-6
+            else:
+                assert abs(t["ry"] + 2.0 * m / 3.0) <= 4e-6
+                assert t["value"] == 0.0
+    # with threshold 0 nothing is nudged: the vertical walks follow the
+    # straight edge, the down walk for maxSearch = 16 texels (-(16 + .5)),
+    # the up walk until the border row 32 (depth 0) ends the discontinuity
+    # after 15 texels (-(15 + .5))
+    t0 = oracle_lib.rbsm_tap(d, _tap_params(oracle_lib, d, thr=0.0), (16.5) / RW, (16.5) / RW, 0.5)
+    assert (t0["down"], t0["up"]) == (-16.5, -15.5)
+
+
+def test_rbsm_depth_threshold_on_the_lit_side(oracle_lib):
+    """Lit tap in column 16 beside umbra column 15, threshold 0.01: every
+    texel a walk visits lies within the threshold of 0.5, so
+    computeDiscontinuityLength lowers the walk's depth to 0.49 at its first
+    step (:196-199) and tests that texel against 0.49.  The left walk's
+    first texel (0.495) then reads lit, not umbra, and the walk stops without
+    an end (-fx) where it would have found one (+fx) with threshold 0; the
+    other three walks stop at once too: (-fx, -(1-fx), -fy, -(1-fy)).  Both
+    axes T = 0, so rel = (0, 0) and the tap keeps its lit value."""
+    d = _step_map(umbra_right=False)
+    rng = np.random.default_rng(24)
+    for fx, fy in _subs(rng, 20):
+        x, y = np.float32((16 + fx) / RW), np.float32((16 + fy) / RW)
+        ufx, ufy = float(x) * RW - 16.0, float(y) * RW - 16.0
+        t = oracle_lib.rbsm_tap(d, _tap_params(oracle_lib, d, thr=0.01), x, y, 0.5)
+        assert (t["r"], t["g"], t["b"]) == (0.5, 0.0, 0.0)
+        np.testing.assert_allclose([t["left"], t["right"], t["down"], t["up"]],
+                                   [-ufx, -(1 - ufx), -ufy, -(1 - ufy)], atol=4e-6)
+        assert (t["rx"], t["ry"], t["value"]) == (0.0, 0.0, 1.0)
+        t0 = oracle_lib.rbsm_tap(d, _tap_params(oracle_lib, d, thr=0.0), x, y, 0.5)
+        assert abs(t0["left"] - ufx) <= 4e-6
+
+
+def _rpcf_taps(x, y, n):
+    incr = np.float32(1.0) / np.float32(n)
+    step = np.float32(2.0) * np.float32(1.0) / np.float32(3.0)
+    ws = []
+    w = np.float32(-1.0)
+    while w <= np.float32(1.0):
+        ws.append(w)
+        w = np.float32(w + step)
+    return [(np.float32(x + a * incr), np.float32(y + b * incr)) for a in ws for b in ws]
+
+
+@pytest.mark.parametrize("case", ["staircase", "notch", "threshold"])
+def test_filtered_rpcf_is_the_mean_of_its_taps(oracle_lib, case):
+    """FilteredRBSM.frag:306-349: the pass is the mean over the 4 x 4 FP32 tap
+    coordinates of mix(revectorized tap, 1, si) (or the mixed hard value
+    where a tap has no discontinuity); with the tap values pinned above this
+    fixes the pass on the same maps."""
+    d = {"staircase": staircase, "notch": _notch, "threshold": lambda: _step_map(True)}[case]()
+    thr = 0.01 if case == "threshold" else 0.0
+    centre = {"staircase": (15, 17), "notch": (11, 20), "threshold": (16, 16)}[case]
+    rng = np.random.default_rng(25)
+    si = np.float32(0.25)
+    p = _tap_params(oracle_lib, d, thr=thr)
+    pts = []
+    for fx, fy in _subs(rng, 25):
+        pts.append((np.float32((centre[0] + fx) / RW), np.float32((centre[1] + fy) / RW), np.float32(0.5)))
+    got = run_hard(oracle_lib, d, pts, technique="rbsm_filtered", si=float(si), kernel_order=3, penumbra_size=1,
+                   max_search=16, depth_threshold=thr)
+    for (x, y, z), g in zip(pts, got):
+        acc = np.float32(0.0)
+        for tx, ty in _rpcf_taps(x, y, RW):
+            v = np.float32(oracle_lib.rbsm_tap(d, p, tx, ty, z, filtered=True)["value"])
+            acc = np.float32(acc + np.float32(v * (np.float32(1.0) - si) + si))
+        assert abs(g - acc / np.float32(16.0)) <= 1e-6
