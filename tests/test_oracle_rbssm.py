@@ -203,3 +203,114 @@ def test_rbssm_background_and_rejects(oracle_lib):
         oracle_lib.rbssm(pos, None, d, p, kernel_size=1)
     with pytest.raises(RuntimeError):
         oracle_lib.rbssm(pos, None, d, p, blocker_search_size=1)
+
+
+# --------------------------------------------------------------------------
+# smoothONDS (RBSSM.frag:311-1117) where an edge ends, turns a corner or is
+# one texel wide: fills traced by hand through the shader's branches
+# --------------------------------------------------------------------------
+def _fill(oracle_lib, d, p, x, y):
+    """(tap, FP32 sub-texel coordinates) of an RBSSM tap at receiver 0.999."""
+    n = d.shape[0]
+    xf, yf = np.float32(x / n), np.float32(y / n)
+    t = oracle_lib.rbssm_tap(d, p, xf, yf, 0.999)
+    xs, ys = float(xf) * n, float(yf) * n
+    return t, xs - np.floor(xs), ys - np.floor(ys)
+
+
+def _cl(v):
+    return min(max(v, 0.0), 1.0)
+
+
+def test_smoothonds_revectorizes_an_edge_between_its_ends(oracle_lib):
+    """The lower part's top edge runs along row 39 from a convex corner at
+    column 20 (column 19 is lit) to a step at column 30 (the taller part).
+    Umbra tap (c, 39), c = 21..29 (r = 0, g = .5, b = 1): the left walk
+    reaches lit column 19 and finds the end, +(c - 19); the right walk loses
+    the lit texel above at column 30, -(30 - c); both vertical walks return
+    -1 at once (r = 0, g != 0).  normalizeDiscontinuitySpace gives
+    len = 10 and nx = 1 - (c - 19)/10 + (1 - ux)/10; the exiting branch for a
+    bottom discontinuity returns clamp(nx - (1 - uy)).  Lit tap (c, 40)
+    (r = 0, g = .25, b = 0): walks -(c - 19) and +(30 - c) (the right walk
+    enters umbra at the step), nx = 1 - (30 - c)/10 + ux/10, and the
+    entering branch for a top discontinuity returns clamp(uy - (nx - 1))
+    (column 29 of row 40 also borders the taller part on its right, a corner
+    case of its own).
+    Both are clamp(Y - 37 - X/10) in texel coordinates: the staircase end
+    (X, Y) = (20, 39) .. (30, 40) is filled along one straight line."""
+    n = 64
+    d = _rect_map(n)
+    p = _params(oracle_lib, d)
+    rng = np.random.default_rng(31)
+    for c in range(21, 30):
+        for row in ((39, 40) if c < 29 else (39,)):
+            for ux, uy in rng.uniform(0.02, 0.98, size=(6, 2)):
+                t, fx, fy = _fill(oracle_lib, d, p, c + ux, row + uy)
+                X, Y = c + fx, row + fy
+                if row == 39:
+                    assert (t["r"], t["g"], t["b"]) == (0.0, 0.5, 1.0)
+                    assert (t["left"], t["right"]) == (c - 19, -(30 - c))
+                else:
+                    assert (t["r"], t["g"], t["b"]) == (0.0, 0.25, 0.0)
+                    assert (t["left"], t["right"]) == (-(c - 19), 30 - c)
+                assert (t["down"], t["up"]) == (-1.0, -1.0)
+                assert abs(t["fill"] - _cl(Y - 37.0 - X / 10.0)) <= 2e-6, (c, row, ux, uy, t)
+
+
+def test_smoothonds_convex_corner(oracle_lib):
+    """Corner texel (20, 39) of the lower part: umbra with lit neighbours left
+    and above, so r = g = .5, b = 1.  Walks: left and up return 1 at once
+    (the neighbour in that direction is the discontinuity), right runs to the
+    step (-10), down follows the lit column 19 for maxSearch steps (-17).
+    normalize: nx = 1 - ux/10 (len 10), ny = 1 - (1 - uy)/16 (len 16).  The
+    corner branch moves the tap two columns left (the shader adds the x step
+    twice) and one row down before its dominant-axis tests; both find an
+    exiting discontinuity, and the left-bottom case returns
+    max(clamp(nx - (1 - uy)), clamp(ny - ux))."""
+    n = 64
+    d = _rect_map(n)
+    p = _params(oracle_lib, d)
+    rng = np.random.default_rng(32)
+    for ux, uy in rng.uniform(0.02, 0.98, size=(40, 2)):
+        t, fx, fy = _fill(oracle_lib, d, p, 20 + ux, 39 + uy)
+        assert (t["r"], t["g"], t["b"]) == (0.5, 0.5, 1.0)
+        assert (t["left"], t["right"], t["down"], t["up"]) == (1.0, -10.0, -17.0, 1.0)
+        nx, ny = 1.0 - fx / 10.0, 1.0 - (1.0 - fy) / 16.0
+        assert abs(t["nx"] - nx) <= 1e-6 and abs(t["ny"] - ny) <= 1e-6
+        assert abs(t["fill"] - max(_cl(nx - (1.0 - fy)), _cl(ny - fx))) <= 2e-6, (ux, uy, t)
+
+
+def _line_map(n=64, row=30, c0=10, c1=60):
+    """A one-texel umbra line (0.995) along `row`, columns c0..c1, in a lit map."""
+    d = np.ones((n, n), np.float32)
+    d[row, c0:c1 + 1] = 0.995
+    return d
+
+
+def test_smoothonds_one_texel_line(oracle_lib):
+    """Umbra tap on a one-texel line: lit above and below, so r = 0,
+    g = .75, b = 1 (exiting top and bottom only).  The neighbours above and
+    below have no discontinuity of their own along x, nor do those of the
+    next column, so the branch (without the accurate shader's scan) falls to
+    its last case: mix(1 - uy, clamp(nx - uy), step(-2, nx)).
+    * Column 20 of a line over columns 10..60: the left walk finds the end at
+      column 9, +11; the right walk runs maxSearch steps, -17; len 16 and
+      nx = (6 - ux)/16, so the fill is clamp((6 - ux)/16 - uy).
+    * Column 25 of a line over columns 10..40: both ends 16 texels away, both
+      walks +16, nx compressed below -2, so the fill is 1 - uy."""
+    rng = np.random.default_rng(33)
+    d = _line_map()
+    p = _params(oracle_lib, d)
+    for ux, uy in rng.uniform(0.02, 0.98, size=(30, 2)):
+        t, fx, fy = _fill(oracle_lib, d, p, 20 + ux, 30 + uy)
+        assert (t["r"], t["g"], t["b"]) == (0.0, 0.75, 1.0)
+        assert (t["left"], t["right"], t["down"], t["up"]) == (11.0, -17.0, -1.0, -1.0)
+        assert abs(t["nx"] - (6.0 - fx) / 16.0) <= 1e-6
+        assert abs(t["fill"] - _cl((6.0 - fx) / 16.0 - fy)) <= 2e-6, (ux, uy, t)
+    d = _line_map(c1=40)
+    p = _params(oracle_lib, d)
+    for ux, uy in rng.uniform(0.02, 0.98, size=(30, 2)):
+        t, fx, fy = _fill(oracle_lib, d, p, 25 + ux, 30 + uy)
+        assert (t["left"], t["right"]) == (16.0, 16.0)
+        assert t["nx"] <= -2.0
+        assert abs(t["fill"] - (1.0 - fy)) <= 2e-6, (ux, uy, t)
