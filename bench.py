@@ -51,6 +51,7 @@ def _parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-frames timing")
     ap.add_argument("--batch-steps", type=int, default=5, help="timed steps of the batched frames")
     ap.add_argument("--no-passes", action="store_true", help="skip the other screen passes' timing")
+    ap.add_argument("--no-rbsm", action="store_true", help="skip the RBSM workload's timing")
     ap.add_argument("--passes-steps", type=int, default=5, help="timed steps of each other screen pass")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
@@ -99,6 +100,84 @@ def tap_offsets(kernel_order, penumbra, inclusive):
     return out
 
 
+def hard_footprint(torch, g, biased, smw, smh, taps, neighbours=False):
+    """Unique shadow-map texels a hard-shadow pass reads at its taps
+    (per-axis offsets `taps` in texels, as the shader's loop forms them) and,
+    for the RBSM family, at the four neighbours of each tap that
+    computeDiscontinuity reads.  The edge walks beyond those are not counted,
+    so for RBSM the figure is a lower bound of the pass's bytes."""
+    dev = g.position.device
+    pos = g.position.reshape(3, -1)
+    fg = pos[0] != 0
+    x, y, z = pos[0][fg], pos[1][fg], pos[2][fg]
+    m = torch.tensor(list(biased), dtype=torch.float32, device=dev)
+    c = [((m[0 * 4 + i] * x + m[1 * 4 + i] * y) + m[2 * 4 + i] * z) + m[3 * 4 + i] for i in range(4)]
+    sx, sy = c[0] / c[3], c[1] / c[3]
+    mark = torch.zeros(smh * smw, dtype=torch.bool, device=dev)
+    nb = [(0, 0)] + ([(1, 0), (-1, 0), (0, 1), (0, -1)] if neighbours else [])
+    for a in taps:
+        u = torch.floor((sx + float(a) / smw) * smw)
+        for b in taps:
+            v = torch.floor((sy + float(b) / smh) * smh)
+            for du, dv in nb:
+                uu, vv = u + du, v + dv
+                ok = (uu >= 0) & (uu < smw) & (vv >= 0) & (vv < smh)
+                mark[(vv[ok] * smw + uu[ok]).long()] = True
+    return int(mark.sum())
+
+
+def config2_timing(gi, torch, steps):
+    """RBSM hard shadows on the RBSM workload at its stated size: a
+    200,000-triangle soup (seed 1) over a 40x40 quad, directional light with a
+    fitted orthographic frustum, 2048^2 FP32 map, 1920x1080 view, maxSearch
+    16, depthThreshold 2.5e-6, pre-evaluation on (NonConservativeSMSR.frag
+    and FilteredRBSM.frag call it).  Binary RBSM, binary RPCF and fractional
+    RBSM (FilteredRBSM's 4 x 4 loop), each timed with HIP events on the
+    current stream over `steps` launches; inputs resident.  Algorithmic bytes
+    per launch: G-buffer position + normal (24 B) and the R32F output (4 B)
+    per foreground pixel, 8 B per background pixel (position x read, 0
+    written), and 4 B per unique texel of hard_footprint."""
+    import numpy as np
+
+    from globalillumination_amd import passes
+
+    wl = gi.scenes.config2()
+    W, H = wl.camera.width, wl.camera.height
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, passes.view_matrices(wl.camera)[2], W, H)
+    mvps = np.stack([passes.view_matrices(v)[2] for v in wl.lights])
+    depth = passes.render_shadow_map(dm, mvps, 2048, 2048)
+    MV, nm, lpos = passes.phong_uniforms(wl.camera, wl.lights[0].eye)
+    out = torch.empty((H, W), dtype=torch.float32, device=depth.device)
+    stream = torch.cuda.current_stream()
+    nfg = int((g.position[0] != 0).sum())
+    gbytes = nfg * (24 + 4) + (W * H - nfg) * 8
+    biased = passes.biased(mvps[0], wl.zbias)
+    res = {"workload": f"RBSM hard shadows: {wl.mesh.ntris:,}-triangle soup + 40x40 ground, 2048^2 map, "
+                       f"{W}x{H} view, ortho directional light, maxSearch 16", "steps": steps,
+           "foreground_pixels": nfg}
+    for tech, taps in (("rbsm", [0.0]), ("rpcf", tap_offsets(3, 1.0, True)), ("rbsm_filtered", tap_offsets(3, 1.0, True))):
+        sp = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, technique=tech,
+                             zbias=wl.zbias, preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos,
+                             maxSearch=16, depthThreshold=2.5e-6, kernelOrder=3, penumbraSize=1)
+        passes.compute_hard_shadows(g, depth[0], sp, out=out)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(steps):
+            passes.compute_hard_shadows(g, depth[0], sp, out=out)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        texels = hard_footprint(torch, g, biased, 2048, 2048, taps, neighbours=True)
+        nbytes = gbytes + texels * 4
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[tech] = {"ms": round(ms, 4), "Mpix_per_s": round(W * H / (ms * 1e-3) / 1e6, 1),
+                     "bytes": int(nbytes), "unique_texels": texels, "GB/s": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return res
+
+
 def cpu_baseline(wl, mvps, g_pos, g_nrm, g_fac, phong, smw, samples, gi):
     """The oracle (FP32 C port of the reference passes, OpenMP) on a bounded
     sample of the frame: `samples` light samples, full 1080p view, plus the
@@ -107,7 +186,8 @@ def cpu_baseline(wl, mvps, g_pos, g_nrm, g_fac, phong, smw, samples, gi):
 
     import oracle
 
-    cores = min(os.cpu_count() or 1, 16)
+    # every core this process may run on (the affinity mask), as OpenMP threads
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     oracle.set_threads(cores)
     sub = mvps[:samples]
     t0 = time.perf_counter()
@@ -125,7 +205,8 @@ def cpu_baseline(wl, mvps, g_pos, g_nrm, g_fac, phong, smw, samples, gi):
     t_phong = time.perf_counter() - t0
     per_frame = (t_depth + t_screen) * (len(mvps) / samples) + t_phong
     npx = g_pos.shape[1] * g_pos.shape[2]
-    return {"value": npx / per_frame / 1e6, "unit": "Mpix/s", "cores": cores, "kind": "port",
+    return {"value": npx / per_frame / 1e6, "unit": "Mpix/s", "cores": cores, "threads": oracle.threads(),
+            "nproc": os.cpu_count(), "kind": "port",
             "sample": f"{samples} of {len(mvps)} light samples (depth maps + screen pass) on the full "
                       f"{g_pos.shape[2]}x{g_pos.shape[1]} view, scaled x{len(mvps) // samples}, plus one "
                       f"Phong pass",
@@ -169,10 +250,27 @@ def diff_timing(gi, torch, steps, warmup):
     torch.cuda.synchronize()
     fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     fb = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    # Algorithmic bytes (lower bounds: the texels the taps read are not
+    # counted).  Forward, per texel of the S maps: the 64-bit visibility key
+    # written and read back, the FP64 depth written (24 B); per (map,
+    # triangle): the vertices read twice (72 B), the 80-byte plane record
+    # written and read back and its size flag (161 B); per pixel: position in,
+    # output out (16 B).  Backward repeats the key and plane work and adds the
+    # FP32 texel adjoint cleared, scatter-added and read (12 B per texel), the
+    # key read by the texel adjoint (8 B) and the pixel adjoint's output
+    # gradient read (4 B per pixel).
+    S, msz, nt, npx = len(light.offsets), light.map_size ** 2, wl.mesh.ntris, g.width * g.height
+    fwd_bytes = S * msz * 24 + S * nt * (72 + 161) + npx * 16
+    bwd_bytes = S * msz * (16 + 12 + 8) + S * nt * (72 + 161) + npx * 20
     return {"workload": "differentiable soft shadows: 16 samples, 1024^2 maps, 512x512 view, "
                         f"{wl.mesh.ntris:,}-triangle displaced icosphere + ground",
             "fwd_ms": round(fwd, 4), "fwd_bwd_ms": round(fb, 4), "fwd_bwd_over_fwd": round(fb / fwd, 3),
-            "steps": steps}
+            "steps": steps,
+            "roofline": {"fwd_bytes": int(fwd_bytes), "fwd_GB/s": round(fwd_bytes / (fwd * 1e-3) / 1e9, 1),
+                         "fwd_frac": round(fwd_bytes / (fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "fwd_bwd_bytes": int(fwd_bytes + bwd_bytes),
+                         "fwd_bwd_GB/s": round((fwd_bytes + bwd_bytes) / (fb * 1e-3) / 1e9, 1),
+                         "fwd_bwd_frac": round((fwd_bytes + bwd_bytes) / (fb * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def batch_timing(gi, torch, steps):
@@ -220,6 +318,12 @@ def batch_timing(gi, torch, steps):
     graph.close()
     b.close()
     npx = sum(wl.camera.width * wl.camera.height for wl in wls)
+    # Algorithmic bytes of one batch (a lower bound: the texels the soft pass
+    # gathers are not counted): every map written once (4 B per texel), the
+    # triangle vertices read per map (36 B), position in and shadow out per
+    # pixel (16 B)
+    nbytes = sum(len(wl.lights) * (wl.lights[0].width ** 2 * 4 + wl.mesh.ntris * 36) +
+                 wl.camera.width * wl.camera.height * 16 for wl in wls)
     return {"workload": "batched frames: 32 instances, maps 512^2-4096^2, views 640x480-1920x1080, "
                         "5k-300k-triangle soups, 1-16 light samples (seed 4)",
             "instances": len(wls), "view_pixels": int(npx), "light_samples": int(sum(len(w.lights) for w in wls)),
@@ -227,7 +331,9 @@ def batch_timing(gi, torch, steps):
             "unbatched_ms": round(t_unb, 3), "batched_ms": round(t_run, 3), "graph_ms": round(t_graph, 3),
             "speedup_graph_vs_unbatched": round(t_unb / t_graph, 3),
             "graph_Mpix_per_s": round(npx / (t_graph * 1e-3) / 1e6, 1),
-            "tuning": tuning, "autotune_ms": round(tuned_ms, 3), "steps": steps}
+            "tuning": tuning, "autotune_ms": round(tuned_ms, 3), "steps": steps,
+            "roofline": {"bytes": int(nbytes), "GB/s": round(nbytes / (t_graph * 1e-3) / 1e9, 1),
+                         "frac": round(nbytes / (t_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def passes_timing(gi, torch, steps):
@@ -268,25 +374,55 @@ def passes_timing(gi, torch, steps):
 
     res = {"workload": "far-light frame: 100,822-triangle displaced icosphere + ground, 1920x1080 view, "
                        "2048^2 maps (light 40x further, 5-degree frustum)", "steps": steps}
+    # Algorithmic bytes per launch (`roofline` below): per foreground pixel the
+    # G-buffer planes read and the output written, 8 B per background pixel,
+    # and 4 B per unique texel (hard_footprint) of the taps every pixel
+    # reads.  RBSM walks, PCSS/RBSSM's PCF taps (their extent depends on the
+    # blocker search's result) and the revectorized passes' texels are not
+    # counted: those figures are lower bounds.
+    nfg = int((g.position[0] != 0).sum())
+    roof = {}
+
+    def note(key, ms, nbytes):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        roof[key] = {"bytes": int(nbytes), "GB/s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    b0 = passes.biased(mvps[0])
     d0 = maps[0]
+    tapsets = {"naive": ([0.0], False), "pcf": (tap_offsets(3, 1.0, False), False), "rbsm": ([0.0], True),
+               "rpcf": (tap_offsets(3, 1.0, True), True), "rbsm_filtered": (tap_offsets(3, 1.0, True), True),
+               "rbsm_conservative": ([0.0], True), "rpcf_conservative": (tap_offsets(3, 1.0, True), True)}
     for tech in ("naive", "pcf", "rbsm", "rpcf", "rbsm_filtered", "rbsm_conservative", "rpcf_conservative"):
         sp = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, technique=tech,
                              preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos,
                              depthThreshold=2.5e-5, maxSearch=16)
         res[f"hard_{tech}_ms"] = timed(lambda: passes.compute_hard_shadows(g, d0, sp, out=out))
+        taps, nb = tapsets[tech]
+        note(f"hard_{tech}", res[f"hard_{tech}_ms"],
+             nfg * 28 + (W * H - nfg) * 8 + 4 * hard_footprint(torch, g, b0, 2048, 2048, taps, nb))
+    # the 7 x 7 blocker search of PCSS and RBSSM: offsets w * (radius / 1024) / 3
+    # in texture coordinates (2048-texel map), i.e. w * 2048 * (64/1024) / 3 texels
+    search = [w * 2048.0 * (64.0 / 1024.0) / 3.0 for w in range(-3, 4)]
+    search_texels = hard_footprint(torch, g, b0, 2048, 2048, search, False)
     pc = gi.PCSSParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, lightSourceRadius=64,
                        preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos)
     res["pcss_ms"] = timed(lambda: passes.render_soft_shadows(g, d0, pc, out=out))
+    note("pcss", res["pcss_ms"], nfg * 28 + (W * H - nfg) * 8 + 4 * search_texels)
     rb = gi.RBSSMParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, lightSourceRadius=64,
                         preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos, maxSearch=16,
                         depthThreshold=2.5e-5)
     res["rbssm_ms"] = timed(lambda: passes.render_rbssm(g, d0, rb, out=out))
+    note("rbssm", res["rbssm_ms"], nfg * 28 + (W * H - nfg) * 8 + 4 * search_texels)
     samp = maps[1:].contiguous()
     biased = np.stack([passes.biased(m) for m in mvps[1:]])
     disc = passes.rev_discontinuity_layers(g)
     eyes = [tuple(float(x) for x in v.eye) for v in wl.lights[1:]]
     res["rev_discontinuity_ms"] = timed(
         lambda: passes.rev_discontinuity(g, samp, 0, biased[0], MV, nm, eyes[0], 0.25, disc))
+    # position + normal in, one discontinuity byte out per pixel; the tap and
+    # its four neighbours
+    note("rev_discontinuity", res["rev_discontinuity_ms"],
+         nfg * 25 + (W * H - nfg) * 5 + 4 * hard_footprint(torch, g, biased[0], 2048, 2048, [0.0], True))
     for i in range(1, 9):
         passes.rev_discontinuity(g, samp, i, biased[i], MV, nm, eyes[i], 0.25, disc)
     vis = torch.zeros((H, W), dtype=torch.float32, device=maps.device)
@@ -306,13 +442,20 @@ def passes_timing(gi, torch, steps):
                   passes._dptr(disc), int(disc.shape[0]), 0, 1, passes._dptr(img), passes._dptr(vis),
                   passes._dptr(count_dev), passes._stream_handle(None))
     res["rev_node_test_ms"] = timed(node_test)
+    # position in, four corner taps, four discontinuity bytes, image and
+    # classification out (texels not counted)
+    note("rev_node_test", res["rev_node_test_ms"], nfg * (12 + 4 + 4 + 4) + (W * H - nfg) * 12)
     common, trans = passes.adaptive_trans(mvps[1:], [1, 2, 1, 2, 1, 2, 1, 2, 1])
     pen = float(((vis == 0.5) & (g.position[0] != 0)).float().mean())
     res["rev_penumbra_pixel_fraction"] = round(pen, 4)
     for rp in (False, True):
         prm = gi.RevSoftParams(MV, nm, lpos, RPCF=rp, maxSearch=16, depthThreshold=2.5e-5)
-        res[f"rev_soft{'_rpcf' if rp else ''}_ms"] = timed(
+        key = f"rev_soft{'_rpcf' if rp else ''}"
+        res[f"{key}_ms"] = timed(
             lambda: passes.render_rev_adaptive(g, samp, common, trans, disc, vis, prm, out=out))
+        # position + normal + classification in, output out (texels not counted)
+        note(key, res[f"{key}_ms"], nfg * (24 + 4 + 4) + (W * H - nfg) * 8)
+    res["roofline"] = roof
     return res
 
 
@@ -394,15 +537,22 @@ def main():
     phong_ms = float(np.mean([e[2].elapsed_time(e[5]) for e in evs]))
     value = W * H / (ms_step * 1e-3) / 1e6
 
-    # Algorithmic bytes per launch.  Screen pass (k_soft): G-buffer position
-    # reads (12 B/px) + the R32F output (4 B/px) + every unique depth texel the
-    # taps gather (4 B each).  Tile raster (k_raster): the maps written once
-    # (4 B/texel) + the triangle vertices read per map (3 x 12 B per triangle).
+    # Algorithmic bytes per launch.  Screen pass (k_soft): per foreground
+    # pixel the G-buffer position (12 B) and the R32F output (4 B), per
+    # background pixel the position x that identifies it (4 B) and the 0
+    # written (4 B), plus every unique depth texel the taps gather (4 B
+    # each).  Tile raster (k_raster): the maps written once (4 B/texel) + the
+    # triangle vertices read per map (3 x 12 B per triangle).
     taps = tap_offsets(3, 1.0, False)
     texels = 0 if args.no_footprint else touched_texels(gi, torch, g, mvps, smw, smw, taps)
     npx = W * H
-    screen_bytes = npx * (12 + 4) + texels * 4
+    nfg = int((g.position[0] != 0).sum())
+    screen_bytes = nfg * (12 + 4) + (npx - nfg) * 8 + texels * 4
     raster_bytes = S * smw * smw * 4 + S * wl.mesh.ntris * 36
+    # Phong: position, normal, facing and shadow in (28 B), RGBA out (16 B) per
+    # foreground pixel; position x in and the clear colour out (20 B) per
+    # background pixel
+    phong_bytes = nfg * 48 + (npx - nfg) * 20
     screen_gbs = screen_bytes / (screen_ms * 1e-3) / 1e9
     raster_gbs = raster_bytes / (raster_ms * 1e-3) / 1e9
     if raster_ms >= screen_ms:
@@ -424,8 +574,9 @@ def main():
                                              if measured_traffic("k_soft") else None)},
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),
                            "raster_GB/s": round(raster_gbs, 1), "raster_bytes": int(raster_bytes)},
-            "phong_pass": {"kernel": "k_phong", "ms": round(phong_ms, 4), "bytes": int(npx * 48),
-                           "GB/s": round(npx * 48 / (phong_ms * 1e-3) / 1e9, 1)}}
+            "phong_pass": {"kernel": "k_phong", "ms": round(phong_ms, 4), "bytes": int(phong_bytes),
+                           "GB/s": round(phong_bytes / (phong_ms * 1e-3) / 1e9, 1),
+                           "frac": round(phong_bytes / (phong_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
     # The reference's own soft-shadow PCF loop (AccurateSoftShadow.frag:101-119,
     # `<=`) takes 4 x 4 taps per sample where the workload names 3 x 3: the
     # screen pass in that mode, over the frame's maps, and the frame it gives.
@@ -444,6 +595,7 @@ def main():
                 "frame_Mpix_per_s_events": round(npx / ((depth_ms + screen16_ms + phong_ms) * 1e-3) / 1e6, 1)}
     diff = None if args.no_diff else diff_timing(gi, torch, args.diff_steps, 2)
     other = None if args.no_passes else passes_timing(gi, torch, args.passes_steps)
+    rbsm = None if args.no_rbsm else config2_timing(gi, torch, args.passes_steps)
     batched = None if args.no_batch else batch_timing(gi, torch, args.batch_steps)
     cpu = None
     if not args.no_cpu_baseline:
@@ -459,7 +611,7 @@ def main():
                        "global_batch": 1, "samples": S, "shadow_map": smw, "view": [W, H],
                        "triangles": wl.mesh.ntris},
             "roofline": roof, "cpu_baseline": cpu, "reference_pcf_loop": ref_loop, "differentiable": diff,
-            "batched": batched, "other_passes": other}
+            "batched": batched, "other_passes": other, "rbsm_workload": rbsm}
     print(json.dumps(line), flush=True)
 
 

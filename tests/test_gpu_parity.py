@@ -373,6 +373,52 @@ def test_rbsm_soup(gi, oracle_lib, cuda, technique):
     assert bits_equal(got, want), mismatch_report(got, want, technique)
 
 
+@pytest.fixture(scope="module")
+def config2_frame(gi, oracle_lib, cuda):
+    """The RBSM workload at its stated size: 200,000-triangle soup (seed 1)
+    over a 40x40 ground quad, directional light with a fitted orthographic
+    frustum, 2048^2 map, 1920x1080 view; the depth map is checked against
+    the oracle's raster once for every technique below."""
+    wl = gi.scenes.config2()
+    assert wl.mesh.ntris == 200_002 and (wl.camera.width, wl.camera.height) == (1920, 1080)
+    dm = gi.DeviceMesh.from_mesh(wl.mesh)
+    g = gi.render_gbuffer(dm, camera_mvp(wl), 1920, 1080)
+    mvps = light_mvps(wl)[:1]
+    depth = gi.render_shadow_map(dm, mvps, 2048, 2048)
+    want = oracle_lib.depth_raster(wl.mesh.positions, wl.mesh.indices, mvps[0], 2048, 2048)
+    assert bits_equal(_np(depth[0]), want), mismatch_report(_np(depth[0]), want, "config-2 depth map")
+    return wl, g, mvps, depth
+
+
+@pytest.mark.parametrize("technique,thr", [("rbsm", 2.5e-6), ("rpcf", 2.5e-6), ("rbsm_filtered", 2.5e-6),
+                                           ("rbsm", 5e-4), ("rbsm_filtered", 5e-4)])
+def test_rbsm_config2_full_size(gi, oracle_lib, config2_frame, technique, thr):
+    """Binary, RPCF and fractional RBSM on the full config-2 frame
+    (maxSearch 16, kernelOrder 3, penumbraSize 1) against the oracle over
+    every pixel; depthThreshold 2.5e-6 as the workload sets it, and 5e-4, at
+    which the walks' depth nudges and the newDepth carry take effect."""
+    wl, g, mvps, depth = config2_frame
+    print(f"config 2: {technique}, depthThreshold {thr:g}", flush=True)
+    params = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, technique=technique,
+                             zbias=wl.zbias, maxSearch=16, depthThreshold=thr, kernelOrder=3, penumbraSize=1)
+    got = _np(gi.compute_hard_shadows(g, depth[0], params))
+    op = oracle_lib.make_params(gi.biased(mvps[0], wl.zbias), 2048, 2048, technique, kernel_order=3,
+                                penumbra_size=1, max_search=16, depth_threshold=thr, shadow_intensity=0.25)
+    want = oracle_lib.hard_shadow(_np(g.position), _np(g.normal), _np(depth[0]), op)
+    fg = _np(g.position)[0] != 0
+    assert fg.mean() > 0.5
+    assert np.abs(got - want).max() <= TOL, mismatch_report(got, want, technique)
+    assert bits_equal(got, want), mismatch_report(got, want, technique)
+    # the frame exercises the revectorized branches: shadowed, lit and, for
+    # the filtering techniques, fractional pixels are all present
+    v = want[fg]
+    if technique == "rbsm":
+        assert (v == 0.25).mean() > 0.05 and (v == 1.0).mean() > 0.05
+    else:  # the soup's shadow is fragmented at texel scale: most 4x4 averages are fractional
+        assert (v == 0.25).mean() > 0.05 and (v == 1.0).mean() > 0.005
+        assert ((v > 0.25) & (v < 1.0)).mean() > 0.1
+
+
 def _soft_case(gi, oracle_lib, wl, sm, params):
     dm = gi.DeviceMesh.from_mesh(wl.mesh)
     cam_mvp = camera_mvp(wl)

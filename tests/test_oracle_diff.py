@@ -146,3 +146,106 @@ def test_bias_matrix_is_glm_bias():
     depth bias)."""
     p = np.array([-1.0, 1.0, 0.0, 1.0])
     assert np.allclose(BIAS @ p, [0.0, 1.0, 0.5, 1.0])
+
+
+# --------------------------------------------------------------------------
+# The oracle's reverse-mode adjoint (or_diff_adjoint) against central finite
+# differences of the model it differentiates: the error must fall as h^2
+# (second-order differences) down to FP64 rounding, for vertex coordinates,
+# the light parameters, and both loss kinds.
+# --------------------------------------------------------------------------
+class _SmallScene:
+    def __init__(self, oracle_lib):
+        from globalillumination_amd import passes
+
+        ico = scenes.displaced_icosphere(12, seed=5)
+        self.mesh = scenes.Mesh.concat(ico, scenes.ground_quad(20.0, 20.0, 0.0))
+        self.light = AreaLight((0.0, 0.0, 0.0), 2.0 * scenes.stratified_jitter(4, seed=5), 256)
+        cam = scenes.View((0.0, 3.0, 6.0), (0.0, 1.0, 0.0), 160, 120, fov=60.0)
+        self.gpos = oracle_lib.gbuffer(self.mesh.positions, self.mesh.normals, self.mesh.indices,
+                                       passes.view_matrices(cam)[2], cam.width, cam.height)[0]
+        self.idx = self.mesh.indices.astype(np.int32)
+        self.pos64 = self.mesh.positions.astype(np.float64)
+        self.L, self.r = np.array([3.0, 7.0, 2.0]), 0.25
+        self.o = oracle_lib
+        M = self.light.matrices(self.L, self.r)
+        self.vis = np.stack([oracle_lib.visibility_raster(self.mesh.positions, self.idx,
+                                                          colmajor(m).astype(np.float32), 256, 256) for m in M])
+
+    def forward(self, L, r, pos64=None):
+        pos64 = self.pos64 if pos64 is None else pos64
+        M = self.light.matrices(np.asarray(L, np.float64), float(r))
+        maps = np.stack([self.o.diff_depth(pos64, self.idx, colmajor(m), v) for m, v in zip(M, self.vis)])
+        return self.o.diff_soft(self.gpos, maps, np.stack([colmajor(BIAS @ m) for m in M]), TAU, BIAS_Z, SI)
+
+    def adjoint(self, grad_out):
+        M = self.light.matrices(self.L, self.r)
+        gp, gm, gb = self.o.diff_adjoint(self.pos64, self.idx, np.stack([colmajor(m) for m in M]),
+                                         np.stack([colmajor(BIAS @ m) for m in M]), self.vis, self.gpos, TAU, BIAS_Z,
+                                         SI, grad_out)
+        Gm = gm.reshape(-1, 4, 4).transpose(0, 2, 1)
+        Gb = gb.reshape(-1, 4, 4).transpose(0, 2, 1)
+        G = Gm + np.einsum("ji,njk->nik", BIAS, Gb)
+        return gp, np.einsum("nij,nijk->k", G, self.light.jacobian(self.L, self.r))
+
+
+@pytest.fixture(scope="module")
+def small_scene(oracle_lib):
+    return _SmallScene(oracle_lib)
+
+
+@pytest.mark.parametrize("loss_kind", ["linear", "l2"])
+def test_diff_adjoint_converges_to_central_differences(small_scene, loss_kind):
+    sc = small_scene
+    S0 = sc.forward(sc.L, sc.r)
+    if loss_kind == "linear":
+        g = np.random.default_rng(13).standard_normal(S0.shape)
+        loss, go = (lambda S: float(np.sum(g * S))), g
+    else:
+        T = sc.forward(sc.L + np.array([0.2, -0.1, 0.15]), 1.3 * sc.r)
+        loss, go = (lambda S: float(np.sum((S - T) ** 2))), 2.0 * (S0 - T)
+    gp, glight = sc.adjoint(go)
+    # coordinates of vertices whose triangles own texels, with a gradient
+    # above FP64 noise (the sigmoid tails give values down to 1e-100)
+    nz = np.argwhere(np.abs(gp) > 1e-6 * np.abs(gp).max())
+    pick = nz[np.random.default_rng(7).choice(len(nz), size=16, replace=False)]
+    th = np.array([*sc.L, sc.r])
+    errs = {}
+    for h in (1e-4, 1e-5, 1e-6):
+        fdv = []
+        for v, a in pick:
+            pp, pm = sc.pos64.copy(), sc.pos64.copy()
+            pp[v, a] += h
+            pm[v, a] -= h
+            fdv.append((loss(sc.forward(sc.L, sc.r, pp)) - loss(sc.forward(sc.L, sc.r, pm))) / (2 * h))
+        fdl = []
+        for k in range(4):
+            tp, tm = th.copy(), th.copy()
+            tp[k] += h
+            tm[k] -= h
+            fdl.append((loss(sc.forward(tp[:3], tp[3])) - loss(sc.forward(tm[:3], tm[3]))) / (2 * h))
+        adj_v = np.array([gp[v, a] for v, a in pick])
+        errs[h] = (np.linalg.norm(adj_v - fdv) / np.linalg.norm(fdv),
+                   np.max(np.abs(glight - np.array(fdl)) / np.abs(fdl)))
+    print({h: tuple(f"{e:.2e}" for e in v) for h, v in errs.items()})
+    for q in range(2):
+        # second order: a tenth of the step leaves about a hundredth of the
+        # error (first order would leave a tenth; at h = 1e-6 the vertex
+        # quotients reach FP64 rounding, ~1e-9)
+        assert errs[1e-5][q] < 0.05 * errs[1e-4][q]
+        assert errs[1e-6][q] < 1e-6
+
+
+def test_diff_adjoint_is_zero_off_the_support(small_scene):
+    """Vertices of triangles that own no texel of any map, and receivers in
+    umbra or lit regions far from every sample's sigmoid, contribute
+    nothing: the vertex gradient is exactly 0 outside the owners' vertices."""
+    sc = small_scene
+    gp, _ = sc.adjoint(np.ones(sc.gpos.shape[1:]))
+    owned = np.zeros(len(sc.idx), bool)
+    for v in sc.vis:
+        k = v[v != np.uint64(0xFFFFFFFFFFFFFFFF)]
+        owned[np.unique((k & np.uint64(0xFFFFFFFF)) >> np.uint64(3)).astype(np.int64)] = True
+    touched = np.zeros(len(sc.pos64), bool)
+    touched[sc.idx[owned].reshape(-1)] = True
+    assert (gp[~touched] == 0).all() and np.abs(gp[touched]).sum() > 0
