@@ -175,6 +175,10 @@ _SIGS = {
                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(GiRasterOpts), ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_size_t, ctypes.c_void_p]),
+    "gi_depth_pass_noalloc_max_bytes": (ctypes.c_size_t, []),
+    "gi_depth_pass_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(GiRasterOpts), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_gbuffer_pass_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 3),
     "gi_gbuffer_pass": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,

@@ -21,7 +21,7 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libgi.so"
 
-HIP_SOURCES = ["raster.hip", "screen.hip", "diff.hip", "batch.hip"]
+HIP_SOURCES = ["raster.hip", "screen.hip", "diff.hip", "batch.hip", "noalloc.hip"]
 CXX_SOURCES = [CSRC / "gi_math.cpp", PKG / "host" / "scene_loader.cpp"]
 
 ARCH = os.environ.get("GI_OFFLOAD_ARCH", "gfx950")

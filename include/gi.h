@@ -111,6 +111,21 @@ int gi_depth_pass(const float* positions_dev, int nverts, const int32_t* indices
                   const gi_raster_opts* opts, float* depth_out_dev,
                   void* workspace_dev, size_t workspace_bytes, gi_stream stream);
 
+/* gi_depth_pass without a workspace argument.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * workspace is a static device array of gi_depth_pass_noalloc_max_bytes()
+ * (64 MiB) that each GPU the library runs on holds until the process ends;
+ * the call allocates nothing.  Returns GI_ENOSPACE when
+ * gi_depth_pass_workspace_size() exceeds it (e.g. 16 maps of 1024^2 with 50k
+ * triangles fit; 64 maps of 2048^2 do not).  Every call on one GPU shares the
+ * array: calls that overlap on a GPU (other streams, other threads) race;
+ * calls ordered on one stream do not.  Other arguments as gi_depth_pass. */
+size_t gi_depth_pass_noalloc_max_bytes(void);
+int gi_depth_pass_noalloc(const float* positions_dev, int nverts, const int32_t* indices_dev, int ntris,
+                          const float* mvps_dev, int nmaps, int width, int height,
+                          const gi_raster_opts* opts, float* depth_out_dev, gi_stream stream);
+
 /* gi_depth_pass in two steps, so that the tile raster can run a range of
  * maps at a time: a frame can then be rastered and shaded in runs of maps
  * (gi_soft_shadow_pass_chunk) with one run's maps in memory, or with a run's
