@@ -113,8 +113,8 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
     dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
     switch (soft_variant(u)) {
-    case 3: hipLaunchKernelGGL(k_soft<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
-    case 4: hipLaunchKernelGGL(k_soft<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    case 3: hipLaunchKernelGGL((k_soft_wide<3, 1, 1>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    case 4: hipLaunchKernelGGL((k_soft_wide<4, 1, 1>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
     default: hipLaunchKernelGGL(k_soft<0>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
     }
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
@@ -152,8 +152,12 @@ int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_de
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
     dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
     switch (soft_variant(u)) {
-    case 3: hipLaunchKernelGGL((k_soft<3, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
-    case 4: hipLaunchKernelGGL((k_soft<4, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
+    case 3:
+        hipLaunchKernelGGL((k_soft_wide<3, 1, 1, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_soft_wide<4, 1, 1, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);
+        break;
     default: hipLaunchKernelGGL((k_soft<0, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
     }
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
