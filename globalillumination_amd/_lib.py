@@ -216,8 +216,9 @@ _SIGS = {
                                                     ctypes.POINTER(GiSoftParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_pcss_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiPcssParams),
                                     ctypes.c_void_p, ctypes.c_void_p]),
+    "gi_rbssm_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 2),
     "gi_rbssm_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.POINTER(GiRbssmParams),
-                                     ctypes.c_void_p, ctypes.c_void_p]),
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]),
     "gi_rev_discontinuity_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_int,
                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
@@ -231,7 +232,9 @@ _SIGS = {
                                                         ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                         ctypes.c_void_p, ctypes.POINTER(GiRevSoftParams),
-                                                        ctypes.c_void_p, ctypes.c_void_p]),
+                                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                                        ctypes.c_void_p]),
+    "gi_rev_soft_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 2),
     "gi_phong_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(GiPhongParams), ctypes.c_void_p, ctypes.c_void_p]),
     "gi_diff_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 6),

@@ -517,68 +517,129 @@ struct Rbssm {
         return gmix(smooth<true>(x, y, z, nx, ny, dr, dg, db, ux, uy), 1.0f, si);
     }
 
-    // RBSSM :1202-1339 over the penumbra kernel
-    __device__ float kernel(const PcssUniforms& q, float si, float pw, float x, float y, float z) const {
-        float illum = 0.0f;
-        const int h0 = loop_start(q.kfw);
-        for (int h = h0; (float)h <= q.kfw; ++h) {
-            const float ty = y + ((float)h * pw) / q.kfw;
-            for (int w = h0; (float)w <= q.kfw; ++w) {
-                const float tx = x + ((float)w * pw) / q.kfw;
-                const float ux = gfract(tx * rb.D.fw), uy = gfract(ty * rb.D.fh);
-                const float d = tex(tx, ty);
-                float dr, dg, db;
-                rb.discontinuity(tx, ty, z, d, dr, dg, db);
-                if (dr > 0.0f || dg > 0.0f) {
-                    const float left = length(dr, dg, db, tx, ty, z, -1.0f, 0.0f);
-                    const float right = length(dr, dg, db, tx, ty, z, 1.0f, 0.0f);
-                    const float down = length(dr, dg, db, tx, ty, z, 0.0f, -1.0f);
-                    const float up = length(dr, dg, db, tx, ty, z, 0.0f, 1.0f);
-                    const float nx = normalize(left, right, ux), ny = normalize(down, up, uy);
-                    float fill = smooth<false>(tx, ty, z, nx, ny, dr, dg, db, ux, uy);
-                    fill = gmix(fill, 1.0f, si);
-                    illum = illum + fill;
-                } else {
-                    illum = illum + ((z <= d) ? 1.0f : si);
-                }
-            }
+    // One tap of RBSSM's kernel loop (:1202-1339) at loop indices (h, w):
+    // revectorized (oriented, normalized, filled by smoothONDS) where the tap
+    // sits on a discontinuity, else the plain depth test, mixed with si.
+    __device__ float tap(const PcssUniforms& q, float si, float pw, float x, float y, float z, int h, int w) const {
+        const float ty = y + ((float)h * pw) / q.kfw;
+        const float tx = x + ((float)w * pw) / q.kfw;
+        const float ux = gfract(tx * rb.D.fw), uy = gfract(ty * rb.D.fh);
+        const float d = tex(tx, ty);
+        float dr, dg, db;
+        rb.discontinuity(tx, ty, z, d, dr, dg, db);
+        if (dr > 0.0f || dg > 0.0f) {
+            const float left = length(dr, dg, db, tx, ty, z, -1.0f, 0.0f);
+            const float right = length(dr, dg, db, tx, ty, z, 1.0f, 0.0f);
+            const float down = length(dr, dg, db, tx, ty, z, 0.0f, -1.0f);
+            const float up = length(dr, dg, db, tx, ty, z, 0.0f, 1.0f);
+            const float nx = normalize(left, right, ux), ny = normalize(down, up, uy);
+            const float fill = smooth<false>(tx, ty, z, nx, ny, dr, dg, db, ux, uy);
+            return gmix(fill, 1.0f, si);
         }
-        return illum / (float)(q.ks * q.ks);
+        return (z <= d) ? 1.0f : si;
     }
 };
 
-// main :1362-1378 with revectorizationBasedShadowMappingSmoothing :1341-1360
-__global__ __launch_bounds__(kBlockThreads) void k_rbssm(GBufferView gb, DepthMap D, HardUniforms u, PcssUniforms q,
-                                                          float* __restrict__ out) {
+// Loop indices of RBSSM's kernel: `for (int h = int(-kfw); float(h) <= kfw;
+// h++)`, the same for w.  Returns the count per axis (kernelSize for odd
+// sizes, one less for even ones) and the first index in h0.
+__device__ __forceinline__ int rbssm_side(const PcssUniforms& q, int& h0) {
+    h0 = loop_start(q.kfw);
+    int n = 0;
+    for (int h = h0; (float)h <= q.kfw; ++h) ++n;
+    return n;
+}
+
+// Workspace of the RBSSM pass: the list length (in a 256-byte header), then
+// one (pixel, penumbra width) pair per view pixel.
+constexpr size_t kListHeader = 256;
+inline size_t rbssm_workspace_bytes(int w, int h) { return kListHeader + (size_t)w * h * 8u; }
+
+// Clears a list's length before the classifying kernel appends to it (a
+// kernel rather than a memset, so the pass can be captured in a graph).
+__global__ void k_list_reset(unsigned* count) {
+    if (threadIdx.x == 0) *count = 0u;
+}
+
+// RBSSM main :1362-1378 up to the kernel loop: pre-evaluation, blocker
+// search, penumbra width.  Pixels that skip the loop get their value here;
+// the others go to the list with their penumbra width for k_rbssm_taps.
+__global__ __launch_bounds__(kBlockThreads) void k_rbssm_classify(GBufferView gb, DepthMap D, HardUniforms u,
+                                                                   PcssUniforms q, float* __restrict__ out,
+                                                                   unsigned* __restrict__ counters,
+                                                                   uint2* __restrict__ list) {
     int x, y;
     pixel_of_block(gb.w, gb.h, x, y);
-    if (x >= gb.w || y >= gb.h) return;
-    const size_t plane = (size_t)gb.w * gb.h;
-    const size_t k = (size_t)y * gb.w + x;
-    const float vx = gb.pos[k];
-    if (vx == 0.0f) {
-        out[k] = 0.0f;
-        return;
-    }
-    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
-    float sc[4];
-    mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
-    const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
-    float shadow = 1.0f;
-    if (u.pre_eval)
-        shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
-    if (sc[3] > 0.0f && shadow == 1.0f) {
-        const float avg = pcss_blocker(D, q, n0, n1, n2);
-        const float pw = pcss_penumbra(q, avg, n2);
-        const float stepSize = 2.0f * pw / (float)q.ks;
-        if (stepSize <= 0.0f || stepSize >= 1.0f) {
+    bool loop = false;
+    size_t k = 0;
+    float pw = 0.0f;
+    if (x < gb.w && y < gb.h) {
+        const size_t plane = (size_t)gb.w * gb.h;
+        k = (size_t)y * gb.w + x;
+        const float vx = gb.pos[k];
+        float shadow = 0.0f;  // background: the cleared target holds 0
+        if (vx != 0.0f) {
+            const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+            float sc[4];
+            mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
+            const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
             shadow = 1.0f;
-        } else {
-            Rbssm r{Rbsm<false>{D, u.stepx, u.stepy, u.thr, u.max_search}, 0.0f, 0};
-            shadow = r.kernel(q, u.si, pw, n0, n1, n2);
+            if (u.pre_eval)
+                shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+            if (sc[3] > 0.0f && shadow == 1.0f) {
+                const float avg = pcss_blocker(D, q, n0, n1, n2);
+                pw = pcss_penumbra(q, avg, n2);
+                const float stepSize = 2.0f * pw / (float)q.ks;
+                loop = !(stepSize <= 0.0f || stepSize >= 1.0f);
+            }
         }
+        if (!loop) out[k] = shadow;
     }
-    out[k] = shadow;
+    const unsigned slot = wave_append(loop, &counters[0]);
+    if (loop) list[slot] = make_uint2((unsigned)k, __float_as_uint(pw));
+}
+
+// RBSSM's kernel loop for the listed pixels, a lane per tap: 64 / taps
+// pixels per wave group when the kernel has at most 64 taps, else one pixel
+// in passes of 64 taps.  The taps are summed in the loop's order (h, then w)
+// and divided by kernelSize^2, as the shader does (:1337).
+__global__ __launch_bounds__(kBlockThreads) void k_rbssm_taps(GBufferView gb, DepthMap D, HardUniforms u,
+                                                               PcssUniforms q, float* __restrict__ out,
+                                                               unsigned* __restrict__ counters,
+                                                               const uint2* __restrict__ list) {
+    const unsigned n = counters[0];
+    int h0;
+    const int side = rbssm_side(q, h0);
+    const int ntaps = side * side;
+    const int per = ntaps < 64 ? ntaps : 64;
+    const int P = 64 / per;
+    const int lane = (int)(threadIdx.x & 63);
+    const int slot = lane / per, sl = lane - slot * per;
+    const int g0 = (slot < P ? slot : P - 1) * per;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const Rbssm r{Rbsm<false>{D, u.stepx, u.stepy, u.thr, u.max_search}, 0.0f, 0};
+    for (WaveGroups wg(n, (unsigned)P); wg.more(); wg.advance()) {
+        const unsigned qi = wg.base + (unsigned)slot;
+        const bool has = slot < P && qi < n;
+        const uint2 e = list[has ? qi : wg.base];
+        const size_t k = e.x;
+        const float pw = __uint_as_float(e.y);
+        const float vx = gb.pos[k], vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+        float sc[4];
+        mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
+        const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        float illum = 0.0f;
+        for (int t0 = 0; t0 < ntaps; t0 += per) {
+            const int t = t0 + sl;
+            float v = 0.0f;
+            if (has && t < ntaps) {
+                const int th = t / side;
+                v = r.tap(q, u.si, pw, n0, n1, n2, h0 + th, h0 + (t - th * side));
+            }
+            illum = ordered_sum(illum, v, g0, ntaps - t0 < per ? ntaps - t0 : per);
+        }
+        if (has && sl == 0) out[k] = illum / (float)(q.ks * q.ks);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -702,33 +763,116 @@ struct RevSoftUniforms {
     int cap;          // scan bound of the accurate smoothONDS
 };
 
-// RevectorizationBasedAccurateSoftShadow.frag:1252-1331.  Pixels the
-// pre-evaluation lets through take their classification (clamped to
-// [si, 1]); penumbra pixels (0.5) run every sample, weighted by its
-// quad-tree level.  kRpcf == false: one tap, revectorized where the stored
-// discontinuity of that sample is set; kRpcf == true: a 3 x 3 tap kernel,
-// each tap's discontinuity computed on the fly and revectorized.
+// One light sample of RevectorizationBasedAccurateSoftShadow.frag:1275-1325
+// for the pixel at world position v: the sample's shadow value times its
+// quad-tree weight, and the weight.  kRpcf == false: one tap, revectorized
+// where the stored discontinuity of that sample is set; kRpcf == true: a
+// 3 x 3 tap kernel, each tap's discontinuity computed on the fly and
+// revectorized.
 template <bool kRpcf>
-__global__ __launch_bounds__(kBlockThreads) void k_rev_soft(GBufferView gb, const float* __restrict__ maps, int smw,
-                                                             int smh, const float4* __restrict__ trans,
-                                                             const unsigned char* __restrict__ disc,
-                                                             const float* __restrict__ vis, RevSoftUniforms u,
-                                                             float* __restrict__ out) {
+__device__ __forceinline__ void rev_sample(const float* __restrict__ maps, int smw, int smh,
+                                           const float4* __restrict__ trans, const unsigned char* __restrict__ disc,
+                                           size_t plane, size_t k, const RevSoftUniforms& u, const float cm[4], int i,
+                                           float& term, float& af) {
+    const float si = u.h.si;
+    float4 T = trans[i];
+    af = adaptive_weight(T);
+    const float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
+    const float sx = cx / cw, sy = cy / cw, sz = cz / cw;
+    const DepthMap D{maps + (size_t)i * ((size_t)smw * smh), smw, smh, (float)smw, (float)smh};
+    const Rbssm r{Rbsm<false>{D, u.h.stepx, u.h.stepy, u.h.thr, u.h.max_search}, 0.0f, u.cap};
+    float value;
+    if constexpr (!kRpcf) {
+        const float d = D.fetch(sx, sy);
+        const int layer = i < u.disc_layers ? i : u.disc_layers - 1;
+        float dr, dg, db;
+        disc_decode(disc[(size_t)layer * plane + k], dr, dg, db);
+        if (dr > 0.0f || dg > 0.0f)
+            value = r.rsmss(sx, sy, sz, dr, dg, db, si);
+        else
+            value = (sz <= d) ? 1.0f : si;
+    } else {
+        const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
+        float illum = 0.0f;
+        int n = 0;
+        for (float w = -1.0f; w <= 1.0f; w += 1.0f) {
+            for (float hh = -1.0f; hh <= 1.0f; hh += 1.0f) {
+                const float tx = sx + w * incrWidth, ty = sy + hh * incrHeight;
+                const float d = D.fetch(tx, ty);
+                float dr, dg, db;
+                r.rb.discontinuity(tx, ty, sz, d, dr, dg, db);
+                if (dr > 0.0f || dg > 0.0f)
+                    illum = illum + r.rsmss(tx, ty, sz, dr, dg, db, si);
+                else
+                    illum = illum + ((sz <= d) ? 1.0f : si);
+                ++n;
+            }
+        }
+        value = illum / (float)n;
+    }
+    term = value * af;
+}
+
+// Workspace of the revectorization-based final pass: the list length (in a
+// 256-byte header), then one pixel index per view pixel.
+inline size_t rev_soft_workspace_bytes(int w, int h) { return kListHeader + (size_t)w * h * 4u; }
+
+// RevectorizationBasedAccurateSoftShadow.frag:1252-1331 up to the sample
+// loop.  Pixels the pre-evaluation lets through take their classification
+// (clamped to [si, 1]); penumbra pixels (0.5) go to the list for
+// k_rev_soft_samples.
+__global__ __launch_bounds__(kBlockThreads) void k_rev_soft_classify(GBufferView gb, const float* __restrict__ vis,
+                                                                      RevSoftUniforms u, float* __restrict__ out,
+                                                                      unsigned* __restrict__ counters,
+                                                                      unsigned* __restrict__ list) {
     int x, y;
     pixel_of_block(gb.w, gb.h, x, y);
-    if (x >= gb.w || y >= gb.h) return;
-    const size_t plane = (size_t)gb.w * gb.h;
-    const size_t k = (size_t)y * gb.w + x;
-    const float vx = gb.pos[k];
-    if (vx == 0.0f) {  // discard: the cleared target holds 0
-        out[k] = 0.0f;
-        return;
+    bool pen = false;
+    size_t k = 0;
+    if (x < gb.w && y < gb.h) {
+        const size_t plane = (size_t)gb.w * gb.h;
+        k = (size_t)y * gb.w + x;
+        const float vx = gb.pos[k];
+        float shadow = 0.0f;  // discard: the cleared target holds 0
+        if (vx != 0.0f) {
+            const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+            shadow = pre_evaluation(u.h, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+            if (shadow == 1.0f) shadow = gclamp(vis[k], u.h.si, 1.0f);
+            pen = shadow == 0.5f;
+        }
+        if (!pen) out[k] = shadow;
     }
-    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
-    const float si = u.h.si;
-    float shadow = pre_evaluation(u.h, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
-    if (shadow == 1.0f) shadow = gclamp(vis[k], si, 1.0f);
-    if (shadow == 0.5f) {
+    const unsigned slot = wave_append(pen, &counters[0]);
+    if (pen) list[slot] = (unsigned)k;
+}
+
+// The sample loop of the listed penumbra pixels, a lane per light sample:
+// 64 / nsamples pixels per wave group when nsamples <= 64, else one pixel in
+// passes of 64 samples.  Penumbra pixels are a small fraction of the view
+// (those along shadow edges), so a lane per pixel would leave most of the
+// GPU idle while a few waves run every sample.  The weighted values and the
+// weights are summed in sample order, the shader's `acc += v * w; count += w`
+// operation for operation.
+template <bool kRpcf>
+__global__ __launch_bounds__(kBlockThreads) void k_rev_soft_samples(GBufferView gb, const float* __restrict__ maps,
+                                                                     int smw, int smh, const float4* __restrict__ trans,
+                                                                     const unsigned char* __restrict__ disc,
+                                                                     RevSoftUniforms u, float* __restrict__ out,
+                                                                     unsigned* __restrict__ counters,
+                                                                     const unsigned* __restrict__ list) {
+    const unsigned n = counters[0];
+    const int S = u.nsamples;
+    const int per = S < 64 ? S : 64;
+    const int P = 64 / per;
+    const int lane = (int)(threadIdx.x & 63);
+    const int slot = lane / per, sl = lane - slot * per;
+    const int g0 = (slot < P ? slot : P - 1) * per;
+    const size_t plane = (size_t)gb.w * gb.h;
+    for (WaveGroups wg(n, (unsigned)P); wg.more(); wg.advance()) {
+        const unsigned qi = wg.base + (unsigned)slot;
+        const bool has = slot < P && qi < n;
+        const size_t k = list[has ? qi : wg.base];
+        const float vx = gb.pos[k], vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
         float cm[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -737,49 +881,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_rev_soft(GBufferView gb, cons
             a = a + u.m[2 * 4 + i] * vz;
             cm[i] = a;
         }
-        const size_t mapsz = (size_t)smw * smh;
-        DepthMap D{maps, smw, smh, (float)smw, (float)smh};
-        const float incrWidth = 1.0f / D.fw, incrHeight = 1.0f / D.fh;
         float acc = 0.0f, count = 0.0f;
-        for (int i = 0; i < u.nsamples; ++i) {
-            float4 T = trans[i];
-            const float af = adaptive_weight(T);
-            const float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
-            const float sx = cx / cw, sy = cy / cw, sz = cz / cw;
-            D.texels = maps + (size_t)i * mapsz;
-            const Rbssm r{Rbsm<false>{D, u.h.stepx, u.h.stepy, u.h.thr, u.h.max_search}, 0.0f, u.cap};
-            if constexpr (!kRpcf) {
-                const float d = D.fetch(sx, sy);
-                const int layer = i < u.disc_layers ? i : u.disc_layers - 1;
-                float dr, dg, db;
-                disc_decode(disc[(size_t)layer * plane + k], dr, dg, db);
-                if (dr > 0.0f || dg > 0.0f)
-                    acc = acc + r.rsmss(sx, sy, sz, dr, dg, db, si) * af;
-                else
-                    acc = acc + ((sz <= d) ? 1.0f : si) * af;
-            } else {
-                float illum = 0.0f;
-                int n = 0;
-                for (float w = -1.0f; w <= 1.0f; w += 1.0f) {
-                    for (float hh = -1.0f; hh <= 1.0f; hh += 1.0f) {
-                        const float tx = sx + w * incrWidth, ty = sy + hh * incrHeight;
-                        const float d = D.fetch(tx, ty);
-                        float dr, dg, db;
-                        r.rb.discontinuity(tx, ty, sz, d, dr, dg, db);
-                        if (dr > 0.0f || dg > 0.0f)
-                            illum = illum + r.rsmss(tx, ty, sz, dr, dg, db, si);
-                        else
-                            illum = illum + ((sz <= d) ? 1.0f : si);
-                        ++n;
-                    }
-                }
-                acc = acc + (illum / (float)n) * af;
-            }
-            count = count + af;
+        for (int i0 = 0; i0 < S; i0 += per) {
+            const int i = i0 + sl;
+            float term = 0.0f, af = 0.0f;
+            if (has && i < S) rev_sample<kRpcf>(maps, smw, smh, trans, disc, plane, k, u, cm, i, term, af);
+            const int cnt = S - i0 < per ? S - i0 : per;
+            acc = ordered_sum(acc, term, g0, cnt);
+            count = ordered_sum(count, af, g0, cnt);
         }
-        shadow = acc / count;
+        if (has && sl == 0) out[k] = acc / count;
     }
-    out[k] = shadow;
 }
 
 }  // namespace

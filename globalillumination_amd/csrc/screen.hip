@@ -17,6 +17,7 @@
 //   based adaptive sampling).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 
@@ -40,6 +41,35 @@ bool gbuffer_ok(const gi_gbuffer* g) {
 }
 
 dim3 grid_for(int w, int h) { return dim3((w + kBlockX - 1) / kBlockX, (h + kBlockY - 1) / kBlockY); }
+
+// CUs of the current device (looked up once per device).
+int device_cus() {
+    static std::atomic<int> cus[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    int n = cus[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cus[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// Workgroups of a kernel that expands a pixel list into lanes: as many as
+// every CU holds at once (the kernel's occupancy), so each wave slot of the
+// GPU takes groups from the list until it runs out.
+template <typename K>
+unsigned expand_blocks(K kernel) {
+    static std::atomic<int> cached{0};  // one gfx950 code object: the same on every device
+    int per_cu = cached.load(std::memory_order_relaxed);
+    if (per_cu <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlockThreads, 0) != hipSuccess ||
+            per_cu <= 0)
+            per_cu = 4;
+        cached.store(per_cu, std::memory_order_relaxed);
+    }
+    return (unsigned)(device_cus() * per_cu);
+}
 
 }  // namespace
 }  // namespace gi
@@ -246,13 +276,21 @@ int gi_rev_quadtree_node_test(const gi_gbuffer* gbuffer, const float* maps_dev, 
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
+size_t gi_rev_soft_workspace_size(int width, int height) {
+    if (width <= 0 || height <= 0 || width > GI_MAX_EDGE || height > GI_MAX_EDGE) return 0;
+    return rev_soft_workspace_bytes(width, height);
+}
+
 int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw,
                                      int smh, const float light_mvp[16], const float* trans_dev,
                                      const unsigned char* disc_dev, int disc_layers, const float* visibility_dev,
-                                     const gi_rev_soft_params* p, float* out_dev, gi_stream stream) {
+                                     const gi_rev_soft_params* p, float* out_dev, void* workspace_dev,
+                                     size_t workspace_bytes, gi_stream stream) {
     if (!gbuffer_ok(gbuffer) || !gbuffer->normal_dev || !maps_dev || !light_mvp || !trans_dev || !disc_dev ||
-        !visibility_dev || !p || !out_dev)
+        !visibility_dev || !p || !out_dev || !workspace_dev)
         return GI_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(workspace_dev) & 15u) != 0) return GI_EINVAL;
+    if (workspace_bytes < rev_soft_workspace_bytes(gbuffer->width, gbuffer->height)) return GI_ENOSPACE;
     if (nsamples <= 0 || nsamples > GI_MAX_SAMPLES) return GI_EINVAL;
     if (smw <= 0 || smh <= 0 || smw > GI_MAX_EDGE || smh > GI_MAX_EDGE || disc_layers <= 0) return GI_EINVAL;
     if (p->max_search < 0 || p->max_search > 64) return GI_EINVAL;
@@ -276,13 +314,18 @@ int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* map
     GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
-    dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
+    unsigned* counters = static_cast<unsigned*>(workspace_dev);
+    unsigned* list = reinterpret_cast<unsigned*>(static_cast<unsigned char*>(workspace_dev) + kListHeader);
+    dim3 block(kBlockThreads);
+    hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, counters);
+    hipLaunchKernelGGL(k_rev_soft_classify, grid_for(gb.w, gb.h), block, 0, s, gb, visibility_dev, u, out_dev,
+                       counters, list);
     if (p->rpcf)
-        hipLaunchKernelGGL(k_rev_soft<true>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, disc_dev,
-                           visibility_dev, u, out_dev);
+        hipLaunchKernelGGL(k_rev_soft_samples<true>, dim3(expand_blocks(k_rev_soft_samples<true>)), block, 0, s, gb,
+                           maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
     else
-        hipLaunchKernelGGL(k_rev_soft<false>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, disc_dev,
-                           visibility_dev, u, out_dev);
+        hipLaunchKernelGGL(k_rev_soft_samples<false>, dim3(expand_blocks(k_rev_soft_samples<false>)), block, 0, s, gb,
+                           maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
@@ -352,11 +395,18 @@ int gi_pcss_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_pcs
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
+size_t gi_rbssm_workspace_size(int width, int height) {
+    if (width <= 0 || height <= 0 || width > GI_MAX_EDGE || height > GI_MAX_EDGE) return 0;
+    return rbssm_workspace_bytes(width, height);
+}
+
 int gi_rbssm_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_rbssm_params* p, float* out_dev,
-                  gi_stream stream) {
+                  void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
     if (!p) return GI_EINVAL;
     const gi_pcss_params& c = p->pcss;
-    if (!gbuffer_ok(gbuffer) || !depth_dev || !out_dev) return GI_EINVAL;
+    if (!gbuffer_ok(gbuffer) || !depth_dev || !out_dev || !workspace_dev) return GI_EINVAL;
+    if ((reinterpret_cast<uintptr_t>(workspace_dev) & 15u) != 0) return GI_EINVAL;
+    if (workspace_bytes < rbssm_workspace_bytes(gbuffer->width, gbuffer->height)) return GI_ENOSPACE;
     if (c.shadow_map_width <= 0 || c.shadow_map_height <= 0 || c.shadow_map_width > GI_MAX_EDGE ||
         c.shadow_map_height > GI_MAX_EDGE)
         return GI_EINVAL;
@@ -395,7 +445,13 @@ int gi_rbssm_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_rb
     DepthMap D{depth_dev, c.shadow_map_width, c.shadow_map_height, (float)c.shadow_map_width,
                (float)c.shadow_map_height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_rbssm, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u, q, out_dev);
+    unsigned* counters = static_cast<unsigned*>(workspace_dev);
+    uint2* list = reinterpret_cast<uint2*>(static_cast<unsigned char*>(workspace_dev) + kListHeader);
+    hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, counters);
+    hipLaunchKernelGGL(k_rbssm_classify, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u, q, out_dev,
+                       counters, list);
+    hipLaunchKernelGGL(k_rbssm_taps, dim3(expand_blocks(k_rbssm_taps)), dim3(kBlockThreads), 0, s, gb, D, u, q,
+                       out_dev, counters, list);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

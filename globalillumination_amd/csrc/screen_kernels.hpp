@@ -30,6 +30,43 @@ __device__ __forceinline__ void wave_tile_xy(int tid, int& px, int& py) {
     py = (wave >> 1) * 8 + (lane >> 3);
 }
 
+// Appends the lanes with `want` to a list whose length is *count (one atomic
+// per wave; slots in lane order) and returns each such lane's slot.  All 64
+// lanes of the wave must call it.
+__device__ __forceinline__ unsigned wave_append(bool want, unsigned* count) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0ull) return 0u;
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned base = 0u;
+    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
+    base = (unsigned)__shfl((int)base, leader, 64);
+    return base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Work distribution of the kernels that expand a list of pixels into lanes:
+// wave gw of nw takes the groups of P entries starting at gw * P, (gw + nw) *
+// P, ... until the list (length n) runs out.  A shared counter handing out
+// groups would cost an atomic per group on one address, and those serialize
+// across the GPU (about 11 ns each), slower than the groups themselves.
+struct WaveGroups {
+    unsigned n, step, base;
+    __device__ WaveGroups(unsigned n_, unsigned P) : n(n_) {
+        const unsigned wpb = blockDim.x / 64u;
+        step = gridDim.x * wpb * P;
+        base = (blockIdx.x * wpb + (threadIdx.x >> 6)) * P;
+    }
+    __device__ bool more() const { return base < n; }
+    __device__ void advance() { base += step; }
+};
+
+// Sums `v` over the `cnt` lanes starting at lane g0 in lane order, one
+// rounded addition at a time (`acc + v0 + v1 + ...` as a loop writes it).
+__device__ __forceinline__ float ordered_sum(float acc, float v, int g0, int cnt) {
+    for (int j = 0; j < cnt; ++j) acc = acc + __shfl(v, g0 + j, 64);
+    return acc;
+}
+
 struct GBufferView {
     const float* __restrict__ pos;  // 3 planes
     const float* __restrict__ nrm;  // 3 planes (facing-flipped) or null
@@ -398,47 +435,70 @@ struct RbsmConservative {
 
 // Pixel index with an XCD-aware block order.  Hardware hands consecutive
 // workgroups to the 8 XCDs round-robin; block b is remapped so that each XCD
-// drains one contiguous range of a strip-major order: vertical strips of
-// ceil(bx_n / 8) block columns, row-major inside each strip.  Neighbouring
-// pixels' shadow-map texels then stay in one XCD's L2, and every XCD's strip
-// crosses the whole image height, which balances background (cheap) and lit
-// (costly) pixels across the XCDs better than horizontal bands do.
-// Block (bx, by) at position lin of the strip-major order of a bx_n x by_n
-// grid: vertical strips of ceil(bx_n / 8) block columns, row-major inside
-// each strip.
+// drains one contiguous range of a strip-major order: 8 * kSpx vertical
+// strips of equal width (the last ones narrower or empty), dealt to the XCDs
+// in turn (XCD k drains strips k, k + 8, ...), row-major inside each strip.
+// Neighbouring pixels' shadow-map texels then stay in one XCD's L2.  One strip
+// per XCD (kSpx = 1) suits the passes that stream the G-buffer: a strip edge
+// splits the 128-byte lines of a pixel row between two XCDs, and wide strips
+// have few edges.  Passes whose cost per pixel varies across the view (area-
+// light and filtered revectorized passes) take more, narrower strips per XCD,
+// so each XCD samples several parts of the view and the XCDs finish together.
+// Block (bx, by) at position lin of that order on a bx_n x by_n grid:
+template <int kSpx>
 __device__ __forceinline__ void strip_block(int lin, int bx_n, int by_n, int& bx, int& by) {
-    const int strip = (bx_n + 7) / 8;
-    const int nstrips = (bx_n + strip - 1) / strip;
-    int k = lin / (strip * by_n);
-    if (k > nstrips - 1) k = nstrips - 1;
-    const int q = lin - k * strip * by_n;
-    const int wk = min(strip, bx_n - k * strip);
-    bx = k * strip + q % wk;
-    by = q / wk;
+    constexpr int kStrips = 8 * kSpx;
+    const int sw = (bx_n + kStrips - 1) / kStrips;
+    int acc = 0;
+    bx = 0;
+    by = 0;
+    for (int o = 0; o < kStrips; ++o) {
+        const int s = (o % kSpx) * 8 + o / kSpx;
+        const int x0 = s * sw;
+        const int wk = x0 >= bx_n ? 0 : min(sw, bx_n - x0);
+        if (lin < acc + wk * by_n) {
+            const int q = lin - acc;
+            bx = x0 + q % wk;
+            by = q / wk;
+            return;
+        }
+        acc += wk * by_n;
+    }
 }
 
-__device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
-    const int bx_n = (w + kBlockX - 1) / kBlockX;
-    const int by_n = (int)gridDim.y;
-    const int nblocks = bx_n * by_n;
+// Position of this workgroup in the strip-major order: XCD k (block b with
+// b % 8 == k) drains the k-th of 8 contiguous ranges.
+__device__ __forceinline__ int xcd_order(int nblocks) {
     const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
     const int per = nblocks / 8;
     const int rem = nblocks - per * 8;
     const int xcd = b % 8, slot = b / 8;
-    int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-    if (lin >= nblocks) lin = b;
+    const int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+    return lin < nblocks ? lin : b;
+}
+
+template <int kSpx = 1>
+__device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
+    const int bx_n = (w + kBlockX - 1) / kBlockX;
+    const int by_n = (int)gridDim.y;
     int bx, by, px, py;
-    strip_block(lin, bx_n, by_n, bx, by);
+    strip_block<kSpx>(xcd_order(bx_n * by_n), bx_n, by_n, bx, by);
     wave_tile_xy((int)threadIdx.x, px, py);
     x = bx * kBlockX + px;
     y = by * kBlockY + py;
 }
 
+// Strips per XCD of the passes whose per-pixel cost varies most: filtered
+// revectorization and PCSS (2), the area-light pass (4).
+constexpr int kSpxFiltered = 2;
+constexpr int kSpxSoft = 4;
+
 template <int kTech>
 __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap D, HardUniforms u,
                                                             float* __restrict__ out) {
     int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
+    constexpr bool kFiltered = kTech == GI_RPCF || kTech == GI_RBSM_FILTERED || kTech == GI_RPCF_CONSERVATIVE;
+    pixel_of_block<kFiltered ? kSpxFiltered : 1>(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     const size_t plane = (size_t)gb.w * gb.h;
     const size_t k = (size_t)y * gb.w + x;
@@ -548,7 +608,7 @@ __device__ __forceinline__ float pcss_pcf(const DepthMap& D, const PcssUniforms&
 __global__ __launch_bounds__(kBlockThreads) void k_pcss(GBufferView gb, DepthMap D, HardUniforms u, PcssUniforms q,
                                                          float* __restrict__ out) {
     int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
+    pixel_of_block<kSpxFiltered>(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     const size_t plane = (size_t)gb.w * gb.h;
     const size_t k = (size_t)y * gb.w + x;
@@ -816,7 +876,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const fl
                                                             int smh, const float4* __restrict__ trans,
                                                             SoftUniforms u, float* __restrict__ out) {
     int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
+    pixel_of_block<kSpxSoft>(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     soft_pixel<NT, kAdaptive>(gb, maps, smw, smh, trans, u, out, x, y);
 }
@@ -835,15 +895,8 @@ __device__ __forceinline__ void wide_block_origin(int w, int h, int& ox, int& oy
     const int bw = 16 * PX, bh = 16 * PY;
     const int bx_n = (w + bw - 1) / bw;
     const int by_n = (h + bh - 1) / bh;
-    const int nblocks = bx_n * by_n;
-    const int b = (int)(blockIdx.y * gridDim.x + blockIdx.x);
-    const int per = nblocks / 8;
-    const int rem = nblocks - per * 8;
-    const int xcd = b % 8, slot = b / 8;
-    int lin = (xcd < rem) ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-    if (lin >= nblocks) lin = b;
     int bx, by;
-    strip_block(lin, bx_n, by_n, bx, by);
+    strip_block<kSpxSoft>(xcd_order(bx_n * by_n), bx_n, by_n, bx, by);
     ox = bx * bw;
     oy = by * bh;
 }
@@ -933,7 +986,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft_chunk(GBufferView gb, co
                                                                   SoftUniforms u, SoftChunk ch,
                                                                   float* __restrict__ out) {
     int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
+    pixel_of_block<kSpxSoft>(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     soft_pixel<NT, false, true>(gb, maps, smw, smh, trans, u, out, x, y, ch);
 }

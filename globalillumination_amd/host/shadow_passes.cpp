@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <memory>
+#include <initializer_list>
 #include <cstring>
 
 namespace gi::host {
@@ -306,10 +307,16 @@ int SoftShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight
     GI_TRY(disc_.allocate(static_cast<size_t>(GI_REV_DISC_LAYERS) * npx));
     GI_TRY(hip_ok(hipMemsetAsync(disc_.get(), 0, disc_.size(), stream)));
     GI_TRY(hip_ok(hipMemsetAsync(qtVis_.get(), 0, qtVis_.size(), stream)));
-    size_t ws = gi_depth_pass_workspace_size(ntris_, maxSamples, smw_, smh_);
-    size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
-    if (ws == 0 || wg == 0) return GI_ENOTSUP;
-    return ws_.allocate(ws > wg ? ws : wg);
+    // one workspace for the passes of a frame, which run one after another
+    size_t need = 0;
+    for (size_t b : {gi_depth_pass_workspace_size(ntris_, maxSamples, smw_, smh_),
+                     gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight),
+                     gi_rbssm_workspace_size(windowWidth, windowHeight),
+                     gi_rev_soft_workspace_size(windowWidth, windowHeight)}) {
+        if (b == 0) return GI_ENOTSUP;
+        need = b > need ? b : need;
+    }
+    return ws_.allocate(need);
 }
 
 int SoftShadowPipeline::renderGBuffer() {
@@ -393,7 +400,7 @@ int SoftShadowPipeline::renderSoftShadows() {
     gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
     if (rbssm) {
         gi_rbssm_params r{p, maxSearch, depthThreshold};
-        return gi_rbssm_pass(&g, maps_.as<float>(), &r, soft_.as<float>(), stream);
+        return gi_rbssm_pass(&g, maps_.as<float>(), &r, soft_.as<float>(), ws_.get(), ws_.size(), stream);
     }
     return gi_pcss_pass(&g, maps_.as<float>(), &p, soft_.as<float>(), stream);
 }
@@ -619,7 +626,7 @@ int SoftShadowPipeline::renderAdaptiveLightSourceSampling() {
         gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
         GI_TRY(gi_rev_adaptive_soft_shadow_pass(&g, maps_.as<float>(), n, smw_, smh_, common, trans_.as<float>(),
                                                 disc_.as<unsigned char>(), GI_REV_DISC_LAYERS, qtVis_.as<float>(),
-                                                &rp, soft_.as<float>(), stream));
+                                                &rp, soft_.as<float>(), ws_.get(), ws_.size(), stream));
         // the next frame's RPCF (main.cpp:880-885)
         if (RPCF && n <= 9)
             RPCF = false;

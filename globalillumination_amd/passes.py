@@ -355,7 +355,9 @@ def render_rbssm(gbuf: GBuffer, depth: torch.Tensor, params: RBSSMParams, out: t
         out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=gbuf.position.device)
     g = gbuf.to_c(with_normals=params.preEvaluation)
     p = params.to_c()
-    _lib.call("gi_rbssm_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out), _stream_handle(stream))
+    ws = _WS.get(_lib.lib().gi_rbssm_workspace_size(gbuf.width, gbuf.height), gbuf.position.device)
+    _lib.call("gi_rbssm_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out), _dptr(ws),
+              ctypes.c_size_t(ws.numel()), _stream_handle(stream))
     return out
 
 
@@ -600,7 +602,8 @@ def render_rev_adaptive(gbuf: GBuffer, maps: torch.Tensor, common: np.ndarray, t
         out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=maps.device)
     g = gbuf.to_c(with_normals=True)
     p = params.to_c()
+    ws = _WS.get(_lib.lib().gi_rev_soft_workspace_size(gbuf.width, gbuf.height), maps.device)
     _lib.call("gi_rev_adaptive_soft_shadow_pass", ctypes.byref(g), _dptr(maps), S, smw, smh, _f(common, 16),
               _dptr(trans_dev), _dptr(disc), int(disc.shape[0]), _dptr(visibility), ctypes.byref(p), _dptr(out),
-              _stream_handle(stream))
+              _dptr(ws), ctypes.c_size_t(ws.numel()), _stream_handle(stream))
     return out

@@ -330,16 +330,25 @@ typedef struct gi_rev_soft_params {
     float light_position[3];
 } gi_rev_soft_params;
 
+/* Device workspace of gi_rev_adaptive_soft_shadow_pass for a width x height
+ * view: a 256-byte header and 4 bytes per pixel (the list of penumbra
+ * pixels).  0 for sizes out of range. */
+size_t gi_rev_soft_workspace_size(int width, int height);
+
 /* The final pass (main.cpp:868-871): RevectorizationBasedAccurateSoftShadow.frag
  * :1252-1331.  Arguments as gi_adaptive_soft_shadow_pass plus the
  * discontinuity layers and the visibility classification of the last node
  * test; the gbuffer needs its normals.  A lit pixel takes its
  * classification clamped to [si, 1]; pixels at 0.5 average every sample,
- * revectorizing taps on a discontinuity (RSMSS :1175-1186). */
+ * revectorizing taps on a discontinuity (RSMSS :1175-1186).  The penumbra
+ * pixels are listed in the workspace (16-byte aligned, at least
+ * gi_rev_soft_workspace_size bytes, else GI_ENOSPACE) and their samples
+ * spread over the GPU; calls sharing a workspace must not overlap. */
 int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int nsamples, int smw,
                                      int smh, const float light_mvp[16], const float* trans_dev,
                                      const unsigned char* disc_dev, int disc_layers, const float* visibility_dev,
-                                     const gi_rev_soft_params* params, float* out_dev, gi_stream stream);
+                                     const gi_rev_soft_params* params, float* out_dev, void* workspace_dev,
+                                     size_t workspace_bytes, gi_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Percentage-closer soft shadows (plausible soft shadows, PCSS)             */
@@ -385,9 +394,18 @@ typedef struct gi_rbssm_params {
     float depth_threshold;  /* SceneLoader `d` (main.cpp:1618) */
 } gi_rbssm_params;
 
-/* out_dev: width*height R channel (0 for background).  kernel_size 2..31. */
+/* Device workspace of gi_rbssm_pass for a width x height view: a 256-byte
+ * header and 8 bytes per pixel (the list of pixels that run the kernel
+ * loop, with their penumbra widths).  0 for sizes out of range. */
+size_t gi_rbssm_workspace_size(int width, int height);
+
+/* out_dev: width*height R channel (0 for background).  kernel_size 2..31.
+ * Pixels that run the kernel loop are listed in the workspace (16-byte
+ * aligned, at least gi_rbssm_workspace_size bytes, else GI_ENOSPACE) and
+ * their taps spread over the GPU; calls sharing a workspace must not
+ * overlap. */
 int gi_rbssm_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_rbssm_params* params, float* out_dev,
-                  gi_stream stream);
+                  void* workspace_dev, size_t workspace_bytes, gi_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Deferred Phong shading                                                     */

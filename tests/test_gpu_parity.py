@@ -558,13 +558,15 @@ def test_rev_node_test_matches_oracle(gi, oracle_lib, cuda):
         assert (want_img == 5.0).sum() > 0 and (vis_np == 0.5).sum() > 0
 
 
-@pytest.mark.parametrize("rpcf,si", [(False, 0.25), (True, 0.25), (False, 0.5)])
-def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si):
+@pytest.mark.parametrize("rpcf,si,nsamples", [(False, 0.25, 9), (True, 0.25, 9), (False, 0.5, 9),
+                                               (False, 0.25, 81), (True, 0.25, 81)])
+def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si, nsamples):
     """RevectorizationBasedAccurateSoftShadow.frag with mixed quad-tree levels,
-    discontinuity layers for the first 6 of 9 maps (the rest read a
-    3-layer-clamped or stale layer as the reference does), and the visibility
-    classification of one level-0 node test."""
-    wl, g, mvps, maps, biased, MV, nm, lpos = _rev_frame(gi, nsamples=9)
+    discontinuity layers for the first 6 maps (the rest read a clamped or
+    stale layer as the reference does), and the visibility classification of
+    one level-0 node test.  9 samples put 7 pixels in a wave's lanes; 81 run
+    one pixel in two passes of 64 and 17 samples."""
+    wl, g, mvps, maps, biased, MV, nm, lpos = _rev_frame(gi, nsamples=nsamples)
     gp, gn = _np(g.position), _np(g.normal)
     layers = 6
     disc = gi.passes.rev_discontinuity_layers(g, layers=layers)
@@ -579,7 +581,7 @@ def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si):
     vis_np = np.zeros((g.height, g.width), np.float32)
     gi.passes.rev_quadtree_node_test(g, maps, [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc, 0, vis)
     oracle_lib.rev_node_test(gp, _np(maps), [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc_np, 0, vis_np)
-    levels = [0, 1, 2, 3, 4, 1, 6, 2, 0]
+    levels = ([0, 1, 2, 3, 4, 1, 6, 2, 0] * 9)[:nsamples]
     common, trans = gi.passes.adaptive_trans(mvps, levels)
     params = gi.RevSoftParams(MV, nm, lpos, shadowIntensity=si, RPCF=rpcf, maxSearch=16, depthThreshold=2.5e-6)
     print(f"rev soft rpcf={rpcf} si={si}", flush=True)
@@ -619,10 +621,20 @@ def test_rev_rejects(gi, cuda):
         gi._lib.GI_EINVAL
     common, trans = gi.passes.adaptive_trans(mvps, [0, 0, 0, 0])
     tr = torch.from_numpy(np.ascontiguousarray(trans)).cuda()
+    nws = L.gi_rev_soft_workspace_size(g.width, g.height)
+    assert nws == 256 + g.width * g.height * 4
+    ws = torch.empty((nws,), dtype=torch.uint8, device="cuda")
     p = gi.RevSoftParams(MV, nm, lpos, maxSearch=65).to_c()
     assert L.gi_rev_adaptive_soft_shadow_pass(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, F16(*common),
                                               tr.data_ptr(), disc.data_ptr(), 4, vis.data_ptr(), ctypes.byref(p),
-                                              out.data_ptr(), None) == gi._lib.GI_EINVAL
+                                              out.data_ptr(), ws.data_ptr(), nws, None) == gi._lib.GI_EINVAL
+    p = gi.RevSoftParams(MV, nm, lpos).to_c()
+    assert L.gi_rev_adaptive_soft_shadow_pass(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, F16(*common),
+                                              tr.data_ptr(), disc.data_ptr(), 4, vis.data_ptr(), ctypes.byref(p),
+                                              out.data_ptr(), ws.data_ptr(), nws - 1, None) == gi._lib.GI_ENOSPACE
+    assert L.gi_rev_adaptive_soft_shadow_pass(ctypes.byref(c), maps.data_ptr(), 4, 1024, 1024, F16(*common),
+                                              tr.data_ptr(), disc.data_ptr(), 4, vis.data_ptr(), ctypes.byref(p),
+                                              out.data_ptr(), None, nws, None) == gi._lib.GI_EINVAL
 
 
 def _pcss_scene(gi, sm):
@@ -718,10 +730,13 @@ def test_rbssm_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval,
         assert (pcf != want).any()
 
 
-@pytest.mark.parametrize("ks,smw,smh", [(15, 1024, 1024), (4, 1024, 1024), (7, 1536, 1024)])
+@pytest.mark.parametrize("ks,smw,smh", [(15, 1024, 1024), (4, 1024, 1024), (7, 1536, 1024), (2, 1024, 1024),
+                                        (9, 1024, 1024)])
 def test_rbssm_kernel_sizes_and_map_shapes(gi, oracle_lib, cuda, ks, smw, smh):
     """RBSSM with the reference's other kernel sizes (15; an even size runs
-    int(-1.5) = -1 .. 1 and divides by 16) and a non-square map."""
+    int(-1.5) = -1 .. 1 and divides by 16) and a non-square map.  The tap
+    lanes: 1 tap (size 2) puts 64 pixels in a wave, 9 taps 7 pixels, 49 one
+    pixel; 81 and 225 taps run one pixel in passes of 64."""
     wl = gi.scenes.config3(nsamples=1, view_w=480, view_h=270, sm=smw, freq=71)
     wl.lights = [gi.scenes.View((120.0, 280.0, 80.0), (0.0, 0.0, 0.0), smw, smh, fov=5.0)]
     dm = gi.DeviceMesh.from_mesh(wl.mesh)
@@ -740,7 +755,10 @@ def test_rbssm_kernel_sizes_and_map_shapes(gi, oracle_lib, cuda, ks, smw, smh):
     want = oracle_lib.rbssm(_np(g.position), _np(g.normal), _np(depth[0]), op, blocker_search_size=7,
                             kernel_size=ks, light_source_radius=64, z_near=1)
     assert bits_equal(got, want), mismatch_report(got, want, f"rbssm ks={ks} {smw}x{smh}")
-    assert ((want > 0.25) & (want < 1.0)).sum() > 100
+    if ks > 2:
+        assert ((want > 0.25) & (want < 1.0)).sum() > 100
+    else:  # one tap divided by 4: the loop's pixels end in [si / 4, 1 / 4]
+        assert ((want > 0.0) & (want < 0.25)).sum() > 100
 
 
 def test_rbssm_rejects(gi, cuda):
@@ -752,6 +770,10 @@ def test_rbssm_rejects(gi, cuda):
     depth = torch.ones((64, 64), device="cuda")
     out = torch.empty((48, 64), device="cuda")
     L = gi.lib()
+    nws = L.gi_rbssm_workspace_size(64, 48)
+    assert nws == 256 + 64 * 48 * 8
+    assert L.gi_rbssm_workspace_size(0, 48) == 0
+    ws = torch.empty((nws,), dtype=torch.uint8, device="cuda")
     base = gi.RBSSMParams(lightMVP=light_mvps(wl)[0], shadowMapWidth=64, shadowMapHeight=64)
     for field, val in (("kernel_size", 1), ("kernel_size", 32), ("blocker_search_size", 1),
                        ("light_source_radius", -1), ("max_search", -1), ("max_search", 65)):
@@ -761,13 +783,20 @@ def test_rbssm_rejects(gi, cuda):
         else:
             setattr(p.pcss, field, val)
         c = g.to_c(with_normals=False)
-        assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
-            gi._lib.GI_EINVAL, (field, val)
+        assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), ws.data_ptr(),
+                               nws, None) == gi._lib.GI_EINVAL, (field, val)
     p = base.to_c()
     p.depth_threshold = float("nan")
     c = g.to_c(with_normals=False)
-    assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None) == \
-        gi._lib.GI_ENONFINITE
+    assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), ws.data_ptr(), nws,
+                           None) == gi._lib.GI_ENONFINITE
+    p = base.to_c()
+    assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), ws.data_ptr(),
+                           nws - 1, None) == gi._lib.GI_ENOSPACE
+    assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(), None, nws,
+                           None) == gi._lib.GI_EINVAL
+    assert L.gi_rbssm_pass(ctypes.byref(c), depth.data_ptr(), ctypes.byref(p), out.data_ptr(),
+                           ws.data_ptr() + 4, nws - 4, None) == gi._lib.GI_EINVAL
 
 
 def _write_obj(path, mesh):
