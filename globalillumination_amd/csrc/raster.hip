@@ -86,8 +86,8 @@ int raster_stage(const MeshView& mesh, const MvpSource& mvps, int map_begin, int
                  hipEvent_t ev_end = nullptr) {
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     if (ev_begin && hipEventRecord(ev_begin, s) != hipSuccess) return GI_EDEVICE;
-    hipLaunchKernelGGL((k_raster<kTile, kVis>), dim3(ntiles, map_count), dim3(kRasterThreads), 0, s, mesh, mvps, cfg,
-                       ws.bins, out, map_begin);
+    hipLaunchKernelGGL((k_raster<kTile, kVis>), dim3((unsigned)(ntiles * map_count)), dim3(kRasterThreads), 0, s, mesh,
+                       mvps, cfg, ws.bins, out, map_begin, map_count);
     if (ev_end && hipEventRecord(ev_end, s) != hipSuccess) return GI_EDEVICE;
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }

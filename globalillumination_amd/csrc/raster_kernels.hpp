@@ -27,8 +27,11 @@ constexpr int kMaxPoly = 9;
 #ifndef GI_SELF_RASTER_MAX_PX
 #define GI_SELF_RASTER_MAX_PX 256
 #endif
+// 63 setup records (64 B each) beside the 16 KiB tile and the record count
+// keep a workgroup's LDS at 20,420 bytes, so 8 workgroups fit in a CU's
+// 160 KiB (64 records made it 20,484 bytes: 7 workgroups, k_raster 3% slower).
 #ifndef GI_REC_QUEUE
-#define GI_REC_QUEUE 64
+#define GI_REC_QUEUE 63
 #endif
 constexpr int kSelfRasterMaxPx = GI_SELF_RASTER_MAX_PX;  // bbox∩tile area a lone thread rasterizes itself
 constexpr int kScanThreads = 1024;
@@ -949,16 +952,23 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
 }
 
 // Binned raster: one workgroup per (tile, map); overflowed maps are left to
-// k_raster_fallback.
+// k_raster_fallback.  Workgroups go out tile-major (the map_count maps of
+// tile 0, then of tile 1, ...).  The hardware deals workgroups to CUs in
+// turn, about equally many to each, whatever their cost; in map-major order
+// the costly tiles (those under the occluder, about 7x the time of a tile
+// holding only the ground) arrive in runs that pile onto a few CUs while the
+// rest idle.  Tile-major order spreads each costly tile's maps over the XCDs
+// and CUs.
 template <int TS, bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
-                                                           void* out, int map_base) {
+                                                           void* out, int map_base, int map_count) {
     __shared__ typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
-    const int map = map_base + (int)blockIdx.y;
+    const int map = map_base + (int)(blockIdx.x % (unsigned)map_count);
+    const int tile_id = (int)(blockIdx.x / (unsigned)map_count);
     if (b.overflow[map]) return;
-    raster_tile<TS, kVis, false>(mesh, mvps, cfg, b, out, map, blockIdx.x, tile, rec, nrec);
+    raster_tile<TS, kVis, false>(mesh, mvps, cfg, b, out, map, tile_id, tile, rec, nrec);
 }
 
 // One workgroup per tile, looping over the maps whose lists overflowed.
