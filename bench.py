@@ -544,16 +544,18 @@ def main():
     # each).  Tile raster (k_raster): the maps written once (4 B/texel) + the
     # triangle vertices read per map (3 x 12 B per triangle).
     taps = tap_offsets(3, 1.0, False)
-    texels = 0 if args.no_footprint else touched_texels(gi, torch, g, mvps, smw, smw, taps)
+    # --no-footprint (profiling runs) leaves the texels uncounted: the
+    # screen pass's algorithmic bytes, and every figure built on them, are null
+    texels = None if args.no_footprint else touched_texels(gi, torch, g, mvps, smw, smw, taps)
     npx = W * H
     nfg = int((g.position[0] != 0).sum())
-    screen_bytes = nfg * (12 + 4) + (npx - nfg) * 8 + texels * 4
+    screen_bytes = None if texels is None else nfg * (12 + 4) + (npx - nfg) * 8 + texels * 4
     raster_bytes = S * smw * smw * 4 + S * wl.mesh.ntris * 36
     # Phong: position, normal, facing and shadow in (28 B), RGBA out (16 B) per
     # foreground pixel; position x in and the clear colour out (20 B) per
     # background pixel
     phong_bytes = nfg * 48 + (npx - nfg) * 20
-    screen_gbs = screen_bytes / (screen_ms * 1e-3) / 1e9
+    screen_gbs = None if screen_bytes is None else screen_bytes / (screen_ms * 1e-3) / 1e9
     raster_gbs = raster_bytes / (raster_ms * 1e-3) / 1e9
     if raster_ms >= screen_ms:
         dom = {"kernel": "k_raster (tile raster of the depth pass)", "achieved": raster_gbs, "bytes": raster_bytes,
@@ -562,14 +564,18 @@ def main():
         dom = {"kernel": "k_soft (soft-shadow screen pass)", "achieved": screen_gbs, "bytes": screen_bytes,
                "ms": screen_ms}
     dom_name = "k_raster" if dom["kernel"].startswith("k_raster") else "k_soft"
-    roof = {"bound": "hbm", "achieved": round(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom["achieved"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom_name),
+    def rnd(x, n):
+        return None if x is None else round(x, n)
+
+    roof = {"bound": "hbm", "achieved": rnd(dom["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": rnd(None if dom["achieved"] is None else dom["achieved"] / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(dom_name),
             "traffic_source": str(TRAFFIC_SUMMARY.relative_to(ROOT)), "kernel": dom["kernel"],
-            "algorithmic_bytes": int(dom["bytes"]), "ms": round(dom["ms"], 4),
-            "screen_pass": {"kernel": "k_soft", "ms": round(screen_ms, 4), "GB/s": round(screen_gbs, 1),
+            "algorithmic_bytes": None if dom["bytes"] is None else int(dom["bytes"]), "ms": round(dom["ms"], 4),
+            "screen_pass": {"kernel": "k_soft", "ms": round(screen_ms, 4), "GB/s": rnd(screen_gbs, 1),
                             "taps_per_s": round(npx * S * len(taps) ** 2 / (screen_ms * 1e-3), -6),
-                            "frac": round(screen_gbs / HBM_PEAK_GBS, 4), "bytes": int(screen_bytes),
-                            "unique_texels": int(texels), "traffic": measured_traffic("k_soft"),
+                            "frac": rnd(None if screen_gbs is None else screen_gbs / HBM_PEAK_GBS, 4),
+                            "bytes": screen_bytes, "unique_texels": texels, "traffic": measured_traffic("k_soft"),
                             "traffic_GB/s": (round(measured_traffic("k_soft") / (screen_ms * 1e-3) / 1e9, 1)
                                              if measured_traffic("k_soft") else None)},
             "depth_pass": {"ms": round(depth_ms, 4), "raster_kernel_ms": round(raster_ms, 4),

@@ -55,15 +55,16 @@ int device_cus() {
     return n;
 }
 
-// Workgroups of a kernel that expands a pixel list into lanes: as many as
-// every CU holds at once (the kernel's occupancy), so each wave slot of the
-// GPU takes groups from the list until it runs out.
-template <typename K>
-unsigned expand_blocks(K kernel) {
-    static std::atomic<int> cached{0};  // one gfx950 code object: the same on every device
+// Workgroups a list-expanding kernel gets: exactly as many as the GPU holds
+// at once, since its waves take fixed shares of the list and a second round
+// of workgroups would start late.  Looked up once per kernel (one gfx950 code
+// object: the same on every device).
+template <auto kKernel>
+unsigned expand_blocks() {
+    static std::atomic<int> cached{0};
     int per_cu = cached.load(std::memory_order_relaxed);
     if (per_cu <= 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlockThreads, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kKernel, kBlockThreads, 0) != hipSuccess ||
             per_cu <= 0)
             per_cu = 4;
         cached.store(per_cu, std::memory_order_relaxed);
@@ -321,11 +322,11 @@ int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* map
     hipLaunchKernelGGL(k_rev_soft_classify, grid_for(gb.w, gb.h), block, 0, s, gb, visibility_dev, u, out_dev,
                        counters, list);
     if (p->rpcf)
-        hipLaunchKernelGGL(k_rev_soft_samples<true>, dim3(expand_blocks(k_rev_soft_samples<true>)), block, 0, s, gb,
-                           maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
+        hipLaunchKernelGGL(k_rev_soft_samples<true>, dim3(expand_blocks<k_rev_soft_samples<true>>()), block, 0, s,
+                           gb, maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
     else
-        hipLaunchKernelGGL(k_rev_soft_samples<false>, dim3(expand_blocks(k_rev_soft_samples<false>)), block, 0, s, gb,
-                           maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
+        hipLaunchKernelGGL(k_rev_soft_samples<false>, dim3(expand_blocks<k_rev_soft_samples<false>>()), block, 0, s,
+                           gb, maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 
@@ -450,7 +451,7 @@ int gi_rbssm_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_rb
     hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, counters);
     hipLaunchKernelGGL(k_rbssm_classify, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, D, u, q, out_dev,
                        counters, list);
-    hipLaunchKernelGGL(k_rbssm_taps, dim3(expand_blocks(k_rbssm_taps)), dim3(kBlockThreads), 0, s, gb, D, u, q,
+    hipLaunchKernelGGL(k_rbssm_taps, dim3(expand_blocks<k_rbssm_taps>()), dim3(kBlockThreads), 0, s, gb, D, u, q,
                        out_dev, counters, list);
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
