@@ -730,9 +730,9 @@ __device__ __forceinline__ void edges_at(const Setup& s, int i, int j, Edges& E)
 __device__ __forceinline__ uint32_t plane_bits(const Setup& s, int dx, int dy) {
     float z = s.z0 + (s.A * (float)dx + s.B * (float)dy);
     z = z + s.zoff;
-    if (!(z > 0.0f)) z = 0.0f;
-    if (z > 1.0f) z = 1.0f;
-    return fbits(z);
+    if (!(z > 0.0f)) z = 0.0f;  // NaN and -0 become +0
+    // z is +0 or positive here (never NaN), where min(z, 1) is `z > 1 ? 1 : z`
+    return fbits(__builtin_fminf(z, 1.0f));
 }
 
 template <bool kVis>
@@ -824,6 +824,7 @@ __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0
     if (j > ry1) return;
     if (rect_inside(s, rx0, rx1, ry0, ry1)) {
         // every pixel centre of the rectangle passes the three edge tests
+#pragma unroll 4
         for (; j <= ry1; j += kRowStep)
             store_min<kVis>(s, plane_bits(s, i * 256 + 128 - s.X[0], j * 256 + 128 - s.Y[0]),
                             (j - y0) * TS + (i - x0), tile);

@@ -596,6 +596,30 @@ def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si, nsamples):
     assert ((want > si) & (want < 1.0)).sum() > 100
 
 
+def test_rev_soft_without_penumbra_pixels(gi, oracle_lib, cuda):
+    """A visibility map with no pixel at 0.5 leaves the penumbra list empty:
+    every pixel takes its clamped classification, and the sample kernel
+    launches with nothing to do."""
+    wl, g, mvps, maps, biased, MV, nm, lpos = _rev_frame(gi, nsamples=4)
+    gp, gn = _np(g.position), _np(g.normal)
+    disc = gi.passes.rev_discontinuity_layers(g, layers=4)
+    vis_np = np.where((np.arange(g.width)[None, :] // 7) % 2 == 0, 1.0, 0.0).astype(np.float32)
+    vis_np = np.broadcast_to(vis_np, (g.height, g.width)).copy()
+    vis = torch.from_numpy(vis_np).cuda()
+    common, trans = gi.passes.adaptive_trans(mvps, [0, 1, 2, 3])
+    disc_np = _np(disc)
+    for rpcf in (False, True):
+        params = gi.RevSoftParams(MV, nm, lpos, shadowIntensity=0.25, RPCF=rpcf, maxSearch=16)
+        print(f"rev soft, empty list, rpcf={rpcf}", flush=True)
+        got = _np(gi.passes.render_rev_adaptive(g, maps, common, trans, disc, vis, params))
+        op = oracle_lib.make_params(common, 1024, 1024, "naive", shadow_intensity=0.25, max_search=16,
+                                    pre_evaluation=True, mv=MV, normal_matrix=nm, light_position=lpos)
+        want = oracle_lib.rev_soft(gp, gn, _np(maps), common, trans, disc_np, vis_np, op, rpcf=rpcf)
+        assert bits_equal(got, want)
+        fg = gp[0] != 0
+        assert set(np.unique(want[fg]).tolist()) <= {0.25, 1.0}
+
+
 def test_rev_rejects(gi, cuda):
     import ctypes
 
