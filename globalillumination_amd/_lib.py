@@ -184,8 +184,10 @@ _SIGS = {
                                        ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                        ctypes.c_size_t, ctypes.c_void_p]),
+    "gi_hard_shadow_workspace_size": (ctypes.c_size_t, [ctypes.c_int] * 2),
     "gi_hard_shadow_pass": (ctypes.c_int, [ctypes.POINTER(GiGBuffer), ctypes.c_void_p,
-                                           ctypes.POINTER(GiShadowParams), ctypes.c_void_p, ctypes.c_void_p]),
+                                           ctypes.POINTER(GiShadowParams), ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_size_t, ctypes.c_void_p]),
     "gi_depth_pass_bin": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.POINTER(GiRasterOpts), ctypes.c_void_p, ctypes.c_void_p,

@@ -79,8 +79,13 @@ using namespace gi;
 
 extern "C" {
 
+size_t gi_hard_shadow_workspace_size(int width, int height) {
+    if (width <= 0 || height <= 0 || width > GI_MAX_EDGE || height > GI_MAX_EDGE) return 0;
+    return hard_list_bytes(width, height);
+}
+
 int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const gi_shadow_params* p,
-                        float* out_dev, gi_stream stream) {
+                        float* out_dev, void* workspace_dev, size_t workspace_bytes, gi_stream stream) {
     if (!gbuffer_ok(gbuffer) || !depth_dev || !p || !out_dev) return GI_EINVAL;
     if (p->technique < GI_NAIVE || p->technique > GI_RPCF_CONSERVATIVE) return GI_EINVAL;
     if (p->shadow_map_width <= 0 || p->shadow_map_height <= 0 || p->shadow_map_width > GI_MAX_EDGE ||
@@ -96,6 +101,11 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
                       p->penumbra_size > 64))
         return GI_EINVAL;
     if (p->technique >= GI_RBSM && (p->max_search < 0 || p->max_search > 4096)) return GI_EINVAL;
+    const bool listed = p->technique == GI_RPCF || p->technique == GI_RBSM_FILTERED;
+    if (listed) {
+        if (!workspace_dev || (reinterpret_cast<uintptr_t>(workspace_dev) & 15u) != 0) return GI_EINVAL;
+        if (workspace_bytes < hard_list_bytes(gbuffer->width, gbuffer->height)) return GI_ENOSPACE;
+    }
     if (p->pre_evaluation) {
         if (!gbuffer->normal_dev) return GI_EINVAL;
         if (!finite_all(p->mv, 16) || !finite_all(p->normal_matrix, 9) || !finite_all(p->light_position, 3))
@@ -119,12 +129,26 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
                (float)p->shadow_map_height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     dim3 grid = grid_for(gb.w, gb.h), block(kBlockThreads);
+    if (listed) {
+        // the loop's pixels listed, then a lane per tap
+        unsigned* count = static_cast<unsigned*>(workspace_dev);
+        unsigned* list = reinterpret_cast<unsigned*>(static_cast<unsigned char*>(workspace_dev) + kHardListHeader);
+        hipLaunchKernelGGL(k_list_reset, dim3(1), dim3(64), 0, s, count);
+        if (p->technique == GI_RPCF) {
+            hipLaunchKernelGGL(k_hard_classify<GI_RPCF>, grid, block, 0, s, gb, u, out_dev, count, list);
+            hipLaunchKernelGGL(k_hard_taps<GI_RPCF>, dim3(expand_blocks<k_hard_taps<GI_RPCF>>()), block, 0, s, gb, D,
+                               u, out_dev, count, list);
+        } else {
+            hipLaunchKernelGGL(k_hard_classify<GI_RBSM_FILTERED>, grid, block, 0, s, gb, u, out_dev, count, list);
+            hipLaunchKernelGGL(k_hard_taps<GI_RBSM_FILTERED>, dim3(expand_blocks<k_hard_taps<GI_RBSM_FILTERED>>()),
+                               block, 0, s, gb, D, u, out_dev, count, list);
+        }
+        return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
+    }
     switch (p->technique) {
     case GI_NAIVE: hipLaunchKernelGGL(k_hard<GI_NAIVE>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_PCF: hipLaunchKernelGGL(k_hard<GI_PCF>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_RBSM: hipLaunchKernelGGL(k_hard<GI_RBSM>, grid, block, 0, s, gb, D, u, out_dev); break;
-    case GI_RPCF: hipLaunchKernelGGL(k_hard<GI_RPCF>, grid, block, 0, s, gb, D, u, out_dev); break;
-    case GI_RBSM_FILTERED: hipLaunchKernelGGL(k_hard<GI_RBSM_FILTERED>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_RBSM_CONSERVATIVE:
         hipLaunchKernelGGL(k_hard<GI_RBSM_CONSERVATIVE>, grid, block, 0, s, gb, D, u, out_dev);
         break;

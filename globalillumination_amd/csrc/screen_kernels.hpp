@@ -286,29 +286,35 @@ struct Rbsm {
         return gmix(shadow, 1.0f, si);
     }
 
+    // One tap of computeShadowFromAccurateRPCF's loop (:313-345) at offsets
+    // (w, h) in texels: revectorized where the tap sits on a discontinuity,
+    // mixed with si.
+    __device__ float rpcf_tap(float x, float y, float z, float si, float w, float h) const {
+        const float incrWidth = 1.0f / D.fw;
+        const float incrHeight = 1.0f / D.fh;
+        float px = x + w * incrWidth;
+        float py = y + h * incrHeight;
+        float d = tex(px, py);
+        float shadow = le01(z, d);
+        float r, g, b;
+        discontinuity(px, py, z, d, r, g, b);
+        if (r > 0.0f || g > 0.0f) {
+            float subx = gfract(px * D.fw);
+            float suby = gfract(py * D.fh);
+            return gmix(revectorized(r, g, b, px, py, z, subx, suby, shadow), 1.0f, si);
+        }
+        return gmix(shadow, 1.0f, si);
+    }
+
     // computeShadowFromAccurateRPCF :306-349
     __device__ float rpcf(float x, float y, float z, float si, int kernel_order, int penumbra) const {
-        float incrWidth = 1.0f / D.fw;
-        float incrHeight = 1.0f / D.fh;
         float illum = 0.0f;
         float offset = (float)penumbra;
         float stepSize = 2.0f * offset / (float)kernel_order;
         int count = 0;
         for (float w = -offset; w <= offset; w += stepSize) {
             for (float h = -offset; h <= offset; h += stepSize) {
-                float px = x + w * incrWidth;
-                float py = y + h * incrHeight;
-                float d = tex(px, py);
-                float shadow = le01(z, d);
-                float r, g, b;
-                discontinuity(px, py, z, d, r, g, b);
-                if (r > 0.0f || g > 0.0f) {
-                    float subx = gfract(px * D.fw);
-                    float suby = gfract(py * D.fh);
-                    illum = illum + gmix(revectorized(r, g, b, px, py, z, subx, suby, shadow), 1.0f, si);
-                } else {
-                    illum = illum + gmix(shadow, 1.0f, si);
-                }
+                illum = illum + rpcf_tap(x, y, z, si, w, h);
                 count++;
             }
         }
@@ -542,6 +548,99 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap
         }
     }
     out[k] = shadow;
+}
+
+// Offsets of the RPCF tap loop `for (float w = -offset; w <= offset; w +=
+// stepSize)` (NonConservativeSMSR.frag:313-314, FilteredRBSM.frag:313-314):
+// the count per axis, and the offset after i steps, accumulated one addition
+// at a time as the loop does.
+__device__ __forceinline__ int rpcf_side(const HardUniforms& u) {
+    const float offset = (float)u.penumbra, stepSize = 2.0f * offset / (float)u.kernel_order;
+    int n = 0;
+    for (float w = -offset; w <= offset; w += stepSize) ++n;
+    return n;
+}
+__device__ __forceinline__ float rpcf_offset(const HardUniforms& u, int i) {
+    const float offset = (float)u.penumbra, stepSize = 2.0f * offset / (float)u.kernel_order;
+    float w = -offset;
+    for (int k = 0; k < i; ++k) w += stepSize;
+    return w;
+}
+
+// Workspace of the RPCF and FilteredRBSM passes: the list length (in a
+// 256-byte header), then one pixel index per view pixel.
+constexpr size_t kHardListHeader = 256;
+inline size_t hard_list_bytes(int w, int h) { return kHardListHeader + (size_t)w * h * 4u; }
+
+// The pixels the pre-evaluation lets through run RPCF's tap loop; the others
+// get their value here (k_hard's branches), and the loop's pixels go to the
+// list for k_hard_taps.
+template <int kTech>
+__global__ __launch_bounds__(kBlockThreads) void k_hard_classify(GBufferView gb, HardUniforms u,
+                                                                  float* __restrict__ out, unsigned* __restrict__ count,
+                                                                  unsigned* __restrict__ list) {
+    int x, y;
+    pixel_of_block<kSpxFiltered>(gb.w, gb.h, x, y);
+    bool loop = false;
+    size_t k = 0;
+    if (x < gb.w && y < gb.h) {
+        const size_t plane = (size_t)gb.w * gb.h;
+        k = (size_t)y * gb.w + x;
+        const float vx = gb.pos[k];
+        float shadow = 0.0f;  // Shadow.frag:244 discards; the cleared target holds 0
+        if (vx != 0.0f) {
+            shadow = 1.0f;
+            if (u.pre_eval) {
+                const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+                shadow = pre_evaluation(u, vx, vy, vz, gb.nrm[k], gb.nrm[plane + k], gb.nrm[2 * plane + k]);
+            }
+            loop = shadow == 1.0f;
+        }
+        if (!loop) out[k] = shadow;
+    }
+    const unsigned slot = wave_append(loop, count);
+    if (loop) list[slot] = (unsigned)k;
+}
+
+// computeShadowFromAccurateRPCF (:306-349) for the listed pixels, a lane per
+// tap: 64 / taps pixels per wave group when the loop has at most 64 taps,
+// else one pixel in passes of 64 taps.  The taps are summed in the loop's
+// order (w, then h) and divided by their count, as the shader does.
+template <int kTech>
+__global__ __launch_bounds__(kBlockThreads) void k_hard_taps(GBufferView gb, DepthMap D, HardUniforms u,
+                                                              float* __restrict__ out,
+                                                              const unsigned* __restrict__ count,
+                                                              const unsigned* __restrict__ list) {
+    const unsigned n = *count;
+    const int side = rpcf_side(u);
+    const int ntaps = side * side;
+    const int per = ntaps < 64 ? ntaps : 64;
+    const int P = 64 / per;
+    const int lane = (int)(threadIdx.x & 63);
+    const int slot = lane / per, sl = lane - slot * per;
+    const int g0 = (slot < P ? slot : P - 1) * per;
+    const size_t plane = (size_t)gb.w * gb.h;
+    const Rbsm<kTech == GI_RBSM_FILTERED> rb{D, u.stepx, u.stepy, u.thr, u.max_search};
+    for (WaveGroups wg(n, (unsigned)P); wg.more(); wg.advance()) {
+        const unsigned qi = wg.base + (unsigned)slot;
+        const bool has = slot < P && qi < n;
+        const size_t k = list[has ? qi : wg.base];
+        const float vx = gb.pos[k], vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+        float sc[4];
+        mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
+        const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
+        float illum = 0.0f;
+        for (int t0 = 0; t0 < ntaps; t0 += per) {
+            const int t = t0 + sl;
+            float v = 0.0f;
+            if (has && t < ntaps) {
+                const int tw = t / side;
+                v = rb.rpcf_tap(n0, n1, n2, u.si, rpcf_offset(u, tw), rpcf_offset(u, t - tw * side));
+            }
+            illum = ordered_sum(illum, v, g0, ntaps - t0 < per ? ntaps - t0 : per);
+        }
+        if (has && sl == 0) out[k] = illum / (float)ntaps;
+    }
 }
 
 // ---------------------------------------------------------------------------

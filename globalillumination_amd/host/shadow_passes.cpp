@@ -122,10 +122,15 @@ int HardShadowPipeline::init(const Mesh& mesh, int windowWidth, int windowHeight
     GI_TRY(gfac_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(hard_.allocate(bytes_of(npx, sizeof(float))));
     GI_TRY(rgba_.allocate(bytes_of(npx, 4 * sizeof(float))));
-    size_t ws = gi_depth_pass_workspace_size(ntris_, 1, shadowMapWidth, shadowMapHeight);
-    size_t wg = gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight);
-    if (ws == 0 || wg == 0) return GI_ENOTSUP;
-    return ws_.allocate(ws > wg ? ws : wg);
+    // one workspace for the frame's passes, which run one after another
+    size_t need = 0;
+    for (size_t b : {gi_depth_pass_workspace_size(ntris_, 1, shadowMapWidth, shadowMapHeight),
+                     gi_gbuffer_pass_workspace_size(ntris_, windowWidth, windowHeight),
+                     gi_hard_shadow_workspace_size(windowWidth, windowHeight)}) {
+        if (b == 0) return GI_ENOTSUP;
+        need = b > need ? b : need;
+    }
+    return ws_.allocate(need);
 }
 
 // displaySceneFromLightPOV (main.cpp:221-274) inside renderShadowMap
@@ -195,7 +200,7 @@ int HardShadowPipeline::computeHardShadows() {
     p.light_position[1] = lightEye[1];
     p.light_position[2] = -lightEye[2];
     gi_gbuffer g{gpos_.as<float>(), gnrm_.as<float>(), windowWidth_, windowHeight_, gfac_.as<float>()};
-    return gi_hard_shadow_pass(&g, depth_.as<float>(), &p, hard_.as<float>(), stream);
+    return gi_hard_shadow_pass(&g, depth_.as<float>(), &p, hard_.as<float>(), ws_.get(), ws_.size(), stream);
 }
 
 // configurePhong(lightEye rotated 180 degrees about y, cameraEye)

@@ -272,8 +272,9 @@ def compute_hard_shadows(gbuf: GBuffer, depth: torch.Tensor, params: ShadowParam
         out = torch.empty((gbuf.height, gbuf.width), dtype=torch.float32, device=gbuf.position.device)
     g = gbuf.to_c(with_normals=params.preEvaluation)
     p = params.to_c()
-    _lib.call("gi_hard_shadow_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out),
-              _stream_handle(stream))
+    ws = _WS.get(_lib.lib().gi_hard_shadow_workspace_size(gbuf.width, gbuf.height), gbuf.position.device)
+    _lib.call("gi_hard_shadow_pass", ctypes.byref(g), _dptr(depth), ctypes.byref(p), _dptr(out), _dptr(ws),
+              ctypes.c_size_t(ws.numel()), _stream_handle(stream))
     return out
 
 

@@ -210,12 +210,22 @@ typedef struct gi_shadow_params {
     float light_position[3]; /* lightPosition as configurePhong uploads it */
 } gi_shadow_params;
 
+/* Device workspace of gi_hard_shadow_pass for a width x height view: a
+ * 256-byte header and 4 bytes per pixel.  0 for sizes out of range. */
+size_t gi_hard_shadow_workspace_size(int width, int height);
+
 /* Replaces computeHardShadows -> displaySceneFromGBuffer(shader)
  * (ShadowMapping/src/main.cpp:400-414, :302-348).  depth_dev: one map of
  * shadow_map_height x shadow_map_width; out_dev: width*height floats holding
- * the R channel (1 = lit, shadow_intensity = shadowed, 0 = background). */
+ * the R channel (1 = lit, shadow_intensity = shadowed, 0 = background).
+ * GI_RPCF and GI_RBSM_FILTERED list the pixels that run their tap loop in the
+ * workspace (16-byte aligned, at least gi_hard_shadow_workspace_size bytes,
+ * else GI_ENOSPACE; NULL is GI_EINVAL) and spread their taps over the GPU;
+ * the other techniques ignore it (NULL is fine).  Calls sharing a workspace
+ * must not overlap. */
 int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev,
-                        const gi_shadow_params* params, float* out_dev, gi_stream stream);
+                        const gi_shadow_params* params, float* out_dev, void* workspace_dev,
+                        size_t workspace_bytes, gi_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Area-light soft-shadow screen pass (a13)                                  */

@@ -96,17 +96,29 @@ def test_hard_pass_argument_checks(gi):
     p.kernel_order, p.penumbra_size, p.max_search = 3, 0, 16
     p.shadow_intensity = 0.25
     # zero penumbra: the PCF loop would never advance
-    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None) == _lib.GI_EINVAL
+    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_EINVAL
     p.penumbra_size = 1
     p.technique = 9
-    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None) == _lib.GI_EINVAL
+    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_EINVAL
     p.technique = _lib.GI_NAIVE
     p.light_mvp[5] = float("inf")
-    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None) == _lib.GI_ENONFINITE
+    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_ENONFINITE
     p.light_mvp[5] = 1.0
     p.pre_evaluation = 1  # needs normals
-    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None) == _lib.GI_EINVAL
-    assert lib.gi_hard_shadow_pass(None, _fake(), ctypes.byref(p), _fake(2), None) == _lib.GI_EINVAL
+    assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_EINVAL
+    assert lib.gi_hard_shadow_pass(None, _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_EINVAL
+    # RPCF and FilteredRBSM need the workspace: missing, misaligned or short
+    p.pre_evaluation = 0
+    assert lib.gi_hard_shadow_workspace_size(64, 32) == 256 + 64 * 32 * 4
+    assert lib.gi_hard_shadow_workspace_size(0, 32) == 0
+    for tech in (_lib.GI_RPCF, _lib.GI_RBSM_FILTERED):
+        p.technique = tech
+        assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 1 << 20,
+                                       None) == _lib.GI_EINVAL
+        assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), ctypes.c_void_p(0x3004),
+                                       1 << 20, None) == _lib.GI_EINVAL
+        assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), _fake(16),
+                                       256 + 64 * 32 * 4 - 1, None) == _lib.GI_ENOSPACE
 
 
 def test_soft_pass_argument_checks(gi):

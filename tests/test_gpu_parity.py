@@ -373,6 +373,23 @@ def test_rbsm_soup(gi, oracle_lib, cuda, technique):
     assert bits_equal(got, want), mismatch_report(got, want, technique)
 
 
+@pytest.mark.parametrize("technique", ["rpcf", "rbsm_filtered"])
+@pytest.mark.parametrize("order,penumbra,pre_eval", [(1, 1, False), (2, 3, True), (5, 1, False), (9, 2, False)])
+def test_rpcf_tap_loops(gi, oracle_lib, cuda, technique, order, penumbra, pre_eval):
+    """RPCF and FilteredRBSM tap loops of other sizes (the float loop's tap
+    count: 2, 3, 6 and 10 per axis): one lane per tap, 16, 7 or 1 pixels per
+    wave, and 100 taps in passes of 64."""
+    mesh = shrink_soup(30_000)
+    wl = gi.scenes.config2(ntris=10, view_w=640, view_h=360, sm=1024)
+    wl.mesh = mesh
+    wl.lights = [gi.scenes.ortho_fit((0.3, -1.0, 0.2), mesh, 1024, 1024)]
+    got, want = _hard_case(gi, oracle_lib, wl, technique, pre_eval, 1024, 1024,
+                           {"maxSearch": 16, "depthThreshold": 2.5e-6, "kernelOrder": order,
+                            "penumbraSize": penumbra})
+    assert bits_equal(got, want), mismatch_report(got, want, f"{technique} order {order} penumbra {penumbra}")
+    assert ((want > 0.25) & (want < 1.0)).sum() > 100
+
+
 @pytest.fixture(scope="module")
 def config2_frame(gi, oracle_lib, cuda):
     """The RBSM workload at its stated size: 200,000-triangle soup (seed 1)
