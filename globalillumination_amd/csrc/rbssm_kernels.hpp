@@ -595,7 +595,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_rbssm_classify(GBufferView gb
         }
         if (!loop) out[k] = shadow;
     }
-    const unsigned slot = wave_append(loop, &counters[0]);
+    __shared__ unsigned lds[2];
+    const unsigned slot = block_append(loop, &counters[0], lds);
     if (loop) list[slot] = make_uint2((unsigned)k, __float_as_uint(pw));
 }
 
@@ -842,7 +843,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_rev_soft_classify(GBufferView
         }
         if (!pen) out[k] = shadow;
     }
-    const unsigned slot = wave_append(pen, &counters[0]);
+    __shared__ unsigned lds[2];
+    const unsigned slot = block_append(pen, &counters[0], lds);
     if (pen) list[slot] = (unsigned)k;
 }
 

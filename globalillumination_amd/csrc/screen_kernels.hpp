@@ -30,18 +30,25 @@ __device__ __forceinline__ void wave_tile_xy(int tid, int& px, int& py) {
     py = (wave >> 1) * 8 + (lane >> 3);
 }
 
-// Appends the lanes with `want` to a list whose length is *count (one atomic
-// per wave; slots in lane order) and returns each such lane's slot.  All 64
-// lanes of the wave must call it.
-__device__ __forceinline__ unsigned wave_append(bool want, unsigned* count) {
+// Appends the threads with `want` of a workgroup to a list whose length is
+// *count, one device-scope atomic per workgroup (those on one address
+// serialize across the GPU, about 11 ns each, so one per wave costs tens of
+// microseconds when thousands of waves append).  Returns each such thread's
+// slot.  Every thread of the workgroup must call it; it holds two barriers.
+// `lds` is two words of the workgroup's shared memory.
+__device__ __forceinline__ unsigned block_append(bool want, unsigned* count, unsigned* lds) {
+    if (threadIdx.x == 0) lds[0] = 0u;
+    __syncthreads();
     const unsigned long long m = __ballot(want);
-    if (m == 0ull) return 0u;
     const int lane = (int)(threadIdx.x & 63);
-    const int leader = __ffsll((long long)m) - 1;
-    unsigned base = 0u;
-    if (lane == leader) base = atomicAdd(count, (unsigned)__popcll(m));
-    base = (unsigned)__shfl((int)base, leader, 64);
-    return base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    const int leader = m ? __ffsll((long long)m) - 1 : 0;
+    unsigned off = 0u;
+    if (m && lane == leader) off = atomicAdd(&lds[0], (unsigned)__popcll(m));
+    off = (unsigned)__shfl((int)off, leader, 64);
+    __syncthreads();
+    if (threadIdx.x == 0) lds[1] = lds[0] ? atomicAdd(count, lds[0]) : 0u;
+    __syncthreads();
+    return lds[1] + off + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
 }
 
 // Work distribution of the kernels that expand a list of pixels into lanes:
@@ -598,7 +605,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard_classify(GBufferView gb,
         }
         if (!loop) out[k] = shadow;
     }
-    const unsigned slot = wave_append(loop, count);
+    __shared__ unsigned lds[2];
+    const unsigned slot = block_append(loop, count, lds);
     if (loop) list[slot] = (unsigned)k;
 }
 
