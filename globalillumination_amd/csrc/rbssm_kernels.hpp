@@ -896,5 +896,99 @@ __global__ __launch_bounds__(kBlockThreads) void k_rev_soft_samples(GBufferView 
     }
 }
 
+// The RPCF sample loop of the listed penumbra pixels, a lane per tap of each
+// sample's 3 x 3 kernel.  Lane per sample ran each sample's nine taps (and
+// their edge walks) one after another on a few thousand waves; here a wave
+// holds 7 (pixel, sample) pairs of 9 lanes and a workgroup 28 pairs, so the
+// same work spreads over nine times as many waves with one tap each.
+//
+// A workgroup takes units of Q = 28 / nsamples whole pixels (one pixel when
+// nsamples > 28, in passes of 28 samples).  Each pair's nine taps are summed
+// in the loop's order (w outer, h inner) by shuffles and divided by 9; the
+// pair's weighted value and weight go to LDS, and one lane per pixel adds
+// them in sample order, `acc += v * w; count += w` as the shader does.
+constexpr int kRevPairsPerWave = 7;
+constexpr int kRevPairs = kRevPairsPerWave * (kBlockThreads / 64);
+__global__ __launch_bounds__(kBlockThreads) void k_rev_soft_taps(GBufferView gb, const float* __restrict__ maps,
+                                                                  int smw, int smh,
+                                                                  const float4* __restrict__ trans,
+                                                                  RevSoftUniforms u, float* __restrict__ out,
+                                                                  const unsigned* __restrict__ counters,
+                                                                  const unsigned* __restrict__ list) {
+    __shared__ float lterm[kRevPairs], laf[kRevPairs];
+    const unsigned n = counters[0];
+    const int S = u.nsamples;
+    const int Q = S <= kRevPairs ? kRevPairs / S : 1;  // pixels per unit
+    const int pairs = Q * S;                              // pairs per unit
+    const unsigned units = (n + (unsigned)Q - 1u) / (unsigned)Q;
+    const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);
+    const int pw = lane / 9, tap = lane - pw * 9;  // pair within the wave, tap within the pair
+    const int slot = wave * kRevPairsPerWave + pw;  // pair within the pass (lane 63: none)
+    const int g0 = (pw < kRevPairsPerWave ? pw : kRevPairsPerWave - 1) * 9;
+    const float tw = (float)(tap / 3) - 1.0f, th = (float)(tap % 3) - 1.0f;  // the loop's w and h
+    const size_t plane = (size_t)gb.w * gb.h;
+    const size_t mapsz = (size_t)smw * smh;
+    const float fw = (float)smw, fh = (float)smh;
+    const float incrWidth = 1.0f / fw, incrHeight = 1.0f / fh;
+    const int owner = (int)threadIdx.x;  // thread q < Q sums pixel q of the unit
+    float acc = 0.0f, count = 0.0f;
+    for (unsigned unit = blockIdx.x; unit < units; unit += gridDim.x) {
+        for (int p0 = 0; p0 < pairs; p0 += kRevPairs) {
+            const int pl = p0 + slot;
+            const unsigned q = unit * (unsigned)Q + (unsigned)(pl / S);
+            const bool has = pw < kRevPairsPerWave && pl < pairs && q < n;
+            float v = 0.0f, af = 0.0f;
+            if (has) {
+                const size_t k = list[q];
+                const int i = pl % S;
+                const float vx = gb.pos[k], vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
+                float cm[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    float a = u.m[0 * 4 + c] * vx;
+                    a = a + u.m[1 * 4 + c] * vy;
+                    a = a + u.m[2 * 4 + c] * vz;
+                    cm[c] = a;
+                }
+                float4 T = trans[i];
+                af = adaptive_weight(T);
+                const float cx = cm[0] + T.x, cy = cm[1] + T.y, cz = cm[2] + T.z, cw = cm[3] + T.w;
+                const float sx = cx / cw, sy = cy / cw, sz = cz / cw;
+                const DepthMap D{maps + (size_t)i * mapsz, smw, smh, fw, fh};
+                const Rbssm r{Rbsm<false>{D, u.h.stepx, u.h.stepy, u.h.thr, u.h.max_search}, 0.0f, u.cap};
+                const float tx = sx + tw * incrWidth, ty = sy + th * incrHeight;
+                const float d = D.fetch(tx, ty);
+                float dr, dg, db;
+                r.rb.discontinuity(tx, ty, sz, d, dr, dg, db);
+                if (dr > 0.0f || dg > 0.0f)
+                    v = r.rsmss(tx, ty, sz, dr, dg, db, u.h.si);
+                else
+                    v = (sz <= d) ? 1.0f : u.h.si;
+            }
+            const float illum = ordered_sum(0.0f, v, g0, 9);
+            if (has && tap == 0) {
+                lterm[slot] = (illum / 9.0f) * af;
+                laf[slot] = af;
+            }
+            __syncthreads();
+            if (owner < Q) {
+                // this pixel's pairs in this pass, in sample order
+                const int a0 = owner * S, a1 = a0 + S;
+                const int lo = a0 > p0 ? a0 : p0, hi = a1 < p0 + kRevPairs ? a1 : p0 + kRevPairs;
+                const unsigned qo = unit * (unsigned)Q + (unsigned)owner;
+                if (qo < n) {
+                    if (lo == a0) acc = 0.0f, count = 0.0f;
+                    for (int e = lo; e < hi; ++e) {
+                        acc = acc + lterm[e - p0];
+                        count = count + laf[e - p0];
+                    }
+                    if (hi == a1) out[list[qo]] = acc / count;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 }  // namespace
 }  // namespace gi

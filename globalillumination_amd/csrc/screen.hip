@@ -361,8 +361,8 @@ int gi_rev_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* map
     hipLaunchKernelGGL(k_rev_soft_classify, grid_for(gb.w, gb.h), block, 0, s, gb, visibility_dev, u, out_dev,
                        counters, list);
     if (p->rpcf)
-        hipLaunchKernelGGL(k_rev_soft_samples<true>, dim3(expand_blocks<k_rev_soft_samples<true>>()), block, 0, s,
-                           gb, maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
+        hipLaunchKernelGGL(k_rev_soft_taps, dim3(expand_blocks<k_rev_soft_taps>()), block, 0, s, gb, maps_dev, smw,
+                           smh, tr, u, out_dev, counters, list);
     else
         hipLaunchKernelGGL(k_rev_soft_samples<false>, dim3(expand_blocks<k_rev_soft_samples<false>>()), block, 0, s,
                            gb, maps_dev, smw, smh, tr, disc_dev, u, out_dev, counters, list);
