@@ -52,7 +52,8 @@ def _parse():
     ap.add_argument("--batch-steps", type=int, default=5, help="timed steps of the batched frames")
     ap.add_argument("--no-passes", action="store_true", help="skip the other screen passes' timing")
     ap.add_argument("--no-rbsm", action="store_true", help="skip the RBSM workload's timing")
-    ap.add_argument("--passes-steps", type=int, default=5, help="timed steps of each other screen pass")
+    ap.add_argument("--passes-steps", type=int, default=5,
+                    help="timed steps per run of each other screen pass (median of 3 runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device copy (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     return ap.parse_args()
@@ -126,6 +127,24 @@ def hard_footprint(torch, g, biased, smw, smh, taps, neighbours=False):
     return int(mark.sum())
 
 
+def event_ms(torch, fn, steps, stream, reps=3, warm=3):
+    """Milliseconds per call of fn on `stream`: `warm` untimed calls, then the
+    median over `reps` runs of `steps` calls bracketed by HIP events."""
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    runs = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        runs.append(e0.elapsed_time(e1) / steps)
+    return sorted(runs)[len(runs) // 2]
+
+
 def config2_timing(gi, torch, steps):
     """RBSM hard shadows on the RBSM workload at its stated size: a
     200,000-triangle soup (seed 1) over a 40x40 quad, directional light with a
@@ -160,15 +179,7 @@ def config2_timing(gi, torch, steps):
         sp = gi.ShadowParams(lightMVP=mvps[0], shadowMapWidth=2048, shadowMapHeight=2048, technique=tech,
                              zbias=wl.zbias, preEvaluation=True, MV=MV, normalMatrix=nm, lightPosition=lpos,
                              maxSearch=16, depthThreshold=2.5e-6, kernelOrder=3, penumbraSize=1)
-        passes.compute_hard_shadows(g, depth[0], sp, out=out)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        for _ in range(steps):
-            passes.compute_hard_shadows(g, depth[0], sp, out=out)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / steps
+        ms = event_ms(torch, lambda: passes.compute_hard_shadows(g, depth[0], sp, out=out), steps, stream)
         texels = hard_footprint(torch, g, biased, 2048, 2048, taps, neighbours=True)
         nbytes = gbytes + texels * 4
         gbs = nbytes / (ms * 1e-3) / 1e9
@@ -362,15 +373,7 @@ def passes_timing(gi, torch, steps):
     stream = torch.cuda.current_stream()
 
     def timed(fn):
-        fn()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        for _ in range(steps):
-            fn()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return round(e0.elapsed_time(e1) / steps, 4)
+        return round(event_ms(torch, fn, steps, stream), 4)
 
     res = {"workload": "far-light frame: 100,822-triangle displaced icosphere + ground, 1920x1080 view, "
                        "2048^2 maps (light 40x further, 5-degree frustum)", "steps": steps}

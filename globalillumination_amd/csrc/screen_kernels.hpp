@@ -526,9 +526,9 @@ __device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
 }
 
 // Strips per XCD of the passes whose per-pixel cost varies most: filtered
-// revectorization and PCSS (2), the area-light pass (4).
+// revectorization and PCSS (2); the area-light pass has its own block shape
+// and 4 strips (soft_block_pixel).
 constexpr int kSpxFiltered = 2;
-constexpr int kSpxSoft = 4;
 
 template <int kTech>
 __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap D, HardUniforms u,
@@ -1095,75 +1095,72 @@ __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* _
     }
 }
 
-// One thread per pixel.  Blocks are ordered XCD-aware (pixel_of_block) so each
-// XCD's L2 serves one contiguous band of the view.
+// Block shape of the soft pass.  Each (block, sample) pair fetches every
+// cache line its taps touch, and a line that straddles two blocks is fetched
+// by both, since neighbouring blocks reach a given sample at different times
+// (a model of this over the 64-sample frame reproduces the measured HBM
+// traffic of 16x16 blocks to 3%).  A 128-byte line holds 32 texels of a map
+// row, so blocks wider than tall straddle fewer lines per pixel: the model
+// gives 1.83 GB for 16x16 blocks, 1.52 GB for 32x8 and 1.45 GB for 64x4.
+// Each wave covers a 16 x 4 patch.  On the 64-sample 1080p frame (3x3 taps)
+// the pass takes 0.355 ms with 64x4 blocks of 16x4 waves, against 0.395 ms
+// with 16x16 blocks of 8x8 waves, 0.358 ms with 32x8 blocks of 16x4 waves,
+// 0.362 ms with 64x4 blocks of 32x2 waves, 0.402 ms with 64x1 waves and
+// 0.417 ms with 128x2 blocks; 2 or 8 strips per XCD instead of 4 took
+// 0.394 and 0.360 ms.
+constexpr int kSoftBW = 64, kSoftBH = kBlockThreads / kSoftBW;
+constexpr int kSoftWX = 16, kSoftWY = 64 / kSoftWX;
+constexpr int kSpxSoftWide = 4;
+static_assert(kSoftBW % kSoftWX == 0 && kSoftBH % kSoftWY == 0, "wave tiles must tile the block");
+inline int soft_blocks(int w, int h) { return ((w + kSoftBW - 1) / kSoftBW) * ((h + kSoftBH - 1) / kSoftBH); }
+__device__ __forceinline__ void soft_block_pixel(int w, int h, int& x, int& y) {
+    const int bx_n = (w + kSoftBW - 1) / kSoftBW;
+    const int by_n = (h + kSoftBH - 1) / kSoftBH;
+    int bx, by;
+    strip_block<kSpxSoftWide>(xcd_order(bx_n * by_n), bx_n, by_n, bx, by);
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    constexpr int kWavesX = kSoftBW / kSoftWX;
+    x = bx * kSoftBW + (wave % kWavesX) * kSoftWX + lane % kSoftWX;
+    y = by * kSoftBH + (wave / kWavesX) * kSoftWY + lane / kSoftWX;
+}
+
+// One thread per pixel, any tap count.  Blocks are ordered XCD-aware
+// (soft_block_pixel) so each XCD's L2 serves one contiguous band of the view.
 template <int NT, bool kAdaptive = false>
 __global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
                                                             int smh, const float4* __restrict__ trans,
                                                             SoftUniforms u, float* __restrict__ out) {
     int x, y;
-    pixel_of_block<kSpxSoft>(gb.w, gb.h, x, y);
+    soft_block_pixel(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     soft_pixel<NT, kAdaptive>(gb, maps, smw, smh, trans, u, out, x, y);
 }
 
-// Wide blocks for the soft pass.  Each (block, sample) pair fetches every
-// cache line its taps touch, and a line that straddles two blocks is fetched
-// by both, since neighbouring blocks reach a given sample at different times
-// (a model of this over the 64-sample frame reproduces the measured HBM
-// traffic of 16x16 blocks to 3%).  Fewer, wider blocks cut those straddling
-// lines.  A lane holds PX x PY pixels strided by the 8 x 8 wave tile, so each
-// load instruction of the wave still covers one compact 8 x 8 patch, and
-// adjacent patches are adjacent in x: the block covers 16 PX x 16 PY pixels.
 // Every pixel adds its samples in order, as soft_pixel does (same bits).
-template <int PX, int PY>
-__device__ __forceinline__ void wide_block_origin(int w, int h, int& ox, int& oy) {
-    const int bw = 16 * PX, bh = 16 * PY;
-    const int bx_n = (w + bw - 1) / bw;
-    const int by_n = (h + bh - 1) / bh;
-    int bx, by;
-    strip_block<kSpxSoft>(xcd_order(bx_n * by_n), bx_n, by_n, bx, by);
-    ox = bx * bw;
-    oy = by * bh;
-}
-
-template <int NT, int PX, int PY, bool kAdaptive = false>
+template <int NT, bool kAdaptive = false>
 __global__ __launch_bounds__(kBlockThreads) void k_soft_wide(GBufferView gb, const float* __restrict__ maps, int smw,
                                                                  int smh, const float4* __restrict__ trans,
                                                                  SoftUniforms u, float* __restrict__ out) {
-    constexpr int P = PX * PY;
-    int ox, oy;
-    wide_block_origin<PX, PY>(gb.w, gb.h, ox, oy);
-    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
-    const int lx = ox + (wave & 1) * 8 * PX + (lane & 7);
-    const int ly = oy + (wave >> 1) * 8 * PY + (lane >> 3);
+    int x, y;
+    soft_block_pixel(gb.w, gb.h, x, y);
     const size_t plane = (size_t)gb.w * gb.h;
-    float cm[P][4];
-    float acc[P];
-    bool fg[P];
+    float cm[4];
+    float acc = 0.0f;
+    bool fg = false;
+    if (x < gb.w && y < gb.h) {
+        const size_t k = (size_t)y * gb.w + x;
+        const float vx = gb.pos[k];
+        if (vx == 0.0f) {  // Shadow.frag:244 background test; the cleared target holds 0
+            out[k] = 0.0f;
+        } else {
+            fg = true;
+            const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
 #pragma unroll
-    for (int j = 0; j < PY; ++j) {
-#pragma unroll
-        for (int i = 0; i < PX; ++i) {
-            const int p = j * PX + i, x = lx + 8 * i, y = ly + 8 * j;
-            acc[p] = 0.0f;
-            fg[p] = false;
-            if (x < gb.w && y < gb.h) {
-                const size_t k = (size_t)y * gb.w + x;
-                const float vx = gb.pos[k];
-                if (vx == 0.0f) {  // Shadow.frag:244 background test; the cleared target holds 0
-                    out[k] = 0.0f;
-                } else {
-                    fg[p] = true;
-                    const float vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {  // AccurateSoftShadow.frag:70-73, v.w ignored
-                        float a = u.m[0 * 4 + c] * vx;
-                        a = a + u.m[1 * 4 + c] * vy;
-                        a = a + u.m[2 * 4 + c] * vz;
-                        cm[p][c] = a;
-                    }
-                }
+            for (int c = 0; c < 4; ++c) {  // AccurateSoftShadow.frag:70-73, v.w ignored
+                float a = u.m[0 * 4 + c] * vx;
+                a = a + u.m[1 * 4 + c] * vy;
+                a = a + u.m[2 * 4 + c] * vz;
+                cm[c] = a;
             }
         }
     }
@@ -1184,25 +1181,16 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft_wide(GBufferView gb, con
         float4 T = trans[s];
         float af = 1.0f;
         if constexpr (kAdaptive) af = adaptive_weight(T);
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            if (!fg[p]) continue;
-            const float term = soft_term<NT>(D, u, cm[p], T, incrWidth, incrHeight, qt);
+        if (fg) {
+            const float term = soft_term<NT>(D, u, cm, T, incrWidth, incrHeight, qt);
             if constexpr (kAdaptive)
-                acc[p] = acc[p] + term * af;
+                acc = acc + term * af;
             else
-                acc[p] = acc[p] + term;
+                acc = acc + term;
         }
         count = count + af;
     }
-#pragma unroll
-    for (int j = 0; j < PY; ++j) {
-#pragma unroll
-        for (int i = 0; i < PX; ++i) {
-            const int p = j * PX + i;
-            if (fg[p]) out[(size_t)(ly + 8 * j) * gb.w + (lx + 8 * i)] = acc[p] / count;
-        }
-    }
+    if (fg) out[(size_t)y * gb.w + x] = acc / count;
 }
 
 // One run of samples of a Monte-Carlo frame (SoftChunk).
@@ -1212,7 +1200,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft_chunk(GBufferView gb, co
                                                                   SoftUniforms u, SoftChunk ch,
                                                                   float* __restrict__ out) {
     int x, y;
-    pixel_of_block<kSpxSoft>(gb.w, gb.h, x, y);
+    soft_block_pixel(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     soft_pixel<NT, false, true>(gb, maps, smw, smh, trans, u, out, x, y, ch);
 }

@@ -1,9 +1,12 @@
-"""The screen passes' workgroup-to-block order (pixel_of_block<k> in
-globalillumination_amd/csrc/screen_kernels.hpp), restated: for every view
-size and every strip count k per XCD, the 8-XCD round-robin of workgroups
-followed by the strip-major order must visit each 16 x 16 block exactly once.
-With one strip per XCD each XCD's blocks lie in at most two neighbouring
-strips; with several, each XCD's blocks spread across the view."""
+"""The screen passes' workgroup-to-block order (pixel_of_block<k> and
+soft_block_pixel in globalillumination_amd/csrc/screen_kernels.hpp),
+restated: for every view size and every strip count k per XCD, the 8-XCD
+round-robin of workgroups followed by the strip-major order must visit each
+16 x 16 block exactly once.  With one strip per XCD each XCD's blocks lie in
+at most two neighbouring strips; with several, each XCD's blocks spread
+across the view.  The soft pass's 64 x 4 blocks of 16 x 4 waves must cover
+every pixel exactly once."""
+import numpy as np
 import pytest
 
 BLOCK = 16
@@ -69,3 +72,28 @@ def test_full_hd_gives_each_xcd_one_whole_strip():
     for b in range(120 * 68):
         x, y, xcd, k = block_of(b, 120, 68)
         assert k == xcd and 15 * xcd <= x < 15 * xcd + 15
+
+
+def soft_block_pixels(w, h, bw=64, wx=16, spx=4):
+    """Pixels of every thread of the soft pass's grid (soft_block_pixel)."""
+    bh, wy = 256 // bw, 64 // wx
+    bx_n, by_n = (w + bw - 1) // bw, (h + bh - 1) // bh
+    t = np.arange(256)
+    lane, wave = t & 63, t >> 6
+    ox = (wave % (bw // wx)) * wx + lane % wx
+    oy = (wave // (bw // wx)) * wy + lane // wx
+    xs, ys = [], []
+    for b in range(bx_n * by_n):
+        bx, by, _, _ = block_of(b, bx_n, by_n, spx)
+        xs.append(bx * bw + ox)
+        ys.append(by * bh + oy)
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (17, 5), (100, 257), (640, 480), (1920, 1080), (4000, 3), (3, 4000)])
+def test_soft_blocks_cover_each_pixel_once(w, h):
+    x, y = soft_block_pixels(w, h)
+    inside = (x < w) & (y < h)
+    k = y[inside].astype(np.int64) * w + x[inside]
+    assert len(k) == w * h
+    assert len(np.unique(k)) == w * h
