@@ -20,7 +20,7 @@ from . import _lib
 from .passes import DeviceMesh, GBuffer, SoftParams, _dptr, _stream_handle, render_monte_carlo, render_shadow_map, \
     soft_uniforms
 
-SHAPES = {0: (64, 4), 1: (32, 8), 2: (16, 16), 3: "16x16 of 8x8 wave tiles"}
+SHAPES = {0: (64, 4), 1: (32, 8), 2: (16, 16), 3: "16x16 of 8x8 wave tiles", 4: "64x4 of 16x4 wave tiles"}
 
 
 @dataclass

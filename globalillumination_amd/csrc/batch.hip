@@ -44,7 +44,8 @@ constexpr int kFillChunk = 4096;  // words cleared per fill item
 constexpr int kXcds = 8;
 constexpr int kScreenThreads = 256;
 constexpr int kNtGroups = 3;  // soft variants: generic loop, 3 taps per axis, 4 taps per axis
-constexpr int kShapes = 4;    // screen blocks 64x4, 32x8, 16x16 (row-major waves), 16x16 of 8x8 wave tiles
+constexpr int kShapes = 5;    // screen blocks 64x4, 32x8, 16x16 (row-major waves), 16x16 of 8x8 wave
+                              // tiles, 64x4 of 16x4 wave tiles
 constexpr int kCounters = 64;  // [0] raster queue, [1 + 8 g + x] screen queue of group g on XCD x,
                                // [63] the autotuner's overflow flag
 
@@ -231,7 +232,12 @@ __global__ __launch_bounds__(kScreenThreads) void kb_soft(SoftTable t) {
     const int xcd = (int)(blockIdx.x % kXcds);
     const uint32_t lo = t.xr[2 * xcd], hi = t.xr[2 * xcd + 1];
     int tx = (int)threadIdx.x % BX, ty = (int)threadIdx.x / BX;
-    if (kWaveTiles) wave_tile_xy((int)threadIdx.x, tx, ty);
+    if constexpr (kWaveTiles && BX == 64) {  // four 16x4 wave tiles side by side
+        tx = ((int)threadIdx.x >> 6) * 16 + ((int)threadIdx.x & 15);
+        ty = ((int)threadIdx.x & 63) >> 4;
+    } else if constexpr (kWaveTiles) {
+        wave_tile_xy((int)threadIdx.x, tx, ty);
+    }
     for (;;) {
         if (threadIdx.x == 0) s_item = lo + atomicAdd(&t.counters[xcd], 1u);
         __syncthreads();
@@ -251,8 +257,8 @@ __global__ __launch_bounds__(kScreenThreads) void kb_soft(SoftTable t) {
 // ---------------------------------------------------------------------------
 inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-const int kShapeX[kShapes] = {64, 32, 16, 16};
-const int kShapeY[kShapes] = {4, 8, 16, 16};
+const int kShapeX[kShapes] = {64, 32, 16, 16, 64};
+const int kShapeY[kShapes] = {4, 8, 16, 16, 4};
 const int kGroupNt[kNtGroups] = {0, 3, 4};
 const int kTiles[2] = {32, 64};
 
@@ -390,7 +396,8 @@ void launch_soft(int shape, unsigned grid, const SoftTable& tab, hipStream_t s) 
     case 0: hipLaunchKernelGGL((kb_soft<NT, 64, 4, false>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
     case 1: hipLaunchKernelGGL((kb_soft<NT, 32, 8, false>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
     case 2: hipLaunchKernelGGL((kb_soft<NT, 16, 16, false>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
-    default: hipLaunchKernelGGL((kb_soft<NT, 16, 16, true>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
+    case 3: hipLaunchKernelGGL((kb_soft<NT, 16, 16, true>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
+    default: hipLaunchKernelGGL((kb_soft<NT, 64, 4, true>), dim3(grid), dim3(kScreenThreads), 0, s, tab); break;
     }
 }
 

@@ -519,7 +519,8 @@ typedef struct gi_batch_tuning {
     int tile;          /* raster tile edge: 32 or 64 */
     int raster_groups; /* persistent workgroups per CU in the raster stages: 1..8 */
     int screen_shape;  /* screen-pass pixel block: 0 = 64x4, 1 = 32x8, 2 = 16x16 (waves are row-major
-                          strips), 3 = 16x16 made of four 8x8 wave tiles */
+                          strips), 3 = 16x16 made of four 8x8 wave tiles, 4 = 64x4 made of four 16x4
+                          wave tiles */
     int screen_groups; /* persistent workgroups per CU in the screen stage: 1..8 */
 } gi_batch_tuning;
 

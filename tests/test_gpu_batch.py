@@ -75,7 +75,7 @@ def small(cuda, gi):
     return insts, maps, outs
 
 
-@pytest.mark.parametrize("tile,rg,shape,sg", [(64, 4, 0, 4), (32, 1, 1, 2), (32, 8, 2, 1), (64, 2, 3, 8)])
+@pytest.mark.parametrize("tile,rg,shape,sg", [(64, 4, 0, 4), (32, 1, 1, 2), (32, 8, 2, 1), (64, 2, 3, 8), (64, 4, 4, 3)])
 def test_batch_matches_single_passes(small, tile, rg, shape, sg):
     from globalillumination_amd.batch import Batch
 
