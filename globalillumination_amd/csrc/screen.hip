@@ -181,7 +181,7 @@ int gi_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_dev, int ns
     GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
-    dim3 grid(soft_blocks(gb.w, gb.h)), block(kBlockThreads);  // the soft pass's block shape
+    dim3 grid(soft_blocks(gb.w, gb.h)), block(kSoftThreads);  // the soft pass's block shape
     switch (soft_variant(u)) {
     case 3: hipLaunchKernelGGL((k_soft_wide<3>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
     case 4: hipLaunchKernelGGL((k_soft_wide<4>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev); break;
@@ -202,7 +202,7 @@ int gi_soft_shadow_pass_chunk(const gi_gbuffer* gbuffer, const float* maps_dev, 
     GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
-    dim3 grid(soft_blocks(gb.w, gb.h)), block(kBlockThreads);  // the soft pass's block shape
+    dim3 grid(soft_blocks(gb.w, gb.h)), block(kSoftThreads);  // the soft pass's block shape
     switch (soft_variant(u)) {
     case 3: hipLaunchKernelGGL(k_soft_chunk<3>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, ch, out_dev); break;
     case 4: hipLaunchKernelGGL(k_soft_chunk<4>, grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, ch, out_dev); break;
@@ -220,7 +220,7 @@ int gi_adaptive_soft_shadow_pass(const gi_gbuffer* gbuffer, const float* maps_de
     GBufferView gb{gbuffer->position_dev, nullptr, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const float4* tr = reinterpret_cast<const float4*>(trans_dev);
-    dim3 grid(soft_blocks(gb.w, gb.h)), block(kBlockThreads);  // the soft pass's block shape
+    dim3 grid(soft_blocks(gb.w, gb.h)), block(kSoftThreads);  // the soft pass's block shape
     switch (soft_variant(u)) {
     case 3:
         hipLaunchKernelGGL((k_soft_wide<3, true>), grid, block, 0, s, gb, maps_dev, smw, smh, tr, u, out_dev);

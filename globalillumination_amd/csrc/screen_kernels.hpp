@@ -1107,8 +1107,11 @@ __device__ __forceinline__ void soft_pixel(const GBufferView& gb, const float* _
 // with 16x16 blocks of 8x8 waves, 0.358 ms with 32x8 blocks of 16x4 waves,
 // 0.362 ms with 64x4 blocks of 32x2 waves, 0.402 ms with 64x1 waves and
 // 0.417 ms with 128x2 blocks; 2 or 8 strips per XCD instead of 4 took
-// 0.394 and 0.360 ms.
-constexpr int kSoftBW = 64, kSoftBH = kBlockThreads / kSoftBW;
+// 0.394 and 0.360 ms.  Larger workgroups straddle fewer lines still (the
+// model: 1.33 GB for 64x8, 1.23 GB for 128x8) but ran slower: 0.386 ms for
+// 64x8, 0.413 for 128x4, 0.483 for 128x8, 0.492 for 256x4.
+constexpr int kSoftBW = 64, kSoftBH = 4;
+constexpr int kSoftThreads = kSoftBW * kSoftBH;
 constexpr int kSoftWX = 16, kSoftWY = 64 / kSoftWX;
 constexpr int kSpxSoftWide = 4;
 static_assert(kSoftBW % kSoftWX == 0 && kSoftBH % kSoftWY == 0, "wave tiles must tile the block");
@@ -1127,7 +1130,7 @@ __device__ __forceinline__ void soft_block_pixel(int w, int h, int& x, int& y) {
 // One thread per pixel, any tap count.  Blocks are ordered XCD-aware
 // (soft_block_pixel) so each XCD's L2 serves one contiguous band of the view.
 template <int NT, bool kAdaptive = false>
-__global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
+__global__ __launch_bounds__(kSoftThreads) void k_soft(GBufferView gb, const float* __restrict__ maps, int smw,
                                                             int smh, const float4* __restrict__ trans,
                                                             SoftUniforms u, float* __restrict__ out) {
     int x, y;
@@ -1138,7 +1141,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft(GBufferView gb, const fl
 
 // Every pixel adds its samples in order, as soft_pixel does (same bits).
 template <int NT, bool kAdaptive = false>
-__global__ __launch_bounds__(kBlockThreads) void k_soft_wide(GBufferView gb, const float* __restrict__ maps, int smw,
+__global__ __launch_bounds__(kSoftThreads) void k_soft_wide(GBufferView gb, const float* __restrict__ maps, int smw,
                                                                  int smh, const float4* __restrict__ trans,
                                                                  SoftUniforms u, float* __restrict__ out) {
     int x, y;
@@ -1195,7 +1198,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_soft_wide(GBufferView gb, con
 
 // One run of samples of a Monte-Carlo frame (SoftChunk).
 template <int NT>
-__global__ __launch_bounds__(kBlockThreads) void k_soft_chunk(GBufferView gb, const float* __restrict__ maps, int smw,
+__global__ __launch_bounds__(kSoftThreads) void k_soft_chunk(GBufferView gb, const float* __restrict__ maps, int smw,
                                                                   int smh, const float4* __restrict__ trans,
                                                                   SoftUniforms u, SoftChunk ch,
                                                                   float* __restrict__ out) {
