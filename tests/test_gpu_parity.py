@@ -374,11 +374,14 @@ def test_rbsm_soup(gi, oracle_lib, cuda, technique):
 
 
 @pytest.mark.parametrize("technique", ["rpcf", "rbsm_filtered"])
-@pytest.mark.parametrize("order,penumbra,pre_eval", [(1, 1, False), (2, 3, True), (5, 1, False), (9, 2, False)])
+@pytest.mark.parametrize("order,penumbra,pre_eval", [(1, 1, False), (2, 3, True), (5, 1, False), (7, 1, True),
+                                                     (9, 2, False)])
 def test_rpcf_tap_loops(gi, oracle_lib, cuda, technique, order, penumbra, pre_eval):
     """RPCF and FilteredRBSM tap loops of other sizes (the float loop's tap
-    count: 2, 3, 6 and 10 per axis): one lane per tap, 16, 7 or 1 pixels per
-    wave, and 100 taps in passes of 64."""
+    count: 2, 3, 6, 8 and 10 per axis): one lane per tap, 16, 7 or 1 pixels
+    per wave group with three groups' discontinuity taps packed for the edge
+    walks (64 taps fill the wave), and 100 taps in passes of 64 on the
+    one-pixel-per-wave kernel."""
     mesh = shrink_soup(30_000)
     wl = gi.scenes.config2(ntris=10, view_w=640, view_h=360, sm=1024)
     wl.mesh = mesh
@@ -576,16 +579,23 @@ def test_rev_node_test_matches_oracle(gi, oracle_lib, cuda):
 
 
 @pytest.mark.parametrize("rpcf,si,nsamples", [(False, 0.25, 9), (True, 0.25, 9), (False, 0.5, 9),
-                                               (False, 0.25, 81), (True, 0.25, 81)])
+                                               (False, 0.25, 81), (True, 0.25, 81), (True, 0.25, 4),
+                                               (True, 0.5, 16), (True, 0.25, 36)])
 def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si, nsamples):
     """RevectorizationBasedAccurateSoftShadow.frag with mixed quad-tree levels,
     discontinuity layers for the first 6 maps (the rest read a clamped or
     stale layer as the reference does), and the visibility classification of
-    one level-0 node test.  9 samples put 7 pixels in a wave's lanes; 81 run
-    one pixel in two passes of 64 and 17 samples."""
+    one level-0 node test.  Binary: 9 samples put 7 pixels in a wave's lanes,
+    81 run one pixel in two passes of 64 and 17 samples.  RPCF (28
+    pixel-sample pairs per workgroup pass): 4 samples give 7 pixels per pass,
+    9 three, 16 one pixel with 12 pairs idle, 36 one pixel in passes of 28
+    and 8, 81 one pixel in passes of 28, 28 and 25.  The node test takes the
+    light's four corner samples."""
     wl, g, mvps, maps, biased, MV, nm, lpos = _rev_frame(gi, nsamples=nsamples)
     gp, gn = _np(g.position), _np(g.normal)
-    layers = 6
+    layers = min(6, nsamples)
+    k = int(round(nsamples ** 0.5))
+    corners = [0, k - 1, k * (k - 1), k * k - 1]
     disc = gi.passes.rev_discontinuity_layers(g, layers=layers)
     disc_np = np.zeros((layers, g.height, g.width), np.uint8)
     for i in range(layers):
@@ -596,8 +606,8 @@ def test_rev_soft_matches_oracle(gi, oracle_lib, cuda, rpcf, si, nsamples):
         disc_np[i] = oracle_lib.rev_discontinuity(gp, gn, _np(maps[i]), op)
     vis = torch.zeros((g.height, g.width), dtype=torch.float32, device="cuda")
     vis_np = np.zeros((g.height, g.width), np.float32)
-    gi.passes.rev_quadtree_node_test(g, maps, [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc, 0, vis)
-    oracle_lib.rev_node_test(gp, _np(maps), [0, 2, 6, 8], biased[[0, 2, 6, 8]], disc_np, 0, vis_np)
+    gi.passes.rev_quadtree_node_test(g, maps, corners, biased[corners], disc, 0, vis)
+    oracle_lib.rev_node_test(gp, _np(maps), corners, biased[corners], disc_np, 0, vis_np)
     levels = ([0, 1, 2, 3, 4, 1, 6, 2, 0] * 9)[:nsamples]
     common, trans = gi.passes.adaptive_trans(mvps, levels)
     params = gi.RevSoftParams(MV, nm, lpos, shadowIntensity=si, RPCF=rpcf, maxSearch=16, depthThreshold=2.5e-6)
