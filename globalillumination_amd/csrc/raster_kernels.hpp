@@ -918,28 +918,32 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
         if (tid == 0) nrec = 0;
         __syncthreads();
     }
-    // Write the tile once.
+    // Write the tile once.  Depth maps go out with non-temporal stores: a
+    // frame writes far more map bytes than the caches hold (64 x 16 MiB), and
+    // with ordinary stores the last-written lines stay dirty in the MALL and
+    // are written back while the screen pass reads the maps.  On the
+    // 64-sample 1080p frame this took the raster from 0.363 to 0.346 ms, the
+    // soft pass that follows from 0.374 to 0.341 ms and the frame from 0.991
+    // to 0.940 ms.
     const size_t plane = (size_t)cfg.w * cfg.h;
     if constexpr (!kVis) {
         float* dst = reinterpret_cast<float*>(out) + (size_t)map * plane;
         const bool full = (x1 - x0 + 1 == TS) && ((cfg.w & 3) == 0);
         if (full) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
             for (int q = tid; q < kTilePx / 4; q += kRasterThreads) {
                 int r = q / (TS / 4), cidx = (q % (TS / 4)) * 4;
                 int j = y0 + r;
                 if (j > y1) continue;
-                float4 v;
-                v.x = __uint_as_float(tile[r * TS + cidx + 0]);
-                v.y = __uint_as_float(tile[r * TS + cidx + 1]);
-                v.z = __uint_as_float(tile[r * TS + cidx + 2]);
-                v.w = __uint_as_float(tile[r * TS + cidx + 3]);
-                *reinterpret_cast<float4*>(dst + (size_t)j * cfg.w + x0 + cidx) = v;
+                const f4 v = {__uint_as_float(tile[r * TS + cidx + 0]), __uint_as_float(tile[r * TS + cidx + 1]),
+                              __uint_as_float(tile[r * TS + cidx + 2]), __uint_as_float(tile[r * TS + cidx + 3])};
+                __builtin_nontemporal_store(v, reinterpret_cast<f4*>(dst + (size_t)j * cfg.w + x0 + cidx));
             }
         } else {
             for (int p = tid; p < kTilePx; p += kRasterThreads) {
                 int i = x0 + p % TS, j = y0 + p / TS;
                 if (i > x1 || j > y1) continue;
-                dst[(size_t)j * cfg.w + i] = __uint_as_float(tile[p]);
+                __builtin_nontemporal_store(__uint_as_float(tile[p]), dst + (size_t)j * cfg.w + i);
             }
         }
     } else {
