@@ -105,7 +105,8 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
                       p->penumbra_size > 64))
         return GI_EINVAL;
     if (p->technique >= GI_RBSM && (p->max_search < 0 || p->max_search > 4096)) return GI_EINVAL;
-    const bool listed = p->technique == GI_RPCF || p->technique == GI_RBSM_FILTERED;
+    const bool listed =
+        p->technique == GI_RPCF || p->technique == GI_RBSM_FILTERED || p->technique == GI_RPCF_CONSERVATIVE;
     if (listed) {
         if (!workspace_dev || (reinterpret_cast<uintptr_t>(workspace_dev) & 15u) != 0) return GI_EINVAL;
         if (workspace_bytes < hard_list_bytes(gbuffer->width, gbuffer->height)) return GI_ENOSPACE;
@@ -143,20 +144,27 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
         if (p->technique == GI_RPCF) {
             hipLaunchKernelGGL(k_hard_classify<GI_RPCF>, grid, block, 0, s, gb, u, out_dev, count, list);
             if (wide)
-                hipLaunchKernelGGL(k_hard_taps_wide<GI_RPCF>, dim3(expand_blocks<k_hard_taps_wide<GI_RPCF>>()), block,
-                                   0, s, gb, D, u, out_dev, count, list);
+                hipLaunchKernelGGL(k_hard_taps_lanes<GI_RPCF>, dim3(expand_blocks<k_hard_taps_lanes<GI_RPCF>>()),
+                                   block, 0, s, gb, D, u, out_dev, count, list);
             else
                 hipLaunchKernelGGL(k_hard_taps<GI_RPCF>, dim3(expand_blocks<k_hard_taps<GI_RPCF>>()), block, 0, s,
                                    gb, D, u, out_dev, count, list);
-        } else {
+        } else if (p->technique == GI_RBSM_FILTERED) {
             hipLaunchKernelGGL(k_hard_classify<GI_RBSM_FILTERED>, grid, block, 0, s, gb, u, out_dev, count, list);
             if (wide)
-                hipLaunchKernelGGL(k_hard_taps_wide<GI_RBSM_FILTERED>,
-                                   dim3(expand_blocks<k_hard_taps_wide<GI_RBSM_FILTERED>>()), block, 0, s, gb, D, u,
+                hipLaunchKernelGGL(k_hard_taps_lanes<GI_RBSM_FILTERED>,
+                                   dim3(expand_blocks<k_hard_taps_lanes<GI_RBSM_FILTERED>>()), block, 0, s, gb, D, u,
                                    out_dev, count, list);
             else
                 hipLaunchKernelGGL(k_hard_taps<GI_RBSM_FILTERED>, dim3(expand_blocks<k_hard_taps<GI_RBSM_FILTERED>>()),
                                    block, 0, s, gb, D, u, out_dev, count, list);
+        } else {
+            // ConservativeSMSR's RPCF: a lane per tap at any loop size
+            hipLaunchKernelGGL(k_hard_classify<GI_RPCF_CONSERVATIVE>, grid, block, 0, s, gb, u, out_dev, count,
+                               list);
+            hipLaunchKernelGGL(k_hard_taps_lanes<GI_RPCF_CONSERVATIVE>,
+                               dim3(expand_blocks<k_hard_taps_lanes<GI_RPCF_CONSERVATIVE>>()), block, 0, s, gb, D, u,
+                               out_dev, count, list);
         }
         return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
     }
@@ -164,10 +172,9 @@ int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev, const
     case GI_NAIVE: hipLaunchKernelGGL(k_hard<GI_NAIVE>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_PCF: hipLaunchKernelGGL(k_hard<GI_PCF>, grid, block, 0, s, gb, D, u, out_dev); break;
     case GI_RBSM: hipLaunchKernelGGL(k_hard<GI_RBSM>, grid, block, 0, s, gb, D, u, out_dev); break;
-    case GI_RBSM_CONSERVATIVE:
+    default:  // GI_RBSM_CONSERVATIVE (the other techniques are listed above)
         hipLaunchKernelGGL(k_hard<GI_RBSM_CONSERVATIVE>, grid, block, 0, s, gb, D, u, out_dev);
         break;
-    default: hipLaunchKernelGGL(k_hard<GI_RPCF_CONSERVATIVE>, grid, block, 0, s, gb, D, u, out_dev); break;
     }
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }

@@ -397,21 +397,8 @@ struct Rbsm {
         if (rpcf_front(x, y, z, si, w, h, t, v)) return gmix(rpcf_walks(t, z), 1.0f, si);
         return v;
     }
-
-    // computeShadowFromAccurateRPCF :306-349
-    __device__ float rpcf(float x, float y, float z, float si, int kernel_order, int penumbra) const {
-        float illum = 0.0f;
-        float offset = (float)penumbra;
-        float stepSize = 2.0f * offset / (float)kernel_order;
-        int count = 0;
-        for (float w = -offset; w <= offset; w += stepSize) {
-            for (float h = -offset; h <= offset; h += stepSize) {
-                illum = illum + rpcf_tap(x, y, z, si, w, h);
-                count++;
-            }
-        }
-        return illum / (float)count;
-    }
+    // computeShadowFromAccurateRPCF's loop (:306-349) runs over the pixel
+    // list: k_hard_classify, then k_hard_taps / k_hard_taps_lanes.
 };
 
 // ---------------------------------------------------------------------------
@@ -501,34 +488,24 @@ struct RbsmConservative {
         return revectorized(x, y, z, gfract(x * D.fw), gfract(y * D.fh));
     }
 
-    // computeShadowFromRPCF :146-198
-    __device__ float rpcf(float x, float y, float z, int kernel_order, int penumbra) const {
+    // One tap of computeShadowFromRPCF's loop (:146-198) at offsets (w, h)
+    // in texels.
+    __device__ float rpcf_tap(float x, float y, float z, float w, float h) const {
         const float incrWidth = 1.0f / D.fw;
         const float incrHeight = 1.0f / D.fh;
-        float illum = 0.0f;
-        const float offset = (float)penumbra;
-        const float stepSize = 2.0f * offset / (float)kernel_order;
-        int count = 0;
-        for (float w = -offset; w <= offset; w += stepSize) {
-            for (float h = -offset; h <= offset; h += stepSize) {
-                const float px = x + w * incrWidth;
-                const float py = y + h * incrHeight;
-                const float shadow = (z <= tex(px, py)) ? 1.0f : si;
-                if (shadow == 1.0f) {
-                    float lr[4];
-                    disc(px, py, z, lr);
-                    if (lr[0] == 0.0f && lr[1] == 0.0f)
-                        illum = illum + 1.0f;
-                    else
-                        illum = illum + revectorized(px, py, z, gfract(px * D.fw), gfract(py * D.fh));
-                } else {
-                    illum = illum + si;
-                }
-                count++;
-            }
+        const float px = x + w * incrWidth;
+        const float py = y + h * incrHeight;
+        const float shadow = (z <= tex(px, py)) ? 1.0f : si;
+        if (shadow == 1.0f) {
+            float lr[4];
+            disc(px, py, z, lr);
+            if (lr[0] == 0.0f && lr[1] == 0.0f) return 1.0f;
+            return revectorized(px, py, z, gfract(px * D.fw), gfract(py * D.fh));
         }
-        return illum / (float)count;
+        return si;
     }
+    // computeShadowFromRPCF's loop (:146-198) runs over the pixel list:
+    // k_hard_classify, then k_hard_taps_lanes.
 };
 
 // Pixel index with an XCD-aware block order.  Hardware hands consecutive
@@ -591,12 +568,16 @@ __device__ __forceinline__ void pixel_of_block(int w, int h, int& x, int& y) {
 // and 4 strips (soft_block_pixel).
 constexpr int kSpxFiltered = 2;
 
+// The techniques without a tap loop of revectorized taps (naive, PCF, RBSM,
+// conservative RBSM): a thread per pixel.  RPCF, FilteredRBSM and
+// conservative RPCF run over a pixel list (k_hard_classify).
 template <int kTech>
 __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap D, HardUniforms u,
                                                             float* __restrict__ out) {
+    static_assert(kTech == GI_NAIVE || kTech == GI_PCF || kTech == GI_RBSM || kTech == GI_RBSM_CONSERVATIVE,
+                  "the listed techniques run k_hard_classify");
     int x, y;
-    constexpr bool kFiltered = kTech == GI_RPCF || kTech == GI_RBSM_FILTERED || kTech == GI_RPCF_CONSERVATIVE;
-    pixel_of_block<kFiltered ? kSpxFiltered : 1>(gb.w, gb.h, x, y);
+    pixel_of_block(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     const size_t plane = (size_t)gb.w * gb.h;
     const size_t k = (size_t)y * gb.w + x;
@@ -621,23 +602,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard(GBufferView gb, DepthMap
                 shadow = pcf(D, u, n0, n1, n2);
             }
         }
-    } else if constexpr (kTech == GI_RBSM_CONSERVATIVE || kTech == GI_RPCF_CONSERVATIVE) {
+    } else if constexpr (kTech == GI_RBSM_CONSERVATIVE) {
         // ConservativeSMSR.frag:215-235 (no shadowCoord.w test)
         RbsmConservative rb{D, u.stepx, u.stepy, u.thr, u.max_search, u.si};
-        if (shadow == 1.0f) {
-            if constexpr (kTech == GI_RBSM_CONSERVATIVE)
-                shadow = rb.smsr(n0, n1, n2);
-            else
-                shadow = rb.rpcf(n0, n1, n2, u.kernel_order, u.penumbra);
-        }
+        if (shadow == 1.0f) shadow = rb.smsr(n0, n1, n2);
     } else {
-        Rbsm<kTech == GI_RBSM_FILTERED> rb{D, u.stepx, u.stepy, u.thr, u.max_search};
-        if (shadow == 1.0f) {
-            if constexpr (kTech == GI_RBSM)
-                shadow = rb.smsr(n0, n1, n2, u.si);
-            else
-                shadow = rb.rpcf(n0, n1, n2, u.si, u.kernel_order, u.penumbra);
-        }
+        Rbsm<false> rb{D, u.stepx, u.stepy, u.thr, u.max_search};
+        if (shadow == 1.0f) shadow = rb.smsr(n0, n1, n2, u.si);
     }
     out[k] = shadow;
 }
@@ -699,36 +670,54 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard_classify(GBufferView gb,
 // tap.  The taps are summed in the loop's order (w, then h) and divided by
 // their count, as the shader does.
 //
-// Loops of more than 64 taps: one pixel per wave, in passes of 64 taps.
+// A lane per tap without packing: 64 / taps pixels per wave group when the
+// loop has at most 64 taps, else one pixel in passes of 64 taps.  Serves
+// RPCF and FilteredRBSM loops of more than 64 taps, and ConservativeSMSR's
+// RPCF (RbsmConservative::rpcf_tap) at any size.
 template <int kTech>
-__global__ __launch_bounds__(kBlockThreads) void k_hard_taps_wide(GBufferView gb, DepthMap D, HardUniforms u,
-                                                                   float* __restrict__ out,
-                                                                   const unsigned* __restrict__ count,
-                                                                   const unsigned* __restrict__ list) {
-    using R = Rbsm<kTech == GI_RBSM_FILTERED>;
+__device__ __forceinline__ float hard_tap(const DepthMap& D, const HardUniforms& u, float x, float y, float z,
+                                          float w, float h) {
+    if constexpr (kTech == GI_RPCF_CONSERVATIVE) {
+        const RbsmConservative rb{D, u.stepx, u.stepy, u.thr, u.max_search, u.si};
+        return rb.rpcf_tap(x, y, z, w, h);
+    } else {
+        const Rbsm<kTech == GI_RBSM_FILTERED> rb{D, u.stepx, u.stepy, u.thr, u.max_search};
+        return rb.rpcf_tap(x, y, z, u.si, w, h);
+    }
+}
+template <int kTech>
+__global__ __launch_bounds__(kBlockThreads) void k_hard_taps_lanes(GBufferView gb, DepthMap D, HardUniforms u,
+                                                                    float* __restrict__ out,
+                                                                    const unsigned* __restrict__ count,
+                                                                    const unsigned* __restrict__ list) {
     const unsigned n = *count;
     const int side = rpcf_side(u);
     const int ntaps = side * side;
+    const int per = ntaps < 64 ? ntaps : 64;
+    const int P = 64 / per;
     const int lane = (int)(threadIdx.x & 63);
+    const int slot = lane / per, sl = lane - slot * per;
+    const int g0 = (slot < P ? slot : P - 1) * per;
     const size_t plane = (size_t)gb.w * gb.h;
-    const R rb{D, u.stepx, u.stepy, u.thr, u.max_search};
-    for (WaveGroups wg(n, 1u); wg.more(); wg.advance()) {
-        const size_t k = list[wg.base];
+    for (WaveGroups wg(n, (unsigned)P); wg.more(); wg.advance()) {
+        const unsigned qi = wg.base + (unsigned)slot;
+        const bool has = slot < P && qi < n;
+        const size_t k = list[has ? qi : wg.base];
         const float vx = gb.pos[k], vy = gb.pos[plane + k], vz = gb.pos[2 * plane + k];
         float sc[4];
         mat4_vec4(u.light_mvp.m, vx, vy, vz, 1.0f, sc);
         const float n0 = sc[0] / sc[3], n1 = sc[1] / sc[3], n2 = sc[2] / sc[3];
         float illum = 0.0f;
-        for (int t0 = 0; t0 < ntaps; t0 += 64) {
-            const int t = t0 + lane;
+        for (int t0 = 0; t0 < ntaps; t0 += per) {
+            const int t = t0 + sl;
             float v = 0.0f;
-            if (t < ntaps) {
+            if (has && t < ntaps) {
                 const int tw = t / side;
-                v = rb.rpcf_tap(n0, n1, n2, u.si, rpcf_offset(u, tw), rpcf_offset(u, t - tw * side));
+                v = hard_tap<kTech>(D, u, n0, n1, n2, rpcf_offset(u, tw), rpcf_offset(u, t - tw * side));
             }
-            illum = ordered_sum(illum, v, 0, ntaps - t0 < 64 ? ntaps - t0 : 64);
+            illum = ordered_sum(illum, v, g0, ntaps - t0 < per ? ntaps - t0 : per);
         }
-        if (lane == 0) out[k] = illum / (float)ntaps;
+        if (has && sl == 0) out[k] = illum / (float)ntaps;
     }
 }
 
@@ -756,7 +745,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_hard_taps(GBufferView gb, Dep
     __shared__ float pk[kBlockThreads / 64][4][64 * kTapPack];
     const unsigned n = *count;
     const int side = rpcf_side(u);
-    const int per = side * side;  // at most 64: the host launches k_hard_taps_wide beyond
+    const int per = side * side;  // at most 64: the host launches k_hard_taps_lanes beyond
     if (per > 64) return;
     const int P = 64 / per;
     const int lane = (int)(threadIdx.x & 63), wave = (int)(threadIdx.x >> 6);

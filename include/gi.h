@@ -218,11 +218,11 @@ size_t gi_hard_shadow_workspace_size(int width, int height);
  * (ShadowMapping/src/main.cpp:400-414, :302-348).  depth_dev: one map of
  * shadow_map_height x shadow_map_width; out_dev: width*height floats holding
  * the R channel (1 = lit, shadow_intensity = shadowed, 0 = background).
- * GI_RPCF and GI_RBSM_FILTERED list the pixels that run their tap loop in the
- * workspace (16-byte aligned, at least gi_hard_shadow_workspace_size bytes,
- * else GI_ENOSPACE; NULL is GI_EINVAL) and spread their taps over the GPU;
- * the other techniques ignore it (NULL is fine).  Calls sharing a workspace
- * must not overlap. */
+ * GI_RPCF, GI_RBSM_FILTERED and GI_RPCF_CONSERVATIVE list the pixels that
+ * run their tap loop in the workspace (16-byte aligned, at least
+ * gi_hard_shadow_workspace_size bytes, else GI_ENOSPACE; NULL is GI_EINVAL)
+ * and spread their taps over the GPU; the other techniques ignore it (NULL is
+ * fine).  Calls sharing a workspace must not overlap. */
 int gi_hard_shadow_pass(const gi_gbuffer* gbuffer, const float* depth_dev,
                         const gi_shadow_params* params, float* out_dev, void* workspace_dev,
                         size_t workspace_bytes, gi_stream stream);

@@ -107,11 +107,12 @@ def test_hard_pass_argument_checks(gi):
     p.pre_evaluation = 1  # needs normals
     assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_EINVAL
     assert lib.gi_hard_shadow_pass(None, _fake(), ctypes.byref(p), _fake(2), None, 0, None) == _lib.GI_EINVAL
-    # RPCF and FilteredRBSM need the workspace: missing, misaligned or short
+    # RPCF, FilteredRBSM and conservative RPCF need the workspace: missing,
+    # misaligned or short
     p.pre_evaluation = 0
     assert lib.gi_hard_shadow_workspace_size(64, 32) == 256 + 64 * 32 * 4
     assert lib.gi_hard_shadow_workspace_size(0, 32) == 0
-    for tech in (_lib.GI_RPCF, _lib.GI_RBSM_FILTERED):
+    for tech in (_lib.GI_RPCF, _lib.GI_RBSM_FILTERED, _lib.GI_RPCF_CONSERVATIVE):
         p.technique = tech
         assert lib.gi_hard_shadow_pass(ctypes.byref(g), _fake(), ctypes.byref(p), _fake(2), None, 1 << 20,
                                        None) == _lib.GI_EINVAL

@@ -373,15 +373,15 @@ def test_rbsm_soup(gi, oracle_lib, cuda, technique):
     assert bits_equal(got, want), mismatch_report(got, want, technique)
 
 
-@pytest.mark.parametrize("technique", ["rpcf", "rbsm_filtered"])
+@pytest.mark.parametrize("technique", ["rpcf", "rbsm_filtered", "rpcf_conservative"])
 @pytest.mark.parametrize("order,penumbra,pre_eval", [(1, 1, False), (2, 3, True), (5, 1, False), (7, 1, True),
                                                      (9, 2, False)])
 def test_rpcf_tap_loops(gi, oracle_lib, cuda, technique, order, penumbra, pre_eval):
-    """RPCF and FilteredRBSM tap loops of other sizes (the float loop's tap
-    count: 2, 3, 6, 8 and 10 per axis): one lane per tap, 16, 7 or 1 pixels
-    per wave group with three groups' discontinuity taps packed for the edge
-    walks (64 taps fill the wave), and 100 taps in passes of 64 on the
-    one-pixel-per-wave kernel."""
+    """RPCF, FilteredRBSM and conservative RPCF tap loops of other sizes (the
+    float loop's tap count: 2, 3, 6, 8 and 10 per axis): one lane per tap,
+    16, 7 or 1 pixels per wave group (RPCF and FilteredRBSM pack three
+    groups' discontinuity taps for the edge walks; 64 taps fill the wave),
+    and 100 taps in passes of 64."""
     mesh = shrink_soup(30_000)
     wl = gi.scenes.config2(ntris=10, view_w=640, view_h=360, sm=1024)
     wl.mesh = mesh
