@@ -182,7 +182,7 @@ __global__ void kb_overflow_any(RasterTable t, uint32_t* __restrict__ flag) {
 // (instance, map, tile) items from a work queue: tiles differ widely in cost.
 template <int TS>
 __global__ __launch_bounds__(kRasterThreads) void kb_raster(RasterTable t) {
-    __shared__ uint32_t tile[TS * TS];
+    __shared__ __align__(16) uint32_t tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     __shared__ uint32_t s_item;

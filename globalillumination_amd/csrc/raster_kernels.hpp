@@ -852,7 +852,12 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
     const int x0 = tx * TS, y0 = ty * TS;
     const int x1 = min(x0 + TS - 1, cfg.w - 1), y1 = min(y0 + TS - 1, cfg.h - 1);
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
-    for (int p = tid; p < kTilePx; p += kRasterThreads) tile[p] = KeyOf<kVis>::clear;
+    if constexpr (!kVis) {  // 16-byte LDS writes
+        constexpr uint32_t c = KeyOf<kVis>::clear;
+        for (int q = tid; q < kTilePx / 4; q += kRasterThreads) reinterpret_cast<uint4*>(tile)[q] = make_uint4(c, c, c, c);
+    } else {
+        for (int p = tid; p < kTilePx; p += kRasterThreads) tile[p] = KeyOf<kVis>::clear;
+    }
     if (tid == 0) nrec = 0;
     float mvp[16];
     mvps.load(map, mvp);
@@ -967,7 +972,7 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
 template <int TS, bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSource mvps, RasterCfg cfg, BinBufs b,
                                                            void* out, int map_base, int map_count) {
-    __shared__ typename KeyOf<kVis>::T tile[TS * TS];
+    __shared__ __align__(16) typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     const int map = map_base + (int)(blockIdx.x % (unsigned)map_count);
@@ -980,7 +985,7 @@ __global__ __launch_bounds__(kRasterThreads) void k_raster(MeshView mesh, MvpSou
 template <int TS, bool kVis>
 __global__ __launch_bounds__(kRasterThreads) void k_raster_fallback(MeshView mesh, MvpSource mvps, RasterCfg cfg,
                                                                     BinBufs b, void* out, int nmaps) {
-    __shared__ typename KeyOf<kVis>::T tile[TS * TS];
+    __shared__ __align__(16) typename KeyOf<kVis>::T tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
     // The usual case is no overflowed map: the threads read the flags
