@@ -696,9 +696,12 @@ def _pcss_scene(gi, sm):
     return wl
 
 
-@pytest.mark.parametrize("sm,radius,znear,pre_eval", [(1024, 8, 1, True), (2048, 8, 1, False),
-                                                      (2048, 256, 1, True), (1024, 64, 0, True)])
-def test_pcss_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval):
+@pytest.mark.parametrize("sm,radius,znear,pre_eval,si,ks", [(1024, 8, 1, True, 0.25, 15), (2048, 8, 1, False, 0.25, 15),
+                                                            (2048, 256, 1, True, 0.25, 15), (1024, 64, 0, True, 0.25, 15),
+                                                            (2048, 256, 1, True, 0.3, 15), (2048, 256, 1, True, 0.25, 17)])
+def test_pcss_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval, si, ks):
+    """PlausibleSoftShadow.frag with PCSS == 1, also with a shadow intensity
+    off the 2^-16 grid (0.3) and a 17 x 17 PCF kernel."""
     wl = _pcss_scene(gi, sm)
     dm = gi.DeviceMesh.from_mesh(wl.mesh)
     g = gi.render_gbuffer(dm, camera_mvp(wl), wl.camera.width, wl.camera.height)
@@ -706,18 +709,19 @@ def test_pcss_matches_oracle(gi, oracle_lib, cuda, sm, radius, znear, pre_eval):
     depth = gi.render_shadow_map(dm, mvps, sm, sm, polygon_offset=(4.0, 20.0))
     MV, nm, lpos = gi.phong_uniforms(wl.camera, wl.lights[0].eye)
     params = gi.PCSSParams(lightMVP=mvps[0], shadowMapWidth=sm, shadowMapHeight=sm, lightSourceRadius=radius,
-                           zNear=znear, preEvaluation=pre_eval, MV=MV, normalMatrix=nm, lightPosition=lpos)
+                           zNear=znear, preEvaluation=pre_eval, MV=MV, normalMatrix=nm, lightPosition=lpos,
+                           shadowIntensity=si, kernelSize=ks)
     got = _np(gi.render_soft_shadows(g, depth[0], params))
-    op = oracle_lib.make_params(gi.biased(mvps[0]), sm, sm, "naive", shadow_intensity=0.25,
+    op = oracle_lib.make_params(gi.biased(mvps[0]), sm, sm, "naive", shadow_intensity=si,
                                 pre_evaluation=pre_eval, mv=MV, normal_matrix=nm, light_position=lpos)
     want = oracle_lib.pcss(_np(g.position), _np(g.normal) if pre_eval else None, _np(depth[0]), op,
-                           blocker_search_size=7, kernel_size=15, light_source_radius=radius, z_near=znear)
+                           blocker_search_size=7, kernel_size=ks, light_source_radius=radius, z_near=znear)
     assert bits_equal(got, want), mismatch_report(got, want, f"pcss sm={sm} radius={radius} zNear={znear}")
     geo = _np(g.position)[0] != 0
     if znear == 0 and not pre_eval:
         assert (want[geo] == 1.0).all()
     if radius == 256:
-        pen = (want > 0.25) & (want < 1.0)
+        pen = (want > si) & (want < 1.0)
         assert pen.mean() > 0.001  # a penumbra exists
 
 
