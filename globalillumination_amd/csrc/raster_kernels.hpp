@@ -73,7 +73,12 @@ struct CV {
 
 template <int NA>
 struct Tri {
-    long long X[3], Y[3];
+    // 24.8 fixed-point window coordinates: within the guard band (|x_ndc| <=
+    // kGuard) and a view of at most GI_MAX_EDGE pixels they stay below 2^24
+    // in magnitude, so they, their differences and pixel-centre offsets fit
+    // in int32; edge functions and areas are formed as 32 x 32 -> 64-bit
+    // products.
+    int X[3], Y[3];
     float w[3];
     float a[3][NA > 0 ? NA : 1];
     int front;
@@ -82,8 +87,6 @@ struct Tri {
     float z0, A, B, zoff;  // depth plane about vertex 0, per 1/256 pixel
 };
 
-__device__ __forceinline__ long long llmin(long long a, long long b) { return a < b ? a : b; }
-__device__ __forceinline__ long long llmax(long long a, long long b) { return a < b ? b : a; }
 
 __device__ __forceinline__ float plane_dist(const float c[4], int p) {
     switch (p) {
@@ -139,28 +142,28 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
         float xw = xn * hw + hw;
         float yw = yn * hh + hh;
         z[i] = zn * 0.5f + 0.5f;
-        t.X[i] = (long long)rintf(xw * 256.0f);
-        t.Y[i] = (long long)rintf(yw * 256.0f);
+        t.X[i] = (int)rintf(xw * 256.0f);
+        t.Y[i] = (int)rintf(yw * 256.0f);
         t.w[i] = c[3];
 #pragma unroll
         for (int q = 0; q < NA; ++q) t.a[i][q] = vs[i]->a[q];
     }
-    long long area = (t.X[1] - t.X[0]) * (t.Y[2] - t.Y[0]) - (t.Y[1] - t.Y[0]) * (t.X[2] - t.X[0]);
+    long long area = (long long)(t.X[1] - t.X[0]) * (t.Y[2] - t.Y[0]) - (long long)(t.Y[1] - t.Y[0]) * (t.X[2] - t.X[0]);
     if (area == 0) return false;
     t.front = area > 0;
     if (area < 0) {
-        long long sx = t.X[1]; t.X[1] = t.X[2]; t.X[2] = sx;
-        long long sy = t.Y[1]; t.Y[1] = t.Y[2]; t.Y[2] = sy;
+        int sx = t.X[1]; t.X[1] = t.X[2]; t.X[2] = sx;
+        int sy = t.Y[1]; t.Y[1] = t.Y[2]; t.Y[2] = sy;
         float sz = z[1]; z[1] = z[2]; z[2] = sz;
         float sw = t.w[1]; t.w[1] = t.w[2]; t.w[2] = sw;
 #pragma unroll
         for (int q = 0; q < NA; ++q) { float sa = t.a[1][q]; t.a[1][q] = t.a[2][q]; t.a[2][q] = sa; }
         area = -area;
     }
-    long long minX = llmin(t.X[0], llmin(t.X[1], t.X[2])), maxX = llmax(t.X[0], llmax(t.X[1], t.X[2]));
-    long long minY = llmin(t.Y[0], llmin(t.Y[1], t.Y[2])), maxY = llmax(t.Y[0], llmax(t.Y[1], t.Y[2]));
-    long long px0 = (minX - 128 + 255) >> 8, px1 = (maxX - 128) >> 8;
-    long long py0 = (minY - 128 + 255) >> 8, py1 = (maxY - 128) >> 8;
+    int minX = min(t.X[0], min(t.X[1], t.X[2])), maxX = max(t.X[0], max(t.X[1], t.X[2]));
+    int minY = min(t.Y[0], min(t.Y[1], t.Y[2])), maxY = max(t.Y[0], max(t.Y[1], t.Y[2]));
+    int px0 = (minX - 128 + 255) >> 8, px1 = (maxX - 128) >> 8;
+    int py0 = (minY - 128 + 255) >> 8, py1 = (maxY - 128) >> 8;
     if (px0 < 0) px0 = 0;
     if (py0 < 0) py0 = 0;
     if (px1 > cfg.w - 1) px1 = cfg.w - 1;
@@ -170,7 +173,7 @@ __device__ bool setup_piece(const CV<NA>& v0, const CV<NA>& v1, const CV<NA>& v2
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         int a = (i + 1) % 3, b = (i + 2) % 3;
-        long long dy = t.Y[b] - t.Y[a], dx = t.X[b] - t.X[a];
+        int dy = t.Y[b] - t.Y[a], dx = t.X[b] - t.X[a];
         t.tl[i] = (dy < 0 || (dy == 0 && dx < 0)) ? 1 : 0;
     }
     const double dz1 = (double)z[1] - (double)z[0];
@@ -288,11 +291,11 @@ __device__ __forceinline__ int classify(const MeshView& m, const float mvp[16], 
 // Edge functions at the centre of pixel (i, j); false when not covered.
 template <int NA>
 __device__ __forceinline__ bool eval_pixel(const Tri<NA>& t, int i, int j, long long e[3], float& zout) {
-    long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+    const int Px = i * 256 + 128, Py = j * 256 + 128;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         int a = (k + 1) % 3, b = (k + 2) % 3;
-        e[k] = (t.X[b] - t.X[a]) * (Py - t.Y[a]) - (t.Y[b] - t.Y[a]) * (Px - t.X[a]);
+        e[k] = (long long)(t.X[b] - t.X[a]) * (Py - t.Y[a]) - (long long)(t.Y[b] - t.Y[a]) * (Px - t.X[a]);
     }
     if (e[0] + t.tl[0] <= 0 || e[1] + t.tl[1] <= 0 || e[2] + t.tl[2] <= 0) return false;
     float dx = (float)(Px - t.X[0]), dy = (float)(Py - t.Y[0]);
@@ -310,10 +313,10 @@ __device__ __forceinline__ bool rect_overlap(const Tri<NA>& t, int x0, int x1, i
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         int a = (k + 1) % 3, b = (k + 2) % 3;
-        long long ex = t.X[b] - t.X[a], ey = t.Y[b] - t.Y[a];
-        long long Px = (long long)((ey < 0) ? x1 : x0) * 256 + 128;
-        long long Py = (long long)((ex > 0) ? y1 : y0) * 256 + 128;
-        long long e = ex * (Py - t.Y[a]) - ey * (Px - t.X[a]);
+        const int ex = t.X[b] - t.X[a], ey = t.Y[b] - t.Y[a];
+        const int Px = ((ey < 0) ? x1 : x0) * 256 + 128;
+        const int Py = ((ex > 0) ? y1 : y0) * 256 + 128;
+        const long long e = (long long)ex * (Py - t.Y[a]) - (long long)ey * (Px - t.X[a]);
         if (e + t.tl[k] <= 0) return false;
     }
     return true;
@@ -338,8 +341,8 @@ __device__ __forceinline__ Setup to_setup(const Tri<0>& t, uint32_t id) {
     Setup s;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        s.X[i] = (int)t.X[i];
-        s.Y[i] = (int)t.Y[i];
+        s.X[i] = t.X[i];
+        s.Y[i] = t.Y[i];
     }
     s.px0 = t.px0; s.px1 = t.px1; s.py0 = t.py0; s.py1 = t.py1;
     s.tl = t.tl[0] | (t.tl[1] << 1) | (t.tl[2] << 2);
@@ -363,8 +366,8 @@ __device__ __forceinline__ SetupRec to_rec(const Tri<0>& t) {
     SetupRec r;
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        r.X[i] = (int)t.X[i];
-        r.Y[i] = (int)t.Y[i];
+        r.X[i] = t.X[i];
+        r.Y[i] = t.Y[i];
     }
     r.z0 = t.z0; r.A = t.A; r.B = t.B; r.zoff = t.zoff;
     r.bx = (uint32_t)t.px0 | ((uint32_t)t.px1 << 16);
