@@ -389,7 +389,7 @@ __device__ __forceinline__ Setup from_rec(const SetupRec& r, uint32_t id) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         int a = (i + 1) % 3, b = (i + 2) % 3;
-        long long dy = (long long)r.Y[b] - r.Y[a], dx = (long long)r.X[b] - r.X[a];
+        const int dy = r.Y[b] - r.Y[a], dx = r.X[b] - r.X[a];
         s.tl |= (dy < 0 || (dy == 0 && dx < 0)) ? (1 << i) : 0;
     }
     s.id = id;
@@ -399,7 +399,7 @@ __device__ __forceinline__ Setup from_rec(const SetupRec& r, uint32_t id) {
 
 // Same arithmetic as eval_pixel on the compact record.
 __device__ __forceinline__ bool eval_setup(const Setup& s, int i, int j, float& zout) {
-    const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+    const int Px = i * 256 + 128, Py = j * 256 + 128;
     long long e0 = (long long)(s.X[2] - s.X[1]) * (Py - s.Y[1]) - (long long)(s.Y[2] - s.Y[1]) * (Px - s.X[1]);
     long long e1 = (long long)(s.X[0] - s.X[2]) * (Py - s.Y[2]) - (long long)(s.Y[0] - s.Y[2]) * (Px - s.X[2]);
     long long e2 = (long long)(s.X[1] - s.X[0]) * (Py - s.Y[0]) - (long long)(s.Y[1] - s.Y[0]) * (Px - s.X[0]);
@@ -417,10 +417,10 @@ __device__ __forceinline__ bool rect_overlap_s(const Setup& s, int x0, int x1, i
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         int a = (k + 1) % 3, b = (k + 2) % 3;
-        long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
-        long long Px = (long long)((ey < 0) ? x1 : x0) * 256 + 128;
-        long long Py = (long long)((ex > 0) ? y1 : y0) * 256 + 128;
-        long long e = ex * (Py - s.Y[a]) - ey * (Px - s.X[a]);
+        const int ex = s.X[b] - s.X[a], ey = s.Y[b] - s.Y[a];
+        const int Px = ((ey < 0) ? x1 : x0) * 256 + 128;
+        const int Py = ((ex > 0) ? y1 : y0) * 256 + 128;
+        const long long e = (long long)ex * (Py - s.Y[a]) - (long long)ey * (Px - s.X[a]);
         if (e + ((s.tl >> k) & 1) <= 0) return false;
     }
     return true;
@@ -718,14 +718,14 @@ struct Edges {
 };
 
 __device__ __forceinline__ void edges_at(const Setup& s, int i, int j, Edges& E) {
-    const long long Px = (long long)i * 256 + 128, Py = (long long)j * 256 + 128;
+    const int Px = i * 256 + 128, Py = j * 256 + 128;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int a = (k + 1) % 3, b = (k + 2) % 3;
-        const long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
-        E.e[k] = ex * (Py - s.Y[a]) - ey * (Px - s.X[a]);
-        E.dx[k] = -ey * 256;
-        E.dy[k] = ex * 256;
+        const int ex = s.X[b] - s.X[a], ey = s.Y[b] - s.Y[a];
+        E.e[k] = (long long)ex * (Py - s.Y[a]) - (long long)ey * (Px - s.X[a]);
+        E.dx[k] = (long long)ey * -256;
+        E.dy[k] = (long long)ex * 256;
     }
 }
 
@@ -759,17 +759,17 @@ __device__ __forceinline__ void plot(const Setup& s, const E e[3], int i, int j,
 // minimum over the rectangle is at a corner, and the test `e + tl > 0` holds
 // everywhere once it holds at the four corners.
 __device__ __forceinline__ bool rect_inside(const Setup& s, int rx0, int rx1, int ry0, int ry1) {
-    const long long cx[2] = {(long long)rx0 * 256 + 128, (long long)rx1 * 256 + 128};
-    const long long cy[2] = {(long long)ry0 * 256 + 128, (long long)ry1 * 256 + 128};
+    const int cx[2] = {rx0 * 256 + 128, rx1 * 256 + 128};
+    const int cy[2] = {ry0 * 256 + 128, ry1 * 256 + 128};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int a = (k + 1) % 3, b = (k + 2) % 3;
-        const long long ex = (long long)s.X[b] - s.X[a], ey = (long long)s.Y[b] - s.Y[a];
+        const int ex = s.X[b] - s.X[a], ey = s.Y[b] - s.Y[a];
         const long long tl = (s.tl >> k) & 1;
         // the corner minimising ex * Py - ey * Px
-        const long long px = ey > 0 ? cx[1] : cx[0];
-        const long long py = ex > 0 ? cy[0] : cy[1];
-        if (ex * (py - s.Y[a]) - ey * (px - s.X[a]) + tl <= 0) return false;
+        const int px = ey > 0 ? cx[1] : cx[0];
+        const int py = ex > 0 ? cy[0] : cy[1];
+        if ((long long)ex * (py - s.Y[a]) - (long long)ey * (px - s.X[a]) + tl <= 0) return false;
     }
     return true;
 }
