@@ -397,8 +397,9 @@ int gi_phong_pass(const gi_gbuffer* gbuffer, const float* shadow_dev, const floa
     }
     GBufferView gb{gbuffer->position_dev, gbuffer->normal_dev, gbuffer->width, gbuffer->height};
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_phong, grid_for(gb.w, gb.h), dim3(kBlockThreads), 0, s, gb, gbuffer->facing_dev, shadow_dev,
-                       reinterpret_cast<const float4*>(color_dev), u, reinterpret_cast<float4*>(rgba_out_dev));
+    hipLaunchKernelGGL(k_phong, dim3(row_blocks(gb.w, gb.h)), dim3(kBlockThreads), 0, s, gb, gbuffer->facing_dev,
+                       shadow_dev, reinterpret_cast<const float4*>(color_dev), u,
+                       reinterpret_cast<float4*>(rgba_out_dev));
     return hipGetLastError() == hipSuccess ? GI_OK : GI_EDEVICE;
 }
 

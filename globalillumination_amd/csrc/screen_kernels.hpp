@@ -1426,12 +1426,23 @@ __device__ __forceinline__ float4 phong_pixel(const GBufferView& gb, const float
     return make_float4(o[0], o[1], o[2], o[3]);
 }
 
+// Rows of 64 pixels per wave (a workgroup covers 64 x 4): each load of a
+// G-buffer plane reads two whole 128-byte lines.
+inline int row_blocks(int w, int h) { return ((w + 63) / 64) * ((h + 3) / 4); }
+__device__ __forceinline__ void row_block_pixel(int w, int h, int& x, int& y) {
+    const int bx_n = (w + 63) / 64, by_n = (h + 3) / 4;
+    int bx, by;
+    strip_block<1>(xcd_order(bx_n * by_n), bx_n, by_n, bx, by);
+    x = bx * 64 + (int)(threadIdx.x & 63);
+    y = by * 4 + (int)(threadIdx.x >> 6);
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_phong(GBufferView gb, const float* __restrict__ facing,
                                                           const float* __restrict__ shadow,
                                                           const float4* __restrict__ color, PhongUniforms u,
                                                           float4* __restrict__ out) {
     int x, y;
-    pixel_of_block(gb.w, gb.h, x, y);
+    row_block_pixel(gb.w, gb.h, x, y);
     if (x >= gb.w || y >= gb.h) return;
     const size_t k = (size_t)y * gb.w + x;
     out[k] = phong_pixel(gb, facing, shadow, color, u, k);
