@@ -69,7 +69,12 @@ int bin_stage(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Rast
     }
     hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, b, ntiles);
     if (mesh.nt > 0) {
-        hipLaunchKernelGGL((k_bin<kTile, true>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        if constexpr (wide_span<kTile>() <= 4) {
+            const dim3 gs((mesh.nt + 256 * kScatterTris - 1) / (256 * kScatterTris), nmaps);
+            hipLaunchKernelGGL((k_bin_scatter<kTile, kScatterTris>), gs, dim3(256), 0, s, mesh, cfg, b);
+        } else {
+            hipLaunchKernelGGL((k_bin<kTile, true>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+        }
         hipLaunchKernelGGL((k_bin_wide<kTile, true>), gw, dim3(256), 0, s, cfg, b);
     }
     hipLaunchKernelGGL((k_raster_fallback<kTile, kVis>), dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b,
