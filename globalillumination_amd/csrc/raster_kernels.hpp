@@ -211,7 +211,7 @@ __device__ __forceinline__ int clip_fan(CV<NA>* poly, CV<NA>* tmp, int piece, co
     for (int p = 0; p < 6; ++p) {
         int m = 0;
         for (int k = 0; k < n; ++k) {
-            const CV<NA>& prev = poly[(k + n - 1) % n];
+            const CV<NA>& prev = poly[k == 0 ? n - 1 : k - 1];
             const CV<NA>& cur = poly[k];
             float dp = plane_dist(prev.c, p), dc = plane_dist(cur.c, p);
             if (dc >= 0.0f) {
