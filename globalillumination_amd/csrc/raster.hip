@@ -70,12 +70,14 @@ int bin_stage(const MeshView& mesh, const MvpSource& mvps, int nmaps, const Rast
     hipLaunchKernelGGL(k_scan, dim3(nmaps), dim3(kScanThreads), 0, s, b, ntiles);
     if (mesh.nt > 0) {
         if constexpr (wide_span<kTile>() <= 4) {
-            const dim3 gs((mesh.nt + 256 * kScatterTris - 1) / (256 * kScatterTris), nmaps);
-            hipLaunchKernelGGL((k_bin_scatter<kTile, kScatterTris>), gs, dim3(256), 0, s, mesh, cfg, b);
+            // triangles' appends and the wide records' appends in one launch
+            const int tri_blocks = (mesh.nt + 256 * kScatterTris - 1) / (256 * kScatterTris);
+            hipLaunchKernelGGL((k_bin_scatter<kTile, kScatterTris>), dim3(tri_blocks + kWideBlocks, nmaps), dim3(256),
+                               0, s, mesh, cfg, b, tri_blocks);
         } else {
             hipLaunchKernelGGL((k_bin<kTile, true>), gb, dim3(256), 0, s, mesh, mvps, cfg, b);
+            hipLaunchKernelGGL((k_bin_wide<kTile, true>), gw, dim3(256), 0, s, cfg, b);
         }
-        hipLaunchKernelGGL((k_bin_wide<kTile, true>), gw, dim3(256), 0, s, cfg, b);
     }
     hipLaunchKernelGGL((k_raster_fallback<kTile, kVis>), dim3(ntiles), dim3(kRasterThreads), 0, s, mesh, mvps, cfg, b,
                        out, nmaps);

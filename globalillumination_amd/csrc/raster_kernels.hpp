@@ -600,76 +600,6 @@ __global__ __launch_bounds__(256) void k_bin(MeshView mesh, MvpSource mvps, Rast
     bin_body<TS, kScatter>(mesh, mvps, cfg, b, blockIdx.x, blockIdx.y);
 }
 
-// The scatter pass when the count pass stored each triangle's tile code
-// (64-texel tiles): kT triangles per thread.  A workgroup's appends cost two
-// dependent global round trips (the atomics that reserve its slice of each
-// tile's list, then the list offsets), so covering kT x 256 triangles per
-// workgroup divides those rounds by kT.  Entries land in the same lists as
-// bin_body's scatter (their order within a list does not matter: the raster
-// takes the minimum).
-template <int TS, int kT>
-__global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cfg, BinBufs b) {
-    constexpr int kWideSpan = wide_span<TS>();
-    static_assert(kWideSpan <= 4, "tile codes are stored for 64-texel tiles");
-    __shared__ uint32_t hist[kBinHist];
-    const int map = (int)blockIdx.y, tid = (int)threadIdx.x;
-    const int ntiles = cfg.tiles_x * cfg.tiles_y;
-    if (b.overflow[map]) return;  // uniform per workgroup
-    const bool lds = ntiles <= kBinHist;
-    if (lds)
-        for (int p = tid; p < ntiles; p += 256) hist[p] = 0;
-    int tile_of[kT][kWideSpan];
-    uint32_t rank_of[kT][kWideSpan];
-#pragma unroll
-    for (int t = 0; t < kT; ++t) {
-        const int tri = ((int)blockIdx.x * kT + t) * 256 + tid;
-#pragma unroll
-        for (int k = 0; k < kWideSpan; ++k) tile_of[t][k] = -1;
-        if (tri < mesh.nt) {
-            const uint32_t code = b.tiles[(size_t)map * b.tstride + tri];
-            const int base = (int)(code & 0xFFFFu), ntx = (int)((code >> 16) & 3u) + 1;
-            const uint32_t mask = code >> 18;
-#pragma unroll
-            for (int k = 0; k < kWideSpan; ++k)
-                if (mask & (1u << k)) tile_of[t][k] = base + (k / ntx) * cfg.tiles_x + (k % ntx);
-        }
-    }
-    if (!lds) {
-#pragma unroll
-        for (int t = 0; t < kT; ++t)
-#pragma unroll
-            for (int k = 0; k < kWideSpan; ++k)
-                if (tile_of[t][k] >= 0)
-                    bin_tile(b, map, ntiles, tile_of[t][k], (uint32_t)(((int)blockIdx.x * kT + t) * 256 + tid) * 8u,
-                             true);
-        return;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int t = 0; t < kT; ++t)
-#pragma unroll
-        for (int k = 0; k < kWideSpan; ++k)
-            if (tile_of[t][k] >= 0) rank_of[t][k] = atomicAdd(&hist[tile_of[t][k]], 1u);
-    __syncthreads();
-    uint32_t* cnt = b.counts + (size_t)map * ntiles;
-    for (int p = tid; p < ntiles; p += 256) {
-        const uint32_t c = hist[p];
-        if (c != 0) hist[p] = atomicAdd(&cnt[p], c);  // this workgroup's base within the tile's list
-    }
-    __syncthreads();
-    const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
-#pragma unroll
-    for (int t = 0; t < kT; ++t) {
-        const uint32_t entry = (uint32_t)(((int)blockIdx.x * kT + t) * 256 + tid) * 8u;
-#pragma unroll
-        for (int k = 0; k < kWideSpan; ++k) {
-            if (tile_of[t][k] < 0) continue;
-            const uint32_t pos = off[tile_of[t][k]] + hist[tile_of[t][k]] + rank_of[t][k];
-            if (pos < b.cap) b.entries[(size_t)map * b.cap + pos] = entry;
-        }
-    }
-}
-constexpr int kScatterTris = 4;  // triangles per thread in k_bin_scatter
 
 // Clips the listed triangles; every non-empty fan piece becomes a setup
 // record.  Runs between the count passes, before k_bin_wide.  One thread per
@@ -730,6 +660,85 @@ template <int TS, bool kScatter>
 __global__ __launch_bounds__(256) void k_bin_wide(RasterCfg cfg, BinBufs b) {
     bin_wide_body<TS, kScatter>(cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
 }
+
+// The scatter pass when the count pass stored each triangle's tile code
+// (64-texel tiles): kT triangles per thread.  A workgroup's appends cost two
+// dependent global round trips (the atomics that reserve its slice of each
+// tile's list, then the list offsets), so covering kT x 256 triangles per
+// workgroup divides those rounds by kT.  Entries land in the same lists as
+// bin_body's scatter (their order within a list does not matter: the raster
+// takes the minimum).
+//
+// Workgroups past `tri_blocks` bin the wide pieces' records instead
+// (bin_wide_body's scatter, kWideBlocks of them per map), so the two appends
+// share one launch and overlap.
+template <int TS, int kT>
+__global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cfg, BinBufs b, int tri_blocks) {
+    constexpr int kWideSpan = wide_span<TS>();
+    static_assert(kWideSpan <= 4, "tile codes are stored for 64-texel tiles");
+    __shared__ uint32_t hist[kBinHist];
+    const int map = (int)blockIdx.y, tid = (int)threadIdx.x;
+    if ((int)blockIdx.x >= tri_blocks) {  // uniform per workgroup
+        bin_wide_body<TS, true>(cfg, b, (int)blockIdx.x - tri_blocks, kWideBlocks, map);
+        return;
+    }
+    const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    if (b.overflow[map]) return;  // uniform per workgroup
+    const bool lds = ntiles <= kBinHist;
+    if (lds)
+        for (int p = tid; p < ntiles; p += 256) hist[p] = 0;
+    int tile_of[kT][kWideSpan];
+    uint32_t rank_of[kT][kWideSpan];
+#pragma unroll
+    for (int t = 0; t < kT; ++t) {
+        const int tri = ((int)blockIdx.x * kT + t) * 256 + tid;
+#pragma unroll
+        for (int k = 0; k < kWideSpan; ++k) tile_of[t][k] = -1;
+        if (tri < mesh.nt) {
+            const uint32_t code = b.tiles[(size_t)map * b.tstride + tri];
+            const int base = (int)(code & 0xFFFFu), ntx = (int)((code >> 16) & 3u) + 1;
+            const uint32_t mask = code >> 18;
+#pragma unroll
+            for (int k = 0; k < kWideSpan; ++k)
+                if (mask & (1u << k)) tile_of[t][k] = base + (k / ntx) * cfg.tiles_x + (k % ntx);
+        }
+    }
+    if (!lds) {
+#pragma unroll
+        for (int t = 0; t < kT; ++t)
+#pragma unroll
+            for (int k = 0; k < kWideSpan; ++k)
+                if (tile_of[t][k] >= 0)
+                    bin_tile(b, map, ntiles, tile_of[t][k], (uint32_t)(((int)blockIdx.x * kT + t) * 256 + tid) * 8u,
+                             true);
+        return;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kT; ++t)
+#pragma unroll
+        for (int k = 0; k < kWideSpan; ++k)
+            if (tile_of[t][k] >= 0) rank_of[t][k] = atomicAdd(&hist[tile_of[t][k]], 1u);
+    __syncthreads();
+    uint32_t* cnt = b.counts + (size_t)map * ntiles;
+    for (int p = tid; p < ntiles; p += 256) {
+        const uint32_t c = hist[p];
+        if (c != 0) hist[p] = atomicAdd(&cnt[p], c);  // this workgroup's base within the tile's list
+    }
+    __syncthreads();
+    const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
+#pragma unroll
+    for (int t = 0; t < kT; ++t) {
+        const uint32_t entry = (uint32_t)(((int)blockIdx.x * kT + t) * 256 + tid) * 8u;
+#pragma unroll
+        for (int k = 0; k < kWideSpan; ++k) {
+            if (tile_of[t][k] < 0) continue;
+            const uint32_t pos = off[tile_of[t][k]] + hist[tile_of[t][k]] + rank_of[t][k];
+            if (pos < b.cap) b.entries[(size_t)map * b.cap + pos] = entry;
+        }
+    }
+}
+constexpr int kScatterTris = 4;  // triangles per thread in k_bin_scatter
 
 // Exclusive scan of one map's tile counts; resets counts to 0 (they become
 // the scatter cursors) and flags maps whose lists exceed the workspace.
