@@ -935,6 +935,41 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
     const int x0 = tx * TS, y0 = ty * TS;
     const int x1 = min(x0 + TS - 1, cfg.w - 1), y1 = min(y0 + TS - 1, cfg.h - 1);
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
+    uint32_t begin = 0, end;
+    if constexpr (!kFallback) {
+        begin = b.offsets[(size_t)map * (ntiles + 1) + tile_id];
+        end = b.offsets[(size_t)map * (ntiles + 1) + tile_id + 1];
+    } else {
+        end = (uint32_t)mesh.nt * 8u;
+    }
+    if constexpr (!kFallback && !kVis) {
+        // A whole tile whose list is one wide record covering every pixel
+        // centre (most tiles of a large ground plane): each depth is that
+        // plane's, written straight from registers.  The LDS path would store
+        // the same bits: min(clear, z) is z for every clamped depth.
+        if (end - begin == 1 && x1 - x0 + 1 == TS && y1 - y0 + 1 == TS && (cfg.w & 3) == 0) {
+            const uint32_t entry = b.entries[(size_t)map * b.cap + begin];
+            if (entry & kWideBit) {
+                const Setup& w = b.wide[(size_t)map * b.wcap + (entry & ~kWideBit)];
+                const Setup s = w;
+                if (s.px0 <= x0 && s.px1 >= x1 && s.py0 <= y0 && s.py1 >= y1 && rect_inside(s, x0, x1, y0, y1)) {
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    float* dst = reinterpret_cast<float*>(out) + (size_t)map * ((size_t)cfg.w * cfg.h);
+                    for (int q = tid; q < kTilePx / 4; q += kRasterThreads) {
+                        const int r = q / (TS / 4), cidx = (q % (TS / 4)) * 4;
+                        const int i = x0 + cidx, j = y0 + r;
+                        const int dy = j * 256 + 128 - s.Y[0];
+                        const f4 v = {__uint_as_float(plane_bits(s, i * 256 + 128 - s.X[0], dy)),
+                                      __uint_as_float(plane_bits(s, (i + 1) * 256 + 128 - s.X[0], dy)),
+                                      __uint_as_float(plane_bits(s, (i + 2) * 256 + 128 - s.X[0], dy)),
+                                      __uint_as_float(plane_bits(s, (i + 3) * 256 + 128 - s.X[0], dy))};
+                        __builtin_nontemporal_store(v, reinterpret_cast<f4*>(dst + (size_t)j * cfg.w + i));
+                    }
+                    return;
+                }
+            }
+        }
+    }
     if constexpr (!kVis) {  // 16-byte LDS writes
         constexpr uint32_t c = KeyOf<kVis>::clear;
         for (int q = tid; q < kTilePx / 4; q += kRasterThreads) reinterpret_cast<uint4*>(tile)[q] = make_uint4(c, c, c, c);
@@ -944,13 +979,6 @@ __device__ __forceinline__ void raster_tile(const MeshView& mesh, const MvpSourc
     if (tid == 0) nrec = 0;
     float mvp[16];
     mvps.load(map, mvp);
-    uint32_t begin = 0, end;
-    if constexpr (!kFallback) {
-        begin = b.offsets[(size_t)map * (ntiles + 1) + tile_id];
-        end = b.offsets[(size_t)map * (ntiles + 1) + tile_id + 1];
-    } else {
-        end = (uint32_t)mesh.nt * 8u;
-    }
     __syncthreads();
     for (uint32_t base = begin; base < end; base += kRasterThreads) {
         uint32_t k = base + tid;
