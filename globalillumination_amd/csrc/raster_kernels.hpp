@@ -894,21 +894,26 @@ __device__ __forceinline__ void raster_rect_self(const Setup& s, int x0, int y0,
         raster_rect_self_t<TS, kVis, long long>(s, E, x0, y0, rx0, rx1, ry0, ry1, tile);
 }
 
-// The workgroup covers the rectangle: thread t owns tile column t % TS and
-// every (kRasterThreads / TS)-th row from row t / TS.
+// The workgroup covers the rectangle in row groups of cw columns, cw the
+// smallest power of two (8 to TS) not below the rectangle's width: thread t
+// owns column rx0 + t % cw and every (kRasterThreads / cw)-th row from
+// ry0 + t / cw, so a wave covers 64 / cw rows and at least half its lanes
+// hold a pixel column.
 template <int TS, bool kVis>
 __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0, int rx0, int rx1, int ry0, int ry1,
                                                   int tid, typename KeyOf<kVis>::T* tile) {
-    constexpr int kRowStep = kRasterThreads / TS;
-    const int i = x0 + (tid % TS);
-    if (i < rx0 || i > rx1) return;
-    int j = y0 + tid / TS;
-    if (j < ry0) j += ((ry0 - j + kRowStep - 1) / kRowStep) * kRowStep;
+    const int rw = rx1 - rx0 + 1;
+    int lg = 3;
+    while ((1 << lg) < rw) ++lg;  // rw <= TS
+    const int i = rx0 + (tid & ((1 << lg) - 1));
+    if (i > rx1) return;
+    int j = ry0 + (tid >> lg);
     if (j > ry1) return;
+    const int row_step = kRasterThreads >> lg;
     if (rect_inside(s, rx0, rx1, ry0, ry1)) {
         // every pixel centre of the rectangle passes the three edge tests
 #pragma unroll 4
-        for (; j <= ry1; j += kRowStep)
+        for (; j <= ry1; j += row_step)
             store_min<kVis>(s, plane_bits(s, i * 256 + 128 - s.X[0], j * 256 + 128 - s.Y[0]),
                             (j - y0) * TS + (i - x0), tile);
         return;
@@ -916,10 +921,13 @@ __device__ __forceinline__ void raster_rect_group(const Setup& s, int x0, int y0
     Edges E;
     edges_at(s, i, j, E);
     long long e[3] = {E.e[0], E.e[1], E.e[2]};
-    for (; j <= ry1; j += kRowStep) {
+    long long step[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) step[k] = E.dy[k] * row_step;
+    for (; j <= ry1; j += row_step) {
         plot<kVis, long long>(s, e, i, j, (j - y0) * TS + (i - x0), tile);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) e[k] += E.dy[k] * kRowStep;
+        for (int k = 0; k < 3; ++k) e[k] += step[k];
     }
 }
 
