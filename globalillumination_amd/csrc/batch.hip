@@ -46,7 +46,8 @@ constexpr int kScreenThreads = 256;
 constexpr int kNtGroups = 3;  // soft variants: generic loop, 3 taps per axis, 4 taps per axis
 constexpr int kShapes = 5;    // screen blocks 64x4, 32x8, 16x16 (row-major waves), 16x16 of 8x8 wave
                               // tiles, 64x4 of 16x4 wave tiles
-constexpr int kCounters = 64;  // [0] raster queue, [1 + 8 g + x] screen queue of group g on XCD x,
+constexpr int kCounters = 64;  // [0] raster queue, [1] a map of the depth run overflowed (until the
+                               // screen run clears it), [1 + 8 g + x] screen queue of group g on XCD x,
                                // [63] the autotuner's overflow flag
 
 struct RasterJob {
@@ -65,7 +66,7 @@ struct RasterTable {
     const RasterJob* jobs;
     const uint32_t* prefix;  // kStages x (njobs + 1): items of each stage per job, exclusive prefix
     int njobs;
-    uint32_t* counter;       // raster work queue
+    uint32_t* counter;       // [0] raster work queue, [1] set when a map overflowed
 };
 
 struct SoftJob {
@@ -163,7 +164,11 @@ __global__ __launch_bounds__(kScanThreads) void kb_scan(RasterTable t) {
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const int j = find_job(pre, t.njobs, item);
         const RasterJob& J = t.jobs[j];
-        scan_body(J.b, J.ntiles, (int)(item - pre[j]));
+        const int map = (int)(item - pre[j]);
+        scan_body(J.b, J.ntiles, map);
+        // the thread that decided the map's overflow raises the batch's flag
+        // (kb_raster_fallback does nothing without it)
+        if (threadIdx.x == kScanThreads - 1 && J.b.overflow[map]) t.counter[1] = 1u;
         __syncthreads();
     }
 }
@@ -208,6 +213,7 @@ __global__ __launch_bounds__(kRasterThreads) void kb_raster_fallback(RasterTable
     __shared__ uint32_t tile[TS * TS];
     __shared__ Setup rec[kRecQueue];
     __shared__ int nrec;
+    if (t.counter[1] == 0u) return;  // no map overflowed (kb_scan)
     const uint32_t* pre = t.prefix + kStFallback * (t.njobs + 1);
     const uint32_t total = pre[t.njobs];
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
@@ -364,7 +370,7 @@ int run_depth(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
     const RasterVariant& R = b->rv[v];
     RasterTable tab{R.jobs, R.prefix, b->n, b->counters};
     const unsigned cap = (unsigned)(b->ncu * T.raster_groups);
-    hipLaunchKernelGGL(kb_zero, dim3(1), dim3(kCounters), 0, s, b->counters, 1);
+    hipLaunchKernelGGL(kb_zero, dim3(1), dim3(kCounters), 0, s, b->counters, 2);
     unsigned g;
     if ((g = grid_of(R.total[kStFill], cap))) hipLaunchKernelGGL(kb_fill, dim3(g), dim3(256), 0, s, tab);
 #define GI_DEPTH_STAGES(TS)                                                                                        \

@@ -635,24 +635,75 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(MeshView mesh, MvpSource 
     clip_body(mesh, mvps, cfg, b, blockIdx.x, gridDim.x, blockIdx.y);
 }
 
-// Wide pieces: a workgroup per record, its threads walk the record's tiles.
+// Wide pieces (setup records), kWideThreads threads per workgroup.  Part p
+// of nparts takes records p, p + nparts, p + 2 nparts, ..., one per thread,
+// kWideThreads records a round.  The round's (record, tile) pairs are
+// numbered through a prefix sum of the records' tile spans, and each thread
+// takes every kWideThreads-th pair, so the pairs' appends (whose scatter
+// atomics return the list slot) are all in flight at once whether the round
+// holds many records of a few tiles (a triangle soup on a large map) or one
+// record over the whole map (a ground plane's clipped piece); a map's few
+// records still go to different parts.
+constexpr int kWideThreads = 256;
+
+template <int TS, bool kScatter>
+__device__ __forceinline__ void wide_tile(const RasterCfg& cfg, const BinBufs& b, int map, int ntiles, const Setup& s,
+                                          uint32_t w, int tx0, int ty0, int ntx, int q) {
+    const int tx = tx0 + q % ntx, ty = ty0 + q / ntx;
+    const int x0 = max(tx * TS, s.px0), x1 = min(tx * TS + TS - 1, s.px1);
+    const int y0 = max(ty * TS, s.py0), y1 = min(ty * TS + TS - 1, s.py1);
+    if (rect_overlap_s(s, x0, x1, y0, y1)) bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, kWideBit | w, kScatter);
+}
+
 template <int TS, bool kScatter>
 __device__ __forceinline__ void bin_wide_body(const RasterCfg& cfg, const BinBufs& b, int part, int nparts, int map) {
-    if (kScatter && b.overflow[map]) return;
+    __shared__ int incl[kWideThreads];  // inclusive prefix of the round's spans
+    __shared__ int wave_sum[kWideThreads / 64];
+    if (kScatter && b.overflow[map]) return;  // uniform per workgroup
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     const uint32_t n = min(b.nwide[map], b.wcap);
-    for (uint32_t w = part; w < n; w += nparts) {
-        const Setup s = b.wide[(size_t)map * b.wcap + w];
-        int tx0 = s.px0 / TS, tx1 = s.px1 / TS, ty0 = s.py0 / TS, ty1 = s.py1 / TS;
-        int ntx = tx1 - tx0 + 1;
-        int span = ntx * (ty1 - ty0 + 1);
-        for (int q = threadIdx.x; q < span; q += blockDim.x) {
-            int tx = tx0 + q % ntx, ty = ty0 + q / ntx;
-            int x0 = max(tx * TS, s.px0), x1 = min(tx * TS + TS - 1, s.px1);
-            int y0 = max(ty * TS, s.py0), y1 = min(ty * TS + TS - 1, s.py1);
-            if (!rect_overlap_s(s, x0, x1, y0, y1)) continue;
-            bin_tile(b, map, ntiles, ty * cfg.tiles_x + tx, kWideBit | w, kScatter);
+    const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t stride = (uint32_t)nparts;
+    const Setup* recs = b.wide + (size_t)map * b.wcap;
+    for (uint32_t r0 = 0; (uint32_t)part + stride * r0 < n; r0 += kWideThreads) {  // uniform
+        const uint32_t w = (uint32_t)part + stride * (r0 + (uint32_t)tid);
+        int span = 0;
+        if (w < n) {
+            const Setup& s = recs[w];
+            span = (s.px1 / TS - s.px0 / TS + 1) * (s.py1 / TS - s.py0 / TS + 1);
         }
+        int v = span;  // inclusive scan in the wave, then across the waves
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int u = __shfl_up(v, d, 64);
+            if (lane >= d) v += u;
+        }
+        if (lane == 63) wave_sum[wave] = v;
+        __syncthreads();
+        int base = 0, total = 0;
+#pragma unroll
+        for (int k = 0; k < kWideThreads / 64; ++k) {
+            base += k < wave ? wave_sum[k] : 0;
+            total += wave_sum[k];
+        }
+        incl[tid] = base + v;
+        __syncthreads();
+        for (int p = tid; p < total; p += kWideThreads) {
+            // the round's record holding pair p: the first k with incl[k] > p
+            int lo = 0, hi = kWideThreads - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (incl[mid] > p) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t wk = (uint32_t)part + stride * (r0 + (uint32_t)lo);
+            const Setup s = recs[wk];
+            const int tx0 = s.px0 / TS, ty0 = s.py0 / TS;
+            const int ntx = s.px1 / TS - tx0 + 1;
+            const int q = p - (lo == 0 ? 0 : incl[lo - 1]);
+            wide_tile<TS, kScatter>(cfg, b, map, ntiles, s, wk, tx0, ty0, ntx, q);
+        }
+        __syncthreads();  // incl and wave_sum are rewritten by the next round
     }
 }
 

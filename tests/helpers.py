@@ -58,6 +58,37 @@ def clip_heavy_mesh(n_small: int = 8000, n_clip: int = 200, seed: int = 3):
     return mesh, light
 
 
+def overflow_mesh(n: int = 64, seed: int = 7, size: int = 1024):
+    """n large overlapping triangles under a light at (0, 50, 0.1), fov 90:
+    every one touches every tile, so the tile lists exceed the workspace
+    bound and the map goes through the fallback raster."""
+    from globalillumination_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-30, 30, size=(n, 3, 3)).astype(np.float32)
+    pos[:, :, 1] = rng.uniform(0.0, 2.0, size=(n, 3))
+    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
+    mesh = scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    light = scenes.View((0.0, 50.0, 0.1), (0.0, 0.0, 0.0), size, size, fov=90.0)
+    return mesh, light
+
+
+def wide_overflow_mesh(n: int = 3000, seed: int = 11, size: int = 512):
+    """n mid-sized triangles under a light at (0, 40, 0.1), each wider than
+    the binning's per-thread span: more setup records than the workspace
+    holds, so the map goes through the fallback raster."""
+    from globalillumination_amd import scenes
+
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(-6, 6, size=(n, 1, 3))
+    pos = (cen + rng.uniform(-3, 3, size=(n, 3, 3))).astype(np.float32)
+    pos[:, :, 1] = rng.uniform(0.0, 3.0, size=(n, 3))
+    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
+    mesh = scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    light = scenes.View((0.0, 40.0, 0.1), (0.0, 0.0, 0.0), size, size, fov=45.0)
+    return mesh, light
+
+
 def clipped_count(mesh, mvp, guard=4.0):
     """Triangles the depth pass lists for clipping: not outside any of the six
     planes (z + w, w - z, x + g w, g w - x, y + g w, g w - y >= 0, g the guard

@@ -182,6 +182,40 @@ def test_batch_clip_heavy(cuda, gi):
         b.close()
 
 
+def test_batch_overflow_fallback(cuda, gi):
+    """Maps whose tile lists or setup records exceed the workspace (the
+    depth-pass parity tests' overflow meshes) go through the batch's
+    fallback raster, beside maps that fit and go through the binned raster;
+    the clip-heavy scene's maps fit."""
+    from types import SimpleNamespace
+
+    from globalillumination_amd import scenes
+    from globalillumination_amd.batch import Batch
+    from globalillumination_amd.passes import SoftParams
+    from helpers import clip_heavy_mesh, overflow_mesh, wide_overflow_mesh
+
+    cam = scenes.View((0.0, 6.0, 8.0), (0.0, 0.0, 0.0), 96, 64)
+    over, over_light = overflow_mesh()
+    wide, wide_light = wide_overflow_mesh()
+    clip, clip_light = clip_heavy_mesh()
+    # looking away from the mesh: an empty map beside the overflowing one
+    away = scenes.View((0.0, 50.0, 0.1), (0.0, 100.0, 0.0), 1024, 1024, fov=30.0)
+    wls = [SimpleNamespace(mesh=over, lights=[over_light, away], camera=cam),
+           SimpleNamespace(mesh=clip, lights=[clip_light], camera=cam),
+           SimpleNamespace(mesh=wide, lights=[wide_light], camera=cam)]
+    insts = _instances(wls, [SoftParams(pcf=True, kernelOrder=3, penumbra=1.0), SoftParams(pcf=False),
+                             SoftParams(pcf=True, kernelOrder=3, penumbra=1.0)])
+    maps, outs = _reference(insts)
+    print("batch overflow: 1024^2 (overflowing + empty), 128^2 clip-heavy, 512^2 (overflowing)", flush=True)
+    for tile in (64, 32):
+        b = Batch(insts)
+        b.set_tuning(tile, 2, 0, 2)
+        _clear(insts)
+        b.run()
+        _same(insts, maps, outs, f"overflow, tile {tile}")
+        b.close()
+
+
 def test_batch_full_workload(cuda, gi):
     """32 instances: maps 512^2-4096^2, views 640x480-1920x1080, 5k-300k
     triangles, 1-16 samples (seed 4), autotuned, through a graph."""

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 import torch
 
-from helpers import bits_equal, camera_mvp, clip_heavy_mesh, clipped_count, light_mvps, mismatch_report, shrink_soup
+from helpers import (bits_equal, camera_mvp, clip_heavy_mesh, clipped_count, light_mvps, mismatch_report, overflow_mesh,
+                     shrink_soup, wide_overflow_mesh)
 
 pytestmark = pytest.mark.gpu
 
@@ -168,13 +169,7 @@ def test_depth_pass_empty_mesh(gi, cuda):
 def test_depth_pass_overflow_fallback(gi, oracle_lib, cuda):
     # 64 large overlapping triangles: every one touches every tile, so the tile
     # lists exceed the workspace bound and the raster scans the whole mesh.
-    rng = np.random.default_rng(7)
-    n = 64
-    pos = rng.uniform(-30, 30, size=(n, 3, 3)).astype(np.float32)
-    pos[:, :, 1] = rng.uniform(0.0, 2.0, size=(n, 3))
-    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
-    mesh = gi.scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
-    light = gi.scenes.View((0.0, 50.0, 0.1), (0.0, 0.0, 0.0), 1024, 1024, fov=90.0)
+    mesh, light = overflow_mesh()
     mvps = np.stack([gi.passes.view_matrices(light)[2]])
     got, want = _depth_both(gi, oracle_lib, mesh, mvps, 1024, 1024)
     assert bits_equal(got, want), mismatch_report(got, want, "overflow fallback")
@@ -183,14 +178,7 @@ def test_depth_pass_overflow_fallback(gi, oracle_lib, cuda):
 def test_depth_pass_wide_record_overflow(gi, oracle_lib, cuda):
     # 3000 mid-sized triangles, each wider than the binning's per-thread span:
     # more setup records than the workspace holds, so the map falls back.
-    rng = np.random.default_rng(11)
-    n = 3000
-    cen = rng.uniform(-6, 6, size=(n, 1, 3))
-    pos = (cen + rng.uniform(-3, 3, size=(n, 3, 3))).astype(np.float32)
-    pos[:, :, 1] = rng.uniform(0.0, 3.0, size=(n, 3))
-    nrm = np.tile(np.array([0, 1, 0], np.float32), (3 * n, 1))
-    mesh = gi.scenes.Mesh(pos.reshape(-1, 3), nrm, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
-    light = gi.scenes.View((0.0, 40.0, 0.1), (0.0, 0.0, 0.0), 512, 512, fov=45.0)
+    mesh, light = wide_overflow_mesh()
     mvps = np.stack([gi.passes.view_matrices(light)[2]])
     got, want = _depth_both(gi, oracle_lib, mesh, mvps, 512, 512)
     assert bits_equal(got, want), mismatch_report(got, want, "wide record overflow")
