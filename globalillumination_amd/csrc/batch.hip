@@ -131,7 +131,7 @@ __global__ __launch_bounds__(256) void kb_bin(RasterTable t) {
         const uint32_t local = item - pre[j];
         bin_body<TS, kScatter>(J.mesh, mvp_of(J), J.cfg, J.b, (int)(local % J.tri_blocks),
                                (int)(local / J.tri_blocks));
-        __syncthreads();  // the LDS histogram is reused by the next item
+        lds_barrier();  // the LDS histogram is reused by the next item
     }
 }
 

@@ -476,6 +476,18 @@ __device__ __forceinline__ void bin_tile(const BinBufs& b, int map, int ntiles, 
     }
 }
 
+// A workgroup barrier that orders LDS only.  __syncthreads() also waits for
+// the workgroup's outstanding global stores and atomics (its release fence
+// covers global memory); the binning kernels' barriers guard their LDS
+// histograms and prefix sums, and what they write to global memory is read
+// only by later kernels (an atomic whose result goes to LDS has returned
+// before that LDS write).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Appends a setup record (count pass only); records past the capacity flag
 // the map for the fallback raster.
 __device__ __forceinline__ void push_record(const BinBufs& b, int map, const Tri<0>& t, uint32_t entry) {
@@ -572,11 +584,11 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
             if (tile_of[k] >= 0) bin_tile(b, map, ntiles, tile_of[k], entry, kScatter);
         return;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < kWideSpan; ++k)
         if (tile_of[k] >= 0) rank_of[k] = atomicAdd(&hist[tile_of[k]], 1u);
-    __syncthreads();
+    lds_barrier();
     uint32_t* cnt = b.counts + (size_t)map * ntiles;
     for (int p = tid; p < ntiles; p += blockDim.x) {
         uint32_t c = hist[p];
@@ -585,7 +597,7 @@ __device__ __forceinline__ void bin_body(const MeshView& mesh, const MvpSource& 
         else hist[p] = atomicAdd(&cnt[p], c);  // this workgroup's base within the tile's list
     }
     if (!kScatter) return;
-    __syncthreads();
+    lds_barrier();
     const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
 #pragma unroll
     for (int k = 0; k < kWideSpan; ++k) {
@@ -679,7 +691,7 @@ __device__ __forceinline__ void bin_wide_body(const RasterCfg& cfg, const BinBuf
             if (lane >= d) v += u;
         }
         if (lane == 63) wave_sum[wave] = v;
-        __syncthreads();
+        lds_barrier();
         int base = 0, total = 0;
 #pragma unroll
         for (int k = 0; k < kWideThreads / 64; ++k) {
@@ -687,7 +699,7 @@ __device__ __forceinline__ void bin_wide_body(const RasterCfg& cfg, const BinBuf
             total += wave_sum[k];
         }
         incl[tid] = base + v;
-        __syncthreads();
+        lds_barrier();
         for (int p = tid; p < total; p += kWideThreads) {
             // the round's record holding pair p: the first k with incl[k] > p
             int lo = 0, hi = kWideThreads - 1;
@@ -703,7 +715,7 @@ __device__ __forceinline__ void bin_wide_body(const RasterCfg& cfg, const BinBuf
             const int q = p - (lo == 0 ? 0 : incl[lo - 1]);
             wide_tile<TS, kScatter>(cfg, b, map, ntiles, s, wk, tx0, ty0, ntx, q);
         }
-        __syncthreads();  // incl and wave_sum are rewritten by the next round
+        lds_barrier();  // incl and wave_sum are rewritten by the next round
     }
 }
 
@@ -764,19 +776,19 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cf
                              true);
         return;
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int t = 0; t < kT; ++t)
 #pragma unroll
         for (int k = 0; k < kWideSpan; ++k)
             if (tile_of[t][k] >= 0) rank_of[t][k] = atomicAdd(&hist[tile_of[t][k]], 1u);
-    __syncthreads();
+    lds_barrier();
     uint32_t* cnt = b.counts + (size_t)map * ntiles;
     for (int p = tid; p < ntiles; p += 256) {
         const uint32_t c = hist[p];
         if (c != 0) hist[p] = atomicAdd(&cnt[p], c);  // this workgroup's base within the tile's list
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
 #pragma unroll
     for (int t = 0; t < kT; ++t) {
