@@ -121,15 +121,42 @@ __global__ __launch_bounds__(256) void kb_fill(RasterTable t) {
     }
 }
 
+// A job's binning buffers rebased on the workspace pointer `ws` (a kernel
+// argument): the same addresses, but the compiler knows them for device
+// memory and addresses them with global instructions.  Pointers read from
+// the job table carry no address space, and their flat instructions count
+// against the LDS wait counter, so every LDS barrier would also wait for the
+// workgroup's outstanding tile-count atomics and record stores.
+template <typename T>
+__device__ __forceinline__ T* in_ws(char* ws, T* p) {
+    return reinterpret_cast<T*>(ws + ((uintptr_t)p - (uintptr_t)ws));
+}
+
+__device__ __forceinline__ BinBufs bufs_in_ws(char* ws, const BinBufs& b) {
+    BinBufs g = b;
+    g.counts = in_ws(ws, b.counts);
+    g.offsets = in_ws(ws, b.offsets);
+    g.overflow = in_ws(ws, b.overflow);
+    g.wide_over = in_ws(ws, b.wide_over);
+    g.nwide = in_ws(ws, b.nwide);
+    g.wide = in_ws(ws, b.wide);
+    g.nclip = in_ws(ws, b.nclip);
+    g.clip = in_ws(ws, b.clip);
+    g.entries = in_ws(ws, b.entries);
+    g.tiles = in_ws(ws, b.tiles);
+    g.setups = in_ws(ws, b.setups);
+    return g;
+}
+
 template <int TS, bool kScatter>
-__global__ __launch_bounds__(256) void kb_bin(RasterTable t) {
+__global__ __launch_bounds__(256) void kb_bin(RasterTable t, char* ws) {
     const uint32_t* pre = t.prefix + kStBin * (t.njobs + 1);
     const uint32_t total = pre[t.njobs];
     for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
         const int j = find_job(pre, t.njobs, item);
         const RasterJob& J = t.jobs[j];
         const uint32_t local = item - pre[j];
-        bin_body<TS, kScatter>(J.mesh, mvp_of(J), J.cfg, J.b, (int)(local % J.tri_blocks),
+        bin_body<TS, kScatter>(J.mesh, mvp_of(J), J.cfg, bufs_in_ws(ws, J.b), (int)(local % J.tri_blocks),
                                (int)(local / J.tri_blocks));
         lds_barrier();  // the LDS histogram is reused by the next item
     }
@@ -348,6 +375,7 @@ struct gi_batch {
     RasterVariant rv[2];
     SoftGroup sg[kShapes][kNtGroups];
     uint32_t* counters;
+    char* ws;  // the workspace: every job's binning buffers lie in it
 };
 
 struct gi_batch_graph {
@@ -374,13 +402,13 @@ int run_depth(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
     unsigned g;
     if ((g = grid_of(R.total[kStFill], cap))) hipLaunchKernelGGL(kb_fill, dim3(g), dim3(256), 0, s, tab);
 #define GI_DEPTH_STAGES(TS)                                                                                        \
-    if ((g = grid_of(R.total[kStBin], cap))) hipLaunchKernelGGL((kb_bin<TS, false>), dim3(g), dim3(256), 0, s, tab); \
+    if ((g = grid_of(R.total[kStBin], cap))) hipLaunchKernelGGL((kb_bin<TS, false>), dim3(g), dim3(256), 0, s, tab, b->ws); \
     if ((g = grid_of(R.total[kStClip], cap)))                                                                       \
         hipLaunchKernelGGL(kb_clip, dim3(g), dim3(kClipThreads), 0, s, tab);                                        \
     if ((g = grid_of(R.total[kStWide], cap)))                                                                       \
         hipLaunchKernelGGL((kb_bin_wide<TS, false>), dim3(g), dim3(256), 0, s, tab);                                \
     if ((g = grid_of(R.total[kStScan], cap))) hipLaunchKernelGGL(kb_scan, dim3(g), dim3(kScanThreads), 0, s, tab);   \
-    if ((g = grid_of(R.total[kStBin], cap))) hipLaunchKernelGGL((kb_bin<TS, true>), dim3(g), dim3(256), 0, s, tab);  \
+    if ((g = grid_of(R.total[kStBin], cap))) hipLaunchKernelGGL((kb_bin<TS, true>), dim3(g), dim3(256), 0, s, tab, b->ws); \
     if ((g = grid_of(R.total[kStWide], cap)))                                                                       \
         hipLaunchKernelGGL((kb_bin_wide<TS, true>), dim3(g), dim3(256), 0, s, tab);                                 \
     if ((g = grid_of(R.total[kStRaster], cap)))                                                                     \
@@ -512,6 +540,7 @@ int gi_batch_create(const gi_batch_instance* inst, int n, void* workspace_dev, s
     b->ncu = ncu;
     b->tuning = gi_batch_tuning{64, 4, 3, 4};
     b->counters = reinterpret_cast<uint32_t*>(base + L.counters);
+    b->ws = base;
     for (int v = 0; v < 2; ++v) {
         const int ts = kTiles[v];
         RasterJob* jobs = reinterpret_cast<RasterJob*>(img.data() + L.rjobs[v]);
