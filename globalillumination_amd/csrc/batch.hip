@@ -39,7 +39,7 @@
 namespace gi {
 namespace {
 
-enum Stage { kStFill = 0, kStBin, kStClip, kStWide, kStScan, kStRaster, kStFallback, kStages };
+enum Stage { kStFill = 0, kStBin, kStClip, kStWide, kStScan, kStRaster, kStFallback, kStScatter, kStages };
 constexpr int kFillChunk = 4096;  // words cleared per fill item
 constexpr int kXcds = 8;
 constexpr int kScreenThreads = 256;
@@ -158,6 +158,24 @@ __global__ __launch_bounds__(256) void kb_bin(RasterTable t, char* ws) {
         const uint32_t local = item - pre[j];
         bin_body<TS, kScatter>(J.mesh, mvp_of(J), J.cfg, bufs_in_ws(ws, J.b), (int)(local % J.tri_blocks),
                                (int)(local / J.tri_blocks));
+        lds_barrier();  // the LDS histogram is reused by the next item
+    }
+}
+
+// The scatter stage for 64-texel tiles: kScatterTris triangles per thread
+// (bin_scatter_body, the single-instance scatter's code), so a workgroup's
+// two dependent global round trips cover kScatterTris x 256 triangles.
+template <int TS>
+__global__ __launch_bounds__(256) void kb_scatter(RasterTable t, char* ws) {
+    const uint32_t* pre = t.prefix + kStScatter * (t.njobs + 1);
+    const uint32_t total = pre[t.njobs];
+    for (uint32_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int j = find_job(pre, t.njobs, item);
+        const RasterJob& J = t.jobs[j];
+        const uint32_t local = item - pre[j];
+        const uint32_t blocks = (uint32_t)(J.tri_blocks + kScatterTris - 1) / kScatterTris;
+        bin_scatter_body<TS, kScatterTris>(J.mesh, J.cfg, bufs_in_ws(ws, J.b), (int)(local % blocks),
+                                           (int)(local / blocks));
         lds_barrier();  // the LDS histogram is reused by the next item
     }
 }
@@ -393,6 +411,18 @@ bool tuning_ok(const gi_batch_tuning& t) {
 
 unsigned grid_of(uint32_t total, unsigned cap) { return total == 0 ? 0u : (total < cap ? total : cap); }
 
+template <int TS>
+void launch_scatter(const RasterVariant& R, const RasterTable& tab, char* ws, unsigned cap, hipStream_t s) {
+    unsigned g;
+    if constexpr (wide_span<TS>() <= 4) {  // the count pass stored the tile codes
+        if ((g = grid_of(R.total[kStScatter], cap)))
+            hipLaunchKernelGGL((kb_scatter<TS>), dim3(g), dim3(256), 0, s, tab, ws);
+    } else {
+        if ((g = grid_of(R.total[kStBin], cap)))
+            hipLaunchKernelGGL((kb_bin<TS, true>), dim3(g), dim3(256), 0, s, tab, ws);
+    }
+}
+
 int run_depth(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
     const int v = T.tile == 32 ? 0 : 1;
     const RasterVariant& R = b->rv[v];
@@ -408,7 +438,7 @@ int run_depth(gi_batch* b, const gi_batch_tuning& T, hipStream_t s) {
     if ((g = grid_of(R.total[kStWide], cap)))                                                                       \
         hipLaunchKernelGGL((kb_bin_wide<TS, false>), dim3(g), dim3(256), 0, s, tab);                                \
     if ((g = grid_of(R.total[kStScan], cap))) hipLaunchKernelGGL(kb_scan, dim3(g), dim3(kScanThreads), 0, s, tab);   \
-    if ((g = grid_of(R.total[kStBin], cap))) hipLaunchKernelGGL((kb_bin<TS, true>), dim3(g), dim3(256), 0, s, tab, b->ws); \
+    launch_scatter<TS>(R, tab, b->ws, cap, s);                                                                     \
     if ((g = grid_of(R.total[kStWide], cap)))                                                                       \
         hipLaunchKernelGGL((kb_bin_wide<TS, true>), dim3(g), dim3(256), 0, s, tab);                                 \
     if ((g = grid_of(R.total[kStRaster], cap)))                                                                     \
@@ -545,7 +575,7 @@ int gi_batch_create(const gi_batch_instance* inst, int n, void* workspace_dev, s
         const int ts = kTiles[v];
         RasterJob* jobs = reinterpret_cast<RasterJob*>(img.data() + L.rjobs[v]);
         uint32_t* pre = reinterpret_cast<uint32_t*>(img.data() + L.rprefix[v]);
-        uint32_t acc[kStages] = {0, 0, 0, 0, 0, 0, 0};
+        uint32_t acc[kStages] = {};
         for (int i = 0; i < n; ++i) {
             const gi_batch_instance& I = inst[i];
             Workspace ws;
@@ -568,7 +598,8 @@ int gi_batch_create(const gi_batch_instance* inst, int n, void* workspace_dev, s
                 I.ntris > 0 ? (uint32_t)(J.nmaps * kWideBlocks) : 0u,
                 (uint32_t)J.nmaps,
                 (uint32_t)(J.nmaps * J.ntiles),
-                (uint32_t)J.ntiles};
+                (uint32_t)J.ntiles,
+                I.ntris > 0 ? (uint32_t)(J.nmaps * cdiv((uint32_t)J.tri_blocks, (uint32_t)kScatterTris)) : 0u};
             for (int st = 0; st < kStages; ++st) {
                 pre[st * (n + 1) + i] = acc[st];
                 acc[st] += items[st];

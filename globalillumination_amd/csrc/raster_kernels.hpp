@@ -730,21 +730,15 @@ __global__ __launch_bounds__(256) void k_bin_wide(RasterCfg cfg, BinBufs b) {
 // tile's list, then the list offsets), so covering kT x 256 triangles per
 // workgroup divides those rounds by kT.  Entries land in the same lists as
 // bin_body's scatter (their order within a list does not matter: the raster
-// takes the minimum).
-//
-// Workgroups past `tri_blocks` bin the wide pieces' records instead
-// (bin_wide_body's scatter, kWideBlocks of them per map), so the two appends
-// share one launch and overlap.
+// takes the minimum).  `block` covers triangles [block kT 256, (block + 1) kT 256)
+// of map `map`.
 template <int TS, int kT>
-__global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cfg, BinBufs b, int tri_blocks) {
+__device__ __forceinline__ void bin_scatter_body(const MeshView& mesh, const RasterCfg& cfg, const BinBufs& b,
+                                                 int block, int map) {
     constexpr int kWideSpan = wide_span<TS>();
     static_assert(kWideSpan <= 4, "tile codes are stored for 64-texel tiles");
     __shared__ uint32_t hist[kBinHist];
-    const int map = (int)blockIdx.y, tid = (int)threadIdx.x;
-    if ((int)blockIdx.x >= tri_blocks) {  // uniform per workgroup
-        bin_wide_body<TS, true>(cfg, b, (int)blockIdx.x - tri_blocks, kWideBlocks, map);
-        return;
-    }
+    const int tid = (int)threadIdx.x;
     const int ntiles = cfg.tiles_x * cfg.tiles_y;
     if (b.overflow[map]) return;  // uniform per workgroup
     const bool lds = ntiles <= kBinHist;
@@ -754,7 +748,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cf
     uint32_t rank_of[kT][kWideSpan];
 #pragma unroll
     for (int t = 0; t < kT; ++t) {
-        const int tri = ((int)blockIdx.x * kT + t) * 256 + tid;
+        const int tri = (block * kT + t) * 256 + tid;
 #pragma unroll
         for (int k = 0; k < kWideSpan; ++k) tile_of[t][k] = -1;
         if (tri < mesh.nt) {
@@ -772,7 +766,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cf
 #pragma unroll
             for (int k = 0; k < kWideSpan; ++k)
                 if (tile_of[t][k] >= 0)
-                    bin_tile(b, map, ntiles, tile_of[t][k], (uint32_t)(((int)blockIdx.x * kT + t) * 256 + tid) * 8u,
+                    bin_tile(b, map, ntiles, tile_of[t][k], (uint32_t)((block * kT + t) * 256 + tid) * 8u,
                              true);
         return;
     }
@@ -792,7 +786,7 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cf
     const uint32_t* off = b.offsets + (size_t)map * (ntiles + 1);
 #pragma unroll
     for (int t = 0; t < kT; ++t) {
-        const uint32_t entry = (uint32_t)(((int)blockIdx.x * kT + t) * 256 + tid) * 8u;
+        const uint32_t entry = (uint32_t)((block * kT + t) * 256 + tid) * 8u;
 #pragma unroll
         for (int k = 0; k < kWideSpan; ++k) {
             if (tile_of[t][k] < 0) continue;
@@ -801,6 +795,18 @@ __global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cf
         }
     }
 }
+// The single-instance scatter: workgroups past `tri_blocks` bin the wide
+// pieces' records instead (bin_wide_body's scatter, kWideBlocks of them per
+// map), so the two appends share one launch and overlap.
+template <int TS, int kT>
+__global__ __launch_bounds__(256) void k_bin_scatter(MeshView mesh, RasterCfg cfg, BinBufs b, int tri_blocks) {
+    if ((int)blockIdx.x >= tri_blocks) {  // uniform per workgroup
+        bin_wide_body<TS, true>(cfg, b, (int)blockIdx.x - tri_blocks, kWideBlocks, (int)blockIdx.y);
+        return;
+    }
+    bin_scatter_body<TS, kT>(mesh, cfg, b, (int)blockIdx.x, (int)blockIdx.y);
+}
+
 constexpr int kScatterTris = 4;  // triangles per thread in k_bin_scatter
 
 // Exclusive scan of one map's tile counts; resets counts to 0 (they become
